@@ -1,0 +1,140 @@
+/*
+ * pfm.h -- C-ABI of libpfmds.so, the MI355X (gfx950) HIP implementation of the
+ * PFMDS molecular-dynamics step.
+ *
+ * The reference runs its step loop inside the Fortran subroutine md()
+ * (code_source/MOLECULAR_DYNAMICS/md_simulation.f90:19-275) and calls module
+ * procedures whose arguments are derived types with allocatable components,
+ * which are not C-interoperable.  This header replaces those procedures with a
+ * handle-based interface: the caller (the Fortran front end in
+ * pfmds_amd/fortran, or any C/Python host) keeps the settings file, the
+ * schedule, logging and xyz output, and hands the per-step work to the GPU.
+ *
+ * Conventions
+ *   - Every function returns an int status: PFM_OK (0) or one of PFM_E_*.
+ *     pfm_last_error() returns the message, worded as the reference's
+ *     "error: ..." text, so a caller can print it and stop as md() does.
+ *   - Arrays of atom data are the reference's (3,N) column-major Fortran
+ *     arrays, i.e. C double[N][3]; atom and group indices are 1-based.
+ *   - One handle = one system and one HIP stream; a handle is not
+ *     thread-safe.  Device memory is owned by the handle until pfm_destroy().
+ */
+#ifndef PFM_H
+#define PFM_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PFM_OK             0
+#define PFM_E_OVERFLOW     1  /* "error: too many neighbours" (md_neighbours.f90:80,151) */
+#define PFM_E_OUT_OF_CELL  2  /* "particle out of cell" (md_general.f90:342-364)         */
+#define PFM_E_GRAPHENE_NN  3  /* "not enough / too many gr nearest neibs", "l1>3"
+                                 (graphenenorm.f90:13,26,30; MorseCosine.f90:60,88)       */
+#define PFM_E_NHC_PARAMS   4  /* "error: wrong nhc parameters" (md_integrators.f90:187)   */
+#define PFM_E_UNKNOWN_KIND 5  /* "error: unknown interaction name" (md_interactions.f90:120) */
+#define PFM_E_INVALID      6  /* argument or group layout the reference cannot run       */
+#define PFM_E_HIP          7  /* HIP runtime error                                        */
+#define PFM_E_NO_DEVICE    8  /* no gfx950 device visible                                 */
+
+/* interaction kinds: the names of md_interactions.f90:70-121 */
+#define PFM_LJ     1  /* 'lj'     params: eps sig R1 R2                     (LennardJones.f90:17-18)       */
+#define PFM_LJ1G   2  /* 'lj1g'   params: eps sig R1 R2                     (LennardJones_1g.f90:18-19)    */
+#define PFM_LJC    3  /* 'ljc'    params: eps sig delt R1 R2 simplified     (LennardJonesCosine.f90:19-21) */
+#define PFM_MORSEC 4  /* 'morsec' params: d r a delt R1 R2 simplified       (MorseCosine.f90:19-21)        */
+#define PFM_RJL    5  /* 'rjl'    params: A0 xi p q r0 R1 R2                (RosatoGuillopeLegrand.f90:17-18) */
+#define PFM_TB     6  /* 'tb'     params: d s b r0 delt a0 c0 d0 R1 R2      (TersoffBrenner.f90:18-19)     */
+
+/* integrators: the names of md_simulation.f90:73-80 */
+#define PFM_NVT  1
+#define PFM_NVE  2
+#define PFM_NVMS 3
+
+/* precision of the force arithmetic */
+#define PFM_FP64  0   /* all arithmetic in FP64                                        */
+#define PFM_MIXED 1   /* FP32 pair arithmetic, FP64 dr, accumulation, state, reductions */
+
+/* pfm_run events */
+#define PFM_EV_DONE      0  /* ran through step_end                                            */
+#define PFM_EV_CONVERGED 1  /* nvms: abs(PE-PE_prev) <= ms_de at *last_step (md_simulation.f90:142-147) */
+
+typedef struct pfm_ctx pfm_ctx;
+
+/* Create a system: replaces read_box_size/read_particles state setup
+ * (md_read_write.f90:22-61) and the `particles` type (md_general.f90:13-17).
+ * pos/vel: double[n][3]; mass: double[n] in u. */
+int pfm_create(pfm_ctx** h, int n_atoms, const double box[3], const double* pos,
+               const double* vel, const double* mass, int precision);
+
+/* Define group gid (1-based) as the atom indices idx[0..n-1] (1-based) in the
+ * order create_particle_group builds them (md_general.f90:57-80). */
+int pfm_set_group(pfm_ctx* h, int gid, int n, const int* idx);
+
+/* Group roles and periods of md() (md_simulation.f90:55-62,72). */
+int pfm_set_roles(pfm_ctx* h, int all_moving, int xyz_moving, int z_moving,
+                  int all_atoms, int zero_momentum_period, int invert_z);
+
+/* Add one interaction, in settings order; replaces one iteration of
+ * create_interactions (md_interactions.f90:68-133).  group_nums has 2*nl_n
+ * entries (g1 g2 per list row); nmax/r_cut/period have nl_n entries. */
+int pfm_add_interaction(pfm_ctx* h, int kind, const double* params, int n_params,
+                        int nl_n, const int* group_nums, const int* nmax,
+                        const double* r_cut, const int* period);
+
+/* Add one Nose-Hoover chain; replaces create_nose_hoover_chain +
+ * set_nose_hoover_chain (md_integrators.f90:165-198). */
+int pfm_add_nhc(pfm_ctx* h, int group, double temperature, int M, double q1);
+
+/* Run md steps step_begin..step_end of one integrator with time step dt;
+ * replaces the body of md()'s loop (md_simulation.f90:137-231) for those
+ * steps.  Energies are evaluated at step_end when energy_at_end != 0 and at
+ * every step under nvms.  *last_step is the last step entered; *event is a
+ * PFM_EV_* code. */
+int pfm_run(pfm_ctx* h, int integrator, double dt, int step_begin, int step_end,
+            int energy_at_end, double ms_de, int* last_step, int* event);
+
+/* Energies of the last energy evaluation: e_inter[n_interactions] (eV),
+ * potential, kinetic (eV), temperature (K) of the all_moving group,
+ * e_nhc[n_chains], simulated time (fs).  Replaces the values md() logs at
+ * md_simulation.f90:188-209. Any pointer may be NULL. */
+int pfm_energies(pfm_ctx* h, double* e_inter, double* pe, double* ke, double* temp,
+                 double* e_nhc, double* sim_time);
+
+/* Copy positions/velocities/forces (double[n][3], original atom order) to the
+ * host; any pointer may be NULL.  Used for snapshots, trajectories and final
+ * xyz (md_read_write.f90:65-107). */
+int pfm_get_state(pfm_ctx* h, double* pos, double* vel, double* force);
+
+/* Replace positions and velocities (double[n][3]); forces are recomputed at
+ * the next step. */
+int pfm_set_state(pfm_ctx* h, const double* pos, const double* vel);
+
+/* Download neighbour list k (1-based) of interaction inter (1-based) in the
+ * reference layout of type neighbour_list (md_general.f90:30-35): n_rows,
+ * nnum[row], nlist[row*stride+p] = 1-based group2-local index, lessnnum[row].
+ * Entries of a row are in ascending group2 order, as find_neighbours
+ * (md_neighbours.f90:56-100) produces them. */
+int pfm_get_list(pfm_ctx* h, int inter, int k, int* n_rows, int* nnum, int* nlist,
+                 int* lessnnum, int stride);
+
+/* Graphene normals gr_norm(3,N_g1) of a ljc/morsec interaction
+ * (graphenenorm.f90:38-56), double[N_g1][3] in group order. */
+int pfm_get_normals(pfm_ctx* h, int inter, double* gr_norm);
+
+/* Kernel timing.  pfm_set_timing(h,1) brackets every launch with HIP events on
+ * the handle's stream; pfm_kernel_time() returns the launch count and the
+ * mean duration (ms) of the kernels whose name contains `name`. */
+int pfm_set_timing(pfm_ctx* h, int on);
+int pfm_kernel_time(pfm_ctx* h, const char* name, int* count, double* mean_ms);
+
+/* Tile shape of the pair kernels: lanes per atom row (1,2,4,8,16,32,64) and
+ * workgroup size; 0 keeps the current value. */
+int pfm_set_tuning(pfm_ctx* h, int lanes_per_row, int block);
+
+int  pfm_last_error(pfm_ctx* h, char* buf, int len);
+int  pfm_destroy(pfm_ctx* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,213 @@
+"""ctypes binding of the PFMDS step C-ABI declared in include/pfm.h.
+
+`Context` wraps one handle of libpfmds.so (the gfx950 HIP library).  The same
+class can wrap any library that exports the same entry points under another
+prefix, which is how the tests drive the CPU checker with identical calls.
+
+The product path has no CPU fallback: `Context.gpu()` raises if the HIP
+library is missing or no device is visible.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+# constants of include/pfm.h
+OK, E_OVERFLOW, E_OUT_OF_CELL, E_GRAPHENE_NN, E_NHC_PARAMS, E_UNKNOWN_KIND, E_INVALID, E_HIP, E_NO_DEVICE = range(9)
+KINDS = {"lj": 1, "lj1g": 2, "ljc": 3, "morsec": 4, "rjl": 5, "tb": 6}
+INTEGRATORS = {"nvt": 1, "nve": 2, "nvms": 3}
+FP64, MIXED = 0, 1
+EV_DONE, EV_CONVERGED = 0, 1
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GPU_LIB = os.path.join(HERE, "lib", "libpfmds.so")
+
+_P = C.POINTER
+_d = _P(C.c_double)
+_i = _P(C.c_int)
+_SIGS = {
+    "create": [_P(C.c_void_p), C.c_int, _d, _d, _d, _d],
+    "set_group": [C.c_void_p, C.c_int, C.c_int, _i],
+    "set_roles": [C.c_void_p] + [C.c_int] * 6,
+    "add_interaction": [C.c_void_p, C.c_int, _d, C.c_int, C.c_int, _i, _i, _d, _i],
+    "add_nhc": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double],
+    "run": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, _i, _i],
+    "energies": [C.c_void_p, _d, _d, _d, _d, _d, _d],
+    "get_state": [C.c_void_p, _d, _d, _d],
+    "set_state": [C.c_void_p, _d, _d],
+    "get_list": [C.c_void_p, C.c_int, C.c_int, _i, _i, _i, _i, C.c_int],
+    "get_normals": [C.c_void_p, C.c_int, _d],
+    "last_error": [C.c_void_p, C.c_char_p, C.c_int],
+}
+_libs = {}
+
+
+class PfmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+        self.msg = msg
+
+
+def load_library(path, prefix):
+    key = (os.path.abspath(path), prefix)
+    if key in _libs:
+        return _libs[key]
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, argtypes in _SIGS.items():
+        fn = getattr(lib, f"{prefix}_{name}")
+        fn.argtypes = argtypes
+        fn.restype = C.c_int
+    destroy = getattr(lib, f"{prefix}_destroy")
+    destroy.argtypes = [C.c_void_p]
+    if prefix == "pfm":
+        lib.pfm_create.argtypes = _SIGS["create"] + [C.c_int]
+        lib.pfm_set_timing.argtypes = [C.c_void_p, C.c_int]
+        lib.pfm_set_timing.restype = C.c_int
+        lib.pfm_kernel_time.argtypes = [C.c_void_p, C.c_char_p, _i, _d]
+        lib.pfm_kernel_time.restype = C.c_int
+        lib.pfm_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.pfm_set_tuning.restype = C.c_int
+    _libs[key] = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_d)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_i)
+
+
+class Context:
+    """One system behind the C-ABI (one pfm_ctx handle)."""
+
+    def __init__(self, lib, prefix, box, pos, vel, mass, precision=FP64):
+        self.lib, self.pfx = lib, prefix
+        self.h = C.c_void_p()
+        self.n = len(mass)
+        self._pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.n, 3)
+        self._vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(self.n, 3)
+        box = np.ascontiguousarray(box, dtype=np.float64)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        args = [C.byref(self.h), self.n, _dp(box), _dp(self._pos), _dp(self._vel), _dp(mass)]
+        if prefix == "pfm":
+            args.append(int(precision))
+        rc = self._fn("create")(*args)
+        self.n_inter = 0
+        self.n_nhc = 0
+        self._check(rc)
+
+    @classmethod
+    def gpu(cls, box, pos, vel, mass, precision=FP64, path=GPU_LIB):
+        return cls(load_library(path, "pfm"), "pfm", box, pos, vel, mass, precision)
+
+    def _fn(self, name):
+        return getattr(self.lib, f"{self.pfx}_{name}")
+
+    def last_error(self):
+        buf = C.create_string_buffer(1024)
+        self._fn("last_error")(self.h, buf, 1024)
+        return buf.value.decode(errors="replace")
+
+    def _check(self, rc):
+        if rc != OK:
+            raise PfmError(rc, self.last_error())
+
+    def close(self):
+        if self.h:
+            self._fn("destroy")(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup -------------------------------------------------------------
+    def set_group(self, gid, idx1):
+        idx = np.ascontiguousarray(idx1, dtype=np.int32)
+        self._check(self._fn("set_group")(self.h, gid, len(idx), _ip(idx)))
+
+    def set_roles(self, all_moving, xyz_moving, z_moving, all_atoms, zero_momentum_period, invert_z=False):
+        self._check(self._fn("set_roles")(self.h, all_moving, xyz_moving, z_moving, all_atoms,
+                                          zero_momentum_period, int(bool(invert_z))))
+
+    def add_interaction(self, name, params, lists):
+        kind = KINDS.get(name, 0) if isinstance(name, str) else int(name)
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        g = np.ascontiguousarray([x for row in lists for x in row[:2]], dtype=np.int32)
+        nmax = np.ascontiguousarray([row[2] for row in lists], dtype=np.int32)
+        rcut = np.ascontiguousarray([row[3] for row in lists], dtype=np.float64)
+        per = np.ascontiguousarray([row[4] for row in lists], dtype=np.int32)
+        self._check(self._fn("add_interaction")(self.h, kind, _dp(p), len(p), len(lists), _ip(g), _ip(nmax),
+                                                _dp(rcut), _ip(per)))
+        self.n_inter += 1
+
+    def add_nhc(self, group, temperature, M, q1):
+        self._check(self._fn("add_nhc")(self.h, group, temperature, M, q1))
+        self.n_nhc += 1
+
+    # -- stepping ----------------------------------------------------------
+    def run(self, integrator, dt, step_begin, step_end, energy_at_end=True, ms_de=0.0):
+        integ = INTEGRATORS[integrator] if isinstance(integrator, str) else int(integrator)
+        last = C.c_int(-1)
+        ev = C.c_int(0)
+        rc = self._fn("run")(self.h, integ, float(dt), int(step_begin), int(step_end), int(bool(energy_at_end)),
+                             float(ms_de), C.byref(last), C.byref(ev))
+        self._check(rc)
+        return last.value, ev.value
+
+    def energies(self):
+        e = np.zeros(max(self.n_inter, 1))
+        enhc = np.zeros(max(self.n_nhc, 1))
+        pe, ke, t, st = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        self._check(self._fn("energies")(self.h, _dp(e), C.byref(pe), C.byref(ke), C.byref(t), _dp(enhc),
+                                         C.byref(st)))
+        return dict(e_inter=e[: self.n_inter], pe=pe.value, ke=ke.value, temp=t.value,
+                    e_nhc=enhc[: self.n_nhc], time=st.value)
+
+    def state(self):
+        pos = np.zeros((self.n, 3))
+        vel = np.zeros((self.n, 3))
+        frc = np.zeros((self.n, 3))
+        self._check(self._fn("get_state")(self.h, _dp(pos), _dp(vel), _dp(frc)))
+        return pos, vel, frc
+
+    def set_state(self, pos, vel):
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.n, 3)
+        v = np.ascontiguousarray(vel, dtype=np.float64).reshape(self.n, 3)
+        self._check(self._fn("set_state")(self.h, _dp(p), _dp(v)))
+
+    def get_list(self, inter, k, stride=None):
+        nrows = C.c_int(0)
+        self._check(self._fn("get_list")(self.h, inter, k, C.byref(nrows), None, None, None, 0))
+        n = nrows.value
+        stride = stride or 1024
+        nnum = np.zeros(max(n, 1), dtype=np.int32)
+        less = np.zeros(max(n, 1), dtype=np.int32)
+        nl = np.zeros((max(n, 1), stride), dtype=np.int32)
+        self._check(self._fn("get_list")(self.h, inter, k, C.byref(nrows), _ip(nnum), _ip(nl), _ip(less), stride))
+        return nnum[:n], nl[:n], less[:n]
+
+    def normals(self, inter, n_rows):
+        out = np.zeros((max(n_rows, 1), 3))
+        self._check(self._fn("get_normals")(self.h, inter, _dp(out)))
+        return out[:n_rows]
+
+    # -- GPU-only extras ----------------------------------------------------
+    def set_timing(self, on):
+        self._check(self.lib.pfm_set_timing(self.h, int(bool(on))))
+
+    def kernel_time(self, name):
+        cnt = C.c_int(0)
+        ms = C.c_double(0.0)
+        self._check(self.lib.pfm_kernel_time(self.h, name.encode(), C.byref(cnt), C.byref(ms)))
+        return cnt.value, ms.value
+
+    def set_tuning(self, lanes_per_row=0, block=0):
+        self._check(self.lib.pfm_set_tuning(self.h, int(lanes_per_row), int(block)))

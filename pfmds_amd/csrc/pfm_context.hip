@@ -1,0 +1,985 @@
+// pfm_context.hip -- the C-ABI of include/pfm.h: system setup, the md() step
+// sequence, integrators, Nose-Hoover chains and FP64 reductions on gfx950.
+//
+// Step order per md_step s (md_simulation.f90:137-231):
+//   check_positions, [invert_z]; if s != 0: [nvms exit test], [NHC half],
+//   kick, drift; lists; [zero_momentum]; zero_forces; forces; if s != 0:
+//   kick, [NHC half], [nvms projection]; [energies].
+// All per-atom work runs on the device; the host only sequences launches and
+// reads back scalars at energy steps (every step under nvms) and at the end
+// of pfm_run.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "pfm_internal.h"
+
+#define HIPCHK(h, x)                                                                       \
+  do {                                                                                     \
+    hipError_t _e = (x);                                                                   \
+    if (_e != hipSuccess) {                                                                \
+      (h)->err = std::string("error: HIP ") + hipGetErrorString(_e) + " at " #x;           \
+      return PFM_E_HIP;                                                                    \
+    }                                                                                      \
+  } while (0)
+
+static int fail(pfm_ctx* h, int code, const std::string& msg) {
+  h->err = msg;
+  return code;
+}
+
+// ------------------------------------------------------------------ timing
+
+static hipEvent_t ev_get(pfm_ctx* h) {
+  if (h->evnext >= h->evpool.size()) {
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    h->evpool.push_back(e);
+  }
+  return h->evpool[h->evnext++];
+}
+void tim_begin(pfm_ctx* h, const char* name, hipEvent_t* a, hipEvent_t* b) {
+  (void)name;
+  *a = ev_get(h);
+  *b = ev_get(h);
+  (void)hipEventRecord(*a, h->stream);
+}
+void tim_end(pfm_ctx* h, hipEvent_t a, hipEvent_t b, const char* name) {
+  (void)hipEventRecord(b, h->stream);
+  h->timers.push_back(KTimer{name, a, b});
+}
+
+// ------------------------------------------------------------ device kernels
+
+struct IntP {
+  double ts0, ts1;          // dt, dt/2
+  uint32_t bxyz, bz;        // group bits of xyz-moving and z-moving atoms
+  double box[3];
+  int nchain;
+  uint32_t chain_bit[PFM_MAXNHC];
+  const double* chain_s;    // per-chain velocity scale (device)
+  int scale, invert;
+  double zlo, zhi;
+};
+
+struct NhcDev {
+  double x[16], v[16], q[16];
+  double s, e, kt, L, T;
+  int M;
+  uint32_t bit;
+};
+
+__device__ __forceinline__ double wrap1(double x, double L) {
+  if (x > L) return x - L;
+  if (x < 0.0) return x + L;
+  return x;
+}
+__device__ __forceinline__ bool in_cell(double x, double L) {
+  const double tol = 0.0000001;
+  return x > (0.0 - tol) && x < (L + tol);
+}
+
+// invert_z_velocities (md_general.f90:382-398) on all atoms
+__device__ __forceinline__ void invert_v(double4& v, double z, double lo, double hi) {
+  double mid = (lo + hi) / 2;
+  if ((z > lo && z < mid && v.z > 0.0) || (z < hi && z > mid && v.z < 0.0)) v.z = -v.z;
+}
+
+// first half of a step: [invert_z], [NHC scaling], kick xyz/z, drift xyz/z
+// with single wrap (md_integrators.f90:7-97), and check_positions of the
+// next step (md_general.f90:342-364)
+__global__ void k_kick_drift(int N, double4* __restrict__ pos, double4* __restrict__ vel,
+                             const double4* __restrict__ frc, const uint32_t* __restrict__ mask, IntP P, DevErr* err,
+                             const int* __restrict__ orig, int step) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double4 p = pos[i], v = vel[i];
+  uint32_t m = mask[i];
+  if (P.invert) invert_v(v, p.z, P.zlo, P.zhi);
+  if (P.scale) {
+    for (int c = 0; c < P.nchain; ++c)
+      if (m & P.chain_bit[c]) { double s = P.chain_s[c]; v.x *= s; v.y *= s; v.z *= s; }
+  }
+  bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
+  if (bx || bz) {
+    double4 f = frc[i];
+    double k = 1.0 / v.w / PFM_MASS_COEF;
+    if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
+    if (bz) v.z = v.z + f.z * k * P.ts1;
+    if (bx) {
+      p.x = wrap1(p.x + v.x * P.ts0, P.box[0]);
+      p.y = wrap1(p.y + v.y * P.ts0, P.box[1]);
+      p.z = wrap1(p.z + v.z * P.ts0, P.box[2]);
+    }
+    if (bz) p.z = wrap1(p.z + v.z * P.ts0, P.box[2]);
+    pos[i] = p;
+  }
+  vel[i] = v;
+  if (!(in_cell(p.x, P.box[0]) && in_cell(p.y, P.box[1]) && in_cell(p.z, P.box[2])))
+    raise_err(err, PFM_E_OUT_OF_CELL, orig[i] + 1, 0, step + 1);
+}
+
+__global__ void k_check(int N, const double4* __restrict__ pos, double bx, double by, double bz, DevErr* err,
+                        const int* __restrict__ orig, int step, int invert, double4* __restrict__ vel, double zlo,
+                        double zhi) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double4 p = pos[i];
+  if (!(in_cell(p.x, bx) && in_cell(p.y, by) && in_cell(p.z, bz))) raise_err(err, PFM_E_OUT_OF_CELL, orig[i] + 1, 0, step);
+  if (invert) {
+    double4 v = vel[i];
+    invert_v(v, p.z, zlo, zhi);
+    vel[i] = v;
+  }
+}
+
+// second half: kick xyz/z, then for nvms the projections
+// molecular_static_xyz/1D_velocities (md_integrators.f90:99-145)
+__global__ void k_kick2(int N, double4* __restrict__ vel, const double4* __restrict__ frc,
+                        const uint32_t* __restrict__ mask, IntP P, int nvms) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  uint32_t m = mask[i];
+  bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
+  if (!(bx || bz)) return;
+  double4 v = vel[i], f = frc[i];
+  double k = 1.0 / v.w / PFM_MASS_COEF;
+  if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
+  if (bz) v.z = v.z + f.z * k * P.ts1;
+  if (nvms) {
+    if (bx) {
+      double fv = f.x * v.x + f.y * v.y + f.z * v.z, ff = f.x * f.x + f.y * f.y + f.z * f.z;
+      if (fv > 0.0 && ff > 1e-12) { double c = fv / ff; v.x = c * f.x; v.y = c * f.y; v.z = c * f.z; }
+      else { v.x = 0; v.y = 0; v.z = 0; }
+    }
+    if (bz) {
+      double fv = f.x * v.x + f.y * v.y + f.z * v.z;
+      if (!(fv > 0.0)) { v.x = 0; v.y = 0; v.z = 0; }
+    }
+  }
+  vel[i] = v;
+}
+
+__global__ void k_scale(int N, double4* __restrict__ vel, const uint32_t* __restrict__ mask, IntP P) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  uint32_t m = mask[i];
+  for (int c = 0; c < P.nchain; ++c)
+    if (m & P.chain_bit[c]) {
+      double4 v = vel[i];
+      double s = P.chain_s[c];
+      v.x *= s; v.y *= s; v.z *= s;
+      vel[i] = v;
+    }
+}
+
+__global__ void k_zero_masked(int N, double4* __restrict__ frc, const uint32_t* __restrict__ mask, uint32_t bit) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N && (mask[i] & bit)) frc[i] = make_double4(0, 0, 0, 0);
+}
+
+#define RED_THREADS 256
+#define RED_MAXQ 8
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
+    if (t < o) sh[t] += sh[t + o];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+struct MaskQ {
+  int nq;
+  uint32_t bit[RED_MAXQ];
+};
+
+// per-block partial kinetic energies m v.v/2*mass_coef of nq groups
+// (calculate_kinetic_energy, md_general.f90:161-182)
+__global__ void __launch_bounds__(RED_THREADS) k_ke_part(int N, const double4* __restrict__ vel,
+                                                         const uint32_t* __restrict__ mask, MaskQ Q,
+                                                         double* __restrict__ part) {
+  __shared__ double sh[RED_THREADS];
+  double acc[RED_MAXQ];
+  for (int q = 0; q < Q.nq; ++q) acc[q] = 0.0;
+  for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
+    double4 v = vel[i];
+    uint32_t m = mask[i];
+    double ke = v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF;
+    for (int q = 0; q < Q.nq; ++q)
+      if (m & Q.bit[q]) acc[q] += ke;
+  }
+  for (int q = 0; q < Q.nq; ++q) {
+    double s = block_sum(acc[q], sh);
+    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// per-block partial sums of m*v (3) and m over one group
+__global__ void __launch_bounds__(RED_THREADS) k_mom_part(int N, const double4* __restrict__ vel,
+                                                          const uint32_t* __restrict__ mask, uint32_t bit,
+                                                          double* __restrict__ part) {
+  __shared__ double sh[RED_THREADS];
+  double a[4] = {0, 0, 0, 0};
+  for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
+    if (mask[i] & bit) {
+      double4 v = vel[i];
+      a[0] += v.w * v.x; a[1] += v.w * v.y; a[2] += v.w * v.z; a[3] += v.w;
+    }
+  }
+  for (int q = 0; q < 4; ++q) {
+    double s = block_sum(a[q], sh);
+    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// reduce nq rows of nb partials each (fixed order) into out[q]
+__global__ void __launch_bounds__(RED_THREADS) k_reduce_rows(int nq, int nb, const double* __restrict__ part,
+                                                             double* __restrict__ out) {
+  __shared__ double sh[RED_THREADS];
+  for (int q = 0; q < nq; ++q) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RED_THREADS) a += part[q * nb + b];
+    double s = block_sum(a, sh);
+    if (threadIdx.x == 0) out[q] = s;
+  }
+}
+
+// sum of a double array (per-row energies) into partials
+__global__ void __launch_bounds__(RED_THREADS) k_sum_part(int n, const double* __restrict__ x,
+                                                          double* __restrict__ part) {
+  __shared__ double sh[RED_THREADS];
+  double a = 0.0;
+  for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) a += x[i];
+  double s = block_sum(a, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// zero_momentum: v -= sum(m v)/sum(m) over the group (md_general.f90:236-253)
+__global__ void k_sub_mcv(int N, double4* __restrict__ vel, const uint32_t* __restrict__ mask, uint32_t bit,
+                          const double* __restrict__ sums) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || !(mask[i] & bit)) return;
+  double tm = sums[3];
+  double4 v = vel[i];
+  v.x -= sums[0] / tm; v.y -= sums[1] / tm; v.z -= sums[2] / tm;
+  vel[i] = v;
+}
+
+// integrate_nose_hoover_chain (md_integrators.f90:200-245) for every chain,
+// one thread per chain; ke[c] is the chain group's kinetic energy
+__global__ void k_nhc_half(int nchain, NhcDev* __restrict__ C, const double* __restrict__ ke_in,
+                           double* __restrict__ s_out, double ts1, double ts2, double ts3) {
+  int c = threadIdx.x;
+  if (c >= nchain) return;
+  NhcDev& h = C[c];
+  double ke = ke_in[c], kt = h.kt;
+  double kedif = 2.0 * ke - 3.0 * h.L * kt;
+  int M = h.M;
+  double b = 1.0;
+  double* v = h.v;
+  const double* q = h.q;
+  if (M == 1) {
+    v[0] = v[0] + kedif / q[0] * ts2;
+  } else {
+    v[M - 1] = v[M - 1] + (q[M - 2] * v[M - 2] * v[M - 2] - kt) / q[M - 1] * ts2;
+    for (int i = M - 2; i >= 1; --i) {
+      b = exp(-v[i + 1] * ts3);
+      v[i] = v[i] * (b * b) + (q[i - 1] * v[i - 1] * v[i - 1] - kt) / q[i] * ts2 * b;
+    }
+    b = exp(-v[1] * ts3);
+    v[0] = v[0] * (b * b) + kedif / q[0] * ts2 * b;
+  }
+  h.s = exp(-v[0] * ts1);
+  kedif = 2.0 * ke * (h.s * h.s) - 3.0 * h.L * kt;
+  for (int i = 0; i < M; ++i) h.x[i] = h.x[i] + v[i] * ts1;
+  if (M == 1) {
+    v[0] = v[0] + kedif / q[0] * ts2;
+  } else {
+    v[0] = v[0] * (b * b) + kedif / q[0] * ts2 * b;
+    for (int i = 1; i <= M - 2; ++i) {
+      b = exp(-v[i + 1] * ts3);
+      v[i] = v[i] * (b * b) + (q[i - 1] * v[i - 1] * v[i - 1] - kt) / q[i] * ts2 * b;
+    }
+    v[M - 1] = v[M - 1] + (q[M - 2] * v[M - 2] * v[M - 2] - kt) / q[M - 1] * ts2;
+  }
+  s_out[c] = h.s;
+}
+
+// ------------------------------------------------------------- host helpers
+
+static uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  uint64_t r = 0;
+  for (int b = 0; b < 21; ++b) {
+    r |= (uint64_t)((x >> b) & 1) << (3 * b);
+    r |= (uint64_t)((y >> b) & 1) << (3 * b + 1);
+    r |= (uint64_t)((z >> b) & 1) << (3 * b + 2);
+  }
+  return r;
+}
+
+template <class T>
+static int dalloc(pfm_ctx* h, T** p, size_t n) {
+  HIPCHK(h, hipMalloc((void**)p, sizeof(T) * (n ? n : 1)));
+  HIPCHK(h, hipMemsetAsync(*p, 0, sizeof(T) * (n ? n : 1), h->stream));
+  return PFM_OK;
+}
+
+static bool is_cos(int k) { return k == PFM_LJC || k == PFM_MORSEC; }
+
+// Allocate lists, cells and reductions once all interactions are known.
+static int finalize(pfm_ctx* h) {
+  if (h->sorted_once) return PFM_OK;
+  int rc;
+  double rmax = 0.0;
+  for (auto& it : h->inter) {
+    for (int k = 0; k < it.nl_n; ++k) {
+      ListDev& L = it.nl[k];
+      if (k == 2 && it.nn_from_tb >= 0) continue;
+      double rc_eff = (k == 1 && (it.kind == PFM_LJ || is_cos(it.kind))) ? it.nl[0].rcut : L.rcut;
+      rmax = std::max(rmax, rc_eff);
+    }
+  }
+  if (rmax <= 0.0) rmax = 1.0;
+  h->ncell = 1;
+  for (int d = 0; d < 3; ++d) {
+    int n = (int)std::floor(h->box[d] / rmax);
+    n = std::max(1, std::min(n, 1024));
+    h->nc[d] = n;
+    h->cell_len[d] = h->box[d] / n;
+    h->ncell *= n;
+  }
+  // Morton rank of every cell
+  std::vector<std::pair<uint64_t, int>> key(h->ncell);
+  for (int z = 0; z < h->nc[2]; ++z)
+    for (int y = 0; y < h->nc[1]; ++y)
+      for (int x = 0; x < h->nc[0]; ++x) {
+        int lin = (z * h->nc[1] + y) * h->nc[0] + x;
+        key[lin] = {morton3(x, y, z), lin};
+      }
+  std::sort(key.begin(), key.end());
+  std::vector<int> rank(h->ncell);
+  for (int r = 0; r < h->ncell; ++r) rank[key[r].second] = r;
+  if ((rc = dalloc(h, &h->cell_rank, h->ncell))) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->cell_rank, rank.data(), sizeof(int) * h->ncell, hipMemcpyHostToDevice, h->stream));
+  if ((rc = dalloc(h, &h->cell_count, h->ncell))) return rc;
+  if ((rc = dalloc(h, &h->cell_start, h->ncell))) return rc;
+  if ((rc = dalloc(h, &h->cell_fill, h->ncell))) return rc;
+  // group rows
+  for (int g = 1; g <= PFM_MAXG; ++g) {
+    if (!h->grp.used[g]) continue;
+    if ((rc = dalloc(h, &h->grp.rows[g], h->grp.n[g]))) return rc;
+    if ((rc = dalloc(h, &h->grp.row_of[g], h->N))) return rc;
+  }
+  // list storage
+  for (auto& it : h->inter) {
+    for (int k = 0; k < it.nl_n; ++k) {
+      ListDev& L = it.nl[k];
+      L.stride = L.nmax;
+      if ((rc = dalloc(h, &L.nnum, L.nrows))) return rc;
+      if ((rc = dalloc(h, &L.nbr, (size_t)L.nrows * L.stride))) return rc;
+      if (it.kind == PFM_TB || k == 2)
+        if ((rc = dalloc(h, &L.rev, (size_t)L.nrows * L.stride))) return rc;
+    }
+    it.e_rows = it.nl[0].nrows;
+    if ((rc = dalloc(h, &it.e_row, it.e_rows))) return rc;
+    if (it.kind == PFM_LJ1G)
+      if ((rc = dalloc(h, &it.cut, h->N))) return rc;
+    if (it.kind == PFM_TB) {
+      size_t ne = (size_t)it.nl[0].nrows * it.nl[0].stride;
+      if ((rc = dalloc(h, &it.B, ne))) return rc;
+      if ((rc = dalloc(h, &it.Bp, 2 * ne))) return rc;
+    }
+    if (is_cos(it.kind)) {
+      if ((rc = dalloc(h, &it.gnorm, h->N))) return rc;
+      if ((rc = dalloc(h, &it.gS, h->N))) return rc;
+    }
+  }
+  // group-local index by original atom, for lj1g and list downloads
+  for (auto& it : h->inter) {
+    if (it.kind != PFM_LJ1G) continue;
+    int g = it.nl[0].g2;
+    if (h->grp.loc[g]) continue;
+    std::vector<int> loc(h->N, -1);
+    for (size_t k = 0; k < h->grp.idx[g].size(); ++k) loc[h->grp.idx[g][k]] = (int)k;
+    if ((rc = dalloc(h, &h->grp.loc[g], h->N))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->grp.loc[g], loc.data(), sizeof(int) * h->N, hipMemcpyHostToDevice, h->stream));
+  }
+  // group bit masks
+  std::vector<uint32_t> mask(h->N, 0);
+  for (int g = 1; g <= PFM_MAXG; ++g)
+    for (int a : h->grp.idx[g]) mask[a] |= 1u << (g - 1);
+  HIPCHK(h, hipMemcpyAsync(h->mask, mask.data(), sizeof(uint32_t) * h->N, hipMemcpyHostToDevice, h->stream));
+  // chains
+  if (!h->nhc.empty()) {
+    std::vector<NhcDev> cs(h->nhc.size());
+    for (size_t c = 0; c < h->nhc.size(); ++c) {
+      NhcDev& d = cs[c];
+      memset(&d, 0, sizeof d);
+      d.M = h->nhc[c].M;
+      d.L = (double)h->nhc[c].L;
+      d.T = h->nhc[c].T;
+      d.kt = PFM_KB * h->nhc[c].T;
+      d.q[0] = h->nhc[c].q1;
+      for (int i = 1; i < d.M; ++i) d.q[i] = d.q[0] / (3.0 * h->nhc[c].L);
+      d.s = 1.0;
+      d.bit = 1u << (h->nhc[c].group - 1);
+    }
+    if ((rc = dalloc(h, (NhcDev**)&h->nhc_state, cs.size()))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->nhc_state, cs.data(), sizeof(NhcDev) * cs.size(), hipMemcpyHostToDevice, h->stream));
+    h->e_nhc.assign(h->nhc.size(), 0.0);
+  }
+  return PFM_OK;
+}
+
+static int check_err(pfm_ctx* h, int s1) {
+  HIPCHK(h, hipMemcpyAsync(h->herr, h->derr, sizeof(DevErr), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipGetLastError());
+  DevErr e = *h->herr;
+  if (e.code == 0) return PFM_OK;
+  if (e.code == PFM_E_OUT_OF_CELL && e.step > s1) {
+    // found by the last drift: the reference stops at the next step's check
+    HIPCHK(h, hipMemsetAsync(h->derr, 0, sizeof(DevErr), h->stream));
+    return PFM_OK;
+  }
+  HIPCHK(h, hipMemsetAsync(h->derr, 0, sizeof(DevErr), h->stream));
+  char buf[256];
+  switch (e.code) {
+    case PFM_E_OVERFLOW: snprintf(buf, sizeof buf, "error: too many neighbours %d %d (step %d)", e.atom, e.count, e.step); break;
+    case PFM_E_OUT_OF_CELL: snprintf(buf, sizeof buf, "%d particle out of cell (step %d)", e.atom, e.step); break;
+    case PFM_E_GRAPHENE_NN:
+      if (e.count < 0) snprintf(buf, sizeof buf, "l1>3 (atom %d, step %d)", e.atom, e.step);
+      else if (e.count > 3) snprintf(buf, sizeof buf, "error: too many gr nearest neibs %d %d (step %d)", e.atom, e.count, e.step);
+      else snprintf(buf, sizeof buf, "error: not enough gr nearest neibs %d %d (step %d)", e.atom, e.count, e.step);
+      break;
+    default: snprintf(buf, sizeof buf, "error: device error %d", e.code);
+  }
+  return fail(h, e.code, buf);
+}
+
+static IntP int_params(pfm_ctx* h, double dt) {
+  IntP P;
+  memset(&P, 0, sizeof P);
+  P.ts0 = dt;
+  P.ts1 = dt / 2.0;
+  P.bxyz = h->roles[1] ? 1u << (h->roles[1] - 1) : 0;
+  P.bz = h->roles[2] ? 1u << (h->roles[2] - 1) : 0;
+  for (int d = 0; d < 3; ++d) P.box[d] = h->box[d];
+  P.nchain = (int)h->nhc.size();
+  for (int c = 0; c < P.nchain; ++c) P.chain_bit[c] = 1u << (h->nhc[c].group - 1);
+  P.chain_s = h->red_out + 16;
+  P.zlo = 0.8 * h->box[2];
+  P.zhi = 0.9 * h->box[2];
+  return P;
+}
+
+static void nhc_half(pfm_ctx* h, double dt) {
+  int nb = h->red_blocks;
+  MaskQ Q;
+  Q.nq = (int)h->nhc.size();
+  for (int c = 0; c < Q.nq; ++c) Q.bit[c] = 1u << (h->nhc[c].group - 1);
+  PFM_LAUNCH(h, "nhc", k_ke_part, nb, RED_THREADS, h->N, h->vel4, h->mask, Q, h->red_part);
+  PFM_LAUNCH(h, "nhc", k_reduce_rows, 1, RED_THREADS, Q.nq, nb, h->red_part, h->red_out + 8);
+  PFM_LAUNCH(h, "nhc", k_nhc_half, 1, 64, Q.nq, (NhcDev*)h->nhc_state, h->red_out + 8, h->red_out + 16, dt / 2.0,
+             dt / 4.0, dt / 8.0);
+}
+
+// Energies of the step: per-interaction sums of the per-row energies and the
+// all_moving group's kinetic energy; one blocking read-back.
+static int energies(pfm_ctx* h, bool nvt) {
+  int nb = h->red_blocks;
+  int ni = (int)h->inter.size();
+  for (int k = 0; k < ni; ++k) {
+    Inter& it = h->inter[k];
+    PFM_LAUNCH(h, "energy", k_sum_part, nb, RED_THREADS, it.e_rows, it.e_row, h->red_part + (size_t)k * nb);
+  }
+  MaskQ Q;
+  Q.nq = 1;
+  Q.bit[0] = 1u << (h->roles[0] - 1);
+  PFM_LAUNCH(h, "energy", k_ke_part, nb, RED_THREADS, h->N, h->vel4, h->mask, Q, h->red_part + (size_t)ni * nb);
+  PFM_LAUNCH(h, "energy", k_reduce_rows, 1, RED_THREADS, ni + 1, nb, h->red_part, h->red_out + 32);
+  HIPCHK(h, hipMemcpyAsync(h->h_red, h->red_out, sizeof(double) * (32 + PFM_MAXI + 1), hipMemcpyDeviceToHost,
+                           h->stream));
+  std::vector<NhcDev> cs(nvt ? h->nhc.size() : 0);
+  if (!cs.empty())
+    HIPCHK(h, hipMemcpyAsync(cs.data(), h->nhc_state, sizeof(NhcDev) * cs.size(), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double pe = 0.0;
+  for (int k = 0; k < ni; ++k) {
+    h->inter[k].energy = h->h_red[32 + k];
+    pe += h->inter[k].energy;
+  }
+  h->pe = pe;
+  h->ke = h->h_red[32 + ni];
+  h->temp = 2 * h->ke / PFM_KB / (3 * h->grp.n[h->roles[0]]);
+  for (size_t c = 0; c < cs.size(); ++c) {
+    // calculate_nose_hoover_chain_energy (md_integrators.f90:247-260)
+    const NhcDev& d = cs[c];
+    double e = d.q[0] / 2 * d.v[0] * d.v[0] + 3.0 * d.L * d.kt * d.x[0];
+    for (int i = 1; i < d.M; ++i) e = e + d.q[i] / 2 * d.v[i] * d.v[i] + d.kt * d.x[i];
+    h->e_nhc[c] = e;
+  }
+  return PFM_OK;
+}
+
+// rebuild every list due at step s, in settings order; re-sort the atoms
+// first when all of them are due (step 0 always)
+static int update_lists(pfm_ctx* h, int s) {
+  bool all_due = true;
+  for (auto& it : h->inter)
+    for (int k = 0; k < it.nl_n; ++k) {
+      int per = (k == 1 || (k == 2 && it.nn_from_tb >= 0)) ? -1 : it.nl[k].period;
+      if (k == 2 && it.nn_from_tb >= 0) per = h->inter[it.nn_from_tb].nl[0].period;
+      if (k == 1 && (it.kind == PFM_LJ || is_cos(it.kind))) per = it.nl[0].period;
+      if (per > 0 && s % per != 0) all_due = false;
+    }
+  int rc;
+  if (all_due || !h->sorted_once) {
+    if ((rc = launch_sort_and_bin(h, s))) return rc;
+    h->sorted_once = true;
+    for (auto& it : h->inter)
+      for (int k = 0; k < it.nl_n; ++k) it.nl[k].built = false;
+  }
+  for (size_t ii = 0; ii < h->inter.size(); ++ii) {
+    Inter& it = h->inter[ii];
+    bool due = s % it.nl[0].period == 0 || !it.nl[0].built;
+    if (due) {
+      if ((rc = launch_build_list(h, (int)ii, 0, s))) return rc;
+      if (it.kind == PFM_LJ || is_cos(it.kind))
+        if ((rc = launch_build_list(h, (int)ii, 1, s))) return rc;
+    }
+    if (is_cos(it.kind)) {
+      if (it.nn_from_tb >= 0) {
+        Inter& tb = h->inter[it.nn_from_tb];
+        if (s % tb.nl[0].period == 0 || !it.nl[2].built)
+          if ((rc = launch_nn_from_tb(h, (int)ii, s))) return rc;
+      } else if (s % it.nl[2].period == 0 || !it.nl[2].built) {
+        if ((rc = launch_build_list(h, (int)ii, 2, s))) return rc;
+      }
+    }
+  }
+  return PFM_OK;
+}
+
+static int zero_momentum(pfm_ctx* h) {
+  int nb = h->red_blocks, g = h->roles[3];
+  if (g == 0 || h->grp.n[g] == 0) return PFM_OK;
+  uint32_t bit = 1u << (g - 1);
+  PFM_LAUNCH(h, "zero_momentum", k_mom_part, nb, RED_THREADS, h->N, h->vel4, h->mask, bit, h->red_part);
+  PFM_LAUNCH(h, "zero_momentum", k_reduce_rows, 1, RED_THREADS, 4, nb, h->red_part, h->red_out);
+  PFM_LAUNCH(h, "zero_momentum", k_sub_mcv, (h->N + 255) / 256, 256, h->N, h->vel4, h->mask, bit, h->red_out);
+  return PFM_OK;
+}
+
+// ---------------------------------------------------------------- C-ABI
+
+extern "C" {
+
+int pfm_create(pfm_ctx** out, int n, const double box[3], const double* pos, const double* vel, const double* mass,
+               int precision) {
+  pfm_ctx* h = new pfm_ctx();
+  *out = h;
+  if (n <= 0) return fail(h, PFM_E_INVALID, "error: particles_num must be positive");
+  if (precision != PFM_FP64 && precision != PFM_MIXED) return fail(h, PFM_E_INVALID, "error: unknown precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(h, PFM_E_NO_DEVICE, "error: no HIP device visible (libpfmds needs an MI355X / gfx950)");
+  HIPCHK(h, hipGetDevice(&h->dev));
+  hipDeviceProp_t prop;
+  HIPCHK(h, hipGetDeviceProperties(&prop, h->dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(h, PFM_E_NO_DEVICE, std::string("error: libpfmds is built for gfx950, device is ") + prop.gcnArchName);
+  HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  h->N = n;
+  h->precision = precision;
+  for (int d = 0; d < 3; ++d) {
+    h->box[d] = box[d];
+    h->hbox[d] = 0.5 * box[d];
+    if (!(box[d] > 0)) return fail(h, PFM_E_INVALID, "error: box size must be positive");
+  }
+  int rc;
+  if ((rc = dalloc(h, &h->pos4, n)) || (rc = dalloc(h, &h->vel4, n)) || (rc = dalloc(h, &h->frc4, n)) ||
+      (rc = dalloc(h, &h->pos4b, n)) || (rc = dalloc(h, &h->vel4b, n)) || (rc = dalloc(h, &h->frc4b, n)) ||
+      (rc = dalloc(h, &h->mask, n)) || (rc = dalloc(h, &h->maskb, n)) || (rc = dalloc(h, &h->orig, n)) ||
+      (rc = dalloc(h, &h->origb, n)) || (rc = dalloc(h, &h->perm, n)) || (rc = dalloc(h, &h->atom_rank, n)) ||
+      (rc = dalloc(h, &h->scan_tmp, 2 * (size_t)n)) || (rc = dalloc(h, &h->eb, n)) || (rc = dalloc(h, &h->inv_eb, n)))
+    return rc;
+  h->red_blocks = std::max(1, std::min(1024, (n + RED_THREADS - 1) / RED_THREADS));
+  if ((rc = dalloc(h, &h->red_part, (size_t)(PFM_MAXI + 1 + RED_MAXQ) * h->red_blocks))) return rc;
+  if ((rc = dalloc(h, &h->red_out, 64))) return rc;
+  if ((rc = dalloc(h, &h->derr, 1))) return rc;
+  HIPCHK(h, hipHostMalloc((void**)&h->h_red, sizeof(double) * 64, 0));
+  HIPCHK(h, hipHostMalloc((void**)&h->herr, sizeof(DevErr), 0));
+  std::vector<double4> p(n), v(n);
+  std::vector<int> o(n);
+  h->hmass.assign(mass, mass + n);
+  for (int i = 0; i < n; ++i) {
+    p[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.0);
+    v[i] = make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], mass[i]);
+    o[i] = i;
+  }
+  HIPCHK(h, hipMemcpyAsync(h->pos4, p.data(), sizeof(double4) * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->vel4, v.data(), sizeof(double4) * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->orig, o.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFM_OK;
+}
+
+int pfm_set_group(pfm_ctx* h, int gid, int n, const int* idx) {
+  if (h->sorted_once) return fail(h, PFM_E_INVALID, "error: groups must be set before the first pfm_run");
+  if (gid < 1 || gid > PFM_MAXG) return fail(h, PFM_E_INVALID, "error: group number out of range (1..32)");
+  std::vector<int>& g = h->grp.idx[gid];
+  g.clear();
+  std::vector<char> seen(h->N, 0);
+  for (int k = 0; k < n; ++k) {
+    int a = idx[k] - 1;
+    if (a < 0 || a >= h->N) return fail(h, PFM_E_INVALID, "error: group index out of range");
+    if (seen[a]) return fail(h, PFM_E_INVALID, "error: atom listed twice in a group (species named twice)");
+    seen[a] = 1;
+    g.push_back(a);
+  }
+  h->grp.n[gid] = n;
+  return PFM_OK;
+}
+
+int pfm_set_roles(pfm_ctx* h, int all_moving, int xyz_moving, int z_moving, int all_atoms, int zmp, int invert_z) {
+  int r[4] = {all_moving, xyz_moving, z_moving, all_atoms};
+  for (int k = 0; k < 4; ++k)
+    if (r[k] < 1 || r[k] > PFM_MAXG) return fail(h, PFM_E_INVALID, "error: role group out of range");
+  if (zmp < 1) return fail(h, PFM_E_INVALID, "error: zero_momentum_period must be >= 1");
+  for (int k = 0; k < 4; ++k) h->roles[k] = r[k];
+  h->zm_period = zmp;
+  h->invert_z = invert_z ? 1 : 0;
+  return PFM_OK;
+}
+
+int pfm_add_interaction(pfm_ctx* h, int kind, const double* params, int np, int nl_n, const int* gn,
+                        const int* nmax, const double* rcut, const int* period) {
+  static const int need_np[7] = {0, 4, 4, 6, 7, 7, 10};
+  static const int need_nl[7] = {0, 2, 1, 3, 3, 1, 1};
+  if (h->sorted_once) return fail(h, PFM_E_INVALID, "error: interactions must be added before the first pfm_run");
+  if (kind < 1 || kind > 6) return fail(h, PFM_E_UNKNOWN_KIND, "error: unknown interaction name");
+  if ((int)h->inter.size() >= PFM_MAXI) return fail(h, PFM_E_INVALID, "error: too many interactions");
+  if (np < need_np[kind] || nl_n != need_nl[kind])
+    return fail(h, PFM_E_INVALID, "error: wrong parameter or list count for interaction");
+  Inter it;
+  it.kind = kind;
+  it.nl_n = nl_n;
+  for (int i = 0; i < np && i < 16; ++i) it.p[i] = params[i];
+  for (int k = 0; k < nl_n; ++k) {
+    int g1 = gn[2 * k], g2 = gn[2 * k + 1];
+    if (g1 < 1 || g1 > PFM_MAXG || g2 < 1 || g2 > PFM_MAXG)
+      return fail(h, PFM_E_INVALID, "error: interaction group out of range");
+    if (period[k] < 1) return fail(h, PFM_E_INVALID, "error: update_period must be >= 1");
+    if (nmax[k] < 1) return fail(h, PFM_E_INVALID, "error: neighb_num_max must be >= 1");
+    ListDev& L = it.nl[k];
+    L.g1 = g1; L.g2 = g2; L.nmax = nmax[k]; L.rcut = rcut[k]; L.period = period[k];
+    L.nrows = h->grp.n[g1];
+    h->grp.used[g1] = true;
+  }
+  if (kind == PFM_LJ1G || kind == PFM_RJL || kind == PFM_TB) {
+    // these loops index group-2 entries with group-1 arrays
+    // (LennardJones_1g.f90:83, RosatoGuillopeLegrand.f90:88, TersoffBrenner.f90:56-57,112)
+    if (h->grp.idx[it.nl[0].g1] != h->grp.idx[it.nl[0].g2])
+      return fail(h, PFM_E_INVALID, "error: lj1g/rjl/tb need the same group on both sides of the list");
+  }
+  if (kind == PFM_TB && it.nl[0].nmax > PFM_TB_MAXN)
+    return fail(h, PFM_E_INVALID, "error: tb neighb_num_max above 16 is not supported");
+  if (kind == PFM_LJ1G) h->grp.used[it.nl[0].g2] = true;
+  if (is_cos(kind)) {
+    if (it.nl[2].nmax != 3) return fail(h, PFM_E_GRAPHENE_NN, "error: nl_nn%neighb_num_max/=nnum_nn");
+    if (h->grp.idx[it.nl[2].g1] != h->grp.idx[it.nl[0].g1])
+      return fail(h, PFM_E_INVALID, "error: ljc/morsec nearest-neighbour rows must be the graphene group of list 1");
+    // md_interactions.f90:159-171: the first tb interaction's list feeds nl(3)
+    for (size_t j = 0; j < h->inter.size(); ++j)
+      if (h->inter[j].kind == PFM_TB) { it.nn_from_tb = (int)j; break; }
+    if (it.nn_from_tb >= 0) {
+      Inter& tb = h->inter[it.nn_from_tb];
+      if (h->grp.idx[tb.nl[0].g1] != h->grp.idx[it.nl[0].g1])
+        return fail(h, PFM_E_INVALID, "error: tb group and ljc/morsec graphene group differ");
+      it.nl[2].period = tb.nl[0].period;
+    }
+  }
+  if (kind == PFM_TB) {
+    for (auto& o : h->inter)
+      if (is_cos(o.kind) && o.nn_from_tb < 0)
+        // the reference copies the empty tb list at step 0 and stops
+        return fail(h, PFM_E_GRAPHENE_NN, "error: not enough gr nearest neibs (tb listed after ljc/morsec)");
+  }
+  h->inter.push_back(it);
+  return PFM_OK;
+}
+
+int pfm_add_nhc(pfm_ctx* h, int group, double T, int M, double q1) {
+  if ((int)h->nhc.size() >= PFM_MAXNHC || M < 1 || M > 16 || group < 1 || group > PFM_MAXG)
+    return fail(h, PFM_E_INVALID, "error: bad nhc definition");
+  int L = h->grp.n[group];
+  if (L < 1 || q1 < 0.0 || T < 0.0) return fail(h, PFM_E_NHC_PARAMS, "error: wrong nhc parameters");
+  for (auto& c : h->nhc) {
+    std::vector<int> a = h->grp.idx[c.group], b = h->grp.idx[group];
+    std::sort(a.begin(), a.end());
+    std::sort(b.begin(), b.end());
+    std::vector<int> x;
+    std::set_intersection(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(x));
+    if (!x.empty()) return fail(h, PFM_E_INVALID, "error: thermostat groups overlap");
+  }
+  Nhc c;
+  c.group = group; c.M = M; c.L = L; c.T = T; c.q1 = q1;
+  h->nhc.push_back(c);
+  h->grp.used[group] = true;
+  return PFM_OK;
+}
+
+int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end, double ms_de, int* last_step,
+            int* event) {
+  if (integ < PFM_NVT || integ > PFM_NVMS) return fail(h, PFM_E_INVALID, "error: unknown integrator");
+  if (h->roles[0] == 0) return fail(h, PFM_E_INVALID, "error: pfm_set_roles was not called");
+  for (int k = 0; k < 4; ++k) h->grp.used[h->roles[k]] = true;
+  int rc;
+  if (!h->sorted_once && (rc = finalize(h))) return rc;
+  *event = PFM_EV_DONE;
+  *last_step = s0;
+  IntP P = int_params(h, dt);
+  const int N = h->N, nb = (N + 255) / 256;
+  uint32_t all_bit = 1u << (h->roles[3] - 1);
+  bool everyone = h->grp.n[h->roles[3]] == N;
+  bool first_writes = false;
+  if (everyone && !h->inter.empty()) {
+    Inter& f = h->inter[0];
+    first_writes = (f.kind == PFM_RJL || f.kind == PFM_LJ1G || f.kind == PFM_LJ || f.kind == PFM_TB) &&
+                   f.nl[0].nrows == N;
+  }
+  // check_positions (and invert_z) of the first step of the call
+  PFM_LAUNCH(h, "check", k_check, nb, 256, N, h->pos4, h->box[0], h->box[1], h->box[2], h->derr, h->orig, s0,
+             (s0 == 0 && h->invert_z) ? 1 : 0, h->vel4, P.zlo, P.zhi);
+  if ((rc = check_err(h, s1))) return rc;
+  for (int s = s0; s <= s1; ++s) {
+    *last_step = s;
+    bool energy = (s == s1 && energy_at_end) || integ == PFM_NVMS;
+    if (s != 0) {
+      if (integ == PFM_NVMS && std::fabs(h->pe - h->pe_prev) <= ms_de) {
+        *event = PFM_EV_CONVERGED;
+        return PFM_OK;
+      }
+      IntP Q = P;
+      Q.invert = h->invert_z;
+      if (integ == PFM_NVT && !h->nhc.empty()) {
+        nhc_half(h, dt);
+        Q.scale = 1;
+      }
+      PFM_LAUNCH(h, "kick_drift", k_kick_drift, nb, 256, N, h->pos4, h->vel4, h->frc4, h->mask, Q, h->derr, h->orig, s);
+    }
+    h->pe_prev = h->pe;
+    if ((rc = update_lists(h, s))) return rc;
+    if (s % h->zm_period == 0 && (rc = zero_momentum(h))) return rc;
+    if (!first_writes) PFM_LAUNCH(h, "zero_forces", k_zero_masked, nb, 256, N, h->frc4, h->mask, all_bit);
+    if ((rc = launch_forces(h, energy, first_writes))) return rc;
+    if (s != 0) {
+      PFM_LAUNCH(h, "kick2", k_kick2, nb, 256, N, h->vel4, h->frc4, h->mask, P, integ == PFM_NVMS ? 1 : 0);
+      if (integ == PFM_NVT && !h->nhc.empty()) {
+        nhc_half(h, dt);
+        PFM_LAUNCH(h, "nhc_scale", k_scale, nb, 256, N, h->vel4, h->mask, P);
+      }
+      h->sim_time = h->sim_time + dt;
+    }
+    if (energy) {
+      if ((rc = energies(h, integ == PFM_NVT))) return rc;
+      if ((rc = check_err(h, s))) return rc;
+    }
+  }
+  return check_err(h, s1);
+}
+
+int pfm_energies(pfm_ctx* h, double* e_inter, double* pe, double* ke, double* temp, double* e_nhc, double* t) {
+  if (e_inter)
+    for (size_t i = 0; i < h->inter.size(); ++i) e_inter[i] = h->inter[i].energy;
+  if (pe) *pe = h->pe;
+  if (ke) *ke = h->ke;
+  if (temp) *temp = h->temp;
+  if (e_nhc)
+    for (size_t i = 0; i < h->e_nhc.size(); ++i) e_nhc[i] = h->e_nhc[i];
+  if (t) *t = h->sim_time;
+  return PFM_OK;
+}
+
+static int download_orig(pfm_ctx* h, std::vector<int>& o) {
+  o.resize(h->N);
+  HIPCHK(h, hipMemcpyAsync(o.data(), h->orig, sizeof(int) * h->N, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFM_OK;
+}
+
+int pfm_get_state(pfm_ctx* h, double* pos, double* vel, double* force) {
+  std::vector<int> o;
+  int rc;
+  if ((rc = download_orig(h, o))) return rc;
+  std::vector<double4> buf(h->N);
+  double4* src[3] = {h->pos4, h->vel4, h->frc4};
+  double* dst[3] = {pos, vel, force};
+  for (int k = 0; k < 3; ++k) {
+    if (!dst[k]) continue;
+    HIPCHK(h, hipMemcpyAsync(buf.data(), src[k], sizeof(double4) * h->N, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int i = 0; i < h->N; ++i) {
+      int a = o[i];
+      dst[k][3 * a] = buf[i].x; dst[k][3 * a + 1] = buf[i].y; dst[k][3 * a + 2] = buf[i].z;
+    }
+  }
+  return PFM_OK;
+}
+
+int pfm_set_state(pfm_ctx* h, const double* pos, const double* vel) {
+  std::vector<int> o;
+  int rc;
+  if ((rc = download_orig(h, o))) return rc;
+  std::vector<double4> p(h->N), v(h->N);
+  HIPCHK(h, hipMemcpyAsync(p.data(), h->pos4, sizeof(double4) * h->N, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(v.data(), h->vel4, sizeof(double4) * h->N, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int i = 0; i < h->N; ++i) {
+    int a = o[i];
+    if (pos) { p[i].x = pos[3 * a]; p[i].y = pos[3 * a + 1]; p[i].z = pos[3 * a + 2]; }
+    if (vel) { v[i].x = vel[3 * a]; v[i].y = vel[3 * a + 1]; v[i].z = vel[3 * a + 2]; }
+  }
+  HIPCHK(h, hipMemcpyAsync(h->pos4, p.data(), sizeof(double4) * h->N, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->vel4, v.data(), sizeof(double4) * h->N, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (auto& it : h->inter)
+    for (int k = 0; k < it.nl_n; ++k) it.nl[k].built = false;
+  return PFM_OK;
+}
+
+int pfm_get_list(pfm_ctx* h, int inter, int k, int* n_rows, int* nnum, int* nlist, int* lessnnum, int stride) {
+  if (inter < 1 || inter > (int)h->inter.size()) return fail(h, PFM_E_INVALID, "error: bad interaction index");
+  Inter& it = h->inter[inter - 1];
+  if (k < 1 || k > it.nl_n) return fail(h, PFM_E_INVALID, "error: bad list index");
+  ListDev& L = it.nl[k - 1];
+  *n_rows = L.nrows;
+  if (!nnum) return PFM_OK;
+  if (!L.built || L.nrows == 0) {
+    for (int r = 0; r < L.nrows; ++r) { nnum[r] = 0; if (lessnnum) lessnnum[r] = 0; }
+    return PFM_OK;
+  }
+  std::vector<int> o;
+  int rc;
+  if ((rc = download_orig(h, o))) return rc;
+  std::vector<int> rows(L.nrows), cnt(L.nrows), nb((size_t)L.nrows * L.stride);
+  HIPCHK(h, hipMemcpyAsync(rows.data(), h->grp.rows[L.g1], sizeof(int) * L.nrows, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(cnt.data(), L.nnum, sizeof(int) * L.nrows, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(nb.data(), L.nbr, sizeof(int) * nb.size(), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::vector<int> loc1(h->N, -1), loc2(h->N, -1);
+  for (size_t q = 0; q < h->grp.idx[L.g1].size(); ++q) loc1[h->grp.idx[L.g1][q]] = (int)q;
+  int g2 = (k == 3 && it.nn_from_tb >= 0) ? L.g1 : L.g2;
+  for (size_t q = 0; q < h->grp.idx[g2].size(); ++q) loc2[h->grp.idx[g2][q]] = (int)q;
+  for (int r = 0; r < L.nrows; ++r) {
+    int a = o[rows[r]];
+    int row = loc1[a];
+    int n = cnt[r];
+    if (n > stride) return fail(h, PFM_E_INVALID, "error: stride smaller than a row");
+    std::vector<int> e(n);
+    for (int p = 0; p < n; ++p) e[p] = loc2[o[nb[(size_t)r * L.stride + p]]];
+    std::sort(e.begin(), e.end());
+    nnum[row] = n;
+    int less = -1;
+    for (int p = 0; p < n; ++p) {
+      nlist[(size_t)row * stride + p] = e[p] + 1;
+      if (less < 0 && h->grp.idx[g2][e[p]] > a) less = p;
+    }
+    if (lessnnum) lessnnum[row] = less < 0 ? n : less;
+  }
+  return PFM_OK;
+}
+
+int pfm_get_normals(pfm_ctx* h, int inter, double* out) {
+  if (inter < 1 || inter > (int)h->inter.size()) return fail(h, PFM_E_INVALID, "error: bad interaction index");
+  Inter& it = h->inter[inter - 1];
+  if (!is_cos(it.kind) || !it.gnorm) return fail(h, PFM_E_INVALID, "error: interaction has no graphene normals");
+  std::vector<int> o;
+  int rc;
+  if ((rc = download_orig(h, o))) return rc;
+  std::vector<double4> g(h->N);
+  HIPCHK(h, hipMemcpyAsync(g.data(), it.gnorm, sizeof(double4) * h->N, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::vector<int> loc(h->N, -1);
+  const std::vector<int>& gi = h->grp.idx[it.nl[2].g1];
+  for (size_t q = 0; q < gi.size(); ++q) loc[gi[q]] = (int)q;
+  for (int i = 0; i < h->N; ++i) {
+    int l = loc[o[i]];
+    if (l < 0) continue;
+    out[3 * l] = g[i].x; out[3 * l + 1] = g[i].y; out[3 * l + 2] = g[i].z;
+  }
+  return PFM_OK;
+}
+
+int pfm_set_timing(pfm_ctx* h, int on) {
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->timing = on != 0;
+  h->timers.clear();
+  h->evnext = 0;
+  return PFM_OK;
+}
+
+int pfm_kernel_time(pfm_ctx* h, const char* name, int* count, double* mean_ms) {
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  int c = 0;
+  double tot = 0.0;
+  for (auto& t : h->timers) {
+    if (t.name.find(name) == std::string::npos) continue;
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, t.a, t.b));
+    tot += ms;
+    ++c;
+  }
+  *count = c;
+  *mean_ms = c ? tot / c : 0.0;
+  return PFM_OK;
+}
+
+int pfm_set_tuning(pfm_ctx* h, int lanes, int block) {
+  if (lanes) {
+    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32)
+      return fail(h, PFM_E_INVALID, "error: lanes_per_row must be 1,2,4,8,16 or 32");
+    h->lanes = lanes;
+  }
+  if (block) h->block = block;
+  return PFM_OK;
+}
+
+int pfm_last_error(pfm_ctx* h, char* buf, int len) {
+  if (len > 0) {
+    std::strncpy(buf, h->err.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  return PFM_OK;
+}
+
+int pfm_destroy(pfm_ctx* h) {
+  if (!h) return PFM_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  auto F = [](void* p) { if (p) (void)hipFree(p); };
+  F(h->pos4); F(h->vel4); F(h->frc4); F(h->pos4b); F(h->vel4b); F(h->frc4b); F(h->mask); F(h->maskb);
+  F(h->orig); F(h->origb); F(h->perm); F(h->atom_rank); F(h->scan_tmp); F(h->eb); F(h->inv_eb);
+  F(h->red_part); F(h->red_out); F(h->derr); F(h->cell_rank); F(h->cell_count); F(h->cell_start); F(h->cell_fill);
+  F(h->nhc_state);
+  if (h->h_red) (void)hipHostFree(h->h_red);
+  if (h->herr) (void)hipHostFree(h->herr);
+  for (int g = 0; g <= PFM_MAXG; ++g) { F(h->grp.rows[g]); F(h->grp.row_of[g]); F(h->grp.loc[g]); }
+  for (auto& it : h->inter) {
+    for (int k = 0; k < 3; ++k) { F(it.nl[k].nnum); F(it.nl[k].nbr); F(it.nl[k].rev); }
+    F(it.e_row); F(it.cut); F(it.B); F(it.Bp); F(it.gnorm); F(it.gS);
+  }
+  for (auto e : h->evpool) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return PFM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,641 @@
+// pfm_forces.hip -- force and energy passes on gfx950.
+//
+// Every pass is a gather over full neighbour lists: T lanes share one atom
+// row, each lane walks every T-th entry, the row's force is summed across the
+// T lanes with DPP/shuffle reductions and written by the row's first lane.
+// No atomics; per-atom sums are FP64 in both precisions.  Template parameter
+// R is the pair arithmetic (double, or float for the mixed path: dr formed in
+// FP64 and rounded once), E adds the per-row energy of the reference's
+// *_energy routines.
+//
+//   rjl     density pass + embedding-force pass   (RosatoGuillopeLegrand.f90:23-94)
+//   lj      pair gather over nl(1) and nl(2)      (LennardJones.f90:23-69)
+//   lj1g    pair gather over the full list with the reference's half-list
+//           weights                                (LennardJones_1g.f90:28-117)
+//   tb      bond-order pass + force pass          (TersoffBrenner.f90:26-150)
+//   ljc/morsec  normals, graphene pass, normal-derivative pass, metal pass
+//           (graphenenorm.f90:38-56, LennardJonesCosine.f90, MorseCosine.f90)
+#include "pfm_internal.h"
+
+struct RowArgs {
+  int nrows, stride;
+  const int* rows;
+  const int* nnum;
+  const int* nbr;
+  const double4* pos;
+  double3 hb;
+  double4* frc;
+  int accumulate;  // 0: this pass writes the first contribution of the step
+  double* e_row;
+};
+
+#define ROW_PROLOGUE(T)                                   \
+  int lb = xcd_block(nb_real);                            \
+  if (lb < 0) return;                                     \
+  int gt = lb * blockDim.x + threadIdx.x;                 \
+  int row = gt / (T), t = threadIdx.x & ((T)-1);          \
+  if (row >= a.nrows) return;                             \
+  int i = a.rows[row];                                    \
+  double4 pi = a.pos[i];                                  \
+  int n = a.nnum[row];                                    \
+  const int* nbrow = a.nbr + (size_t)row * a.stride;
+
+__device__ __forceinline__ void write_force(const RowArgs& a, int i, double fx, double fy, double fz) {
+  double4 f = a.accumulate ? a.frc[i] : make_double4(0, 0, 0, 0);
+  f.x += fx; f.y += fy; f.z += fz;
+  a.frc[i] = f;
+}
+
+// ------------------------------------------------------------------- RGL
+
+struct RjlP {
+  double A0, xi, p, q, r0, R1, R2, invw;
+};
+
+template <int T, class R>
+__global__ void __launch_bounds__(256) k_rjl_density(RowArgs a, RjlP P, double* __restrict__ eb,
+                                                     double* __restrict__ inv_eb, int nb_real) {
+  ROW_PROLOGUE(T)
+  const R R1 = P.R1, R2 = P.R2, invw = P.invw, mq = -2.0 * P.q, ir0 = 1.0 / P.r0;
+  double rho = 0.0;
+  for (int p = t; p < n; p += T) {
+    int j = nbrow[p];
+    double dx, dy, dz;
+    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    R r = PM<R>::sqrt_((R)d2);
+    if (r <= R2) {
+      R fc, dfc;
+      cos_cutoff<R>(r, R1, R2, invw, fc, dfc);
+      rho += (double)(PM<R>::exp_(mq * (r * ir0 - R(1))) * fc);
+    }
+  }
+  rho = group_sum<T>(rho);
+  if (t == 0) {
+    double s = sqrt(rho);
+    eb[i] = s;
+    inv_eb[i] = 1.0 / s;
+  }
+}
+
+template <int T, class R, bool E>
+__global__ void __launch_bounds__(256) k_rjl_force(RowArgs a, RjlP P, const double* __restrict__ eb,
+                                                   const double* __restrict__ inv_eb, int nb_real) {
+  ROW_PROLOGUE(T)
+  const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0;
+  const R mp = -P.p, mq = -2.0 * P.q, c2A = 2.0 * P.A0, pr0 = P.p / P.r0, qr0 = P.q / P.r0, xi = P.xi;
+  const R ie = (R)inv_eb[i];
+  double fx = 0, fy = 0, fz = 0, er = 0;
+  for (int p = t; p < n; p += T) {
+    int j = nbrow[p];
+    double dx, dy, dz;
+    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    R r = PM<R>::sqrt_((R)d2);
+    if (r <= R2) {
+      R fc, dfc;
+      cos_cutoff<R>(r, R1, R2, invw, fc, dfc);
+      R x = r * ir0 - R(1);
+      R ep = PM<R>::exp_(mp * x), eq = PM<R>::exp_(mq * x);
+      R c = (c2A * (pr0 * fc - dfc * r) * ep - xi * (qr0 * fc - dfc * R(0.5) * r) * (ie + (R)inv_eb[j]) * eq) *
+            PM<R>::rcp(r);
+      double cd = (double)c;
+      fx -= cd * dx; fy -= cd * dy; fz -= cd * dz;
+      if (E) er += (double)(ep * fc);
+    }
+  }
+  fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
+  if (E) er = group_sum<T>(er);
+  if (t == 0) {
+    write_force(a, i, fx, fy, fz);
+    if (E) a.e_row[row] = P.A0 * er - P.xi * eb[i];
+  }
+}
+
+// -------------------------------------------------------------------- LJ
+
+struct LjP {
+  double eps4, sig, R1, R2, invw;
+};
+
+template <int T, class R, bool E>
+__global__ void __launch_bounds__(256) k_lj(RowArgs a, LjP P, int nb_real) {
+  ROW_PROLOGUE(T)
+  const R R1 = P.R1, R2 = P.R2, invw = P.invw, sig = P.sig, e4 = P.eps4;
+  double fx = 0, fy = 0, fz = 0, e = 0;
+  for (int p = t; p < n; p += T) {
+    int j = nbrow[p];
+    double dx, dy, dz;
+    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    R r = PM<R>::sqrt_((R)d2);
+    if (r < R2) {
+      R fc, dfc;
+      cos_cutoff<R>(r, R1, R2, invw, fc, dfc);
+      R s = sig * PM<R>::rcp(r), s2 = s * s, V = s2 * s2 * s2;
+      R c = e4 * (V * (R(12) * V - R(6)) * PM<R>::rcp(r * r) * fc - V * (V - R(1)) * dfc);
+      double cd = (double)c;
+      fx -= cd * dx; fy -= cd * dy; fz -= cd * dz;
+      if (E) e += (double)(e4 * V * (V - R(1)) * fc);
+    }
+  }
+  fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
+  if (E) e = group_sum<T>(e);
+  if (t == 0) {
+    write_force(a, i, fx, fy, fz);
+    if (E) a.e_row[row] = e;
+  }
+}
+
+// lj1g: the reference sums entries p <= lessnnum(i) of each row once and
+// applies Newton's third law (LennardJones_1g.f90:63-99).  Entry k of row i is
+// in that half iff loc(k) < cut(i); the full-list gather weights pair (i,k)
+// by [loc(k) < cut(i)] + [loc(i) < cut(k)], which reproduces the reference
+// for any group order (for ascending groups the weight is always 1).
+struct Lj1gP {
+  double c6, c12, c6t6, c12t12, R1, R2, invw;
+};
+
+template <int T, class R, bool E>
+__global__ void __launch_bounds__(256) k_lj1g(RowArgs a, Lj1gP P, const int* __restrict__ orig,
+                                              const int* __restrict__ loc, const int* __restrict__ cut, int nb_real) {
+  ROW_PROLOGUE(T)
+  const R R1 = P.R1, R2 = P.R2, invw = P.invw, c6 = P.c6, c12 = P.c12, c6t6 = P.c6t6, c12t12 = P.c12t12;
+  const int li = loc[orig[i]], ci = cut[i];
+  double fx = 0, fy = 0, fz = 0, e = 0;
+  for (int p = t; p < n; p += T) {
+    int j = nbrow[p];
+    int lj = loc[orig[j]];
+    int wh = lj < ci ? 1 : 0;
+    int w = wh + (li < cut[j] ? 1 : 0);
+    if (w == 0) continue;
+    double dx, dy, dz;
+    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    R r = PM<R>::sqrt_((R)d2);
+    // f_dfr_cut (cut_off_poly.f90:30-43), dfr without the 1/(R2-R1) factor
+    R tr = r < R1 ? R1 : (r > R2 ? R2 : r);
+    R x = (tr - R1) * invw, x2 = x * x;
+    R f = R(1) + x2 * x * (R(-10) + R(15) * x - R(6) * x2);
+    R dfr = tr * x2 * (R(-30) + R(60) * x - R(30) * x2);
+    R invr2 = PM<R>::rcp(r * r), U = invr2 * invr2 * invr2;
+    R Fs = U * invr2 * ((c12t12 * U - c6t6) * f - (c12 * U - c6) * dfr);
+    double cd = (double)Fs * (double)w;
+    fx -= cd * dx; fy -= cd * dy; fz -= cd * dz;
+    if (E && wh) {
+      // cut_off_poly f_cut (cut_off_poly.f90:6-17)
+      R fe = (r > R1 && r < R2) ? f : (r <= R1 ? R(1) : R(0));
+      e += (double)(U * (c12 * U - c6) * fe);
+    }
+  }
+  fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
+  if (E) e = group_sum<T>(e);
+  if (t == 0) {
+    write_force(a, i, fx, fy, fz);
+    if (E) a.e_row[row] = e;
+  }
+}
+
+// -------------------------------------------------------------------- TB
+
+struct TbP {
+  double D, S, beta, r0, delt, a0, c02, d02, R1, R2, invw;
+};
+
+// bond order B(p,i) (TersoffBrenner.f90:81-91) and, per entry, the factors
+// of the force pass: Bp = B**(1/delt+1) and
+// BT = delt/2 * Bp * fc * D/(S-1) * S * exp(-sqrt(2S) beta (r-r0)/S)
+template <class R>
+__global__ void __launch_bounds__(256) k_tb_bond(RowArgs a, TbP P, double* __restrict__ B, double* __restrict__ Bp,
+                                                 double* __restrict__ BT, int nb_real) {
+  ROW_PROLOGUE(1)
+  (void)t;
+  const R R1 = P.R1, R2 = P.R2, invw = P.invw, a0 = P.a0, c02 = P.c02, d02 = P.d02;
+  R dxs[PFM_TB_MAXN], dys[PFM_TB_MAXN], dzs[PFM_TB_MAXN], rs[PFM_TB_MAXN], fcs[PFM_TB_MAXN];
+  for (int p = 0; p < n; ++p) {
+    double dx, dy, dz;
+    double d2 = min_image_dist2(pi, a.pos[nbrow[p]], a.hb, dx, dy, dz);
+    dxs[p] = (R)dx; dys[p] = (R)dy; dzs[p] = (R)dz;
+    rs[p] = PM<R>::sqrt_((R)d2);
+    R df;
+    cos_cutoff<R>(rs[p], R1, R2, invw, fcs[p], df);
+  }
+  const R ex = R(1.0 / P.delt + 1.0), md = R(-P.delt);
+  const R tailc = R(P.D / (P.S - 1.0) * P.S), tailk = R(-sqrt(2.0 * P.S) * P.beta / P.S), r0 = P.r0;
+  for (int p = 0; p < n; ++p) {
+    R b = R(0);
+    if (rs[p] < R2) {
+      for (int q = 0; q < n; ++q) {
+        if (q != p && rs[q] < R2) {
+          R cs = R(1) + (dxs[p] * dxs[q] + dys[p] * dys[q] + dzs[p] * dzs[q]) * PM<R>::rcp(rs[p] * rs[q]);
+          b += fcs[q] * (R(1) + c02 / d02 - c02 * PM<R>::rcp(d02 + cs * cs));
+        }
+      }
+      b = PM<R>::pow_(R(1) + a0 * b, md);
+    }
+    size_t e = (size_t)row * a.stride + p;
+    R bp = b > R(0) ? PM<R>::pow_(b, ex) : R(0);
+    B[e] = (double)b;
+    Bp[e] = (double)bp;
+    BT[e] = (double)(R(0.5 * P.delt) * bp * fcs[p] * tailc * PM<R>::exp_(tailk * (rs[p] - r0)));
+  }
+}
+
+// force pass (TersoffBrenner.f90:193-242): for each bond (i,j): dB_ij/dr_i
+// over i's row, dB_ji/dr_i and i as third body of j's bonds over j's row
+template <class R, bool E>
+__global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* __restrict__ row_of,
+                                                  const int* __restrict__ rev, const double* __restrict__ B,
+                                                  const double* __restrict__ Bp, const double* __restrict__ BT,
+                                                  int nb_real) {
+  ROW_PROLOGUE(1)
+  (void)t;
+  const R R1 = P.R1, R2 = P.R2, invw = P.invw, a0 = P.a0, c02 = P.c02, d02 = P.d02;
+  const R D_S1 = R(P.D / (P.S - 1.0)), S = P.S, r0 = P.r0;
+  const R s2S = R(sqrt(2.0 * P.S) * P.beta), s2oS = R(sqrt(2.0 / P.S) * P.beta), hdel = R(-P.delt * 0.5);
+  R dxs[PFM_TB_MAXN], dys[PFM_TB_MAXN], dzs[PFM_TB_MAXN], rs[PFM_TB_MAXN], fcs[PFM_TB_MAXN], dfcs[PFM_TB_MAXN];
+  for (int p = 0; p < n; ++p) {
+    double dx, dy, dz;
+    double d2 = min_image_dist2(pi, a.pos[nbrow[p]], a.hb, dx, dy, dz);
+    dxs[p] = (R)dx; dys[p] = (R)dy; dzs[p] = (R)dz;
+    rs[p] = PM<R>::sqrt_((R)d2);
+    cos_cutoff<R>(rs[p], R1, R2, invw, fcs[p], dfcs[p]);
+  }
+  double fx = 0, fy = 0, fz = 0, e = 0;
+  for (int p = 0; p < n; ++p) {
+    const R xp = dxs[p], yp = dys[p], zp = dzs[p], rp = rs[p];
+    const R irp2 = PM<R>::rcp(rp * rp);
+    R bx = 0, by = 0, bz = 0;
+    for (int q = 0; q < n; ++q) {
+      if (q == p) continue;
+      const R xq = dxs[q], yq = dys[q], zq = dzs[q], rq = rs[q];
+      R rr = PM<R>::rcp(rp * rq);
+      R cosi = (xp * xq + yp * yq + zp * zq) * rr;
+      R tt = d02 + (R(1) + cosi) * (R(1) + cosi);
+      R c1 = fcs[q] * R(2) * a0 * c02 * (R(1) + cosi) * PM<R>::rcp(tt * tt);
+      R c2 = dfcs[q] * a0 * (R(1) + c02 / d02 - c02 * PM<R>::rcp(tt));
+      R irq2 = PM<R>::rcp(rq * rq);
+      bx += c1 * ((xp + xq) * rr - cosi * (xp * irp2 + xq * irq2)) + c2 * xq;
+      by += c1 * ((yp + yq) * rr - cosi * (yp * irp2 + yq * irq2)) + c2 * yq;
+      bz += c1 * ((zp + zq) * rr - cosi * (zp * irp2 + zq * irq2)) + c2 * zq;
+    }
+    size_t ep = (size_t)row * a.stride + p;
+    R bpp = (R)Bp[ep];
+    bx *= bpp; by *= bpp; bz *= bpp;
+    int j = nbrow[p];
+    int jr = row_of[j];
+    int l = rev[ep];
+    if (jr < 0 || l < 0) continue;  // asymmetric list: reported by k_rev
+    int nj = a.nnum[jr];
+    size_t jbase = (size_t)jr * a.stride;
+    const R Bl = (R)B[jbase + l], Bpl = (R)Bp[jbase + l];
+    // vector from j to i and its cutoff
+    const R xl = -xp, yl = -yp, zl = -zp, rl = rp, irl2 = irp2;
+    const R fcl = fcs[p], dfcl = dfcs[p];
+    double4 pj = a.pos[j];
+    R tx = 0, ty = 0, tz = 0;
+    for (int q = 0; q < nj; ++q) {
+      if (q == l) continue;
+      int k = a.nbr[jbase + q];
+      double dx, dy, dz;
+      double d2 = min_image_dist2(pj, a.pos[k], a.hb, dx, dy, dz);
+      const R xq = (R)dx, yq = (R)dy, zq = (R)dz, rq = PM<R>::sqrt_((R)d2);
+      R fq, dfq;
+      cos_cutoff<R>(rq, R1, R2, invw, fq, dfq);
+      R rr = PM<R>::rcp(rl * rq);
+      R cosi = (xl * xq + yl * yq + zl * zq) * rr;
+      R tt = d02 + (R(1) + cosi) * (R(1) + cosi);
+      R gq = R(2) * a0 * c02 * (R(1) + cosi) * PM<R>::rcp(tt * tt);
+      R vx = -xq * rr + cosi * xl * irl2, vy = -yq * rr + cosi * yl * irl2, vz = -zq * rr + cosi * zl * irl2;
+      R c1 = Bpl * fq * gq;
+      bx += c1 * vx; by += c1 * vy; bz += c1 * vz;
+      // i as third body of bond (j,k)
+      R bt = (R)BT[jbase + q];
+      R g2 = dfcl * a0 * (R(1) + c02 / d02 - c02 * PM<R>::rcp(tt));
+      R g1 = fcl * gq;
+      tx += bt * (g1 * vx + xp * g2);
+      ty += bt * (g1 * vy + yp * g2);
+      tz += bt * (g1 * vz + zp * g2);
+    }
+    bx *= hdel; by *= hdel; bz *= hdel;
+    fx += (double)tx; fy += (double)ty; fz += (double)tz;
+    if (rp < R2) {
+      R fc = fcs[p];
+      R dff = dfcs[p] * PM<R>::rcp(fc);
+      R av = -s2S * (rp - r0);
+      R ea = PM<R>::exp_(av), eas = PM<R>::exp_(av * PM<R>::rcp(S));
+      R pre = fc * D_S1;
+      R Bm = (R)B[ep] + Bl;
+      R k1 = (dff - s2S * PM<R>::rcp(rp)) * ea;
+      R k2 = (dff - s2oS * PM<R>::rcp(rp)) * Bm * R(0.5);
+      R se = S * eas;
+      fx += (double)(pre * (xp * k1 - (bx + xp * k2) * se));
+      fy += (double)(pre * (yp * k1 - (by + yp * k2) * se));
+      fz += (double)(pre * (zp * k1 - (bz + zp * k2) * se));
+      if (E) e += 0.5 * (double)(pre * (ea - Bm * R(0.5) * se));
+    }
+  }
+  write_force(a, i, fx, fy, fz);
+  if (E) a.e_row[row] = e;
+}
+
+// ------------------------------------------------------------ ljc / morsec
+
+struct CosP {
+  double pref, sig, a, r0, delt, R1, R2, invw, nn_coef;
+  int morse;
+};
+
+// graphene normal from the three nearest neighbours (graphenenorm.f90:38-56)
+__global__ void k_gnorm(int nrows, const int* __restrict__ rows, const int* __restrict__ nn, const double4* pos,
+                        double3 hb, double4* __restrict__ gnorm, int simplified) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int i = rows[r];
+  if (simplified) {
+    gnorm[i] = make_double4(0, 0, 1, 0);
+    return;
+  }
+  double4 pi = pos[i];
+  double d[3][3];
+  for (int k = 0; k < 3; ++k) min_image_dist2(pi, pos[nn[(size_t)r * 3 + k]], hb, d[k][0], d[k][1], d[k][2]);
+  double ax = d[1][0] - d[0][0], ay = d[1][1] - d[0][1], az = d[1][2] - d[0][2];
+  double bx = d[0][0] - d[2][0], by = d[0][1] - d[2][1], bz = d[0][2] - d[2][2];
+  double gx = ay * bz - az * by, gy = az * bx - ax * bz, gz = ax * by - ay * bx;
+  if (gz < 0.0) { gx = -gx; gy = -gy; gz = -gz; }
+  double s = sqrt(gx * gx + gy * gy + gz * gz);
+  gnorm[i] = make_double4(gx / s, gy / s, gz / s, 0);
+}
+
+template <class R>
+__device__ __forceinline__ void cos_pair(const CosP& P, R r, R nd, R nnorm_r, R& A, R& Bc, R& S_coef, R& epair) {
+  const R R1 = P.R1, R2 = P.R2, invw = P.invw, delt = P.delt;
+  R fc, dfc;
+  cos_cutoff<R>(r, R1, R2, invw, fc, dfc);
+  R ir = PM<R>::rcp(r);
+  R V2;
+  if (P.morse) V2 = PM<R>::exp_(-R(P.a) * (r - R(P.r0)));
+  else { R s = R(P.sig) * ir, s2 = s * s; V2 = s2 * s2 * s2; }
+  R V1 = V2 * V2;
+  R c = nd < R(0) ? -nd : nd;
+  R V3 = PM<R>::pow_(c * nnorm_r, delt);
+  R ind = PM<R>::rcp(nd);
+  if (P.morse) {
+    A = (R(2) * (R(P.a) * V1 - (R(P.a) + delt * ir) * V2 * V3) * ir * fc - (V1 - R(2) * V2 * V3) * dfc);
+    Bc = R(2) * delt * V2 * V3 * ind * fc;
+    epair = V1 - R(2) * V2 * V3;
+  } else {
+    A = ((R(12) * V1 - (R(6) + delt) * V2 * V3) * ir * ir * fc - (V1 - V2 * V3) * dfc);
+    Bc = delt * V2 * V3 * ind * fc;
+    epair = V1 - V2 * V3;
+  }
+  epair *= fc;
+  S_coef = V2 * V3 * fc * ind;
+}
+
+// graphene side, pair and normal terms (*_forces_for_graphene first loop);
+// also S_i = sum V2 V3 fc/(n.dr) dr for the normal-derivative pass
+template <int T, class R, bool E>
+__global__ void __launch_bounds__(256) k_cos_graphene(RowArgs a, CosP P, const double4* __restrict__ gnorm,
+                                                      double4* __restrict__ gS, int nb_real) {
+  ROW_PROLOGUE(T)
+  double4 g = gnorm[i];
+  const R nx = g.x, ny = g.y, nz = g.z, R2 = P.R2;
+  double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, e = 0;
+  for (int p = t; p < n; p += T) {
+    int j = nbrow[p];
+    double dx, dy, dz;
+    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    R r = PM<R>::sqrt_((R)d2);
+    if (r < R2) {
+      R nd = nx * (R)dx + ny * (R)dy + nz * (R)dz;
+      R A, Bc, Sc, ep;
+      cos_pair<R>(P, r, nd, PM<R>::rcp(r), A, Bc, Sc, ep);
+      double Ad = (double)A * P.pref, Bd = (double)Bc * P.pref, Sd = (double)Sc;
+      fx -= Ad * dx + Bd * g.x; fy -= Ad * dy + Bd * g.y; fz -= Ad * dz + Bd * g.z;
+      sx += Sd * dx; sy += Sd * dy; sz += Sd * dz;
+      if (E) e += (double)ep * P.pref;
+    }
+  }
+  fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
+  sx = group_sum<T>(sx); sy = group_sum<T>(sy); sz = group_sum<T>(sz);
+  if (E) e = group_sum<T>(e);
+  if (t == 0) {
+    write_force(a, i, fx, fy, fz);
+    gS[i] = make_double4(sx, sy, sz, 0);
+    if (E) a.e_row[row] = e;
+  }
+}
+
+// normal-derivative term (*_forces_for_graphene second loop): for each of
+// the three nearest neighbours j of i, the sum over j's metal neighbours p of
+//   c * V2 V3 fc /(den (n_j.dr_jp)) (dr_jp . W) n_j
+// equals c/den (S_j . W) n_j with S_j from the graphene pass.
+__global__ void __launch_bounds__(256) k_cos_nn(RowArgs a, CosP P, const int* __restrict__ row_of,
+                                                const int* __restrict__ rev, const double4* __restrict__ gnorm,
+                                                const double4* __restrict__ gS, int nb_real) {
+  ROW_PROLOGUE(1)
+  (void)t; (void)pi;
+  double fx = 0, fy = 0, fz = 0;
+  for (int q = 0; q < 3 && n == 3; ++q) {
+    int j = nbrow[q];
+    int l1 = rev[(size_t)row * 3 + q];
+    int jr = row_of[j];
+    if (jr < 0 || l1 < 0 || l1 > 2) continue;  // reported by k_rev / k_nn_from_tb
+    double4 pj = a.pos[j];
+    double d[3][3];
+    for (int k = 0; k < 3; ++k) min_image_dist2(pj, a.pos[a.nbr[(size_t)jr * 3 + k]], a.hb, d[k][0], d[k][1], d[k][2]);
+    int l2 = (l1 + 1) % 3, l3 = (l1 + 2) % 3;
+    double d12[3], d31[3], d23[3];
+    for (int k = 0; k < 3; ++k) {
+      d12[k] = d[l2][k] - d[l1][k];
+      d31[k] = d[l1][k] - d[l3][k];
+      d23[k] = d[l3][k] - d[l2][k];
+    }
+    double s1212 = d12[0] * d12[0] + d12[1] * d12[1] + d12[2] * d12[2];
+    double s3131 = d31[0] * d31[0] + d31[1] * d31[1] + d31[2] * d31[2];
+    double s1231 = d12[0] * d31[0] + d12[1] * d31[1] + d12[2] * d31[2];
+    double s2331 = d23[0] * d31[0] + d23[1] * d31[1] + d23[2] * d31[2];
+    double s2312 = d23[0] * d12[0] + d23[1] * d12[1] + d23[2] * d12[2];
+    double den = s1212 * s3131 - s1231 * s1231;
+    double4 S = gS[j];
+    double w = S.x * (d12[0] * s2331 - d31[0] * s2312) + S.y * (d12[1] * s2331 - d31[1] * s2312) +
+               S.z * (d12[2] * s2331 - d31[2] * s2312);
+    double c = P.nn_coef * w / den;
+    double4 g = gnorm[j];
+    fx -= c * g.x; fy -= c * g.y; fz -= c * g.z;
+  }
+  write_force(a, i, fx, fy, fz);
+}
+
+// metal side (*_forces_for_other_atoms): rows of nl(2), neighbours are
+// graphene atoms whose normal enters V3 (morsec divides by |n|)
+template <int T, class R>
+__global__ void __launch_bounds__(256) k_cos_metal(RowArgs a, CosP P, const double4* __restrict__ gnorm,
+                                                   int nb_real) {
+  ROW_PROLOGUE(T)
+  const R R2 = P.R2;
+  double fx = 0, fy = 0, fz = 0;
+  for (int p = t; p < n; p += T) {
+    int j = nbrow[p];
+    double dx, dy, dz;
+    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    R r = PM<R>::sqrt_((R)d2);
+    if (r < R2) {
+      double4 g = gnorm[j];
+      R nd = (R)g.x * (R)dx + (R)g.y * (R)dy + (R)g.z * (R)dz;
+      R nn = P.morse ? PM<R>::sqrt_((R)(g.x * g.x + g.y * g.y + g.z * g.z)) * r : r;
+      R A, Bc, Sc, ep;
+      cos_pair<R>(P, r, nd, PM<R>::rcp(nn), A, Bc, Sc, ep);
+      double Ad = (double)A * P.pref, Bd = (double)Bc * P.pref;
+      fx -= Ad * dx + Bd * g.x; fy -= Ad * dy + Bd * g.y; fz -= Ad * dz + Bd * g.z;
+    }
+  }
+  fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
+  if (t == 0) write_force(a, i, fx, fy, fz);
+}
+
+// ----------------------------------------------------------------- dispatch
+
+static RowArgs row_args(pfm_ctx* h, Inter& it, int k, int accumulate) {
+  ListDev& L = it.nl[k];
+  RowArgs a;
+  a.nrows = L.nrows;
+  a.stride = L.stride;
+  a.rows = h->grp.rows[L.g1];
+  a.nnum = L.nnum;
+  a.nbr = L.nbr;
+  a.pos = h->pos4;
+  a.hb = make_double3(h->hbox[0], h->hbox[1], h->hbox[2]);
+  a.frc = h->frc4;
+  a.accumulate = accumulate;
+  a.e_row = it.e_row;
+  return a;
+}
+
+static int row_blocks(int nrows, int T, int block) { return (int)(((long)nrows * T + block - 1) / block); }
+
+#define DISPATCH_T(T_, ...)                       \
+  switch (T_) {                                   \
+    case 1: { constexpr int T = 1; __VA_ARGS__; } break;   \
+    case 2: { constexpr int T = 2; __VA_ARGS__; } break;   \
+    case 4: { constexpr int T = 4; __VA_ARGS__; } break;   \
+    case 16: { constexpr int T = 16; __VA_ARGS__; } break; \
+    case 32: { constexpr int T = 32; __VA_ARGS__; } break; \
+    default: { constexpr int T = 8; __VA_ARGS__; } break;  \
+  }
+
+template <class R>
+static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
+  bool written = !first_writes;  // has any pass written this step's forces yet
+  const int BS = 256;
+  for (size_t ii = 0; ii < h->inter.size(); ++ii) {
+    Inter& it = h->inter[ii];
+    switch (it.kind) {
+      case PFM_RJL: {
+        RowArgs a = row_args(h, it, 0, written ? 1 : 0);
+        if (a.nrows == 0) break;
+        RjlP P{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6], 1.0 / (it.p[6] - it.p[5])};
+        int T = h->lanes;
+        DISPATCH_T(T, {
+          int nb = row_blocks(a.nrows, T, BS);
+          PFM_LAUNCH(h, "rjl_density", (k_rjl_density<T, R>), pad8(nb), BS, a, P, h->eb, h->inv_eb, nb);
+          if (energy) PFM_LAUNCH(h, "rjl_force", (k_rjl_force<T, R, true>), pad8(nb), BS, a, P, h->eb, h->inv_eb, nb);
+          else PFM_LAUNCH(h, "rjl_force", (k_rjl_force<T, R, false>), pad8(nb), BS, a, P, h->eb, h->inv_eb, nb);
+        });
+        written = true;
+        break;
+      }
+      case PFM_LJ: {
+        LjP P{4.0 * it.p[0], it.p[1], it.p[2], it.p[3], 1.0 / (it.p[3] - it.p[2])};
+        for (int k = 0; k < 2; ++k) {
+          RowArgs a = row_args(h, it, k, written ? 1 : 0);
+          if (a.nrows == 0) continue;
+          bool E = energy && k == 0;
+          int T = h->lanes;
+          DISPATCH_T(T, {
+            int nb = row_blocks(a.nrows, T, BS);
+            if (E) PFM_LAUNCH(h, "lj", (k_lj<T, R, true>), pad8(nb), BS, a, P, nb);
+            else PFM_LAUNCH(h, "lj", (k_lj<T, R, false>), pad8(nb), BS, a, P, nb);
+          });
+          written = true;
+        }
+        break;
+      }
+      case PFM_LJ1G: {
+        RowArgs a = row_args(h, it, 0, written ? 1 : 0);
+        if (a.nrows == 0) break;
+        double eps = it.p[0], sig = it.p[1], s6 = sig * sig * sig * sig * sig * sig;
+        Lj1gP P{4.0 * eps * s6, 4.0 * eps * s6 * s6, 6.0 * 4.0 * eps * s6, 12.0 * 4.0 * eps * s6 * s6,
+                it.p[2], it.p[3], 1.0 / (it.p[3] - it.p[2])};
+        const int* loc = h->grp.loc[it.nl[0].g2];
+        int T = h->lanes;
+        DISPATCH_T(T, {
+          int nb = row_blocks(a.nrows, T, BS);
+          if (energy) PFM_LAUNCH(h, "lj1g", (k_lj1g<T, R, true>), pad8(nb), BS, a, P, h->orig, loc, it.cut, nb);
+          else PFM_LAUNCH(h, "lj1g", (k_lj1g<T, R, false>), pad8(nb), BS, a, P, h->orig, loc, it.cut, nb);
+        });
+        written = true;
+        break;
+      }
+      case PFM_TB: {
+        RowArgs a = row_args(h, it, 0, written ? 1 : 0);
+        if (a.nrows == 0) break;
+        TbP P{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6] * it.p[6], it.p[7] * it.p[7],
+              it.p[8], it.p[9], 1.0 / (it.p[9] - it.p[8])};
+        int nb = row_blocks(a.nrows, 1, BS);
+        double* BT = it.Bp + (size_t)it.nl[0].nrows * it.nl[0].stride;
+        PFM_LAUNCH(h, "tb_bond", (k_tb_bond<R>), pad8(nb), BS, a, P, it.B, it.Bp, BT, nb);
+        const int* row_of = h->grp.row_of[it.nl[0].g1];
+        if (energy) PFM_LAUNCH(h, "tb_force", (k_tb_force<R, true>), pad8(nb), BS, a, P, row_of, it.nl[0].rev, it.B, it.Bp, BT, nb);
+        else PFM_LAUNCH(h, "tb_force", (k_tb_force<R, false>), pad8(nb), BS, a, P, row_of, it.nl[0].rev, it.B, it.Bp, BT, nb);
+        written = true;
+        break;
+      }
+      case PFM_LJC:
+      case PFM_MORSEC: {
+        bool morse = it.kind == PFM_MORSEC;
+        bool simplified = (int)it.p[morse ? 6 : 5];
+        CosP P;
+        if (morse) {
+          P = CosP{it.p[0], 0.0, it.p[2], it.p[1], it.p[3], it.p[4], it.p[5], 1.0 / (it.p[5] - it.p[4]),
+                   2.0 * it.p[0] * it.p[3], 1};
+        } else {
+          P = CosP{4.0 * it.p[0], it.p[1], 0.0, 0.0, it.p[2], it.p[3], it.p[4], 1.0 / (it.p[4] - it.p[3]),
+                   4.0 * it.p[0] * it.p[2], 0};
+        }
+        ListDev& NN = it.nl[2];
+        if (NN.nrows > 0)
+          PFM_LAUNCH(h, "gnorm", k_gnorm, (NN.nrows + 255) / 256, 256, NN.nrows, h->grp.rows[NN.g1], NN.nbr, h->pos4,
+                     make_double3(h->hbox[0], h->hbox[1], h->hbox[2]), it.gnorm, simplified ? 1 : 0);
+        RowArgs a = row_args(h, it, 0, written ? 1 : 0);
+        int T = h->lanes;
+        if (a.nrows > 0) {
+          DISPATCH_T(T, {
+            int nb = row_blocks(a.nrows, T, BS);
+            if (energy) PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, true>), pad8(nb), BS, a, P, it.gnorm, it.gS, nb);
+            else PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, false>), pad8(nb), BS, a, P, it.gnorm, it.gS, nb);
+          });
+          written = true;
+        }
+        if (!simplified && NN.nrows > 0) {
+          RowArgs b = row_args(h, it, 2, written ? 1 : 0);
+          int nb = row_blocks(b.nrows, 1, BS);
+          PFM_LAUNCH(h, "cos_nn", k_cos_nn, pad8(nb), BS, b, P, h->grp.row_of[NN.g1], NN.rev, it.gnorm, it.gS, nb);
+          written = true;
+        }
+        RowArgs c = row_args(h, it, 1, written ? 1 : 0);
+        if (c.nrows > 0) {
+          DISPATCH_T(T, {
+            int nb = row_blocks(c.nrows, T, BS);
+            PFM_LAUNCH(h, "cos_metal", (k_cos_metal<T, R>), pad8(nb), BS, c, P, it.gnorm, nb);
+          });
+          written = true;
+        }
+        break;
+      }
+    }
+  }
+  return PFM_OK;
+}
+
+int launch_forces(pfm_ctx* h, bool energy, bool first_writes) {
+  if (h->precision == PFM_MIXED) return forces_T<float>(h, energy, first_writes);
+  return forces_T<double>(h, energy, first_writes);
+}

@@ -1,0 +1,227 @@
+// pfm_internal.h -- shared definitions of the gfx950 HIP implementation of the
+// PFMDS step.  Device layout (all in HBM, atoms in Morton-sorted cell order):
+//   pos4[N]  double4 (x, y, z, -)          vel4[N] double4 (vx, vy, vz, mass)
+//   frc4[N]  double4 (fx, fy, fz, -)       mask[N] uint32 group bits (bit g-1)
+//   orig[N]  int  original 0-based atom index of the internal slot
+// Neighbour lists are full lists of internal atom indices, one padded row per
+// group-1 atom (rows in internal order); distances are never stored: every
+// pass recomputes dr from pos4.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pfm.h"
+
+#define PFM_MAXG 32
+#define PFM_MAXI 16
+#define PFM_MAXNHC 8
+#define PFM_TB_MAXN 16  // max Tersoff-Brenner list row held in registers
+
+// pi as written by the reference cutoff module (cut_off_function.f90:8,20)
+#define PFM_PI 3.14159265358979
+// unit constants, left to right as md_general.f90:165,304
+#define PFM_MASS_COEF (1.6605389217 / 1.6021765654 * 100.0)
+#define PFM_KB (1.3806488 / 1.6021765654 * (1.0 / 10000.0))
+
+// ----------------------------------------------------------------- device math
+
+// find_distance (md_general.f90:423-441), evaluated without contraction so the
+// neighbour test dr2 < r_cut**2 is bit-identical to the reference arithmetic.
+__device__ __forceinline__ double min_image_dist2(const double4& a, const double4& b, const double3& hb,
+                                                  double& dx, double& dy, double& dz) {
+#pragma clang fp contract(off)
+  double x = b.x - a.x, y = b.y - a.y, z = b.z - a.z;
+  x = x - hb.x * (copysign(1.0, x - hb.x) + copysign(1.0, x + hb.x));
+  y = y - hb.y * (copysign(1.0, y - hb.y) + copysign(1.0, y + hb.y));
+  z = z - hb.z * (copysign(1.0, z - hb.z) + copysign(1.0, z + hb.z));
+  dx = x; dy = y; dz = z;
+  return x * x + y * y + z * z;
+}
+
+// Precision policy of the pair arithmetic.  R = double: every operation in
+// FP64 with the library's correctly rounded elementary functions.  R = float:
+// dr is formed in FP64 and rounded once, the pair terms use the gfx950
+// transcendental unit (v_exp_f32, v_sin/cos_f32, v_sqrt_f32, v_rcp_f32).
+template <class R> struct PM;
+template <> struct PM<double> {
+  static __device__ __forceinline__ double exp_(double x) { return exp(x); }
+  static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+  static __device__ __forceinline__ double cos_(double x) { return cos(x); }
+  static __device__ __forceinline__ double sin_(double x) { return sin(x); }
+  static __device__ __forceinline__ double pow_(double x, double y) { return pow(x, y); }
+};
+template <> struct PM<float> {
+  static __device__ __forceinline__ float exp_(float x) { return __expf(x); }
+  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float cos_(float x) { return __cosf(x); }
+  static __device__ __forceinline__ float sin_(float x) { return __sinf(x); }
+  static __device__ __forceinline__ float pow_(float x, float y) {
+    return x > 0.0f ? __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)) : 0.0f;
+  }
+};
+
+// cosine cutoff (cut_off_function.f90:6-28): f, and df = f'/r
+template <class R>
+__device__ __forceinline__ void cos_cutoff(R r, R R1, R R2, R inv_w, R& f, R& df) {
+  if (r < R1) {
+    f = R(1); df = R(0);
+  } else if (r < R2) {
+    R arg = R(PFM_PI) * (r - R1) * inv_w;
+    f = (R(1) + PM<R>::cos_(arg)) * R(0.5);
+    df = -PM<R>::sin_(arg) * R(PFM_PI) * inv_w / r * R(0.5);
+  } else {
+    f = R(0); df = R(0);
+  }
+}
+
+// sum over the aligned group of T lanes that share one atom row
+template <int T, class V>
+__device__ __forceinline__ V group_sum(V v) {
+#pragma unroll
+  for (int o = T / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, T);
+  return v;
+}
+
+// XCD-aware block order: the hardware deals consecutive workgroups round-robin
+// over the 8 XCDs; remap so each XCD (own L2) works on one contiguous range of
+// Morton-ordered rows.  Returns -1 for padding blocks.
+__device__ __forceinline__ int xcd_block(int nblocks_real) {
+  int nb = gridDim.x, b = blockIdx.x;
+  int per = nb >> 3;  // grid is padded to a multiple of 8
+  int lb = (b & 7) * per + (b >> 3);
+  return lb < nblocks_real ? lb : -1;
+}
+
+// device error record: code, 1-based atom, count, step
+struct DevErr {
+  int code, atom, count, step;
+};
+__device__ __forceinline__ void raise_err(DevErr* e, int code, int atom, int count, int step) {
+  if (atomicCAS(&e->code, 0, code) == 0) {
+    e->atom = atom; e->count = count; e->step = step;
+  }
+}
+
+// ------------------------------------------------------------------ host side
+
+struct ListDev {
+  int g1 = 0, g2 = 0, nmax = 0, period = 1, stride = 0;
+  double rcut = 0;
+  int nrows = 0;
+  int* nnum = nullptr;  // [nrows]
+  int* nbr = nullptr;   // [nrows*stride] internal atom indices
+  int* rev = nullptr;   // [nrows*stride] slot of the row atom in the neighbour's row (tb, nn lists)
+  bool built = false;
+};
+
+struct Inter {
+  int kind = 0, nl_n = 0, g[6] = {0, 0, 0, 0, 0, 0};
+  double p[16] = {0};
+  ListDev nl[3];
+  int nn_from_tb = -1;  // ljc/morsec: index of the tb interaction whose list feeds nl(3)
+  double energy = 0.0;
+  double* e_row = nullptr;  // per-row energies of the last energy step
+  int e_rows = 0;
+  // lj1g half-list bookkeeping (LennardJones_1g.f90:42,73 via lessnnum)
+  int* cut = nullptr;  // [N] by internal atom: group-local index bound of the half list
+  // tb tables
+  double* B = nullptr;   // [nrows*stride] bond order B(p,i)
+  double* Bp = nullptr;  // [nrows*stride] B**(1/delt+1)
+  // ljc/morsec
+  double4* gnorm = nullptr;  // [N] by internal atom
+  double4* gS = nullptr;     // [N] sum of V2 V3 fc/(n.dr) dr per graphene atom
+};
+
+struct Nhc {
+  int group = 0, M = 1, L = 0;
+  double T = 0, q1 = 0;
+};
+
+struct KTimer {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+struct Groups {
+  int n[PFM_MAXG + 1] = {0};
+  std::vector<int> idx[PFM_MAXG + 1];  // host: 0-based original indices, group order
+  int* rows[PFM_MAXG + 1] = {nullptr};    // device: members in internal order
+  int* row_of[PFM_MAXG + 1] = {nullptr};  // device: internal atom -> row or -1
+  int* loc[PFM_MAXG + 1] = {nullptr};     // device: original atom -> group-local index or -1
+  bool used[PFM_MAXG + 1] = {false};
+};
+
+struct pfm_ctx {
+  int N = 0, precision = PFM_FP64, dev = 0;
+  double box[3], hbox[3];
+  hipStream_t stream = nullptr;
+  // atom state (internal order) and the spare buffers of the re-sort
+  double4 *pos4 = nullptr, *vel4 = nullptr, *frc4 = nullptr;
+  double4 *pos4b = nullptr, *vel4b = nullptr, *frc4b = nullptr;
+  uint32_t *mask = nullptr, *maskb = nullptr;
+  int *orig = nullptr, *origb = nullptr;
+  int* perm = nullptr;
+  uint32_t host_mask_of_group[PFM_MAXG + 1];
+  std::vector<uint32_t> hmask;
+  std::vector<double> hmass;
+  Groups grp;
+  int roles[4] = {0, 0, 0, 0};  // all_moving, xyz, z, all_atoms
+  int zm_period = 1, invert_z = 0;
+  bool all_atoms_is_everyone = true;
+  std::vector<Inter> inter;
+  std::vector<Nhc> nhc;
+  // NHC chain state on device: x[16], v[16], q[16], s, e, ke per chain
+  double* nhc_state = nullptr;
+  // cells
+  int nc[3] = {1, 1, 1}, ncell = 1;
+  double cell_len[3];
+  int *cell_rank = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr, *atom_rank = nullptr;
+  int* scan_tmp = nullptr;
+  // RGL per-atom densities
+  double *eb = nullptr, *inv_eb = nullptr;
+  // reductions
+  double* red_part = nullptr;  // partial sums
+  double* red_out = nullptr;   // device scalars
+  double* h_red = nullptr;     // pinned host mirror
+  int red_blocks = 0;
+  DevErr* derr = nullptr;
+  DevErr* herr = nullptr;  // pinned
+  // energies
+  double pe = 0, pe_prev = 0, ke = 0, temp = 0, sim_time = 0;
+  std::vector<double> e_nhc;
+  bool sorted_once = false;
+  // tuning
+  int lanes = 8, block = 256;
+  // timing
+  bool timing = false;
+  std::vector<KTimer> timers;
+  std::vector<hipEvent_t> evpool;
+  size_t evnext = 0;
+  std::string err;
+};
+
+// launch bookkeeping (pfm_context.hip)
+void tim_begin(pfm_ctx* h, const char* name, hipEvent_t* a, hipEvent_t* b);
+void tim_end(pfm_ctx* h, hipEvent_t a, hipEvent_t b, const char* name);
+
+#define PFM_LAUNCH(h, name, kern, grid, blk, ...)                                   \
+  do {                                                                              \
+    hipEvent_t _a = nullptr, _b = nullptr;                                          \
+    if ((h)->timing) tim_begin((h), name, &_a, &_b);                                \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(blk), 0, (h)->stream, __VA_ARGS__);   \
+    if ((h)->timing) tim_end((h), _a, _b, name);                                    \
+  } while (0)
+
+static inline int pad8(int nb) { return ((nb + 7) / 8) * 8; }
+
+// kernels' host launchers
+int launch_sort_and_bin(pfm_ctx* h, int step);
+int launch_build_list(pfm_ctx* h, int ii, int k, int step);
+int launch_nn_from_tb(pfm_ctx* h, int ii, int step);
+int launch_forces(pfm_ctx* h, bool energy, bool first_writes);
+void launch_group_rows(pfm_ctx* h);

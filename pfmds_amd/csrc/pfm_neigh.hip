@@ -1,0 +1,383 @@
+// pfm_neigh.hip -- cell binning, Morton re-sort of the atoms and neighbour-list
+// construction on gfx950.
+//
+// Replaces find_neighbours (md_neighbours.f90:56-100, brute force over all
+// N1*N2 pairs), converce_neighbour_list (md_neighbours.f90:128-160, a serial
+// transpose) and find_gr_nearest_neighbors (graphenenorm.f90:8-36).  The
+// membership test is the reference's own: j != i, j in group 2, and
+// dr2 < r_cut**2 with find_distance's arithmetic (min_image_dist2), so the
+// lists are identical to the reference's as sets.  Lists are rebuilt on the
+// reference's schedule (mod(md_step, update_period) == 0); between rebuilds
+// nothing is refreshed because the force passes recompute dr from positions.
+#include "pfm_internal.h"
+
+// ------------------------------------------------------------------ binning
+
+struct CellGrid {
+  int nc[3];
+  double inv_len[3];
+};
+
+__device__ __forceinline__ int cell_coord(double x, double inv_len, int nc) {
+  int c = (int)floor(x * inv_len);
+  return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__global__ void k_cell_keys(int N, const double4* __restrict__ pos, CellGrid g, const int* __restrict__ cell_rank,
+                            int* __restrict__ atom_rank, int* __restrict__ cell_count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double4 p = pos[i];
+  int cx = cell_coord(p.x, g.inv_len[0], g.nc[0]);
+  int cy = cell_coord(p.y, g.inv_len[1], g.nc[1]);
+  int cz = cell_coord(p.z, g.inv_len[2], g.nc[2]);
+  int r = cell_rank[(cz * g.nc[1] + cy) * g.nc[0] + cx];
+  atom_rank[i] = r;
+  atomicAdd(&cell_count[r], 1);
+}
+
+// exclusive scan of n ints by one workgroup of 1024 threads
+__global__ void __launch_bounds__(1024) k_scan_excl(int n, const int* __restrict__ in, int* __restrict__ out,
+                                                    int* __restrict__ total) {
+  __shared__ int s[1024];
+  int t = threadIdx.x;
+  int chunk = (n + 1023) / 1024;
+  int b = t * chunk, e = min(n, b + chunk);
+  int sum = 0;
+  for (int k = b; k < e; ++k) sum += in[k];
+  s[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = t >= o ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  int run = s[t] - sum;
+  for (int k = b; k < e; ++k) {
+    int v = in[k];
+    out[k] = run;
+    run += v;
+  }
+  if (t == 1023 && total) *total = s[1023];
+}
+
+__global__ void k_scatter(int N, const int* __restrict__ atom_rank, const int* __restrict__ cell_start,
+                          int* __restrict__ cell_fill, int* __restrict__ perm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int r = atom_rank[i];
+  perm[cell_start[r] + atomicAdd(&cell_fill[r], 1)] = i;
+}
+
+// order the atoms of each cell by original index, so the layout (and every
+// sum over it) does not depend on the order of the scatter's atomics
+__global__ void k_cell_sort(int ncell, const int* __restrict__ cell_start, const int* __restrict__ cell_count,
+                            int* __restrict__ perm, const int* __restrict__ orig) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int s = cell_start[c], n = cell_count[c];
+  for (int a = 1; a < n; ++a) {
+    int v = perm[s + a], key = orig[v], b = a - 1;
+    while (b >= 0 && orig[perm[s + b]] > key) {
+      perm[s + b + 1] = perm[s + b];
+      --b;
+    }
+    perm[s + b + 1] = v;
+  }
+}
+
+__global__ void k_permute(int N, const int* __restrict__ perm, const double4* __restrict__ p0,
+                          const double4* __restrict__ v0, const double4* __restrict__ f0,
+                          const uint32_t* __restrict__ m0, const int* __restrict__ o0, double4* __restrict__ p1,
+                          double4* __restrict__ v1, double4* __restrict__ f1, uint32_t* __restrict__ m1,
+                          int* __restrict__ o1) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  int i = perm[k];
+  p1[k] = p0[i];
+  v1[k] = v0[i];
+  f1[k] = f0[i];
+  m1[k] = m0[i];
+  o1[k] = o0[i];
+}
+
+static CellGrid make_grid(pfm_ctx* h) {
+  CellGrid g;
+  for (int d = 0; d < 3; ++d) {
+    g.nc[d] = h->nc[d];
+    g.inv_len[d] = 1.0 / h->cell_len[d];
+  }
+  return g;
+}
+
+// Bin atoms into cells and re-order every per-atom array in Morton cell order.
+int launch_sort_and_bin(pfm_ctx* h, int step) {
+  (void)step;
+  int N = h->N, nb = (N + 255) / 256, ncb = (h->ncell + 255) / 256;
+  CellGrid g = make_grid(h);
+  (void)hipMemsetAsync(h->cell_count, 0, sizeof(int) * h->ncell, h->stream);
+  (void)hipMemsetAsync(h->cell_fill, 0, sizeof(int) * h->ncell, h->stream);
+  PFM_LAUNCH(h, "cell_keys", k_cell_keys, nb, 256, N, h->pos4, g, h->cell_rank, h->atom_rank, h->cell_count);
+  PFM_LAUNCH(h, "scan", k_scan_excl, 1, 1024, h->ncell, h->cell_count, h->cell_start, (int*)nullptr);
+  PFM_LAUNCH(h, "scatter", k_scatter, nb, 256, N, h->atom_rank, h->cell_start, h->cell_fill, h->perm);
+  PFM_LAUNCH(h, "cell_sort", k_cell_sort, ncb, 256, h->ncell, h->cell_start, h->cell_count, h->perm, h->orig);
+  PFM_LAUNCH(h, "permute", k_permute, nb, 256, N, h->perm, h->pos4, h->vel4, h->frc4, h->mask, h->orig, h->pos4b,
+             h->vel4b, h->frc4b, h->maskb, h->origb);
+  std::swap(h->pos4, h->pos4b);
+  std::swap(h->vel4, h->vel4b);
+  std::swap(h->frc4, h->frc4b);
+  std::swap(h->mask, h->maskb);
+  std::swap(h->orig, h->origb);
+  launch_group_rows(h);
+  return PFM_OK;
+}
+
+// --------------------------------------------------------------- group rows
+
+__global__ void k_flag(int N, const uint32_t* __restrict__ mask, uint32_t bit, int* __restrict__ flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) flag[i] = (mask[i] & bit) ? 1 : 0;
+}
+__global__ void k_compact(int N, const uint32_t* __restrict__ mask, uint32_t bit, const int* __restrict__ pre,
+                          int* __restrict__ rows, int* __restrict__ row_of) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (mask[i] & bit) {
+    rows[pre[i]] = i;
+    row_of[i] = pre[i];
+  } else {
+    row_of[i] = -1;
+  }
+}
+
+// rows of every group that some list or role uses: members in internal order
+void launch_group_rows(pfm_ctx* h) {
+  int N = h->N, nb = (N + 255) / 256;
+  for (int g = 1; g <= PFM_MAXG; ++g) {
+    if (!h->grp.used[g] || h->grp.n[g] == 0) continue;
+    uint32_t bit = 1u << (g - 1);
+    PFM_LAUNCH(h, "group_rows", k_flag, nb, 256, N, h->mask, bit, h->scan_tmp);
+    PFM_LAUNCH(h, "group_rows", k_scan_excl, 1, 1024, N, h->scan_tmp, h->scan_tmp + N, (int*)nullptr);
+    PFM_LAUNCH(h, "group_rows", k_compact, nb, 256, N, h->mask, bit, h->scan_tmp + N, h->grp.rows[g],
+               h->grp.row_of[g]);
+  }
+}
+
+// ------------------------------------------------------------ list building
+
+struct SearchArgs {
+  int nrows;
+  const int* rows;
+  uint32_t g2bit;
+  const double4* pos;
+  const uint32_t* mask;
+  const int* orig;
+  double3 hb;
+  double rc2;
+  CellGrid g;
+  const int* cell_rank;
+  const int* cell_start;
+  const int* cell_count;
+  int nmax, stride;
+  int* nnum;
+  int* nbr;
+  DevErr* err;
+  int step, err_code;
+  int exact;  // require exactly nmax entries (graphene nearest neighbours)
+};
+
+// T lanes per row atom scan the 27 (or fewer, for boxes under three cells
+// wide) surrounding cells; wave ballots compact the accepted candidates.
+template <int T>
+__global__ void __launch_bounds__(256) k_build(SearchArgs a, int nblocks_real) {
+  int lb = xcd_block(nblocks_real);
+  if (lb < 0) return;
+  int gt = lb * blockDim.x + threadIdx.x;
+  int row = gt / T, t = threadIdx.x & (T - 1);
+  if (row >= a.nrows) return;
+  int lane = threadIdx.x & 63;
+  int gshift = lane & ~(T - 1);
+  unsigned long long gmask = (T == 64) ? ~0ull : (((1ull << T) - 1ull) << gshift);
+  unsigned long long lower = (1ull << lane) - 1ull;
+  int i = a.rows[row];
+  double4 pi = a.pos[i];
+  int c[3];
+  c[0] = cell_coord(pi.x, a.g.inv_len[0], a.g.nc[0]);
+  c[1] = cell_coord(pi.y, a.g.inv_len[1], a.g.nc[1]);
+  c[2] = cell_coord(pi.z, a.g.inv_len[2], a.g.nc[2]);
+  int lo[3], cnt[3];
+  for (int d = 0; d < 3; ++d) {
+    if (a.g.nc[d] >= 3) { lo[d] = c[d] - 1; cnt[d] = 3; }
+    else { lo[d] = 0; cnt[d] = a.g.nc[d]; }
+  }
+  int count = 0;
+  int* out = a.nbr + (size_t)row * a.stride;
+  for (int oz = 0; oz < cnt[2]; ++oz) {
+    int z = lo[2] + oz; z = z < 0 ? z + a.g.nc[2] : (z >= a.g.nc[2] ? z - a.g.nc[2] : z);
+    for (int oy = 0; oy < cnt[1]; ++oy) {
+      int y = lo[1] + oy; y = y < 0 ? y + a.g.nc[1] : (y >= a.g.nc[1] ? y - a.g.nc[1] : y);
+      for (int ox = 0; ox < cnt[0]; ++ox) {
+        int x = lo[0] + ox; x = x < 0 ? x + a.g.nc[0] : (x >= a.g.nc[0] ? x - a.g.nc[0] : x);
+        int r = a.cell_rank[(z * a.g.nc[1] + y) * a.g.nc[0] + x];
+        int s = a.cell_start[r], n = a.cell_count[r];
+        for (int base = 0; base < n; base += T) {
+          int k = base + t;
+          bool pred = false;
+          int j = 0;
+          if (k < n) {
+            j = s + k;
+            if (j != i && (a.mask[j] & a.g2bit)) {
+              double dx, dy, dz;
+              pred = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz) < a.rc2;
+            }
+          }
+          unsigned long long bits = __ballot(pred) & gmask;
+          if (pred) {
+            int slot = count + __popcll(bits & lower);
+            if (slot < a.nmax) out[slot] = j;
+          }
+          count += __popcll(bits);
+        }
+      }
+    }
+  }
+  if (t == 0) {
+    a.nnum[row] = count < a.nmax ? count : a.nmax;
+    if (count > a.nmax) raise_err(a.err, a.err_code, a.orig[i] + 1, count, a.step);
+    else if (a.exact && count != a.nmax) raise_err(a.err, PFM_E_GRAPHENE_NN, a.orig[i] + 1, count, a.step);
+  }
+}
+
+// slot of the row atom inside each neighbour's own row (Tersoff-Brenner's
+// `do l=1,nl%nnum(j); if(nl%nlist(l,j)==i) exit` and graphene's l1 search)
+__global__ void k_rev(int nrows, const int* __restrict__ rows, const int* __restrict__ row_of,
+                      const int* __restrict__ nnum, const int* __restrict__ nbr, int stride, int* __restrict__ rev,
+                      DevErr* err, const int* __restrict__ orig, int step) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int i = rows[r], n = nnum[r];
+  for (int p = 0; p < n; ++p) {
+    int j = nbr[(size_t)r * stride + p];
+    int jr = row_of[j];
+    int l = -1;
+    if (jr >= 0) {
+      int nj = nnum[jr];
+      for (int q = 0; q < nj; ++q)
+        if (nbr[(size_t)jr * stride + q] == i) { l = q; break; }
+    }
+    rev[(size_t)r * stride + p] = l;
+    if (l < 0) raise_err(err, PFM_E_GRAPHENE_NN, orig[i] + 1, -1, step);
+  }
+}
+
+// lj1g: group-local index of the first listed neighbour whose atom index is
+// larger than the row atom's (the reference's lessnnum boundary, in list
+// order = ascending group order)
+__global__ void k_lj1g_cut(int nrows, const int* __restrict__ rows, const int* __restrict__ nnum,
+                           const int* __restrict__ nbr, int stride, const int* __restrict__ orig,
+                           const int* __restrict__ loc, int* __restrict__ cut) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int i = rows[r], oi = orig[i], n = nnum[r], c = 0x7fffffff;
+  for (int p = 0; p < n; ++p) {
+    int oj = orig[nbr[(size_t)r * stride + p]];
+    if (oj > oi) { int l = loc[oj]; c = l < c ? l : c; }
+  }
+  cut[i] = c;
+}
+
+// graphene nearest neighbours from the tb list: entries with |dr| < r_cut_nn,
+// exactly three per atom (find_gr_nearest_neighbors, graphenenorm.f90:8-36)
+__global__ void k_nn_from_tb(int nrows, const int* __restrict__ rows, const int* __restrict__ tnnum,
+                             const int* __restrict__ tnbr, int tstride, const double4* __restrict__ pos, double3 hb,
+                             double rnn, int* __restrict__ nnum, int* __restrict__ nbr, DevErr* err,
+                             const int* __restrict__ orig, int step) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int i = rows[r], n = tnnum[r], k = 0;
+  double4 pi = pos[i];
+  for (int p = 0; p < n; ++p) {
+    int j = tnbr[(size_t)r * tstride + p];
+    double dx, dy, dz;
+    double d = sqrt(min_image_dist2(pi, pos[j], hb, dx, dy, dz));
+    if (d < rnn) {
+      if (k < 3) nbr[(size_t)r * 3 + k] = j;
+      ++k;
+    }
+  }
+  nnum[r] = k < 3 ? k : 3;
+  if (k != 3) raise_err(err, PFM_E_GRAPHENE_NN, orig[i] + 1, k, step);
+}
+
+template <int T>
+static void launch_build_T(pfm_ctx* h, SearchArgs& a) {
+  long threads = (long)a.nrows * T;
+  int nb = (int)((threads + 255) / 256);
+  PFM_LAUNCH(h, "build_list", k_build<T>, pad8(nb), 256, a, nb);
+}
+
+// Build list k (0-based) of interaction ii.  For lj/ljc/morsec, list 1 is the
+// converse of list 0: same r_cut and schedule (md_interactions.f90:152-155),
+// found by searching from the group-2 side, which gives the same set because
+// find_distance's dr2 is symmetric in its two atoms.
+int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
+  Inter& it = h->inter[ii];
+  ListDev& L = it.nl[k];
+  if (L.nrows == 0) { L.built = true; return PFM_OK; }
+  SearchArgs a;
+  a.nrows = L.nrows;
+  a.rows = h->grp.rows[L.g1];
+  a.g2bit = 1u << (L.g2 - 1);
+  a.pos = h->pos4;
+  a.mask = h->mask;
+  a.orig = h->orig;
+  a.hb = make_double3(h->hbox[0], h->hbox[1], h->hbox[2]);
+  double rc = (k == 1 && (it.kind == PFM_LJ || it.kind == PFM_LJC || it.kind == PFM_MORSEC)) ? it.nl[0].rcut : L.rcut;
+  a.rc2 = rc * rc;
+  a.g = make_grid(h);
+  a.cell_rank = h->cell_rank;
+  a.cell_start = h->cell_start;
+  a.cell_count = h->cell_count;
+  a.nmax = L.nmax;
+  a.stride = L.stride;
+  a.nnum = L.nnum;
+  a.nbr = L.nbr;
+  a.err = h->derr;
+  a.step = step;
+  a.err_code = PFM_E_OVERFLOW;
+  a.exact = (k == 2) ? 1 : 0;
+  int T = 8;
+  if (L.nmax <= 8) T = 4;
+  if (T == 4) launch_build_T<4>(h, a);
+  else launch_build_T<8>(h, a);
+  int nb = (L.nrows + 255) / 256;
+  if (it.kind == PFM_TB || (k == 2 && !(int)it.p[it.kind == PFM_LJC ? 5 : 6])) {
+    PFM_LAUNCH(h, "list_rev", k_rev, nb, 256, L.nrows, h->grp.rows[L.g1], h->grp.row_of[L.g1], L.nnum, L.nbr,
+               L.stride, L.rev, h->derr, h->orig, step);
+  }
+  if (it.kind == PFM_LJ1G) {
+    PFM_LAUNCH(h, "list_cut", k_lj1g_cut, nb, 256, L.nrows, h->grp.rows[L.g1], L.nnum, L.nbr, L.stride, h->orig,
+               h->grp.loc[L.g2], it.cut);
+  }
+  L.built = true;
+  return PFM_OK;
+}
+
+// nl(3) of ljc/morsec copied from the tb interaction's freshly rebuilt list
+int launch_nn_from_tb(pfm_ctx* h, int ii, int step) {
+  Inter& it = h->inter[ii];
+  Inter& tb = h->inter[it.nn_from_tb];
+  ListDev& L = it.nl[2];
+  ListDev& T = tb.nl[0];
+  if (L.nrows == 0) return PFM_OK;
+  int nb = (L.nrows + 255) / 256;
+  PFM_LAUNCH(h, "nn_from_tb", k_nn_from_tb, nb, 256, L.nrows, h->grp.rows[L.g1], T.nnum, T.nbr, T.stride, h->pos4,
+             make_double3(h->hbox[0], h->hbox[1], h->hbox[2]), L.rcut, L.nnum, L.nbr, h->derr, h->orig, step);
+  bool simplified = (int)it.p[it.kind == PFM_LJC ? 5 : 6];
+  if (!simplified)
+    PFM_LAUNCH(h, "list_rev", k_rev, nb, 256, L.nrows, h->grp.rows[L.g1], h->grp.row_of[L.g1], L.nnum, L.nbr,
+               L.stride, L.rev, h->derr, h->orig, step);
+  L.built = true;
+  return PFM_OK;
+}
