@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the PFMDS MD step on one MI355X.
+
+Workload: bulk fcc Cu, 32x32x32 conventional cells (131,072 atoms, a=3.615 A),
+Rosato-Guillope-Legrand (rjl) only, cutoff 6.0 A, list r_cut 6.5 A rebuilt
+every 10 steps, seeded Gaussian displacements (0.02 A) and Maxwell velocities
+(300 K), nve at dt = 1 fs.  One "step" is one full md() step (integrator
+halves, list rebuild on its schedule, density + force passes).
+
+`value` is the mixed-precision path (FP32 pair arithmetic, FP64 dr,
+accumulation and integration) in atom-steps/s; the plain FP64 path, the
+force RMSE and energy error of mixed against FP64 at identical positions, and
+the NVE drift are reported beside it.  `roofline` is the dominant kernel's
+algorithmic HBM bytes per launch over its mean HIP-event duration;
+`cpu_baseline` times the CPU restatement of the reference algorithm (brute
+force O(N^2) list build, stored distances) on one rebuild period of the same
+system.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from pfmds_amd import capi, configs, system  # noqa: E402
+
+METRIC = "atom-steps/s on 1 MI355X: mixed/batched/surrogate vs plain FP64 + force RMSE"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def gpu_ctx(s, precision):
+    return system.build_context(s, lambda b, p, v, m: capi.Context.gpu(b, p, v, m, precision))
+
+
+def timed_run(ctx, s, steps, warmup):
+    dt = s.integrators[0].dt
+    ctx.run("nve", dt, 0, warmup, True)
+    e0 = ctx.energies()
+    t0 = time.perf_counter()
+    ctx.run("nve", dt, warmup + 1, warmup + steps, True)
+    t1 = time.perf_counter()
+    return t1 - t0, e0, ctx.energies()
+
+
+def mean_list_len(ctx):
+    nnum, _, _ = ctx.get_list(1, 1, stride=128)
+    return float(np.mean(nnum))
+
+
+def rjl_counts(n_entries):
+    """Algorithmic bytes and flops per atom of the two RGL passes.
+
+    bytes (unique HBM traffic, neighbour gathers assumed cache-resident):
+      density: pos 24 + nnum 4 + indices 4*n + eb,inv_eb write 16
+      force:   pos 24 + nnum 4 + indices 4*n + inv_eb,eb read 16 + force write 24
+    flops per list entry (each + - * / sqrt exp sin cos sign counts 1):
+      distance with minimum image 30; density term 13; force term 40
+    """
+    b_density = 24 + 4 + 4 * n_entries + 16
+    b_force = 24 + 4 + 4 * n_entries + 16 + 24
+    f_density = (30 + 13) * n_entries
+    f_force = (30 + 40) * n_entries
+    return {"rjl_density": (b_density, f_density), "rjl_force": (b_force, f_force)}
+
+
+def cpu_baseline(s, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from tests.harness import oracle_ctx, oracle_lib
+
+    lib = oracle_lib()
+    lib.orc_set_threads.argtypes = [ctypes.c_int]
+    lib.orc_set_threads(threads)
+    o = oracle_ctx(s)
+    period = s.interactions[0].lists[0][4]
+    t0 = time.perf_counter()
+    o.run("nve", s.integrators[0].dt, 0, period - 1, True)
+    t = time.perf_counter() - t0
+    return {"value": s.n * period / t, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "sample": f"steps 0-{period - 1} (one list-rebuild period) of the same {s.n}-atom system, "
+                      f"{t:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cells", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--drift-ps", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the step path has no multi-GPU decomposition "
+              "(replicas or spatial domains would shard it; see the design notes). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    s = configs.fcc_cu_rgl(nx=args.cells, steps=args.steps)
+    res = {}
+    ctxs = {}
+    for name, prec in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
+        ctx = gpu_ctx(s, prec)
+        t, e0, e1 = timed_run(ctx, s, args.steps, args.warmup)
+        res[name] = dict(seconds=t, value=s.n * args.steps / t, e0=e0, e1=e1)
+        ctxs[name] = ctx
+        print(f"# {name}: {t * 1e3 / args.steps:.4f} ms/step, {s.n * args.steps / t:.4e} atom-steps/s", file=sys.stderr,
+              flush=True)
+
+    # accuracy of the mixed path at identical positions
+    pos, vel, _ = ctxs["mixed"].state()
+    for name in ("fp64", "mixed"):
+        ctxs[name].set_state(pos, vel)
+        ctxs[name].run("nve", 1.0, 0, 0, True)
+    f64 = ctxs["fp64"].state()[2]
+    f32 = ctxs["mixed"].state()[2]
+    rmse = float(np.sqrt(np.mean((f32 - f64) ** 2)))
+    pe64, pe32 = ctxs["fp64"].energies()["pe"], ctxs["mixed"].energies()["pe"]
+    nbar = mean_list_len(ctxs["mixed"])
+
+    # kernel durations with HIP events on the library's stream
+    kt = ctxs["mixed"]
+    kt.set_timing(True)
+    kt.run("nve", 1.0, 1, 100, False)
+    counts = rjl_counts(nbar)
+    kern = {}
+    for k in ("rjl_force", "rjl_density", "kick_drift", "kick2", "build_list"):
+        c, ms = kt.kernel_time(k)
+        kern[k] = (c, ms)
+    kt.set_timing(False)
+    dom = max(("rjl_force", "rjl_density"), key=lambda k: kern[k][0] * kern[k][1])
+    cnt, ms = kern[dom]
+    bytes_launch = counts[dom][0] * s.n
+    flops_launch = counts[dom][1] * s.n
+    achieved = bytes_launch / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "mean_us": ms * 1e3,
+                "launches": cnt, "bytes_per_launch": bytes_launch,
+                "valu": {"achieved_tflops": flops_launch / (ms * 1e-3) / 1e12, "peak_tflops": FP32_PEAK_TFLOPS,
+                         "frac": flops_launch / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS}}
+
+    # NVE drift of the mixed path: fitted slope of (E - E0)/N over drift_ps
+    drift = None
+    if args.drift_ps > 0:
+        d = gpu_ctx(s, capi.MIXED)
+        d.run("nve", 1.0, 0, 0, True)
+        nseg = 20
+        seg = max(1, int(args.drift_ps * 1000 / nseg))
+        ts, es = [0.0], [d.energies()["pe"] + d.energies()["ke"]]
+        for k in range(nseg):
+            d.run("nve", 1.0, 1 + k * seg, (k + 1) * seg, True)
+            e = d.energies()
+            ts.append(e["time"])
+            es.append(e["pe"] + e["ke"])
+        slope = np.polyfit(np.array(ts) * 1e-6, (np.array(es) - es[0]) / s.n, 1)[0]
+        drift = float(slope)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(s, threads)
+
+    mixed, fp64 = res["mixed"], res["fp64"]
+    line = {
+        "metric": METRIC,
+        "value": mixed["value"],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["seconds"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair arithmetic, f64 dr/accumulation/integration",
+        "data": "synthetic",
+        "config": {"workload": f"fcc_cu_rgl {args.cells}^3 cells ({s.n} atoms), rjl R1=5.5 R2=6.0 A, "
+                               "r_cut 6.5 A / 10 steps, nve dt=1 fs",
+                   "atoms": s.n, "mean_list_entries": nbar},
+        "fp64_value": fp64["value"],
+        "fp64_ms_per_step": fp64["seconds"] * 1e3 / args.steps,
+        "speedup_vs_fp64": mixed["value"] / fp64["value"],
+        "force_rmse_eV_per_A": rmse,
+        "pe_err_eV_per_atom": abs(pe32 - pe64) / s.n,
+        "nve_drift_eV_per_atom_per_ns": drift,
+        "kernels_us": {k: v[1] * 1e3 for k, v in kern.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -347,13 +347,16 @@ static int finalize(pfm_ctx* h) {
     }
   }
   if (rmax <= 0.0) rmax = 1.0;
+  for (int d = 0; d < 3; ++d) h->nc[d] = std::max(1, std::min((int)std::floor(h->box[d] / rmax), 1024));
+  // at most 2^21 cells: larger cells only add candidates
+  while ((long)h->nc[0] * h->nc[1] * h->nc[2] > (1L << 21)) {
+    int d = (h->nc[0] >= h->nc[1] && h->nc[0] >= h->nc[2]) ? 0 : (h->nc[1] >= h->nc[2] ? 1 : 2);
+    h->nc[d] = std::max(1, h->nc[d] / 2);
+  }
   h->ncell = 1;
   for (int d = 0; d < 3; ++d) {
-    int n = (int)std::floor(h->box[d] / rmax);
-    n = std::max(1, std::min(n, 1024));
-    h->nc[d] = n;
-    h->cell_len[d] = h->box[d] / n;
-    h->ncell *= n;
+    h->cell_len[d] = h->box[d] / h->nc[d];
+    h->ncell *= h->nc[d];
   }
   // Morton rank of every cell
   std::vector<std::pair<uint64_t, int>> key(h->ncell);
@@ -609,7 +612,7 @@ int pfm_create(pfm_ctx** out, int n, const double box[3], const double* pos, con
       (rc = dalloc(h, &h->pos4b, n)) || (rc = dalloc(h, &h->vel4b, n)) || (rc = dalloc(h, &h->frc4b, n)) ||
       (rc = dalloc(h, &h->mask, n)) || (rc = dalloc(h, &h->maskb, n)) || (rc = dalloc(h, &h->orig, n)) ||
       (rc = dalloc(h, &h->origb, n)) || (rc = dalloc(h, &h->perm, n)) || (rc = dalloc(h, &h->atom_rank, n)) ||
-      (rc = dalloc(h, &h->scan_tmp, 2 * (size_t)n)) || (rc = dalloc(h, &h->eb, n)) || (rc = dalloc(h, &h->inv_eb, n)))
+      (rc = dalloc(h, &h->scan_tmp, 3 * (size_t)n + 4096)) || (rc = dalloc(h, &h->eb, n)) || (rc = dalloc(h, &h->inv_eb, n)))
     return rc;
   h->red_blocks = std::max(1, std::min(1024, (n + RED_THREADS - 1) / RED_THREADS));
   if ((rc = dalloc(h, &h->red_part, (size_t)(PFM_MAXI + 1 + RED_MAXQ) * h->red_blocks))) return rc;

@@ -23,7 +23,7 @@ struct RowArgs {
   const int* nnum;
   const int* nbr;
   const double4* pos;
-  double3 hb;
+  double3 hb, L, iL;
   double4* frc;
   int accumulate;  // 0: this pass writes the first contribution of the step
   double* e_row;
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) k_rjl_density(RowArgs a, RjlP P, double* 
   for (int p = t; p < n; p += T) {
     int j = nbrow[p];
     double dx, dy, dz;
-    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
     R r = PM<R>::sqrt_((R)d2);
     if (r <= R2) {
       R fc, dfc;
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(256) k_rjl_force(RowArgs a, RjlP P, const doub
   for (int p = t; p < n; p += T) {
     int j = nbrow[p];
     double dx, dy, dz;
-    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
     R r = PM<R>::sqrt_((R)d2);
     if (r <= R2) {
       R fc, dfc;
@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(256) k_lj(RowArgs a, LjP P, int nb_real) {
   for (int p = t; p < n; p += T) {
     int j = nbrow[p];
     double dx, dy, dz;
-    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
     R r = PM<R>::sqrt_((R)d2);
     if (r < R2) {
       R fc, dfc;
@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(256) k_lj1g(RowArgs a, Lj1gP P, const int* __r
     int w = wh + (li < cut[j] ? 1 : 0);
     if (w == 0) continue;
     double dx, dy, dz;
-    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
     R r = PM<R>::sqrt_((R)d2);
     // f_dfr_cut (cut_off_poly.f90:30-43), dfr without the 1/(R2-R1) factor
     R tr = r < R1 ? R1 : (r > R2 ? R2 : r);
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(256) k_tb_bond(RowArgs a, TbP P, double* __res
   R dxs[PFM_TB_MAXN], dys[PFM_TB_MAXN], dzs[PFM_TB_MAXN], rs[PFM_TB_MAXN], fcs[PFM_TB_MAXN];
   for (int p = 0; p < n; ++p) {
     double dx, dy, dz;
-    double d2 = min_image_dist2(pi, a.pos[nbrow[p]], a.hb, dx, dy, dz);
+    double d2 = fast_dist2(pi, a.pos[nbrow[p]], a.L, a.iL, dx, dy, dz);
     dxs[p] = (R)dx; dys[p] = (R)dy; dzs[p] = (R)dz;
     rs[p] = PM<R>::sqrt_((R)d2);
     R df;
@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* _
   R dxs[PFM_TB_MAXN], dys[PFM_TB_MAXN], dzs[PFM_TB_MAXN], rs[PFM_TB_MAXN], fcs[PFM_TB_MAXN], dfcs[PFM_TB_MAXN];
   for (int p = 0; p < n; ++p) {
     double dx, dy, dz;
-    double d2 = min_image_dist2(pi, a.pos[nbrow[p]], a.hb, dx, dy, dz);
+    double d2 = fast_dist2(pi, a.pos[nbrow[p]], a.L, a.iL, dx, dy, dz);
     dxs[p] = (R)dx; dys[p] = (R)dy; dzs[p] = (R)dz;
     rs[p] = PM<R>::sqrt_((R)d2);
     cos_cutoff<R>(rs[p], R1, R2, invw, fcs[p], dfcs[p]);
@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* _
       if (q == l) continue;
       int k = a.nbr[jbase + q];
       double dx, dy, dz;
-      double d2 = min_image_dist2(pj, a.pos[k], a.hb, dx, dy, dz);
+      double d2 = fast_dist2(pj, a.pos[k], a.L, a.iL, dx, dy, dz);
       const R xq = (R)dx, yq = (R)dy, zq = (R)dz, rq = PM<R>::sqrt_((R)d2);
       R fq, dfq;
       cos_cutoff<R>(rq, R1, R2, invw, fq, dfq);
@@ -401,7 +401,7 @@ __global__ void __launch_bounds__(256) k_cos_graphene(RowArgs a, CosP P, const d
   for (int p = t; p < n; p += T) {
     int j = nbrow[p];
     double dx, dy, dz;
-    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
     R r = PM<R>::sqrt_((R)d2);
     if (r < R2) {
       R nd = nx * (R)dx + ny * (R)dy + nz * (R)dz;
@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(256) k_cos_nn(RowArgs a, CosP P, const int* __
     if (jr < 0 || l1 < 0 || l1 > 2) continue;  // reported by k_rev / k_nn_from_tb
     double4 pj = a.pos[j];
     double d[3][3];
-    for (int k = 0; k < 3; ++k) min_image_dist2(pj, a.pos[a.nbr[(size_t)jr * 3 + k]], a.hb, d[k][0], d[k][1], d[k][2]);
+    for (int k = 0; k < 3; ++k) fast_dist2(pj, a.pos[a.nbr[(size_t)jr * 3 + k]], a.L, a.iL, d[k][0], d[k][1], d[k][2]);
     int l2 = (l1 + 1) % 3, l3 = (l1 + 2) % 3;
     double d12[3], d31[3], d23[3];
     for (int k = 0; k < 3; ++k) {
@@ -475,7 +475,7 @@ __global__ void __launch_bounds__(256) k_cos_metal(RowArgs a, CosP P, const doub
   for (int p = t; p < n; p += T) {
     int j = nbrow[p];
     double dx, dy, dz;
-    double d2 = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz);
+    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
     R r = PM<R>::sqrt_((R)d2);
     if (r < R2) {
       double4 g = gnorm[j];
@@ -503,6 +503,8 @@ static RowArgs row_args(pfm_ctx* h, Inter& it, int k, int accumulate) {
   a.nbr = L.nbr;
   a.pos = h->pos4;
   a.hb = make_double3(h->hbox[0], h->hbox[1], h->hbox[2]);
+  a.L = make_double3(h->box[0], h->box[1], h->box[2]);
+  a.iL = make_double3(1.0 / h->box[0], 1.0 / h->box[1], 1.0 / h->box[2]);
   a.frc = h->frc4;
   a.accumulate = accumulate;
   a.e_row = it.e_row;

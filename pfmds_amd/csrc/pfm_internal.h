@@ -41,6 +41,19 @@ __device__ __forceinline__ double min_image_dist2(const double4& a, const double
   return x * x + y * y + z * z;
 }
 
+// Minimum image for the force passes: the same image as find_distance except
+// at an exact tie |dx| = L/2 (where only the sign of dx can differ), with
+// fused multiply-adds.  Membership tests use min_image_dist2.
+__device__ __forceinline__ double fast_dist2(const double4& a, const double4& b, const double3& L, const double3& iL,
+                                             double& dx, double& dy, double& dz) {
+  double x = b.x - a.x, y = b.y - a.y, z = b.z - a.z;
+  x = fma(-L.x, rint(x * iL.x), x);
+  y = fma(-L.y, rint(y * iL.y), y);
+  z = fma(-L.z, rint(z * iL.z), z);
+  dx = x; dy = y; dz = z;
+  return fma(x, x, fma(y, y, z * z));
+}
+
 // Precision policy of the pair arithmetic.  R = double: every operation in
 // FP64 with the library's correctly rounded elementary functions.  R = float:
 // dr is formed in FP64 and rounded once, the pair terms use the gfx950
@@ -225,3 +238,4 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step);
 int launch_nn_from_tb(pfm_ctx* h, int ii, int step);
 int launch_forces(pfm_ctx* h, bool energy, bool first_writes);
 void launch_group_rows(pfm_ctx* h);
+void scan_excl(pfm_ctx* h, int n, const int* in, int* out, int* tmp);

@@ -36,7 +36,8 @@ __global__ void k_cell_keys(int N, const double4* __restrict__ pos, CellGrid g, 
   atomicAdd(&cell_count[r], 1);
 }
 
-// exclusive scan of n ints by one workgroup of 1024 threads
+// exclusive scan of n ints by one workgroup of 1024 threads (small n: the
+// cell counts, and the block sums of scan_excl below)
 __global__ void __launch_bounds__(1024) k_scan_excl(int n, const int* __restrict__ in, int* __restrict__ out,
                                                     int* __restrict__ total) {
   __shared__ int s[1024];
@@ -62,6 +63,60 @@ __global__ void __launch_bounds__(1024) k_scan_excl(int n, const int* __restrict
   if (t == 1023 && total) *total = s[1023];
 }
 
+#define SCAN_B 1024  // items per workgroup of the multi-block scan (256 threads x 4)
+
+__device__ __forceinline__ int wg_inclusive_scan(int v, int* sh) {
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    int w = t >= o ? sh[t - o] : 0;
+    __syncthreads();
+    sh[t] += w;
+    __syncthreads();
+  }
+  int r = sh[t];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(256) k_scan_blocksum(int n, const int* __restrict__ in, int* __restrict__ bsum) {
+  __shared__ int sh[256];
+  int base = blockIdx.x * SCAN_B + threadIdx.x * 4, v = 0;
+  for (int k = 0; k < 4; ++k)
+    if (base + k < n) v += in[base + k];
+  int inc = wg_inclusive_scan(v, sh);
+  if (threadIdx.x == 255) bsum[blockIdx.x] = inc;
+}
+
+__global__ void __launch_bounds__(256) k_scan_apply(int n, const int* __restrict__ in, const int* __restrict__ boff,
+                                                    int* __restrict__ out) {
+  __shared__ int sh[256];
+  int base = blockIdx.x * SCAN_B + threadIdx.x * 4;
+  int x[4], v = 0;
+  for (int k = 0; k < 4; ++k) {
+    x[k] = base + k < n ? in[base + k] : 0;
+    v += x[k];
+  }
+  int run = wg_inclusive_scan(v, sh) - v + boff[blockIdx.x];
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += x[k];
+  }
+}
+
+// exclusive scan of n ints; tmp holds ceil(n/1024) block sums and their scan
+void scan_excl(pfm_ctx* h, int n, const int* in, int* out, int* tmp) {
+  int nb = (n + SCAN_B - 1) / SCAN_B;
+  if (nb <= 1) {
+    PFM_LAUNCH(h, "scan", k_scan_excl, 1, 1024, n, in, out, (int*)nullptr);
+    return;
+  }
+  PFM_LAUNCH(h, "scan", k_scan_blocksum, nb, 256, n, in, tmp);
+  PFM_LAUNCH(h, "scan", k_scan_excl, 1, 1024, nb, tmp, tmp + nb, (int*)nullptr);
+  PFM_LAUNCH(h, "scan", k_scan_apply, nb, 256, n, in, tmp + nb, out);
+}
+
 __global__ void k_scatter(int N, const int* __restrict__ atom_rank, const int* __restrict__ cell_start,
                           int* __restrict__ cell_fill, int* __restrict__ perm) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -71,19 +126,25 @@ __global__ void k_scatter(int N, const int* __restrict__ atom_rank, const int* _
 }
 
 // order the atoms of each cell by original index, so the layout (and every
-// sum over it) does not depend on the order of the scatter's atomics
-__global__ void k_cell_sort(int ncell, const int* __restrict__ cell_start, const int* __restrict__ cell_count,
-                            int* __restrict__ perm, const int* __restrict__ orig) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+// sum over it) does not depend on the order of the scatter's atomics: one
+// 64-lane workgroup per cell, each lane ranks its entries against the cell
+__global__ void __launch_bounds__(64) k_cell_sort(int ncell, const int* __restrict__ cell_start,
+                                                  const int* __restrict__ cell_count, const int* __restrict__ perm_in,
+                                                  int* __restrict__ perm_out, const int* __restrict__ orig) {
+  __shared__ int key[512];
+  int c = blockIdx.x;
   if (c >= ncell) return;
   int s = cell_start[c], n = cell_count[c];
-  for (int a = 1; a < n; ++a) {
-    int v = perm[s + a], key = orig[v], b = a - 1;
-    while (b >= 0 && orig[perm[s + b]] > key) {
-      perm[s + b + 1] = perm[s + b];
-      --b;
-    }
-    perm[s + b + 1] = v;
+  bool lds = n <= 512;
+  if (lds)
+    for (int k = threadIdx.x; k < n; k += 64) key[k] = orig[perm_in[s + k]];
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += 64) {
+    int v = perm_in[s + k];
+    int kk = lds ? key[k] : orig[v];
+    int rank = 0;
+    for (int m = 0; m < n; ++m) rank += ((lds ? key[m] : orig[perm_in[s + m]]) < kk) ? 1 : 0;
+    perm_out[s + rank] = v;
   }
 }
 
@@ -114,14 +175,15 @@ static CellGrid make_grid(pfm_ctx* h) {
 // Bin atoms into cells and re-order every per-atom array in Morton cell order.
 int launch_sort_and_bin(pfm_ctx* h, int step) {
   (void)step;
-  int N = h->N, nb = (N + 255) / 256, ncb = (h->ncell + 255) / 256;
+  int N = h->N, nb = (N + 255) / 256;
   CellGrid g = make_grid(h);
   (void)hipMemsetAsync(h->cell_count, 0, sizeof(int) * h->ncell, h->stream);
   (void)hipMemsetAsync(h->cell_fill, 0, sizeof(int) * h->ncell, h->stream);
   PFM_LAUNCH(h, "cell_keys", k_cell_keys, nb, 256, N, h->pos4, g, h->cell_rank, h->atom_rank, h->cell_count);
-  PFM_LAUNCH(h, "scan", k_scan_excl, 1, 1024, h->ncell, h->cell_count, h->cell_start, (int*)nullptr);
-  PFM_LAUNCH(h, "scatter", k_scatter, nb, 256, N, h->atom_rank, h->cell_start, h->cell_fill, h->perm);
-  PFM_LAUNCH(h, "cell_sort", k_cell_sort, ncb, 256, h->ncell, h->cell_start, h->cell_count, h->perm, h->orig);
+  scan_excl(h, h->ncell, h->cell_count, h->cell_start, h->scan_tmp);
+  PFM_LAUNCH(h, "scatter", k_scatter, nb, 256, N, h->atom_rank, h->cell_start, h->cell_fill, h->scan_tmp + N);
+  PFM_LAUNCH(h, "cell_sort", k_cell_sort, h->ncell, 64, h->ncell, h->cell_start, h->cell_count, h->scan_tmp + N,
+             h->perm, h->orig);
   PFM_LAUNCH(h, "permute", k_permute, nb, 256, N, h->perm, h->pos4, h->vel4, h->frc4, h->mask, h->orig, h->pos4b,
              h->vel4b, h->frc4b, h->maskb, h->origb);
   std::swap(h->pos4, h->pos4b);
@@ -151,14 +213,23 @@ __global__ void k_compact(int N, const uint32_t* __restrict__ mask, uint32_t bit
   }
 }
 
+__global__ void k_identity_rows(int N, int* __restrict__ rows, int* __restrict__ row_of) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) { rows[i] = i; row_of[i] = i; }
+}
+
 // rows of every group that some list or role uses: members in internal order
 void launch_group_rows(pfm_ctx* h) {
   int N = h->N, nb = (N + 255) / 256;
   for (int g = 1; g <= PFM_MAXG; ++g) {
     if (!h->grp.used[g] || h->grp.n[g] == 0) continue;
     uint32_t bit = 1u << (g - 1);
+    if (h->grp.n[g] == N) {
+      PFM_LAUNCH(h, "group_rows", k_identity_rows, nb, 256, N, h->grp.rows[g], h->grp.row_of[g]);
+      continue;
+    }
     PFM_LAUNCH(h, "group_rows", k_flag, nb, 256, N, h->mask, bit, h->scan_tmp);
-    PFM_LAUNCH(h, "group_rows", k_scan_excl, 1, 1024, N, h->scan_tmp, h->scan_tmp + N, (int*)nullptr);
+    scan_excl(h, N, h->scan_tmp, h->scan_tmp + N, h->scan_tmp + 2 * (size_t)N);
     PFM_LAUNCH(h, "group_rows", k_compact, nb, 256, N, h->mask, bit, h->scan_tmp + N, h->grp.rows[g],
                h->grp.row_of[g]);
   }
@@ -173,8 +244,8 @@ struct SearchArgs {
   const double4* pos;
   const uint32_t* mask;
   const int* orig;
-  double3 hb;
-  double rc2;
+  double3 hb, L, iL;
+  double rc2, rc2_lo, rc2_hi;
   CellGrid g;
   const int* cell_rank;
   const int* cell_start;
@@ -228,8 +299,17 @@ __global__ void __launch_bounds__(256) k_build(SearchArgs a, int nblocks_real) {
           if (k < n) {
             j = s + k;
             if (j != i && (a.mask[j] & a.g2bit)) {
-              double dx, dy, dz;
-              pred = min_image_dist2(pi, a.pos[j], a.hb, dx, dy, dz) < a.rc2;
+              double4 pj = a.pos[j];
+              double ax = pj.x - pi.x, ay = pj.y - pi.y, az = pj.z - pi.z;
+              ax = fma(-a.L.x, rint(ax * a.iL.x), ax);
+              ay = fma(-a.L.y, rint(ay * a.iL.y), ay);
+              az = fma(-a.L.z, rint(az * a.iL.z), az);
+              double d2 = fma(ax, ax, fma(ay, ay, az * az));
+              if (d2 < a.rc2_lo) pred = true;
+              else if (d2 <= a.rc2_hi) {
+                double dx, dy, dz;
+                pred = min_image_dist2(pi, pj, a.hb, dx, dy, dz) < a.rc2;
+              }
             }
           }
           unsigned long long bits = __ballot(pred) & gmask;
@@ -335,6 +415,10 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   a.hb = make_double3(h->hbox[0], h->hbox[1], h->hbox[2]);
   double rc = (k == 1 && (it.kind == PFM_LJ || it.kind == PFM_LJC || it.kind == PFM_MORSEC)) ? it.nl[0].rcut : L.rcut;
   a.rc2 = rc * rc;
+  a.rc2_lo = a.rc2 * (1.0 - 1e-12);
+  a.rc2_hi = a.rc2 * (1.0 + 1e-12);
+  a.L = make_double3(h->box[0], h->box[1], h->box[2]);
+  a.iL = make_double3(1.0 / h->box[0], 1.0 / h->box[1], 1.0 / h->box[2]);
   a.g = make_grid(h);
   a.cell_rank = h->cell_rank;
   a.cell_start = h->cell_start;
