@@ -30,6 +30,10 @@ PARAM_LINES = {
     "tb": [8, 2],
 }
 NL_N = {"lj": 2, "lj1g": 1, "ljc": 3, "morsec": 3, "rjl": 1, "tb": 1}
+def _f(x):
+    return repr(float(x))
+
+
 MASS_COEF = 1.6605389217 / 1.6021765654 * 100.0
 KB = 1.3806488 / 1.6021765654 * (1.0 / 10000.0)
 
@@ -126,18 +130,18 @@ class System:
             "name dt steps period_snapshot period_log",
         ]
         for it in self.integrators:
-            lines.append(f"{it.name} {it.dt!r} {it.length} {it.period_snapshot} {it.period_log}")
-        lines.append(f"ms_de {self.ms_de!r}")
+            lines.append(f"{it.name} {_f(it.dt)} {it.length} {it.period_snapshot} {it.period_log}")
+        lines.append(f"ms_de {_f(self.ms_de)}")
         lines.append(f"nhc_num {len(self.nhcs)}")
         for c in self.nhcs:
-            lines.append(f"{c.group} {c.temperature!r} {c.M} {c.q1!r}")
-        lines.append(f"initial_temperature {self.initial_temperature!r}")
+            lines.append(f"{c.group} {_f(c.temperature)} {c.M} {_f(c.q1)}")
+        lines.append(f"initial_temperature {_f(self.initial_temperature)}")
         lines.append(f"interactions_num {len(self.interactions)}")
         for k, it in enumerate(self.interactions):
             pf = it.param_file or f"{it.name}_{k + 1}.txt"
             lines.append(f"{it.name} {pf}")
             for (g1, g2, nmax, rc, per) in it.lists:
-                lines.append(f"{g1} {g2} {nmax} {rc!r} {per}")
+                lines.append(f"{g1} {g2} {nmax} {_f(rc)} {per}")
             write_params(os.path.join(directory, pf), it)
         with open(os.path.join(directory, settings), "w") as f:
             f.write("\n".join(lines) + "\n")
@@ -153,7 +157,7 @@ def write_params(path, it):
         if it.name in ("ljc", "morsec") and len(out) == 2:
             out.append("T" if chunk[0] else "F")
         else:
-            out.append(" ".join(repr(float(v)) for v in chunk))
+            out.append(" ".join(_f(v) for v in chunk))
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
 
@@ -164,7 +168,7 @@ def write_xyz(path, s, digits=17):
     fmt = f"%.{digits}g"
     with open(path, "w") as f:
         f.write(f"{s.n}\n")
-        f.write(f'Lattice=" {b[0]!r} 0.0 0.0 0.0 {b[1]!r} 0.0 0.0 0.0 {b[2]!r} " '
+        f.write(f'Lattice=" {_f(b[0])} 0.0 0.0 0.0 {_f(b[1])} 0.0 0.0 0.0 {_f(b[2])} " '
                 "Properties=pos:R:3:vel:R:3:mass:R:1:species:S:1\n")
         rows = np.column_stack([s.pos, s.vel, s.mass])
         for r, sp in zip(rows, s.species):

@@ -1,0 +1,66 @@
+"""The Fortran front end (pfmds_amd/bin/run_md_simulation), reading a
+reference-format settings file and driving libpfmds through ISO_C_BINDING,
+against the Python restatement of md()'s schedule driving the CPU checker:
+same log records (f24.6 as written), same nvms exit step, same final xyz."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from pfmds_amd import configs, system
+from tests.harness import oracle_ctx
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "pfmds_amd", "bin", "run_md_simulation")
+
+
+def parse_log(path, n_inter, n_nhc):
+    recs = []
+    for line in open(path):
+        try:
+            name = line[0:6].strip()
+            step = int(line[6:15])
+            vals = [float(line[15 + 24 * k:15 + 24 * (k + 1)]) for k in range(7)]
+            off = 15 + 24 * 7
+            e = [float(line[off + 20 * k:off + 20 * (k + 1)]) for k in range(n_inter + n_nhc)]
+        except ValueError:
+            continue
+        recs.append((name, step, vals, e))
+    return recs
+
+
+@pytest.mark.parametrize("case", ["lj_gas", "gr_cu"])
+def test_frontend_matches_checker(tmp_path, case):
+    if case == "lj_gas":
+        s = configs.binary_lj_gas(n=500, box=30.0, seed=51, schedule=(("nvt", 0.5, 30, 25, 5), ("nve", 0.5, 0, 5, 5),
+                                                                       ("nve", 0.5, 30, 100, 5),
+                                                                       ("nvms", 0.5, 400, 1000, 5)))
+        s.ms_de = 1e-5
+    else:
+        s = configs.graphene_on_cu(cx=10, cy=6, gx=10, gy=6, layers=4, seed=52)
+        s.integrators = [system.Integrator("nvms", 0.5, 40, 1000, 4), system.Integrator("nvt", 0.5, 40, 20, 4)]
+        s.ms_de = 1e-12
+    s.md_step_limit = sum(i.length for i in s.integrators)
+    s.write_files(str(tmp_path))
+    print(f"[frontend] {case} n={s.n}", flush=True)
+    p = subprocess.run([EXE, "-i", "md_run_settings.txt", "-op", "10"], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    got = parse_log(os.path.join(str(tmp_path), s.logfilename), len(s.interactions), len(s.nhcs))
+    o = oracle_ctx(s)
+    want, last = system.run_md(s, o, out_period=10)
+    assert [(r[0], r[1]) for r in got] == [(r.integrator, r.step) for r in want]
+    for g, w in zip(got, want):
+        ref = [w.time, w.conserved, w.e_nhc, w.total, w.pe, w.ke, w.temp]
+        np.testing.assert_allclose(g[2], ref, atol=2e-6, rtol=1e-9)
+        np.testing.assert_allclose(g[3], list(w.e_inter) + list(w.e_chains), atol=2e-9, rtol=1e-9)
+    steps_line = [ln for ln in p.stdout.splitlines() if "steps number:" in ln]
+    assert steps_line and int(steps_line[-1].split(":")[-1]) == last - 1
+    fin = os.path.join(str(tmp_path), "final_" + s.init_xyz_filename)
+    _, pf, vf, _, _ = system.read_xyz(fin)
+    po, vo, _ = o.state()
+    g = s.group_indices(s.all_atoms) - 1
+    np.testing.assert_allclose(pf, po[g], atol=1e-8)
+    np.testing.assert_allclose(vf, vo[g], atol=1e-8)
