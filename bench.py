@@ -71,6 +71,20 @@ def rjl_counts(n_entries):
     return {"rjl_density": (b_density, f_density), "rjl_force": (b_force, f_force)}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of the mixed-precision `kernel` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy);
+    None when no summary for it is committed."""
+    path = os.path.join(ROOT, "profiles", "rjl_fcc_cu_131k_traffic.json")
+    if not os.path.exists(path):
+        return None
+    data = json.load(open(path))["kernels"]
+    for name, v in data.items():
+        if f"k_{kernel}<" in name and ", float" in name and "true" not in name:
+            return v["bytes_per_launch"]
+    return None
+
+
 def cpu_baseline(s, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from tests.harness import oracle_ctx, oracle_lib
@@ -140,7 +154,7 @@ def main():
     flops_launch = counts[dom][1] * s.n
     achieved = bytes_launch / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "mean_us": ms * 1e3,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom, "mean_us": ms * 1e3,
                 "launches": cnt, "bytes_per_launch": bytes_launch,
                 "valu": {"achieved_tflops": flops_launch / (ms * 1e-3) / 1e12, "peak_tflops": FP32_PEAK_TFLOPS,
                          "frac": flops_launch / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS}}

@@ -61,6 +61,7 @@ struct IntP {
   const double* chain_s;    // per-chain velocity scale (device)
   int scale, invert;
   double zlo, zhi;
+  uint32_t zero_bit;  // zero the forces of these atoms after the first kick (zero_forces)
 };
 
 struct NhcDev {
@@ -90,7 +91,7 @@ __device__ __forceinline__ void invert_v(double4& v, double z, double lo, double
 // with single wrap (md_integrators.f90:7-97), and check_positions of the
 // next step (md_general.f90:342-364)
 __global__ void k_kick_drift(int N, double4* __restrict__ pos, double4* __restrict__ vel,
-                             const double4* __restrict__ frc, const uint32_t* __restrict__ mask, IntP P, DevErr* err,
+                             double4* __restrict__ frc, const uint32_t* __restrict__ mask, IntP P, DevErr* err,
                              const int* __restrict__ orig, int step) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -116,6 +117,7 @@ __global__ void k_kick_drift(int N, double4* __restrict__ pos, double4* __restri
     pos[i] = p;
   }
   vel[i] = v;
+  if (m & P.zero_bit) frc[i] = make_double4(0, 0, 0, 0);
   if (!(in_cell(p.x, P.box[0]) && in_cell(p.y, P.box[1]) && in_cell(p.z, P.box[2])))
     raise_err(err, PFM_E_OUT_OF_CELL, orig[i] + 1, 0, step + 1);
 }
@@ -132,6 +134,27 @@ __global__ void k_check(int N, const double4* __restrict__ pos, double bx, doubl
     invert_v(v, p.z, zlo, zhi);
     vel[i] = v;
   }
+}
+
+#define RED_THREADS 256
+#define RED_MAXQ 8
+
+struct MaskQ {
+  int nq;
+  uint32_t bit[RED_MAXQ];
+};
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
+    if (t < o) sh[t] += sh[t + o];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
 }
 
 // second half: kick xyz/z, then for nvms the projections
@@ -161,6 +184,36 @@ __global__ void k_kick2(int N, double4* __restrict__ vel, const double4* __restr
   vel[i] = v;
 }
 
+// second half with the kinetic energies of the chain groups after the kick
+// (the input of the second NHC half step), per block, grid-stride
+__global__ void __launch_bounds__(RED_THREADS) k_kick2_ke(int N, double4* __restrict__ vel,
+                                                          const double4* __restrict__ frc,
+                                                          const uint32_t* __restrict__ mask, IntP P, MaskQ Q,
+                                                          double* __restrict__ part) {
+  __shared__ double sh[RED_THREADS];
+  double acc[RED_MAXQ];
+  for (int q = 0; q < Q.nq; ++q) acc[q] = 0.0;
+  for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
+    uint32_t m = mask[i];
+    bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
+    double4 v = vel[i];
+    if (bx || bz) {
+      double4 f = frc[i];
+      double k = 1.0 / v.w / PFM_MASS_COEF;
+      if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
+      if (bz) v.z = v.z + f.z * k * P.ts1;
+      vel[i] = v;
+    }
+    double ke = v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF;
+    for (int q = 0; q < Q.nq; ++q)
+      if (m & Q.bit[q]) acc[q] += ke;
+  }
+  for (int q = 0; q < Q.nq; ++q) {
+    double s = block_sum(acc[q], sh);
+    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = s;
+  }
+}
+
 __global__ void k_scale(int N, double4* __restrict__ vel, const uint32_t* __restrict__ mask, IntP P) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -179,26 +232,6 @@ __global__ void k_zero_masked(int N, double4* __restrict__ frc, const uint32_t* 
   if (i < N && (mask[i] & bit)) frc[i] = make_double4(0, 0, 0, 0);
 }
 
-#define RED_THREADS 256
-#define RED_MAXQ 8
-
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-  int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
-    if (t < o) sh[t] += sh[t + o];
-    __syncthreads();
-  }
-  double r = sh[0];
-  __syncthreads();
-  return r;
-}
-
-struct MaskQ {
-  int nq;
-  uint32_t bit[RED_MAXQ];
-};
 
 // per-block partial kinetic energies m v.v/2*mass_coef of nq groups
 // (calculate_kinetic_energy, md_general.f90:161-182)
@@ -272,14 +305,10 @@ __global__ void k_sub_mcv(int N, double4* __restrict__ vel, const uint32_t* __re
   vel[i] = v;
 }
 
-// integrate_nose_hoover_chain (md_integrators.f90:200-245) for every chain,
-// one thread per chain; ke[c] is the chain group's kinetic energy
-__global__ void k_nhc_half(int nchain, NhcDev* __restrict__ C, const double* __restrict__ ke_in,
-                           double* __restrict__ s_out, double ts1, double ts2, double ts3) {
-  int c = threadIdx.x;
-  if (c >= nchain) return;
-  NhcDev& h = C[c];
-  double ke = ke_in[c], kt = h.kt;
+// integrate_nose_hoover_chain (md_integrators.f90:200-245) on one chain with
+// the chain group's kinetic energy ke; returns the velocity scale s
+__device__ double nhc_update(NhcDev& h, double ke, double ts1, double ts2, double ts3) {
+  double kt = h.kt;
   double kedif = 2.0 * ke - 3.0 * h.L * kt;
   int M = h.M;
   double b = 1.0;
@@ -309,7 +338,30 @@ __global__ void k_nhc_half(int nchain, NhcDev* __restrict__ C, const double* __r
     }
     v[M - 1] = v[M - 1] + (q[M - 2] * v[M - 2] * v[M - 2] - kt) / q[M - 1] * ts2;
   }
-  s_out[c] = h.s;
+  return h.s;
+}
+
+// Reduce each chain's kinetic-energy partials (fixed order) and apply one NHC
+// half step (twice = 0), or the second half of this step followed by the first
+// half of the next (twice = 1).  Nothing changes the velocities between those
+// two halves except invert_z_velocities, which keeps the kinetic energy, so
+// the second one uses s^2 * ke; s_out receives the product of the scales,
+// applied to the velocities by the next kick.
+__global__ void __launch_bounds__(RED_THREADS) k_nhc_chain(int nchain, NhcDev* __restrict__ C,
+                                                           const double* __restrict__ part, int nb,
+                                                           double* __restrict__ s_out, double ts1, double ts2,
+                                                           double ts3, int twice) {
+  __shared__ double sh[RED_THREADS];
+  for (int c = 0; c < nchain; ++c) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RED_THREADS) a += part[c * nb + b];
+    double ke = block_sum(a, sh);
+    if (threadIdx.x == 0) {
+      double s = nhc_update(C[c], ke, ts1, ts2, ts3);
+      if (twice) s *= nhc_update(C[c], s * s * ke, ts1, ts2, ts3);
+      s_out[c] = s;
+    }
+  }
 }
 
 // ------------------------------------------------------------- host helpers
@@ -371,9 +423,16 @@ static int finalize(pfm_ctx* h) {
   for (int r = 0; r < h->ncell; ++r) rank[key[r].second] = r;
   if ((rc = dalloc(h, &h->cell_rank, h->ncell))) return rc;
   HIPCHK(h, hipMemcpyAsync(h->cell_rank, rank.data(), sizeof(int) * h->ncell, hipMemcpyHostToDevice, h->stream));
-  if ((rc = dalloc(h, &h->cell_count, h->ncell))) return rc;
+  std::vector<int> xyz(h->ncell);
+  for (int r = 0; r < h->ncell; ++r) {
+    int lin = key[r].second, x = lin % h->nc[0], y = (lin / h->nc[0]) % h->nc[1], z = lin / (h->nc[0] * h->nc[1]);
+    xyz[r] = x | (y << 10) | (z << 20);
+  }
+  if ((rc = dalloc(h, &h->cell_xyz, h->ncell))) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->cell_xyz, xyz.data(), sizeof(int) * h->ncell, hipMemcpyHostToDevice, h->stream));
+  if ((rc = dalloc(h, &h->cell_count, 2 * (size_t)h->ncell))) return rc;  // counts, then fill counters
+  h->cell_fill = h->cell_count + h->ncell;
   if ((rc = dalloc(h, &h->cell_start, h->ncell))) return rc;
-  if ((rc = dalloc(h, &h->cell_fill, h->ncell))) return rc;
   // group rows
   for (int g = 1; g <= PFM_MAXG; ++g) {
     if (!h->grp.used[g]) continue;
@@ -483,15 +542,21 @@ static IntP int_params(pfm_ctx* h, double dt) {
   return P;
 }
 
-static void nhc_half(pfm_ctx* h, double dt) {
-  int nb = h->red_blocks;
+static MaskQ chain_masks(pfm_ctx* h) {
   MaskQ Q;
+  memset(&Q, 0, sizeof Q);
   Q.nq = (int)h->nhc.size();
   for (int c = 0; c < Q.nq; ++c) Q.bit[c] = 1u << (h->nhc[c].group - 1);
+  return Q;
+}
+
+// first NHC half step of a step: kinetic energies, then the chains
+static void nhc_first_half(pfm_ctx* h, double dt) {
+  int nb = h->red_blocks;
+  MaskQ Q = chain_masks(h);
   PFM_LAUNCH(h, "nhc", k_ke_part, nb, RED_THREADS, h->N, h->vel4, h->mask, Q, h->red_part);
-  PFM_LAUNCH(h, "nhc", k_reduce_rows, 1, RED_THREADS, Q.nq, nb, h->red_part, h->red_out + 8);
-  PFM_LAUNCH(h, "nhc", k_nhc_half, 1, 64, Q.nq, (NhcDev*)h->nhc_state, h->red_out + 8, h->red_out + 16, dt / 2.0,
-             dt / 4.0, dt / 8.0);
+  PFM_LAUNCH(h, "nhc", k_nhc_chain, 1, RED_THREADS, Q.nq, (NhcDev*)h->nhc_state, h->red_part, nb, h->red_out + 16,
+             dt / 2.0, dt / 4.0, dt / 8.0, 0);
 }
 
 // Energies of the step: per-interaction sums of the per-row energies and the
@@ -612,7 +677,8 @@ int pfm_create(pfm_ctx** out, int n, const double box[3], const double* pos, con
       (rc = dalloc(h, &h->pos4b, n)) || (rc = dalloc(h, &h->vel4b, n)) || (rc = dalloc(h, &h->frc4b, n)) ||
       (rc = dalloc(h, &h->mask, n)) || (rc = dalloc(h, &h->maskb, n)) || (rc = dalloc(h, &h->orig, n)) ||
       (rc = dalloc(h, &h->origb, n)) || (rc = dalloc(h, &h->perm, n)) || (rc = dalloc(h, &h->atom_rank, n)) ||
-      (rc = dalloc(h, &h->scan_tmp, 3 * (size_t)n + 4096)) || (rc = dalloc(h, &h->eb, n)) || (rc = dalloc(h, &h->inv_eb, n)))
+      (rc = dalloc(h, &h->scan_tmp, 3 * (size_t)n + 4096)) || (rc = dalloc(h, &h->eb, n)) || (rc = dalloc(h, &h->inv_eb, n)) ||
+      (rc = dalloc(h, &h->inv_ebf, n)))
     return rc;
   h->red_blocks = std::max(1, std::min(1024, (n + RED_THREADS - 1) / RED_THREADS));
   if ((rc = dalloc(h, &h->red_part, (size_t)(PFM_MAXI + 1 + RED_MAXQ) * h->red_blocks))) return rc;
@@ -763,6 +829,8 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
   PFM_LAUNCH(h, "check", k_check, nb, 256, N, h->pos4, h->box[0], h->box[1], h->box[2], h->derr, h->orig, s0,
              (s0 == 0 && h->invert_z) ? 1 : 0, h->vel4, P.zlo, P.zhi);
   if ((rc = check_err(h, s1))) return rc;
+  const bool nvt = integ == PFM_NVT && !h->nhc.empty();
+  bool nhc_pending = false;
   for (int s = s0; s <= s1; ++s) {
     *last_step = s;
     bool energy = (s == s1 && energy_at_end) || integ == PFM_NVMS;
@@ -773,22 +841,33 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
       }
       IntP Q = P;
       Q.invert = h->invert_z;
-      if (integ == PFM_NVT && !h->nhc.empty()) {
-        nhc_half(h, dt);
+      Q.zero_bit = first_writes ? 0u : all_bit;
+      if (nvt) {
+        if (!nhc_pending) nhc_first_half(h, dt);
         Q.scale = 1;
+        nhc_pending = false;
       }
       PFM_LAUNCH(h, "kick_drift", k_kick_drift, nb, 256, N, h->pos4, h->vel4, h->frc4, h->mask, Q, h->derr, h->orig, s);
+    } else if (!first_writes) {
+      PFM_LAUNCH(h, "zero_forces", k_zero_masked, nb, 256, N, h->frc4, h->mask, all_bit);
     }
     h->pe_prev = h->pe;
     if ((rc = update_lists(h, s))) return rc;
     if (s % h->zm_period == 0 && (rc = zero_momentum(h))) return rc;
-    if (!first_writes) PFM_LAUNCH(h, "zero_forces", k_zero_masked, nb, 256, N, h->frc4, h->mask, all_bit);
     if ((rc = launch_forces(h, energy, first_writes))) return rc;
     if (s != 0) {
-      PFM_LAUNCH(h, "kick2", k_kick2, nb, 256, N, h->vel4, h->frc4, h->mask, P, integ == PFM_NVMS ? 1 : 0);
-      if (integ == PFM_NVT && !h->nhc.empty()) {
-        nhc_half(h, dt);
-        PFM_LAUNCH(h, "nhc_scale", k_scale, nb, 256, N, h->vel4, h->mask, P);
+      if (nvt) {
+        // second half step; on steps with nothing to report, fused with the
+        // first half of the next step (k_nhc_chain)
+        MaskQ C = chain_masks(h);
+        PFM_LAUNCH(h, "kick2", k_kick2_ke, h->red_blocks, RED_THREADS, N, h->vel4, h->frc4, h->mask, P, C, h->red_part);
+        bool fuse = !energy && s < s1;
+        PFM_LAUNCH(h, "nhc", k_nhc_chain, 1, RED_THREADS, C.nq, (NhcDev*)h->nhc_state, h->red_part, h->red_blocks,
+                   h->red_out + 16, dt / 2.0, dt / 4.0, dt / 8.0, fuse ? 1 : 0);
+        if (fuse) nhc_pending = true;
+        else PFM_LAUNCH(h, "nhc_scale", k_scale, nb, 256, N, h->vel4, h->mask, P);
+      } else {
+        PFM_LAUNCH(h, "kick2", k_kick2, nb, 256, N, h->vel4, h->frc4, h->mask, P, integ == PFM_NVMS ? 1 : 0);
       }
       h->sim_time = h->sim_time + dt;
     }
@@ -969,8 +1048,8 @@ int pfm_destroy(pfm_ctx* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(h->pos4); F(h->vel4); F(h->frc4); F(h->pos4b); F(h->vel4b); F(h->frc4b); F(h->mask); F(h->maskb);
-  F(h->orig); F(h->origb); F(h->perm); F(h->atom_rank); F(h->scan_tmp); F(h->eb); F(h->inv_eb);
-  F(h->red_part); F(h->red_out); F(h->derr); F(h->cell_rank); F(h->cell_count); F(h->cell_start); F(h->cell_fill);
+  F(h->orig); F(h->origb); F(h->perm); F(h->atom_rank); F(h->scan_tmp); F(h->eb); F(h->inv_eb); F(h->inv_ebf);
+  F(h->red_part); F(h->red_out); F(h->derr); F(h->cell_rank); F(h->cell_count); F(h->cell_start); F(h->cell_xyz);
   F(h->nhc_state);
   if (h->h_red) (void)hipHostFree(h->h_red);
   if (h->herr) (void)hipHostFree(h->herr);

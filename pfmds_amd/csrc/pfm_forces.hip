@@ -52,56 +52,85 @@ struct RjlP {
   double A0, xi, p, q, r0, R1, R2, invw;
 };
 
+// Software-pipelined walk over a row's entries p = t, t+T, ...: the neighbour
+// index is loaded two entries ahead and its position one entry ahead, so each
+// lane keeps independent loads in flight while it computes.
+#define PIPE_BEGIN(T, EXTRA_T, EXTRA_LOAD)                          \
+  int jn1 = t + (T) < n ? nbrow[t + (T)] : i;                       \
+  int jc = t < n ? nbrow[t] : i;                                    \
+  double4 qc = a.pos[jc];                                           \
+  EXTRA_T xc = EXTRA_LOAD(jc);                                      \
+  for (int p = t; p < n; p += (T)) {                                \
+    int jn2 = p + 2 * (T) < n ? nbrow[p + 2 * (T)] : i;             \
+    double4 qn = a.pos[jn1];                                        \
+    EXTRA_T xn = EXTRA_LOAD(jn1);
+#define PIPE_END \
+    jc = jn1; jn1 = jn2; qc = qn; xc = xn;                          \
+  }
+
+// density pass: rho_i = sum_{r <= R2} exp(-2q(r/r0-1)) fc(r)
 template <int T, class R>
 __global__ void __launch_bounds__(256) k_rjl_density(RowArgs a, RjlP P, double* __restrict__ eb,
-                                                     double* __restrict__ inv_eb, int nb_real) {
+                                                     double* __restrict__ inv_eb, float* __restrict__ inv_ebf,
+                                                     int nb_real) {
   ROW_PROLOGUE(T)
-  const R R1 = P.R1, R2 = P.R2, invw = P.invw, mq = -2.0 * P.q, ir0 = 1.0 / P.r0;
+  const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
+  const R kq = PM<R>::kexp(-2.0 * P.q);
   double rho = 0.0;
-  for (int p = t; p < n; p += T) {
-    int j = nbrow[p];
+#define NOLOAD(j) 0
+  PIPE_BEGIN(T, int, NOLOAD)
+    (void)xc;
     double dx, dy, dz;
-    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
+    double d2 = fast_dist2(pi, qc, a.L, a.iL, dx, dy, dz);
     R r = PM<R>::sqrt_((R)d2);
-    if (r <= R2) {
-      R fc, dfc;
-      cos_cutoff<R>(r, R1, R2, invw, fc, dfc);
-      rho += (double)(PM<R>::exp_(mq * (r * ir0 - R(1))) * fc);
-    }
-  }
+    R fc, dfr;
+    cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
+    R e = PM<R>::expk(kq * (r * ir0 - R(1))) * fc;
+    rho += (double)(r <= R2 ? e : R(0));
+  PIPE_END
+#undef NOLOAD
   rho = group_sum<T>(rho);
   if (t == 0) {
     double s = sqrt(rho);
     eb[i] = s;
     inv_eb[i] = 1.0 / s;
+    inv_ebf[i] = (float)(1.0 / s);
   }
 }
 
+template <class R> __device__ __forceinline__ R load_ieb(const double* d, const float* f, int j);
+template <> __device__ __forceinline__ double load_ieb<double>(const double* d, const float* f, int j) { return d[j]; }
+template <> __device__ __forceinline__ float load_ieb<float>(const double* d, const float* f, int j) { return f[j]; }
+
+// embedding-force pass (RosatoGuillopeLegrand.f90:80-93):
+// F_i -= sum_j [2 A0 (p/r0 fc - df r) e_p - xi (q/r0 fc - df r/2)(1/Eb_i + 1/Eb_j) e_q]/r dr
 template <int T, class R, bool E>
 __global__ void __launch_bounds__(256) k_rjl_force(RowArgs a, RjlP P, const double* __restrict__ eb,
-                                                   const double* __restrict__ inv_eb, int nb_real) {
+                                                   const double* __restrict__ inv_eb,
+                                                   const float* __restrict__ inv_ebf, int nb_real) {
   ROW_PROLOGUE(T)
-  const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0;
-  const R mp = -P.p, mq = -2.0 * P.q, c2A = 2.0 * P.A0, pr0 = P.p / P.r0, qr0 = P.q / P.r0, xi = P.xi;
+  const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
+  const R kp = PM<R>::kexp(-P.p), kq = PM<R>::kexp(-2.0 * P.q);
+  const R c2A = 2.0 * P.A0, pr0 = P.p / P.r0, qr0 = P.q / P.r0, xi = P.xi;
   const R ie = (R)inv_eb[i];
   double fx = 0, fy = 0, fz = 0, er = 0;
-  for (int p = t; p < n; p += T) {
-    int j = nbrow[p];
+#define IEB(j) load_ieb<R>(inv_eb, inv_ebf, j)
+  PIPE_BEGIN(T, R, IEB)
     double dx, dy, dz;
-    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
+    double d2 = fast_dist2(pi, qc, a.L, a.iL, dx, dy, dz);
     R r = PM<R>::sqrt_((R)d2);
-    if (r <= R2) {
-      R fc, dfc;
-      cos_cutoff<R>(r, R1, R2, invw, fc, dfc);
-      R x = r * ir0 - R(1);
-      R ep = PM<R>::exp_(mp * x), eq = PM<R>::exp_(mq * x);
-      R c = (c2A * (pr0 * fc - dfc * r) * ep - xi * (qr0 * fc - dfc * R(0.5) * r) * (ie + (R)inv_eb[j]) * eq) *
-            PM<R>::rcp(r);
-      double cd = (double)c;
-      fx -= cd * dx; fy -= cd * dy; fz -= cd * dz;
-      if (E) er += (double)(ep * fc);
-    }
-  }
+    R ir = PM<R>::rcp(r);
+    R fc, dfr;
+    cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
+    R x = r * ir0 - R(1);
+    R ep = PM<R>::expk(kp * x), eq = PM<R>::expk(kq * x);
+    R c = (c2A * (pr0 * fc - dfr) * ep - xi * (qr0 * fc - R(0.5) * dfr) * (ie + xc) * eq) * ir;
+    bool in = r <= R2;
+    double cd = in ? (double)c : 0.0;
+    fx -= cd * dx; fy -= cd * dy; fz -= cd * dz;
+    if (E) er += in ? (double)(ep * fc) : 0.0;
+  PIPE_END
+#undef IEB
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (E) er = group_sum<T>(er);
   if (t == 0) {
@@ -119,22 +148,22 @@ struct LjP {
 template <int T, class R, bool E>
 __global__ void __launch_bounds__(256) k_lj(RowArgs a, LjP P, int nb_real) {
   ROW_PROLOGUE(T)
-  const R R1 = P.R1, R2 = P.R2, invw = P.invw, sig = P.sig, e4 = P.eps4;
+  const R R1 = P.R1, R2 = P.R2, invw = P.invw, sig = P.sig, e4 = P.eps4, kdf = -0.5 * PFM_PI * P.invw;
   double fx = 0, fy = 0, fz = 0, e = 0;
   for (int p = t; p < n; p += T) {
     int j = nbrow[p];
     double dx, dy, dz;
     double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
     R r = PM<R>::sqrt_((R)d2);
-    if (r < R2) {
-      R fc, dfc;
-      cos_cutoff<R>(r, R1, R2, invw, fc, dfc);
-      R s = sig * PM<R>::rcp(r), s2 = s * s, V = s2 * s2 * s2;
-      R c = e4 * (V * (R(12) * V - R(6)) * PM<R>::rcp(r * r) * fc - V * (V - R(1)) * dfc);
-      double cd = (double)c;
-      fx -= cd * dx; fy -= cd * dy; fz -= cd * dz;
-      if (E) e += (double)(e4 * V * (V - R(1)) * fc);
-    }
+    R ir = PM<R>::rcp(r);
+    R fc, dfr;
+    cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
+    R s = sig * ir, s2 = s * s, V = s2 * s2 * s2;
+    R c = e4 * (V * (R(12) * V - R(6)) * fc * ir - V * (V - R(1)) * dfr) * ir;
+    bool in = r < R2;
+    double cd = in ? (double)c : 0.0;
+    fx -= cd * dx; fy -= cd * dy; fz -= cd * dz;
+    if (E) e += in ? (double)(e4 * V * (V - R(1)) * fc) : 0.0;
   }
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (E) e = group_sum<T>(e);
@@ -365,10 +394,11 @@ __global__ void k_gnorm(int nrows, const int* __restrict__ rows, const int* __re
 
 template <class R>
 __device__ __forceinline__ void cos_pair(const CosP& P, R r, R nd, R nnorm_r, R& A, R& Bc, R& S_coef, R& epair) {
-  const R R1 = P.R1, R2 = P.R2, invw = P.invw, delt = P.delt;
-  R fc, dfc;
-  cos_cutoff<R>(r, R1, R2, invw, fc, dfc);
+  const R R1 = P.R1, invw = P.invw, delt = P.delt, kdf = -0.5 * PFM_PI * P.invw;
   R ir = PM<R>::rcp(r);
+  R fc, dfr;
+  cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
+  R dfc = dfr * ir;
   R V2;
   if (P.morse) V2 = PM<R>::exp_(-R(P.a) * (r - R(P.r0)));
   else { R s = R(P.sig) * ir, s2 = s * s; V2 = s2 * s2 * s2; }
@@ -391,11 +421,28 @@ __device__ __forceinline__ void cos_pair(const CosP& P, R r, R nd, R nnorm_r, R&
 
 // graphene side, pair and normal terms (*_forces_for_graphene first loop);
 // also S_i = sum V2 V3 fc/(n.dr) dr for the normal-derivative pass
+// The row's own normal (graphenenorm.f90:38-56) is formed here from its three
+// nearest neighbours (rows of nl(3) are the rows of nl(1)) and stored for the
+// normal-derivative and metal passes.
 template <int T, class R, bool E>
-__global__ void __launch_bounds__(256) k_cos_graphene(RowArgs a, CosP P, const double4* __restrict__ gnorm,
-                                                      double4* __restrict__ gS, int nb_real) {
+__global__ void __launch_bounds__(256) k_cos_graphene(RowArgs a, CosP P, const int* __restrict__ nn, int simplified,
+                                                      double4* __restrict__ gnorm, double4* __restrict__ gS,
+                                                      int nb_real) {
   ROW_PROLOGUE(T)
-  double4 g = gnorm[i];
+  double4 g;
+  if (simplified) {
+    g = make_double4(0, 0, 1, 0);
+  } else {
+    double d[3][3];
+    for (int k = 0; k < 3; ++k)
+      fast_dist2(pi, a.pos[nn[(size_t)row * 3 + k]], a.L, a.iL, d[k][0], d[k][1], d[k][2]);
+    double ax = d[1][0] - d[0][0], ay = d[1][1] - d[0][1], az = d[1][2] - d[0][2];
+    double bx = d[0][0] - d[2][0], by = d[0][1] - d[2][1], bz = d[0][2] - d[2][2];
+    double gx = ay * bz - az * by, gy = az * bx - ax * bz, gz = ax * by - ay * bx;
+    if (gz < 0.0) { gx = -gx; gy = -gy; gz = -gz; }
+    double inv = 1.0 / sqrt(gx * gx + gy * gy + gz * gz);
+    g = make_double4(gx * inv, gy * inv, gz * inv, 0);
+  }
   const R nx = g.x, ny = g.y, nz = g.z, R2 = P.R2;
   double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, e = 0;
   for (int p = t; p < n; p += T) {
@@ -403,15 +450,14 @@ __global__ void __launch_bounds__(256) k_cos_graphene(RowArgs a, CosP P, const d
     double dx, dy, dz;
     double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
     R r = PM<R>::sqrt_((R)d2);
-    if (r < R2) {
-      R nd = nx * (R)dx + ny * (R)dy + nz * (R)dz;
-      R A, Bc, Sc, ep;
-      cos_pair<R>(P, r, nd, PM<R>::rcp(r), A, Bc, Sc, ep);
-      double Ad = (double)A * P.pref, Bd = (double)Bc * P.pref, Sd = (double)Sc;
-      fx -= Ad * dx + Bd * g.x; fy -= Ad * dy + Bd * g.y; fz -= Ad * dz + Bd * g.z;
-      sx += Sd * dx; sy += Sd * dy; sz += Sd * dz;
-      if (E) e += (double)ep * P.pref;
-    }
+    R nd = nx * (R)dx + ny * (R)dy + nz * (R)dz;
+    R A, Bc, Sc, ep;
+    cos_pair<R>(P, r, nd, PM<R>::rcp(r), A, Bc, Sc, ep);
+    bool in = r < R2;
+    double Ad = in ? (double)A * P.pref : 0.0, Bd = in ? (double)Bc * P.pref : 0.0, Sd = in ? (double)Sc : 0.0;
+    fx -= Ad * dx + Bd * g.x; fy -= Ad * dy + Bd * g.y; fz -= Ad * dz + Bd * g.z;
+    sx += Sd * dx; sy += Sd * dy; sz += Sd * dz;
+    if (E) e += in ? (double)ep * P.pref : 0.0;
   }
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   sx = group_sum<T>(sx); sy = group_sum<T>(sy); sz = group_sum<T>(sz);
@@ -419,6 +465,7 @@ __global__ void __launch_bounds__(256) k_cos_graphene(RowArgs a, CosP P, const d
   if (t == 0) {
     write_force(a, i, fx, fy, fz);
     gS[i] = make_double4(sx, sy, sz, 0);
+    gnorm[i] = g;
     if (E) a.e_row[row] = e;
   }
 }
@@ -477,15 +524,14 @@ __global__ void __launch_bounds__(256) k_cos_metal(RowArgs a, CosP P, const doub
     double dx, dy, dz;
     double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
     R r = PM<R>::sqrt_((R)d2);
-    if (r < R2) {
-      double4 g = gnorm[j];
-      R nd = (R)g.x * (R)dx + (R)g.y * (R)dy + (R)g.z * (R)dz;
-      R nn = P.morse ? PM<R>::sqrt_((R)(g.x * g.x + g.y * g.y + g.z * g.z)) * r : r;
-      R A, Bc, Sc, ep;
-      cos_pair<R>(P, r, nd, PM<R>::rcp(nn), A, Bc, Sc, ep);
-      double Ad = (double)A * P.pref, Bd = (double)Bc * P.pref;
-      fx -= Ad * dx + Bd * g.x; fy -= Ad * dy + Bd * g.y; fz -= Ad * dz + Bd * g.z;
-    }
+    double4 g = gnorm[j];
+    R nd = (R)g.x * (R)dx + (R)g.y * (R)dy + (R)g.z * (R)dz;
+    R nn = P.morse ? PM<R>::sqrt_((R)(g.x * g.x + g.y * g.y + g.z * g.z)) * r : r;
+    R A, Bc, Sc, ep;
+    cos_pair<R>(P, r, nd, PM<R>::rcp(nn), A, Bc, Sc, ep);
+    bool in = r < R2;
+    double Ad = in ? (double)A * P.pref : 0.0, Bd = in ? (double)Bc * P.pref : 0.0;
+    fx -= Ad * dx + Bd * g.x; fy -= Ad * dy + Bd * g.y; fz -= Ad * dz + Bd * g.z;
   }
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (t == 0) write_force(a, i, fx, fy, fz);
@@ -537,9 +583,9 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         int T = h->lanes;
         DISPATCH_T(T, {
           int nb = row_blocks(a.nrows, T, BS);
-          PFM_LAUNCH(h, "rjl_density", (k_rjl_density<T, R>), pad8(nb), BS, a, P, h->eb, h->inv_eb, nb);
-          if (energy) PFM_LAUNCH(h, "rjl_force", (k_rjl_force<T, R, true>), pad8(nb), BS, a, P, h->eb, h->inv_eb, nb);
-          else PFM_LAUNCH(h, "rjl_force", (k_rjl_force<T, R, false>), pad8(nb), BS, a, P, h->eb, h->inv_eb, nb);
+          PFM_LAUNCH(h, "rjl_density", (k_rjl_density<T, R>), pad8(nb), BS, a, P, h->eb, h->inv_eb, h->inv_ebf, nb);
+          if (energy) PFM_LAUNCH(h, "rjl_force", (k_rjl_force<T, R, true>), pad8(nb), BS, a, P, h->eb, h->inv_eb, h->inv_ebf, nb);
+          else PFM_LAUNCH(h, "rjl_force", (k_rjl_force<T, R, false>), pad8(nb), BS, a, P, h->eb, h->inv_eb, h->inv_ebf, nb);
         });
         written = true;
         break;
@@ -603,16 +649,14 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
                    4.0 * it.p[0] * it.p[2], 0};
         }
         ListDev& NN = it.nl[2];
-        if (NN.nrows > 0)
-          PFM_LAUNCH(h, "gnorm", k_gnorm, (NN.nrows + 255) / 256, 256, NN.nrows, h->grp.rows[NN.g1], NN.nbr, h->pos4,
-                     make_double3(h->hbox[0], h->hbox[1], h->hbox[2]), it.gnorm, simplified ? 1 : 0);
         RowArgs a = row_args(h, it, 0, written ? 1 : 0);
         int T = h->lanes;
+        int simp = simplified ? 1 : 0;
         if (a.nrows > 0) {
           DISPATCH_T(T, {
             int nb = row_blocks(a.nrows, T, BS);
-            if (energy) PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, true>), pad8(nb), BS, a, P, it.gnorm, it.gS, nb);
-            else PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, false>), pad8(nb), BS, a, P, it.gnorm, it.gS, nb);
+            if (energy) PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, true>), pad8(nb), BS, a, P, NN.nbr, simp, it.gnorm, it.gS, nb);
+            else PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, false>), pad8(nb), BS, a, P, NN.nbr, simp, it.gnorm, it.gS, nb);
           });
           written = true;
         }

@@ -60,6 +60,11 @@ __device__ __forceinline__ double fast_dist2(const double4& a, const double4& b,
 // transcendental unit (v_exp_f32, v_sin/cos_f32, v_sqrt_f32, v_rcp_f32).
 template <class R> struct PM;
 template <> struct PM<double> {
+  // exponential exp(k*x) with a constant k prepared by kexp(k)
+  static __host__ __device__ __forceinline__ double kexp(double k) { return k; }
+  static __device__ __forceinline__ double expk(double kx) { return exp(kx); }
+  // sin(pi*x), cos(pi*x) with the reference's pi literal
+  static __device__ __forceinline__ void sincospi_(double x, double& s, double& c) { sincos(PFM_PI * x, &s, &c); }
   static __device__ __forceinline__ double exp_(double x) { return exp(x); }
   static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
   static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
@@ -68,6 +73,13 @@ template <> struct PM<double> {
   static __device__ __forceinline__ double pow_(double x, double y) { return pow(x, y); }
 };
 template <> struct PM<float> {
+  static __host__ __device__ __forceinline__ float kexp(double k) { return (float)(k * 1.4426950408889634); }
+  static __device__ __forceinline__ float expk(float kx) { return __builtin_amdgcn_exp2f(kx); }
+  // v_sin_f32 / v_cos_f32 take their argument in revolutions
+  static __device__ __forceinline__ void sincospi_(float x, float& s, float& c) {
+    s = __builtin_amdgcn_sinf(0.5f * x);
+    c = __builtin_amdgcn_cosf(0.5f * x);
+  }
   static __device__ __forceinline__ float exp_(float x) { return __expf(x); }
   static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -90,6 +102,20 @@ __device__ __forceinline__ void cos_cutoff(R r, R R1, R R2, R inv_w, R& f, R& df
   } else {
     f = R(0); df = R(0);
   }
+}
+
+// Branch-free cosine cutoff: x = clamp((r-R1)/(R2-R1), 0, 1) gives f = 1,
+// df = 0 exactly below R1 and the reference's cosine in between; entries at
+// or beyond R2 are masked by the caller.  Returns f and dfr = f'(r)
+// (= df_cut * r of the reference, whose df_cut is f'/r), kdf = -pi/(2 (R2-R1)).
+template <class R>
+__device__ __forceinline__ void cutoff_bl(R r, R R1, R inv_w, R kdf, R& f, R& dfr) {
+  R x = (r - R1) * inv_w;
+  x = x < R(0) ? R(0) : (x > R(1) ? R(1) : x);
+  R sn, cs;
+  PM<R>::sincospi_(x, sn, cs);
+  f = R(0.5) + R(0.5) * cs;
+  dfr = kdf * sn;
 }
 
 // sum over the aligned group of T lanes that share one atom row
@@ -194,9 +220,11 @@ struct pfm_ctx {
   int nc[3] = {1, 1, 1}, ncell = 1;
   double cell_len[3];
   int *cell_rank = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr, *atom_rank = nullptr;
+  int* cell_xyz = nullptr;  // by Morton rank: x | y<<10 | z<<20
   int* scan_tmp = nullptr;
   // RGL per-atom densities
   double *eb = nullptr, *inv_eb = nullptr;
+  float* inv_ebf = nullptr;  // FP32 copy of inv_eb for the mixed path
   // reductions
   double* red_part = nullptr;  // partial sums
   double* red_out = nullptr;   // device scalars
