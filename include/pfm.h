@@ -127,8 +127,9 @@ int pfm_get_normals(pfm_ctx* h, int inter, double* gr_norm);
 int pfm_set_timing(pfm_ctx* h, int on);
 int pfm_kernel_time(pfm_ctx* h, const char* name, int* count, double* mean_ms);
 
-/* Tile shape of the pair kernels: lanes per atom row (1,2,4,8,16,32,64) and
- * workgroup size; 0 keeps the current value. */
+/* Shape of the pair passes: lanes per atom row (1,2,4,8,16,32) and
+ * workgroup size (64..1024); 0 keeps the current value.  No counterpart in
+ * the reference (launch-shape control for tuning and benchmarks). */
 int pfm_set_tuning(pfm_ctx* h, int lanes_per_row, int block);
 
 int  pfm_last_error(pfm_ctx* h, char* buf, int len);

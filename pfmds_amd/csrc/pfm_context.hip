@@ -92,7 +92,7 @@ __device__ __forceinline__ void invert_v(double4& v, double z, double lo, double
 // next step (md_general.f90:342-364)
 __global__ void k_kick_drift(int N, double4* __restrict__ pos, double4* __restrict__ vel,
                              double4* __restrict__ frc, const uint32_t* __restrict__ mask, IntP P, DevErr* err,
-                             const int* __restrict__ orig, int step) {
+                             const int* __restrict__ orig, int step, uint4* __restrict__ qpos, double3 qs) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   double4 p = pos[i], v = vel[i];
@@ -115,11 +115,19 @@ __global__ void k_kick_drift(int N, double4* __restrict__ pos, double4* __restri
     }
     if (bz) p.z = wrap1(p.z + v.z * P.ts0, P.box[2]);
     pos[i] = p;
+    if (qpos) qpos[i] = quantize(p, qs, 0u);
   }
   vel[i] = v;
   if (m & P.zero_bit) frc[i] = make_double4(0, 0, 0, 0);
   if (!(in_cell(p.x, P.box[0]) && in_cell(p.y, P.box[1]) && in_cell(p.z, P.box[2])))
     raise_err(err, PFM_E_OUT_OF_CELL, orig[i] + 1, 0, step + 1);
+}
+
+// fixed-point copy of the positions (mixed path), refreshed at the start of
+// every run so positions set from the host are picked up
+__global__ void k_quantize(int N, const double4* __restrict__ pos, uint4* __restrict__ qpos, double3 qs) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) qpos[i] = quantize(pos[i], qs, 0u);
 }
 
 __global__ void k_check(int N, const double4* __restrict__ pos, double bx, double by, double bz, DevErr* err,
@@ -677,9 +685,10 @@ int pfm_create(pfm_ctx** out, int n, const double box[3], const double* pos, con
       (rc = dalloc(h, &h->pos4b, n)) || (rc = dalloc(h, &h->vel4b, n)) || (rc = dalloc(h, &h->frc4b, n)) ||
       (rc = dalloc(h, &h->mask, n)) || (rc = dalloc(h, &h->maskb, n)) || (rc = dalloc(h, &h->orig, n)) ||
       (rc = dalloc(h, &h->origb, n)) || (rc = dalloc(h, &h->perm, n)) || (rc = dalloc(h, &h->atom_rank, n)) ||
-      (rc = dalloc(h, &h->scan_tmp, 3 * (size_t)n + 4096)) || (rc = dalloc(h, &h->eb, n)) || (rc = dalloc(h, &h->inv_eb, n)) ||
-      (rc = dalloc(h, &h->inv_ebf, n)))
+      (rc = dalloc(h, &h->scan_tmp, 3 * (size_t)n + 4096)) || (rc = dalloc(h, &h->eb, n)))
     return rc;
+  for (int d = 0; d < 3; ++d) (&h->qscale.x)[d] = 4294967296.0 / box[d];
+  if (precision == PFM_MIXED && (rc = dalloc(h, &h->qpos, n))) return rc;
   h->red_blocks = std::max(1, std::min(1024, (n + RED_THREADS - 1) / RED_THREADS));
   if ((rc = dalloc(h, &h->red_part, (size_t)(PFM_MAXI + 1 + RED_MAXQ) * h->red_blocks))) return rc;
   if ((rc = dalloc(h, &h->red_out, 64))) return rc;
@@ -829,6 +838,7 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
   PFM_LAUNCH(h, "check", k_check, nb, 256, N, h->pos4, h->box[0], h->box[1], h->box[2], h->derr, h->orig, s0,
              (s0 == 0 && h->invert_z) ? 1 : 0, h->vel4, P.zlo, P.zhi);
   if ((rc = check_err(h, s1))) return rc;
+  if (h->qpos) PFM_LAUNCH(h, "quantize", k_quantize, nb, 256, N, h->pos4, h->qpos, h->qscale);
   const bool nvt = integ == PFM_NVT && !h->nhc.empty();
   bool nhc_pending = false;
   for (int s = s0; s <= s1; ++s) {
@@ -847,7 +857,8 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
         Q.scale = 1;
         nhc_pending = false;
       }
-      PFM_LAUNCH(h, "kick_drift", k_kick_drift, nb, 256, N, h->pos4, h->vel4, h->frc4, h->mask, Q, h->derr, h->orig, s);
+      PFM_LAUNCH(h, "kick_drift", k_kick_drift, nb, 256, N, h->pos4, h->vel4, h->frc4, h->mask, Q, h->derr, h->orig, s,
+                 h->qpos, h->qscale);
     } else if (!first_writes) {
       PFM_LAUNCH(h, "zero_forces", k_zero_masked, nb, 256, N, h->frc4, h->mask, all_bit);
     }
@@ -1031,7 +1042,11 @@ int pfm_set_tuning(pfm_ctx* h, int lanes, int block) {
       return fail(h, PFM_E_INVALID, "error: lanes_per_row must be 1,2,4,8,16 or 32");
     h->lanes = lanes;
   }
-  if (block) h->block = block;
+  if (block) {
+    if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
+      return fail(h, PFM_E_INVALID, "error: block must be 64, 128, 256, 512 or 1024");
+    h->block = block;
+  }
   return PFM_OK;
 }
 
@@ -1048,7 +1063,7 @@ int pfm_destroy(pfm_ctx* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(h->pos4); F(h->vel4); F(h->frc4); F(h->pos4b); F(h->vel4b); F(h->frc4b); F(h->mask); F(h->maskb);
-  F(h->orig); F(h->origb); F(h->perm); F(h->atom_rank); F(h->scan_tmp); F(h->eb); F(h->inv_eb); F(h->inv_ebf);
+  F(h->orig); F(h->origb); F(h->perm); F(h->atom_rank); F(h->scan_tmp); F(h->eb); F(h->qpos);
   F(h->red_part); F(h->red_out); F(h->derr); F(h->cell_rank); F(h->cell_count); F(h->cell_start); F(h->cell_xyz);
   F(h->nhc_state);
   if (h->h_red) (void)hipHostFree(h->h_red);

@@ -15,6 +15,8 @@
 //   tb      bond-order pass + force pass          (TersoffBrenner.f90:26-150)
 //   ljc/morsec  normals, graphene pass, normal-derivative pass, metal pass
 //           (graphenenorm.f90:38-56, LennardJonesCosine.f90, MorseCosine.f90)
+#include <algorithm>
+
 #include "pfm_internal.h"
 
 struct RowArgs {
@@ -22,12 +24,29 @@ struct RowArgs {
   const int* rows;
   const int* nnum;
   const int* nbr;
-  const double4* pos;
+  const double4* pos;  // FP64 positions
+  const uint4* qpos;   // fixed-point positions (mixed path)
+  float3 hq;           // L / 2^32 per axis
   double3 hb, L, iL;
   double4* frc;
   int accumulate;  // 0: this pass writes the first contribution of the step
   double* e_row;
 };
+
+template <class R> __device__ __forceinline__ Geo<R> geo_of(const RowArgs& a);
+template <> __device__ __forceinline__ Geo<double> geo_of<double>(const RowArgs& a) { return Geo<double>{a.pos, a.L, a.iL}; }
+template <> __device__ __forceinline__ Geo<float> geo_of<float>(const RowArgs& a) {
+  return Geo<float>{a.qpos, a.hq.x, a.hq.y, a.hq.z};
+}
+
+// store v in the w slot of atom i's position record (pos4 or qpos)
+template <class R> __device__ __forceinline__ void store_w(const RowArgs& a, int i, double v);
+template <> __device__ __forceinline__ void store_w<double>(const RowArgs& a, int i, double v) {
+  const_cast<double4*>(a.pos)[i].w = v;
+}
+template <> __device__ __forceinline__ void store_w<float>(const RowArgs& a, int i, double v) {
+  const_cast<uint4*>(a.qpos)[i].w = __float_as_uint((float)v);
+}
 
 #define ROW_PROLOGUE(T)                                   \
   int lb = xcd_block(nb_real);                            \
@@ -36,7 +55,8 @@ struct RowArgs {
   int row = gt / (T), t = threadIdx.x & ((T)-1);          \
   if (row >= a.nrows) return;                             \
   int i = a.rows[row];                                    \
-  double4 pi = a.pos[i];                                  \
+  const Geo<R> G = geo_of<R>(a);                          \
+  const typename Geo<R>::P pi = G.at(i);                  \
   int n = a.nnum[row];                                    \
   const int* nbrow = a.nbr + (size_t)row * a.stride;
 
@@ -52,85 +72,62 @@ struct RjlP {
   double A0, xi, p, q, r0, R1, R2, invw;
 };
 
-// Software-pipelined walk over a row's entries p = t, t+T, ...: the neighbour
-// index is loaded two entries ahead and its position one entry ahead, so each
-// lane keeps independent loads in flight while it computes.
-#define PIPE_BEGIN(T, EXTRA_T, EXTRA_LOAD)                          \
-  int jn1 = t + (T) < n ? nbrow[t + (T)] : i;                       \
-  int jc = t < n ? nbrow[t] : i;                                    \
-  double4 qc = a.pos[jc];                                           \
-  EXTRA_T xc = EXTRA_LOAD(jc);                                      \
-  for (int p = t; p < n; p += (T)) {                                \
-    int jn2 = p + 2 * (T) < n ? nbrow[p + 2 * (T)] : i;             \
-    double4 qn = a.pos[jn1];                                        \
-    EXTRA_T xn = EXTRA_LOAD(jn1);
-#define PIPE_END \
-    jc = jn1; jn1 = jn2; qc = qn; xc = xn;                          \
-  }
+// largest workgroup of the row passes: FP32 pair arithmetic fits 128 VGPRs
+// (16 waves per CU), FP64 needs up to 256
+template <class R> struct PairBlock { static constexpr int max = sizeof(R) == 4 ? 1024 : 256; };
+#define PAIR_BLOCK (PairBlock<R>::max)
 
-// density pass: rho_i = sum_{r <= R2} exp(-2q(r/r0-1)) fc(r)
+// density pass: rho_i = sum_{r <= R2} exp(-2q(r/r0-1)) fc(r); stores
+// Eb_i = sqrt(rho_i) and 1/Eb_i in the w slot of atom i's position record
 template <int T, class R>
-__global__ void __launch_bounds__(256) k_rjl_density(RowArgs a, RjlP P, double* __restrict__ eb,
-                                                     double* __restrict__ inv_eb, float* __restrict__ inv_ebf,
-                                                     int nb_real) {
+__global__ void __launch_bounds__(PAIR_BLOCK) k_rjl_density(RowArgs a, RjlP P, double* __restrict__ eb, int nb_real) {
   ROW_PROLOGUE(T)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
   const R kq = PM<R>::kexp(-2.0 * P.q);
   double rho = 0.0;
-#define NOLOAD(j) 0
-  PIPE_BEGIN(T, int, NOLOAD)
-    (void)xc;
-    double dx, dy, dz;
-    double d2 = fast_dist2(pi, qc, a.L, a.iL, dx, dy, dz);
-    R r = PM<R>::sqrt_((R)d2);
+  for (int p = t; p < n; p += T) {
+    R dx, dy, dz;
+    R r = PM<R>::sqrt_(G.d2(pi, G.at(nbrow[p]), dx, dy, dz));
     R fc, dfr;
     cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
     R e = PM<R>::expk(kq * (r * ir0 - R(1))) * fc;
     rho += (double)(r <= R2 ? e : R(0));
-  PIPE_END
-#undef NOLOAD
+  }
   rho = group_sum<T>(rho);
   if (t == 0) {
     double s = sqrt(rho);
     eb[i] = s;
-    inv_eb[i] = 1.0 / s;
-    inv_ebf[i] = (float)(1.0 / s);
+    store_w<R>(a, i, 1.0 / s);
   }
 }
 
-template <class R> __device__ __forceinline__ R load_ieb(const double* d, const float* f, int j);
-template <> __device__ __forceinline__ double load_ieb<double>(const double* d, const float* f, int j) { return d[j]; }
-template <> __device__ __forceinline__ float load_ieb<float>(const double* d, const float* f, int j) { return f[j]; }
-
 // embedding-force pass (RosatoGuillopeLegrand.f90:80-93):
 // F_i -= sum_j [2 A0 (p/r0 fc - df r) e_p - xi (q/r0 fc - df r/2)(1/Eb_i + 1/Eb_j) e_q]/r dr
+// with 1/Eb read from the w slot of the gathered position records
 template <int T, class R, bool E>
-__global__ void __launch_bounds__(256) k_rjl_force(RowArgs a, RjlP P, const double* __restrict__ eb,
-                                                   const double* __restrict__ inv_eb,
-                                                   const float* __restrict__ inv_ebf, int nb_real) {
+__global__ void __launch_bounds__(PAIR_BLOCK) k_rjl_force(RowArgs a, RjlP P, const double* __restrict__ eb,
+                                                          int nb_real) {
   ROW_PROLOGUE(T)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
   const R kp = PM<R>::kexp(-P.p), kq = PM<R>::kexp(-2.0 * P.q);
   const R c2A = 2.0 * P.A0, pr0 = P.p / P.r0, qr0 = P.q / P.r0, xi = P.xi;
-  const R ie = (R)inv_eb[i];
+  const R ie = Geo<R>::w(pi);
   double fx = 0, fy = 0, fz = 0, er = 0;
-#define IEB(j) load_ieb<R>(inv_eb, inv_ebf, j)
-  PIPE_BEGIN(T, R, IEB)
-    double dx, dy, dz;
-    double d2 = fast_dist2(pi, qc, a.L, a.iL, dx, dy, dz);
-    R r = PM<R>::sqrt_((R)d2);
+  for (int p = t; p < n; p += T) {
+    const typename Geo<R>::P q = G.at(nbrow[p]);
+    R dx, dy, dz;
+    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
     R ir = PM<R>::rcp(r);
     R fc, dfr;
     cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
     R x = r * ir0 - R(1);
     R ep = PM<R>::expk(kp * x), eq = PM<R>::expk(kq * x);
-    R c = (c2A * (pr0 * fc - dfr) * ep - xi * (qr0 * fc - R(0.5) * dfr) * (ie + xc) * eq) * ir;
+    R c = (c2A * (pr0 * fc - dfr) * ep - xi * (qr0 * fc - R(0.5) * dfr) * (ie + Geo<R>::w(q)) * eq) * ir;
     bool in = r <= R2;
-    double cd = in ? (double)c : 0.0;
-    fx -= cd * dx; fy -= cd * dy; fz -= cd * dz;
+    c = in ? c : R(0);
+    fx -= (double)(c * dx); fy -= (double)(c * dy); fz -= (double)(c * dz);
     if (E) er += in ? (double)(ep * fc) : 0.0;
-  PIPE_END
-#undef IEB
+  }
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (E) er = group_sum<T>(er);
   if (t == 0) {
@@ -146,23 +143,21 @@ struct LjP {
 };
 
 template <int T, class R, bool E>
-__global__ void __launch_bounds__(256) k_lj(RowArgs a, LjP P, int nb_real) {
+__global__ void __launch_bounds__(PAIR_BLOCK) k_lj(RowArgs a, LjP P, int nb_real) {
   ROW_PROLOGUE(T)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, sig = P.sig, e4 = P.eps4, kdf = -0.5 * PFM_PI * P.invw;
   double fx = 0, fy = 0, fz = 0, e = 0;
   for (int p = t; p < n; p += T) {
-    int j = nbrow[p];
-    double dx, dy, dz;
-    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
-    R r = PM<R>::sqrt_((R)d2);
+    R dx, dy, dz;
+    R r = PM<R>::sqrt_(G.d2(pi, G.at(nbrow[p]), dx, dy, dz));
     R ir = PM<R>::rcp(r);
     R fc, dfr;
     cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
     R s = sig * ir, s2 = s * s, V = s2 * s2 * s2;
     R c = e4 * (V * (R(12) * V - R(6)) * fc * ir - V * (V - R(1)) * dfr) * ir;
     bool in = r < R2;
-    double cd = in ? (double)c : 0.0;
-    fx -= cd * dx; fy -= cd * dy; fz -= cd * dz;
+    c = in ? c : R(0);
+    fx -= (double)(c * dx); fy -= (double)(c * dy); fz -= (double)(c * dz);
     if (E) e += in ? (double)(e4 * V * (V - R(1)) * fc) : 0.0;
   }
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
@@ -183,7 +178,7 @@ struct Lj1gP {
 };
 
 template <int T, class R, bool E>
-__global__ void __launch_bounds__(256) k_lj1g(RowArgs a, Lj1gP P, const int* __restrict__ orig,
+__global__ void __launch_bounds__(PAIR_BLOCK) k_lj1g(RowArgs a, Lj1gP P, const int* __restrict__ orig,
                                               const int* __restrict__ loc, const int* __restrict__ cut, int nb_real) {
   ROW_PROLOGUE(T)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, c6 = P.c6, c12 = P.c12, c6t6 = P.c6t6, c12t12 = P.c12t12;
@@ -195,9 +190,8 @@ __global__ void __launch_bounds__(256) k_lj1g(RowArgs a, Lj1gP P, const int* __r
     int wh = lj < ci ? 1 : 0;
     int w = wh + (li < cut[j] ? 1 : 0);
     if (w == 0) continue;
-    double dx, dy, dz;
-    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
-    R r = PM<R>::sqrt_((R)d2);
+    R dx, dy, dz;
+    R r = PM<R>::sqrt_(G.d2(pi, G.at(j), dx, dy, dz));
     // f_dfr_cut (cut_off_poly.f90:30-43), dfr without the 1/(R2-R1) factor
     R tr = r < R1 ? R1 : (r > R2 ? R2 : r);
     R x = (tr - R1) * invw, x2 = x * x;
@@ -205,8 +199,8 @@ __global__ void __launch_bounds__(256) k_lj1g(RowArgs a, Lj1gP P, const int* __r
     R dfr = tr * x2 * (R(-30) + R(60) * x - R(30) * x2);
     R invr2 = PM<R>::rcp(r * r), U = invr2 * invr2 * invr2;
     R Fs = U * invr2 * ((c12t12 * U - c6t6) * f - (c12 * U - c6) * dfr);
-    double cd = (double)Fs * (double)w;
-    fx -= cd * dx; fy -= cd * dy; fz -= cd * dz;
+    R cw = Fs * R(w);
+    fx -= (double)(cw * dx); fy -= (double)(cw * dy); fz -= (double)(cw * dz);
     if (E && wh) {
       // cut_off_poly f_cut (cut_off_poly.f90:6-17)
       R fe = (r > R1 && r < R2) ? f : (r <= R1 ? R(1) : R(0));
@@ -238,10 +232,7 @@ __global__ void __launch_bounds__(256) k_tb_bond(RowArgs a, TbP P, double* __res
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, a0 = P.a0, c02 = P.c02, d02 = P.d02;
   R dxs[PFM_TB_MAXN], dys[PFM_TB_MAXN], dzs[PFM_TB_MAXN], rs[PFM_TB_MAXN], fcs[PFM_TB_MAXN];
   for (int p = 0; p < n; ++p) {
-    double dx, dy, dz;
-    double d2 = fast_dist2(pi, a.pos[nbrow[p]], a.L, a.iL, dx, dy, dz);
-    dxs[p] = (R)dx; dys[p] = (R)dy; dzs[p] = (R)dz;
-    rs[p] = PM<R>::sqrt_((R)d2);
+    rs[p] = PM<R>::sqrt_(G.d2(pi, G.at(nbrow[p]), dxs[p], dys[p], dzs[p]));
     R df;
     cos_cutoff<R>(rs[p], R1, R2, invw, fcs[p], df);
   }
@@ -280,10 +271,7 @@ __global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* _
   const R s2S = R(sqrt(2.0 * P.S) * P.beta), s2oS = R(sqrt(2.0 / P.S) * P.beta), hdel = R(-P.delt * 0.5);
   R dxs[PFM_TB_MAXN], dys[PFM_TB_MAXN], dzs[PFM_TB_MAXN], rs[PFM_TB_MAXN], fcs[PFM_TB_MAXN], dfcs[PFM_TB_MAXN];
   for (int p = 0; p < n; ++p) {
-    double dx, dy, dz;
-    double d2 = fast_dist2(pi, a.pos[nbrow[p]], a.L, a.iL, dx, dy, dz);
-    dxs[p] = (R)dx; dys[p] = (R)dy; dzs[p] = (R)dz;
-    rs[p] = PM<R>::sqrt_((R)d2);
+    rs[p] = PM<R>::sqrt_(G.d2(pi, G.at(nbrow[p]), dxs[p], dys[p], dzs[p]));
     cos_cutoff<R>(rs[p], R1, R2, invw, fcs[p], dfcs[p]);
   }
   double fx = 0, fy = 0, fz = 0, e = 0;
@@ -317,14 +305,12 @@ __global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* _
     // vector from j to i and its cutoff
     const R xl = -xp, yl = -yp, zl = -zp, rl = rp, irl2 = irp2;
     const R fcl = fcs[p], dfcl = dfcs[p];
-    double4 pj = a.pos[j];
+    const typename Geo<R>::P pj = G.at(j);
     R tx = 0, ty = 0, tz = 0;
     for (int q = 0; q < nj; ++q) {
       if (q == l) continue;
-      int k = a.nbr[jbase + q];
-      double dx, dy, dz;
-      double d2 = fast_dist2(pj, a.pos[k], a.L, a.iL, dx, dy, dz);
-      const R xq = (R)dx, yq = (R)dy, zq = (R)dz, rq = PM<R>::sqrt_((R)d2);
+      R xq, yq, zq;
+      const R rq = PM<R>::sqrt_(G.d2(pj, G.at(a.nbr[jbase + q]), xq, yq, zq));
       R fq, dfq;
       cos_cutoff<R>(rq, R1, R2, invw, fq, dfq);
       R rr = PM<R>::rcp(rl * rq);
@@ -371,27 +357,6 @@ struct CosP {
   int morse;
 };
 
-// graphene normal from the three nearest neighbours (graphenenorm.f90:38-56)
-__global__ void k_gnorm(int nrows, const int* __restrict__ rows, const int* __restrict__ nn, const double4* pos,
-                        double3 hb, double4* __restrict__ gnorm, int simplified) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  int i = rows[r];
-  if (simplified) {
-    gnorm[i] = make_double4(0, 0, 1, 0);
-    return;
-  }
-  double4 pi = pos[i];
-  double d[3][3];
-  for (int k = 0; k < 3; ++k) min_image_dist2(pi, pos[nn[(size_t)r * 3 + k]], hb, d[k][0], d[k][1], d[k][2]);
-  double ax = d[1][0] - d[0][0], ay = d[1][1] - d[0][1], az = d[1][2] - d[0][2];
-  double bx = d[0][0] - d[2][0], by = d[0][1] - d[2][1], bz = d[0][2] - d[2][2];
-  double gx = ay * bz - az * by, gy = az * bx - ax * bz, gz = ax * by - ay * bx;
-  if (gz < 0.0) { gx = -gx; gy = -gy; gz = -gz; }
-  double s = sqrt(gx * gx + gy * gy + gz * gz);
-  gnorm[i] = make_double4(gx / s, gy / s, gz / s, 0);
-}
-
 template <class R>
 __device__ __forceinline__ void cos_pair(const CosP& P, R r, R nd, R nnorm_r, R& A, R& Bc, R& S_coef, R& epair) {
   const R R1 = P.R1, invw = P.invw, delt = P.delt, kdf = -0.5 * PFM_PI * P.invw;
@@ -425,39 +390,38 @@ __device__ __forceinline__ void cos_pair(const CosP& P, R r, R nd, R nnorm_r, R&
 // nearest neighbours (rows of nl(3) are the rows of nl(1)) and stored for the
 // normal-derivative and metal passes.
 template <int T, class R, bool E>
-__global__ void __launch_bounds__(256) k_cos_graphene(RowArgs a, CosP P, const int* __restrict__ nn, int simplified,
-                                                      double4* __restrict__ gnorm, double4* __restrict__ gS,
-                                                      int nb_real) {
+__global__ void __launch_bounds__(PAIR_BLOCK) k_cos_graphene(RowArgs a, CosP P, const int* __restrict__ nn,
+                                                             int simplified, double4* __restrict__ gnorm,
+                                                             double4* __restrict__ gS, int nb_real) {
   ROW_PROLOGUE(T)
   double4 g;
   if (simplified) {
     g = make_double4(0, 0, 1, 0);
   } else {
-    double d[3][3];
-    for (int k = 0; k < 3; ++k)
-      fast_dist2(pi, a.pos[nn[(size_t)row * 3 + k]], a.L, a.iL, d[k][0], d[k][1], d[k][2]);
-    double ax = d[1][0] - d[0][0], ay = d[1][1] - d[0][1], az = d[1][2] - d[0][2];
-    double bx = d[0][0] - d[2][0], by = d[0][1] - d[2][1], bz = d[0][2] - d[2][2];
-    double gx = ay * bz - az * by, gy = az * bx - ax * bz, gz = ax * by - ay * bx;
-    if (gz < 0.0) { gx = -gx; gy = -gy; gz = -gz; }
-    double inv = 1.0 / sqrt(gx * gx + gy * gy + gz * gz);
+    R d[3][3];
+    for (int k = 0; k < 3; ++k) G.d2(pi, G.at(nn[(size_t)row * 3 + k]), d[k][0], d[k][1], d[k][2]);
+    R ax = d[1][0] - d[0][0], ay = d[1][1] - d[0][1], az = d[1][2] - d[0][2];
+    R bx = d[0][0] - d[2][0], by = d[0][1] - d[2][1], bz = d[0][2] - d[2][2];
+    R gx = ay * bz - az * by, gy = az * bx - ax * bz, gz = ax * by - ay * bx;
+    if (gz < R(0)) { gx = -gx; gy = -gy; gz = -gz; }
+    R inv = R(1) / PM<R>::sqrt_(gx * gx + gy * gy + gz * gz);
     g = make_double4(gx * inv, gy * inv, gz * inv, 0);
   }
-  const R nx = g.x, ny = g.y, nz = g.z, R2 = P.R2;
+  const R nx = g.x, ny = g.y, nz = g.z, R2 = P.R2, pref = P.pref;
   double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, e = 0;
   for (int p = t; p < n; p += T) {
-    int j = nbrow[p];
-    double dx, dy, dz;
-    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
-    R r = PM<R>::sqrt_((R)d2);
-    R nd = nx * (R)dx + ny * (R)dy + nz * (R)dz;
+    R dx, dy, dz;
+    R r = PM<R>::sqrt_(G.d2(pi, G.at(nbrow[p]), dx, dy, dz));
+    R nd = nx * dx + ny * dy + nz * dz;
     R A, Bc, Sc, ep;
     cos_pair<R>(P, r, nd, PM<R>::rcp(r), A, Bc, Sc, ep);
     bool in = r < R2;
-    double Ad = in ? (double)A * P.pref : 0.0, Bd = in ? (double)Bc * P.pref : 0.0, Sd = in ? (double)Sc : 0.0;
-    fx -= Ad * dx + Bd * g.x; fy -= Ad * dy + Bd * g.y; fz -= Ad * dz + Bd * g.z;
-    sx += Sd * dx; sy += Sd * dy; sz += Sd * dz;
-    if (E) e += in ? (double)ep * P.pref : 0.0;
+    A = in ? A * pref : R(0);
+    Bc = in ? Bc * pref : R(0);
+    Sc = in ? Sc : R(0);
+    fx -= (double)(A * dx + Bc * nx); fy -= (double)(A * dy + Bc * ny); fz -= (double)(A * dz + Bc * nz);
+    sx += (double)(Sc * dx); sy += (double)(Sc * dy); sz += (double)(Sc * dz);
+    if (E) e += in ? (double)(ep * pref) : 0.0;
   }
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   sx = group_sum<T>(sx); sy = group_sum<T>(sy); sz = group_sum<T>(sz);
@@ -477,8 +441,9 @@ __global__ void __launch_bounds__(256) k_cos_graphene(RowArgs a, CosP P, const i
 __global__ void __launch_bounds__(256) k_cos_nn(RowArgs a, CosP P, const int* __restrict__ row_of,
                                                 const int* __restrict__ rev, const double4* __restrict__ gnorm,
                                                 const double4* __restrict__ gS, int nb_real) {
+  using R = double;  // FP64 in both precisions: three entries per graphene atom
   ROW_PROLOGUE(1)
-  (void)t; (void)pi;
+  (void)t; (void)pi; (void)G;
   double fx = 0, fy = 0, fz = 0;
   for (int q = 0; q < 3 && n == 3; ++q) {
     int j = nbrow[q];
@@ -514,24 +479,25 @@ __global__ void __launch_bounds__(256) k_cos_nn(RowArgs a, CosP P, const int* __
 // metal side (*_forces_for_other_atoms): rows of nl(2), neighbours are
 // graphene atoms whose normal enters V3 (morsec divides by |n|)
 template <int T, class R>
-__global__ void __launch_bounds__(256) k_cos_metal(RowArgs a, CosP P, const double4* __restrict__ gnorm,
-                                                   int nb_real) {
+__global__ void __launch_bounds__(PAIR_BLOCK) k_cos_metal(RowArgs a, CosP P, const double4* __restrict__ gnorm,
+                                                          int nb_real) {
   ROW_PROLOGUE(T)
-  const R R2 = P.R2;
+  const R R2 = P.R2, pref = P.pref;
   double fx = 0, fy = 0, fz = 0;
   for (int p = t; p < n; p += T) {
     int j = nbrow[p];
-    double dx, dy, dz;
-    double d2 = fast_dist2(pi, a.pos[j], a.L, a.iL, dx, dy, dz);
-    R r = PM<R>::sqrt_((R)d2);
+    R dx, dy, dz;
+    R r = PM<R>::sqrt_(G.d2(pi, G.at(j), dx, dy, dz));
     double4 g = gnorm[j];
-    R nd = (R)g.x * (R)dx + (R)g.y * (R)dy + (R)g.z * (R)dz;
-    R nn = P.morse ? PM<R>::sqrt_((R)(g.x * g.x + g.y * g.y + g.z * g.z)) * r : r;
+    const R gx = g.x, gy = g.y, gz = g.z;
+    R nd = gx * dx + gy * dy + gz * dz;
+    R nn = P.morse ? PM<R>::sqrt_(gx * gx + gy * gy + gz * gz) * r : r;
     R A, Bc, Sc, ep;
     cos_pair<R>(P, r, nd, PM<R>::rcp(nn), A, Bc, Sc, ep);
     bool in = r < R2;
-    double Ad = in ? (double)A * P.pref : 0.0, Bd = in ? (double)Bc * P.pref : 0.0;
-    fx -= Ad * dx + Bd * g.x; fy -= Ad * dy + Bd * g.y; fz -= Ad * dz + Bd * g.z;
+    A = in ? A * pref : R(0);
+    Bc = in ? Bc * pref : R(0);
+    fx -= (double)(A * dx + Bc * gx); fy -= (double)(A * dy + Bc * gy); fz -= (double)(A * dz + Bc * gz);
   }
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (t == 0) write_force(a, i, fx, fy, fz);
@@ -548,6 +514,9 @@ static RowArgs row_args(pfm_ctx* h, Inter& it, int k, int accumulate) {
   a.nnum = L.nnum;
   a.nbr = L.nbr;
   a.pos = h->pos4;
+  a.qpos = h->qpos;
+  a.hq = make_float3((float)(h->box[0] / 4294967296.0), (float)(h->box[1] / 4294967296.0),
+                     (float)(h->box[2] / 4294967296.0));
   a.hb = make_double3(h->hbox[0], h->hbox[1], h->hbox[2]);
   a.L = make_double3(h->box[0], h->box[1], h->box[2]);
   a.iL = make_double3(1.0 / h->box[0], 1.0 / h->box[1], 1.0 / h->box[2]);
@@ -572,7 +541,8 @@ static int row_blocks(int nrows, int T, int block) { return (int)(((long)nrows *
 template <class R>
 static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
   bool written = !first_writes;  // has any pass written this step's forces yet
-  const int BS = 256;
+  const int BS = std::min(h->block, PairBlock<R>::max);  // row passes (T lanes per row)
+  const int BS1 = 256;           // one-lane-per-row passes (tb, cos_nn)
   for (size_t ii = 0; ii < h->inter.size(); ++ii) {
     Inter& it = h->inter[ii];
     switch (it.kind) {
@@ -583,9 +553,9 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         int T = h->lanes;
         DISPATCH_T(T, {
           int nb = row_blocks(a.nrows, T, BS);
-          PFM_LAUNCH(h, "rjl_density", (k_rjl_density<T, R>), pad8(nb), BS, a, P, h->eb, h->inv_eb, h->inv_ebf, nb);
-          if (energy) PFM_LAUNCH(h, "rjl_force", (k_rjl_force<T, R, true>), pad8(nb), BS, a, P, h->eb, h->inv_eb, h->inv_ebf, nb);
-          else PFM_LAUNCH(h, "rjl_force", (k_rjl_force<T, R, false>), pad8(nb), BS, a, P, h->eb, h->inv_eb, h->inv_ebf, nb);
+          PFM_LAUNCH(h, "rjl_density", (k_rjl_density<T, R>), pad8(nb), BS, a, P, h->eb, nb);
+          if (energy) PFM_LAUNCH(h, "rjl_force", (k_rjl_force<T, R, true>), pad8(nb), BS, a, P, h->eb, nb);
+          else PFM_LAUNCH(h, "rjl_force", (k_rjl_force<T, R, false>), pad8(nb), BS, a, P, h->eb, nb);
         });
         written = true;
         break;
@@ -627,12 +597,12 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         if (a.nrows == 0) break;
         TbP P{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6] * it.p[6], it.p[7] * it.p[7],
               it.p[8], it.p[9], 1.0 / (it.p[9] - it.p[8])};
-        int nb = row_blocks(a.nrows, 1, BS);
+        int nb = row_blocks(a.nrows, 1, BS1);
         double* BT = it.Bp + (size_t)it.nl[0].nrows * it.nl[0].stride;
-        PFM_LAUNCH(h, "tb_bond", (k_tb_bond<R>), pad8(nb), BS, a, P, it.B, it.Bp, BT, nb);
+        PFM_LAUNCH(h, "tb_bond", (k_tb_bond<R>), pad8(nb), BS1, a, P, it.B, it.Bp, BT, nb);
         const int* row_of = h->grp.row_of[it.nl[0].g1];
-        if (energy) PFM_LAUNCH(h, "tb_force", (k_tb_force<R, true>), pad8(nb), BS, a, P, row_of, it.nl[0].rev, it.B, it.Bp, BT, nb);
-        else PFM_LAUNCH(h, "tb_force", (k_tb_force<R, false>), pad8(nb), BS, a, P, row_of, it.nl[0].rev, it.B, it.Bp, BT, nb);
+        if (energy) PFM_LAUNCH(h, "tb_force", (k_tb_force<R, true>), pad8(nb), BS1, a, P, row_of, it.nl[0].rev, it.B, it.Bp, BT, nb);
+        else PFM_LAUNCH(h, "tb_force", (k_tb_force<R, false>), pad8(nb), BS1, a, P, row_of, it.nl[0].rev, it.B, it.Bp, BT, nb);
         written = true;
         break;
       }
@@ -662,8 +632,8 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         }
         if (!simplified && NN.nrows > 0) {
           RowArgs b = row_args(h, it, 2, written ? 1 : 0);
-          int nb = row_blocks(b.nrows, 1, BS);
-          PFM_LAUNCH(h, "cos_nn", k_cos_nn, pad8(nb), BS, b, P, h->grp.row_of[NN.g1], NN.rev, it.gnorm, it.gS, nb);
+          int nb = row_blocks(b.nrows, 1, BS1);
+          PFM_LAUNCH(h, "cos_nn", k_cos_nn, pad8(nb), BS1, b, P, h->grp.row_of[NN.g1], NN.rev, it.gnorm, it.gS, nb);
           written = true;
         }
         RowArgs c = row_args(h, it, 1, written ? 1 : 0);

@@ -54,9 +54,51 @@ __device__ __forceinline__ double fast_dist2(const double4& a, const double4& b,
   return fma(x, x, fma(y, y, z * z));
 }
 
+// Fixed-point positions of the mixed-precision pair passes: per axis
+// u = round(x * 2^32 / L) mod 2^32.  The wrapped int32 difference of two
+// atoms' u is their minimum-image separation in units of L / 2^32
+// (2.7e-8 A for L = 115 A), exact up to the two quantisation errors, and is
+// rounded once to FP32 when scaled back.  Atoms exactly half a box apart
+// (never within a cutoff) take the -L/2 image.
+__device__ __forceinline__ uint32_t fix32(double x, double s) {
+  return (uint32_t)(unsigned long long)__double2ll_rn(x * s);
+}
+__device__ __forceinline__ uint4 quantize(const double4& p, const double3& s, uint32_t w) {
+  return make_uint4(fix32(p.x, s.x), fix32(p.y, s.y), fix32(p.z, s.z), w);
+}
+
+// Positions as a pair pass reads them.  Geo<double>: pos4 and the rint
+// minimum image in FP64.  Geo<float>: qpos, dr from the fixed-point
+// difference.  w() is the per-atom value the previous pass stored beside the
+// coordinates.
+template <class R> struct Geo;
+template <> struct Geo<double> {
+  using P = double4;
+  const double4* p;
+  double3 L, iL;
+  __device__ __forceinline__ P at(int j) const { return p[j]; }
+  __device__ __forceinline__ double d2(const P& a, const P& b, double& dx, double& dy, double& dz) const {
+    return fast_dist2(a, b, L, iL, dx, dy, dz);
+  }
+  static __device__ __forceinline__ double w(const P& a) { return a.w; }
+};
+template <> struct Geo<float> {
+  using P = uint4;
+  const uint4* p;
+  float hx, hy, hz;  // L / 2^32 per axis
+  __device__ __forceinline__ P at(int j) const { return p[j]; }
+  __device__ __forceinline__ float d2(const P& a, const P& b, float& dx, float& dy, float& dz) const {
+    dx = (float)(int)(b.x - a.x) * hx;
+    dy = (float)(int)(b.y - a.y) * hy;
+    dz = (float)(int)(b.z - a.z) * hz;
+    return dx * dx + dy * dy + dz * dz;
+  }
+  static __device__ __forceinline__ float w(const P& a) { return __uint_as_float(a.w); }
+};
+
 // Precision policy of the pair arithmetic.  R = double: every operation in
 // FP64 with the library's correctly rounded elementary functions.  R = float:
-// dr is formed in FP64 and rounded once, the pair terms use the gfx950
+// dr comes from the fixed-point positions (Geo<float>), the pair terms use the gfx950
 // transcendental unit (v_exp_f32, v_sin/cos_f32, v_sqrt_f32, v_rcp_f32).
 template <class R> struct PM;
 template <> struct PM<double> {
@@ -223,8 +265,10 @@ struct pfm_ctx {
   int* cell_xyz = nullptr;  // by Morton rank: x | y<<10 | z<<20
   int* scan_tmp = nullptr;
   // RGL per-atom densities
-  double *eb = nullptr, *inv_eb = nullptr;
-  float* inv_ebf = nullptr;  // FP32 copy of inv_eb for the mixed path
+  double* eb = nullptr;
+  // fixed-point positions of the mixed path (null in FP64) and their scale
+  uint4* qpos = nullptr;
+  double3 qscale;  // 2^32 / L per axis
   // reductions
   double* red_part = nullptr;  // partial sums
   double* red_out = nullptr;   // device scalars
@@ -237,7 +281,7 @@ struct pfm_ctx {
   std::vector<double> e_nhc;
   bool sorted_once = false;
   // tuning
-  int lanes = 8, block = 256;
+  int lanes = 8, block = 256;  // lanes per list row and workgroup size of the pair passes
   // timing
   bool timing = false;
   std::vector<KTimer> timers;

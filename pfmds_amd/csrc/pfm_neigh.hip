@@ -157,11 +157,13 @@ __global__ void k_permute(int N, const int* __restrict__ perm, const double4* __
                           const double4* __restrict__ v0, const double4* __restrict__ f0,
                           const uint32_t* __restrict__ m0, const int* __restrict__ o0, double4* __restrict__ p1,
                           double4* __restrict__ v1, double4* __restrict__ f1, uint32_t* __restrict__ m1,
-                          int* __restrict__ o1) {
+                          int* __restrict__ o1, uint4* __restrict__ q1, double3 qs) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= N) return;
   int i = perm[k];
-  p1[k] = p0[i];
+  double4 p = p0[i];
+  p1[k] = p;
+  if (q1) q1[k] = quantize(p, qs, 0u);
   v1[k] = v0[i];
   f1[k] = f0[i];
   m1[k] = m0[i];
@@ -189,7 +191,7 @@ int launch_sort_and_bin(pfm_ctx* h, int step) {
   PFM_LAUNCH(h, "cell_sort", k_cell_sort, h->ncell, 64, h->ncell, h->cell_start, h->cell_count, h->scan_tmp + N,
              h->perm, h->orig);
   PFM_LAUNCH(h, "permute", k_permute, nb, 256, N, h->perm, h->pos4, h->vel4, h->frc4, h->mask, h->orig, h->pos4b,
-             h->vel4b, h->frc4b, h->maskb, h->origb);
+             h->vel4b, h->frc4b, h->maskb, h->origb, h->qpos, h->qscale);
   std::swap(h->pos4, h->pos4b);
   std::swap(h->vel4, h->vel4b);
   std::swap(h->frc4, h->frc4b);
@@ -403,7 +405,7 @@ __global__ void k_nn_from_tb(int nrows, const int* __restrict__ rows, const int*
 // a relative 3e-5 of r_cut**2, where the reference's FP64 find_distance test
 // (min_image_dist2) on the global positions decides.  Candidates are visited
 // in the same order as k_build, so rows come out identical.
-#define TILE_MAX 3072
+#define TILE_MAX 1536  // 24 KB of LDS: six workgroups per CU
 
 template <int T>
 __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __restrict__ rank_xyz,
@@ -445,20 +447,23 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
   const int R = c_off[27];
   const bool staged = R <= TILE_MAX;
   if (staged) {
-    for (int k = 0; k < 27; ++k) {
-      int n = c_cnt[k], s0 = c_start[k], of = c_off[k];
-      int o[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
-      for (int m = tid; m < n; m += 256) {
-        int j = s0 + m;
-        double4 p = a.pos[j];
-        // unwrap: the image of atom j next to the centre cell
-        double x = p.x - org[0], y = p.y - org[1], z = p.z - org[2];
-        if (o[0] < 0 && x > 0.0) x -= a.L.x; else if (o[0] > 0 && x < 0.0) x += a.L.x;
-        if (o[1] < 0 && y > 0.0) y -= a.L.y; else if (o[1] > 0 && y < 0.0) y += a.L.y;
-        if (o[2] < 0 && z > 0.0) z -= a.L.z; else if (o[2] > 0 && z < 0.0) z += a.L.z;
-        int v = j | ((a.mask[j] & a.g2bit) ? 0 : (int)0x80000000);
-        st[of + m] = make_float4((float)x, (float)y, (float)z, __int_as_float(v));
-      }
+    // one pass over all R tile atoms: slot m belongs to the last tile cell k
+    // with c_off[k] <= m (c_off is non-decreasing, c_off[0] = 0)
+    for (int m = tid; m < R; m += 256) {
+      int k = 0;
+#pragma unroll
+      for (int sft = 16; sft > 0; sft >>= 1)
+        if (k + sft < 27 && c_off[k + sft] <= m) k += sft;
+      int j = c_start[k] + (m - c_off[k]);
+      int o0 = k % 3 - 1, o1 = (k / 3) % 3 - 1, o2 = k / 9 - 1;
+      double4 p = a.pos[j];
+      // unwrap: the image of atom j next to the centre cell
+      double x = p.x - org[0], y = p.y - org[1], z = p.z - org[2];
+      if (o0 < 0 && x > 0.0) x -= a.L.x; else if (o0 > 0 && x < 0.0) x += a.L.x;
+      if (o1 < 0 && y > 0.0) y -= a.L.y; else if (o1 > 0 && y < 0.0) y += a.L.y;
+      if (o2 < 0 && z > 0.0) z -= a.L.z; else if (o2 > 0 && z < 0.0) z += a.L.z;
+      int v = j | ((a.mask[j] & a.g2bit) ? 0 : (int)0x80000000);
+      st[m] = make_float4((float)x, (float)y, (float)z, __int_as_float(v));
     }
   }
   __syncthreads();
