@@ -396,32 +396,42 @@ __global__ void k_nn_from_tb(int nrows, const int* __restrict__ rows, const int*
   if (k != 3) raise_err(err, PFM_E_GRAPHENE_NN, orig[i] + 1, k, step);
 }
 
-// LDS-staged cell tile: one workgroup per cell (grids at least three cells
-// wide).  The atoms of the 27 surrounding cells are staged once in LDS as
-// FP32 coordinates unwrapped into the frame of the centre cell's origin, with
-// the internal index (bit 31: not in group 2).  Every group-1 atom of the
-// centre cell is a row scanned by T lanes; tile cells farther than r_cut from
-// the row atom are skipped.  The FP32 distance decides unless it lies within
-// a relative 3e-5 of r_cut**2, where the reference's FP64 find_distance test
-// (min_image_dist2) on the global positions decides.  Candidates are visited
-// in the same order as k_build, so rows come out identical.
-#define TILE_MAX 1536  // 24 KB of LDS: six workgroups per CU
+// Cell-tile list search: one workgroup per cell (grids at least three cells
+// wide).  The atoms of the 27 surrounding cells are staged in LDS as FP32
+// coordinates unwrapped into the frame of the centre cell's origin, with the
+// atom index in w; atoms outside group 2, and the padding after the last
+// atom, are staged far away, so they never pass the distance test.  Every
+// group-1 atom of the centre cell is a row scanned by T lanes; all rows of a
+// wavefront walk the tile in the same order, so their LDS reads are
+// broadcasts.  Each lane takes 32 of its candidates (every T-th slot) into a
+// register bitmask, two instructions per candidate after the distance; the
+// row's lanes then take their output offsets from a prefix sum over the T
+// lanes and write their hits.  The FP32 distance decides unless it lies
+// within a relative 3e-5 of r_cut**2, where the reference's FP64
+// find_distance test (min_image_dist2) on the global positions decides.
+// Tiles larger than the LDS buffer are staged in several passes.
+#define TILE_MAX 1536      // staged candidates per pass (24 KB of LDS), a multiple of 32 * T
+#define TILE_FAR 1.0e30f   // staged coordinate of atoms outside group 2 and of padding
+
+struct TileCells {
+  int start[27], cnt[27], off[28];
+};
 
 template <int T>
 __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __restrict__ rank_xyz,
                                                     const int* __restrict__ row_of, int nblocks_real) {
+  static_assert(TILE_MAX % (32 * T) == 0, "tile passes hold whole bitmask rounds");
   __shared__ float4 st[TILE_MAX];
-  __shared__ int c_off[28], c_start[27], c_cnt[27];
-  __shared__ float c_lo[27][3];
+  __shared__ TileCells tc;
   int c = xcd_block(nblocks_real);
   if (c < 0) return;
-  int ncen = a.cell_count[c];
+  const int ncen = a.cell_count[c];
   if (ncen == 0) return;
-  int cs = a.cell_start[c];
-  int pk = rank_xyz[c];
-  int cc[3] = {pk & 1023, (pk >> 10) & 1023, (pk >> 20) & 1023};
-  double org[3] = {cc[0] * a.clen.x, cc[1] * a.clen.y, cc[2] * a.clen.z};
-  int tid = threadIdx.x;
+  const int cs = a.cell_start[c];
+  const int tid = threadIdx.x;
+  const int pk = rank_xyz[c];
+  const int cc[3] = {pk & 1023, (pk >> 10) & 1023, (pk >> 20) & 1023};
+  const double org[3] = {cc[0] * a.clen.x, cc[1] * a.clen.y, cc[2] * a.clen.z};
   if (tid < 27) {
     int o[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
     int q[3];
@@ -431,104 +441,114 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
       q[d] = q[d] < 0 ? q[d] + nd : (q[d] >= nd ? q[d] - nd : q[d]);
     }
     int r = a.cell_rank[(q[2] * a.g.nc[1] + q[1]) * a.g.nc[0] + q[0]];
-    c_start[tid] = a.cell_start[r];
-    c_cnt[tid] = a.cell_count[r];
-    c_lo[tid][0] = (float)(o[0] * a.clen.x);
-    c_lo[tid][1] = (float)(o[1] * a.clen.y);
-    c_lo[tid][2] = (float)(o[2] * a.clen.z);
+    tc.start[tid] = a.cell_start[r];
+    tc.cnt[tid] = a.cell_count[r];
   }
   __syncthreads();
   if (tid == 0) {
     int off = 0;
-    for (int k = 0; k < 27; ++k) { c_off[k] = off; off += c_cnt[k]; }
-    c_off[27] = off;
+    for (int k = 0; k < 27; ++k) { tc.off[k] = off; off += tc.cnt[k]; }
+    tc.off[27] = off;
   }
   __syncthreads();
-  const int R = c_off[27];
-  const bool staged = R <= TILE_MAX;
-  if (staged) {
-    // one pass over all R tile atoms: slot m belongs to the last tile cell k
-    // with c_off[k] <= m (c_off is non-decreasing, c_off[0] = 0)
-    for (int m = tid; m < R; m += 256) {
-      int k = 0;
+  const int R = tc.off[27];
+  const int Rp = (R + 32 * T - 1) / (32 * T) * (32 * T);  // padded to whole rounds
+  const float hi = (float)(a.rc2 * (1.0 + 3e-5)), lo = (float)(a.rc2 * (1.0 - 3e-5));
+
+  // stage tile slots [m0, m1): slot m < R belongs to the last tile cell k with
+  // off[k] <= m (off is non-decreasing, off[0] = 0); slots >= R are padding
+  auto stage = [&](int m0, int m1) {
+    for (int m = m0 + tid; m < m1; m += 256) {
+      float4 v = make_float4(TILE_FAR, TILE_FAR, TILE_FAR, __int_as_float(-1));
+      if (m < R) {
+        int k = 0;
 #pragma unroll
-      for (int sft = 16; sft > 0; sft >>= 1)
-        if (k + sft < 27 && c_off[k + sft] <= m) k += sft;
-      int j = c_start[k] + (m - c_off[k]);
-      int o0 = k % 3 - 1, o1 = (k / 3) % 3 - 1, o2 = k / 9 - 1;
-      double4 p = a.pos[j];
-      // unwrap: the image of atom j next to the centre cell
-      double x = p.x - org[0], y = p.y - org[1], z = p.z - org[2];
-      if (o0 < 0 && x > 0.0) x -= a.L.x; else if (o0 > 0 && x < 0.0) x += a.L.x;
-      if (o1 < 0 && y > 0.0) y -= a.L.y; else if (o1 > 0 && y < 0.0) y += a.L.y;
-      if (o2 < 0 && z > 0.0) z -= a.L.z; else if (o2 > 0 && z < 0.0) z += a.L.z;
-      int v = j | ((a.mask[j] & a.g2bit) ? 0 : (int)0x80000000);
-      st[m] = make_float4((float)x, (float)y, (float)z, __int_as_float(v));
+        for (int sft = 16; sft > 0; sft >>= 1)
+          if (k + sft < 27 && tc.off[k + sft] <= m) k += sft;
+        int j = tc.start[k] + (m - tc.off[k]);
+        int o0 = k % 3 - 1, o1 = (k / 3) % 3 - 1, o2 = k / 9 - 1;
+        double4 p = a.pos[j];
+        // unwrap: the image of atom j next to the centre cell
+        double x = p.x - org[0], y = p.y - org[1], z = p.z - org[2];
+        if (o0 < 0 && x > 0.0) x -= a.L.x; else if (o0 > 0 && x < 0.0) x += a.L.x;
+        if (o1 < 0 && y > 0.0) y -= a.L.y; else if (o1 > 0 && y < 0.0) y += a.L.y;
+        if (o2 < 0 && z > 0.0) z -= a.L.z; else if (o2 > 0 && z < 0.0) z += a.L.z;
+        v.w = __int_as_float(j);
+        if (a.mask[j] & a.g2bit) { v.x = (float)x; v.y = (float)y; v.z = (float)z; }
+      }
+      st[m - m0] = v;
     }
-  }
-  __syncthreads();
-  const float rc2_lo = (float)(a.rc2 * (1.0 - 3e-5)), rc2_hi = (float)(a.rc2 * (1.0 + 3e-5));
-  const float clx = (float)a.clen.x, cly = (float)a.clen.y, clz = (float)a.clen.z;
-  int lane = tid & 63;
-  int gshift = lane & ~(T - 1);
-  unsigned long long gmask = (T == 64) ? ~0ull : (((1ull << T) - 1ull) << gshift);
-  unsigned long long lower = (1ull << lane) - 1ull;
-  int t = tid & (T - 1);
+  };
+  const bool single = Rp <= TILE_MAX;
+  if (single) stage(0, Rp);
+
+  const int t = tid & (T - 1);
   for (int q0 = 0; q0 < ncen; q0 += 256 / T) {
-    int qq = q0 + tid / T;
-    if (qq >= ncen) break;
-    int i = cs + qq;
-    int row = row_of[i];
-    if (row < 0) continue;
-    double4 pi = a.pos[i];
-    float px = (float)(pi.x - org[0]), py = (float)(pi.y - org[1]), pz = (float)(pi.z - org[2]);
+    const int qq = q0 + tid / T;
+    const bool inrow = qq < ncen;
+    const int i = cs + (inrow ? qq : 0);
+    const int row = inrow ? row_of[i] : -1;
+    const bool active = row >= 0;
+    const double4 pi = a.pos[i];
+    const float px = (float)(pi.x - org[0]), py = (float)(pi.y - org[1]), pz = (float)(pi.z - org[2]);
+    const int own = tc.off[13] + qq;  // the row atom's own tile slot
+    int* out = a.nbr + (size_t)(active ? row : 0) * a.stride;
     int count = 0;
-    int* out = a.nbr + (size_t)row * a.stride;
-    for (int k = 0; k < 27; ++k) {
-      const int n = c_cnt[k];
-      if (n == 0) continue;
-      const float lx = c_lo[k][0], ly = c_lo[k][1], lz = c_lo[k][2];
-      // distance from the row atom to tile cell k (its box in the unwrapped frame)
-      float bx = fmaxf(0.0f, fmaxf(lx - px, px - (lx + clx)));
-      float by = fmaxf(0.0f, fmaxf(ly - py, py - (ly + cly)));
-      float bz = fmaxf(0.0f, fmaxf(lz - pz, pz - (lz + clz)));
-      if (fmaf(bx, bx, fmaf(by, by, bz * bz)) > rc2_hi) continue;
-      const int of = c_off[k], s0 = c_start[k];
-      for (int base = 0; base < n; base += T) {
-        const int m = base + t;
-        const bool valid = m < n;
-        int j;
-        bool in2;
-        float dx, dy, dz;
-        if (staged) {
-          float4 v = st[of + (valid ? m : 0)];
-          int w = __float_as_int(v.w);
-          j = w & 0x7fffffff;
-          in2 = w >= 0;
-          dx = v.x - px; dy = v.y - py; dz = v.z - pz;
-        } else {
-          j = s0 + (valid ? m : 0);
-          in2 = (a.mask[j] & a.g2bit) != 0;
-          double ex, ey, ez;
-          fast_dist2(pi, a.pos[j], a.L, a.iL, ex, ey, ez);
-          dx = (float)ex; dy = (float)ey; dz = (float)ez;
+    for (int m0 = 0; m0 < Rp; m0 += TILE_MAX) {
+      const int m1 = Rp < m0 + TILE_MAX ? Rp : m0 + TILE_MAX;
+      if (!single) {
+        __syncthreads();
+        stage(m0, m1);
+      }
+      __syncthreads();
+      if (!active) continue;
+      for (int c0 = m0; c0 < m1; c0 += 32 * T) {
+        // bit 31-b <-> candidate slot c0 + t + b*T
+        const float4* sv = st + (c0 - m0) + t;
+        unsigned hit = 0u;
+#pragma unroll 8
+        for (int b = 0; b < 32; ++b) {
+          const float4 v = sv[b * T];
+          float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
+          float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          hit = hit + hit + (d2 <= hi ? 1u : 0u);
         }
-        float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        bool pred = valid && in2 && j != i && d2 <= rc2_hi;
-        if (pred && d2 >= rc2_lo) {
-          // within the FP32 error window: the reference's FP64 test decides
-          double ex, ey, ez;
-          pred = min_image_dist2(pi, a.pos[j], a.hb, ex, ey, ez) < a.rc2;
+        // the row atom is not its own neighbour
+        const int ob = own - c0 - t;
+        if (ob >= 0 && ob < 32 * T && ob % T == 0) hit &= ~(0x80000000u >> (ob / T));
+        // entries inside the FP32 window: the FP64 test decides
+        unsigned chk = hit;
+        while (chk) {
+          const int b = __clz(chk);
+          chk &= ~(0x80000000u >> b);
+          const float4 v = sv[b * T];
+          float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
+          if (fmaf(dx, dx, fmaf(dy, dy, dz * dz)) >= lo) {
+            double ex, ey, ez;
+            if (!(min_image_dist2(pi, a.pos[__float_as_int(v.w)], a.hb, ex, ey, ez) < a.rc2))
+              hit &= ~(0x80000000u >> b);
+          }
         }
-        unsigned long long bits = __ballot(pred) & gmask;
-        if (pred) {
-          int slot = count + __popcll(bits & lower);
-          if (slot < a.nmax) out[slot] = j;
+        // offsets of this lane's hits inside the row
+        const int nh = __popc(hit);
+        int pre = nh;
+#pragma unroll
+        for (int o = 1; o < T; o <<= 1) {
+          int v = __shfl_up(pre, o, T);
+          if (t >= o) pre += v;
         }
-        count += __popcll(bits);
+        const int tot = __shfl(pre, T - 1, T);
+        int slot = count + pre - nh;
+        while (hit) {
+          const int b = __clz(hit);
+          hit &= ~(0x80000000u >> b);
+          if (slot < a.nmax) out[slot] = __float_as_int(sv[b * T].w);
+          ++slot;
+        }
+        count += tot;
       }
     }
-    if (t == 0) {
+    if (active && t == 0) {
       a.nnum[row] = count < a.nmax ? count : a.nmax;
       if (count > a.nmax) raise_err(a.err, a.err_code, a.orig[i] + 1, count, a.step);
       else if (a.exact && count != a.nmax) raise_err(a.err, PFM_E_GRAPHENE_NN, a.orig[i] + 1, count, a.step);
@@ -538,13 +558,20 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
 
 template <int T>
 static void launch_build_T(pfm_ctx* h, SearchArgs& a, const int* row_of) {
+  long threads = (long)a.nrows * T;
+  int nb = (int)((threads + 255) / 256);
+  PFM_LAUNCH(h, "build_list", k_build<T>, pad8(nb), 256, a, nb);
+}
+
+static void launch_build(pfm_ctx* h, SearchArgs& a, const int* row_of) {
   if (h->nc[0] >= 3 && h->nc[1] >= 3 && h->nc[2] >= 3) {
     int nb = h->ncell;
-    PFM_LAUNCH(h, "build_list", k_build_tile<T>, pad8(nb), 256, a, h->cell_xyz, row_of, nb);
+    if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", k_build_tile<4>, pad8(nb), 256, a, h->cell_xyz, row_of, nb);
+    else PFM_LAUNCH(h, "build_list", k_build_tile<8>, pad8(nb), 256, a, h->cell_xyz, row_of, nb);
+  } else if (a.nmax <= 8) {
+    launch_build_T<4>(h, a, row_of);
   } else {
-    long threads = (long)a.nrows * T;
-    int nb = (int)((threads + 255) / 256);
-    PFM_LAUNCH(h, "build_list", k_build<T>, pad8(nb), 256, a, nb);
+    launch_build_T<8>(h, a, row_of);
   }
 }
 
@@ -584,8 +611,7 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   a.err_code = PFM_E_OVERFLOW;
   a.exact = (k == 2) ? 1 : 0;
   const int* row_of = h->grp.row_of[L.g1];
-  if (L.nmax <= 8) launch_build_T<4>(h, a, row_of);
-  else launch_build_T<8>(h, a, row_of);
+  launch_build(h, a, row_of);
   int nb = (L.nrows + 255) / 256;
   if (it.kind == PFM_TB || (k == 2 && !(int)it.p[it.kind == PFM_LJC ? 5 : 6])) {
     PFM_LAUNCH(h, "list_rev", k_rev, nb, 256, L.nrows, h->grp.rows[L.g1], h->grp.row_of[L.g1], L.nnum, L.nbr,

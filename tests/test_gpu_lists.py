@@ -1,0 +1,60 @@
+"""Neighbour-list search on the GPU against the CPU checker, at the corners of
+the cell-tile search: distances tied with r_cut to the last bit, cells with
+more rows than a wavefront and tiles larger than one LDS pass, and rows that
+overflow nmax.
+
+Bar: lists identical as sorted sets; the overflow reported with the same
+status as the reference's "too many neighbours" stop.
+"""
+import numpy as np
+import pytest
+
+from pfmds_amd import capi, configs
+from tests.harness import gpu_ctx, list_sets, oracle_ctx, rel_inf
+
+pytestmark = pytest.mark.gpu
+
+
+def _lists_equal(s):
+    g, o = gpu_ctx(s), oracle_ctx(s)
+    integ = s.integrators[0].name
+    g.run(integ, s.integrators[0].dt, 0, 0, True)
+    o.run(integ, s.integrators[0].dt, 0, 0, True)
+    gs, _ = list_sets(g, 1, 1)
+    os_, _ = list_sets(o, 1, 1)
+    assert gs == os_
+    return g, o, gs
+
+
+def test_lists_shell_ties():
+    # perfect fcc lattice, r_cut on the a*sqrt(3) shell: the eight body-diagonal
+    # neighbours of every atom sit at r_cut up to rounding, inside the FP32
+    # window, and the FP64 find_distance test decides each one
+    a = 3.615
+    s = configs.fcc_cu_rgl(nx=6, a=a, sigma=0.0, skin=a * np.sqrt(3.0) - 6.0, nmax=96)
+    print(f"[ties] n={s.n} r_cut={s.interactions[0].lists[0][3]!r}", flush=True)
+    _, _, rows = _lists_equal(s)
+    counts = {len(r) for r in rows}
+    assert min(counts) >= 78  # the five shells inside a*sqrt(3)
+
+
+def test_lists_dense_cells():
+    # r_cut 9.5 A in fcc Cu: about 107 atoms per cell (two row chunks of one
+    # wavefront) and 27-cell tiles of about 2900 atoms (three LDS passes)
+    s = configs.fcc_cu_rgl(nx=9, skin=3.5, nmax=340, seed=21)
+    print(f"[dense] n={s.n}", flush=True)
+    g, o, rows = _lists_equal(s)
+    assert max(len(r) for r in rows) > 250
+    assert rel_inf(g.state()[2], o.state()[2]) < 1e-10
+
+
+def test_list_overflow_status():
+    s = configs.fcc_cu_rgl(nx=6, nmax=40, seed=22)
+    print(f"[overflow] n={s.n}", flush=True)
+    codes = []
+    for ctx in (gpu_ctx(s), oracle_ctx(s)):
+        with pytest.raises(capi.PfmError) as ei:
+            ctx.run("nve", 1.0, 0, 0, True)
+        codes.append(ei.value.code)
+        assert "too many neighbours" in ei.value.msg
+    assert codes == [capi.E_OVERFLOW, capi.E_OVERFLOW]
