@@ -62,6 +62,7 @@ struct IntP {
   int scale, invert;
   double zlo, zhi;
   uint32_t zero_bit;  // zero the forces of these atoms after the first kick (zero_forces)
+  int pre_kick;       // first apply the previous step's second half kick (nve)
 };
 
 struct NhcDev {
@@ -97,14 +98,20 @@ __global__ void k_kick_drift(int N, double4* __restrict__ pos, double4* __restri
   if (i >= N) return;
   double4 p = pos[i], v = vel[i];
   uint32_t m = mask[i];
+  bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
+  double4 f = (bx || bz) ? frc[i] : make_double4(0, 0, 0, 0);
+  if (P.pre_kick && (bx || bz)) {
+    // second half of the previous step (same operations as k_kick2)
+    double k = 1.0 / v.w / PFM_MASS_COEF;
+    if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
+    if (bz) v.z = v.z + f.z * k * P.ts1;
+  }
   if (P.invert) invert_v(v, p.z, P.zlo, P.zhi);
   if (P.scale) {
     for (int c = 0; c < P.nchain; ++c)
       if (m & P.chain_bit[c]) { double s = P.chain_s[c]; v.x *= s; v.y *= s; v.z *= s; }
   }
-  bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
   if (bx || bz) {
-    double4 f = frc[i];
     double k = 1.0 / v.w / PFM_MASS_COEF;
     if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
     if (bz) v.z = v.z + f.z * k * P.ts1;
@@ -192,35 +199,7 @@ __global__ void k_kick2(int N, double4* __restrict__ vel, const double4* __restr
   vel[i] = v;
 }
 
-// second half with the kinetic energies of the chain groups after the kick
-// (the input of the second NHC half step), per block, grid-stride
-__global__ void __launch_bounds__(RED_THREADS) k_kick2_ke(int N, double4* __restrict__ vel,
-                                                          const double4* __restrict__ frc,
-                                                          const uint32_t* __restrict__ mask, IntP P, MaskQ Q,
-                                                          double* __restrict__ part) {
-  __shared__ double sh[RED_THREADS];
-  double acc[RED_MAXQ];
-  for (int q = 0; q < Q.nq; ++q) acc[q] = 0.0;
-  for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
-    uint32_t m = mask[i];
-    bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
-    double4 v = vel[i];
-    if (bx || bz) {
-      double4 f = frc[i];
-      double k = 1.0 / v.w / PFM_MASS_COEF;
-      if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
-      if (bz) v.z = v.z + f.z * k * P.ts1;
-      vel[i] = v;
-    }
-    double ke = v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF;
-    for (int q = 0; q < Q.nq; ++q)
-      if (m & Q.bit[q]) acc[q] += ke;
-  }
-  for (int q = 0; q < Q.nq; ++q) {
-    double s = block_sum(acc[q], sh);
-    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = s;
-  }
-}
+
 
 __global__ void k_scale(int N, double4* __restrict__ vel, const uint32_t* __restrict__ mask, IntP P) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -354,21 +333,58 @@ __device__ double nhc_update(NhcDev& h, double ke, double ts1, double ts2, doubl
 // half of the next (twice = 1).  Nothing changes the velocities between those
 // two halves except invert_z_velocities, which keeps the kinetic energy, so
 // the second one uses s^2 * ke; s_out receives the product of the scales,
-// applied to the velocities by the next kick.
-__global__ void __launch_bounds__(RED_THREADS) k_nhc_chain(int nchain, NhcDev* __restrict__ C,
-                                                           const double* __restrict__ part, int nb,
-                                                           double* __restrict__ s_out, double ts1, double ts2,
-                                                           double ts3, int twice) {
+// applied to the velocities by the next kick.  One block.
+struct ChainArgs {
+  NhcDev* C;
+  double* s_out;
+  double ts1, ts2, ts3;
+  int twice;
+};
+
+__global__ void __launch_bounds__(RED_THREADS) k_nhc_chain(int nchain, ChainArgs A, const double* __restrict__ part,
+                                                           int nb) {
   __shared__ double sh[RED_THREADS];
   for (int c = 0; c < nchain; ++c) {
     double a = 0.0;
     for (int b = threadIdx.x; b < nb; b += RED_THREADS) a += part[c * nb + b];
     double ke = block_sum(a, sh);
     if (threadIdx.x == 0) {
-      double s = nhc_update(C[c], ke, ts1, ts2, ts3);
-      if (twice) s *= nhc_update(C[c], s * s * ke, ts1, ts2, ts3);
-      s_out[c] = s;
+      double s = nhc_update(A.C[c], ke, A.ts1, A.ts2, A.ts3);
+      if (A.twice) s *= nhc_update(A.C[c], s * s * ke, A.ts1, A.ts2, A.ts3);
+      A.s_out[c] = s;
     }
+  }
+}
+
+
+
+// second half with the kinetic energies of the chain groups after the kick
+// (the input of the second NHC half step), per block, grid-stride
+__global__ void __launch_bounds__(RED_THREADS) k_kick2_ke(int N, double4* __restrict__ vel,
+                                                          const double4* __restrict__ frc,
+                                                          const uint32_t* __restrict__ mask, IntP P, MaskQ Q,
+                                                          double* __restrict__ part) {
+  __shared__ double sh[RED_THREADS];
+  double acc[RED_MAXQ];
+  for (int q = 0; q < Q.nq; ++q) acc[q] = 0.0;
+  for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
+    uint32_t m = mask[i];
+    bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
+    double4 v = vel[i];
+    if (bx || bz) {
+      double4 f = frc[i];
+      double k = 1.0 / v.w / PFM_MASS_COEF;
+      if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
+      if (bz) v.z = v.z + f.z * k * P.ts1;
+      vel[i] = v;
+    }
+    double ke = v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF;
+    for (int q = 0; q < Q.nq; ++q)
+      if (m & Q.bit[q]) acc[q] += ke;
+  }
+  for (int q = 0; q < Q.nq; ++q) {
+    double s = block_sum(acc[q], sh);
+    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = s;
   }
 }
 
@@ -469,6 +485,11 @@ static int finalize(pfm_ctx* h) {
     if (is_cos(it.kind)) {
       if ((rc = dalloc(h, &it.gnorm, h->N))) return rc;
       if ((rc = dalloc(h, &it.gS, h->N))) return rc;
+      std::vector<char> in1(h->N, 0);
+      for (int x : h->grp.idx[it.nl[0].g1]) in1[x] = 1;
+      it.rows_disjoint = true;
+      for (int x : h->grp.idx[it.nl[0].g2])
+        if (in1[x]) { it.rows_disjoint = false; break; }
     }
   }
   // group-local index by original atom, for lj1g and list downloads
@@ -562,9 +583,9 @@ static MaskQ chain_masks(pfm_ctx* h) {
 static void nhc_first_half(pfm_ctx* h, double dt) {
   int nb = h->red_blocks;
   MaskQ Q = chain_masks(h);
+  ChainArgs A{(NhcDev*)h->nhc_state, h->red_out + 16, dt / 2.0, dt / 4.0, dt / 8.0, 0};
   PFM_LAUNCH(h, "nhc", k_ke_part, nb, RED_THREADS, h->N, h->vel4, h->mask, Q, h->red_part);
-  PFM_LAUNCH(h, "nhc", k_nhc_chain, 1, RED_THREADS, Q.nq, (NhcDev*)h->nhc_state, h->red_part, nb, h->red_out + 16,
-             dt / 2.0, dt / 4.0, dt / 8.0, 0);
+  PFM_LAUNCH(h, "nhc", k_nhc_chain, 1, RED_THREADS, Q.nq, A, h->red_part, nb);
 }
 
 // Energies of the step: per-interaction sums of the per-row energies and the
@@ -841,6 +862,7 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
   if (h->qpos) PFM_LAUNCH(h, "quantize", k_quantize, nb, 256, N, h->pos4, h->qpos, h->qscale);
   const bool nvt = integ == PFM_NVT && !h->nhc.empty();
   bool nhc_pending = false;
+  bool kick_pending = false;  // second half kick deferred into the next kick_drift
   for (int s = s0; s <= s1; ++s) {
     *last_step = s;
     bool energy = (s == s1 && energy_at_end) || integ == PFM_NVMS;
@@ -850,6 +872,8 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
         return PFM_OK;
       }
       IntP Q = P;
+      Q.pre_kick = kick_pending ? 1 : 0;
+      kick_pending = false;
       Q.invert = h->invert_z;
       Q.zero_bit = first_writes ? 0u : all_bit;
       if (nvt) {
@@ -868,15 +892,18 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
     if ((rc = launch_forces(h, energy, first_writes))) return rc;
     if (s != 0) {
       if (nvt) {
-        // second half step; on steps with nothing to report, fused with the
-        // first half of the next step (k_nhc_chain)
+        // second half step; on steps with nothing to report, the chain update
+        // also takes the first half of the next step (k_nhc_chain)
         MaskQ C = chain_masks(h);
-        PFM_LAUNCH(h, "kick2", k_kick2_ke, h->red_blocks, RED_THREADS, N, h->vel4, h->frc4, h->mask, P, C, h->red_part);
         bool fuse = !energy && s < s1;
-        PFM_LAUNCH(h, "nhc", k_nhc_chain, 1, RED_THREADS, C.nq, (NhcDev*)h->nhc_state, h->red_part, h->red_blocks,
-                   h->red_out + 16, dt / 2.0, dt / 4.0, dt / 8.0, fuse ? 1 : 0);
+        ChainArgs A{(NhcDev*)h->nhc_state, h->red_out + 16, dt / 2.0, dt / 4.0, dt / 8.0, fuse ? 1 : 0};
+        PFM_LAUNCH(h, "kick2", k_kick2_ke, h->red_blocks, RED_THREADS, N, h->vel4, h->frc4, h->mask, P, C, h->red_part);
+        PFM_LAUNCH(h, "nhc", k_nhc_chain, 1, RED_THREADS, C.nq, A, h->red_part, h->red_blocks);
         if (fuse) nhc_pending = true;
         else PFM_LAUNCH(h, "nhc_scale", k_scale, nb, 256, N, h->vel4, h->mask, P);
+      } else if (integ != PFM_NVMS && !energy && s < s1) {
+        // nothing reads the velocities before the next step's first half
+        kick_pending = true;
       } else {
         PFM_LAUNCH(h, "kick2", k_kick2, nb, 256, N, h->vel4, h->frc4, h->mask, P, integ == PFM_NVMS ? 1 : 0);
       }

@@ -48,8 +48,9 @@ template <> __device__ __forceinline__ void store_w<float>(const RowArgs& a, int
   const_cast<uint4*>(a.qpos)[i].w = __float_as_uint((float)v);
 }
 
-#define ROW_PROLOGUE(T)                                   \
-  int lb = xcd_block(nb_real);                            \
+#define ROW_PROLOGUE(T) ROW_PROLOGUE_AT(T, xcd_block(nb_real))
+#define ROW_PROLOGUE_AT(T, LB)                            \
+  int lb = (LB);                                          \
   if (lb < 0) return;                                     \
   int gt = lb * blockDim.x + threadIdx.x;                 \
   int row = gt / (T), t = threadIdx.x & ((T)-1);          \
@@ -221,39 +222,74 @@ struct TbP {
   double D, S, beta, r0, delt, a0, c02, d02, R1, R2, invw;
 };
 
+// Tersoff-Brenner rows are short (three bonds in graphene): TB_T lanes share
+// a row and lane t owns bonds p = t, t + TB_T, ...; the angular sums of a bond
+// over the row's other bonds read their geometry from the owning lane
+// (shuffles inside the row's lane group).
+#define TB_T 4
+#define TB_U (PFM_TB_MAXN / TB_T)
+
+// geometry of this lane's bonds: dr, r, cutoff and its derivative/r
+template <class R>
+__device__ __forceinline__ void tb_own_bonds(const Geo<R>& G, const typename Geo<R>::P& pi, const int* nbrow, int n,
+                                             int t, const TbP& P, R* xs, R* ys, R* zs, R* rs, R* fcs, R* dfcs) {
+  const R R1 = P.R1, R2 = P.R2, invw = P.invw;
+#pragma unroll
+  for (int u = 0; u < TB_U; ++u) {
+    const int p = t + TB_T * u;
+    xs[u] = ys[u] = zs[u] = R(0);
+    rs[u] = R(1);
+    fcs[u] = dfcs[u] = R(0);
+    if (p < n) {
+      rs[u] = PM<R>::sqrt_(G.d2(pi, G.at(nbrow[p]), xs[u], ys[u], zs[u]));
+      cos_cutoff<R>(rs[u], R1, R2, invw, fcs[u], dfcs[u]);
+    }
+  }
+}
+
 // bond order B(p,i) (TersoffBrenner.f90:81-91) and, per entry, the factors
 // of the force pass: Bp = B**(1/delt+1) and
 // BT = delt/2 * Bp * fc * D/(S-1) * S * exp(-sqrt(2S) beta (r-r0)/S)
 template <class R>
 __global__ void __launch_bounds__(256) k_tb_bond(RowArgs a, TbP P, double* __restrict__ B, double* __restrict__ Bp,
                                                  double* __restrict__ BT, int nb_real) {
-  ROW_PROLOGUE(1)
-  (void)t;
-  const R R1 = P.R1, R2 = P.R2, invw = P.invw, a0 = P.a0, c02 = P.c02, d02 = P.d02;
-  R dxs[PFM_TB_MAXN], dys[PFM_TB_MAXN], dzs[PFM_TB_MAXN], rs[PFM_TB_MAXN], fcs[PFM_TB_MAXN];
-  for (int p = 0; p < n; ++p) {
-    rs[p] = PM<R>::sqrt_(G.d2(pi, G.at(nbrow[p]), dxs[p], dys[p], dzs[p]));
-    R df;
-    cos_cutoff<R>(rs[p], R1, R2, invw, fcs[p], df);
+  constexpr int T = TB_T;
+  ROW_PROLOGUE(T)
+  const R R2 = P.R2, a0 = P.a0, c02 = P.c02, d02 = P.d02;
+  R xs[TB_U], ys[TB_U], zs[TB_U], rs[TB_U], fcs[TB_U], dfcs[TB_U];
+  tb_own_bonds<R>(G, pi, nbrow, n, t, P, xs, ys, zs, rs, fcs, dfcs);
+  R b[TB_U];
+#pragma unroll
+  for (int u = 0; u < TB_U; ++u) b[u] = R(0);
+#pragma unroll
+  for (int uq = 0; uq < TB_U; ++uq) {
+    for (int tq = 0; tq < T; ++tq) {
+      const int q = tq + T * uq;
+      if (q >= n) break;  // uniform over the row's lanes
+      const R xq = __shfl(xs[uq], tq, T), yq = __shfl(ys[uq], tq, T), zq = __shfl(zs[uq], tq, T);
+      const R rq = __shfl(rs[uq], tq, T), fq = __shfl(fcs[uq], tq, T);
+#pragma unroll
+      for (int u = 0; u < TB_U; ++u) {
+        const int p = t + T * u;
+        if (p < n && q != p && rs[u] < R2 && rq < R2) {
+          R cs = R(1) + (xs[u] * xq + ys[u] * yq + zs[u] * zq) * PM<R>::rcp(rs[u] * rq);
+          b[u] += fq * (R(1) + c02 / d02 - c02 * PM<R>::rcp(d02 + cs * cs));
+        }
+      }
+    }
   }
   const R ex = R(1.0 / P.delt + 1.0), md = R(-P.delt);
   const R tailc = R(P.D / (P.S - 1.0) * P.S), tailk = R(-sqrt(2.0 * P.S) * P.beta / P.S), r0 = P.r0;
-  for (int p = 0; p < n; ++p) {
-    R b = R(0);
-    if (rs[p] < R2) {
-      for (int q = 0; q < n; ++q) {
-        if (q != p && rs[q] < R2) {
-          R cs = R(1) + (dxs[p] * dxs[q] + dys[p] * dys[q] + dzs[p] * dzs[q]) * PM<R>::rcp(rs[p] * rs[q]);
-          b += fcs[q] * (R(1) + c02 / d02 - c02 * PM<R>::rcp(d02 + cs * cs));
-        }
-      }
-      b = PM<R>::pow_(R(1) + a0 * b, md);
-    }
+#pragma unroll
+  for (int u = 0; u < TB_U; ++u) {
+    const int p = t + T * u;
+    if (p >= n) continue;
+    R bb = rs[u] < R2 ? PM<R>::pow_(R(1) + a0 * b[u], md) : R(0);
     size_t e = (size_t)row * a.stride + p;
-    R bp = b > R(0) ? PM<R>::pow_(b, ex) : R(0);
-    B[e] = (double)b;
+    R bp = bb > R(0) ? PM<R>::pow_(bb, ex) : R(0);
+    B[e] = (double)bb;
     Bp[e] = (double)bp;
-    BT[e] = (double)(R(0.5 * P.delt) * bp * fcs[p] * tailc * PM<R>::exp_(tailk * (rs[p] - r0)));
+    BT[e] = (double)(R(0.5 * P.delt) * bp * fcs[u] * tailc * PM<R>::exp_(tailk * (rs[u] - r0)));
   }
 }
 
@@ -264,37 +300,53 @@ __global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* _
                                                   const int* __restrict__ rev, const double* __restrict__ B,
                                                   const double* __restrict__ Bp, const double* __restrict__ BT,
                                                   int nb_real) {
-  ROW_PROLOGUE(1)
-  (void)t;
+  constexpr int T = TB_T;
+  ROW_PROLOGUE(T)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, a0 = P.a0, c02 = P.c02, d02 = P.d02;
   const R D_S1 = R(P.D / (P.S - 1.0)), S = P.S, r0 = P.r0;
   const R s2S = R(sqrt(2.0 * P.S) * P.beta), s2oS = R(sqrt(2.0 / P.S) * P.beta), hdel = R(-P.delt * 0.5);
-  R dxs[PFM_TB_MAXN], dys[PFM_TB_MAXN], dzs[PFM_TB_MAXN], rs[PFM_TB_MAXN], fcs[PFM_TB_MAXN], dfcs[PFM_TB_MAXN];
-  for (int p = 0; p < n; ++p) {
-    rs[p] = PM<R>::sqrt_(G.d2(pi, G.at(nbrow[p]), dxs[p], dys[p], dzs[p]));
-    cos_cutoff<R>(rs[p], R1, R2, invw, fcs[p], dfcs[p]);
+  R xs[TB_U], ys[TB_U], zs[TB_U], rs[TB_U], fcs[TB_U], dfcs[TB_U];
+  tb_own_bonds<R>(G, pi, nbrow, n, t, P, xs, ys, zs, rs, fcs, dfcs);
+  // dB(p,i)/dr_i over i's other bonds q
+  R bx[TB_U], by[TB_U], bz[TB_U];
+#pragma unroll
+  for (int u = 0; u < TB_U; ++u) bx[u] = by[u] = bz[u] = R(0);
+#pragma unroll
+  for (int uq = 0; uq < TB_U; ++uq) {
+    for (int tq = 0; tq < T; ++tq) {
+      const int q = tq + T * uq;
+      if (q >= n) break;  // uniform over the row's lanes
+      const R xq = __shfl(xs[uq], tq, T), yq = __shfl(ys[uq], tq, T), zq = __shfl(zs[uq], tq, T);
+      const R rq = __shfl(rs[uq], tq, T), fq = __shfl(fcs[uq], tq, T), dfq = __shfl(dfcs[uq], tq, T);
+#pragma unroll
+      for (int u = 0; u < TB_U; ++u) {
+        const int p = t + T * u;
+        if (p < n && q != p) {
+          const R xp = xs[u], yp = ys[u], zp = zs[u], rp = rs[u];
+          const R irp2 = PM<R>::rcp(rp * rp);
+          R rr = PM<R>::rcp(rp * rq);
+          R cosi = (xp * xq + yp * yq + zp * zq) * rr;
+          R tt = d02 + (R(1) + cosi) * (R(1) + cosi);
+          R c1 = fq * R(2) * a0 * c02 * (R(1) + cosi) * PM<R>::rcp(tt * tt);
+          R c2 = dfq * a0 * (R(1) + c02 / d02 - c02 * PM<R>::rcp(tt));
+          R irq2 = PM<R>::rcp(rq * rq);
+          bx[u] += c1 * ((xp + xq) * rr - cosi * (xp * irp2 + xq * irq2)) + c2 * xq;
+          by[u] += c1 * ((yp + yq) * rr - cosi * (yp * irp2 + yq * irq2)) + c2 * yq;
+          bz[u] += c1 * ((zp + zq) * rr - cosi * (zp * irp2 + zq * irq2)) + c2 * zq;
+        }
+      }
+    }
   }
   double fx = 0, fy = 0, fz = 0, e = 0;
-  for (int p = 0; p < n; ++p) {
-    const R xp = dxs[p], yp = dys[p], zp = dzs[p], rp = rs[p];
+#pragma unroll
+  for (int u = 0; u < TB_U; ++u) {
+    const int p = t + T * u;
+    if (p >= n) continue;
+    const R xp = xs[u], yp = ys[u], zp = zs[u], rp = rs[u];
     const R irp2 = PM<R>::rcp(rp * rp);
-    R bx = 0, by = 0, bz = 0;
-    for (int q = 0; q < n; ++q) {
-      if (q == p) continue;
-      const R xq = dxs[q], yq = dys[q], zq = dzs[q], rq = rs[q];
-      R rr = PM<R>::rcp(rp * rq);
-      R cosi = (xp * xq + yp * yq + zp * zq) * rr;
-      R tt = d02 + (R(1) + cosi) * (R(1) + cosi);
-      R c1 = fcs[q] * R(2) * a0 * c02 * (R(1) + cosi) * PM<R>::rcp(tt * tt);
-      R c2 = dfcs[q] * a0 * (R(1) + c02 / d02 - c02 * PM<R>::rcp(tt));
-      R irq2 = PM<R>::rcp(rq * rq);
-      bx += c1 * ((xp + xq) * rr - cosi * (xp * irp2 + xq * irq2)) + c2 * xq;
-      by += c1 * ((yp + yq) * rr - cosi * (yp * irp2 + yq * irq2)) + c2 * yq;
-      bz += c1 * ((zp + zq) * rr - cosi * (zp * irp2 + zq * irq2)) + c2 * zq;
-    }
     size_t ep = (size_t)row * a.stride + p;
     R bpp = (R)Bp[ep];
-    bx *= bpp; by *= bpp; bz *= bpp;
+    R gx = bx[u] * bpp, gy = by[u] * bpp, gz = bz[u] * bpp;
     int j = nbrow[p];
     int jr = row_of[j];
     int l = rev[ep];
@@ -304,7 +356,7 @@ __global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* _
     const R Bl = (R)B[jbase + l], Bpl = (R)Bp[jbase + l];
     // vector from j to i and its cutoff
     const R xl = -xp, yl = -yp, zl = -zp, rl = rp, irl2 = irp2;
-    const R fcl = fcs[p], dfcl = dfcs[p];
+    const R fcl = fcs[u], dfcl = dfcs[u];
     const typename Geo<R>::P pj = G.at(j);
     R tx = 0, ty = 0, tz = 0;
     for (int q = 0; q < nj; ++q) {
@@ -319,7 +371,7 @@ __global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* _
       R gq = R(2) * a0 * c02 * (R(1) + cosi) * PM<R>::rcp(tt * tt);
       R vx = -xq * rr + cosi * xl * irl2, vy = -yq * rr + cosi * yl * irl2, vz = -zq * rr + cosi * zl * irl2;
       R c1 = Bpl * fq * gq;
-      bx += c1 * vx; by += c1 * vy; bz += c1 * vz;
+      gx += c1 * vx; gy += c1 * vy; gz += c1 * vz;
       // i as third body of bond (j,k)
       R bt = (R)BT[jbase + q];
       R g2 = dfcl * a0 * (R(1) + c02 / d02 - c02 * PM<R>::rcp(tt));
@@ -328,11 +380,11 @@ __global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* _
       ty += bt * (g1 * vy + yp * g2);
       tz += bt * (g1 * vz + zp * g2);
     }
-    bx *= hdel; by *= hdel; bz *= hdel;
+    gx *= hdel; gy *= hdel; gz *= hdel;
     fx += (double)tx; fy += (double)ty; fz += (double)tz;
     if (rp < R2) {
-      R fc = fcs[p];
-      R dff = dfcs[p] * PM<R>::rcp(fc);
+      R fc = fcs[u];
+      R dff = dfcs[u] * PM<R>::rcp(fc);
       R av = -s2S * (rp - r0);
       R ea = PM<R>::exp_(av), eas = PM<R>::exp_(av * PM<R>::rcp(S));
       R pre = fc * D_S1;
@@ -340,14 +392,18 @@ __global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* _
       R k1 = (dff - s2S * PM<R>::rcp(rp)) * ea;
       R k2 = (dff - s2oS * PM<R>::rcp(rp)) * Bm * R(0.5);
       R se = S * eas;
-      fx += (double)(pre * (xp * k1 - (bx + xp * k2) * se));
-      fy += (double)(pre * (yp * k1 - (by + yp * k2) * se));
-      fz += (double)(pre * (zp * k1 - (bz + zp * k2) * se));
+      fx += (double)(pre * (xp * k1 - (gx + xp * k2) * se));
+      fy += (double)(pre * (yp * k1 - (gy + yp * k2) * se));
+      fz += (double)(pre * (zp * k1 - (gz + zp * k2) * se));
       if (E) e += 0.5 * (double)(pre * (ea - Bm * R(0.5) * se));
     }
   }
-  write_force(a, i, fx, fy, fz);
-  if (E) a.e_row[row] = e;
+  fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
+  if (E) e = group_sum<T>(e);
+  if (t == 0) {
+    write_force(a, i, fx, fy, fz);
+    if (E) a.e_row[row] = e;
+  }
 }
 
 // ------------------------------------------------------------ ljc / morsec
@@ -438,11 +494,11 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_cos_graphene(RowArgs a, CosP P, 
 // the three nearest neighbours j of i, the sum over j's metal neighbours p of
 //   c * V2 V3 fc /(den (n_j.dr_jp)) (dr_jp . W) n_j
 // equals c/den (S_j . W) n_j with S_j from the graphene pass.
-__global__ void __launch_bounds__(256) k_cos_nn(RowArgs a, CosP P, const int* __restrict__ row_of,
-                                                const int* __restrict__ rev, const double4* __restrict__ gnorm,
-                                                const double4* __restrict__ gS, int nb_real) {
+__device__ __forceinline__ void cos_nn_rows(const RowArgs& a, const CosP& P, const int* __restrict__ row_of,
+                                            const int* __restrict__ rev, const double4* __restrict__ gnorm,
+                                            const double4* __restrict__ gS, int block) {
   using R = double;  // FP64 in both precisions: three entries per graphene atom
-  ROW_PROLOGUE(1)
+  ROW_PROLOGUE_AT(1, block)
   (void)t; (void)pi; (void)G;
   double fx = 0, fy = 0, fz = 0;
   for (int q = 0; q < 3 && n == 3; ++q) {
@@ -476,12 +532,18 @@ __global__ void __launch_bounds__(256) k_cos_nn(RowArgs a, CosP P, const int* __
   write_force(a, i, fx, fy, fz);
 }
 
+__global__ void __launch_bounds__(256) k_cos_nn(RowArgs a, CosP P, const int* __restrict__ row_of,
+                                                const int* __restrict__ rev, const double4* __restrict__ gnorm,
+                                                const double4* __restrict__ gS, int nb_real) {
+  cos_nn_rows(a, P, row_of, rev, gnorm, gS, xcd_block(nb_real));
+}
+
 // metal side (*_forces_for_other_atoms): rows of nl(2), neighbours are
 // graphene atoms whose normal enters V3 (morsec divides by |n|)
 template <int T, class R>
-__global__ void __launch_bounds__(PAIR_BLOCK) k_cos_metal(RowArgs a, CosP P, const double4* __restrict__ gnorm,
-                                                          int nb_real) {
-  ROW_PROLOGUE(T)
+__device__ __forceinline__ void cos_metal_rows(const RowArgs& a, const CosP& P, const double4* __restrict__ gnorm,
+                                               int block) {
+  ROW_PROLOGUE_AT(T, block)
   const R R2 = P.R2, pref = P.pref;
   double fx = 0, fy = 0, fz = 0;
   for (int p = t; p < n; p += T) {
@@ -501,6 +563,25 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_cos_metal(RowArgs a, CosP P, con
   }
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (t == 0) write_force(a, i, fx, fy, fz);
+}
+
+template <int T, class R>
+__global__ void __launch_bounds__(PAIR_BLOCK) k_cos_metal(RowArgs a, CosP P, const double4* __restrict__ gnorm,
+                                                          int nb_real) {
+  cos_metal_rows<T, R>(a, P, gnorm, xcd_block(nb_real));
+}
+
+// the normal-derivative rows (graphene atoms) and the metal rows in one
+// launch, for groups without common atoms (every force is written once):
+// logical blocks [0, nb_nn) take the graphene rows, the rest the metal rows
+template <int T, class R>
+__global__ void __launch_bounds__(256) k_cos_nn_metal(RowArgs an, RowArgs am, CosP P, const int* __restrict__ row_of,
+                                                      const int* __restrict__ rev, const double4* __restrict__ gnorm,
+                                                      const double4* __restrict__ gS, int nb_nn, int nb_real) {
+  int lb = xcd_block(nb_real);
+  if (lb < 0) return;
+  if (lb < nb_nn) cos_nn_rows(an, P, row_of, rev, gnorm, gS, lb);
+  else cos_metal_rows<T, R>(am, P, gnorm, lb - nb_nn);
 }
 
 // ----------------------------------------------------------------- dispatch
@@ -597,7 +678,7 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         if (a.nrows == 0) break;
         TbP P{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6] * it.p[6], it.p[7] * it.p[7],
               it.p[8], it.p[9], 1.0 / (it.p[9] - it.p[8])};
-        int nb = row_blocks(a.nrows, 1, BS1);
+        int nb = row_blocks(a.nrows, TB_T, BS1);
         double* BT = it.Bp + (size_t)it.nl[0].nrows * it.nl[0].stride;
         PFM_LAUNCH(h, "tb_bond", (k_tb_bond<R>), pad8(nb), BS1, a, P, it.B, it.Bp, BT, nb);
         const int* row_of = h->grp.row_of[it.nl[0].g1];
@@ -630,13 +711,26 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
           });
           written = true;
         }
-        if (!simplified && NN.nrows > 0) {
+        RowArgs c = row_args(h, it, 1, written ? 1 : 0);
+        const bool nn = !simplified && NN.nrows > 0;
+        if (nn && c.nrows > 0 && it.rows_disjoint && BS == BS1) {
+          RowArgs b = row_args(h, it, 2, written ? 1 : 0);
+          int nbn = row_blocks(b.nrows, 1, BS1);
+          DISPATCH_T(T, {
+            int nb = nbn + row_blocks(c.nrows, T, BS1);
+            PFM_LAUNCH(h, "cos_nn_metal", (k_cos_nn_metal<T, R>), pad8(nb), BS1, b, c, P, h->grp.row_of[NN.g1],
+                       NN.rev, it.gnorm, it.gS, nbn, nb);
+          });
+          written = true;
+          break;
+        }
+        if (nn) {
           RowArgs b = row_args(h, it, 2, written ? 1 : 0);
           int nb = row_blocks(b.nrows, 1, BS1);
           PFM_LAUNCH(h, "cos_nn", k_cos_nn, pad8(nb), BS1, b, P, h->grp.row_of[NN.g1], NN.rev, it.gnorm, it.gS, nb);
           written = true;
         }
-        RowArgs c = row_args(h, it, 1, written ? 1 : 0);
+        c.accumulate = written ? 1 : 0;
         if (c.nrows > 0) {
           DISPATCH_T(T, {
             int nb = row_blocks(c.nrows, T, BS);

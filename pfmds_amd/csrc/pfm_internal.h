@@ -215,6 +215,7 @@ struct Inter {
   double* Bp = nullptr;  // [nrows*stride] B**(1/delt+1)
   // ljc/morsec
   double4* gnorm = nullptr;  // [N] by internal atom
+  bool rows_disjoint = false;  // groups 1 and 2 share no atom (passes over both may share a launch)
   double4* gS = nullptr;     // [N] sum of V2 V3 fc/(n.dr) dr per graphene atom
 };
 
