@@ -56,16 +56,17 @@ def mean_list_len(ctx):
 
 
 def rjl_counts(n_entries):
-    """Algorithmic bytes and flops per atom of the two RGL passes.
+    """Algorithmic bytes and flops per atom of the two mixed-precision RGL passes.
 
-    bytes (unique HBM traffic, neighbour gathers assumed cache-resident):
-      density: pos 24 + nnum 4 + indices 4*n + eb,inv_eb write 16
-      force:   pos 24 + nnum 4 + indices 4*n + inv_eb,eb read 16 + force write 24
+    bytes (unique HBM traffic; neighbour records are gathered from L2):
+      density: own position record 16 + nnum 4 + indices 4*n + rho**0.5 write 8
+               + 1/sqrt(rho) into the record's w slot 4
+      force:   own position record 16 + nnum 4 + indices 4*n + force write 32
     flops per list entry (each + - * / sqrt exp sin cos sign counts 1):
       distance with minimum image 30; density term 13; force term 40
     """
-    b_density = 24 + 4 + 4 * n_entries + 16
-    b_force = 24 + 4 + 4 * n_entries + 16 + 24
+    b_density = 16 + 4 + 4 * n_entries + 8 + 4
+    b_force = 16 + 4 + 4 * n_entries + 32
     f_density = (30 + 13) * n_entries
     f_force = (30 + 40) * n_entries
     return {"rjl_density": (b_density, f_density), "rjl_force": (b_force, f_force)}
@@ -83,6 +84,36 @@ def measured_traffic(kernel):
         if f"k_{kernel}<" in name and ", float" in name and "true" not in name:
             return v["bytes_per_launch"]
     return None
+
+
+def graphene_cu(steps, warmup):
+    """Graphene on Cu(111) (94,560 atoms: tb + morsec + rjl, nvt at 300 K,
+    dt = 0.5 fs) in FP64 and mixed precision, and the mixed force RMSE at
+    identical positions."""
+    s = configs.graphene_on_cu()
+    dt = s.integrators[-1].dt
+    out, ctxs = {}, {}
+    for name, prec in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
+        ctx = gpu_ctx(s, prec)
+        ctx.run("nvt", dt, 0, warmup, True)
+        t0 = time.perf_counter()
+        ctx.run("nvt", dt, warmup + 1, warmup + steps, True)
+        t = time.perf_counter() - t0
+        out[name + "_ms_per_step"] = t * 1e3 / steps
+        out[name + "_atom_steps_per_s"] = s.n * steps / t
+        ctxs[name] = ctx
+    pos, vel, _ = ctxs["mixed"].state()
+    for ctx in ctxs.values():
+        ctx.set_state(pos, vel)
+        ctx.run("nvt", dt, 0, 0, True)
+    f64, f32 = ctxs["fp64"].state()[2], ctxs["mixed"].state()[2]
+    out["speedup_vs_fp64"] = out["mixed_atom_steps_per_s"] / out["fp64_atom_steps_per_s"]
+    out["force_rmse_eV_per_A"] = float(np.sqrt(np.mean((f32 - f64) ** 2)))
+    out["atoms"] = s.n
+    out["steps"] = steps
+    print(f"# graphene/Cu: mixed {out['mixed_ms_per_step']:.4f} ms/step, fp64 {out['fp64_ms_per_step']:.4f} ms/step",
+          file=sys.stderr, flush=True)
+    return out
 
 
 def cpu_baseline(s, threads):
@@ -110,12 +141,17 @@ def main():
     ap.add_argument("--cells", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--drift-ps", type=float, default=20.0)
+    ap.add_argument("--no-c2", action="store_true", help="skip the graphene/Cu workload")
+    ap.add_argument("--c2-only", action="store_true", help="time only the graphene/Cu workload (profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step path has no multi-GPU decomposition "
               "(replicas or spatial domains would shard it; see the design notes). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if args.c2_only:
+        print(json.dumps({"graphene_cu_nvt": graphene_cu(args.steps, args.warmup)}))
+        return
     s = configs.fcc_cu_rgl(nx=args.cells, steps=args.steps)
     res = {}
     ctxs = {}
@@ -175,6 +211,8 @@ def main():
         slope = np.polyfit(np.array(ts) * 1e-6, (np.array(es) - es[0]) / s.n, 1)[0]
         drift = float(slope)
 
+    c2 = None if args.no_c2 else graphene_cu(200, 20)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -205,6 +243,7 @@ def main():
         "nve_drift_eV_per_atom_per_ns": drift,
         "kernels_us": {k: v[1] * 1e3 for k, v in kern.items()},
         "roofline": roofline,
+        "graphene_cu_nvt": c2,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
