@@ -66,6 +66,25 @@ typedef struct pfm_ctx pfm_ctx;
 int pfm_create(pfm_ctx** h, int n_atoms, const double box[3], const double* pos,
                const double* vel, const double* mass, int precision);
 
+/* Create a batch of n_rep independent systems (replicas) that share every
+ * pass of the step: replica r has n_atoms[r] atoms in its own periodic box
+ * boxes[3r..3r+2]; pos/vel/mass hold the replicas one after another, so atom
+ * numbering (groups, lists, downloads) runs over the concatenation.  Lists
+ * never link atoms of different replicas, and zero_momentum, per-replica
+ * energies (pfm_replica_energies) and the nvms stop test (every replica
+ * within ms_de) treat each replica as if it ran alone.  Nose-Hoover chains
+ * are refused in batches (a chain's group would span replicas).  No
+ * counterpart in the reference, which runs one system per md() call and farms
+ * systems over MPI ranks (run_md_simulation_mpi.f90:68-88); pfm_create is the
+ * batch of one. */
+int pfm_create_batch(pfm_ctx** h, int n_rep, const int* n_atoms, const double* boxes,
+                     const double* pos, const double* vel, const double* mass, int precision);
+
+/* Energies of every replica at the last energy evaluation: e_inter[n_rep][n_interactions],
+ * pe[n_rep], ke[n_rep] (all_moving atoms of the replica) and temp[n_rep]; any
+ * pointer may be NULL.  For a single system the values of pfm_energies. */
+int pfm_replica_energies(pfm_ctx* h, double* e_inter, double* pe, double* ke, double* temp);
+
 /* Define group gid (1-based) as the atom indices idx[0..n-1] (1-based) in the
  * order create_particle_group builds them (md_general.f90:57-80). */
 int pfm_set_group(pfm_ctx* h, int gid, int n, const int* idx);

@@ -64,6 +64,10 @@ def load_library(path, prefix):
     destroy.argtypes = [C.c_void_p]
     if prefix == "pfm":
         lib.pfm_create.argtypes = _SIGS["create"] + [C.c_int]
+        lib.pfm_create_batch.argtypes = [_P(C.c_void_p), C.c_int, _i, _d, _d, _d, _d, C.c_int]
+        lib.pfm_create_batch.restype = C.c_int
+        lib.pfm_replica_energies.argtypes = [C.c_void_p, _d, _d, _d, _d]
+        lib.pfm_replica_energies.restype = C.c_int
         lib.pfm_set_timing.argtypes = [C.c_void_p, C.c_int]
         lib.pfm_set_timing.restype = C.c_int
         lib.pfm_kernel_time.argtypes = [C.c_void_p, C.c_char_p, _i, _d]
@@ -85,7 +89,9 @@ def _ip(a):
 class Context:
     """One system behind the C-ABI (one pfm_ctx handle)."""
 
-    def __init__(self, lib, prefix, box, pos, vel, mass, precision=FP64):
+    def __init__(self, lib, prefix, box, pos, vel, mass, precision=FP64, n_atoms=None):
+        """One system, or with `n_atoms` (atoms per replica) a batch whose
+        `box` holds one row of three box lengths per replica."""
         self.lib, self.pfx = lib, prefix
         self.h = C.c_void_p()
         self.n = len(mass)
@@ -93,17 +99,31 @@ class Context:
         self._vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(self.n, 3)
         box = np.ascontiguousarray(box, dtype=np.float64)
         mass = np.ascontiguousarray(mass, dtype=np.float64)
+        self.n_inter = 0
+        self.n_nhc = 0
+        self.n_rep = 1
+        if n_atoms is not None:
+            if prefix != "pfm":
+                raise ValueError("batches need the HIP library")
+            na = np.ascontiguousarray(n_atoms, dtype=np.int32)
+            self.n_rep = len(na)
+            rc = self.lib.pfm_create_batch(C.byref(self.h), self.n_rep, _ip(na), _dp(box.reshape(-1)),
+                                           _dp(self._pos), _dp(self._vel), _dp(mass), int(precision))
+            self._check(rc)
+            return
         args = [C.byref(self.h), self.n, _dp(box), _dp(self._pos), _dp(self._vel), _dp(mass)]
         if prefix == "pfm":
             args.append(int(precision))
         rc = self._fn("create")(*args)
-        self.n_inter = 0
-        self.n_nhc = 0
         self._check(rc)
 
     @classmethod
     def gpu(cls, box, pos, vel, mass, precision=FP64, path=GPU_LIB):
         return cls(load_library(path, "pfm"), "pfm", box, pos, vel, mass, precision)
+
+    @classmethod
+    def gpu_batch(cls, n_atoms, boxes, pos, vel, mass, precision=FP64, path=GPU_LIB):
+        return cls(load_library(path, "pfm"), "pfm", boxes, pos, vel, mass, precision, n_atoms=n_atoms)
 
     def _fn(self, name):
         return getattr(self.lib, f"{self.pfx}_{name}")
@@ -200,6 +220,14 @@ class Context:
         return out[:n_rows]
 
     # -- GPU-only extras ----------------------------------------------------
+    def replica_energies(self):
+        """Per-replica energies of the last evaluation (arrays over replicas)."""
+        r, k = self.n_rep, max(self.n_inter, 1)
+        e = np.zeros((r, k))
+        pe, ke, t = np.zeros(r), np.zeros(r), np.zeros(r)
+        self._check(self.lib.pfm_replica_energies(self.h, _dp(e), _dp(pe), _dp(ke), _dp(t)))
+        return dict(e_inter=e[:, : self.n_inter], pe=pe, ke=ke, temp=t)
+
     def set_timing(self, on):
         self._check(self.lib.pfm_set_timing(self.h, int(bool(on))))
 

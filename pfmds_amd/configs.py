@@ -195,3 +195,63 @@ def graphene_on_cu(cx=100, cy=58, layers=6, frozen=2, gx=104, gy=60, seed=1002, 
                   integrators=ints, nhcs=[NHC(3, temperature, 3, q1)], all_moving=3, xyz_moving=3, z_moving=6,
                   all_atoms=4, traj=1, period_traj=10**9, zero_momentum_period=10**9, ms_de=1e-8,
                   md_step_limit=100000 + nvt_steps, init_xyz_filename="graphene_on_cu.xyz")
+
+
+# ------------------------------------------------------------- replica batches
+
+def lj_gas_lattice(n, density, seed, temperature, dt=0.5, steps=2000, r1=6.5, r2=7.5, skin=1.0, nmax=160,
+                   period=20, jitter=0.1):
+    """Binary LJ gas (the interactions of binary_lj_gas) of n atoms at
+    `density` (A^-3), on a jittered simple cubic lattice: spacing
+    (1/density)**(1/3) - 2*jitter stays above binary_lj_gas's 3.0 A minimum
+    for density <= 0.03, and the start costs O(n)."""
+    rng = SplitMix64(seed)
+    L = float((n / density) ** (1.0 / 3.0))
+    m = int(np.ceil(n ** (1.0 / 3.0)))
+    a = L / m
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+    pick = np.argsort(rng.uniform(len(g)), kind="stable")[:n]
+    pos = (g[np.sort(pick)] + 0.5) * a + jitter * (2.0 * rng.uniform(3 * n).reshape(n, 3) - 1.0)
+    pos = np.mod(pos, L)
+    species = ["A" if i < n // 2 else "B" for i in range(n)]
+    mass = np.where(np.array(species) == "A", 39.95, 83.80)
+    vel = maxwell(rng, mass, temperature)
+    eA, sA, eB, sB = 0.0104, 3.40, 0.0140, 3.65
+    eAB, sAB = float(np.sqrt(eA * eB)), 0.5 * (sA + sB)
+    rc = r2 + skin
+    inters = [
+        Interaction("lj1g", [eA, sA, r1, r2], [(1, 1, nmax, rc, period)]),
+        Interaction("lj1g", [eB, sB, r1, r2], [(2, 2, nmax, rc, period)]),
+        Interaction("lj", [eAB, sAB, r1, r2], [(1, 2, nmax, rc, period), (2, 1, nmax, rc, period)]),
+    ]
+    return System(box=np.array([L, L, L]), pos=pos, vel=vel, mass=mass, species=species,
+                  groups=[["A"], ["B"], ["A", "B"], ["#"]], interactions=inters,
+                  integrators=[Integrator("nve", dt, steps, 10**9, 10**9)], all_moving=3, xyz_moving=3, z_moving=4,
+                  all_atoms=3, traj=3, md_step_limit=steps, init_xyz_filename="lj_gas.xyz")
+
+
+def replica_mix(n_rep=512, seed=1003, n_min=256, n_max=32768, steps=2000, dt=0.5):
+    """Independent replicas of mixed size and kind: N_k = round(exp U(ln n_min,
+    ln n_max)); even k a binary LJ gas at T ~ U(50, 300) K and density
+    ~ U(0.01, 0.03) A^-3, odd k fcc Cu (RGL) of nx*ny*nz*4 ~ N_k atoms with at
+    least four cells per side (list nmax 128) at T ~ U(300, 1200) K; replica k is seeded
+    seed*10**4 + k.  All step with nve at dt.  Returns the list of Systems."""
+    rng = SplitMix64(seed)
+    u = rng.uniform(4 * n_rep).reshape(n_rep, 4)
+    out = []
+    for k in range(n_rep):
+        nk = int(round(np.exp(np.log(n_min) + u[k, 0] * (np.log(n_max) - np.log(n_min)))))
+        sk = seed * 10**4 + k
+        if k % 2 == 0:
+            t = 50.0 + 250.0 * u[k, 1]
+            rho = 0.01 + 0.02 * u[k, 2]
+            s = lj_gas_lattice(max(nk, 2), rho, sk, t, dt=dt, steps=steps)
+        else:
+            t = 300.0 + 900.0 * u[k, 1]
+            c = max(4, int(round((nk / 4.0) ** (1.0 / 3.0))))
+            nz = max(4, int(round(nk / (4.0 * c * c))))
+            # nmax 128: up to 1200 K the density fluctuations put more than
+            # 96 atoms inside the 6.5 A list cutoff
+            s = fcc_cu_rgl(nx=c, ny=c, nz=nz, seed=sk, temperature=t, steps=steps, dt=dt, nmax=128)
+        out.append(s)
+    return out

@@ -93,9 +93,15 @@ __device__ __forceinline__ void invert_v(double4& v, double z, double lo, double
 // next step (md_general.f90:342-364)
 __global__ void k_kick_drift(int N, double4* __restrict__ pos, double4* __restrict__ vel,
                              double4* __restrict__ frc, const uint32_t* __restrict__ mask, IntP P, DevErr* err,
-                             const int* __restrict__ orig, int step, uint4* __restrict__ qpos, double3 qs) {
+                             const int* __restrict__ orig, int step, uint4* __restrict__ qpos, double3 qs,
+                             const int* __restrict__ rep, const RepDev* __restrict__ rdev) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2], zlo = P.zlo, zhi = P.zhi;
+  if (rep) {
+    const RepDev& R = rdev[rep[i]];
+    Lx = R.L.x; Ly = R.L.y; Lz = R.L.z; zlo = R.zlo; zhi = R.zhi; qs = R.qs;
+  }
   double4 p = pos[i], v = vel[i];
   uint32_t m = mask[i];
   bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
@@ -106,7 +112,7 @@ __global__ void k_kick_drift(int N, double4* __restrict__ pos, double4* __restri
     if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
     if (bz) v.z = v.z + f.z * k * P.ts1;
   }
-  if (P.invert) invert_v(v, p.z, P.zlo, P.zhi);
+  if (P.invert) invert_v(v, p.z, zlo, zhi);
   if (P.scale) {
     for (int c = 0; c < P.nchain; ++c)
       if (m & P.chain_bit[c]) { double s = P.chain_s[c]; v.x *= s; v.y *= s; v.z *= s; }
@@ -116,32 +122,37 @@ __global__ void k_kick_drift(int N, double4* __restrict__ pos, double4* __restri
     if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
     if (bz) v.z = v.z + f.z * k * P.ts1;
     if (bx) {
-      p.x = wrap1(p.x + v.x * P.ts0, P.box[0]);
-      p.y = wrap1(p.y + v.y * P.ts0, P.box[1]);
-      p.z = wrap1(p.z + v.z * P.ts0, P.box[2]);
+      p.x = wrap1(p.x + v.x * P.ts0, Lx);
+      p.y = wrap1(p.y + v.y * P.ts0, Ly);
+      p.z = wrap1(p.z + v.z * P.ts0, Lz);
     }
-    if (bz) p.z = wrap1(p.z + v.z * P.ts0, P.box[2]);
+    if (bz) p.z = wrap1(p.z + v.z * P.ts0, Lz);
     pos[i] = p;
     if (qpos) qpos[i] = quantize(p, qs, 0u);
   }
   vel[i] = v;
   if (m & P.zero_bit) frc[i] = make_double4(0, 0, 0, 0);
-  if (!(in_cell(p.x, P.box[0]) && in_cell(p.y, P.box[1]) && in_cell(p.z, P.box[2])))
+  if (!(in_cell(p.x, Lx) && in_cell(p.y, Ly) && in_cell(p.z, Lz)))
     raise_err(err, PFM_E_OUT_OF_CELL, orig[i] + 1, 0, step + 1);
 }
 
 // fixed-point copy of the positions (mixed path), refreshed at the start of
 // every run so positions set from the host are picked up
-__global__ void k_quantize(int N, const double4* __restrict__ pos, uint4* __restrict__ qpos, double3 qs) {
+__global__ void k_quantize(int N, const double4* __restrict__ pos, uint4* __restrict__ qpos, double3 qs,
+                           const int* __restrict__ rep, const RepDev* __restrict__ rdev) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < N) qpos[i] = quantize(pos[i], qs, 0u);
+  if (i < N) qpos[i] = quantize(pos[i], rep ? rdev[rep[i]].qs : qs, 0u);
 }
 
 __global__ void k_check(int N, const double4* __restrict__ pos, double bx, double by, double bz, DevErr* err,
                         const int* __restrict__ orig, int step, int invert, double4* __restrict__ vel, double zlo,
-                        double zhi) {
+                        double zhi, const int* __restrict__ rep, const RepDev* __restrict__ rdev) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  if (rep) {
+    const RepDev& R = rdev[rep[i]];
+    bx = R.L.x; by = R.L.y; bz = R.L.z; zlo = R.zlo; zhi = R.zhi;
+  }
   double4 p = pos[i];
   if (!(in_cell(p.x, bx) && in_cell(p.y, by) && in_cell(p.z, bz))) raise_err(err, PFM_E_OUT_OF_CELL, orig[i] + 1, 0, step);
   if (invert) {
@@ -292,6 +303,67 @@ __global__ void k_sub_mcv(int N, double4* __restrict__ vel, const uint32_t* __re
   vel[i] = v;
 }
 
+// Batches: replica r owns internal atoms [row0[r], row0[r+1]) (slot 0 of the
+// row-offset table) and, in group g, rows [row0[g*(nrep+1)+r], ...+1); one
+// block per replica.
+
+// zero_momentum (md_general.f90:96-112) of every replica on its own
+__global__ void __launch_bounds__(RED_THREADS) k_rep_zero_mom(int nrep, const int* __restrict__ row0,
+                                                              double4* __restrict__ vel,
+                                                              const uint32_t* __restrict__ mask, uint32_t bit) {
+  __shared__ double sh[RED_THREADS];
+  const int r = blockIdx.x;
+  const int a0 = row0[r], a1 = row0[r + 1];
+  double a[4] = {0, 0, 0, 0};
+  for (int i = a0 + threadIdx.x; i < a1; i += RED_THREADS) {
+    if (mask[i] & bit) {
+      double4 v = vel[i];
+      a[0] += v.w * v.x; a[1] += v.w * v.y; a[2] += v.w * v.z; a[3] += v.w;
+    }
+  }
+  for (int q = 0; q < 4; ++q) a[q] = block_sum(a[q], sh);
+  if (a[3] == 0.0) return;
+  for (int i = a0 + threadIdx.x; i < a1; i += RED_THREADS) {
+    if (mask[i] & bit) {
+      double4 v = vel[i];
+      v.x -= a[0] / a[3]; v.y -= a[1] / a[3]; v.z -= a[2] / a[3];
+      vel[i] = v;
+    }
+  }
+}
+
+struct RepEnergyArgs {
+  int ni, nrep;
+  const double* e_row[PFM_MAXI];
+  int g1[PFM_MAXI];
+  const int* row0;
+  const double4* vel;
+  const uint32_t* mask;
+  uint32_t bit;
+  double* out;  // [nrep][ni + 1]: interaction energies, kinetic energy
+};
+
+__global__ void __launch_bounds__(RED_THREADS) k_rep_energy(RepEnergyArgs A) {
+  __shared__ double sh[RED_THREADS];
+  const int r = blockIdx.x, w = A.nrep + 1;
+  for (int k = 0; k < A.ni; ++k) {
+    const int r0 = A.row0[A.g1[k] * w + r], r1 = A.row0[A.g1[k] * w + r + 1];
+    double e = 0.0;
+    for (int q = r0 + threadIdx.x; q < r1; q += RED_THREADS) e += A.e_row[k][q];
+    e = block_sum(e, sh);
+    if (threadIdx.x == 0) A.out[r * (A.ni + 1) + k] = e;
+  }
+  double ke = 0.0;
+  for (int i = A.row0[r] + threadIdx.x; i < A.row0[r + 1]; i += RED_THREADS) {
+    if (A.mask[i] & A.bit) {
+      double4 v = A.vel[i];
+      ke += v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF;
+    }
+  }
+  ke = block_sum(ke, sh);
+  if (threadIdx.x == 0) A.out[r * (A.ni + 1) + A.ni] = ke;
+}
+
 // integrate_nose_hoover_chain (md_integrators.f90:200-245) on one chain with
 // the chain group's kinetic energy ke; returns the velocity scale s
 __device__ double nhc_update(NhcDev& h, double ke, double ts1, double ts2, double ts3) {
@@ -413,47 +485,104 @@ static bool is_cos(int k) { return k == PFM_LJC || k == PFM_MORSEC; }
 static int finalize(pfm_ctx* h) {
   if (h->sorted_once) return PFM_OK;
   int rc;
-  double rmax = 0.0;
+  const int nrep = h->nrep;
+  // replica of every original atom; atoms of every group per replica
+  std::vector<int> rep_of(h->N);
+  for (int r = 0; r < nrep; ++r)
+    for (int k = 0; k < h->rep_n[r]; ++k) rep_of[h->rep_start[r] + k] = r;
+  std::vector<std::vector<int>> gcnt(PFM_MAXG + 1, std::vector<int>(nrep, 0));
+  for (int g = 1; g <= PFM_MAXG; ++g)
+    for (int a : h->grp.idx[g]) ++gcnt[g][rep_of[a]];
+  // cell size of every replica: the largest list r_cut among the lists whose
+  // groups have atoms in it
+  std::vector<double> rmax(nrep, 0.0);
   for (auto& it : h->inter) {
     for (int k = 0; k < it.nl_n; ++k) {
       ListDev& L = it.nl[k];
       if (k == 2 && it.nn_from_tb >= 0) continue;
       double rc_eff = (k == 1 && (it.kind == PFM_LJ || is_cos(it.kind))) ? it.nl[0].rcut : L.rcut;
-      rmax = std::max(rmax, rc_eff);
+      for (int r = 0; r < nrep; ++r)
+        if (gcnt[L.g1][r] > 0 || gcnt[L.g2][r] > 0) rmax[r] = std::max(rmax[r], rc_eff);
     }
   }
-  if (rmax <= 0.0) rmax = 1.0;
-  for (int d = 0; d < 3; ++d) h->nc[d] = std::max(1, std::min((int)std::floor(h->box[d] / rmax), 1024));
-  // at most 2^21 cells: larger cells only add candidates
-  while ((long)h->nc[0] * h->nc[1] * h->nc[2] > (1L << 21)) {
-    int d = (h->nc[0] >= h->nc[1] && h->nc[0] >= h->nc[2]) ? 0 : (h->nc[1] >= h->nc[2] ? 1 : 2);
-    h->nc[d] = std::max(1, h->nc[d] / 2);
+  // grids, Morton ranks offset by each replica's first cell
+  h->hrep.assign(nrep, RepDev());
+  std::vector<int> rank_all, xyz_all, crep_all;
+  h->ncell = 0;
+  h->any_tile = h->any_rows = false;
+  for (int r = 0; r < nrep; ++r) {
+    RepDev& R = h->hrep[r];
+    memset(&R, 0, sizeof R);
+    const double* bx = &h->rep_box[3 * r];
+    double rm = rmax[r] > 0.0 ? rmax[r] : 1.0;
+    int nc[3];
+    for (int d = 0; d < 3; ++d) nc[d] = std::max(1, std::min((int)std::floor(bx[d] / rm), 1024));
+    // at most 2^21 cells: larger cells only add candidates
+    while ((long)nc[0] * nc[1] * nc[2] > (1L << 21)) {
+      int d = (nc[0] >= nc[1] && nc[0] >= nc[2]) ? 0 : (nc[1] >= nc[2] ? 1 : 2);
+      nc[d] = std::max(1, nc[d] / 2);
+    }
+    R.L = make_double3(bx[0], bx[1], bx[2]);
+    R.iL = make_double3(1.0 / bx[0], 1.0 / bx[1], 1.0 / bx[2]);
+    R.hb = make_double3(0.5 * bx[0], 0.5 * bx[1], 0.5 * bx[2]);
+    R.qs = make_double3(4294967296.0 / bx[0], 4294967296.0 / bx[1], 4294967296.0 / bx[2]);
+    R.hq = make_float3((float)(bx[0] / 4294967296.0), (float)(bx[1] / 4294967296.0), (float)(bx[2] / 4294967296.0));
+    double cl[3];
+    for (int d = 0; d < 3; ++d) {
+      R.nc[d] = nc[d];
+      cl[d] = bx[d] / nc[d];
+    }
+    R.clen = make_double3(cl[0], cl[1], cl[2]);
+    R.inv_clen = make_double3(1.0 / cl[0], 1.0 / cl[1], 1.0 / cl[2]);
+    R.cell_base = h->ncell;
+    R.tile = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
+    R.zlo = 0.8 * bx[2];
+    R.zhi = 0.9 * bx[2];
+    (R.tile ? h->any_tile : h->any_rows) = true;
+    const int ncr = nc[0] * nc[1] * nc[2];
+    std::vector<std::pair<uint64_t, int>> key(ncr);
+    for (int z = 0; z < nc[2]; ++z)
+      for (int y = 0; y < nc[1]; ++y)
+        for (int x = 0; x < nc[0]; ++x) {
+          int lin = (z * nc[1] + y) * nc[0] + x;
+          key[lin] = {morton3(x, y, z), lin};
+        }
+    std::sort(key.begin(), key.end());
+    rank_all.resize(h->ncell + ncr);
+    xyz_all.resize(h->ncell + ncr);
+    crep_all.resize(h->ncell + ncr);
+    for (int k = 0; k < ncr; ++k) {
+      int lin = key[k].second, x = lin % nc[0], y = (lin / nc[0]) % nc[1], z = lin / (nc[0] * nc[1]);
+      rank_all[h->ncell + lin] = h->ncell + k;
+      xyz_all[h->ncell + k] = x | (y << 10) | (z << 20);
+      crep_all[h->ncell + k] = r;
+    }
+    h->ncell += ncr;
   }
-  h->ncell = 1;
   for (int d = 0; d < 3; ++d) {
-    h->cell_len[d] = h->box[d] / h->nc[d];
-    h->ncell *= h->nc[d];
+    h->nc[d] = h->hrep[0].nc[d];
+    h->cell_len[d] = h->rep_box[d] / h->nc[d];
   }
-  // Morton rank of every cell
-  std::vector<std::pair<uint64_t, int>> key(h->ncell);
-  for (int z = 0; z < h->nc[2]; ++z)
-    for (int y = 0; y < h->nc[1]; ++y)
-      for (int x = 0; x < h->nc[0]; ++x) {
-        int lin = (z * h->nc[1] + y) * h->nc[0] + x;
-        key[lin] = {morton3(x, y, z), lin};
-      }
-  std::sort(key.begin(), key.end());
-  std::vector<int> rank(h->ncell);
-  for (int r = 0; r < h->ncell; ++r) rank[key[r].second] = r;
+  if ((rc = dalloc(h, &h->rdev, nrep))) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->rdev, h->hrep.data(), sizeof(RepDev) * nrep, hipMemcpyHostToDevice, h->stream));
   if ((rc = dalloc(h, &h->cell_rank, h->ncell))) return rc;
-  HIPCHK(h, hipMemcpyAsync(h->cell_rank, rank.data(), sizeof(int) * h->ncell, hipMemcpyHostToDevice, h->stream));
-  std::vector<int> xyz(h->ncell);
-  for (int r = 0; r < h->ncell; ++r) {
-    int lin = key[r].second, x = lin % h->nc[0], y = (lin / h->nc[0]) % h->nc[1], z = lin / (h->nc[0] * h->nc[1]);
-    xyz[r] = x | (y << 10) | (z << 20);
-  }
+  HIPCHK(h, hipMemcpyAsync(h->cell_rank, rank_all.data(), sizeof(int) * h->ncell, hipMemcpyHostToDevice, h->stream));
   if ((rc = dalloc(h, &h->cell_xyz, h->ncell))) return rc;
-  HIPCHK(h, hipMemcpyAsync(h->cell_xyz, xyz.data(), sizeof(int) * h->ncell, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->cell_xyz, xyz_all.data(), sizeof(int) * h->ncell, hipMemcpyHostToDevice, h->stream));
+  if (nrep > 1) {
+    if ((rc = dalloc(h, &h->cell_rep, h->ncell))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->cell_rep, crep_all.data(), sizeof(int) * h->ncell, hipMemcpyHostToDevice, h->stream));
+  }
+  // rows of every group: replica r owns rows [row0[r], row0[r+1])
+  std::vector<int> row0((size_t)(PFM_MAXG + 1) * (nrep + 1), 0);
+  for (int r = 0; r < nrep; ++r) row0[r + 1] = h->rep_start[r] + h->rep_n[r];
+  for (int g = 1; g <= PFM_MAXG; ++g) {
+    h->rep_row0[g].assign(nrep + 1, 0);
+    for (int r = 0; r < nrep; ++r) h->rep_row0[g][r + 1] = h->rep_row0[g][r] + gcnt[g][r];
+    std::copy(h->rep_row0[g].begin(), h->rep_row0[g].end(), row0.begin() + (size_t)g * (nrep + 1));
+  }
+  if ((rc = dalloc(h, &h->d_rep_row0, row0.size()))) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->d_rep_row0, row0.data(), sizeof(int) * row0.size(), hipMemcpyHostToDevice, h->stream));
   if ((rc = dalloc(h, &h->cell_count, 2 * (size_t)h->ncell))) return rc;  // counts, then fill counters
   h->cell_fill = h->cell_count + h->ncell;
   if ((rc = dalloc(h, &h->cell_start, h->ncell))) return rc;
@@ -616,6 +745,43 @@ static int energies(pfm_ctx* h, bool nvt) {
   h->pe = pe;
   h->ke = h->h_red[32 + ni];
   h->temp = 2 * h->ke / PFM_KB / (3 * h->grp.n[h->roles[0]]);
+  if (h->nrep > 1) {
+    const int nrep = h->nrep;
+    if (!h->d_rep_out) {
+      int rc;
+      if ((rc = dalloc(h, &h->d_rep_out, (size_t)nrep * (ni + 1)))) return rc;
+    }
+    RepEnergyArgs A;
+    memset(&A, 0, sizeof A);
+    A.ni = ni;
+    A.nrep = nrep;
+    for (int k = 0; k < ni; ++k) {
+      A.e_row[k] = h->inter[k].e_row;
+      A.g1[k] = h->inter[k].nl[0].g1;
+    }
+    A.row0 = h->d_rep_row0;
+    A.vel = h->vel4;
+    A.mask = h->mask;
+    A.bit = 1u << (h->roles[0] - 1);
+    A.out = h->d_rep_out;
+    PFM_LAUNCH(h, "energy", k_rep_energy, nrep, RED_THREADS, A);
+    std::vector<double> o((size_t)nrep * (ni + 1));
+    HIPCHK(h, hipMemcpyAsync(o.data(), h->d_rep_out, sizeof(double) * o.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->rep_e.assign(o.begin(), o.end());
+    h->rep_pe.assign(nrep, 0.0);
+    h->rep_ke.assign(nrep, 0.0);
+    h->rep_temp.assign(nrep, 0.0);
+    const std::vector<int>& m0 = h->rep_row0[h->roles[0]];
+    for (int r = 0; r < nrep; ++r) {
+      double p = 0.0;
+      for (int k = 0; k < ni; ++k) p += o[(size_t)r * (ni + 1) + k];
+      h->rep_pe[r] = p;
+      h->rep_ke[r] = o[(size_t)r * (ni + 1) + ni];
+      int nm = m0[r + 1] - m0[r];
+      h->rep_temp[r] = nm > 0 ? 2 * h->rep_ke[r] / PFM_KB / (3 * nm) : 0.0;
+    }
+  }
   for (size_t c = 0; c < cs.size(); ++c) {
     // calculate_nose_hoover_chain_energy (md_integrators.f90:247-260)
     const NhcDev& d = cs[c];
@@ -639,10 +805,19 @@ static int update_lists(pfm_ctx* h, int s) {
     }
   int rc;
   if (all_due || !h->sorted_once) {
-    if ((rc = launch_sort_and_bin(h, s))) return rc;
+    if ((rc = launch_sort_and_bin(h, s, true))) return rc;
     h->sorted_once = true;
     for (auto& it : h->inter)
       for (int k = 0; k < it.nl_n; ++k) it.nl[k].built = false;
+  } else {
+    // some lists are due, others keep their entries: bin the atoms at their
+    // current positions without moving them
+    bool any_due = false;
+    for (auto& it : h->inter) {
+      any_due = any_due || s % it.nl[0].period == 0;
+      if (is_cos(it.kind) && it.nn_from_tb < 0) any_due = any_due || s % it.nl[2].period == 0;
+    }
+    if (any_due && (rc = launch_sort_and_bin(h, s, false))) return rc;
   }
   for (size_t ii = 0; ii < h->inter.size(); ++ii) {
     Inter& it = h->inter[ii];
@@ -669,6 +844,10 @@ static int zero_momentum(pfm_ctx* h) {
   int nb = h->red_blocks, g = h->roles[3];
   if (g == 0 || h->grp.n[g] == 0) return PFM_OK;
   uint32_t bit = 1u << (g - 1);
+  if (h->nrep > 1) {
+    PFM_LAUNCH(h, "zero_momentum", k_rep_zero_mom, h->nrep, RED_THREADS, h->nrep, h->d_rep_row0, h->vel4, h->mask, bit);
+    return PFM_OK;
+  }
   PFM_LAUNCH(h, "zero_momentum", k_mom_part, nb, RED_THREADS, h->N, h->vel4, h->mask, bit, h->red_part);
   PFM_LAUNCH(h, "zero_momentum", k_reduce_rows, 1, RED_THREADS, 4, nb, h->red_part, h->red_out);
   PFM_LAUNCH(h, "zero_momentum", k_sub_mcv, (h->N + 255) / 256, 256, h->N, h->vel4, h->mask, bit, h->red_out);
@@ -681,9 +860,27 @@ extern "C" {
 
 int pfm_create(pfm_ctx** out, int n, const double box[3], const double* pos, const double* vel, const double* mass,
                int precision) {
+  return pfm_create_batch(out, 1, &n, box, pos, vel, mass, precision);
+}
+
+int pfm_create_batch(pfm_ctx** out, int nrep, const int* n_atoms, const double* boxes, const double* pos,
+                     const double* vel, const double* mass, int precision) {
   pfm_ctx* h = new pfm_ctx();
   *out = h;
-  if (n <= 0) return fail(h, PFM_E_INVALID, "error: particles_num must be positive");
+  if (nrep <= 0) return fail(h, PFM_E_INVALID, "error: a batch needs at least one replica");
+  long nl = 0;
+  for (int r = 0; r < nrep; ++r) {
+    if (n_atoms[r] <= 0) return fail(h, PFM_E_INVALID, "error: particles_num must be positive");
+    for (int d = 0; d < 3; ++d)
+      if (!(boxes[3 * r + d] > 0)) return fail(h, PFM_E_INVALID, "error: box size must be positive");
+    h->rep_start.push_back((int)nl);
+    h->rep_n.push_back(n_atoms[r]);
+    nl += n_atoms[r];
+  }
+  if (nl > (1L << 30)) return fail(h, PFM_E_INVALID, "error: too many atoms in one context");
+  const int n = (int)nl;
+  h->nrep = nrep;
+  h->rep_box.assign(boxes, boxes + 3 * nrep);
   if (precision != PFM_FP64 && precision != PFM_MIXED) return fail(h, PFM_E_INVALID, "error: unknown precision");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -697,9 +894,8 @@ int pfm_create(pfm_ctx** out, int n, const double box[3], const double* pos, con
   h->N = n;
   h->precision = precision;
   for (int d = 0; d < 3; ++d) {
-    h->box[d] = box[d];
-    h->hbox[d] = 0.5 * box[d];
-    if (!(box[d] > 0)) return fail(h, PFM_E_INVALID, "error: box size must be positive");
+    h->box[d] = boxes[d];
+    h->hbox[d] = 0.5 * boxes[d];
   }
   int rc;
   if ((rc = dalloc(h, &h->pos4, n)) || (rc = dalloc(h, &h->vel4, n)) || (rc = dalloc(h, &h->frc4, n)) ||
@@ -708,7 +904,7 @@ int pfm_create(pfm_ctx** out, int n, const double box[3], const double* pos, con
       (rc = dalloc(h, &h->origb, n)) || (rc = dalloc(h, &h->perm, n)) || (rc = dalloc(h, &h->atom_rank, n)) ||
       (rc = dalloc(h, &h->scan_tmp, 3 * (size_t)n + 4096)) || (rc = dalloc(h, &h->eb, n)))
     return rc;
-  for (int d = 0; d < 3; ++d) (&h->qscale.x)[d] = 4294967296.0 / box[d];
+  for (int d = 0; d < 3; ++d) (&h->qscale.x)[d] = 4294967296.0 / boxes[d];
   if (precision == PFM_MIXED && (rc = dalloc(h, &h->qpos, n))) return rc;
   h->red_blocks = std::max(1, std::min(1024, (n + RED_THREADS - 1) / RED_THREADS));
   if ((rc = dalloc(h, &h->red_part, (size_t)(PFM_MAXI + 1 + RED_MAXQ) * h->red_blocks))) return rc;
@@ -727,6 +923,13 @@ int pfm_create(pfm_ctx** out, int n, const double box[3], const double* pos, con
   HIPCHK(h, hipMemcpyAsync(h->pos4, p.data(), sizeof(double4) * n, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->vel4, v.data(), sizeof(double4) * n, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->orig, o.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+  if (nrep > 1) {
+    std::vector<int> rp(n);
+    for (int r = 0; r < nrep; ++r)
+      for (int k = 0; k < h->rep_n[r]; ++k) rp[h->rep_start[r] + k] = r;
+    if ((rc = dalloc(h, &h->rep, n)) || (rc = dalloc(h, &h->repb, n))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->rep, rp.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+  }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return PFM_OK;
 }
@@ -840,6 +1043,8 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
             int* event) {
   if (integ < PFM_NVT || integ > PFM_NVMS) return fail(h, PFM_E_INVALID, "error: unknown integrator");
   if (h->roles[0] == 0) return fail(h, PFM_E_INVALID, "error: pfm_set_roles was not called");
+  if (h->nrep > 1 && integ == PFM_NVT && !h->nhc.empty())
+    return fail(h, PFM_E_INVALID, "error: Nose-Hoover chains span replicas; run batches with nve or nvms");
   for (int k = 0; k < 4; ++k) h->grp.used[h->roles[k]] = true;
   int rc;
   if (!h->sorted_once && (rc = finalize(h))) return rc;
@@ -857,9 +1062,9 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
   }
   // check_positions (and invert_z) of the first step of the call
   PFM_LAUNCH(h, "check", k_check, nb, 256, N, h->pos4, h->box[0], h->box[1], h->box[2], h->derr, h->orig, s0,
-             (s0 == 0 && h->invert_z) ? 1 : 0, h->vel4, P.zlo, P.zhi);
+             (s0 == 0 && h->invert_z) ? 1 : 0, h->vel4, P.zlo, P.zhi, h->rep, h->rdev);
   if ((rc = check_err(h, s1))) return rc;
-  if (h->qpos) PFM_LAUNCH(h, "quantize", k_quantize, nb, 256, N, h->pos4, h->qpos, h->qscale);
+  if (h->qpos) PFM_LAUNCH(h, "quantize", k_quantize, nb, 256, N, h->pos4, h->qpos, h->qscale, h->rep, h->rdev);
   const bool nvt = integ == PFM_NVT && !h->nhc.empty();
   bool nhc_pending = false;
   bool kick_pending = false;  // second half kick deferred into the next kick_drift
@@ -867,7 +1072,13 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
     *last_step = s;
     bool energy = (s == s1 && energy_at_end) || integ == PFM_NVMS;
     if (s != 0) {
-      if (integ == PFM_NVMS && std::fabs(h->pe - h->pe_prev) <= ms_de) {
+      bool conv = std::fabs(h->pe - h->pe_prev) <= ms_de;
+      if (h->nrep > 1 && integ == PFM_NVMS) {
+        // a batch stops when every replica meets the criterion
+        conv = h->rep_pe.size() == (size_t)h->nrep && h->rep_pe_prev.size() == (size_t)h->nrep;
+        for (int r = 0; conv && r < h->nrep; ++r) conv = std::fabs(h->rep_pe[r] - h->rep_pe_prev[r]) <= ms_de;
+      }
+      if (integ == PFM_NVMS && conv) {
         *event = PFM_EV_CONVERGED;
         return PFM_OK;
       }
@@ -882,11 +1093,12 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
         nhc_pending = false;
       }
       PFM_LAUNCH(h, "kick_drift", k_kick_drift, nb, 256, N, h->pos4, h->vel4, h->frc4, h->mask, Q, h->derr, h->orig, s,
-                 h->qpos, h->qscale);
+                 h->qpos, h->qscale, h->rep, h->rdev);
     } else if (!first_writes) {
       PFM_LAUNCH(h, "zero_forces", k_zero_masked, nb, 256, N, h->frc4, h->mask, all_bit);
     }
     h->pe_prev = h->pe;
+    h->rep_pe_prev = h->rep_pe;
     if ((rc = update_lists(h, s))) return rc;
     if (s % h->zm_period == 0 && (rc = zero_momentum(h))) return rc;
     if ((rc = launch_forces(h, energy, first_writes))) return rc;
@@ -915,6 +1127,27 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
     }
   }
   return check_err(h, s1);
+}
+
+int pfm_replica_energies(pfm_ctx* h, double* e_inter, double* pe, double* ke, double* temp) {
+  const int nrep = h->nrep, ni = (int)h->inter.size();
+  if (nrep == 1) {
+    if (e_inter)
+      for (int k = 0; k < ni; ++k) e_inter[k] = h->inter[k].energy;
+    if (pe) *pe = h->pe;
+    if (ke) *ke = h->ke;
+    if (temp) *temp = h->temp;
+    return PFM_OK;
+  }
+  if (h->rep_pe.size() != (size_t)nrep) return fail(h, PFM_E_INVALID, "error: no energies evaluated yet");
+  for (int r = 0; r < nrep; ++r) {
+    if (e_inter)
+      for (int k = 0; k < ni; ++k) e_inter[(size_t)r * ni + k] = h->rep_e[(size_t)r * (ni + 1) + k];
+    if (pe) pe[r] = h->rep_pe[r];
+    if (ke) ke[r] = h->rep_ke[r];
+    if (temp) temp[r] = h->rep_temp[r];
+  }
+  return PFM_OK;
 }
 
 int pfm_energies(pfm_ctx* h, double* e_inter, double* pe, double* ke, double* temp, double* e_nhc, double* t) {
@@ -1090,7 +1323,7 @@ int pfm_destroy(pfm_ctx* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(h->pos4); F(h->vel4); F(h->frc4); F(h->pos4b); F(h->vel4b); F(h->frc4b); F(h->mask); F(h->maskb);
-  F(h->orig); F(h->origb); F(h->perm); F(h->atom_rank); F(h->scan_tmp); F(h->eb); F(h->qpos);
+  F(h->orig); F(h->origb); F(h->perm); F(h->atom_rank); F(h->scan_tmp); F(h->eb); F(h->qpos); F(h->rdev); F(h->rep); F(h->repb); F(h->cell_rep); F(h->d_rep_row0); F(h->d_rep_out);
   F(h->red_part); F(h->red_out); F(h->derr); F(h->cell_rank); F(h->cell_count); F(h->cell_start); F(h->cell_xyz);
   F(h->nhc_state);
   if (h->h_red) (void)hipHostFree(h->h_red);

@@ -28,14 +28,27 @@ struct RowArgs {
   const uint4* qpos;   // fixed-point positions (mixed path)
   float3 hq;           // L / 2^32 per axis
   double3 hb, L, iL;
+  const int* rep;      // replica of every internal atom (null: one replica, geometry above)
+  const RepDev* rdev;
   double4* frc;
   int accumulate;  // 0: this pass writes the first contribution of the step
   double* e_row;
 };
 
-template <class R> __device__ __forceinline__ Geo<R> geo_of(const RowArgs& a);
-template <> __device__ __forceinline__ Geo<double> geo_of<double>(const RowArgs& a) { return Geo<double>{a.pos, a.L, a.iL}; }
-template <> __device__ __forceinline__ Geo<float> geo_of<float>(const RowArgs& a) {
+// geometry of the replica that holds atom i
+template <class R> __device__ __forceinline__ Geo<R> geo_of(const RowArgs& a, int i);
+template <> __device__ __forceinline__ Geo<double> geo_of<double>(const RowArgs& a, int i) {
+  if (a.rep) {
+    const RepDev& R = a.rdev[a.rep[i]];
+    return Geo<double>{a.pos, R.L, R.iL};
+  }
+  return Geo<double>{a.pos, a.L, a.iL};
+}
+template <> __device__ __forceinline__ Geo<float> geo_of<float>(const RowArgs& a, int i) {
+  if (a.rep) {
+    const float3 hq = a.rdev[a.rep[i]].hq;
+    return Geo<float>{a.qpos, hq.x, hq.y, hq.z};
+  }
   return Geo<float>{a.qpos, a.hq.x, a.hq.y, a.hq.z};
 }
 
@@ -56,7 +69,7 @@ template <> __device__ __forceinline__ void store_w<float>(const RowArgs& a, int
   int row = gt / (T), t = threadIdx.x & ((T)-1);          \
   if (row >= a.nrows) return;                             \
   int i = a.rows[row];                                    \
-  const Geo<R> G = geo_of<R>(a);                          \
+  const Geo<R> G = geo_of<R>(a, i);                       \
   const typename Geo<R>::P pi = G.at(i);                  \
   int n = a.nnum[row];                                    \
   const int* nbrow = a.nbr + (size_t)row * a.stride;
@@ -508,7 +521,7 @@ __device__ __forceinline__ void cos_nn_rows(const RowArgs& a, const CosP& P, con
     if (jr < 0 || l1 < 0 || l1 > 2) continue;  // reported by k_rev / k_nn_from_tb
     double4 pj = a.pos[j];
     double d[3][3];
-    for (int k = 0; k < 3; ++k) fast_dist2(pj, a.pos[a.nbr[(size_t)jr * 3 + k]], a.L, a.iL, d[k][0], d[k][1], d[k][2]);
+    for (int k = 0; k < 3; ++k) G.d2(pj, G.at(a.nbr[(size_t)jr * 3 + k]), d[k][0], d[k][1], d[k][2]);
     int l2 = (l1 + 1) % 3, l3 = (l1 + 2) % 3;
     double d12[3], d31[3], d23[3];
     for (int k = 0; k < 3; ++k) {
@@ -601,6 +614,8 @@ static RowArgs row_args(pfm_ctx* h, Inter& it, int k, int accumulate) {
   a.hb = make_double3(h->hbox[0], h->hbox[1], h->hbox[2]);
   a.L = make_double3(h->box[0], h->box[1], h->box[2]);
   a.iL = make_double3(1.0 / h->box[0], 1.0 / h->box[1], 1.0 / h->box[2]);
+  a.rep = h->rep;
+  a.rdev = h->rdev;
   a.frc = h->frc4;
   a.accumulate = accumulate;
   a.e_row = it.e_row;

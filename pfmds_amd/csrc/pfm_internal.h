@@ -188,6 +188,20 @@ __device__ __forceinline__ void raise_err(DevErr* e, int code, int atom, int cou
   }
 }
 
+// Geometry of one replica of a batch (a single system is a batch of one):
+// box, minimum-image and fixed-point scales, its cell grid and the first
+// global cell rank of that grid.
+struct RepDev {
+  double3 L, iL, hb;    // box, 1/box, box/2
+  double3 qs;           // 2^32 / L
+  float3 hq;            // L / 2^32
+  double3 clen, inv_clen;
+  int nc[3];
+  int cell_base;        // global rank (and linear index) of the grid's first cell
+  int tile;             // grid at least three cells wide in every direction
+  double zlo, zhi;      // invert_z_velocities band
+};
+
 // ------------------------------------------------------------------ host side
 
 struct ListDev {
@@ -240,7 +254,21 @@ struct Groups {
 
 struct pfm_ctx {
   int N = 0, precision = PFM_FP64, dev = 0;
-  double box[3], hbox[3];
+  double box[3], hbox[3];  // replica 0 (the only one of a single system)
+  // batch of independent replicas: atoms concatenated in original order
+  int nrep = 1;
+  std::vector<int> rep_n, rep_start;  // atoms and first original atom of every replica
+  std::vector<double> rep_box;        // 3 per replica
+  std::vector<RepDev> hrep;           // host copy of rdev
+  RepDev* rdev = nullptr;             // [nrep]
+  int *rep = nullptr, *repb = nullptr;  // [N] replica of every internal atom (batches only)
+  int* cell_rep = nullptr;            // [ncell] replica of every cell (batches only)
+  bool any_tile = false, any_rows = false;  // some replica searched by cell tiles / by rows
+  // per replica and group: first row of the replica in the group's rows
+  std::vector<int> rep_row0[PFM_MAXG + 1];
+  int* d_rep_row0 = nullptr;  // [(PFM_MAXG + 1) * (nrep + 1)]; slot 0: first internal atom of every replica
+  double* d_rep_out = nullptr;  // [nrep * (n_interactions + 1)] per-replica energies
+  std::vector<double> rep_pe, rep_pe_prev, rep_ke, rep_temp, rep_e;  // per-replica energies (batches)
   hipStream_t stream = nullptr;
   // atom state (internal order) and the spare buffers of the re-sort
   double4 *pos4 = nullptr, *vel4 = nullptr, *frc4 = nullptr;
@@ -248,6 +276,7 @@ struct pfm_ctx {
   uint32_t *mask = nullptr, *maskb = nullptr;
   int *orig = nullptr, *origb = nullptr;
   int* perm = nullptr;
+  const int* bin = nullptr;  // cell-ordered atoms of the last binning when the data was not permuted
   uint32_t host_mask_of_group[PFM_MAXG + 1];
   std::vector<uint32_t> hmask;
   std::vector<double> hmass;
@@ -306,7 +335,7 @@ void tim_end(pfm_ctx* h, hipEvent_t a, hipEvent_t b, const char* name);
 static inline int pad8(int nb) { return ((nb + 7) / 8) * 8; }
 
 // kernels' host launchers
-int launch_sort_and_bin(pfm_ctx* h, int step);
+int launch_sort_and_bin(pfm_ctx* h, int step, bool permute);
 int launch_build_list(pfm_ctx* h, int ii, int k, int step);
 int launch_nn_from_tb(pfm_ctx* h, int ii, int step);
 int launch_forces(pfm_ctx* h, bool energy, bool first_writes);

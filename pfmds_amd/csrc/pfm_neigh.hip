@@ -13,11 +13,6 @@
 
 // ------------------------------------------------------------------ binning
 
-struct CellGrid {
-  int nc[3];
-  double inv_len[3];
-};
-
 __device__ __forceinline__ int cell_coord(double x, double inv_len, int nc) {
   int c = (int)floor(x * inv_len);
   return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
@@ -28,15 +23,17 @@ __global__ void k_fill_int(int n, int* __restrict__ x, int v) {
   if (i < n) x[i] = v;
 }
 
-__global__ void k_cell_keys(int N, const double4* __restrict__ pos, CellGrid g, const int* __restrict__ cell_rank,
+__global__ void k_cell_keys(int N, const double4* __restrict__ pos, const int* __restrict__ rep,
+                            const RepDev* __restrict__ rdev, const int* __restrict__ cell_rank,
                             int* __restrict__ atom_rank, int* __restrict__ cell_count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const RepDev& R = rdev[rep ? rep[i] : 0];
   double4 p = pos[i];
-  int cx = cell_coord(p.x, g.inv_len[0], g.nc[0]);
-  int cy = cell_coord(p.y, g.inv_len[1], g.nc[1]);
-  int cz = cell_coord(p.z, g.inv_len[2], g.nc[2]);
-  int r = cell_rank[(cz * g.nc[1] + cy) * g.nc[0] + cx];
+  int cx = cell_coord(p.x, R.inv_clen.x, R.nc[0]);
+  int cy = cell_coord(p.y, R.inv_clen.y, R.nc[1]);
+  int cz = cell_coord(p.z, R.inv_clen.z, R.nc[2]);
+  int r = cell_rank[R.cell_base + (cz * R.nc[1] + cy) * R.nc[0] + cx];
   atom_rank[i] = r;
   atomicAdd(&cell_count[r], 1);
 }
@@ -157,46 +154,48 @@ __global__ void k_permute(int N, const int* __restrict__ perm, const double4* __
                           const double4* __restrict__ v0, const double4* __restrict__ f0,
                           const uint32_t* __restrict__ m0, const int* __restrict__ o0, double4* __restrict__ p1,
                           double4* __restrict__ v1, double4* __restrict__ f1, uint32_t* __restrict__ m1,
-                          int* __restrict__ o1, uint4* __restrict__ q1, double3 qs) {
+                          int* __restrict__ o1, uint4* __restrict__ q1, const int* __restrict__ r0,
+                          int* __restrict__ r1, const RepDev* __restrict__ rdev) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= N) return;
   int i = perm[k];
   double4 p = p0[i];
   p1[k] = p;
-  if (q1) q1[k] = quantize(p, qs, 0u);
+  int r = 0;
+  if (r0) { r = r0[i]; r1[k] = r; }
+  if (q1) q1[k] = quantize(p, rdev[r].qs, 0u);
   v1[k] = v0[i];
   f1[k] = f0[i];
   m1[k] = m0[i];
   o1[k] = o0[i];
 }
 
-static CellGrid make_grid(pfm_ctx* h) {
-  CellGrid g;
-  for (int d = 0; d < 3; ++d) {
-    g.nc[d] = h->nc[d];
-    g.inv_len[d] = 1.0 / h->cell_len[d];
-  }
-  return g;
-}
-
 // Bin atoms into cells and re-order every per-atom array in Morton cell order.
-int launch_sort_and_bin(pfm_ctx* h, int step) {
+int launch_sort_and_bin(pfm_ctx* h, int step, bool permute) {
   (void)step;
   int N = h->N, nb = (N + 255) / 256;
-  CellGrid g = make_grid(h);
   PFM_LAUNCH(h, "cell_keys", k_fill_int, (2 * h->ncell + 255) / 256, 256, 2 * h->ncell, h->cell_count, 0);
-  PFM_LAUNCH(h, "cell_keys", k_cell_keys, nb, 256, N, h->pos4, g, h->cell_rank, h->atom_rank, h->cell_count);
+  PFM_LAUNCH(h, "cell_keys", k_cell_keys, nb, 256, N, h->pos4, h->rep, h->rdev, h->cell_rank, h->atom_rank,
+             h->cell_count);
   scan_excl(h, h->ncell, h->cell_count, h->cell_start, h->scan_tmp);
   PFM_LAUNCH(h, "scatter", k_scatter, nb, 256, N, h->atom_rank, h->cell_start, h->cell_fill, h->scan_tmp + N);
   PFM_LAUNCH(h, "cell_sort", k_cell_sort, h->ncell, 64, h->ncell, h->cell_start, h->cell_count, h->scan_tmp + N,
              h->perm, h->orig);
+  if (!permute) {
+    // lists that are not due keep their internal indices: the search reads
+    // the atoms of every cell through the cell-ordered index
+    h->bin = h->perm;
+    return PFM_OK;
+  }
   PFM_LAUNCH(h, "permute", k_permute, nb, 256, N, h->perm, h->pos4, h->vel4, h->frc4, h->mask, h->orig, h->pos4b,
-             h->vel4b, h->frc4b, h->maskb, h->origb, h->qpos, h->qscale);
+             h->vel4b, h->frc4b, h->maskb, h->origb, h->qpos, h->rep, h->repb, h->rdev);
   std::swap(h->pos4, h->pos4b);
   std::swap(h->vel4, h->vel4b);
   std::swap(h->frc4, h->frc4b);
   std::swap(h->mask, h->maskb);
   std::swap(h->orig, h->origb);
+  std::swap(h->rep, h->repb);
+  h->bin = nullptr;  // atoms are in cell order
   launch_group_rows(h);
   return PFM_OK;
 }
@@ -250,9 +249,11 @@ struct SearchArgs {
   const double4* pos;
   const uint32_t* mask;
   const int* orig;
-  double3 hb, L, iL, clen;
+  const RepDev* rdev;   // replica geometry
+  const int* rep;       // replica of every internal atom (null: one replica)
+  const int* cell_rep;  // replica of every cell (null: one replica)
+  const int* bin;       // atoms of the cells in cell order (null: atoms are stored in cell order)
   double rc2, rc2_lo, rc2_hi;
-  CellGrid g;
   const int* cell_rank;
   const int* cell_start;
   const int* cell_count;
@@ -278,43 +279,45 @@ __global__ void __launch_bounds__(256) k_build(SearchArgs a, int nblocks_real) {
   unsigned long long gmask = (T == 64) ? ~0ull : (((1ull << T) - 1ull) << gshift);
   unsigned long long lower = (1ull << lane) - 1ull;
   int i = a.rows[row];
+  const RepDev R = a.rdev[a.rep ? a.rep[i] : 0];
+  if (R.tile) return;  // searched by k_build_tile
   double4 pi = a.pos[i];
   int c[3];
-  c[0] = cell_coord(pi.x, a.g.inv_len[0], a.g.nc[0]);
-  c[1] = cell_coord(pi.y, a.g.inv_len[1], a.g.nc[1]);
-  c[2] = cell_coord(pi.z, a.g.inv_len[2], a.g.nc[2]);
+  c[0] = cell_coord(pi.x, R.inv_clen.x, R.nc[0]);
+  c[1] = cell_coord(pi.y, R.inv_clen.y, R.nc[1]);
+  c[2] = cell_coord(pi.z, R.inv_clen.z, R.nc[2]);
   int lo[3], cnt[3];
   for (int d = 0; d < 3; ++d) {
-    if (a.g.nc[d] >= 3) { lo[d] = c[d] - 1; cnt[d] = 3; }
-    else { lo[d] = 0; cnt[d] = a.g.nc[d]; }
+    if (R.nc[d] >= 3) { lo[d] = c[d] - 1; cnt[d] = 3; }
+    else { lo[d] = 0; cnt[d] = R.nc[d]; }
   }
   int count = 0;
   int* out = a.nbr + (size_t)row * a.stride;
   for (int oz = 0; oz < cnt[2]; ++oz) {
-    int z = lo[2] + oz; z = z < 0 ? z + a.g.nc[2] : (z >= a.g.nc[2] ? z - a.g.nc[2] : z);
+    int z = lo[2] + oz; z = z < 0 ? z + R.nc[2] : (z >= R.nc[2] ? z - R.nc[2] : z);
     for (int oy = 0; oy < cnt[1]; ++oy) {
-      int y = lo[1] + oy; y = y < 0 ? y + a.g.nc[1] : (y >= a.g.nc[1] ? y - a.g.nc[1] : y);
+      int y = lo[1] + oy; y = y < 0 ? y + R.nc[1] : (y >= R.nc[1] ? y - R.nc[1] : y);
       for (int ox = 0; ox < cnt[0]; ++ox) {
-        int x = lo[0] + ox; x = x < 0 ? x + a.g.nc[0] : (x >= a.g.nc[0] ? x - a.g.nc[0] : x);
-        int r = a.cell_rank[(z * a.g.nc[1] + y) * a.g.nc[0] + x];
+        int x = lo[0] + ox; x = x < 0 ? x + R.nc[0] : (x >= R.nc[0] ? x - R.nc[0] : x);
+        int r = a.cell_rank[R.cell_base + (z * R.nc[1] + y) * R.nc[0] + x];
         int s = a.cell_start[r], n = a.cell_count[r];
         for (int base = 0; base < n; base += T) {
           int k = base + t;
           bool pred = false;
           int j = 0;
           if (k < n) {
-            j = s + k;
+            j = a.bin ? a.bin[s + k] : s + k;
             if (j != i && (a.mask[j] & a.g2bit)) {
               double4 pj = a.pos[j];
               double ax = pj.x - pi.x, ay = pj.y - pi.y, az = pj.z - pi.z;
-              ax = fma(-a.L.x, rint(ax * a.iL.x), ax);
-              ay = fma(-a.L.y, rint(ay * a.iL.y), ay);
-              az = fma(-a.L.z, rint(az * a.iL.z), az);
+              ax = fma(-R.L.x, rint(ax * R.iL.x), ax);
+              ay = fma(-R.L.y, rint(ay * R.iL.y), ay);
+              az = fma(-R.L.z, rint(az * R.iL.z), az);
               double d2 = fma(ax, ax, fma(ay, ay, az * az));
               if (d2 < a.rc2_lo) pred = true;
               else if (d2 <= a.rc2_hi) {
                 double dx, dy, dz;
-                pred = min_image_dist2(pi, pj, a.hb, dx, dy, dz) < a.rc2;
+                pred = min_image_dist2(pi, pj, R.hb, dx, dy, dz) < a.rc2;
               }
             }
           }
@@ -376,12 +379,14 @@ __global__ void k_lj1g_cut(int nrows, const int* __restrict__ rows, const int* _
 // graphene nearest neighbours from the tb list: entries with |dr| < r_cut_nn,
 // exactly three per atom (find_gr_nearest_neighbors, graphenenorm.f90:8-36)
 __global__ void k_nn_from_tb(int nrows, const int* __restrict__ rows, const int* __restrict__ tnnum,
-                             const int* __restrict__ tnbr, int tstride, const double4* __restrict__ pos, double3 hb,
-                             double rnn, int* __restrict__ nnum, int* __restrict__ nbr, DevErr* err,
-                             const int* __restrict__ orig, int step) {
+                             const int* __restrict__ tnbr, int tstride, const double4* __restrict__ pos,
+                             const int* __restrict__ rep, const RepDev* __restrict__ rdev, double rnn,
+                             int* __restrict__ nnum, int* __restrict__ nbr, DevErr* err, const int* __restrict__ orig,
+                             int step) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   int i = rows[r], n = tnnum[r], k = 0;
+  const double3 hb = rdev[rep ? rep[i] : 0].hb;
   double4 pi = pos[i];
   for (int p = 0; p < n; ++p) {
     int j = tnbr[(size_t)r * tstride + p];
@@ -427,20 +432,23 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
   if (c < 0) return;
   const int ncen = a.cell_count[c];
   if (ncen == 0) return;
+  const RepDev& RG = a.rdev[a.cell_rep ? a.cell_rep[c] : 0];
+  if (!RG.tile) return;  // searched by k_build
   const int cs = a.cell_start[c];
   const int tid = threadIdx.x;
   const int pk = rank_xyz[c];
   const int cc[3] = {pk & 1023, (pk >> 10) & 1023, (pk >> 20) & 1023};
-  const double org[3] = {cc[0] * a.clen.x, cc[1] * a.clen.y, cc[2] * a.clen.z};
+  const double org[3] = {cc[0] * RG.clen.x, cc[1] * RG.clen.y, cc[2] * RG.clen.z};
+  const double3 boxL = RG.L, hb = RG.hb;
   if (tid < 27) {
     int o[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
     int q[3];
     for (int d = 0; d < 3; ++d) {
-      int nd = a.g.nc[d];
+      int nd = RG.nc[d];
       q[d] = cc[d] + o[d];
       q[d] = q[d] < 0 ? q[d] + nd : (q[d] >= nd ? q[d] - nd : q[d]);
     }
-    int r = a.cell_rank[(q[2] * a.g.nc[1] + q[1]) * a.g.nc[0] + q[0]];
+    int r = a.cell_rank[RG.cell_base + (q[2] * RG.nc[1] + q[1]) * RG.nc[0] + q[0]];
     tc.start[tid] = a.cell_start[r];
     tc.cnt[tid] = a.cell_count[r];
   }
@@ -466,13 +474,14 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
         for (int sft = 16; sft > 0; sft >>= 1)
           if (k + sft < 27 && tc.off[k + sft] <= m) k += sft;
         int j = tc.start[k] + (m - tc.off[k]);
+        if (a.bin) j = a.bin[j];
         int o0 = k % 3 - 1, o1 = (k / 3) % 3 - 1, o2 = k / 9 - 1;
         double4 p = a.pos[j];
         // unwrap: the image of atom j next to the centre cell
         double x = p.x - org[0], y = p.y - org[1], z = p.z - org[2];
-        if (o0 < 0 && x > 0.0) x -= a.L.x; else if (o0 > 0 && x < 0.0) x += a.L.x;
-        if (o1 < 0 && y > 0.0) y -= a.L.y; else if (o1 > 0 && y < 0.0) y += a.L.y;
-        if (o2 < 0 && z > 0.0) z -= a.L.z; else if (o2 > 0 && z < 0.0) z += a.L.z;
+        if (o0 < 0 && x > 0.0) x -= boxL.x; else if (o0 > 0 && x < 0.0) x += boxL.x;
+        if (o1 < 0 && y > 0.0) y -= boxL.y; else if (o1 > 0 && y < 0.0) y += boxL.y;
+        if (o2 < 0 && z > 0.0) z -= boxL.z; else if (o2 > 0 && z < 0.0) z += boxL.z;
         v.w = __int_as_float(j);
         if (a.mask[j] & a.g2bit) { v.x = (float)x; v.y = (float)y; v.z = (float)z; }
       }
@@ -486,7 +495,7 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
   for (int q0 = 0; q0 < ncen; q0 += 256 / T) {
     const int qq = q0 + tid / T;
     const bool inrow = qq < ncen;
-    const int i = cs + (inrow ? qq : 0);
+    const int i = a.bin ? a.bin[cs + (inrow ? qq : 0)] : cs + (inrow ? qq : 0);
     const int row = inrow ? row_of[i] : -1;
     const bool active = row >= 0;
     const double4 pi = a.pos[i];
@@ -525,7 +534,7 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
           float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
           if (fmaf(dx, dx, fmaf(dy, dy, dz * dz)) >= lo) {
             double ex, ey, ez;
-            if (!(min_image_dist2(pi, a.pos[__float_as_int(v.w)], a.hb, ex, ey, ez) < a.rc2))
+            if (!(min_image_dist2(pi, a.pos[__float_as_int(v.w)], hb, ex, ey, ez) < a.rc2))
               hit &= ~(0x80000000u >> b);
           }
         }
@@ -564,14 +573,14 @@ static void launch_build_T(pfm_ctx* h, SearchArgs& a, const int* row_of) {
 }
 
 static void launch_build(pfm_ctx* h, SearchArgs& a, const int* row_of) {
-  if (h->nc[0] >= 3 && h->nc[1] >= 3 && h->nc[2] >= 3) {
+  if (h->any_tile) {
     int nb = h->ncell;
     if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", k_build_tile<4>, pad8(nb), 256, a, h->cell_xyz, row_of, nb);
     else PFM_LAUNCH(h, "build_list", k_build_tile<8>, pad8(nb), 256, a, h->cell_xyz, row_of, nb);
-  } else if (a.nmax <= 8) {
-    launch_build_T<4>(h, a, row_of);
-  } else {
-    launch_build_T<8>(h, a, row_of);
+  }
+  if (h->any_rows) {
+    if (a.nmax <= 8) launch_build_T<4>(h, a, row_of);
+    else launch_build_T<8>(h, a, row_of);
   }
 }
 
@@ -590,15 +599,14 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   a.pos = h->pos4;
   a.mask = h->mask;
   a.orig = h->orig;
-  a.hb = make_double3(h->hbox[0], h->hbox[1], h->hbox[2]);
+  a.rdev = h->rdev;
+  a.rep = h->rep;
+  a.cell_rep = h->cell_rep;
+  a.bin = h->bin;
   double rc = (k == 1 && (it.kind == PFM_LJ || it.kind == PFM_LJC || it.kind == PFM_MORSEC)) ? it.nl[0].rcut : L.rcut;
   a.rc2 = rc * rc;
   a.rc2_lo = a.rc2 * (1.0 - 1e-12);
   a.rc2_hi = a.rc2 * (1.0 + 1e-12);
-  a.L = make_double3(h->box[0], h->box[1], h->box[2]);
-  a.iL = make_double3(1.0 / h->box[0], 1.0 / h->box[1], 1.0 / h->box[2]);
-  a.clen = make_double3(h->cell_len[0], h->cell_len[1], h->cell_len[2]);
-  a.g = make_grid(h);
   a.cell_rank = h->cell_rank;
   a.cell_start = h->cell_start;
   a.cell_count = h->cell_count;
@@ -634,7 +642,7 @@ int launch_nn_from_tb(pfm_ctx* h, int ii, int step) {
   if (L.nrows == 0) return PFM_OK;
   int nb = (L.nrows + 255) / 256;
   PFM_LAUNCH(h, "nn_from_tb", k_nn_from_tb, nb, 256, L.nrows, h->grp.rows[L.g1], T.nnum, T.nbr, T.stride, h->pos4,
-             make_double3(h->hbox[0], h->hbox[1], h->hbox[2]), L.rcut, L.nnum, L.nbr, h->derr, h->orig, step);
+             h->rep, h->rdev, L.rcut, L.nnum, L.nbr, h->derr, h->orig, step);
   bool simplified = (int)it.p[it.kind == PFM_LJC ? 5 : 6];
   if (!simplified)
     PFM_LAUNCH(h, "list_rev", k_rev, nb, 256, L.nrows, h->grp.rows[L.g1], h->grp.row_of[L.g1], L.nnum, L.nbr,

@@ -270,6 +270,84 @@ def build_context(s: System, make):
     return ctx
 
 
+@dataclasses.dataclass
+class Batch:
+    """Independent replicas stepped together in one context: `system` holds
+    the concatenated atoms with batch-wide groups and interactions, replica r
+    has n_atoms[r] atoms in box boxes[r]."""
+    system: System
+    n_atoms: np.ndarray
+    boxes: np.ndarray
+
+    @property
+    def n_rep(self):
+        return len(self.n_atoms)
+
+    def replica_slice(self, r):
+        a = int(np.sum(self.n_atoms[:r]))
+        return slice(a, a + int(self.n_atoms[r]))
+
+
+def merge_replicas(systems):
+    """Concatenate systems into one Batch.  Groups are matched by their species
+    lists, so a group of the batch holds the matching atoms of every replica;
+    interactions equal in name, parameters and lists are shared.  Role groups
+    that differ between replicas become the union of their species lists.
+    All replicas must share the integrator schedule and periods; chains are
+    not allowed (a chain's group would span replicas)."""
+    first = systems[0]
+    gid = {}
+    groups = []
+
+    def group_of(names):
+        key = tuple(names)
+        if key not in gid:
+            groups.append(list(names))
+            gid[key] = len(groups)
+        return gid[key]
+
+    inter_keys, inters = {}, []
+    for s in systems:
+        if s.nhcs:
+            raise ValueError("replicas of a batch cannot carry Nose-Hoover chains")
+        if [(i.name, i.dt) for i in s.integrators] != [(i.name, i.dt) for i in first.integrators]:
+            raise ValueError("replicas of a batch share one integrator schedule")
+        if s.zero_momentum_period != first.zero_momentum_period or s.invert_z != first.invert_z:
+            raise ValueError("replicas of a batch share zero_momentum_period and invert_z")
+        for it in s.interactions:
+            lists = tuple((group_of(s.groups[g1 - 1]), group_of(s.groups[g2 - 1]), nm, float(rc), per)
+                          for g1, g2, nm, rc, per in it.lists)
+            key = (it.name, tuple(float(x) for x in it.params), lists)
+            if key not in inter_keys:
+                inter_keys[key] = len(inters)
+                inters.append(Interaction(it.name, list(it.params), [list(x) for x in lists]))
+
+    def role(attr):
+        names = []
+        for s in systems:
+            for n in s.groups[getattr(s, attr) - 1]:
+                if n not in names:
+                    names.append(n)
+        return group_of(names)
+
+    roles = {a: role(a) for a in ("all_moving", "xyz_moving", "z_moving", "all_atoms", "traj")}
+    batch = System(box=np.array(first.box, dtype=float), pos=np.concatenate([s.pos for s in systems]),
+                   vel=np.concatenate([s.vel for s in systems]), mass=np.concatenate([s.mass for s in systems]),
+                   species=[x for s in systems for x in s.species], groups=groups, interactions=inters,
+                   integrators=list(first.integrators), period_traj=first.period_traj,
+                   zero_momentum_period=first.zero_momentum_period, invert_z=first.invert_z, ms_de=first.ms_de,
+                   md_step_limit=first.md_step_limit, **roles)
+    return Batch(batch, np.array([s.n for s in systems], dtype=np.int32),
+                 np.array([np.asarray(s.box, dtype=float) for s in systems]))
+
+
+def build_batch_context(b: Batch, precision):
+    """GPU context of a Batch (the C-ABI's pfm_create_batch)."""
+    from . import capi
+    return build_context(b.system, lambda box, p, v, m: capi.Context.gpu_batch(b.n_atoms, b.boxes, p, v, m,
+                                                                              precision))
+
+
 def _next_multiple(step, period):
     return ((step + period - 1) // period) * period
 

@@ -58,3 +58,28 @@ def test_list_overflow_status():
         codes.append(ei.value.code)
         assert "too many neighbours" in ei.value.msg
     assert codes == [capi.E_OVERFLOW, capi.E_OVERFLOW]
+
+
+def test_mixed_periods_trajectory():
+    # lists with different rebuild periods: the atoms are re-binned for a due
+    # list without reordering them (the other lists keep their entries); atoms
+    # crossing the periodic boundary between sorts must still be found
+    s = configs.lj_gas_lattice(1000, 0.02, 31, 1000.0, dt=1.0, steps=25)
+    s.interactions[0].lists = [(1, 1, 160, 8.5, 20)]
+    s.interactions[1].lists = [(2, 2, 160, 8.5, 7)]
+    s.interactions[2].lists = [(1, 2, 160, 8.5, 10), (2, 1, 160, 8.5, 10)]
+    a = s.box[0] / 10.0  # lattice spacing: a plane of atoms straddles the boundary
+    s.pos = np.mod(s.pos + (0.5 * a - 0.02), s.box)
+    print(f"[periods] n={s.n} box={s.box[0]:.2f}", flush=True)
+    g, o = gpu_ctx(s), oracle_ctx(s)
+    for c in (g, o):
+        c.run("nve", 1.0, 0, 25, True)
+    pg, vg, fg = g.state()
+    po, vo, fo = o.state()
+    crossed = np.any(np.abs(pg - s.pos) > 0.5 * s.box[0])
+    assert crossed, "no atom crossed the boundary: the case is not exercised"
+    assert rel_inf(pg, po) < 1e-8 and rel_inf(vg, vo) < 1e-8 and rel_inf(fg, fo) < 1e-8
+    for k in (1, 2, 3):
+        gs, _ = list_sets(g, k, 1)
+        os_, _ = list_sets(o, k, 1)
+        assert gs == os_

@@ -32,7 +32,9 @@ def test_library_exports_every_declared_symbol():
 def test_oracle_mirrors_the_abi():
     lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liborc.so"))
     for s in header_symbols():
-        if s in ("pfm_set_timing", "pfm_kernel_time", "pfm_set_tuning"):
+        # GPU-only extras: profiling, launch shape, and replica batches (pinned
+        # against single-system GPU runs, not the checker)
+        if s in ("pfm_set_timing", "pfm_kernel_time", "pfm_set_tuning", "pfm_create_batch", "pfm_replica_energies"):
             continue
         assert hasattr(lib, s.replace("pfm_", "orc_")), s
 
@@ -121,3 +123,40 @@ def test_fortran_frontend_fails_loudly_without_gpu(tmp_path):
                        text=True, timeout=60)
     assert p.returncode != 0
     assert "no HIP device" in p.stdout
+
+
+def test_merge_replicas_groups_roles_interactions():
+    """Batch layout: groups matched by species list, shared interactions
+    deduplicated, differing role groups united, replica slices in order."""
+    from pfmds_amd import system
+    a = configs.lj_gas_lattice(100, 0.02, 1, 100.0, steps=10)
+    b = configs.fcc_cu_rgl(nx=4, seed=2, steps=10, dt=0.5)
+    c = configs.lj_gas_lattice(120, 0.03, 3, 200.0, steps=10)
+    bt = system.merge_replicas([a, b, c])
+    s = bt.system
+    assert s.groups == [["A"], ["B"], ["Cu"], ["A", "B", "Cu"], ["#"]]
+    assert [it.name for it in s.interactions] == ["lj1g", "lj1g", "lj", "rjl"]
+    assert s.interactions[3].lists == [[3, 3, 96, 6.5, 10]]
+    assert s.groups[s.all_moving - 1] == ["A", "B", "Cu"] and s.groups[s.z_moving - 1] == ["#"]
+    assert list(bt.n_atoms) == [100, 256, 120] and s.n == 476
+    assert bt.replica_slice(1) == slice(100, 356)
+    np.testing.assert_array_equal(s.pos[bt.replica_slice(2)], c.pos)
+    assert np.allclose(bt.boxes[1], b.box)
+    # group A of the batch holds the A atoms of both gas replicas, in order
+    ga = s.group_indices(1) - 1
+    assert list(ga) == list(range(0, 50)) + list(range(356, 416))
+
+
+def test_replica_mix_sizes():
+    reps = configs.replica_mix(n_rep=16, seed=7)
+    ns = np.array([r.n for r in reps])
+    assert ns.min() >= 128 and ns.max() <= 40000
+    for k, r in enumerate(reps):
+        if k % 2 == 0:
+            d = r.pos[:, None, :] - r.pos[None, :, :] if r.n <= 2000 else None
+            if d is not None:
+                d -= r.box * np.round(d / r.box)
+                dist = np.sqrt((d ** 2).sum(-1)) + np.eye(r.n) * 1e9
+                assert dist.min() > 3.0
+        else:
+            assert min(r.box) >= 4 * 3.615 - 1e-9
