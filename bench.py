@@ -116,6 +116,47 @@ def graphene_cu(steps, warmup):
     return out
 
 
+def replica_batch(steps, fp64_steps, check_steps=100, n_check=8):
+    """512 independent replicas (binary LJ gas and fcc Cu, 256-32768 atoms
+    each, about 3.4M atoms) stepped together with nve at dt = 0.5 fs, in mixed
+    precision and in FP64, and the largest per-replica deviation of forces and
+    potential energy from running the same replicas one at a time."""
+    reps = configs.replica_mix()
+    b = system.merge_replicas(reps)
+    out = {"replicas": b.n_rep, "atoms": int(b.system.n)}
+    for name, prec, n in (("mixed", capi.MIXED, steps), ("fp64", capi.FP64, fp64_steps)):
+        g = system.build_batch_context(b, prec)
+        g.run("nve", 0.5, 0, 20, True)
+        t0 = time.perf_counter()
+        g.run("nve", 0.5, 21, 20 + n, True)
+        t = time.perf_counter() - t0
+        out[name + "_ms_per_step"] = t * 1e3 / n
+        out[name + "_atom_steps_per_s"] = b.system.n * n / t
+        out[name + "_steps"] = n
+        g.close()
+    out["speedup_vs_fp64"] = out["mixed_atom_steps_per_s"] / out["fp64_atom_steps_per_s"]
+    # per-replica deviation from single runs, mixed precision, on a sample
+    g = system.build_batch_context(b, capi.MIXED)
+    g.run("nve", 0.5, 0, check_steps, True)
+    _, _, frc = g.state()
+    pe = g.replica_energies()["pe"]
+    idx = np.linspace(0, b.n_rep - 1, n_check).astype(int)
+    fdev, edev = 0.0, 0.0
+    for r in idx:
+        c = gpu_ctx(reps[r], capi.MIXED)
+        c.run("nve", 0.5, 0, check_steps, True)
+        f1 = c.state()[2]
+        sl = b.replica_slice(r)
+        fdev = max(fdev, float(np.abs(frc[sl] - f1).max() / max(np.abs(f1).max(), 1e-30)))
+        edev = max(edev, abs(pe[r] - c.energies()["pe"]) / max(abs(c.energies()["pe"]), 1e-30))
+        c.close()
+    out["check"] = {"replicas": [int(x) for x in idx], "steps": check_steps,
+                    "max_force_dev_rel": fdev, "max_pe_dev_rel": edev}
+    print(f"# replicas: mixed {out['mixed_ms_per_step']:.3f} ms/step, fp64 {out['fp64_ms_per_step']:.3f} ms/step",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(s, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from tests.harness import oracle_ctx, oracle_lib
@@ -143,6 +184,8 @@ def main():
     ap.add_argument("--drift-ps", type=float, default=20.0)
     ap.add_argument("--no-c2", action="store_true", help="skip the graphene/Cu workload")
     ap.add_argument("--c2-only", action="store_true", help="time only the graphene/Cu workload (profiling)")
+    ap.add_argument("--no-c3", action="store_true", help="skip the replica batch")
+    ap.add_argument("--c3-only", action="store_true", help="time only the replica batch (profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step path has no multi-GPU decomposition "
@@ -151,6 +194,9 @@ def main():
         sys.exit(2)
     if args.c2_only:
         print(json.dumps({"graphene_cu_nvt": graphene_cu(args.steps, args.warmup)}))
+        return
+    if args.c3_only:
+        print(json.dumps({"replica_batch": replica_batch(args.steps, args.steps, check_steps=10, n_check=2)}))
         return
     s = configs.fcc_cu_rgl(nx=args.cells, steps=args.steps)
     res = {}
@@ -212,6 +258,7 @@ def main():
         drift = float(slope)
 
     c2 = None if args.no_c2 else graphene_cu(200, 20)
+    c3 = None if args.no_c3 else replica_batch(2000, 500)
 
     cpu = None
     if not args.no_cpu:
@@ -244,6 +291,7 @@ def main():
         "kernels_us": {k: v[1] * 1e3 for k, v in kern.items()},
         "roofline": roofline,
         "graphene_cu_nvt": c2,
+        "replica_batch": c3,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -605,7 +605,7 @@ static int finalize(pfm_ctx* h) {
     it.e_rows = it.nl[0].nrows;
     if ((rc = dalloc(h, &it.e_row, it.e_rows))) return rc;
     if (it.kind == PFM_LJ1G)
-      if ((rc = dalloc(h, &it.cut, h->N))) return rc;
+      if ((rc = dalloc(h, &it.lc, h->N))) return rc;
     if (it.kind == PFM_TB) {
       size_t ne = (size_t)it.nl[0].nrows * it.nl[0].stride;
       if ((rc = dalloc(h, &it.B, ne))) return rc;
@@ -1331,7 +1331,7 @@ int pfm_destroy(pfm_ctx* h) {
   for (int g = 0; g <= PFM_MAXG; ++g) { F(h->grp.rows[g]); F(h->grp.row_of[g]); F(h->grp.loc[g]); }
   for (auto& it : h->inter) {
     for (int k = 0; k < 3; ++k) { F(it.nl[k].nnum); F(it.nl[k].nbr); F(it.nl[k].rev); }
-    F(it.e_row); F(it.cut); F(it.B); F(it.Bp); F(it.gnorm); F(it.gS);
+    F(it.e_row); F(it.lc); F(it.B); F(it.Bp); F(it.gnorm); F(it.gS);
   }
   for (auto e : h->evpool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);

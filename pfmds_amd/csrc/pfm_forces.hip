@@ -192,17 +192,18 @@ struct Lj1gP {
 };
 
 template <int T, class R, bool E>
-__global__ void __launch_bounds__(PAIR_BLOCK) k_lj1g(RowArgs a, Lj1gP P, const int* __restrict__ orig,
-                                              const int* __restrict__ loc, const int* __restrict__ cut, int nb_real) {
+__global__ void __launch_bounds__(PAIR_BLOCK) k_lj1g(RowArgs a, Lj1gP P,
+                                              const int2* __restrict__ lc, int nb_real) {
   ROW_PROLOGUE(T)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, c6 = P.c6, c12 = P.c12, c6t6 = P.c6t6, c12t12 = P.c12t12;
-  const int li = loc[orig[i]], ci = cut[i];
+  const int2 mi = lc[i];
+  const int li = mi.x, ci = mi.y;
   double fx = 0, fy = 0, fz = 0, e = 0;
   for (int p = t; p < n; p += T) {
     int j = nbrow[p];
-    int lj = loc[orig[j]];
-    int wh = lj < ci ? 1 : 0;
-    int w = wh + (li < cut[j] ? 1 : 0);
+    const int2 mj = lc[j];
+    int wh = mj.x < ci ? 1 : 0;
+    int w = wh + (li < mj.y ? 1 : 0);
     if (w == 0) continue;
     R dx, dy, dz;
     R r = PM<R>::sqrt_(G.d2(pi, G.at(j), dx, dy, dz));
@@ -678,12 +679,11 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         double eps = it.p[0], sig = it.p[1], s6 = sig * sig * sig * sig * sig * sig;
         Lj1gP P{4.0 * eps * s6, 4.0 * eps * s6 * s6, 6.0 * 4.0 * eps * s6, 12.0 * 4.0 * eps * s6 * s6,
                 it.p[2], it.p[3], 1.0 / (it.p[3] - it.p[2])};
-        const int* loc = h->grp.loc[it.nl[0].g2];
         int T = h->lanes;
         DISPATCH_T(T, {
           int nb = row_blocks(a.nrows, T, BS);
-          if (energy) PFM_LAUNCH(h, "lj1g", (k_lj1g<T, R, true>), pad8(nb), BS, a, P, h->orig, loc, it.cut, nb);
-          else PFM_LAUNCH(h, "lj1g", (k_lj1g<T, R, false>), pad8(nb), BS, a, P, h->orig, loc, it.cut, nb);
+          if (energy) PFM_LAUNCH(h, "lj1g", (k_lj1g<T, R, true>), pad8(nb), BS, a, P, it.lc, nb);
+          else PFM_LAUNCH(h, "lj1g", (k_lj1g<T, R, false>), pad8(nb), BS, a, P, it.lc, nb);
         });
         written = true;
         break;

@@ -223,7 +223,7 @@ struct Inter {
   double* e_row = nullptr;  // per-row energies of the last energy step
   int e_rows = 0;
   // lj1g half-list bookkeeping (LennardJones_1g.f90:42,73 via lessnnum)
-  int* cut = nullptr;  // [N] by internal atom: group-local index bound of the half list
+  int2* lc = nullptr;  // [N] by internal atom: (group-local index, bound of the half list)
   // tb tables
   double* B = nullptr;   // [nrows*stride] bond order B(p,i)
   double* Bp = nullptr;  // [nrows*stride] B**(1/delt+1)

@@ -23,19 +23,42 @@ __global__ void k_fill_int(int n, int* __restrict__ x, int v) {
   if (i < n) x[i] = v;
 }
 
+// ctr[key] += 1 for every valid lane of the wavefront, with one atomic per run
+// of equal keys in consecutive lanes (atoms arrive nearly in cell order, so
+// runs are long); returns the lane's previous count (its slot).  Invalid
+// lanes (past the end of the array) are the top lanes of the last wavefront.
+__device__ __forceinline__ int run_atomic_inc(int* ctr, int key, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1);
+  const bool head = valid && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const int nvalid = __popcll(__ballot(valid));
+  const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
+  const int start = valid ? 63 - __clzll(heads & upto) : lane;
+  const unsigned long long above = heads & ~upto;
+  const int next = above ? __ffsll(above) - 1 : 64;
+  int base = 0;
+  if (head) base = atomicAdd(&ctr[key], (next < nvalid ? next : nvalid) - lane);
+  base = __shfl(base, start);
+  return base + (lane - start);
+}
+
 __global__ void k_cell_keys(int N, const double4* __restrict__ pos, const int* __restrict__ rep,
                             const RepDev* __restrict__ rdev, const int* __restrict__ cell_rank,
                             int* __restrict__ atom_rank, int* __restrict__ cell_count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const RepDev& R = rdev[rep ? rep[i] : 0];
-  double4 p = pos[i];
-  int cx = cell_coord(p.x, R.inv_clen.x, R.nc[0]);
-  int cy = cell_coord(p.y, R.inv_clen.y, R.nc[1]);
-  int cz = cell_coord(p.z, R.inv_clen.z, R.nc[2]);
-  int r = cell_rank[R.cell_base + (cz * R.nc[1] + cy) * R.nc[0] + cx];
-  atom_rank[i] = r;
-  atomicAdd(&cell_count[r], 1);
+  const bool valid = i < N;
+  int r = -1;
+  if (valid) {
+    const RepDev& R = rdev[rep ? rep[i] : 0];
+    double4 p = pos[i];
+    int cx = cell_coord(p.x, R.inv_clen.x, R.nc[0]);
+    int cy = cell_coord(p.y, R.inv_clen.y, R.nc[1]);
+    int cz = cell_coord(p.z, R.inv_clen.z, R.nc[2]);
+    r = cell_rank[R.cell_base + (cz * R.nc[1] + cy) * R.nc[0] + cx];
+    atom_rank[i] = r;
+  }
+  run_atomic_inc(cell_count, r, valid);
 }
 
 // exclusive scan of n ints by one workgroup of 1024 threads (small n: the
@@ -122,9 +145,10 @@ void scan_excl(pfm_ctx* h, int n, const int* in, int* out, int* tmp) {
 __global__ void k_scatter(int N, const int* __restrict__ atom_rank, const int* __restrict__ cell_start,
                           int* __restrict__ cell_fill, int* __restrict__ perm) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  int r = atom_rank[i];
-  perm[cell_start[r] + atomicAdd(&cell_fill[r], 1)] = i;
+  const bool valid = i < N;
+  const int r = valid ? atom_rank[i] : -1;
+  const int slot = run_atomic_inc(cell_fill, r, valid);
+  if (valid) perm[cell_start[r] + slot] = i;
 }
 
 // order the atoms of each cell by original index, so the layout (and every
@@ -362,10 +386,11 @@ __global__ void k_rev(int nrows, const int* __restrict__ rows, const int* __rest
 
 // lj1g: group-local index of the first listed neighbour whose atom index is
 // larger than the row atom's (the reference's lessnnum boundary, in list
-// order = ascending group order)
+// order = ascending group order), stored with the row atom's own group-local
+// index for the force pass
 __global__ void k_lj1g_cut(int nrows, const int* __restrict__ rows, const int* __restrict__ nnum,
                            const int* __restrict__ nbr, int stride, const int* __restrict__ orig,
-                           const int* __restrict__ loc, int* __restrict__ cut) {
+                           const int* __restrict__ loc, int2* __restrict__ lc) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   int i = rows[r], oi = orig[i], n = nnum[r], c = 0x7fffffff;
@@ -373,7 +398,7 @@ __global__ void k_lj1g_cut(int nrows, const int* __restrict__ rows, const int* _
     int oj = orig[nbr[(size_t)r * stride + p]];
     if (oj > oi) { int l = loc[oj]; c = l < c ? l : c; }
   }
-  cut[i] = c;
+  lc[i] = make_int2(loc[oi], c);
 }
 
 // graphene nearest neighbours from the tb list: entries with |dr| < r_cut_nn,
@@ -627,7 +652,7 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   }
   if (it.kind == PFM_LJ1G) {
     PFM_LAUNCH(h, "list_cut", k_lj1g_cut, nb, 256, L.nrows, h->grp.rows[L.g1], L.nnum, L.nbr, L.stride, h->orig,
-               h->grp.loc[L.g2], it.cut);
+               h->grp.loc[L.g2], it.lc);
   }
   L.built = true;
   return PFM_OK;
