@@ -596,9 +596,12 @@ static int finalize(pfm_ctx* h) {
   for (auto& it : h->inter) {
     for (int k = 0; k < it.nl_n; ++k) {
       ListDev& L = it.nl[k];
-      L.stride = L.nmax;
+      // rows of the pair lists start on 16 bytes and end 32 entries early
+      // enough that the passes' four-entry loads stay inside the array; the
+      // graphene nearest-neighbour rows (k == 2) are three entries wide
+      L.stride = k == 2 ? L.nmax : (L.nmax + 3) / 4 * 4;
       if ((rc = dalloc(h, &L.nnum, L.nrows))) return rc;
-      if ((rc = dalloc(h, &L.nbr, (size_t)L.nrows * L.stride))) return rc;
+      if ((rc = dalloc(h, &L.nbr, (size_t)L.nrows * L.stride + 32))) return rc;
       if (it.kind == PFM_TB || k == 2)
         if ((rc = dalloc(h, &L.rev, (size_t)L.nrows * L.stride))) return rc;
     }

@@ -74,6 +74,62 @@ template <> __device__ __forceinline__ void store_w<float>(const RowArgs& a, int
   int n = a.nnum[row];                                    \
   const int* nbrow = a.nbr + (size_t)row * a.stride;
 
+// Walk a row's entries four at a time: lane t of the row's T lanes takes
+// entries 4t..4t+3 of every block of 4T, with one 16-byte index load (rows
+// start on 16 bytes and are padded past their end), then the four
+// position-record gathers, then the arithmetic f(j, record, valid).  A block
+// waits on two memory latencies.  Entries past the end of the row gather the
+// row atom's own record and reach f with valid = false (f masks their
+// contribution without branching).
+// The FP64 passes are bound by their arithmetic; they keep one entry per
+// iteration (every T-th entry), which costs fewer registers.
+template <int T, class R, class F>
+__device__ __forceinline__ void walk_row(const Geo<R>& G, const int* __restrict__ nbrow, int n, int t, int self, F f) {
+  if constexpr (sizeof(R) == 8) {
+    for (int p = t; p < n; p += T) {
+      const int j = nbrow[p];
+      f(j, G.at(j), true);
+    }
+    return;
+  }
+  for (int p0 = 4 * t; p0 < n; p0 += 4 * T) {
+    const int4 v = *reinterpret_cast<const int4*>(nbrow + p0);
+    const int jj[4] = {v.x, p0 + 1 < n ? v.y : self, p0 + 2 < n ? v.z : self, p0 + 3 < n ? v.w : self};
+    typename Geo<R>::P q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = G.at(jj[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) f(jj[u], q[u], p0 + u < n);
+  }
+}
+
+// walk_row with one per-atom side value of each neighbour (aux[j]) gathered
+// together with its position record: f(j, record, aux value, valid)
+template <int T, class R, class A, class F>
+__device__ __forceinline__ void walk_row_aux(const Geo<R>& G, const int* __restrict__ nbrow, int n, int t, int self,
+                                             const A* __restrict__ aux, F f) {
+  if constexpr (sizeof(R) == 8) {
+    for (int p = t; p < n; p += T) {
+      const int j = nbrow[p];
+      f(j, G.at(j), aux[j], true);
+    }
+    return;
+  }
+  for (int p0 = 4 * t; p0 < n; p0 += 4 * T) {
+    const int4 v = *reinterpret_cast<const int4*>(nbrow + p0);
+    const int jj[4] = {v.x, p0 + 1 < n ? v.y : self, p0 + 2 < n ? v.z : self, p0 + 3 < n ? v.w : self};
+    typename Geo<R>::P q[4];
+    A x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      q[u] = G.at(jj[u]);
+      x[u] = aux[jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) f(jj[u], q[u], x[u], p0 + u < n);
+  }
+}
+
 __device__ __forceinline__ void write_force(const RowArgs& a, int i, double fx, double fy, double fz) {
   double4 f = a.accumulate ? a.frc[i] : make_double4(0, 0, 0, 0);
   f.x += fx; f.y += fy; f.z += fz;
@@ -99,14 +155,14 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_rjl_density(RowArgs a, RjlP P, d
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
   const R kq = PM<R>::kexp(-2.0 * P.q);
   double rho = 0.0;
-  for (int p = t; p < n; p += T) {
+  walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
     R dx, dy, dz;
-    R r = PM<R>::sqrt_(G.d2(pi, G.at(nbrow[p]), dx, dy, dz));
+    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
     R fc, dfr;
     cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
     R e = PM<R>::expk(kq * (r * ir0 - R(1))) * fc;
-    rho += (double)(r <= R2 ? e : R(0));
-  }
+    rho += (double)(valid && r <= R2 ? e : R(0));
+  });
   rho = group_sum<T>(rho);
   if (t == 0) {
     double s = sqrt(rho);
@@ -127,8 +183,7 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_rjl_force(RowArgs a, RjlP P, con
   const R c2A = 2.0 * P.A0, pr0 = P.p / P.r0, qr0 = P.q / P.r0, xi = P.xi;
   const R ie = Geo<R>::w(pi);
   double fx = 0, fy = 0, fz = 0, er = 0;
-  for (int p = t; p < n; p += T) {
-    const typename Geo<R>::P q = G.at(nbrow[p]);
+  walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
     R dx, dy, dz;
     R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
     R ir = PM<R>::rcp(r);
@@ -137,11 +192,11 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_rjl_force(RowArgs a, RjlP P, con
     R x = r * ir0 - R(1);
     R ep = PM<R>::expk(kp * x), eq = PM<R>::expk(kq * x);
     R c = (c2A * (pr0 * fc - dfr) * ep - xi * (qr0 * fc - R(0.5) * dfr) * (ie + Geo<R>::w(q)) * eq) * ir;
-    bool in = r <= R2;
+    bool in = valid && r <= R2;
     c = in ? c : R(0);
     fx -= (double)(c * dx); fy -= (double)(c * dy); fz -= (double)(c * dz);
     if (E) er += in ? (double)(ep * fc) : 0.0;
-  }
+  });
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (E) er = group_sum<T>(er);
   if (t == 0) {
@@ -161,19 +216,19 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_lj(RowArgs a, LjP P, int nb_real
   ROW_PROLOGUE(T)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, sig = P.sig, e4 = P.eps4, kdf = -0.5 * PFM_PI * P.invw;
   double fx = 0, fy = 0, fz = 0, e = 0;
-  for (int p = t; p < n; p += T) {
+  walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
     R dx, dy, dz;
-    R r = PM<R>::sqrt_(G.d2(pi, G.at(nbrow[p]), dx, dy, dz));
+    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
     R ir = PM<R>::rcp(r);
     R fc, dfr;
     cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
     R s = sig * ir, s2 = s * s, V = s2 * s2 * s2;
     R c = e4 * (V * (R(12) * V - R(6)) * fc * ir - V * (V - R(1)) * dfr) * ir;
-    bool in = r < R2;
+    bool in = valid && r < R2;
     c = in ? c : R(0);
     fx -= (double)(c * dx); fy -= (double)(c * dy); fz -= (double)(c * dz);
     if (E) e += in ? (double)(e4 * V * (V - R(1)) * fc) : 0.0;
-  }
+  });
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (E) e = group_sum<T>(e);
   if (t == 0) {
@@ -199,14 +254,11 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_lj1g(RowArgs a, Lj1gP P,
   const int2 mi = lc[i];
   const int li = mi.x, ci = mi.y;
   double fx = 0, fy = 0, fz = 0, e = 0;
-  for (int p = t; p < n; p += T) {
-    int j = nbrow[p];
-    const int2 mj = lc[j];
-    int wh = mj.x < ci ? 1 : 0;
-    int w = wh + (li < mj.y ? 1 : 0);
-    if (w == 0) continue;
+  walk_row_aux<T, R>(G, nbrow, n, t, i, lc, [&](int, const typename Geo<R>::P& q, const int2& mj, bool valid) {
+    int wh = valid && mj.x < ci ? 1 : 0;
+    int w = wh + (valid && li < mj.y ? 1 : 0);
     R dx, dy, dz;
-    R r = PM<R>::sqrt_(G.d2(pi, G.at(j), dx, dy, dz));
+    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
     // f_dfr_cut (cut_off_poly.f90:30-43), dfr without the 1/(R2-R1) factor
     R tr = r < R1 ? R1 : (r > R2 ? R2 : r);
     R x = (tr - R1) * invw, x2 = x * x;
@@ -214,14 +266,14 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_lj1g(RowArgs a, Lj1gP P,
     R dfr = tr * x2 * (R(-30) + R(60) * x - R(30) * x2);
     R invr2 = PM<R>::rcp(r * r), U = invr2 * invr2 * invr2;
     R Fs = U * invr2 * ((c12t12 * U - c6t6) * f - (c12 * U - c6) * dfr);
-    R cw = Fs * R(w);
+    R cw = w ? Fs * R(w) : R(0);
     fx -= (double)(cw * dx); fy -= (double)(cw * dy); fz -= (double)(cw * dz);
-    if (E && wh) {
+    if (E) {
       // cut_off_poly f_cut (cut_off_poly.f90:6-17)
       R fe = (r > R1 && r < R2) ? f : (r <= R1 ? R(1) : R(0));
-      e += (double)(U * (c12 * U - c6) * fe);
+      e += wh ? (double)(U * (c12 * U - c6) * fe) : 0.0;
     }
-  }
+  });
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (E) e = group_sum<T>(e);
   if (t == 0) {
@@ -479,20 +531,20 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_cos_graphene(RowArgs a, CosP P, 
   }
   const R nx = g.x, ny = g.y, nz = g.z, R2 = P.R2, pref = P.pref;
   double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, e = 0;
-  for (int p = t; p < n; p += T) {
+  walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
     R dx, dy, dz;
-    R r = PM<R>::sqrt_(G.d2(pi, G.at(nbrow[p]), dx, dy, dz));
+    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
     R nd = nx * dx + ny * dy + nz * dz;
     R A, Bc, Sc, ep;
     cos_pair<R>(P, r, nd, PM<R>::rcp(r), A, Bc, Sc, ep);
-    bool in = r < R2;
+    bool in = valid && r < R2;
     A = in ? A * pref : R(0);
     Bc = in ? Bc * pref : R(0);
     Sc = in ? Sc : R(0);
     fx -= (double)(A * dx + Bc * nx); fy -= (double)(A * dy + Bc * ny); fz -= (double)(A * dz + Bc * nz);
     sx += (double)(Sc * dx); sy += (double)(Sc * dy); sz += (double)(Sc * dz);
     if (E) e += in ? (double)(ep * pref) : 0.0;
-  }
+  });
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   sx = group_sum<T>(sx); sy = group_sum<T>(sy); sz = group_sum<T>(sz);
   if (E) e = group_sum<T>(e);
@@ -560,21 +612,19 @@ __device__ __forceinline__ void cos_metal_rows(const RowArgs& a, const CosP& P, 
   ROW_PROLOGUE_AT(T, block)
   const R R2 = P.R2, pref = P.pref;
   double fx = 0, fy = 0, fz = 0;
-  for (int p = t; p < n; p += T) {
-    int j = nbrow[p];
+  walk_row_aux<T, R>(G, nbrow, n, t, i, gnorm, [&](int, const typename Geo<R>::P& q, const double4& g, bool valid) {
     R dx, dy, dz;
-    R r = PM<R>::sqrt_(G.d2(pi, G.at(j), dx, dy, dz));
-    double4 g = gnorm[j];
+    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
     const R gx = g.x, gy = g.y, gz = g.z;
     R nd = gx * dx + gy * dy + gz * dz;
     R nn = P.morse ? PM<R>::sqrt_(gx * gx + gy * gy + gz * gz) * r : r;
     R A, Bc, Sc, ep;
     cos_pair<R>(P, r, nd, PM<R>::rcp(nn), A, Bc, Sc, ep);
-    bool in = r < R2;
+    bool in = valid && r < R2;
     A = in ? A * pref : R(0);
     Bc = in ? Bc * pref : R(0);
     fx -= (double)(A * dx + Bc * gx); fy -= (double)(A * dy + Bc * gy); fz -= (double)(A * dz + Bc * gz);
-  }
+  });
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (t == 0) write_force(a, i, fx, fy, fz);
 }
