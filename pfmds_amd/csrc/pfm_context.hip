@@ -52,18 +52,6 @@ void tim_end(pfm_ctx* h, hipEvent_t a, hipEvent_t b, const char* name) {
 
 // ------------------------------------------------------------ device kernels
 
-struct IntP {
-  double ts0, ts1;          // dt, dt/2
-  uint32_t bxyz, bz;        // group bits of xyz-moving and z-moving atoms
-  double box[3];
-  int nchain;
-  uint32_t chain_bit[PFM_MAXNHC];
-  const double* chain_s;    // per-chain velocity scale (device)
-  int scale, invert;
-  double zlo, zhi;
-  uint32_t zero_bit;  // zero the forces of these atoms after the first kick (zero_forces)
-  int pre_kick;       // first apply the previous step's second half kick (nve)
-};
 
 struct NhcDev {
   double x[16], v[16], q[16];
@@ -72,68 +60,14 @@ struct NhcDev {
   uint32_t bit;
 };
 
-__device__ __forceinline__ double wrap1(double x, double L) {
-  if (x > L) return x - L;
-  if (x < 0.0) return x + L;
-  return x;
-}
-__device__ __forceinline__ bool in_cell(double x, double L) {
-  const double tol = 0.0000001;
-  return x > (0.0 - tol) && x < (L + tol);
-}
-
-// invert_z_velocities (md_general.f90:382-398) on all atoms
-__device__ __forceinline__ void invert_v(double4& v, double z, double lo, double hi) {
-  double mid = (lo + hi) / 2;
-  if ((z > lo && z < mid && v.z > 0.0) || (z < hi && z > mid && v.z < 0.0)) v.z = -v.z;
-}
-
-// first half of a step: [invert_z], [NHC scaling], kick xyz/z, drift xyz/z
-// with single wrap (md_integrators.f90:7-97), and check_positions of the
-// next step (md_general.f90:342-364)
+// first half of a step for every atom (kick_drift_atom)
 __global__ void k_kick_drift(int N, double4* __restrict__ pos, double4* __restrict__ vel,
                              double4* __restrict__ frc, const uint32_t* __restrict__ mask, IntP P, DevErr* err,
                              const int* __restrict__ orig, int step, uint4* __restrict__ qpos, double3 qs,
                              const int* __restrict__ rep, const RepDev* __restrict__ rdev) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2], zlo = P.zlo, zhi = P.zhi;
-  if (rep) {
-    const RepDev& R = rdev[rep[i]];
-    Lx = R.L.x; Ly = R.L.y; Lz = R.L.z; zlo = R.zlo; zhi = R.zhi; qs = R.qs;
-  }
-  double4 p = pos[i], v = vel[i];
-  uint32_t m = mask[i];
-  bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
-  double4 f = (bx || bz) ? frc[i] : make_double4(0, 0, 0, 0);
-  if (P.pre_kick && (bx || bz)) {
-    // second half of the previous step (same operations as k_kick2)
-    double k = 1.0 / v.w / PFM_MASS_COEF;
-    if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
-    if (bz) v.z = v.z + f.z * k * P.ts1;
-  }
-  if (P.invert) invert_v(v, p.z, zlo, zhi);
-  if (P.scale) {
-    for (int c = 0; c < P.nchain; ++c)
-      if (m & P.chain_bit[c]) { double s = P.chain_s[c]; v.x *= s; v.y *= s; v.z *= s; }
-  }
-  if (bx || bz) {
-    double k = 1.0 / v.w / PFM_MASS_COEF;
-    if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
-    if (bz) v.z = v.z + f.z * k * P.ts1;
-    if (bx) {
-      p.x = wrap1(p.x + v.x * P.ts0, Lx);
-      p.y = wrap1(p.y + v.y * P.ts0, Ly);
-      p.z = wrap1(p.z + v.z * P.ts0, Lz);
-    }
-    if (bz) p.z = wrap1(p.z + v.z * P.ts0, Lz);
-    pos[i] = p;
-    if (qpos) qpos[i] = quantize(p, qs, 0u);
-  }
-  vel[i] = v;
-  if (m & P.zero_bit) frc[i] = make_double4(0, 0, 0, 0);
-  if (!(in_cell(p.x, Lx) && in_cell(p.y, Ly) && in_cell(p.z, Lz)))
-    raise_err(err, PFM_E_OUT_OF_CELL, orig[i] + 1, 0, step + 1);
+  kick_drift_atom(i, pos[i], mask[i], P, rep, rdev, qs, pos, vel, frc, qpos, false, err, orig, step);
 }
 
 // fixed-point copy of the positions (mixed path), refreshed at the start of

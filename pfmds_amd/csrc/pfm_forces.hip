@@ -16,6 +16,7 @@
 //   ljc/morsec  normals, graphene pass, normal-derivative pass, metal pass
 //           (graphenenorm.f90:38-56, LennardJonesCosine.f90, MorseCosine.f90)
 #include <algorithm>
+#include <cstring>
 
 #include "pfm_internal.h"
 

@@ -202,6 +202,87 @@ struct RepDev {
   double zlo, zhi;      // invert_z_velocities band
 };
 
+// ------------------------------------------------------------- integration
+
+struct IntP {
+  double ts0, ts1;          // dt, dt/2
+  uint32_t bxyz, bz;        // group bits of xyz-moving and z-moving atoms
+  double box[3];
+  int nchain;
+  uint32_t chain_bit[PFM_MAXNHC];
+  const double* chain_s;    // per-chain velocity scale (device)
+  int scale, invert;
+  double zlo, zhi;
+  uint32_t zero_bit;  // zero the forces of these atoms after the first kick (zero_forces)
+  int pre_kick;       // first apply the previous step's second half kick (nve)
+};
+
+__device__ __forceinline__ double wrap1(double x, double L) {
+  if (x > L) return x - L;
+  if (x < 0.0) return x + L;
+  return x;
+}
+__device__ __forceinline__ bool in_cell(double x, double L) {
+  const double tol = 0.0000001;
+  return x > (0.0 - tol) && x < (L + tol);
+}
+
+// invert_z_velocities (md_general.f90:382-398) on all atoms
+__device__ __forceinline__ void invert_v(double4& v, double z, double lo, double hi) {
+  double mid = (lo + hi) / 2;
+  if ((z > lo && z < mid && v.z > 0.0) || (z < hi && z > mid && v.z < 0.0)) v.z = -v.z;
+}
+
+// First half of a step for atom i: [the previous step's second half kick
+// (deferred), same operations as k_kick2], [invert_z_velocities], [NHC
+// scaling], kick xyz/z, drift xyz/z with single wrap (md_integrators.f90:7-97),
+// the fixed-point copy, zero_forces, and check_positions of the next step
+// (md_general.f90:342-364).  The position goes to pos_out and the fixed-point
+// copy to q_out; with out_of_place both are written for every atom (moving or
+// not), otherwise only where the atom moved.
+__device__ __forceinline__ void kick_drift_atom(int i, double4 p, uint32_t m, const IntP& P, const int* rep,
+                                                const RepDev* rdev, double3 qs, double4* pos_out, double4* vel,
+                                                double4* frc, uint4* q_out, bool out_of_place, DevErr* err,
+                                                const int* orig, int step) {
+  double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2], zlo = P.zlo, zhi = P.zhi;
+  if (rep) {
+    const RepDev& R = rdev[rep[i]];
+    Lx = R.L.x; Ly = R.L.y; Lz = R.L.z; zlo = R.zlo; zhi = R.zhi; qs = R.qs;
+  }
+  double4 v = vel[i];
+  bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
+  double4 f = (bx || bz) ? frc[i] : make_double4(0, 0, 0, 0);
+  if (P.pre_kick && (bx || bz)) {
+    double k = 1.0 / v.w / PFM_MASS_COEF;
+    if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
+    if (bz) v.z = v.z + f.z * k * P.ts1;
+  }
+  if (P.invert) invert_v(v, p.z, zlo, zhi);
+  if (P.scale) {
+    for (int c = 0; c < P.nchain; ++c)
+      if (m & P.chain_bit[c]) { double s = P.chain_s[c]; v.x *= s; v.y *= s; v.z *= s; }
+  }
+  if (bx || bz) {
+    double k = 1.0 / v.w / PFM_MASS_COEF;
+    if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
+    if (bz) v.z = v.z + f.z * k * P.ts1;
+    if (bx) {
+      p.x = wrap1(p.x + v.x * P.ts0, Lx);
+      p.y = wrap1(p.y + v.y * P.ts0, Ly);
+      p.z = wrap1(p.z + v.z * P.ts0, Lz);
+    }
+    if (bz) p.z = wrap1(p.z + v.z * P.ts0, Lz);
+  }
+  if (bx || bz || out_of_place) {
+    pos_out[i] = p;
+    if (q_out) q_out[i] = quantize(p, qs, 0u);
+  }
+  vel[i] = v;
+  if (m & P.zero_bit) frc[i] = make_double4(0, 0, 0, 0);
+  if (!(in_cell(p.x, Lx) && in_cell(p.y, Ly) && in_cell(p.z, Lz)))
+    raise_err(err, PFM_E_OUT_OF_CELL, orig[i] + 1, 0, step + 1);
+}
+
 // ------------------------------------------------------------------ host side
 
 struct ListDev {

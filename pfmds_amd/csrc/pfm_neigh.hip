@@ -384,21 +384,34 @@ __global__ void k_rev(int nrows, const int* __restrict__ rows, const int* __rest
   }
 }
 
+// lj1g: every atom's group-local index (the group of the list's second side)
+__global__ void k_lj1g_loc(int N, const int* __restrict__ orig, const int* __restrict__ loc, int2* __restrict__ lc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) lc[i].x = loc[orig[i]];
+}
+
 // lj1g: group-local index of the first listed neighbour whose atom index is
 // larger than the row atom's (the reference's lessnnum boundary, in list
-// order = ascending group order), stored with the row atom's own group-local
-// index for the force pass
-__global__ void k_lj1g_cut(int nrows, const int* __restrict__ rows, const int* __restrict__ nnum,
-                           const int* __restrict__ nbr, int stride, const int* __restrict__ orig,
-                           const int* __restrict__ loc, int2* __restrict__ lc) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+// order = ascending group order); T lanes per row, the minimum over the
+// row's entries
+template <int T>
+__global__ void __launch_bounds__(256) k_lj1g_cut(int nrows, const int* __restrict__ rows,
+                                                  const int* __restrict__ nnum, const int* __restrict__ nbr, int stride,
+                                                  const int* __restrict__ orig, int2* __restrict__ lc) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = gt / T, t = threadIdx.x & (T - 1);
   if (r >= nrows) return;
   int i = rows[r], oi = orig[i], n = nnum[r], c = 0x7fffffff;
-  for (int p = 0; p < n; ++p) {
-    int oj = orig[nbr[(size_t)r * stride + p]];
-    if (oj > oi) { int l = loc[oj]; c = l < c ? l : c; }
+  for (int p = t; p < n; p += T) {
+    int j = nbr[(size_t)r * stride + p];
+    if (orig[j] > oi) { int l = lc[j].x; c = l < c ? l : c; }
   }
-  lc[i] = make_int2(loc[oi], c);
+#pragma unroll
+  for (int o = T / 2; o > 0; o >>= 1) {
+    int v = __shfl_xor(c, o, T);
+    c = v < c ? v : c;
+  }
+  if (t == 0) lc[i].y = c;
 }
 
 // graphene nearest neighbours from the tb list: entries with |dr| < r_cut_nn,
@@ -461,6 +474,13 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
   if (!RG.tile) return;  // searched by k_build
   const int cs = a.cell_start[c];
   const int tid = threadIdx.x;
+  // cells without a row of this list (another species, another replica's
+  // interactions) stage nothing
+  {
+    int has = 0;
+    for (int m = tid; m < ncen; m += 256) has |= row_of[a.bin ? a.bin[cs + m] : cs + m] >= 0;
+    if (!__syncthreads_or(has)) return;
+  }
   const int pk = rank_xyz[c];
   const int cc[3] = {pk & 1023, (pk >> 10) & 1023, (pk >> 20) & 1023};
   const double org[3] = {cc[0] * RG.clen.x, cc[1] * RG.clen.y, cc[2] * RG.clen.z};
@@ -594,7 +614,7 @@ template <int T>
 static void launch_build_T(pfm_ctx* h, SearchArgs& a, const int* row_of) {
   long threads = (long)a.nrows * T;
   int nb = (int)((threads + 255) / 256);
-  PFM_LAUNCH(h, "build_list", k_build<T>, pad8(nb), 256, a, nb);
+  PFM_LAUNCH(h, "build_rows", k_build<T>, pad8(nb), 256, a, nb);
 }
 
 static void launch_build(pfm_ctx* h, SearchArgs& a, const int* row_of) {
@@ -651,8 +671,9 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
                L.stride, L.rev, h->derr, h->orig, step);
   }
   if (it.kind == PFM_LJ1G) {
-    PFM_LAUNCH(h, "list_cut", k_lj1g_cut, nb, 256, L.nrows, h->grp.rows[L.g1], L.nnum, L.nbr, L.stride, h->orig,
-               h->grp.loc[L.g2], it.lc);
+    PFM_LAUNCH(h, "list_cut", k_lj1g_loc, (h->N + 255) / 256, 256, h->N, h->orig, h->grp.loc[L.g2], it.lc);
+    PFM_LAUNCH(h, "list_cut", k_lj1g_cut<8>, (int)(((long)L.nrows * 8 + 255) / 256), 256, L.nrows, h->grp.rows[L.g1],
+               L.nnum, L.nbr, L.stride, h->orig, it.lc);
   }
   L.built = true;
   return PFM_OK;
