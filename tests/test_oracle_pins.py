@@ -349,3 +349,53 @@ def test_nvms_lj_dimer_minimum():
     p, _, _ = o.state()
     r = np.linalg.norm(p[1] - p[0])
     assert r == pytest.approx(2 ** (1 / 6) * sig, rel=1e-5)
+
+
+def test_nvms_velocity_rule():
+    """One nvms step against its definition (md_integrators.f90:99-145):
+    after the velocity-Verlet kicks, an xyz atom keeps only the projection
+    (v.F/F.F) F of its velocity when v.F > 0 and is stopped otherwise; a
+    z-only atom keeps its velocity when v.F > 0 and is stopped otherwise. The
+    forces at the new positions come from a fresh checker context."""
+    eps, sig, dt = 0.0104, 3.40, 2.0
+    pos = np.array([[5.0, 5.0, 5.0], [8.7, 5.4, 5.3], [5.3, 8.9, 4.6], [8.5, 8.6, 8.9]])
+    box = np.array([30.0, 30.0, 30.0])
+    mass = np.array([40.0, 39.0, 41.0, 38.0])
+
+    def dimer_system(p):
+        return System(box=box, pos=p.copy(), vel=np.zeros((4, 3)), mass=mass, species=["A", "A", "B", "B"],
+                      groups=[["A"], ["B"], ["A", "B"]],
+                      interactions=[Interaction("lj1g", [eps, sig, 6.5, 7.5], [(3, 3, 4, 8.5, 10)])],
+                      integrators=[Integrator("nvms", dt, 10, 10**9, 10**9)], all_moving=3, xyz_moving=1,
+                      z_moving=2, all_atoms=3)
+
+    s = dimer_system(pos)
+    o = step0(s)
+    f0 = o.state()[2]
+    fh = f0 / np.linalg.norm(f0, axis=1)[:, None]
+    # atoms 0 and 2 move along their force, atoms 1 and 3 against it
+    v0 = 0.02 * np.array([fh[0], -fh[1] + [0.0, 0.3, 0.0], fh[2], -fh[3]])
+    o.set_state(pos, v0)
+    o.run("nvms", dt, 1, 1, True, -1.0)
+    p1, v1, f1 = o.state()
+
+    a = dt / 2 / MASS_COEF / mass[:, None]
+    vh = v0 + f0 * a
+    xyz = np.array([True, True, False, False])
+    want_p = pos.copy()
+    want_p[xyz] += vh[xyz] * dt
+    want_p[~xyz, 2] += vh[~xyz, 2] * dt
+    np.testing.assert_allclose(p1, want_p, rtol=0, atol=1e-13)
+    f1_ref = step0(dimer_system(p1)).state()[2]
+    np.testing.assert_allclose(f1, f1_ref, rtol=1e-13, atol=1e-16)
+    vk = vh + f1 * a
+    vk[~xyz, :2] = v0[~xyz, :2]
+    want_v = np.zeros((4, 3))
+    for i in range(4):
+        fv = vk[i] @ f1[i]
+        if xyz[i] and fv > 0:
+            want_v[i] = fv / (f1[i] @ f1[i]) * f1[i]
+        elif not xyz[i] and fv > 0:
+            want_v[i] = vk[i]
+    assert (np.abs(want_v[[0, 2]]).max(1) > 0).all() and (want_v[[1, 3]] == 0).all()
+    np.testing.assert_allclose(v1, want_v, rtol=1e-12, atol=1e-18)
