@@ -116,6 +116,94 @@ def graphene_cu(steps, warmup):
     return out
 
 
+def surrogate_c4(train_iters=3000, frames_full=125, frames_sub=875, steps=10000, judge_every=100,
+                 threshold=0.05, mixed_ms=None, fp64_ms=None, log=None):
+    """Learned surrogate on graphene/Cu (94,560 atoms): frames from plain nvt
+    runs at 300-900 K on the full system and on a 1/16 sub-box (labels: the
+    FP64 tb + morsec path), training on the GPU, held-out force error, and a
+    judged surrogate nvt run at 300 K (rjl + sur, the plain potential every
+    `judge_every` steps, fallback past `threshold` eV/A)."""
+    from pfmds_amd import surrogate as sg
+    log = log or (lambda m: print("# c4: " + m, file=sys.stderr, flush=True))
+    temps = [300.0, 500.0, 700.0, 900.0]
+    s = configs.graphene_on_cu()
+    sub = configs.graphene_on_cu(cx=25, cy=15, gx=26, gy=16, seed=1014)
+    t0 = time.perf_counter()
+    fr_full = sg.generate_frames(s, temps, frames_full, every=50, warmup=200)
+    fr_sub = sg.generate_frames(sub, temps, frames_sub, every=50, warmup=200)
+    t_frames = time.perf_counter() - t0
+    log(f"{len(fr_full)} + {len(fr_sub)} frames in {t_frames:.1f} s")
+    # held out: every 25th full-system frame
+    hold = np.arange(len(fr_full)) % 25 == 24
+    test_f = fr_full.subset(np.nonzero(hold)[0])
+    train_full = fr_full.subset(np.nonzero(~hold)[0])
+    ctxs = []
+    for sysm in (s, sub):
+        _, sur_s, _ = sg.split_system(sysm)
+        ctxs.append(gpu_ctx(sg.training_system(sur_s), capi.MIXED))
+    n_c = len(s.group_indices(1))
+    mu, isg = sg.feature_scaling(ctxs[0], 1, train_full, n_c)
+    theta0 = sg.init_theta(mu, isg, float(np.mean(train_full.energy)) / n_c, seed=1004)
+    g_all = s.group_indices(s.all_atoms) - 1
+    ctxs[0].sur_set_params(1, theta0)
+    rmse0 = sg.force_rmse(ctxs[0], test_f, g_all)
+    t0 = time.perf_counter()
+    theta, hist = sg.train(ctxs, 1, [train_full, fr_sub], theta0, train_iters, lr=3e-3, lr_end=1e-4,
+                           seed=1004, log=log, log_every=500)
+    t_train = time.perf_counter() - t0
+    rmse1 = sg.force_rmse(ctxs[0], test_f, g_all)
+    rms_label = float(np.sqrt(np.mean(test_f.force[:, g_all].astype(np.float64) ** 2)))
+    log(f"trained {train_iters} iterations in {t_train:.1f} s: held-out force RMSE {rmse0:.4f} -> {rmse1:.4f} "
+        f"eV/A (labels {rms_label:.4f})")
+    for c in ctxs:
+        c.close()
+    # judged inference at 300 K
+    _, sur_s, _ = sg.split_system(s)
+    dt = s.integrators[-1].dt
+    sc = sg.surrogate_context(sur_s, theta)
+    pc = gpu_ctx(s, capi.MIXED)
+    moving = s.group_indices(s.all_moving) - 1
+    jr = sg.JudgedRun(sc, pc, moving, judge_every=judge_every, threshold=threshold)
+    jr.run("nvt", dt, 0, 20)
+    sc.run("nvt", dt, 21, 21, False)
+    t0 = time.perf_counter()
+    end = jr.run("nvt", dt, 22, 21 + steps)
+    sc.energies()
+    t_run = time.perf_counter() - t0
+    # the surrogate step alone (no judge), same handle
+    t0 = time.perf_counter()
+    if end is sc:
+        sc.run("nvt", dt, 22 + steps, 21 + 2 * steps // 5 + steps, False)
+        sc.energies()
+    t_pure = (time.perf_counter() - t0) / (2 * steps // 5) if end is sc else None
+    rm = [r for _, r in jr.judged]
+    out = {
+        "atoms": s.n, "carbon_atoms": n_c,
+        "frames": {"full_system": len(fr_full), "sub_box": len(fr_sub), "sub_box_atoms": sub.n,
+                   "temperatures_K": temps, "every_steps": 50, "seconds": t_frames},
+        "train": {"iterations": train_iters, "seconds": t_train, "loss_first_100": float(hist[:100, 0].mean()),
+                  "loss_last_100": float(hist[-100:, 0].mean())},
+        "heldout_force_rmse_eV_per_A": {"trained": rmse1, "untrained": rmse0, "labels_rms": rms_label,
+                                        "frames": int(hold.sum())},
+        "run": {"steps": steps, "judge_every": judge_every, "threshold_eV_per_A": threshold,
+                "judged_rmse_max": max(rm) if rm else None, "judged_rmse_mean": float(np.mean(rm)) if rm else None,
+                "judges": len(rm), "fallback_step": jr.switch_step, "judge_seconds": jr.t_judge},
+        "ms_per_step_with_judge": t_run * 1e3 / steps,
+        "atom_steps_per_s_with_judge": s.n * steps / t_run,
+        "ms_per_step_surrogate": t_pure * 1e3 if t_pure else None,
+        "atom_steps_per_s_surrogate": s.n / t_pure if t_pure else None,
+    }
+    if mixed_ms:
+        out["vs_mixed"] = mixed_ms / (t_run * 1e3 / steps)
+    if fp64_ms:
+        out["vs_fp64"] = fp64_ms / (t_run * 1e3 / steps)
+    log(f"surrogate run: {out['ms_per_step_with_judge']:.4f} ms/step with judge, judged RMSE max "
+        f"{out['run']['judged_rmse_max']}, fallback at {jr.switch_step}")
+    sc.close()
+    pc.close()
+    return out
+
+
 def replica_batch(steps, fp64_steps, check_steps=100, n_check=8):
     """512 independent replicas (binary LJ gas and fcc Cu, 256-32768 atoms
     each, about 3.4M atoms) stepped together with nve at dt = 0.5 fs, in mixed
@@ -186,6 +274,9 @@ def main():
     ap.add_argument("--c2-only", action="store_true", help="time only the graphene/Cu workload (profiling)")
     ap.add_argument("--no-c3", action="store_true", help="skip the replica batch")
     ap.add_argument("--c3-only", action="store_true", help="time only the replica batch (profiling)")
+    ap.add_argument("--no-c4", action="store_true", help="skip the learned surrogate")
+    ap.add_argument("--c4-only", action="store_true", help="run only the learned surrogate")
+    ap.add_argument("--c4-iters", type=int, default=3000, help="surrogate training iterations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step path has no multi-GPU decomposition "
@@ -197,6 +288,11 @@ def main():
         return
     if args.c3_only:
         print(json.dumps({"replica_batch": replica_batch(args.steps, args.steps, check_steps=10, n_check=2)}))
+        return
+    if args.c4_only:
+        c2 = graphene_cu(200, 20)
+        print(json.dumps({"surrogate": surrogate_c4(args.c4_iters, mixed_ms=c2["mixed_ms_per_step"],
+                                                    fp64_ms=c2["fp64_ms_per_step"]), "graphene_cu_nvt": c2}))
         return
     s = configs.fcc_cu_rgl(nx=args.cells, steps=args.steps)
     res = {}
@@ -266,6 +362,10 @@ def main():
 
     c2 = None if args.no_c2 else graphene_cu(200, 20)
     c3 = None if args.no_c3 else replica_batch(2000, 500)
+    c4 = None
+    if not args.no_c4:
+        c4 = surrogate_c4(args.c4_iters, mixed_ms=c2["mixed_ms_per_step"] if c2 else None,
+                          fp64_ms=c2["fp64_ms_per_step"] if c2 else None)
 
     cpu = None
     if not args.no_cpu:
@@ -299,6 +399,7 @@ def main():
         "roofline": roofline,
         "graphene_cu_nvt": c2,
         "replica_batch": c3,
+        "surrogate": c4,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -44,6 +44,14 @@ extern "C" {
 #define PFM_MORSEC 4  /* 'morsec' params: d r a delt R1 R2 simplified       (MorseCosine.f90:19-21)        */
 #define PFM_RJL    5  /* 'rjl'    params: A0 xi p q r0 R1 R2                (RosatoGuillopeLegrand.f90:17-18) */
 #define PFM_TB     6  /* 'tb'     params: d s b r0 delt a0 c0 d0 R1 R2      (TersoffBrenner.f90:18-19)     */
+#define PFM_SUR    7  /* 'sur'    params: r_c.  Learned stand-in for the graphene terms (tb + morsec):
+                         no counterpart in the reference.  Lists: nl(1) g1 -> g2 (g1 = atoms that
+                         carry an energy, the carbon; g2 = every atom their descriptors see,
+                         g1 included), nl(2) g2 -> g1, the converse (list r_cut >= r_c).  The
+                         network comes from pfm_sur_set_params.                                     */
+
+/* number of surrogate parameters: mu[40] isg[40] W1[64][40] b1[64] W2[64][64] b2[64] w3[64] b3 */
+#define PFM_SUR_NTHETA 6929
 
 /* integrators: the names of md_simulation.f90:73-80 */
 #define PFM_NVT  1
@@ -150,6 +158,28 @@ int pfm_kernel_time(pfm_ctx* h, const char* name, int* count, double* mean_ms);
  * workgroup size (64..1024); 0 keeps the current value.  No counterpart in
  * the reference (launch-shape control for tuning and benchmarks). */
 int pfm_set_tuning(pfm_ctx* h, int lanes_per_row, int block);
+
+/* Learned surrogate (interaction kind PFM_SUR).  None of these has a
+ * counterpart in the reference; they serve the training and inference of a
+ * per-atom network that stands in for TB_forces + MorseC_forces
+ * (TersoffBrenner.f90:74-150, MorseCosine.f90:53-139) on graphene over a
+ * metal.  E = sum over g1 atoms of NN((G_i - mu) * isg), G_i the 40
+ * descriptors of atom i, NN a 40-64-64-1 tanh perceptron on BF16 MFMA.
+ *
+ * pfm_sur_set_params: the network, in the PFM_SUR_NTHETA layout
+ *   mu[40] isg[40] W1[64][40] b1[64] W2[64][64] b2[64] w3[64] b3.
+ * pfm_sur_descriptors: G of every g1 atom at the current positions,
+ *   double[N_g1][40] in group order (lists from the last step).
+ * pfm_sur_loss_grad: at the current positions (lists from the last step),
+ *   the loss of one reference frame
+ *     L = w_e ((E - e_ref)/N_g1)^2 + w_f/(3 N_g2) sum_{a in g2} |F_a - f_ref,a|^2,
+ *   F the surrogate's own forces (they replace the handle's forces), f_ref
+ *   double[n_atoms][3] in original order; loss[3] = {L, energy term, force
+ *   term}; grad[PFM_SUR_NTHETA] = dL/dtheta (zero for mu and isg). */
+int pfm_sur_set_params(pfm_ctx* h, int inter, const double* theta, int n_theta);
+int pfm_sur_descriptors(pfm_ctx* h, int inter, double* G);
+int pfm_sur_loss_grad(pfm_ctx* h, int inter, const double* f_ref, double e_ref, double w_e, double w_f,
+                      double* loss, double* grad);
 
 int  pfm_last_error(pfm_ctx* h, char* buf, int len);
 int  pfm_destroy(pfm_ctx* h);

@@ -14,7 +14,8 @@ import numpy as np
 
 # constants of include/pfm.h
 OK, E_OVERFLOW, E_OUT_OF_CELL, E_GRAPHENE_NN, E_NHC_PARAMS, E_UNKNOWN_KIND, E_INVALID, E_HIP, E_NO_DEVICE = range(9)
-KINDS = {"lj": 1, "lj1g": 2, "ljc": 3, "morsec": 4, "rjl": 5, "tb": 6}
+KINDS = {"lj": 1, "lj1g": 2, "ljc": 3, "morsec": 4, "rjl": 5, "tb": 6, "sur": 7}
+SUR_NTHETA = 6929  # PFM_SUR_NTHETA
 INTEGRATORS = {"nvt": 1, "nve": 2, "nvms": 3}
 FP64, MIXED = 0, 1
 EV_DONE, EV_CONVERGED = 0, 1
@@ -74,6 +75,12 @@ def load_library(path, prefix):
         lib.pfm_kernel_time.restype = C.c_int
         lib.pfm_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.pfm_set_tuning.restype = C.c_int
+        lib.pfm_sur_set_params.argtypes = [C.c_void_p, C.c_int, _d, C.c_int]
+        lib.pfm_sur_set_params.restype = C.c_int
+        lib.pfm_sur_descriptors.argtypes = [C.c_void_p, C.c_int, _d]
+        lib.pfm_sur_descriptors.restype = C.c_int
+        lib.pfm_sur_loss_grad.argtypes = [C.c_void_p, C.c_int, _d, C.c_double, C.c_double, C.c_double, _d, _d]
+        lib.pfm_sur_loss_grad.restype = C.c_int
     _libs[key] = lib
     return lib
 
@@ -239,3 +246,26 @@ class Context:
 
     def set_tuning(self, lanes_per_row=0, block=0):
         self._check(self.lib.pfm_set_tuning(self.h, int(lanes_per_row), int(block)))
+
+    # -- learned surrogate (interaction kind 'sur') ---------------------------
+    def sur_set_params(self, inter, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float64)
+        if t.size != SUR_NTHETA:
+            raise ValueError(f"surrogate parameters: {t.size} values, expected {SUR_NTHETA}")
+        self._check(self.lib.pfm_sur_set_params(self.h, int(inter), _dp(t), int(t.size)))
+
+    def sur_descriptors(self, inter, n_rows):
+        """Descriptors of the surrogate's group-1 atoms, [n_rows, 40] in group order."""
+        g = np.zeros((max(n_rows, 1), 40))
+        self._check(self.lib.pfm_sur_descriptors(self.h, int(inter), _dp(g)))
+        return g[:n_rows]
+
+    def sur_loss_grad(self, inter, f_ref, e_ref, w_e, w_f):
+        """Loss {total, energy term, force term} of one reference frame at
+        the current positions and its gradient over the parameters."""
+        f = np.ascontiguousarray(f_ref, dtype=np.float64).reshape(self.n, 3)
+        loss = np.zeros(3)
+        grad = np.zeros(SUR_NTHETA)
+        self._check(self.lib.pfm_sur_loss_grad(self.h, int(inter), _dp(f), float(e_ref), float(w_e), float(w_f),
+                                               _dp(loss), _dp(grad)))
+        return loss, grad

@@ -182,7 +182,9 @@ def graphene_on_cu(cx=100, cy=58, layers=6, frozen=2, gx=104, gy=60, seed=1002, 
     box = np.array([Lx, Ly, lz])
     groups = [["C"], ["Cu_fixed", "Cu"], ["Cu", "C"], ["Cu_fixed", "Cu", "C"], ["Cu"], ["#"]]
     inters = [
-        Interaction("rjl", list(RJL_CU), [(2, 2, 96, RJL_CU[6] + 0.5, period)]),
+        # fcc Cu holds 86 neighbours within 6.5 A at rest; thermal motion in
+        # long runs takes single atoms past 96
+        Interaction("rjl", list(RJL_CU), [(2, 2, 128, RJL_CU[6] + 0.5, period)]),
         Interaction("tb", list(TB_BRENNER_I), [(1, 1, 8, TB_BRENNER_I[9] + 0.3, period)]),
         Interaction("morsec", list(MORSEC_CCU),
                     [(1, 2, 96, MORSEC_CCU[5] + 0.5, period), (2, 1, 128, MORSEC_CCU[5] + 0.5, period),

@@ -15,15 +15,6 @@
 
 #include "pfm_internal.h"
 
-#define HIPCHK(h, x)                                                                       \
-  do {                                                                                     \
-    hipError_t _e = (x);                                                                   \
-    if (_e != hipSuccess) {                                                                \
-      (h)->err = std::string("error: HIP ") + hipGetErrorString(_e) + " at " #x;           \
-      return PFM_E_HIP;                                                                    \
-    }                                                                                      \
-  } while (0)
-
 static int fail(pfm_ctx* h, int code, const std::string& msg) {
   h->err = msg;
   return code;
@@ -434,7 +425,7 @@ static int finalize(pfm_ctx* h) {
     for (int k = 0; k < it.nl_n; ++k) {
       ListDev& L = it.nl[k];
       if (k == 2 && it.nn_from_tb >= 0) continue;
-      double rc_eff = (k == 1 && (it.kind == PFM_LJ || is_cos(it.kind))) ? it.nl[0].rcut : L.rcut;
+      double rc_eff = (k == 1 && has_converse(it.kind)) ? it.nl[0].rcut : L.rcut;
       for (int r = 0; r < nrep; ++r)
         if (gcnt[L.g1][r] > 0 || gcnt[L.g2][r] > 0) rmax[r] = std::max(rmax[r], rc_eff);
     }
@@ -536,13 +527,20 @@ static int finalize(pfm_ctx* h) {
       L.stride = k == 2 ? L.nmax : (L.nmax + 3) / 4 * 4;
       if ((rc = dalloc(h, &L.nnum, L.nrows))) return rc;
       if ((rc = dalloc(h, &L.nbr, (size_t)L.nrows * L.stride + 32))) return rc;
-      if (it.kind == PFM_TB || k == 2)
+      if (it.kind == PFM_TB || k == 2 || (it.kind == PFM_SUR && k == 1))
         if ((rc = dalloc(h, &L.rev, (size_t)L.nrows * L.stride))) return rc;
     }
     it.e_rows = it.nl[0].nrows;
     if ((rc = dalloc(h, &it.e_row, it.e_rows))) return rc;
     if (it.kind == PFM_LJ1G)
       if ((rc = dalloc(h, &it.lc, h->N))) return rc;
+    if (it.kind == PFM_SUR) {
+      if ((rc = sur_alloc(h, it))) return rc;
+      if (!it.sur_theta.empty()) {
+        int ii = (int)(&it - h->inter.data()) + 1;
+        if ((rc = pfm_sur_set_params(h, ii, it.sur_theta.data(), (int)it.sur_theta.size()))) return rc;
+      }
+    }
     if (it.kind == PFM_TB) {
       size_t ne = (size_t)it.nl[0].nrows * it.nl[0].stride;
       if ((rc = dalloc(h, &it.B, ne))) return rc;
@@ -737,7 +735,7 @@ static int update_lists(pfm_ctx* h, int s) {
     for (int k = 0; k < it.nl_n; ++k) {
       int per = (k == 1 || (k == 2 && it.nn_from_tb >= 0)) ? -1 : it.nl[k].period;
       if (k == 2 && it.nn_from_tb >= 0) per = h->inter[it.nn_from_tb].nl[0].period;
-      if (k == 1 && (it.kind == PFM_LJ || is_cos(it.kind))) per = it.nl[0].period;
+      if (k == 1 && has_converse(it.kind)) per = it.nl[0].period;
       if (per > 0 && s % per != 0) all_due = false;
     }
   int rc;
@@ -761,8 +759,9 @@ static int update_lists(pfm_ctx* h, int s) {
     bool due = s % it.nl[0].period == 0 || !it.nl[0].built;
     if (due) {
       if ((rc = launch_build_list(h, (int)ii, 0, s))) return rc;
-      if (it.kind == PFM_LJ || is_cos(it.kind))
+      if (has_converse(it.kind))
         if ((rc = launch_build_list(h, (int)ii, 1, s))) return rc;
+      if (it.kind == PFM_SUR) launch_sur_rev(h, it);
     }
     if (is_cos(it.kind)) {
       if (it.nn_from_tb >= 0) {
@@ -901,10 +900,10 @@ int pfm_set_roles(pfm_ctx* h, int all_moving, int xyz_moving, int z_moving, int 
 
 int pfm_add_interaction(pfm_ctx* h, int kind, const double* params, int np, int nl_n, const int* gn,
                         const int* nmax, const double* rcut, const int* period) {
-  static const int need_np[7] = {0, 4, 4, 6, 7, 7, 10};
-  static const int need_nl[7] = {0, 2, 1, 3, 3, 1, 1};
+  static const int need_np[8] = {0, 4, 4, 6, 7, 7, 10, 1};
+  static const int need_nl[8] = {0, 2, 1, 3, 3, 1, 1, 2};
   if (h->sorted_once) return fail(h, PFM_E_INVALID, "error: interactions must be added before the first pfm_run");
-  if (kind < 1 || kind > 6) return fail(h, PFM_E_UNKNOWN_KIND, "error: unknown interaction name");
+  if (kind < 1 || kind > 7) return fail(h, PFM_E_UNKNOWN_KIND, "error: unknown interaction name");
   if ((int)h->inter.size() >= PFM_MAXI) return fail(h, PFM_E_INVALID, "error: too many interactions");
   if (np < need_np[kind] || nl_n != need_nl[kind])
     return fail(h, PFM_E_INVALID, "error: wrong parameter or list count for interaction");
@@ -928,6 +927,17 @@ int pfm_add_interaction(pfm_ctx* h, int kind, const double* params, int np, int 
     // (LennardJones_1g.f90:83, RosatoGuillopeLegrand.f90:88, TersoffBrenner.f90:56-57,112)
     if (h->grp.idx[it.nl[0].g1] != h->grp.idx[it.nl[0].g2])
       return fail(h, PFM_E_INVALID, "error: lj1g/rjl/tb need the same group on both sides of the list");
+  }
+  if (kind == PFM_SUR) {
+    std::vector<char> in2(h->N, 0);
+    for (int x : h->grp.idx[it.nl[0].g2]) in2[x] = 1;
+    for (int x : h->grp.idx[it.nl[0].g1])
+      if (!in2[x]) return fail(h, PFM_E_INVALID, "error: sur group 1 must be part of group 2");
+    if (it.nl[1].g1 != it.nl[0].g2 || it.nl[1].g2 != it.nl[0].g1)
+      return fail(h, PFM_E_INVALID, "error: sur list 2 must be the converse of list 1");
+    if (!(it.p[0] > 0.0) || it.nl[0].rcut < it.p[0])
+      return fail(h, PFM_E_INVALID, "error: sur list r_cut must be at least the descriptor cutoff");
+    h->grp.used[it.nl[0].g2] = true;
   }
   if (kind == PFM_TB && it.nl[0].nmax > PFM_TB_MAXN)
     return fail(h, PFM_E_INVALID, "error: tb neighb_num_max above 16 is not supported");
@@ -1269,6 +1279,7 @@ int pfm_destroy(pfm_ctx* h) {
   for (auto& it : h->inter) {
     for (int k = 0; k < 3; ++k) { F(it.nl[k].nnum); F(it.nl[k].nbr); F(it.nl[k].rev); }
     F(it.e_row); F(it.lc); F(it.B); F(it.Bp); F(it.gnorm); F(it.gS);
+    sur_free(it);
   }
   for (auto e : h->evpool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -753,6 +753,15 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         written = true;
         break;
       }
+      case PFM_SUR: {
+        int rc = launch_sur_forces(h, it, written ? 1 : 0);
+        if (rc) {
+          h->err = "error: surrogate parameters not set";
+          return rc;
+        }
+        written = true;
+        break;
+      }
       case PFM_LJC:
       case PFM_MORSEC: {
         bool morse = it.kind == PFM_MORSEC;

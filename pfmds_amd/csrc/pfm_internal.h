@@ -295,6 +295,23 @@ struct ListDev {
   bool built = false;
 };
 
+// device buffers of a surrogate interaction (pfm_surrogate.hip)
+struct SurBuf {
+  float *G = nullptr, *M = nullptr, *gG = nullptr;  // descriptors, moments, dE/dG per row
+  float4* d = nullptr;                              // dE_i/d(dr_ij) per list entry
+  float* fp = nullptr;                              // biases, output weights, feature scaling
+  uint16_t* W = nullptr;                            // BF16 W1, W1^T, W2, W2^T
+  bool have_params = false;
+  // training
+  float* dG = nullptr;
+  float4* res = nullptr;
+  double* fref = nullptr;
+  uint16_t* arr = nullptr;
+  float *bpart = nullptr, *wpart = nullptr;
+  double *rpart = nullptr, *scal = nullptr, *grad = nullptr;
+  int Np = 0, S = 0;
+};
+
 struct Inter {
   int kind = 0, nl_n = 0, g[6] = {0, 0, 0, 0, 0, 0};
   double p[16] = {0};
@@ -312,6 +329,9 @@ struct Inter {
   double4* gnorm = nullptr;  // [N] by internal atom
   bool rows_disjoint = false;  // groups 1 and 2 share no atom (passes over both may share a launch)
   double4* gS = nullptr;     // [N] sum of V2 V3 fc/(n.dr) dr per graphene atom
+  // surrogate
+  SurBuf* sur = nullptr;
+  std::vector<double> sur_theta;  // parameters set before the buffers exist
 };
 
 struct Nhc {
@@ -401,6 +421,15 @@ struct pfm_ctx {
   std::string err;
 };
 
+#define HIPCHK(h, x)                                                                       \
+  do {                                                                                     \
+    hipError_t _e = (x);                                                                   \
+    if (_e != hipSuccess) {                                                                \
+      (h)->err = std::string("error: HIP ") + hipGetErrorString(_e) + " at " #x;           \
+      return PFM_E_HIP;                                                                    \
+    }                                                                                      \
+  } while (0)
+
 // launch bookkeeping (pfm_context.hip)
 void tim_begin(pfm_ctx* h, const char* name, hipEvent_t* a, hipEvent_t* b);
 void tim_end(pfm_ctx* h, hipEvent_t a, hipEvent_t b, const char* name);
@@ -422,3 +451,11 @@ int launch_nn_from_tb(pfm_ctx* h, int ii, int step);
 int launch_forces(pfm_ctx* h, bool energy, bool first_writes);
 void launch_group_rows(pfm_ctx* h);
 void scan_excl(pfm_ctx* h, int n, const int* in, int* out, int* tmp);
+int sur_alloc(pfm_ctx* h, Inter& it);
+void sur_free(Inter& it);
+void launch_sur_rev(pfm_ctx* h, Inter& it);
+int launch_sur_forces(pfm_ctx* h, Inter& it, int accumulate);
+// interactions with a converse list nl(2) built with nl(1)'s cutoff and period
+static inline bool has_converse(int kind) {
+  return kind == PFM_LJ || kind == PFM_LJC || kind == PFM_MORSEC || kind == PFM_SUR;
+}

@@ -648,7 +648,7 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   a.rep = h->rep;
   a.cell_rep = h->cell_rep;
   a.bin = h->bin;
-  double rc = (k == 1 && (it.kind == PFM_LJ || it.kind == PFM_LJC || it.kind == PFM_MORSEC)) ? it.nl[0].rcut : L.rcut;
+  double rc = (k == 1 && has_converse(it.kind)) ? it.nl[0].rcut : L.rcut;
   a.rc2 = rc * rc;
   a.rc2_lo = a.rc2 * (1.0 - 1e-12);
   a.rc2_hi = a.rc2 * (1.0 + 1e-12);
