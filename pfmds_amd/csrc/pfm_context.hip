@@ -759,9 +759,9 @@ static int update_lists(pfm_ctx* h, int s) {
     bool due = s % it.nl[0].period == 0 || !it.nl[0].built;
     if (due) {
       if ((rc = launch_build_list(h, (int)ii, 0, s))) return rc;
-      if (has_converse(it.kind))
+      if (it.kind == PFM_SUR) launch_sur_converse(h, it, s);
+      else if (has_converse(it.kind))
         if ((rc = launch_build_list(h, (int)ii, 1, s))) return rc;
-      if (it.kind == PFM_SUR) launch_sur_rev(h, it);
     }
     if (is_cos(it.kind)) {
       if (it.nn_from_tb >= 0) {

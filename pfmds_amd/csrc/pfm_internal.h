@@ -299,6 +299,7 @@ struct ListDev {
 struct SurBuf {
   float *G = nullptr, *M = nullptr, *gG = nullptr;  // descriptors, moments, dE/dG per row
   float4* d = nullptr;                              // dE_i/d(dr_ij) per list entry
+  double4* fself = nullptr;                         // sum_j dE_i/d(dr_ij) per row
   float* fp = nullptr;                              // biases, output weights, feature scaling
   uint16_t* W = nullptr;                            // BF16 W1, W1^T, W2, W2^T
   bool have_params = false;
@@ -453,7 +454,7 @@ void launch_group_rows(pfm_ctx* h);
 void scan_excl(pfm_ctx* h, int n, const int* in, int* out, int* tmp);
 int sur_alloc(pfm_ctx* h, Inter& it);
 void sur_free(Inter& it);
-void launch_sur_rev(pfm_ctx* h, Inter& it);
+void launch_sur_converse(pfm_ctx* h, Inter& it, int step);
 int launch_sur_forces(pfm_ctx* h, Inter& it, int accumulate);
 // interactions with a converse list nl(2) built with nl(1)'s cutoff and period
 static inline bool has_converse(int kind) {

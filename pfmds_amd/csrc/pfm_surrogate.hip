@@ -99,6 +99,7 @@ struct SurK {
   float* M;       // [rows * 4 * 36] moments: set q = (species q/2, eta q%2), 35 monomials + sum w^2
   float* gG;      // [rows * 40] dE_i/dG_i
   float4* d;      // [rows * stride0] dE_i/d(dr_ij)
+  double4* fself; // [rows] sum_j dE_i/d(dr_ij)
   float* dG;      // [rows * 40] tangent descriptors
   const float4* res;  // [N] force residual (training)
   const float* fp;    // b1[64] b2[64] w3[64] mu[48] isg[48] b3
@@ -142,9 +143,11 @@ template <> __device__ __forceinline__ Geo<float> sgeo<float>(const RowArgsS& a,
   return Geo<float>{a.qpos, a.hq.x, a.hq.y, a.hq.z};
 }
 
-// Eight lanes per group-1 row: lanes 0-3 own the moment set q (species q/2,
-// angular eta q%2), lanes 4-7 own three radial etas for both species.  Every
-// lane walks the whole row.
+// Eight lanes per group-1 row: lane q takes the angular eta q & 1 and the
+// entries p = q/2, q/2 + 4, ... of the row; it accumulates both species
+// through arithmetic masks (no divergence between carbon and metal
+// neighbours) and six of the twelve radial etas.  The four entry groups are
+// summed by butterfly shuffles at the end.
 #define SUR_ROW_PROLOGUE                                  \
   int lb = xcd_block(nb_real);                            \
   if (lb < 0) return;                                     \
@@ -156,21 +159,46 @@ template <> __device__ __forceinline__ Geo<float> sgeo<float>(const RowArgsS& a,
   const typename Geo<R>::P pi = Gm.at(i);                 \
   int n = a.nnum[row];                                    \
   const int* nbrow = a.nbr + (size_t)row * a.stride;      \
-  const int lane = threadIdx.x & 63;                      \
-  const bool ang = q < 4;                                 \
-  const int sp = (q >> 1) & 1;                            \
-  const int ke = q & 1;                                   \
-  const int k0 = 3 * ((q - 4) & 3);
+  const int ke = q & 1, eg = q >> 1;                      \
+  const float eta = K.etaa[ke];                           \
+  float e2k[6];                                           \
+  _Pragma("unroll") for (int k = 0; k < 6; ++k) e2k[k] = K.eta2[6 * ke + k];
 
-// separation of entry j, in FP32 (dr in FP64 then rounded, or fixed point)
+// sum over the four entry groups of a row (lanes q, q^2, q^4, q^6)
+template <int N, class V>
+__device__ __forceinline__ void sum_groups(V* v) {
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    v[c] += __shfl_xor(v[c], 2, 8);
+    v[c] += __shfl_xor(v[c], 4, 8);
+  }
+}
+
+// separation of a gathered entry, in FP32 (dr in FP64 then rounded, or fixed point)
 template <class R>
-__device__ __forceinline__ float sur_dr(const Geo<R>& Gm, const typename Geo<R>::P& pi, int j, float& x, float& y,
-                                        float& z) {
+__device__ __forceinline__ float sur_dr_rec(const Geo<R>& Gm, const typename Geo<R>::P& pi,
+                                            const typename Geo<R>::P& pj, float& x, float& y, float& z) {
   R dx, dy, dz;
-  Gm.d2(pi, Gm.at(j), dx, dy, dz);
+  Gm.d2(pi, pj, dx, dy, dz);
   x = (float)dx; y = (float)dy; z = (float)dz;
   return x * x + y * y + z * z;
 }
+
+// The row walks gather the next entry's record (index, position, group
+// bits) before the arithmetic on the current one.
+#define SUR_WALK_BEGIN                                                        \
+  int jn = eg < n ? nbrow[eg] : i;                                            \
+  typename Geo<R>::P pn = Gm.at(jn);                                          \
+  uint32_t mn = mask[jn];                                                     \
+  for (int p = eg; p < n; p += 4) {                                           \
+    const int j = jn;                                                         \
+    const typename Geo<R>::P pj = pn;                                         \
+    const uint32_t mj = mn;                                                   \
+    if (p + 4 < n) {                                                          \
+      jn = nbrow[p + 4];                                                      \
+      pn = Gm.at(jn);                                                         \
+      mn = mask[jn];                                                          \
+    }
 
 __device__ __forceinline__ void sur_cut(const SurK& K, float r, float& fc, float& fcp) {
   float s, c;
@@ -199,68 +227,68 @@ __device__ __forceinline__ void g5_adj(const float* g, float* dS) {
 template <class R>
 __global__ void __launch_bounds__(256) k_sur_desc(RowArgsS a, SurK K, const uint32_t* __restrict__ mask, int nb_real) {
   SUR_ROW_PROLOGUE
-  float acc[NMON + 1];
+  float mc[NMON], mx[NMON], dd[1] = {0.0f}, gc[6], gx[6];
 #pragma unroll
-  for (int c = 0; c <= NMON; ++c) acc[c] = 0.0f;
-  const float eta = ang ? K.etaa[ke] : 0.0f;
-  const float e0 = K.eta2[k0], e1 = K.eta2[k0 + 1], e2 = K.eta2[k0 + 2];
-  for (int p = 0; p < n; ++p) {
-    int j = nbrow[p];
+  for (int t = 0; t < NMON; ++t) { mc[t] = 0.0f; mx[t] = 0.0f; }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) { gc[k] = 0.0f; gx[k] = 0.0f; }
+  SUR_WALK_BEGIN
+    (void)j;
     float dx, dy, dz;
-    float r2 = sur_dr<R>(Gm, pi, j, dx, dy, dz);
+    float r2 = sur_dr_rec<R>(Gm, pi, pj, dx, dy, dz);
     if (!(r2 < K.rc2)) continue;
-    int sj = (mask[j] & K.bit1) ? 0 : 1;
+    bool cj = (mj & K.bit1) != 0;
     float r = __builtin_amdgcn_sqrtf(r2), ir = 1.0f / r;
     float fc, fcp;
     sur_cut(K, r, fc, fcp);
-    if (ang) {
-      if (sj == sp) {
-        float w = __expf(-eta * r2) * fc;
-        float u[3] = {dx * ir, dy * ir, dz * ir};
-        float mono[NMON];
-        mono[0] = 1.0f;
+    (void)fcp;
+    float w = __expf(-eta * r2) * fc;
+    float wc = cj ? w : 0.0f, wx = w - wc;
+    float u[3] = {dx * ir, dy * ir, dz * ir};
+    float mono[NMON];
+    mono[0] = 1.0f;
 #pragma unroll
-        for (int t = 1; t < NMON; ++t) mono[t] = mono[kPar[t]] * u[kCrd[t]];
+    for (int t = 1; t < NMON; ++t) mono[t] = mono[kPar[t]] * u[kCrd[t]];
 #pragma unroll
-        for (int t = 0; t < NMON; ++t) acc[t] += w * mono[t];
-        acc[NMON] += w * w;
-      }
-    } else {
-      float v0 = __expf(-e0 * r2) * fc, v1 = __expf(-e1 * r2) * fc, v2 = __expf(-e2 * r2) * fc;
-      if (sj == 0) { acc[0] += v0; acc[1] += v1; acc[2] += v2; }
-      else { acc[3] += v0; acc[4] += v1; acc[5] += v2; }
+    for (int t = 0; t < NMON; ++t) { mc[t] += wc * mono[t]; mx[t] += wx * mono[t]; }
+    dd[0] += wc * w;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      float v = __expf(-e2k[k] * r2) * fc;
+      gc[k] += cj ? v : 0.0f;
+      gx[k] += cj ? 0.0f : v;
     }
   }
+  sum_groups<NMON>(mc);
+  sum_groups<NMON>(mx);
+  sum_groups<1>(dd);
+  sum_groups<6>(gc);
+  sum_groups<6>(gx);
+  if (eg != 0) return;
   float* Gr = K.G + (size_t)row * NF;
-  if (!ang) {
-    Gr[k0] = acc[0]; Gr[k0 + 1] = acc[1]; Gr[k0 + 2] = acc[2];
-    Gr[12 + k0] = acc[3]; Gr[12 + k0 + 1] = acc[4]; Gr[12 + k0 + 2] = acc[5];
-  } else {
-    float* Mr = K.M + ((size_t)row * 4 + q) * 36;
 #pragma unroll
-    for (int c = 0; c <= NMON; ++c) Mr[c] = acc[c];
+  for (int k = 0; k < 6; ++k) { Gr[6 * ke + k] = gc[k]; Gr[12 + 6 * ke + k] = gx[k]; }
+  float* Mc = K.M + ((size_t)row * 4 + ke) * 36;
+  float* Mx = K.M + ((size_t)row * 4 + 2 + ke) * 36;
+#pragma unroll
+  for (int t = 0; t < NMON; ++t) { Mc[t] = mc[t]; Mx[t] = mx[t]; }
+  Mc[NMON] = dd[0];
+  Mx[NMON] = 0.0f;
+  float Scc[5] = {0, 0, 0, 0, 0}, Scx[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int t = 0; t < NMON; ++t) {
+    Scc[kDeg[t]] += kMul[t] * mc[t] * mc[t];
+    Scx[kDeg[t]] += kMul[t] * mc[t] * mx[t];
   }
-  // lanes 0, 1 (carbon sets) combine with lanes 2, 3 (metal sets, same eta)
-  float pv[NMON];
 #pragma unroll
-  for (int c = 0; c < NMON; ++c) pv[c] = __shfl(acc[c], (lane + 2) & 63);
-  if (q < 2) {
-    float Scc[5] = {0, 0, 0, 0, 0}, Scx[5] = {0, 0, 0, 0, 0};
+  for (int m = 0; m < 5; ++m) Scc[m] -= dd[0];
+  float o[4];
+  g5_of(Scc, o);
 #pragma unroll
-    for (int t = 0; t < NMON; ++t) {
-      Scc[kDeg[t]] += kMul[t] * acc[t] * acc[t];
-      Scx[kDeg[t]] += kMul[t] * acc[t] * pv[t];
-    }
+  for (int m = 0; m < 4; ++m) Gr[24 + ke * 4 + m] = o[m];
+  g5_of(Scx, o);
 #pragma unroll
-    for (int m = 0; m < 5; ++m) Scc[m] -= acc[NMON];
-    float o[4];
-    g5_of(Scc, o);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) Gr[24 + ke * 4 + m] = o[m];
-    g5_of(Scx, o);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) Gr[32 + ke * 4 + m] = o[m];
-  }
+  for (int m = 0; m < 4; ++m) Gr[32 + ke * 4 + m] = o[m];
 }
 
 // Perceptron on MFMA: 16 atoms per wavefront.  Writes E_i (e_row) and
@@ -341,91 +369,90 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
   }
 }
 
-// dE_i/d(dr_ij) of every entry of the row
+// dE_i/d(dr_ij) of every entry of the row, and the row atom's own share
+// sum_j dE_i/d(dr_ij) (FP64)
 template <class R>
 __global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint32_t* __restrict__ mask, int nb_real) {
   SUR_ROW_PROLOGUE
-  (void)lane;
   const float* g = K.gG + (size_t)row * NF;
-  float Ab[NMON], dD = 0.0f;
-  float eta = 0.0f, c0[3] = {0, 0, 0}, c1[3] = {0, 0, 0}, e0 = 0, e1 = 0, e2 = 0;
-  if (ang) {
-    eta = K.etaa[ke];
-    const float* Mc = K.M + ((size_t)row * 4 + ke) * 36;
-    const float* Mx = K.M + ((size_t)row * 4 + 2 + ke) * 36;
-    float gcc[4], gcx[4], dScc[5], dScx[5];
+  const float* Mc = K.M + ((size_t)row * 4 + ke) * 36;
+  const float* Mx = K.M + ((size_t)row * 4 + 2 + ke) * 36;
+  float gcc[4], gcx[4], dScc[5], dScx[5];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) { gcc[m] = g[24 + ke * 4 + m]; gcx[m] = g[32 + ke * 4 + m]; }
-    g5_adj(gcc, dScc);
-    g5_adj(gcx, dScx);
+  for (int m = 0; m < 4; ++m) { gcc[m] = g[24 + ke * 4 + m]; gcx[m] = g[32 + ke * 4 + m]; }
+  g5_adj(gcc, dScc);
+  g5_adj(gcx, dScx);
+  float ac[NMON], ax[NMON];
 #pragma unroll
-    for (int t = 0; t < NMON; ++t) {
-      float mc = Mc[t], mx = Mx[t];
-      Ab[t] = kMul[t] * (sp == 0 ? 2.0f * mc * dScc[kDeg[t]] + mx * dScx[kDeg[t]] : mc * dScx[kDeg[t]]);
-    }
-    if (sp == 0) dD = -(dScc[0] + dScc[1] + dScc[2] + dScc[3] + dScc[4]);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) { c0[k] = g[k0 + k]; c1[k] = g[12 + k0 + k]; }
-    e0 = K.eta2[k0]; e1 = K.eta2[k0 + 1]; e2 = K.eta2[k0 + 2];
-#pragma unroll
-    for (int t = 0; t < NMON; ++t) Ab[t] = 0.0f;
+  for (int t = 0; t < NMON; ++t) {
+    float mcv = Mc[t], mxv = Mx[t];
+    ac[t] = kMul[t] * (2.0f * mcv * dScc[kDeg[t]] + mxv * dScx[kDeg[t]]);
+    ax[t] = kMul[t] * mcv * dScx[kDeg[t]];
   }
+  const float dD = -(dScc[0] + dScc[1] + dScc[2] + dScc[3] + dScc[4]);
+  float cc[6], cx[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) { cc[k] = g[6 * ke + k]; cx[k] = g[12 + 6 * ke + k]; }
   float4* drow = K.d + (size_t)row * K.stride0;
-  for (int p = 0; p < n; ++p) {
-    int j = nbrow[p];
+  double sf[3] = {0.0, 0.0, 0.0};
+  // the two eta lanes of an entry group walk the same entries
+  SUR_WALK_BEGIN
+    (void)j;
     float dx, dy, dz;
-    float r2 = sur_dr<R>(Gm, pi, j, dx, dy, dz);
-    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    float r2 = sur_dr_rec<R>(Gm, pi, pj, dx, dy, dz);
+    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
     if (r2 < K.rc2) {
-      int sj = (mask[j] & K.bit1) ? 0 : 1;
+      bool cj = (mj & K.bit1) != 0;
       float r = __builtin_amdgcn_sqrtf(r2), ir = 1.0f / r;
       float fc, fcp;
       sur_cut(K, r, fc, fcp);
       float u[3] = {dx * ir, dy * ir, dz * ir};
-      if (ang) {
-        if (sj == sp) {
-          float ex = __expf(-eta * r2);
-          float w = ex * fc, wp = ex * (fcp - 2.0f * eta * r * fc);
-          float mono[NMON], adj[NMON];
-          mono[0] = 1.0f;
+      float ex = __expf(-eta * r2);
+      float w = ex * fc, wp = ex * (fcp - 2.0f * eta * r * fc);
+      float mono[NMON], adj[NMON];
+      mono[0] = 1.0f;
 #pragma unroll
-          for (int t = 1; t < NMON; ++t) mono[t] = mono[kPar[t]] * u[kCrd[t]];
-          float P = 0.0f;
+      for (int t = 1; t < NMON; ++t) mono[t] = mono[kPar[t]] * u[kCrd[t]];
+      float P = 0.0f;
 #pragma unroll
-          for (int t = 0; t < NMON; ++t) { P += Ab[t] * mono[t]; adj[t] = Ab[t]; }
-          float gu[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-          for (int t = NMON - 1; t >= 1; --t) {
-            adj[kPar[t]] += adj[t] * u[kCrd[t]];
-            gu[kCrd[t]] += adj[t] * mono[kPar[t]];
-          }
-          float Q = u[0] * gu[0] + u[1] * gu[1] + u[2] * gu[2];
-          float wr = w * ir;
-          float cu = wp * P - wr * Q + 2.0f * dD * w * wp;
-          cx = cu * u[0] + wr * gu[0];
-          cy = cu * u[1] + wr * gu[1];
-          cz = cu * u[2] + wr * gu[2];
-        }
-      } else {
-        float x0 = __expf(-e0 * r2), x1 = __expf(-e1 * r2), x2 = __expf(-e2 * r2);
-        float p0 = x0 * (fcp - 2.0f * e0 * r * fc), p1 = x1 * (fcp - 2.0f * e1 * r * fc),
-              p2 = x2 * (fcp - 2.0f * e2 * r * fc);
-        float cu = sj == 0 ? c0[0] * p0 + c0[1] * p1 + c0[2] * p2 : c1[0] * p0 + c1[1] * p1 + c1[2] * p2;
-        cx = cu * u[0]; cy = cu * u[1]; cz = cu * u[2];
+      for (int t = 0; t < NMON; ++t) {
+        adj[t] = cj ? ac[t] : ax[t];
+        P += adj[t] * mono[t];
       }
+      float gu[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int t = NMON - 1; t >= 1; --t) {
+        adj[kPar[t]] += adj[t] * u[kCrd[t]];
+        gu[kCrd[t]] += adj[t] * mono[kPar[t]];
+      }
+      float Q = u[0] * gu[0] + u[1] * gu[1] + u[2] * gu[2];
+      float wr = w * ir;
+      float cu = wp * P - wr * Q + (cj ? 2.0f * dD * w * wp : 0.0f);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        float x = __expf(-e2k[k] * r2);
+        cu += (cj ? cc[k] : cx[k]) * x * (fcp - 2.0f * e2k[k] * r * fc);
+      }
+      c0 = cu * u[0] + wr * gu[0];
+      c1 = cu * u[1] + wr * gu[1];
+      c2 = cu * u[2] + wr * gu[2];
     }
-    cx = group_sum<8>(cx);
-    cy = group_sum<8>(cy);
-    cz = group_sum<8>(cz);
-    if (q == 0) drow[p] = make_float4(cx, cy, cz, 0.0f);
+    c0 += __shfl_xor(c0, 1, 8);
+    c1 += __shfl_xor(c1, 1, 8);
+    c2 += __shfl_xor(c2, 1, 8);
+    if (ke == 0) drow[p] = make_float4(c0, c1, c2, 0.0f);
+    sf[0] += c0; sf[1] += c1; sf[2] += c2;
   }
+  sum_groups<3>(sf);
+  if (q == 0) K.fself[row] = make_double4(sf[0], sf[1], sf[2], 0.0);
 }
 
-// F_a = sum_{j in N(a)} d_aj - sum_{i: a in N(i)} d_ia, rows of the converse list
+// F_a = sum_{j in N(a)} d_aj - sum_{i: a in N(i)} d_ia over the rows of the
+// converse list; the first sum is the row share k_sur_back left for a, the
+// second is exact in 64-bit fixed point (|d| < 2^23 eV/A)
 template <int T>
 __global__ void __launch_bounds__(256) k_sur_gather(RowArgsS c, const int* __restrict__ row_of1,
-                                                    const int* __restrict__ nnum0, int stride0,
+                                                    const double4* __restrict__ fself,
                                                     const int* __restrict__ rev, const float4* __restrict__ d,
                                                     double4* __restrict__ frc, int accumulate, int nb_real) {
   int lb = xcd_block(nb_real);
@@ -434,48 +461,70 @@ __global__ void __launch_bounds__(256) k_sur_gather(RowArgsS c, const int* __res
   int row = gt / T, t = threadIdx.x & (T - 1);
   if (row >= c.nrows) return;
   int a = c.rows[row];
-  double fx = 0.0, fy = 0.0, fz = 0.0;
-  int r0 = row_of1[a];
-  if (r0 >= 0) {
-    int n0 = nnum0[r0];
-    const float4* dr = d + (size_t)r0 * stride0;
-    for (int p = t; p < n0; p += T) { float4 v = dr[p]; fx += v.x; fy += v.y; fz += v.z; }
-  }
+  // the converse entries in units of 2^-40 eV/A: integer sums do not depend
+  // on the order of the row
+  long long qx = 0, qy = 0, qz = 0;
   int n = c.nnum[row];
   const int* rv = rev + (size_t)row * c.stride;
   for (int p = t; p < n; p += T) {
-    int e = rv[p];
-    if (e >= 0) { float4 v = d[e]; fx -= v.x; fy -= v.y; fz -= v.z; }
+    float4 v = d[rv[p]];
+    qx -= __float2ll_rn(v.x * 1099511627776.0f);
+    qy -= __float2ll_rn(v.y * 1099511627776.0f);
+    qz -= __float2ll_rn(v.z * 1099511627776.0f);
   }
-  fx = group_sum<T>(fx);
-  fy = group_sum<T>(fy);
-  fz = group_sum<T>(fz);
+  qx = group_sum<T>(qx);
+  qy = group_sum<T>(qy);
+  qz = group_sum<T>(qz);
+  double fx = (double)qx * (1.0 / 1099511627776.0), fy = (double)qy * (1.0 / 1099511627776.0),
+         fz = (double)qz * (1.0 / 1099511627776.0);
   if (t == 0) {
     double4 f = accumulate ? frc[a] : make_double4(0, 0, 0, 0);
+    int r0 = row_of1[a];
+    if (r0 >= 0) {
+      double4 s = fself[r0];
+      fx += s.x; fy += s.y; fz += s.z;
+    }
     f.x += fx; f.y += fy; f.z += fz;
     frc[a] = f;
   }
 }
 
-// slot of every converse entry's row atom in the neighbour's list-1 row
-__global__ void k_sur_rev(int nrows1, int stride1, const int* __restrict__ rows1, const int* __restrict__ nnum1,
-                          const int* __restrict__ nbr1, const int* __restrict__ row_of0, const int* __restrict__ nnum0,
-                          const int* __restrict__ nbr0, int stride0, int* __restrict__ rev) {
+// The converse list nl(2) as the transpose of nl(1) (the reference builds
+// its converse lists the same way, converce_neighbour_list,
+// md_neighbours.f90:128-160): every entry (i, p) -> a of nl(1) is appended to
+// a's converse row with its flat index row*stride + p (rev1) and i (nbr1).
+// The append order within a row varies from run to run; the gather sums the
+// row in fixed point, where the order does not matter.  cnt (the row
+// lengths) must start at zero.
+__global__ void k_sur_transpose(int nrows0, int stride0, const int* __restrict__ rows0,
+                                const int* __restrict__ nnum0, const int* __restrict__ nbr0,
+                                const int* __restrict__ row_of1, int stride1, int nmax1, int* __restrict__ cnt,
+                                int* __restrict__ nbr1, int* __restrict__ rev1, DevErr* err,
+                                const int* __restrict__ orig, int step) {
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)nrows1 * stride1) return;
-  int row = (int)(e / stride1), p = (int)(e % stride1);
-  if (p >= nnum1[row]) return;
-  int a = rows1[row];
-  int i = nbr1[e];
-  int r0 = row_of0[i];
-  int out = -1;
-  if (r0 >= 0) {
-    const int* nb = nbr0 + (size_t)r0 * stride0;
-    int n0 = nnum0[r0];
-    for (int s = 0; s < n0; ++s)
-      if (nb[s] == a) { out = r0 * stride0 + s; break; }
+  if (e >= (long)nrows0 * stride0) return;
+  int r0 = (int)(e / stride0), p = (int)(e % stride0);
+  if (p >= nnum0[r0]) return;
+  int a = nbr0[e];
+  int r1 = row_of1[a];
+  if (r1 < 0) return;
+  int s = atomicAdd(&cnt[r1], 1);
+  if (s < nmax1) {
+    rev1[(size_t)r1 * stride1 + s] = (int)e;
+    nbr1[(size_t)r1 * stride1 + s] = rows0[r0];
+  } else {
+    raise_err(err, PFM_E_OVERFLOW, orig[a] + 1, s + 1, step);
   }
-  rev[e] = out;
+}
+
+__global__ void k_sur_clip(int n, int* __restrict__ x, int hi) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[k] = min(x[k], hi);
+}
+
+__global__ void k_sur_fill(long n, int* __restrict__ x, int v) {
+  long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[k] = v;
 }
 
 // ------------------------------------------------------------------ training
@@ -512,18 +561,17 @@ template <class R>
 __global__ void __launch_bounds__(256) k_sur_tangent(RowArgsS a, SurK K, const uint32_t* __restrict__ mask,
                                                      int nb_real) {
   SUR_ROW_PROLOGUE
-  float dacc[NMON + 1];
+  float dmc[NMON], dmx[NMON], ddd[1] = {0.0f}, dgc[6], dgx[6];
 #pragma unroll
-  for (int c = 0; c <= NMON; ++c) dacc[c] = 0.0f;
-  const float eta = ang ? K.etaa[ke] : 0.0f;
-  const float e0 = K.eta2[k0], e1 = K.eta2[k0 + 1], e2 = K.eta2[k0 + 2];
+  for (int t = 0; t < NMON; ++t) { dmc[t] = 0.0f; dmx[t] = 0.0f; }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) { dgc[k] = 0.0f; dgx[k] = 0.0f; }
   const float4 ri = K.res[i];
-  for (int p = 0; p < n; ++p) {
-    int j = nbrow[p];
+  SUR_WALK_BEGIN
     float dx, dy, dz;
-    float r2 = sur_dr<R>(Gm, pi, j, dx, dy, dz);
+    float r2 = sur_dr_rec<R>(Gm, pi, pj, dx, dy, dz);
     if (!(r2 < K.rc2)) continue;
-    int sj = (mask[j] & K.bit1) ? 0 : 1;
+    bool cj = (mj & K.bit1) != 0;
     float r = __builtin_amdgcn_sqrtf(r2), ir = 1.0f / r;
     float fc, fcp;
     sur_cut(K, r, fc, fcp);
@@ -531,60 +579,59 @@ __global__ void __launch_bounds__(256) k_sur_tangent(RowArgsS a, SurK K, const u
     float4 rj = K.res[j];
     float de[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
     float ud = u[0] * de[0] + u[1] * de[1] + u[2] * de[2];
-    if (ang) {
-      if (sj == sp) {
-        float ex = __expf(-eta * r2);
-        float w = ex * fc, wp = ex * (fcp - 2.0f * eta * r * fc);
-        float dw = wp * ud;
-        float du[3] = {(de[0] - u[0] * ud) * ir, (de[1] - u[1] * ud) * ir, (de[2] - u[2] * ud) * ir};
-        float mono[NMON], dmono[NMON];
-        mono[0] = 1.0f;
-        dmono[0] = 0.0f;
+    float ex = __expf(-eta * r2);
+    float w = ex * fc, wp = ex * (fcp - 2.0f * eta * r * fc);
+    float dw = wp * ud;
+    float du[3] = {(de[0] - u[0] * ud) * ir, (de[1] - u[1] * ud) * ir, (de[2] - u[2] * ud) * ir};
+    float mono[NMON], dmono[NMON];
+    mono[0] = 1.0f;
+    dmono[0] = 0.0f;
 #pragma unroll
-        for (int t = 1; t < NMON; ++t) {
-          mono[t] = mono[kPar[t]] * u[kCrd[t]];
-          dmono[t] = dmono[kPar[t]] * u[kCrd[t]] + mono[kPar[t]] * du[kCrd[t]];
-        }
-#pragma unroll
-        for (int t = 0; t < NMON; ++t) dacc[t] += dw * mono[t] + w * dmono[t];
-        dacc[NMON] += 2.0f * w * dw;
-      }
-    } else {
-      float x0 = __expf(-e0 * r2), x1 = __expf(-e1 * r2), x2 = __expf(-e2 * r2);
-      float p0 = x0 * (fcp - 2.0f * e0 * r * fc) * ud, p1 = x1 * (fcp - 2.0f * e1 * r * fc) * ud,
-            p2 = x2 * (fcp - 2.0f * e2 * r * fc) * ud;
-      if (sj == 0) { dacc[0] += p0; dacc[1] += p1; dacc[2] += p2; }
-      else { dacc[3] += p0; dacc[4] += p1; dacc[5] += p2; }
+    for (int t = 1; t < NMON; ++t) {
+      mono[t] = mono[kPar[t]] * u[kCrd[t]];
+      dmono[t] = dmono[kPar[t]] * u[kCrd[t]] + mono[kPar[t]] * du[kCrd[t]];
     }
-  }
-  float* dGr = K.dG + (size_t)row * NF;
-  if (!ang) {
-    dGr[k0] = dacc[0]; dGr[k0 + 1] = dacc[1]; dGr[k0 + 2] = dacc[2];
-    dGr[12 + k0] = dacc[3]; dGr[12 + k0 + 1] = dacc[4]; dGr[12 + k0 + 2] = dacc[5];
-  }
-  float pv[NMON];
-#pragma unroll
-  for (int c = 0; c < NMON; ++c) pv[c] = __shfl(dacc[c], (lane + 2) & 63);
-  if (q < 2) {
-    const float* Mc = K.M + ((size_t)row * 4 + ke) * 36;
-    const float* Mx = K.M + ((size_t)row * 4 + 2 + ke) * 36;
-    float Scc[5] = {0, 0, 0, 0, 0}, Scx[5] = {0, 0, 0, 0, 0};
+    float wc = cj ? w : 0.0f, wx = w - wc, dwc = cj ? dw : 0.0f, dwx = dw - dwc;
 #pragma unroll
     for (int t = 0; t < NMON; ++t) {
-      float mc = Mc[t], mx = Mx[t];
-      Scc[kDeg[t]] += kMul[t] * 2.0f * mc * dacc[t];
-      Scx[kDeg[t]] += kMul[t] * (dacc[t] * mx + mc * pv[t]);
+      dmc[t] += dwc * mono[t] + wc * dmono[t];
+      dmx[t] += dwx * mono[t] + wx * dmono[t];
     }
+    ddd[0] += 2.0f * wc * dw;
 #pragma unroll
-    for (int m = 0; m < 5; ++m) Scc[m] -= dacc[NMON];
-    float o[4];
-    g5_of(Scc, o);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) dGr[24 + ke * 4 + m] = o[m];
-    g5_of(Scx, o);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) dGr[32 + ke * 4 + m] = o[m];
+    for (int k = 0; k < 6; ++k) {
+      float pk = __expf(-e2k[k] * r2) * (fcp - 2.0f * e2k[k] * r * fc) * ud;
+      dgc[k] += cj ? pk : 0.0f;
+      dgx[k] += cj ? 0.0f : pk;
+    }
   }
+  sum_groups<NMON>(dmc);
+  sum_groups<NMON>(dmx);
+  sum_groups<1>(ddd);
+  sum_groups<6>(dgc);
+  sum_groups<6>(dgx);
+  if (eg != 0) return;
+  float* dGr = K.dG + (size_t)row * NF;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) { dGr[6 * ke + k] = dgc[k]; dGr[12 + 6 * ke + k] = dgx[k]; }
+  const float* Mc = K.M + ((size_t)row * 4 + ke) * 36;
+  const float* Mx = K.M + ((size_t)row * 4 + 2 + ke) * 36;
+  float Scc[5] = {0, 0, 0, 0, 0}, Scx[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int t = 0; t < NMON; ++t) {
+    float mc = Mc[t], mx = Mx[t];
+    Scc[kDeg[t]] += kMul[t] * 2.0f * mc * dmc[t];
+    Scx[kDeg[t]] += kMul[t] * (dmc[t] * mx + mc * dmx[t]);
+  }
+#pragma unroll
+  for (int m = 0; m < 5; ++m) Scc[m] -= ddd[0];
+  float o[4];
+  g5_of(Scc, o);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) dGr[24 + ke * 4 + m] = o[m];
+  g5_of(Scx, o);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) dGr[32 + ke * 4 + m] = o[m];
 }
 
 // E = sum_i E_i, the loss terms and the seeds of the weight gradient:
@@ -855,7 +902,7 @@ static SurK sur_k(pfm_ctx* h, Inter& it) {
   K.etaa[1] = 0.5f;
   K.bit1 = 1u << (it.nl[0].g1 - 1);
   K.stride0 = it.nl[0].stride;
-  K.G = B.G; K.M = B.M; K.gG = B.gG; K.d = B.d; K.dG = B.dG; K.res = B.res;
+  K.G = B.G; K.M = B.M; K.gG = B.gG; K.d = B.d; K.fself = B.fself; K.dG = B.dG; K.res = B.res;
   K.fp = B.fp; K.W = B.W; K.e_row = it.e_row;
   return K;
 }
@@ -874,6 +921,7 @@ int sur_alloc(pfm_ctx* h, Inter& it) {
   if ((rc = al((void**)&B.M, nr * 4 * 36 * 4))) return rc;
   if ((rc = al((void**)&B.gG, nr * NF * 4))) return rc;
   if ((rc = al((void**)&B.d, (nr * it.nl[0].stride + 1) * 16))) return rc;
+  if ((rc = al((void**)&B.fself, nr * sizeof(double4)))) return rc;
   if ((rc = al((void**)&B.fp, SUR_NFP_ * 4))) return rc;
   if ((rc = al((void**)&B.W, SUR_NW * 2))) return rc;
   return PFM_OK;
@@ -906,21 +954,27 @@ static int sur_alloc_train(pfm_ctx* h, Inter& it) {
 void sur_free(Inter& it) {
   if (!it.sur) return;
   SurBuf& B = *it.sur;
-  void* ps[] = {B.G, B.M, B.gG, B.d, B.fp, B.W, B.dG, B.res, B.fref, B.arr, B.bpart, B.wpart, B.rpart, B.scal, B.grad};
+  void* ps[] = {B.G, B.M, B.gG, B.d, B.fself, B.fp, B.W, B.dG, B.res, B.fref, B.arr, B.bpart, B.wpart, B.rpart, B.scal, B.grad};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete it.sur;
   it.sur = nullptr;
 }
 
-// slots of the converse list after both lists were rebuilt
-void launch_sur_rev(pfm_ctx* h, Inter& it) {
+// converse list nl(2) of a surrogate from its freshly rebuilt nl(1)
+void launch_sur_converse(pfm_ctx* h, Inter& it, int step) {
   ListDev& L0 = it.nl[0];
   ListDev& L1 = it.nl[1];
+  L1.built = true;
   if (L1.nrows == 0) return;
-  long ne = (long)L1.nrows * L1.stride;
-  PFM_LAUNCH(h, "list_rev", k_sur_rev, (int)((ne + 255) / 256), 256, L1.nrows, L1.stride, h->grp.rows[L1.g1],
-             L1.nnum, L1.nbr, h->grp.row_of[L0.g1], L0.nnum, L0.nbr, L0.stride, L1.rev);
+  PFM_LAUNCH(h, "list_conv", k_sur_fill, (L1.nrows + 255) / 256, 256, (long)L1.nrows, L1.nnum, 0);
+  if (L0.nrows > 0) {
+    long n0 = (long)L0.nrows * L0.stride;
+    PFM_LAUNCH(h, "list_conv", k_sur_transpose, (int)((n0 + 255) / 256), 256, L0.nrows, L0.stride, h->grp.rows[L0.g1],
+               L0.nnum, L0.nbr, h->grp.row_of[L1.g1], L1.stride, L1.nmax, L1.nnum, L1.nbr, L1.rev, h->derr, h->orig,
+               step);
+  }
+  PFM_LAUNCH(h, "list_conv", k_sur_clip, (L1.nrows + 255) / 256, 256, L1.nrows, L1.nnum, L1.nmax);
 }
 
 template <class R>
@@ -938,8 +992,8 @@ static void sur_forces_T(pfm_ctx* h, Inter& it, int accumulate, double4* frc) {
   if (c.nrows > 0) {
     constexpr int T = 8;
     int nb = (int)(((long)c.nrows * T + 255) / 256);
-    PFM_LAUNCH(h, "sur_gather", k_sur_gather<T>, pad8(nb), 256, c, h->grp.row_of[it.nl[0].g1], it.nl[0].nnum,
-               it.nl[0].stride, it.nl[1].rev, it.sur->d, frc, accumulate, nb);
+    PFM_LAUNCH(h, "sur_gather", k_sur_gather<T>, pad8(nb), 256, c, h->grp.row_of[it.nl[0].g1], it.sur->fself,
+               it.nl[1].rev, it.sur->d, frc, accumulate, nb);
   }
 }
 

@@ -1,7 +1,11 @@
 """The learned surrogate on the GPU against its CPU definition
 (tests/sur_ref.py) on a small graphene/Cu slab.
 
-Bars: descriptors within 2e-5 of the float64 textbook sums (FP32 arithmetic);
+Bars: both lists equal brute-force neighbour sets (the converse list is the
+transpose of the first); descriptors within 1e-4 of each feature's largest
+value over the slab (FP32 arithmetic; the (1 - cos)^4 angular functions
+combine degree sums that cancel to a few parts in 1e4 on the small metal
+features, far below the BF16 rounding of the network's scaled inputs);
 energy within 1e-5 and forces within 1e-3 (relative RMSE) of a CPU replay
 that rounds to BF16 where the kernels do (network inputs, hidden activations
 and the backward signals), forces within 3e-2 of the float64 network (the
@@ -86,8 +90,23 @@ def test_descriptors_match_textbook_sums(precision):
     print(f"[sur desc] n={s.n} rows={len(g1)} max|G|={np.abs(G).max():.3g}", flush=True)
     scale = np.abs(G).max(0)
     err = np.abs(Gg - G) / np.maximum(scale, 1e-12)
-    print(f"  max relative error {err.max():.2e}", flush=True)
-    assert err.max() < 2e-5
+    print(f"  max relative error {err.max():.2e} (feature {int(np.argmax(err.max(0)))})", flush=True)
+    assert err.max() < 1e-4
+
+
+def test_lists_are_brute_force_sets_and_converse():
+    from tests.test_oracle_pins import brute_sets
+    from tests.harness import list_sets
+    s = slab(seed=56)
+    G, _ = ref_descriptors(s, brute=False)
+    ctx, tr = sur_ctx(s, theta_for(G))
+    (g1n, g2n, _, rc, _), _ = tr.interactions[0].lists
+    g1, g2 = s.group_indices(g1n), s.group_indices(g2n)
+    l1, _ = list_sets(ctx, 1, 1)
+    l2, _ = list_sets(ctx, 1, 2)
+    assert l1 == brute_sets(s.pos, s.box, g1 - 1, g2 - 1, rc)
+    assert l2 == brute_sets(s.pos, s.box, g2 - 1, g1 - 1, rc)
+    print(f"[sur lists] rows {len(l1)} / {len(l2)}, entries {sum(map(len, l1))} = {sum(map(len, l2))}", flush=True)
 
 
 def test_energy_and_forces_against_cpu():
@@ -181,3 +200,20 @@ def test_judge_switches_to_plain_potential():
         assert [j[0] for j in jr.judged] == ([10] if thr < 1 else [10, 20, 30])
         assert (end is pc) == (thr < 1)
     assert runs[0].switch_step == 10 and runs[1].switch_step is None
+
+
+def test_surrogate_runs_are_bit_identical():
+    # the converse rows are filled in a varying order; the gather's fixed-point
+    # sums make the forces independent of it
+    s = slab(seed=57)
+    G, _ = ref_descriptors(s, brute=False)
+    _, sur_s, _ = surrogate.split_system(s)
+    theta = theta_for(G, seed=9, scale=0.3)
+    out = []
+    for _ in range(2):
+        c = surrogate.surrogate_context(sur_s, theta)
+        c.run("nvt", 0.5, 0, 30, True)
+        out.append(c.state())
+        c.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
