@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the learned surrogate")
     ap.add_argument("--c4-only", action="store_true", help="run only the learned surrogate")
     ap.add_argument("--c4-iters", type=int, default=3000, help="surrogate training iterations")
+    ap.add_argument("--c4-small", action="store_true",
+                    help="surrogate with 10 + 40 frames and a 2,000-step run (profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step path has no multi-GPU decomposition "
@@ -291,8 +293,9 @@ def main():
         return
     if args.c4_only:
         c2 = graphene_cu(200, 20)
+        kw = dict(frames_full=10, frames_sub=40, steps=2000) if args.c4_small else {}
         print(json.dumps({"surrogate": surrogate_c4(args.c4_iters, mixed_ms=c2["mixed_ms_per_step"],
-                                                    fp64_ms=c2["fp64_ms_per_step"]), "graphene_cu_nvt": c2}))
+                                                    fp64_ms=c2["fp64_ms_per_step"], **kw), "graphene_cu_nvt": c2}))
         return
     s = configs.fcc_cu_rgl(nx=args.cells, steps=args.steps)
     res = {}
