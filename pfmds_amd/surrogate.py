@@ -177,11 +177,12 @@ class Adam:
 
 
 def train(ctx, inter, frames, theta, iters, lr=2e-3, lr_end=2e-4, w_e=1.0, w_f=1.0, seed=0, log=None,
-          log_every=100):
-    """Adam over randomly drawn frames, one frame per iteration; the loss and
-    gradient come from pfm_sur_loss_grad.  `ctx`/`frames` may be lists (one
-    handle per system size); frames are drawn over all sets in proportion to
-    their counts.  Returns the final parameters and the per-iteration losses."""
+          log_every=100, batch=1):
+    """Adam over randomly drawn frames, `batch` frames (gradients averaged)
+    per iteration; the loss and gradient of each frame come from
+    pfm_sur_loss_grad.  `ctx`/`frames` may be lists (one handle per system
+    size); frames are drawn over all sets in proportion to their counts.
+    Returns the final parameters and the per-iteration losses."""
     ctxs = ctx if isinstance(ctx, (list, tuple)) else [ctx]
     sets = frames if isinstance(frames, (list, tuple)) else [frames]
     sizes = np.array([len(f) for f in sets], dtype=float)
@@ -192,13 +193,17 @@ def train(ctx, inter, frames, theta, iters, lr=2e-3, lr_end=2e-4, w_e=1.0, w_f=1
     for c in ctxs:
         c.sur_set_params(inter, theta)
     for it in range(iters):
-        w = int(rng.choice(len(sets), p=sizes / sizes.sum()))
-        c, fr = ctxs[w], sets[w]
-        k = int(rng.integers(len(fr)))
-        pos = fr.positions(k)
-        c.set_state(pos, np.zeros_like(pos))
-        c.run("nve", 0.5, 0, 0, False)
-        loss, g = c.sur_loss_grad(inter, fr.force[k].astype(np.float64), fr.energy[k], w_e, w_f)
+        loss, g = 0.0, 0.0
+        for _ in range(batch):
+            w = int(rng.choice(len(sets), p=sizes / sizes.sum()))
+            c, fr = ctxs[w], sets[w]
+            k = int(rng.integers(len(fr)))
+            pos = fr.positions(k)
+            c.set_state(pos, np.zeros_like(pos))
+            c.run("nve", 0.5, 0, 0, False)
+            lk, gk = c.sur_loss_grad(inter, fr.force[k].astype(np.float64), fr.energy[k], w_e, w_f)
+            loss = loss + lk / batch
+            g = g + gk / batch
         hist.append(loss)
         a = lr * (lr_end / lr) ** (it / max(iters - 1, 1))
         theta = opt.step(theta, g, a)
