@@ -28,7 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from pfmds_amd import capi, configs, system  # noqa: E402
+from pfmds_amd import autotune, capi, configs, system  # noqa: E402
 
 METRIC = "atom-steps/s on 1 MI355X: mixed/batched/surrogate vs plain FP64 + force RMSE"
 HBM_PEAK_GBS = 8000.0
@@ -40,14 +40,22 @@ def gpu_ctx(s, precision):
     return system.build_context(s, lambda b, p, v, m: capi.Context.gpu(b, p, v, m, precision))
 
 
-def timed_run(ctx, s, steps, warmup):
+def tuned(ctx, integrator, dt, precision, first):
+    """Launch shape from the autotuner, tried on the live handle from step
+    `first` on; returns ("LxB", next step)."""
+    lanes, block, _, nxt = autotune.tune(ctx, integrator, dt, precision, first, steps=100, repeats=3)
+    return f"{lanes}x{block}", nxt
+
+
+def timed_run(ctx, s, steps, warmup, precision):
     dt = s.integrators[0].dt
     ctx.run("nve", dt, 0, warmup, True)
+    shape, first = tuned(ctx, "nve", dt, precision, warmup + 1)
     e0 = ctx.energies()
     t0 = time.perf_counter()
-    ctx.run("nve", dt, warmup + 1, warmup + steps, True)
+    ctx.run("nve", dt, first, first + steps - 1, True)
     t1 = time.perf_counter()
-    return t1 - t0, e0, ctx.energies()
+    return t1 - t0, e0, ctx.energies(), shape
 
 
 def mean_list_len(ctx):
@@ -96,9 +104,11 @@ def graphene_cu(steps, warmup):
     for name, prec in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
         ctx = gpu_ctx(s, prec)
         ctx.run("nvt", dt, 0, warmup, True)
+        shape, first = tuned(ctx, "nvt", dt, prec, warmup + 1)
         t0 = time.perf_counter()
-        ctx.run("nvt", dt, warmup + 1, warmup + steps, True)
+        ctx.run("nvt", dt, first, first + steps - 1, True)
         t = time.perf_counter() - t0
+        out[name + "_launch_shape"] = shape
         out[name + "_ms_per_step"] = t * 1e3 / steps
         out[name + "_atom_steps_per_s"] = s.n * steps / t
         ctxs[name] = ctx
@@ -217,9 +227,11 @@ def replica_batch(steps, fp64_steps, check_steps=100, n_check=8):
     for name, prec, n in (("mixed", capi.MIXED, steps), ("fp64", capi.FP64, fp64_steps)):
         g = system.build_batch_context(b, prec)
         g.run("nve", 0.5, 0, 20, True)
+        shape, first = tuned(g, "nve", 0.5, prec, 21)
         t0 = time.perf_counter()
-        g.run("nve", 0.5, 21, 20 + n, True)
+        g.run("nve", 0.5, first, first + n - 1, True)
         t = time.perf_counter() - t0
+        out[name + "_launch_shape"] = shape
         out[name + "_ms_per_step"] = t * 1e3 / n
         out[name + "_atom_steps_per_s"] = b.system.n * n / t
         out[name + "_steps"] = n
@@ -304,8 +316,8 @@ def main():
     ctxs = {}
     for name, prec in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
         ctx = gpu_ctx(s, prec)
-        t, e0, e1 = timed_run(ctx, s, args.steps, args.warmup)
-        res[name] = dict(seconds=t, value=s.n * args.steps / t, e0=e0, e1=e1)
+        t, e0, e1, shape = timed_run(ctx, s, args.steps, args.warmup, prec)
+        res[name] = dict(seconds=t, value=s.n * args.steps / t, e0=e0, e1=e1, shape=shape)
         ctxs[name] = ctx
         print(f"# {name}: {t * 1e3 / args.steps:.4f} ms/step, {s.n * args.steps / t:.4e} atom-steps/s", file=sys.stderr,
               flush=True)
@@ -397,6 +409,9 @@ def main():
         "fp64_value": fp64["value"],
         "fp64_ms_per_step": fp64["seconds"] * 1e3 / args.steps,
         "speedup_vs_fp64": mixed["value"] / fp64["value"],
+        "launch_shape": {"mixed": mixed["shape"], "fp64": fp64["shape"],
+                         "note": "lanes per row x workgroup size of the pair passes, picked by pfmds_amd.autotune "
+                                 "on the live handle before the timed steps"},
         "force_rmse_eV_per_A": rmse,
         "pe_err_eV_per_atom": abs(pe32 - pe64) / s.n,
         "nve_drift_eV_per_atom_per_ns": drift,

@@ -1,0 +1,102 @@
+"""Launch-shape autotuner for the pair passes of libpfmds.
+
+The pair passes (rjl density and force, lj, lj1g, ljc/morsec graphene and
+metal rows) walk each atom row with T lanes inside workgroups of B threads
+(`pfm_set_tuning`).  The best shape depends on the row length, the precision
+and the system size.  `tune` times a short run of the live handle for every
+candidate shape, keeps the fastest and leaves it set; results are cached in a
+JSON file keyed by a signature of the workload.  No counterpart in the
+reference, which tunes only its OpenMP thread count
+(perfomance_settings.f90:13-25).
+"""
+import json
+import os
+import time
+
+# (lanes per row, workgroup size); FP64 passes are limited to 256 threads
+CANDIDATES = [(4, 256), (8, 128), (8, 256), (8, 512), (16, 256), (16, 512), (32, 256)]
+CACHE = os.environ.get("PFMDS_TUNING_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "pfmds_amd", "tuning.json"))
+
+
+def signature(s, precision, n_rep=1):
+    """Key of a workload: atom count, precision, and per interaction its kind
+    and list shapes (group sizes, nmax, r_cut)."""
+    parts = [f"n{s.n}", f"p{int(precision)}", f"r{n_rep}"]
+    for it in s.interactions:
+        for (g1, g2, nmax, rc, per) in it.lists:
+            parts.append(f"{it.name}:{len(s.group_indices(g1))}x{len(s.group_indices(g2))}:{nmax}:{rc:.3f}:{per}")
+    return "|".join(parts)
+
+
+def valid(lanes, block, precision):
+    return block <= 256 or int(precision) != 0
+
+
+def time_shape(ctx, integrator, dt, first, steps):
+    """Wall time of steps first..first+steps-1 (energy at the end forces the
+    device to finish before the clock stops)."""
+    t0 = time.perf_counter()
+    ctx.run(integrator, dt, first, first + steps - 1, True)
+    return time.perf_counter() - t0
+
+
+def tune(ctx, integrator, dt, precision, first_step=1, steps=40, repeats=2, candidates=CANDIDATES):
+    """Try every candidate shape on the handle, from step `first_step` on
+    (the handle must have run step first_step - 1), set the fastest and return
+    (lanes, block, {shape: seconds}, next step)."""
+    table = {}
+    step = first_step
+    for lanes, block in candidates:
+        if not valid(lanes, block, precision):
+            continue
+        ctx.set_tuning(lanes, block)
+        best = None
+        for _ in range(repeats):
+            t = time_shape(ctx, integrator, dt, step, steps)
+            step += steps
+            best = t if best is None else min(best, t)
+        table[f"{lanes}x{block}"] = best
+    pick = min(table, key=table.get)
+    lanes, block = (int(x) for x in pick.split("x"))
+    ctx.set_tuning(lanes, block)
+    return lanes, block, table, step
+
+
+def load(path=CACHE):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save(key, lanes, block, table, path=CACHE):
+    data = load(path)
+    data[key] = {"lanes": lanes, "block": block, "seconds": table}
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def apply_cached(ctx, key, path=CACHE):
+    """Set the cached shape of `key` on the handle; False when none is cached."""
+    e = load(path).get(key)
+    if not e:
+        return False
+    ctx.set_tuning(int(e["lanes"]), int(e["block"]))
+    return True
+
+
+def autotune(ctx, s, integrator, dt, precision, first_step=1, n_rep=1, path=CACHE, steps=40, retune=False):
+    """Cached shape for this workload if there is one, else tune and cache.
+    Returns (lanes, block, source, next step) with source 'cache' or 'tuned'."""
+    key = signature(s, precision, n_rep)
+    if not retune and apply_cached(ctx, key, path):
+        e = load(path)[key]
+        return int(e["lanes"]), int(e["block"]), "cache", first_step
+    lanes, block, table, step = tune(ctx, integrator, dt, precision, first_step, steps)
+    save(key, lanes, block, table, path)
+    return lanes, block, "tuned", step
