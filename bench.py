@@ -41,10 +41,10 @@ def gpu_ctx(s, precision):
 
 
 def tuned(ctx, integrator, dt, precision, first):
-    """Launch shape from the autotuner, tried on the live handle from step
-    `first` on; returns ("LxB", next step)."""
-    lanes, block, _, nxt = autotune.tune(ctx, integrator, dt, precision, first, steps=100, repeats=3)
-    return f"{lanes}x{block}", nxt
+    """Launch shape and launch mode from the autotuner, tried on the live
+    handle from step `first` on; returns ("LxB[+graphs]", next step)."""
+    lanes, block, graphs, _, nxt = autotune.tune(ctx, integrator, dt, precision, first, steps=100, repeats=3)
+    return f"{lanes}x{block}" + ("+graphs" if graphs else ""), nxt
 
 
 def timed_run(ctx, s, steps, warmup, precision):
@@ -410,7 +410,8 @@ def main():
         "fp64_ms_per_step": fp64["seconds"] * 1e3 / args.steps,
         "speedup_vs_fp64": mixed["value"] / fp64["value"],
         "launch_shape": {"mixed": mixed["shape"], "fp64": fp64["shape"],
-                         "note": "lanes per row x workgroup size of the pair passes, picked by pfmds_amd.autotune "
+                         "note": "lanes per row x workgroup size of the pair passes, and +graphs when steps "
+                                 "launched from captured HIP graphs were faster, picked by pfmds_amd.autotune "
                                  "on the live handle before the timed steps"},
         "force_rmse_eV_per_A": rmse,
         "pe_err_eV_per_atom": abs(pe32 - pe64) / s.n,

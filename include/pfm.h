@@ -159,6 +159,16 @@ int pfm_kernel_time(pfm_ctx* h, const char* name, int* count, double* mean_ms);
  * the reference (launch-shape control for tuning and benchmarks). */
 int pfm_set_tuning(pfm_ctx* h, int lanes_per_row, int block);
 
+/* Captured steps (default on).  pfm_run records a block of steps whose list
+ * and zero-momentum schedule repeats (the least common multiple of the list
+ * periods, up to 64 steps) and that holds no energy step as one HIP graph,
+ * and replays it for every later such block; results are identical to
+ * launching the steps one by one.  pfm_set_graphs(h, 0) launches every step
+ * directly.  No counterpart in the reference (launch mechanics). */
+int pfm_set_graphs(pfm_ctx* h, int on);
+/* Blocks captured and blocks launched from graphs since the handle was made. */
+int pfm_graph_count(pfm_ctx* h, long* captured, long* launched);
+
 /* Learned surrogate (interaction kind PFM_SUR).  None of these has a
  * counterpart in the reference; they serve the training and inference of a
  * per-atom network that stands in for TB_forces + MorseC_forces

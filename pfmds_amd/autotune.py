@@ -4,8 +4,10 @@ The pair passes (rjl density and force, lj, lj1g, ljc/morsec graphene and
 metal rows) walk each atom row with T lanes inside workgroups of B threads
 (`pfm_set_tuning`).  The best shape depends on the row length, the precision
 and the system size.  `tune` times a short run of the live handle for every
-candidate shape, keeps the fastest and leaves it set; results are cached in a
-JSON file keyed by a signature of the workload.  No counterpart in the
+candidate shape, keeps the fastest, then times the steps launched from
+captured graphs against launched one by one (`pfm_set_graphs`: graphs pay
+off where launches dominate, small systems) and leaves the faster set;
+results are cached in a JSON file keyed by a signature of the workload.  No counterpart in the
 reference, which tunes only its OpenMP thread count
 (perfomance_settings.f90:13-25).
 """
@@ -43,24 +45,35 @@ def time_shape(ctx, integrator, dt, first, steps):
 
 def tune(ctx, integrator, dt, precision, first_step=1, steps=40, repeats=2, candidates=CANDIDATES):
     """Try every candidate shape on the handle, from step `first_step` on
-    (the handle must have run step first_step - 1), set the fastest and return
-    (lanes, block, {shape: seconds}, next step)."""
+    (the handle must have run step first_step - 1), set the fastest, then the
+    faster of graph and direct launches, and return (lanes, block, graphs,
+    {setting: seconds}, next step)."""
     table = {}
     step = first_step
-    for lanes, block in candidates:
-        if not valid(lanes, block, precision):
-            continue
-        ctx.set_tuning(lanes, block)
+
+    def best_of():
+        nonlocal step
         best = None
         for _ in range(repeats):
             t = time_shape(ctx, integrator, dt, step, steps)
             step += steps
             best = t if best is None else min(best, t)
-        table[f"{lanes}x{block}"] = best
+        return best
+
+    ctx.set_graphs(False)
+    for lanes, block in candidates:
+        if not valid(lanes, block, precision):
+            continue
+        ctx.set_tuning(lanes, block)
+        table[f"{lanes}x{block}"] = best_of()
     pick = min(table, key=table.get)
     lanes, block = (int(x) for x in pick.split("x"))
     ctx.set_tuning(lanes, block)
-    return lanes, block, table, step
+    ctx.set_graphs(True)
+    table[f"{pick}+graphs"] = best_of()
+    graphs = table[f"{pick}+graphs"] < table[pick]
+    ctx.set_graphs(graphs)
+    return lanes, block, graphs, table, step
 
 
 def load(path=CACHE):
@@ -71,9 +84,9 @@ def load(path=CACHE):
         return {}
 
 
-def save(key, lanes, block, table, path=CACHE):
+def save(key, lanes, block, graphs, table, path=CACHE):
     data = load(path)
-    data[key] = {"lanes": lanes, "block": block, "seconds": table}
+    data[key] = {"lanes": lanes, "block": block, "graphs": bool(graphs), "seconds": table}
     os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
     tmp = path + ".tmp"
     with open(tmp, "w") as f:
@@ -87,16 +100,18 @@ def apply_cached(ctx, key, path=CACHE):
     if not e:
         return False
     ctx.set_tuning(int(e["lanes"]), int(e["block"]))
+    ctx.set_graphs(bool(e.get("graphs", False)))
     return True
 
 
 def autotune(ctx, s, integrator, dt, precision, first_step=1, n_rep=1, path=CACHE, steps=40, retune=False):
     """Cached shape for this workload if there is one, else tune and cache.
-    Returns (lanes, block, source, next step) with source 'cache' or 'tuned'."""
+    Returns (lanes, block, graphs, source, next step) with source 'cache' or
+    'tuned'."""
     key = signature(s, precision, n_rep)
     if not retune and apply_cached(ctx, key, path):
         e = load(path)[key]
-        return int(e["lanes"]), int(e["block"]), "cache", first_step
-    lanes, block, table, step = tune(ctx, integrator, dt, precision, first_step, steps)
-    save(key, lanes, block, table, path)
-    return lanes, block, "tuned", step
+        return int(e["lanes"]), int(e["block"]), bool(e.get("graphs", False)), "cache", first_step
+    lanes, block, graphs, table, step = tune(ctx, integrator, dt, precision, first_step, steps)
+    save(key, lanes, block, graphs, table, path)
+    return lanes, block, graphs, "tuned", step

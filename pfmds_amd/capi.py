@@ -75,6 +75,10 @@ def load_library(path, prefix):
         lib.pfm_kernel_time.restype = C.c_int
         lib.pfm_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.pfm_set_tuning.restype = C.c_int
+        lib.pfm_set_graphs.argtypes = [C.c_void_p, C.c_int]
+        lib.pfm_set_graphs.restype = C.c_int
+        lib.pfm_graph_count.argtypes = [C.c_void_p, _P(C.c_long), _P(C.c_long)]
+        lib.pfm_graph_count.restype = C.c_int
         lib.pfm_sur_set_params.argtypes = [C.c_void_p, C.c_int, _d, C.c_int]
         lib.pfm_sur_set_params.restype = C.c_int
         lib.pfm_sur_descriptors.argtypes = [C.c_void_p, C.c_int, _d]
@@ -246,6 +250,15 @@ class Context:
 
     def set_tuning(self, lanes_per_row=0, block=0):
         self._check(self.lib.pfm_set_tuning(self.h, int(lanes_per_row), int(block)))
+
+    def set_graphs(self, on):
+        self._check(self.lib.pfm_set_graphs(self.h, int(bool(on))))
+
+    def graph_count(self):
+        """(blocks captured, blocks launched from graphs)"""
+        a, b = C.c_long(0), C.c_long(0)
+        self._check(self.lib.pfm_graph_count(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     # -- learned surrogate (interaction kind 'sur') ---------------------------
     def sur_set_params(self, inter, theta):

@@ -790,6 +790,59 @@ static int zero_momentum(pfm_ctx* h) {
   return PFM_OK;
 }
 
+// ------------------------------------------------------- captured blocks
+
+__global__ void k_set_base(DevErr* e, int base) {
+  if (threadIdx.x == 0) e->base = base;
+}
+
+static long gcd_l(long a, long b) { return b ? gcd_l(b, a % b) : a; }
+
+// length of the repeating schedule of list rebuilds and zero_momentum, or 0
+// when it is longer than 64 steps
+static int graph_block(pfm_ctx* h) {
+  long p = 1;
+  for (auto& it : h->inter)
+    for (int k = 0; k < it.nl_n; ++k) {
+      long q = it.nl[k].period;
+      p = p / gcd_l(p, q) * q;
+      if (p > 64) return 0;
+    }
+  if (h->zm_period <= 64) p = p / gcd_l(p, h->zm_period) * h->zm_period;
+  return p > 64 ? 0 : (int)p;
+}
+
+static std::vector<uint64_t> graph_buffers(pfm_ctx* h) {
+  return {(uint64_t)h->pos4, (uint64_t)h->vel4, (uint64_t)h->frc4, (uint64_t)h->mask, (uint64_t)h->orig,
+          (uint64_t)h->rep, (uint64_t)h->pos4b, (uint64_t)h->vel4b, (uint64_t)h->frc4b, (uint64_t)h->maskb,
+          (uint64_t)h->origb, (uint64_t)h->repb, (uint64_t)h->bin};
+}
+
+static void set_graph_buffers(pfm_ctx* h, const std::vector<uint64_t>& b) {
+  h->pos4 = (double4*)b[0]; h->vel4 = (double4*)b[1]; h->frc4 = (double4*)b[2];
+  h->mask = (uint32_t*)b[3]; h->orig = (int*)b[4]; h->rep = (int*)b[5];
+  h->pos4b = (double4*)b[6]; h->vel4b = (double4*)b[7]; h->frc4b = (double4*)b[8];
+  h->maskb = (uint32_t*)b[9]; h->origb = (int*)b[10]; h->repb = (int*)b[11];
+  h->bin = (const int*)b[12];
+}
+
+// what the launches of a block starting at step s depend on
+static std::vector<uint64_t> graph_key(pfm_ctx* h, int integ, double dt, int blk, bool first_writes, int s) {
+  uint64_t dtb;
+  memcpy(&dtb, &dt, 8);
+  std::vector<uint64_t> k = graph_buffers(h);
+  bool zm = false;
+  for (int t = s; t < s + blk; ++t) zm = zm || t % h->zm_period == 0;
+  k.insert(k.end(), {(uint64_t)integ, dtb, (uint64_t)blk, (uint64_t)first_writes, (uint64_t)zm,
+                     (uint64_t)h->invert_z, (uint64_t)h->lanes, (uint64_t)h->block});
+  return k;
+}
+
+static void clear_graphs(pfm_ctx* h) {
+  for (auto& e : h->graphs) (void)hipGraphExecDestroy(e.exec);
+  h->graphs.clear();
+}
+
 // ---------------------------------------------------------------- C-ABI
 
 extern "C" {
@@ -1015,7 +1068,10 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
   const bool nvt = integ == PFM_NVT && !h->nhc.empty();
   bool nhc_pending = false;
   bool kick_pending = false;  // second half kick deferred into the next kick_drift
-  for (int s = s0; s <= s1; ++s) {
+  const int CONVERGED = -1;
+  // one md() step: the body of md_simulation.f90:137-231
+  auto body = [&](int s) -> int {
+    int rc;
     *last_step = s;
     bool energy = (s == s1 && energy_at_end) || integ == PFM_NVMS;
     if (s != 0) {
@@ -1025,10 +1081,7 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
         conv = h->rep_pe.size() == (size_t)h->nrep && h->rep_pe_prev.size() == (size_t)h->nrep;
         for (int r = 0; conv && r < h->nrep; ++r) conv = std::fabs(h->rep_pe[r] - h->rep_pe_prev[r]) <= ms_de;
       }
-      if (integ == PFM_NVMS && conv) {
-        *event = PFM_EV_CONVERGED;
-        return PFM_OK;
-      }
+      if (integ == PFM_NVMS && conv) return CONVERGED;
       IntP Q = P;
       Q.pre_kick = kick_pending ? 1 : 0;
       kick_pending = false;
@@ -1039,8 +1092,8 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
         Q.scale = 1;
         nhc_pending = false;
       }
-      PFM_LAUNCH(h, "kick_drift", k_kick_drift, nb, 256, N, h->pos4, h->vel4, h->frc4, h->mask, Q, h->derr, h->orig, s,
-                 h->qpos, h->qscale, h->rep, h->rdev);
+      PFM_LAUNCH(h, "kick_drift", k_kick_drift, nb, 256, N, h->pos4, h->vel4, h->frc4, h->mask, Q, h->derr, h->orig,
+                 s - h->step_base, h->qpos, h->qscale, h->rep, h->rdev);
     } else if (!first_writes) {
       PFM_LAUNCH(h, "zero_forces", k_zero_masked, nb, 256, N, h->frc4, h->mask, all_bit);
     }
@@ -1072,6 +1125,77 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
       if ((rc = energies(h, integ == PFM_NVT))) return rc;
       if ((rc = check_err(h, s))) return rc;
     }
+    return PFM_OK;
+  };
+  // Steady-state blocks of `blk` steps (the list and zero-momentum schedule
+  // repeats with that period) that start on a multiple of blk and hold no
+  // energy step are captured once as a HIP graph and replayed: the launches
+  // of a block depend only on the schedule, the run's parameters and which
+  // of the two re-sort buffers is current, so the cache is keyed by those.
+  const int blk = graph_block(h);
+  const bool graphs = h->use_graphs && !h->timing && integ != PFM_NVMS && blk > 0;
+  for (int s = s0; s <= s1;) {
+    const bool steady = nvt ? nhc_pending : kick_pending;
+    if (graphs && s > 0 && s % blk == 0 && s + blk - 1 < s1 && steady) {
+      std::vector<uint64_t> key = graph_key(h, integ, dt, blk, first_writes, s);
+      pfm_ctx::GraphEntry* g = nullptr;
+      for (auto& e : h->graphs)
+        if (e.key == key) g = &e;
+      if (!g) {
+        hipGraph_t graph;
+        h->step_base = s;
+        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+        int rc2 = PFM_OK;
+        for (int t = s; t < s + blk && rc2 == PFM_OK; ++t) rc2 = body(t);
+        hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+        h->step_base = 0;
+        if (rc2 != PFM_OK) {
+          if (ec == hipSuccess) (void)hipGraphDestroy(graph);
+          return rc2 == CONVERGED ? fail(h, PFM_E_INVALID, "error: convergence test inside a captured block") : rc2;
+        }
+        HIPCHK(h, ec);
+        if (nvt ? !nhc_pending : !kick_pending) {
+          (void)hipGraphDestroy(graph);
+          return fail(h, PFM_E_INVALID, "error: captured block does not end in its starting state");
+        }
+        pfm_ctx::GraphEntry e;
+        e.key = key;
+        e.after = graph_buffers(h);
+        HIPCHK(h, hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        if (h->graphs.size() >= 4) {
+          (void)hipGraphExecDestroy(h->graphs.front().exec);
+          h->graphs.erase(h->graphs.begin());
+        }
+        h->graphs.push_back(e);
+        ++h->graph_captures;
+        ++h->graph_launches;
+        // the capture ran the block's host side; its kernels run now
+        PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, s);
+        HIPCHK(h, hipGraphLaunch(h->graphs.back().exec, h->stream));
+        PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, 0);
+      } else {
+        PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, s);
+        HIPCHK(h, hipGraphLaunch(g->exec, h->stream));
+        PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, 0);
+        ++h->graph_launches;
+        // the host side of the block: buffers, time, energies carried over
+        set_graph_buffers(h, g->after);
+        *last_step = s + blk - 1;
+        h->sim_time = h->sim_time + blk * dt;
+        h->pe_prev = h->pe;
+        h->rep_pe_prev = h->rep_pe;
+      }
+      s += blk;
+      continue;
+    }
+    int rc2 = body(s);
+    if (rc2 == CONVERGED) {
+      *event = PFM_EV_CONVERGED;
+      return PFM_OK;
+    }
+    if (rc2) return rc2;
+    ++s;
   }
   return check_err(h, s1);
 }
@@ -1265,9 +1389,25 @@ int pfm_last_error(pfm_ctx* h, char* buf, int len) {
   return PFM_OK;
 }
 
+int pfm_set_graphs(pfm_ctx* h, int on) {
+  h->use_graphs = on != 0;
+  if (!h->use_graphs) {
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    clear_graphs(h);
+  }
+  return PFM_OK;
+}
+
+int pfm_graph_count(pfm_ctx* h, long* captured, long* launched) {
+  if (captured) *captured = h->graph_captures;
+  if (launched) *launched = h->graph_launches;
+  return PFM_OK;
+}
+
 int pfm_destroy(pfm_ctx* h) {
   if (!h) return PFM_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  clear_graphs(h);
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(h->pos4); F(h->vel4); F(h->frc4); F(h->pos4b); F(h->vel4b); F(h->frc4b); F(h->mask); F(h->maskb);
   F(h->orig); F(h->origb); F(h->perm); F(h->atom_rank); F(h->scan_tmp); F(h->eb); F(h->qpos); F(h->rdev); F(h->rep); F(h->repb); F(h->cell_rep); F(h->d_rep_row0); F(h->d_rep_out);

@@ -178,13 +178,16 @@ __device__ __forceinline__ int xcd_block(int nblocks_real) {
   return lb < nblocks_real ? lb : -1;
 }
 
-// device error record: code, 1-based atom, count, step
+// device error record: code, 1-based atom, count, step.  Kernels replayed
+// from a captured graph pass their step relative to the block they were
+// captured in; `base` holds the first step of the block being replayed (0
+// outside graphs).
 struct DevErr {
-  int code, atom, count, step;
+  int code, atom, count, step, base;
 };
 __device__ __forceinline__ void raise_err(DevErr* e, int code, int atom, int count, int step) {
   if (atomicCAS(&e->code, 0, code) == 0) {
-    e->atom = atom; e->count = count; e->step = step;
+    e->atom = atom; e->count = count; e->step = step + e->base;
   }
 }
 
@@ -414,6 +417,16 @@ struct pfm_ctx {
   bool sorted_once = false;
   // tuning
   int lanes = 8, block = 256;  // lanes per list row and workgroup size of the pair passes
+  // captured steady-state blocks of steps (pfm_run)
+  bool use_graphs = true;
+  int step_base = 0;  // first step of the block being captured (kernel step arguments are relative to it)
+  struct GraphEntry {
+    std::vector<uint64_t> key;    // configuration and buffer identities at the block start
+    std::vector<uint64_t> after;  // buffer identities after the block
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  long graph_captures = 0, graph_launches = 0;
   // timing
   bool timing = false;
   std::vector<KTimer> timers;

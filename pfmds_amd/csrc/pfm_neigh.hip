@@ -660,7 +660,7 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   a.nnum = L.nnum;
   a.nbr = L.nbr;
   a.err = h->derr;
-  a.step = step;
+  a.step = step - h->step_base;
   a.err_code = PFM_E_OVERFLOW;
   a.exact = (k == 2) ? 1 : 0;
   const int* row_of = h->grp.row_of[L.g1];
@@ -668,7 +668,7 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   int nb = (L.nrows + 255) / 256;
   if (it.kind == PFM_TB || (k == 2 && !(int)it.p[it.kind == PFM_LJC ? 5 : 6])) {
     PFM_LAUNCH(h, "list_rev", k_rev, nb, 256, L.nrows, h->grp.rows[L.g1], h->grp.row_of[L.g1], L.nnum, L.nbr,
-               L.stride, L.rev, h->derr, h->orig, step);
+               L.stride, L.rev, h->derr, h->orig, step - h->step_base);
   }
   if (it.kind == PFM_LJ1G) {
     PFM_LAUNCH(h, "list_cut", k_lj1g_loc, (h->N + 255) / 256, 256, h->N, h->orig, h->grp.loc[L.g2], it.lc);
@@ -688,11 +688,11 @@ int launch_nn_from_tb(pfm_ctx* h, int ii, int step) {
   if (L.nrows == 0) return PFM_OK;
   int nb = (L.nrows + 255) / 256;
   PFM_LAUNCH(h, "nn_from_tb", k_nn_from_tb, nb, 256, L.nrows, h->grp.rows[L.g1], T.nnum, T.nbr, T.stride, h->pos4,
-             h->rep, h->rdev, L.rcut, L.nnum, L.nbr, h->derr, h->orig, step);
+             h->rep, h->rdev, L.rcut, L.nnum, L.nbr, h->derr, h->orig, step - h->step_base);
   bool simplified = (int)it.p[it.kind == PFM_LJC ? 5 : 6];
   if (!simplified)
     PFM_LAUNCH(h, "list_rev", k_rev, nb, 256, L.nrows, h->grp.rows[L.g1], h->grp.row_of[L.g1], L.nnum, L.nbr,
-               L.stride, L.rev, h->derr, h->orig, step);
+               L.stride, L.rev, h->derr, h->orig, step - h->step_base);
   L.built = true;
   return PFM_OK;
 }

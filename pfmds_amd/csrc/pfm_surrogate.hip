@@ -972,7 +972,7 @@ void launch_sur_converse(pfm_ctx* h, Inter& it, int step) {
     long n0 = (long)L0.nrows * L0.stride;
     PFM_LAUNCH(h, "list_conv", k_sur_transpose, (int)((n0 + 255) / 256), 256, L0.nrows, L0.stride, h->grp.rows[L0.g1],
                L0.nnum, L0.nbr, h->grp.row_of[L1.g1], L1.stride, L1.nmax, L1.nnum, L1.nbr, L1.rev, h->derr, h->orig,
-               step);
+               step - h->step_base);
   }
   PFM_LAUNCH(h, "list_conv", k_sur_clip, (L1.nrows + 255) / 256, 256, L1.nrows, L1.nnum, L1.nmax);
 }

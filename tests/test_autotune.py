@@ -9,16 +9,21 @@ from pfmds_amd import autotune, capi, configs
 class FakeCtx:
     """Steps cost a time that depends on the launch shape."""
 
-    def __init__(self, cost):
+    def __init__(self, cost, graph_gain=1.0):
         self.cost, self.shape, self.set_calls, self.runs = cost, None, [], []
+        self.graphs, self.graph_gain = False, graph_gain
 
     def set_tuning(self, lanes, block):
         self.shape = (lanes, block)
         self.set_calls.append(self.shape)
 
+    def set_graphs(self, on):
+        self.graphs = on
+
     def run(self, integ, dt, s0, s1, energy):
         self.runs.append((s0, s1))
-        time.sleep(self.cost[self.shape] * (s1 - s0 + 1))
+        t = self.cost[self.shape] * (s1 - s0 + 1)
+        time.sleep(t / self.graph_gain if self.graphs else t)
 
 
 def test_tune_sets_and_caches_the_fastest_shape(tmp_path):
@@ -27,21 +32,30 @@ def test_tune_sets_and_caches_the_fastest_shape(tmp_path):
     ctx = FakeCtx(cost)
     s = configs.fcc_cu_rgl(nx=3)
     path = str(tmp_path / "t.json")
-    lanes, block, src, nxt = autotune.autotune(ctx, s, "nve", 1.0, capi.MIXED, first_step=11, path=path, steps=5)
+    lanes, block, graphs, src, nxt = autotune.autotune(ctx, s, "nve", 1.0, capi.MIXED, first_step=11, path=path,
+                                                       steps=5)
     assert (lanes, block, src) == (16, 512, "tuned") and ctx.shape == (16, 512)
     # consecutive, non-overlapping step ranges starting at first_step
     assert ctx.runs[0][0] == 11 and all(a[1] + 1 == b[0] for a, b in zip(ctx.runs, ctx.runs[1:]))
     assert nxt == ctx.runs[-1][1] + 1
     ctx2 = FakeCtx(cost)
-    lanes, block, src, nxt = autotune.autotune(ctx2, s, "nve", 1.0, capi.MIXED, first_step=5, path=path)
-    assert (lanes, block, src, nxt) == (16, 512, "cache", 5) and ctx2.runs == []
+    lanes, block, graphs2, src, nxt = autotune.autotune(ctx2, s, "nve", 1.0, capi.MIXED, first_step=5, path=path)
+    assert (lanes, block, graphs2, src, nxt) == (16, 512, graphs, "cache", 5) and ctx2.runs == []
+
+
+def test_graphs_chosen_when_faster():
+    cost = {c: 1e-4 for c in autotune.CANDIDATES}
+    fast = FakeCtx(cost, graph_gain=3.0)
+    assert autotune.tune(fast, "nve", 1.0, capi.MIXED, steps=3, repeats=1)[2] is True and fast.graphs
+    slow = FakeCtx(cost, graph_gain=0.3)
+    assert autotune.tune(slow, "nve", 1.0, capi.MIXED, steps=3, repeats=1)[2] is False and not slow.graphs
 
 
 def test_fp64_skips_large_workgroups(tmp_path):
     cost = {c: 1e-4 for c in autotune.CANDIDATES}
     ctx = FakeCtx(cost)
-    _, _, table, _ = autotune.tune(ctx, "nve", 1.0, capi.FP64, steps=2, repeats=1)
-    assert all(int(k.split("x")[1]) <= 256 for k in table)
+    _, _, _, table, _ = autotune.tune(ctx, "nve", 1.0, capi.FP64, steps=2, repeats=1)
+    assert all(int(k.split("+")[0].split("x")[1]) <= 256 for k in table)
     assert "8x256" in table
 
 

@@ -210,10 +210,14 @@ def test_surrogate_runs_are_bit_identical():
     _, sur_s, _ = surrogate.split_system(s)
     theta = theta_for(G, seed=9, scale=0.3)
     out = []
-    for _ in range(2):
+    for graphs in (True, True, False):
         c = surrogate.surrogate_context(sur_s, theta)
+        c.set_graphs(graphs)
         c.run("nvt", 0.5, 0, 30, True)
         out.append(c.state())
+        assert (c.graph_count()[1] > 0) == graphs
         c.close()
-    for a, b in zip(out[0], out[1]):
-        assert np.array_equal(a, b)
+    # twice with captured blocks, once launched step by step
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a, b)

@@ -35,7 +35,9 @@ def test_oracle_mirrors_the_abi():
         # GPU-only extras: profiling, launch shape, replica batches (pinned
         # against single-system GPU runs, not the checker) and the learned
         # surrogate (pinned against tests/sur_ref.py)
-        if s in ("pfm_set_timing", "pfm_kernel_time", "pfm_set_tuning", "pfm_create_batch", "pfm_replica_energies") \
+        if s in ("pfm_set_timing", "pfm_kernel_time", "pfm_set_tuning", "pfm_set_graphs", "pfm_graph_count",
+                 "pfm_create_batch",
+                 "pfm_replica_energies") \
                 or s.startswith("pfm_sur_"):
             continue
         assert hasattr(lib, s.replace("pfm_", "orc_")), s
