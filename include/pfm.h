@@ -50,6 +50,10 @@ extern "C" {
                          g1 included), nl(2) g2 -> g1, the converse (list r_cut >= r_c).  The
                          network comes from pfm_sur_set_params.                                     */
 
+#define PFM_REBOSC 8 /* 'rebosc' params: A Q alpha B1 B2 B3 beta1 beta2 beta3 T g1..g6 R1 R2
+                         (REBOsolidcarbon.f90:17-22); forces by central differences of the energy
+                         (md_interactions.f90:273-310), always in FP64                                */
+
 /* number of surrogate parameters: mu[40] isg[40] W1[64][40] b1[64] W2[64][64] b2[64] w3[64] b3 */
 #define PFM_SUR_NTHETA 6929
 

@@ -65,7 +65,7 @@ typedef struct {
 
 typedef struct {
   int kind, nl_n, g[6];
-  double p[16];
+  double p[24];
   onl nl[3];
   double energy;
   double* gr_norm; /* 3*nl[2].N, graphene normals for ljc/morsec */
@@ -207,16 +207,16 @@ static void nl_alloc(onl* nl) {
 
 int orc_add_interaction(octx* h, int kind, const double* prm, int np, int nl_n, const int* gn,
                         const int* nmax, const double* rcut, const int* period) {
-  static const int need_np[7] = {0, 4, 4, 6, 7, 7, 10};
-  static const int need_nl[7] = {0, 2, 1, 3, 3, 1, 1};
-  if (kind < 1 || kind > 6) return oerr(h, PFM_E_UNKNOWN_KIND, "error: unknown interaction name");
+  static const int need_np[9] = {0, 4, 4, 6, 7, 7, 10, 0, 18};
+  static const int need_nl[9] = {0, 2, 1, 3, 3, 1, 1, 0, 1};
+  if (kind < 1 || kind > 8 || kind == PFM_SUR) return oerr(h, PFM_E_UNKNOWN_KIND, "error: unknown interaction name");
   if (h->ninter >= MAXI) return oerr(h, PFM_E_INVALID, "error: too many interactions");
   if (np < need_np[kind] || nl_n != need_nl[kind])
     return oerr(h, PFM_E_INVALID, "error: wrong parameter or list count for interaction");
   ointer* it = &h->inter[h->ninter];
   memset(it, 0, sizeof *it);
   it->kind = kind; it->nl_n = nl_n;
-  for (int i = 0; i < np && i < 16; ++i) it->p[i] = prm[i];
+  for (int i = 0; i < np && i < 24; ++i) it->p[i] = prm[i];
   for (int j = 0; j < nl_n; ++j) {
     it->g[2 * j] = gn[2 * j]; it->g[2 * j + 1] = gn[2 * j + 1];
     if (gn[2 * j] < 1 || gn[2 * j] > MAXG || gn[2 * j + 1] < 1 || gn[2 * j + 1] > MAXG)
@@ -401,7 +401,7 @@ static int update_lists(octx* h, int step) {
     }
     if (it->kind == PFM_LJC || it->kind == PFM_MORSEC) {
       int j;
-      for (j = 0; j < h->ninter; ++j) if (h->inter[j].kind == PFM_TB) break;
+      for (j = 0; j < h->ninter; ++j) if (h->inter[j].kind == PFM_TB || h->inter[j].kind == PFM_REBOSC) break;
       if (j < h->ninter) {
         /* update_nearest_neighbours_in_graphene (graphenenorm.f90:58-71) */
         onl* tbl = &h->inter[j].nl[0];
@@ -863,6 +863,155 @@ static double tb_energy(const ointer* it, const onl* nl) {
   return E;
 }
 
+
+/* ------------------------------------------------------------- rebosc */
+
+/* REBOsc_energy (REBOsolidcarbon.f90:27-92).  Parameters in read order
+ * (REBOsolidcarbon.f90:17-22): A Q alpha B(3) beta(3) T g(6) R1 R2. */
+static double rebo_energy(const ointer* it, const onl* nl) {
+  const double *P = it->p, A = P[0], Q = P[1], al = P[2], *B = P + 3, *be = P + 6, T = P[9], *g = P + 10;
+  const double R1 = P[16], R2 = P[17];
+  size_t ne = (size_t)nl->N * nl->nmax;
+  double* bsp = (double*)calloc(ne ? ne : 1, sizeof(double));
+  double* bdh = (double*)calloc(ne ? ne : 1, sizeof(double));
+  for (int i = 0; i < nl->N; ++i) {
+    for (int p = 0; p < nl->nnum[i]; ++p) {
+      size_t e = (size_t)i * nl->nmax + p;
+      if (nl->moddr[e] < R2) {
+        int j = nl->nlist[e];
+        const double* dp = &nl->dr[3 * e];
+        for (int q = 0; q < nl->nnum[i]; ++q) {
+          size_t eq = (size_t)i * nl->nmax + q;
+          if (p != q && nl->moddr[eq] < R2) {
+            const double* dq = &nl->dr[3 * eq];
+            double c = (dp[0] * dq[0] + dp[1] * dq[1] + dp[2] * dq[2]) / (nl->moddr[e] * nl->moddr[eq]);
+            bsp[e] = bsp[e] + fcut(nl->moddr[eq], R1, R2) *
+                                  (g[0] + g[1] * c + g[2] * c * c + g[3] * c * c * c + g[4] * c * c * c * c +
+                                   g[5] * c * c * c * c * c);
+            for (int l = 0; l < nl->nnum[j]; ++l) {
+              size_t el = (size_t)j * nl->nmax + l;
+              if (nl->nlist[el] != i && nl->moddr[el] < R2) {
+                const double* dl = &nl->dr[3 * el];
+                double aa = nl->moddr[e] * nl->moddr[e];
+                double ab = dp[0] * dq[0] + dp[1] * dq[1] + dp[2] * dq[2];
+                double ac = dp[0] * dl[0] + dp[1] * dl[1] + dp[2] * dl[2];
+                double bc = dq[0] * dl[0] + dq[1] * dl[1] + dq[2] * dl[2];
+                double x = aa * bc - ab * ac;
+                bdh[e] = bdh[e] + fcut(nl->moddr[eq], R1, R2) * fcut(nl->moddr[el], R1, R2) *
+                                      (1. - x * x / (aa * nl->moddr[eq] * nl->moddr[eq] - ab * ab) /
+                                                (aa * nl->moddr[el] * nl->moddr[el] - ac * ac));
+              }
+            }
+          }
+        }
+        bsp[e] = pow(1. + bsp[e], -0.5);
+        bdh[e] = T * bdh[e];
+      }
+    }
+  }
+  double E = 0.0;
+  for (int i = 0; i < nl->N; ++i) {
+    for (int p = 0; p < nl->nnum[i]; ++p) {
+      size_t e = (size_t)i * nl->nmax + p;
+      double r = nl->moddr[e];
+      if (r < R2) {
+        int j = nl->nlist[e];
+        if (j > i) {
+          int q = find_in_row(nl, j, i);
+          double bq = q < nl->nmax ? bsp[(size_t)j * nl->nmax + q] : 0.0;
+          E = E + fcut(r, R1, R2) * ((1 + Q / r) * A * exp(-al * r) -
+                                     ((bsp[e] + bq) / 2 + bdh[e]) *
+                                         (B[0] * exp(-be[0] * r) + B[1] * exp(-be[1] * r) + B[2] * exp(-be[2] * r)));
+        }
+      }
+    }
+  }
+  free(bsp); free(bdh);
+  return E;
+}
+
+/* calculate_truncated_nl (md_interactions.f90:337-384) for row i with
+ * neighbour order n: the rows of the atoms within n-1 hops of i copied
+ * whole, the atoms at n hops given their first back entry. */
+static void truncated_nl(onl* t, const onl* nl, int i, int n) {
+  for (int j = 0; j < t->N; ++j) { t->pidx[j] = -1; t->nnum[j] = 0; }
+  size_t m = nl->nmax;
+  memcpy(&t->nlist[i * m], &nl->nlist[i * m], sizeof(int) * nl->nnum[i]);
+  memcpy(&t->dr[3 * i * m], &nl->dr[3 * i * m], sizeof(double) * 3 * nl->nnum[i]);
+  memcpy(&t->moddr[i * m], &nl->moddr[i * m], sizeof(double) * nl->nnum[i]);
+  t->nnum[i] = nl->nnum[i];
+  t->pidx[i] = nl->pidx[i];
+  for (int ni = 1; ni <= n; ++ni) {
+    if (ni < n) {
+      for (int j = 0; j < t->N; ++j)
+        for (int p = 0; p < t->nnum[j]; ++p) {
+          int q = t->nlist[j * m + p];
+          if (t->nnum[q] == 0) {
+            memcpy(&t->nlist[q * m], &nl->nlist[q * m], sizeof(int) * nl->nnum[q]);
+            memcpy(&t->dr[3 * q * m], &nl->dr[3 * q * m], sizeof(double) * 3 * nl->nnum[q]);
+            memcpy(&t->moddr[q * m], &nl->moddr[q * m], sizeof(double) * nl->nnum[q]);
+            t->pidx[q] = nl->pidx[q];
+          }
+        }
+      for (int j = 0; j < t->N; ++j)
+        if (t->pidx[j] >= 0) t->nnum[j] = nl->nnum[j];
+    } else {
+      for (int j = 0; j < t->N; ++j)
+        for (int p = 0; p < t->nnum[j]; ++p) {
+          int q = t->nlist[j * m + p];
+          if (t->nnum[q] == 0) {
+            t->nnum[q] = t->nnum[q] + 1;
+            size_t e = q * m + (t->nnum[q] - 1);
+            t->nlist[e] = j;
+            for (int k = 0; k < 3; ++k) t->dr[3 * e + k] = -nl->dr[3 * (j * m + p) + k];
+            t->moddr[e] = nl->moddr[j * m + p];
+            t->pidx[q] = nl->pidx[q];
+          }
+        }
+    }
+  }
+}
+
+/* shift_drs (md_interactions.f90:386-405) for a list of one row set (nl_n = 1) */
+static void shift_drs(onl* t, int inl, int k, double dx) {
+  size_t m = t->nmax;
+  for (int p = 0; p < t->nnum[inl]; ++p) {
+    double* d = &t->dr[3 * (inl * m + p)];
+    d[k] = d[k] - dx;
+    t->moddr[inl * m + p] = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  }
+  for (int j = 0; j < t->N; ++j)
+    for (int p = 0; p < t->nnum[j]; ++p)
+      if (t->nlist[j * m + p] == inl) {
+        double* d = &t->dr[3 * (j * m + p)];
+        d[k] = d[k] + dx;
+        t->moddr[j * m + p] = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      }
+}
+
+/* calculate_forces_numerically (md_interactions.f90:273-310) for rebosc
+ * (neighbour order 3, md_interactions.f90:111): central differences of the
+ * energy of the truncated list, dx = 1e-6 A. */
+static void rebo_forces(octx* h, const ointer* it, const onl* nl) {
+  const double dx = 1e-6;
+  onl t;
+  memset(&t, 0, sizeof t);
+  t.N = nl->N; t.nmax = nl->nmax;
+  nl_alloc(&t);
+  for (int inl = 0; inl < nl->N; ++inl) {
+    truncated_nl(&t, nl, inl, 3);
+    for (int k = 0; k < 3; ++k) {
+      shift_drs(&t, inl, k, -dx);
+      double e1 = rebo_energy(it, &t);
+      shift_drs(&t, inl, k, 2 * dx);
+      double e2 = rebo_energy(it, &t);
+      if (k != 2) shift_drs(&t, inl, k, -dx);
+      h->force[3 * nl->pidx[inl] + k] += (e1 - e2) / 2 / dx;
+    }
+  }
+  free(t.nlist); free(t.nnum); free(t.lessnnum); free(t.pidx); free(t.dr); free(t.moddr);
+}
+
 /* calculate_forces (md_interactions.f90:210-242) */
 static int calc_forces(octx* h) {
   for (int ii = 0; ii < h->ninter; ++ii) {
@@ -881,6 +1030,9 @@ static int calc_forces(octx* h) {
       case PFM_RJL: rjl_forces(h, it, &it->nl[0]); break;
     }
   }
+  /* calculate_forces_numerically (md_simulation.f90:168) */
+  for (int ii = 0; ii < h->ninter; ++ii)
+    if (h->inter[ii].kind == PFM_REBOSC) rebo_forces(h, &h->inter[ii], &h->inter[ii].nl[0]);
   return PFM_OK;
 }
 
@@ -895,6 +1047,7 @@ static void calc_energies(octx* h) {
       case PFM_LJC: case PFM_MORSEC: it->energy = cos_energy(it); break;
       case PFM_TB: it->energy = tb_energy(it, &it->nl[0]); break;
       case PFM_RJL: it->energy = rjl_energy(it, &it->nl[0]); break;
+      case PFM_REBOSC: it->energy = rebo_energy(it, &it->nl[0]); break;
     }
     pe += it->energy;
   }

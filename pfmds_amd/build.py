@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 FLANG = os.path.join(ROCM, "bin", "amdflang")
-HIP_SOURCES = ["pfm_context.hip", "pfm_neigh.hip", "pfm_forces.hip", "pfm_surrogate.hip"]
+HIP_SOURCES = ["pfm_context.hip", "pfm_neigh.hip", "pfm_forces.hip", "pfm_surrogate.hip", "pfm_rebo.hip"]
 # packed FP32 instructions are disabled on purpose: the compiler forms them
 # from plain float code and some forms are unreliable on this hardware
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",

@@ -14,7 +14,7 @@ import numpy as np
 
 # constants of include/pfm.h
 OK, E_OVERFLOW, E_OUT_OF_CELL, E_GRAPHENE_NN, E_NHC_PARAMS, E_UNKNOWN_KIND, E_INVALID, E_HIP, E_NO_DEVICE = range(9)
-KINDS = {"lj": 1, "lj1g": 2, "ljc": 3, "morsec": 4, "rjl": 5, "tb": 6, "sur": 7}
+KINDS = {"lj": 1, "lj1g": 2, "ljc": 3, "morsec": 4, "rjl": 5, "tb": 6, "sur": 7, "rebosc": 8}
 SUR_NTHETA = 6929  # PFM_SUR_NTHETA
 INTEGRATORS = {"nvt": 1, "nve": 2, "nvms": 3}
 FP64, MIXED = 0, 1

@@ -143,6 +143,16 @@ def fcc_cu_rgl(nx=32, ny=None, nz=None, a=3.615, seed=1001, sigma=0.02, temperat
 
 TB_BRENNER_I = [6.325, 1.29, 1.5, 1.315, 0.80469, 0.011304, 19.0, 2.5, 1.7, 2.0]
 MORSEC_CCU = [0.025, 3.05, 1.30, 2.0, 6.0, 7.0, 0.0]
+# rebosc (REBOsolidcarbon.f90:17-22): A Q alpha / B(3) / beta(3) / T / g(6) / R1 R2.
+# The C-C pair terms of Brenner's second-generation REBO (J. Phys.: Condens.
+# Matter 14, 783 (2002), Table 2), its torsion prefactor, and the quintic
+# g(cos) through Brenner's G_C knots (-1, -2/3, -1/2, -1/3, 1) and g(0) = 0.375.
+REBO_C = [10953.544162170, 0.3134602960833, 4.7465390606595,
+          12388.79197798, 17.56740646509, 30.71493208065,
+          4.7204523127, 1.4332132499, 1.3826912506,
+          -0.004048375,
+          0.375, 1.493091717, 2.689174625, 2.427981333, 0.930825375, 0.08392695,
+          1.7, 2.0]
 
 
 def graphene_on_cu(cx=100, cy=58, layers=6, frozen=2, gx=104, gy=60, seed=1002, acu=2.5562, gap=3.2, lz=40.0,
@@ -197,6 +207,28 @@ def graphene_on_cu(cx=100, cy=58, layers=6, frozen=2, gx=104, gy=60, seed=1002, 
                   integrators=ints, nhcs=[NHC(3, temperature, 3, q1)], all_moving=3, xyz_moving=3, z_moving=6,
                   all_atoms=4, traj=1, period_traj=10**9, zero_momentum_period=10**9, ms_de=1e-8,
                   md_step_limit=100000 + nvt_steps, init_xyz_filename="graphene_on_cu.xyz")
+
+
+def graphene_rebo(gx=8, gy=5, acc=1.42, lz=20.0, jitter=0.0, buckle=0.0, seed=1005, temperature=300.0, dt=0.5,
+                  period=10):
+    """Free-standing graphene sheet of gx*gy rectangular 4-atom cells with
+    rebosc: seeded in-plane jitter and out-of-plane buckling amplitudes (A),
+    Maxwell velocities."""
+    rng = SplitMix64(seed)
+    sx, sy = np.sqrt(3.0) * acc, 3.0 * acc
+    cell = np.array([[0.0, 0.0], [0.5, 1.0 / 6.0], [0.5, 0.5], [0.0, 2.0 / 3.0]]) * np.array([sx, sy])
+    ij = np.stack(np.meshgrid(np.arange(gx), np.arange(gy), indexing="ij"), -1).reshape(-1, 2)
+    xy = (ij[:, None, :] * np.array([sx, sy]) + cell[None]).reshape(-1, 2) + 0.3
+    n = len(xy)
+    pos = np.column_stack([xy, np.full(n, 0.5 * lz)])
+    pos[:, :2] += jitter * (2 * rng.uniform(2 * n).reshape(n, 2) - 1)
+    pos[:, 2] += buckle * (2 * rng.uniform(n) - 1)
+    box = np.array([gx * sx, gy * sy, lz])
+    mass = np.full(n, 12.011)
+    vel = maxwell(rng, mass, temperature)
+    return System(box=box, pos=np.mod(pos, box), vel=vel, mass=mass, species=["C"] * n, groups=[["C"], ["#"]],
+                  interactions=[Interaction("rebosc", list(REBO_C), [(1, 1, 8, REBO_C[17] + 0.3, period)])],
+                  integrators=[Integrator("nve", dt, 1000, 10**9, 10**9)], z_moving=2)
 
 
 # ------------------------------------------------------------- replica batches

@@ -527,7 +527,7 @@ static int finalize(pfm_ctx* h) {
       L.stride = k == 2 ? L.nmax : (L.nmax + 3) / 4 * 4;
       if ((rc = dalloc(h, &L.nnum, L.nrows))) return rc;
       if ((rc = dalloc(h, &L.nbr, (size_t)L.nrows * L.stride + 32))) return rc;
-      if (it.kind == PFM_TB || k == 2 || (it.kind == PFM_SUR && k == 1))
+      if (it.kind == PFM_TB || it.kind == PFM_REBOSC || k == 2 || (it.kind == PFM_SUR && k == 1))
         if ((rc = dalloc(h, &L.rev, (size_t)L.nrows * L.stride))) return rc;
     }
     it.e_rows = it.nl[0].nrows;
@@ -556,9 +556,9 @@ static int finalize(pfm_ctx* h) {
         if (in1[x]) { it.rows_disjoint = false; break; }
     }
   }
-  // group-local index by original atom, for lj1g and list downloads
+  // group-local index by original atom, for lj1g and rebosc
   for (auto& it : h->inter) {
-    if (it.kind != PFM_LJ1G) continue;
+    if (it.kind != PFM_LJ1G && it.kind != PFM_REBOSC) continue;
     int g = it.nl[0].g2;
     if (h->grp.loc[g]) continue;
     std::vector<int> loc(h->N, -1);
@@ -953,17 +953,17 @@ int pfm_set_roles(pfm_ctx* h, int all_moving, int xyz_moving, int z_moving, int 
 
 int pfm_add_interaction(pfm_ctx* h, int kind, const double* params, int np, int nl_n, const int* gn,
                         const int* nmax, const double* rcut, const int* period) {
-  static const int need_np[8] = {0, 4, 4, 6, 7, 7, 10, 1};
-  static const int need_nl[8] = {0, 2, 1, 3, 3, 1, 1, 2};
+  static const int need_np[9] = {0, 4, 4, 6, 7, 7, 10, 1, 18};
+  static const int need_nl[9] = {0, 2, 1, 3, 3, 1, 1, 2, 1};
   if (h->sorted_once) return fail(h, PFM_E_INVALID, "error: interactions must be added before the first pfm_run");
-  if (kind < 1 || kind > 7) return fail(h, PFM_E_UNKNOWN_KIND, "error: unknown interaction name");
+  if (kind < 1 || kind > 8) return fail(h, PFM_E_UNKNOWN_KIND, "error: unknown interaction name");
   if ((int)h->inter.size() >= PFM_MAXI) return fail(h, PFM_E_INVALID, "error: too many interactions");
   if (np < need_np[kind] || nl_n != need_nl[kind])
     return fail(h, PFM_E_INVALID, "error: wrong parameter or list count for interaction");
   Inter it;
   it.kind = kind;
   it.nl_n = nl_n;
-  for (int i = 0; i < np && i < 16; ++i) it.p[i] = params[i];
+  for (int i = 0; i < np && i < 24; ++i) it.p[i] = params[i];
   for (int k = 0; k < nl_n; ++k) {
     int g1 = gn[2 * k], g2 = gn[2 * k + 1];
     if (g1 < 1 || g1 > PFM_MAXG || g2 < 1 || g2 > PFM_MAXG)
@@ -975,12 +975,15 @@ int pfm_add_interaction(pfm_ctx* h, int kind, const double* params, int np, int 
     L.nrows = h->grp.n[g1];
     h->grp.used[g1] = true;
   }
-  if (kind == PFM_LJ1G || kind == PFM_RJL || kind == PFM_TB) {
+  if (kind == PFM_LJ1G || kind == PFM_RJL || kind == PFM_TB || kind == PFM_REBOSC) {
     // these loops index group-2 entries with group-1 arrays
-    // (LennardJones_1g.f90:83, RosatoGuillopeLegrand.f90:88, TersoffBrenner.f90:56-57,112)
+    // (LennardJones_1g.f90:83, RosatoGuillopeLegrand.f90:88, TersoffBrenner.f90:56-57,112,
+    // REBOsolidcarbon.f90:49,75)
     if (h->grp.idx[it.nl[0].g1] != h->grp.idx[it.nl[0].g2])
-      return fail(h, PFM_E_INVALID, "error: lj1g/rjl/tb need the same group on both sides of the list");
+      return fail(h, PFM_E_INVALID, "error: lj1g/rjl/tb/rebosc need the same group on both sides of the list");
   }
+  if (kind == PFM_REBOSC && it.nl[0].nmax > 32)
+    return fail(h, PFM_E_INVALID, "error: rebosc neighb_num_max above 32 is not supported");
   if (kind == PFM_SUR) {
     std::vector<char> in2(h->N, 0);
     for (int x : h->grp.idx[it.nl[0].g2]) in2[x] = 1;
@@ -1001,7 +1004,7 @@ int pfm_add_interaction(pfm_ctx* h, int kind, const double* params, int np, int 
       return fail(h, PFM_E_INVALID, "error: ljc/morsec nearest-neighbour rows must be the graphene group of list 1");
     // md_interactions.f90:159-171: the first tb interaction's list feeds nl(3)
     for (size_t j = 0; j < h->inter.size(); ++j)
-      if (h->inter[j].kind == PFM_TB) { it.nn_from_tb = (int)j; break; }
+      if (h->inter[j].kind == PFM_TB || h->inter[j].kind == PFM_REBOSC) { it.nn_from_tb = (int)j; break; }
     if (it.nn_from_tb >= 0) {
       Inter& tb = h->inter[it.nn_from_tb];
       if (h->grp.idx[tb.nl[0].g1] != h->grp.idx[it.nl[0].g1])
@@ -1009,7 +1012,7 @@ int pfm_add_interaction(pfm_ctx* h, int kind, const double* params, int np, int 
       it.nl[2].period = tb.nl[0].period;
     }
   }
-  if (kind == PFM_TB) {
+  if (kind == PFM_TB || kind == PFM_REBOSC) {
     for (auto& o : h->inter)
       if (is_cos(o.kind) && o.nn_from_tb < 0)
         // the reference copies the empty tb list at step 0 and stops

@@ -753,6 +753,12 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         written = true;
         break;
       }
+      case PFM_REBOSC: {
+        int rc = launch_rebo(h, it, energy, written ? 1 : 0);
+        if (rc) return rc;
+        written = true;
+        break;
+      }
       case PFM_SUR: {
         int rc = launch_sur_forces(h, it, written ? 1 : 0);
         if (rc) {

@@ -318,7 +318,7 @@ struct SurBuf {
 
 struct Inter {
   int kind = 0, nl_n = 0, g[6] = {0, 0, 0, 0, 0, 0};
-  double p[16] = {0};
+  double p[24] = {0};
   ListDev nl[3];
   int nn_from_tb = -1;  // ljc/morsec: index of the tb interaction whose list feeds nl(3)
   double energy = 0.0;
@@ -469,6 +469,7 @@ int sur_alloc(pfm_ctx* h, Inter& it);
 void sur_free(Inter& it);
 void launch_sur_converse(pfm_ctx* h, Inter& it, int step);
 int launch_sur_forces(pfm_ctx* h, Inter& it, int accumulate);
+int launch_rebo(pfm_ctx* h, Inter& it, bool energy, int accumulate);
 // interactions with a converse list nl(2) built with nl(1)'s cutoff and period
 static inline bool has_converse(int kind) {
   return kind == PFM_LJ || kind == PFM_LJC || kind == PFM_MORSEC || kind == PFM_SUR;

@@ -666,7 +666,7 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   const int* row_of = h->grp.row_of[L.g1];
   launch_build(h, a, row_of);
   int nb = (L.nrows + 255) / 256;
-  if (it.kind == PFM_TB || (k == 2 && !(int)it.p[it.kind == PFM_LJC ? 5 : 6])) {
+  if (it.kind == PFM_TB || it.kind == PFM_REBOSC || (k == 2 && !(int)it.p[it.kind == PFM_LJC ? 5 : 6])) {
     PFM_LAUNCH(h, "list_rev", k_rev, nb, 256, L.nrows, h->grp.rows[L.g1], h->grp.row_of[L.g1], L.nnum, L.nbr,
                L.stride, L.rev, h->derr, h->orig, step - h->step_base);
   }

@@ -6,7 +6,8 @@ use iso_c_binding
 implicit none
 
 integer(c_int), parameter :: PFM_OK = 0
-integer(c_int), parameter :: PFM_LJ = 1, PFM_LJ1G = 2, PFM_LJC = 3, PFM_MORSEC = 4, PFM_RJL = 5, PFM_TB = 6
+integer(c_int), parameter :: PFM_LJ = 1, PFM_LJ1G = 2, PFM_LJC = 3, PFM_MORSEC = 4, PFM_RJL = 5, PFM_TB = 6, &
+                              PFM_REBOSC = 8
 integer(c_int), parameter :: PFM_NVT = 1, PFM_NVE = 2, PFM_NVMS = 3
 integer(c_int), parameter :: PFM_FP64 = 0, PFM_MIXED = 1
 integer(c_int), parameter :: PFM_EV_DONE = 0, PFM_EV_CONVERGED = 1

@@ -107,7 +107,7 @@ end subroutine append_trajectory
 subroutine read_params(path, kind, p, np)
   character(len=*) :: path
   integer :: kind, np, u
-  real(8) :: p(16)
+  real(8) :: p(24)
   logical :: simp
   p = 0.d0
   open(newunit=u, file=path, status='old')
@@ -136,6 +136,15 @@ subroutine read_params(path, kind, p, np)
     read(u,*) p(1), p(2), p(3), p(4), p(5), p(6), p(7), p(8)
     read(u,*) p(9), p(10)
     np = 10
+  case (PFM_REBOSC)
+    ! read_REBOsc_parameters (REBOsolidcarbon.f90:17-22)
+    read(u,*) p(1), p(2), p(3)
+    read(u,*) p(4), p(5), p(6)
+    read(u,*) p(7), p(8), p(9)
+    read(u,*) p(10)
+    read(u,*) p(11), p(12), p(13), p(14), p(15), p(16)
+    read(u,*) p(17), p(18)
+    np = 18
   end select
   close(u)
 end subroutine read_params
@@ -205,7 +214,7 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
   real(8), allocatable :: idt(:)
   character(len=32), allocatable :: iname(:), names(:,:), types(:)
   real(8), allocatable :: pos(:,:), vel(:,:), frc(:,:), mass(:), e_inter(:), e_nhc(:)
-  real(8) :: box(3), ms_de, t0, nhc_t, nhc_q1, p(16), rc_(3), pe, ke, temp, sim_time, conserved, total, e_nhc_sum
+  real(8) :: box(3), ms_de, t0, nhc_t, nhc_q1, p(24), rc_(3), pe, ke, temp, sim_time, conserved, total, e_nhc_sum
   real(8) :: fs(3), mc(3), mcv(3), tm, vmax, wall0, wall_md, wall_out, wtmp, w2
   integer(c_int) :: gn(6), nmax(3), per(3)
   logical :: newv, invert, energy, with_nhc_energy
@@ -296,6 +305,7 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
     case ('morsec'); kind = PFM_MORSEC; nl_n = 3
     case ('rjl');    kind = PFM_RJL;    nl_n = 1
     case ('tb');     kind = PFM_TB;     nl_n = 1
+    case ('rebosc'); kind = PFM_REBOSC; nl_n = 1
     case default
       write(out_id,*) 'error: unknown interaction name ', trim(inter_name)
       stop 1

@@ -28,8 +28,9 @@ PARAM_LINES = {
     "morsec": [4, 2, 1],
     "rjl": [5, 2],
     "tb": [8, 2],
+    "rebosc": [3, 3, 3, 1, 6, 2],
 }
-NL_N = {"lj": 2, "lj1g": 1, "ljc": 3, "morsec": 3, "rjl": 1, "tb": 1}
+NL_N = {"lj": 2, "lj1g": 1, "ljc": 3, "morsec": 3, "rjl": 1, "tb": 1, "rebosc": 1}
 def _f(x):
     return repr(float(x))
 

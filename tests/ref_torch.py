@@ -114,6 +114,55 @@ def e_tb(pos, box, g, D, S, b, r0, delt, a0, c0, d0, R1, R2, pos_np):
     return 0.5 * pair.sum()
 
 
+def e_rebo(pos, box, g, params, pos_np):
+    """REBO solid-carbon energy (REBOsolidcarbon.f90:27-92) over explicit
+    neighbour sets: per directed bond (i, j) the sigma-pi term
+    (1 + sum_{k != j} fc(r_ik) g(cos theta_jik))^-1/2 and the dihedral term
+    T sum_{k != j} sum_{l != i} fc(r_ik) fc(r_jl) sin^2(dihedral kijl), the bond
+    energy counted once from its lower-index atom."""
+    A, Q, al = params[0:3]
+    B, be = params[3:6], params[6:9]
+    T, gc = params[9], params[10:16]
+    R1, R2 = params[16], params[17]
+    i, j = pairs_within(pos_np, box.numpy(), g, g, R2)
+    d = disp(pos, box, i, j)
+    r = torch.sqrt((d * d).sum(-1))
+    f = fc_cos(r, R1, R2)
+    bonds = {}
+    for e, (a, c) in enumerate(zip(i, j)):
+        bonds.setdefault(int(a), []).append(e)
+    E = torch.zeros((), dtype=pos.dtype)
+    for e, (a, c) in enumerate(zip(i, j)):
+        a, c = int(a), int(c)
+        if c < a:
+            continue
+        def bsp(e0, atom):
+            s = torch.zeros((), dtype=pos.dtype)
+            for e2 in bonds[atom]:
+                if e2 != e0:
+                    cs = (d[e0] * d[e2]).sum() / (r[e0] * r[e2])
+                    s = s + f[e2] * sum(gc[k] * cs ** k for k in range(6))
+            return (1.0 + s) ** -0.5
+        back = [e2 for e2 in bonds[c] if int(j[e2]) == a][0]
+        bdh = torch.zeros((), dtype=pos.dtype)
+        for eq in bonds[a]:
+            if eq == e:
+                continue
+            for el in bonds[c]:
+                if int(j[el]) == a:
+                    continue
+                aa = (d[e] * d[e]).sum()
+                ab = (d[e] * d[eq]).sum()
+                ac = (d[e] * d[el]).sum()
+                bc = (d[eq] * d[el]).sum()
+                bdh = bdh + f[eq] * f[el] * (1.0 - (aa * bc - ab * ac) ** 2 / (aa * r[eq] ** 2 - ab ** 2)
+                                                   / (aa * r[el] ** 2 - ac ** 2))
+        rr = r[e]
+        att = sum(B[k] * torch.exp(-be[k] * rr) for k in range(3))
+        E = E + f[e] * ((1 + Q / rr) * A * torch.exp(-al * rr) - ((bsp(e, a) + bsp(back, c)) / 2 + T * bdh) * att)
+    return E
+
+
 def graphene_normals(pos, box, gc, rnn, pos_np):
     """n_i of graphenenorm: three nearest neighbours within rnn, fixed topology."""
     i, j = pairs_within(pos_np, box.numpy(), gc, gc, rnn)

@@ -31,9 +31,12 @@ def parse_log(path, n_inter, n_nhc):
     return recs
 
 
-@pytest.mark.parametrize("case", ["lj_gas", "gr_cu"])
+@pytest.mark.parametrize("case", ["lj_gas", "gr_cu", "rebo"])
 def test_frontend_matches_checker(tmp_path, case):
-    if case == "lj_gas":
+    if case == "rebo":
+        s = configs.graphene_rebo(gx=6, gy=4, jitter=0.05, buckle=0.1, seed=53)
+        s.integrators = [system.Integrator("nve", 0.5, 40, 1000, 4)]
+    elif case == "lj_gas":
         s = configs.binary_lj_gas(n=500, box=30.0, seed=51, schedule=(("nvt", 0.5, 30, 25, 5), ("nve", 0.5, 0, 5, 5),
                                                                        ("nve", 0.5, 30, 100, 5),
                                                                        ("nvms", 0.5, 400, 1000, 5)))

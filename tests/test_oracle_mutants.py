@@ -94,6 +94,18 @@ MUTANTS = {
     "nvms_projection_unnormalised": (
         "v[k] = fv / ff * f[k];", "v[k] = fv * f[k];",
         pins.test_nvms_velocity_rule, ()),
+    "rebo_dihedral_sign": (
+        "bdh[e] = T * bdh[e];", "bdh[e] = -T * bdh[e];",
+        pins.test_forces_rebo_buckled, ()),
+    "rebo_bsp_exponent": (
+        "bsp[e] = pow(1. + bsp[e], -0.5);", "bsp[e] = pow(1. + bsp[e], -1.0);",
+        pins.test_rebo_perfect_graphene_closed_form, ()),
+    "rebo_truncated_two_hops": (
+        "truncated_nl(&t, nl, inl, 3);", "truncated_nl(&t, nl, inl, 2);",
+        pins.test_forces_rebo_buckled, ()),
+    "rebo_partner_bond_order_dropped": (
+        "double bq = q < nl->nmax ? bsp[(size_t)j * nl->nmax + q] : 0.0;", "double bq = bsp[e];",
+        pins.test_forces_rebo_buckled, ()),
     "nvms_signed_energy_change": (
         "fabs(h->pe - h->pe_prev) <= ms_de", "(h->pe - h->pe_prev) <= ms_de",
         pins.test_nvms_lj_dimer_minimum, ()),

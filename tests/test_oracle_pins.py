@@ -399,3 +399,35 @@ def test_nvms_velocity_rule():
             want_v[i] = vk[i]
     assert (np.abs(want_v[[0, 2]]).max(1) > 0).all() and (want_v[[1, 3]] == 0).all()
     np.testing.assert_allclose(v1, want_v, rtol=1e-12, atol=1e-18)
+
+
+# ------------------------------------------------------------------ rebosc
+
+def test_rebo_perfect_graphene_closed_form():
+    """Planar sheet: every dihedral term vanishes (sin^2 = 0) and each bond
+    sees two others at cos = -1/2, so E/atom = 1.5 [(1+Q/r)A e^(-alpha r) -
+    (1 + 2 g(-1/2))^(-1/2) sum_k B_k e^(-beta_k r)] at r = 1.42 < R1; the
+    central-difference forces vanish to their rounding."""
+    s = configs.graphene_rebo(gx=6, gy=4)
+    o = step0(s)
+    P = configs.REBO_C
+    r = 1.42
+    g = sum(P[10 + k] * (-0.5) ** k for k in range(6))
+    eb = (1 + P[1] / r) * P[0] * np.exp(-P[2] * r) - (1 + 2 * g) ** -0.5 * sum(
+        P[3 + k] * np.exp(-P[6 + k] * r) for k in range(3))
+    assert o.energies()["pe"] / s.n == pytest.approx(1.5 * eb, rel=1e-12)
+    assert np.abs(o.state()[2]).max() < 1e-6
+
+
+def test_forces_rebo_buckled():
+    """Jittered and buckled sheet (dihedral and cutoff terms active): the
+    energy equals an independent restatement to 1e-12 and the reference's
+    central differences (dx = 1e-6 A over truncated lists) equal -grad E by
+    autograd to 1e-7."""
+    s = configs.graphene_rebo(gx=5, gy=4, jitter=0.1, buckle=0.2, seed=72)
+    box = torch.tensor(s.box)
+    g = s.group_indices(1) - 1
+    o = step0(s)
+    e, f = R.forces(lambda x: R.e_rebo(x, box, g, configs.REBO_C, s.pos), s.pos)
+    assert o.energies()["pe"] == pytest.approx(e, rel=1e-12)
+    assert rel_inf(o.state()[2], f) < 1e-7
