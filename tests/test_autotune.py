@@ -27,8 +27,8 @@ class FakeCtx:
 
 
 def test_tune_sets_and_caches_the_fastest_shape(tmp_path):
-    cost = {c: 2e-4 for c in autotune.CANDIDATES}
-    cost[(16, 512)] = 2e-5
+    cost = {c: 2e-3 for c in autotune.CANDIDATES}
+    cost[(16, 512)] = 2e-4
     ctx = FakeCtx(cost)
     s = configs.fcc_cu_rgl(nx=3)
     path = str(tmp_path / "t.json")
@@ -44,7 +44,7 @@ def test_tune_sets_and_caches_the_fastest_shape(tmp_path):
 
 
 def test_graphs_chosen_when_faster():
-    cost = {c: 1e-4 for c in autotune.CANDIDATES}
+    cost = {c: 1e-3 for c in autotune.CANDIDATES}
     fast = FakeCtx(cost, graph_gain=3.0)
     assert autotune.tune(fast, "nve", 1.0, capi.MIXED, steps=3, repeats=1)[2] is True and fast.graphs
     slow = FakeCtx(cost, graph_gain=0.3)
