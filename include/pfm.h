@@ -173,6 +173,47 @@ int pfm_set_graphs(pfm_ctx* h, int on);
 /* Blocks captured and blocks launched from graphs since the handle was made. */
 int pfm_graph_count(pfm_ctx* h, long* captured, long* launched);
 
+/* Output records.  md() stops at every log, out, snapshot and trajectory
+ * step to compute energies and write files (md_simulation.f90:188-229,
+ * md_read_write.f90:65-107).  With records set, pfm_run takes these values
+ * on the device at the steps they are due, without leaving the run:
+ *   energy record at every step some energy period divides: interaction
+ *     energies, kinetic energy and temperature of all_moving, chain
+ *     energies, and the out block's sums (calculate_force_sum,
+ *     calculate_mass_center, calculate_mass_center_velocity over all_atoms,
+ *     md_general.f90:184-234,277-298; find_max_velocity over every atom,
+ *     md_general.f90:366-380);
+ *   frame at every step some frame period divides: positions and velocities
+ *     of every atom, double[n][3] each in original order.
+ * Records are written into device slots (frame_slots frames), copied to
+ * pinned host memory on a second stream, and passed to fn in step order
+ * from inside pfm_run while later steps run; the pointers in the record are
+ * valid during the call only.  The trajectory is identical to stopping at
+ * every recorded step.  Every record of a pfm_run call is delivered before it
+ * returns; a nonzero return from fn ends the run with PFM_E_INVALID.
+ * Periods <= 0 are ignored; no periods, or fn NULL, turns records off.
+ * Single systems only (not batches).  No counterpart in the reference, which
+ * computes and writes these synchronously inside the step loop. */
+typedef struct pfm_record {
+  int step;              /* md_step of the record                          */
+  int integrator;        /* PFM_NVT / PFM_NVE / PFM_NVMS                   */
+  int has_energy;        /* the energy fields below are set                */
+  int has_frame;         /* pos and vel are set                            */
+  double sim_time;       /* simulated time after the step (fs)             */
+  double pe, ke, temp;   /* sum of e_inter; all_moving kinetic energy (eV), temperature (K) */
+  double force_sum[3];   /* sum of forces over all_atoms (eV/A)            */
+  double mass_center[3]; /* centre of mass of all_atoms (A)                */
+  double mass_center_vel[3]; /* centre-of-mass velocity of all_atoms (A/fs) */
+  double max_vel;        /* largest |v| over every atom (A/fs)             */
+  const double* e_inter; /* [n_interactions] (eV)                          */
+  const double* e_nhc;   /* [n_chains] chain energies (eV)                 */
+  const double* pos;     /* double[n][3] or NULL                           */
+  const double* vel;     /* double[n][3] or NULL                           */
+} pfm_record;
+typedef int (*pfm_record_fn)(void* user, const pfm_record* rec);
+int pfm_set_records(pfm_ctx* h, int n_energy_periods, const int* energy_periods, int n_frame_periods,
+                    const int* frame_periods, int frame_slots, pfm_record_fn fn, void* user);
+
 /* Learned surrogate (interaction kind PFM_SUR).  None of these has a
  * counterpart in the reference; they serve the training and inference of a
  * per-atom network that stands in for TB_forces + MorseC_forces

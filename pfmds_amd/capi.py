@@ -43,6 +43,18 @@ _SIGS = {
 _libs = {}
 
 
+class Record(C.Structure):
+    """struct pfm_record of include/pfm.h."""
+    _fields_ = [("step", C.c_int), ("integrator", C.c_int), ("has_energy", C.c_int), ("has_frame", C.c_int),
+                ("sim_time", C.c_double), ("pe", C.c_double), ("ke", C.c_double), ("temp", C.c_double),
+                ("force_sum", C.c_double * 3), ("mass_center", C.c_double * 3),
+                ("mass_center_vel", C.c_double * 3), ("max_vel", C.c_double), ("e_inter", _d), ("e_nhc", _d),
+                ("pos", _d), ("vel", _d)]
+
+
+RECORD_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, _P(Record))
+
+
 class PfmError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
@@ -85,6 +97,8 @@ def load_library(path, prefix):
         lib.pfm_sur_descriptors.restype = C.c_int
         lib.pfm_sur_loss_grad.argtypes = [C.c_void_p, C.c_int, _d, C.c_double, C.c_double, C.c_double, _d, _d]
         lib.pfm_sur_loss_grad.restype = C.c_int
+        lib.pfm_set_records.argtypes = [C.c_void_p, C.c_int, _i, C.c_int, _i, C.c_int, RECORD_FN, C.c_void_p]
+        lib.pfm_set_records.restype = C.c_int
     _libs[key] = lib
     return lib
 
@@ -113,6 +127,8 @@ class Context:
         self.n_inter = 0
         self.n_nhc = 0
         self.n_rep = 1
+        self._rec_fn = None
+        self._rec_exc = None
         if n_atoms is not None:
             if prefix != "pfm":
                 raise ValueError("batches need the HIP library")
@@ -190,6 +206,9 @@ class Context:
         ev = C.c_int(0)
         rc = self._fn("run")(self.h, integ, float(dt), int(step_begin), int(step_end), int(bool(energy_at_end)),
                              float(ms_de), C.byref(last), C.byref(ev))
+        if self._rec_exc is not None:
+            exc, self._rec_exc = self._rec_exc, None
+            raise exc
         self._check(rc)
         return last.value, ev.value
 
@@ -253,6 +272,45 @@ class Context:
 
     def set_graphs(self, on):
         self._check(self.lib.pfm_set_graphs(self.h, int(bool(on))))
+
+    def set_records(self, energy_periods=(), frame_periods=(), frame_slots=4, callback=None):
+        """Output records (pfm_set_records): `callback(rec)` receives a dict
+        per recorded step, in step order, during run(): step, integrator,
+        sim_time; with an energy record e_inter, pe, ke, temp, e_nhc,
+        force_sum, mass_center, mass_center_vel, max_vel; with a frame pos
+        and vel ([n, 3], original order).  Arrays are copies.  An exception
+        in the callback stops the run and is raised from run()."""
+        ep = np.ascontiguousarray(list(energy_periods) or [0], dtype=np.int32)
+        xp = np.ascontiguousarray(list(frame_periods) or [0], dtype=np.int32)
+        if callback is None:
+            self._check(self.lib.pfm_set_records(self.h, 0, _ip(ep), 0, _ip(xp), 0, RECORD_FN(), None))
+            self._rec_fn = None
+            return
+        n, ni, nq = self.n, self.n_inter, self.n_nhc
+
+        def tramp(_user, rp):
+            try:
+                r = rp.contents
+                d = dict(step=r.step, integrator=r.integrator, sim_time=r.sim_time)
+                if r.has_energy:
+                    d.update(pe=r.pe, ke=r.ke, temp=r.temp, force_sum=np.array(r.force_sum[:]),
+                             mass_center=np.array(r.mass_center[:]), mass_center_vel=np.array(r.mass_center_vel[:]),
+                             max_vel=r.max_vel,
+                             e_inter=np.ctypeslib.as_array(r.e_inter, (max(ni, 1),))[:ni].copy(),
+                             e_nhc=np.ctypeslib.as_array(r.e_nhc, (nq,)).copy() if nq else np.zeros(0))
+                if r.has_frame:
+                    d["pos"] = np.ctypeslib.as_array(r.pos, (n, 3)).copy()
+                    d["vel"] = np.ctypeslib.as_array(r.vel, (n, 3)).copy()
+                callback(d)
+                return 0
+            except BaseException as e:  # noqa: BLE001 -- re-raised from run()
+                self._rec_exc = e
+                return 1
+
+        fn = RECORD_FN(tramp)
+        self._check(self.lib.pfm_set_records(self.h, len(energy_periods), _ip(ep), len(frame_periods), _ip(xp),
+                                             int(frame_slots), fn, None))
+        self._rec_fn = fn
 
     def graph_count(self):
         """(blocks captured, blocks launched from graphs)"""

@@ -228,6 +228,97 @@ __global__ void k_sub_mcv(int N, double4* __restrict__ vel, const uint32_t* __re
   vel[i] = v;
 }
 
+// ------------------------------------------------------------- records
+
+// slots of the next record: ctr = {energy records issued, frames issued,
+// current energy slot, current frame slot}
+__global__ void k_rec_slot(int* ctr, int ecap, int xcap, int e, int x) {
+  if (threadIdx.x != 0) return;
+  if (e) { ctr[2] = ctr[0] % ecap; ctr[0] = ctr[0] + 1; }
+  if (x) { ctr[3] = ctr[1] % xcap; ctr[1] = ctr[1] + 1; }
+}
+
+// per-block partials of md()'s out block: rows 0-2 the force sum, 3-5 sum m x,
+// 6-8 sum m v, 9 sum m over the all_atoms group (calculate_force_sum,
+// calculate_mass_center[_velocity], md_general.f90:184-234,277-298); row 10
+// the largest v.v over every atom (find_max_velocity, md_general.f90:366-380)
+__global__ void __launch_bounds__(RED_THREADS) k_out_part(int N, const double4* __restrict__ pos,
+                                                          const double4* __restrict__ vel,
+                                                          const double4* __restrict__ frc,
+                                                          const uint32_t* __restrict__ mask, uint32_t bit,
+                                                          double* __restrict__ part) {
+  __shared__ double sh[RED_THREADS];
+  double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double vmax = -1.0;
+  for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
+    double4 v = vel[i];
+    double v2 = v.x * v.x + v.y * v.y + v.z * v.z;
+    vmax = v2 > vmax ? v2 : vmax;
+    if (mask[i] & bit) {
+      double4 p = pos[i], f = frc[i];
+      a[0] += f.x; a[1] += f.y; a[2] += f.z;
+      a[3] += v.w * p.x; a[4] += v.w * p.y; a[5] += v.w * p.z;
+      a[6] += v.w * v.x; a[7] += v.w * v.y; a[8] += v.w * v.z;
+      a[9] += v.w;
+    }
+  }
+  for (int q = 0; q < 10; ++q) {
+    double s = block_sum(a[q], sh);
+    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = s;
+  }
+  const int t = threadIdx.x;
+  sh[t] = vmax;
+  __syncthreads();
+  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
+    if (t < o) sh[t] = sh[t + o] > sh[t] ? sh[t + o] : sh[t];
+    __syncthreads();
+  }
+  if (t == 0) part[10 * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// One energy record into its slot: ne rows of energy partials reduced as
+// k_reduce_rows does (interaction energies, then the kinetic energy), the
+// out-block rows, and a copy of the chain states.
+__global__ void __launch_bounds__(RED_THREADS) k_rec_energy(int ne, int nb, const double* __restrict__ epart,
+                                                            const double* __restrict__ opart,
+                                                            const double* __restrict__ chains, int nchain_dbl,
+                                                            double* __restrict__ slots, int width,
+                                                            const int* __restrict__ ctr) {
+  __shared__ double sh[RED_THREADS];
+  double* out = slots + (size_t)ctr[2] * width;
+  for (int q = 0; q < ne + 10; ++q) {
+    const double* part = q < ne ? epart + (size_t)q * nb : opart + (size_t)(q - ne) * nb;
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RED_THREADS) a += part[b];
+    double s = block_sum(a, sh);
+    if (threadIdx.x == 0) out[q] = s;
+  }
+  const int t = threadIdx.x;
+  double m = -1.0;
+  for (int b = t; b < nb; b += RED_THREADS) m = opart[10 * nb + b] > m ? opart[10 * nb + b] : m;
+  sh[t] = m;
+  __syncthreads();
+  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
+    if (t < o) sh[t] = sh[t + o] > sh[t] ? sh[t + o] : sh[t];
+    __syncthreads();
+  }
+  if (t == 0) out[ne + 10] = sh[0];
+  for (int j = t; j < nchain_dbl; j += RED_THREADS) out[ne + 11 + j] = chains[j];
+}
+
+// One frame into its slot: positions then velocities, double[n][3] each, in
+// original atom order
+__global__ void k_rec_frame(int N, const double4* __restrict__ pos, const double4* __restrict__ vel,
+                            const int* __restrict__ orig, double* __restrict__ slots, const int* __restrict__ ctr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double* out = slots + (size_t)ctr[3] * 6 * N;
+  const size_t a = 3 * (size_t)orig[i];
+  double4 p = pos[i], v = vel[i];
+  out[a] = p.x; out[a + 1] = p.y; out[a + 2] = p.z;
+  out[3 * (size_t)N + a] = v.x; out[3 * (size_t)N + a + 1] = v.y; out[3 * (size_t)N + a + 2] = v.z;
+}
+
 // Batches: replica r owns internal atoms [row0[r], row0[r+1]) (slot 0 of the
 // row-offset table) and, in group g, rows [row0[g*(nrep+1)+r], ...+1); one
 // block per replica.
@@ -798,6 +889,16 @@ __global__ void k_set_base(DevErr* e, int base) {
 
 static long gcd_l(long a, long b) { return b ? gcd_l(b, a % b) : a; }
 
+// steps that take output records (pfm_set_records)
+static bool rec_on(const pfm_ctx* h) { return h->rec.fn && (!h->rec.eper.empty() || !h->rec.xper.empty()); }
+static bool due(const std::vector<int>& per, int s) {
+  for (int p : per)
+    if (s % p == 0) return true;
+  return false;
+}
+static bool rec_e_at(const pfm_ctx* h, int s) { return rec_on(h) && due(h->rec.eper, s); }
+static bool rec_x_at(const pfm_ctx* h, int s) { return rec_on(h) && due(h->rec.xper, s); }
+
 // length of the repeating schedule of list rebuilds and zero_momentum, or 0
 // when it is longer than 64 steps
 static int graph_block(pfm_ctx* h) {
@@ -833,14 +934,240 @@ static std::vector<uint64_t> graph_key(pfm_ctx* h, int integ, double dt, int blk
   std::vector<uint64_t> k = graph_buffers(h);
   bool zm = false;
   for (int t = s; t < s + blk; ++t) zm = zm || t % h->zm_period == 0;
+  uint64_t em = 0, xm = 0;
+  for (int t = s; t < s + blk; ++t) {
+    em |= (uint64_t)rec_e_at(h, t) << (t - s);
+    xm |= (uint64_t)rec_x_at(h, t) << (t - s);
+  }
   k.insert(k.end(), {(uint64_t)integ, dtb, (uint64_t)blk, (uint64_t)first_writes, (uint64_t)zm,
-                     (uint64_t)h->invert_z, (uint64_t)h->lanes, (uint64_t)h->block});
+                     (uint64_t)h->invert_z, (uint64_t)h->lanes, (uint64_t)h->block, em, xm});
   return k;
 }
 
 static void clear_graphs(pfm_ctx* h) {
   for (auto& e : h->graphs) (void)hipGraphExecDestroy(e.exec);
   h->graphs.clear();
+}
+
+// ------------------------------------------------------------- records
+
+static_assert(sizeof(NhcDev) % sizeof(double) == 0, "chain state copied as doubles");
+static const int REC_ECAP = 256;  // energy slots
+#define REC_OUT 11                // out-block values per energy record
+
+
+static void rec_free(pfm_ctx* h) {
+  auto& R = h->rec;
+  if (R.copy) (void)hipStreamSynchronize(R.copy);
+  if (R.d_e) (void)hipFree(R.d_e);
+  if (R.d_x) (void)hipFree(R.d_x);
+  if (R.d_part) (void)hipFree(R.d_part);
+  if (R.d_ctr) (void)hipFree(R.d_ctr);
+  if (R.h_e) (void)hipHostFree(R.h_e);
+  if (R.h_x) (void)hipHostFree(R.h_x);
+  R.d_e = R.d_x = R.d_part = R.h_e = R.h_x = nullptr;
+  R.d_ctr = nullptr;
+  R.ecap = R.xcap = R.ewidth = 0;
+  for (auto& p : R.q) R.free_ev.push_back(p.done);
+  R.q.clear();
+  R.pending_e = R.pending_x = 0;
+}
+
+// device and pinned host rings for the current interactions and chains
+static int rec_prepare(pfm_ctx* h) {
+  auto& R = h->rec;
+  const int ni = (int)h->inter.size(), nq = (int)h->nhc.size();
+  const int width = ni + 1 + REC_OUT + nq * (int)(sizeof(NhcDev) / sizeof(double));
+  const int want_x = R.xper.empty() ? 0 : R.xslots;
+  if (R.d_ctr && R.ewidth == width && R.xcap == want_x) return PFM_OK;
+  rec_free(h);
+  if (!R.copy) HIPCHK(h, hipStreamCreateWithFlags(&R.copy, hipStreamNonBlocking));
+  if (!R.issued) HIPCHK(h, hipEventCreateWithFlags(&R.issued, hipEventDisableTiming));
+  int rc;
+  R.ewidth = width;
+  R.ecap = REC_ECAP;
+  R.xcap = want_x;
+  if ((rc = dalloc(h, &R.d_e, (size_t)R.ecap * width)) || (rc = dalloc(h, &R.d_part, (size_t)REC_OUT * h->red_blocks)) ||
+      (rc = dalloc(h, &R.d_ctr, 4)))
+    return rc;
+  HIPCHK(h, hipHostMalloc((void**)&R.h_e, sizeof(double) * R.ecap * width, 0));
+  if (R.xcap > 0) {
+    if ((rc = dalloc(h, &R.d_x, (size_t)R.xcap * 6 * h->N))) return rc;
+    HIPCHK(h, hipHostMalloc((void**)&R.h_x, sizeof(double) * (size_t)R.xcap * 6 * h->N, 0));
+  }
+  return PFM_OK;
+}
+
+// launches that take the records of the current step (captured with it)
+static void rec_launch(pfm_ctx* h, bool e, bool x) {
+  auto& R = h->rec;
+  PFM_LAUNCH(h, "record", k_rec_slot, 1, 64, R.d_ctr, R.ecap, R.xcap > 0 ? R.xcap : 1, e ? 1 : 0, x ? 1 : 0);
+  if (e) {
+    const int nb = h->red_blocks, ni = (int)h->inter.size();
+    for (int k = 0; k < ni; ++k) {
+      Inter& it = h->inter[k];
+      PFM_LAUNCH(h, "record", k_sum_part, nb, RED_THREADS, it.e_rows, it.e_row, h->red_part + (size_t)k * nb);
+    }
+    MaskQ Q;
+    Q.nq = 1;
+    Q.bit[0] = 1u << (h->roles[0] - 1);
+    PFM_LAUNCH(h, "record", k_ke_part, nb, RED_THREADS, h->N, h->vel4, h->mask, Q, h->red_part + (size_t)ni * nb);
+    PFM_LAUNCH(h, "record", k_out_part, nb, RED_THREADS, h->N, h->pos4, h->vel4, h->frc4, h->mask,
+               1u << (h->roles[3] - 1), R.d_part);
+    PFM_LAUNCH(h, "record", k_rec_energy, 1, RED_THREADS, ni + 1, nb, h->red_part, R.d_part, h->nhc_state,
+               (int)(h->nhc.size() * sizeof(NhcDev) / sizeof(double)), R.d_e, R.ewidth, R.d_ctr);
+  }
+  if (x) PFM_LAUNCH(h, "record", k_rec_frame, (h->N + 255) / 256, 256, h->N, h->pos4, h->vel4, h->orig, R.d_x, R.d_ctr);
+}
+
+static double nhc_energy(const NhcDev& d) {
+  // calculate_nose_hoover_chain_energy (md_integrators.f90:247-260)
+  double e = d.q[0] / 2 * d.v[0] * d.v[0] + 3.0 * d.L * d.kt * d.x[0];
+  for (int i = 1; i < d.M; ++i) e = e + d.q[i] / 2 * d.v[i] * d.v[i] + d.kt * d.x[i];
+  return e;
+}
+
+// hand the oldest pending record to the callback; wait for its copy when
+// `wait`, else only if it has arrived.  Returns 1 when nothing was handed on.
+static int rec_deliver_one(pfm_ctx* h, bool wait, int* rc) {
+  auto& R = h->rec;
+  *rc = PFM_OK;
+  if (R.q.empty()) return 1;
+  pfm_ctx::Records::Pending P = R.q.front();
+  if (wait) {
+    hipError_t e = hipEventSynchronize(P.done);
+    if (e != hipSuccess) {
+      *rc = fail(h, PFM_E_HIP, std::string("error: HIP ") + hipGetErrorString(e) + " waiting for a record");
+      return 1;
+    }
+  } else {
+    hipError_t e = hipEventQuery(P.done);
+    if (e == hipErrorNotReady) return 1;
+    if (e != hipSuccess) {
+      *rc = fail(h, PFM_E_HIP, std::string("error: HIP ") + hipGetErrorString(e) + " polling a record");
+      return 1;
+    }
+  }
+  R.q.pop_front();
+  R.free_ev.push_back(P.done);
+  pfm_record r;
+  memset(&r, 0, sizeof r);
+  r.step = P.step;
+  r.integrator = P.integ;
+  r.sim_time = P.time;
+  const int ni = (int)h->inter.size(), nq = (int)h->nhc.size();
+  std::vector<double> enhc(h->e_nhc);
+  enhc.resize(nq, 0.0);
+  if (P.eslot >= 0) {
+    --R.pending_e;
+    const double* e = R.h_e + (size_t)P.eslot * R.ewidth;
+    r.has_energy = 1;
+    double pe = 0.0;
+    for (int k = 0; k < ni; ++k) pe += e[k];
+    r.pe = pe;
+    r.ke = e[ni];
+    r.temp = 2 * r.ke / PFM_KB / (3 * h->grp.n[h->roles[0]]);
+    const double* o = e + ni + 1;
+    const double tm = o[9];
+    for (int d = 0; d < 3; ++d) {
+      r.force_sum[d] = o[d];
+      r.mass_center[d] = o[3 + d] / tm;
+      r.mass_center_vel[d] = o[6 + d] / tm;
+    }
+    r.max_vel = std::sqrt(o[10]);
+    if (P.integ == PFM_NVT) {
+      const NhcDev* cs = (const NhcDev*)(e + ni + 1 + REC_OUT);
+      for (int c = 0; c < nq; ++c) {
+        NhcDev d;
+        memcpy(&d, cs + c, sizeof d);
+        enhc[c] = nhc_energy(d);
+      }
+    }
+    r.e_inter = e;
+    r.e_nhc = enhc.data();
+    if (P.integ != PFM_NVMS) {
+      // the record is the handle's latest energy evaluation (pfm_energies)
+      for (int k = 0; k < ni; ++k) h->inter[k].energy = e[k];
+      h->pe = pe;
+      h->ke = r.ke;
+      h->temp = r.temp;
+      h->e_nhc = enhc;
+    }
+  }
+  if (P.xslot >= 0) {
+    --R.pending_x;
+    r.has_frame = 1;
+    r.pos = R.h_x + (size_t)P.xslot * 6 * h->N;
+    r.vel = r.pos + 3 * (size_t)h->N;
+  }
+  int u = R.fn(R.user, &r);
+  if (u != 0) {
+    char buf[96];
+    snprintf(buf, sizeof buf, "error: record callback returned %d at step %d", u, P.step);
+    *rc = fail(h, PFM_E_INVALID, buf);
+    return 1;
+  }
+  return 0;
+}
+
+// free slots for e energy records and x frames: hand on the oldest records
+// (waiting for their copies) until they fit
+static int rec_reserve(pfm_ctx* h, int e, int x) {
+  auto& R = h->rec;
+  int rc;
+  while (R.pending_e + e > R.ecap || R.pending_x + x > R.xcap) {
+    if (rec_deliver_one(h, true, &rc)) return rc ? rc : fail(h, PFM_E_INVALID, "error: record rings too small");
+  }
+  return PFM_OK;
+}
+
+// queue the copies of records just launched on the handle's stream: the copy
+// stream waits for the launches, copies every slot to pinned memory and marks
+// each record's arrival with an event
+struct RecNote {
+  int step;
+  double time;
+  bool e, x;
+};
+static int rec_issue(pfm_ctx* h, int integ, const std::vector<RecNote>& notes) {
+  auto& R = h->rec;
+  if (notes.empty()) return PFM_OK;
+  HIPCHK(h, hipEventRecord(R.issued, h->stream));
+  HIPCHK(h, hipStreamWaitEvent(R.copy, R.issued, 0));
+  for (const RecNote& n : notes) {
+    pfm_ctx::Records::Pending P{n.step, integ, n.time, -1, -1, nullptr};
+    if (n.e) {
+      P.eslot = (int)(R.ne++ % R.ecap);
+      ++R.pending_e;
+      HIPCHK(h, hipMemcpyAsync(R.h_e + (size_t)P.eslot * R.ewidth, R.d_e + (size_t)P.eslot * R.ewidth,
+                               sizeof(double) * R.ewidth, hipMemcpyDeviceToHost, R.copy));
+    }
+    if (n.x) {
+      P.xslot = (int)(R.nx++ % R.xcap);
+      ++R.pending_x;
+      const size_t w = (size_t)6 * h->N;
+      HIPCHK(h, hipMemcpyAsync(R.h_x + P.xslot * w, R.d_x + P.xslot * w, sizeof(double) * w, hipMemcpyDeviceToHost,
+                               R.copy));
+    }
+    if (R.free_ev.empty()) {
+      hipEvent_t ev;
+      HIPCHK(h, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      R.free_ev.push_back(ev);
+    }
+    P.done = R.free_ev.back();
+    R.free_ev.pop_back();
+    HIPCHK(h, hipEventRecord(P.done, R.copy));
+    R.q.push_back(P);
+  }
+  return PFM_OK;
+}
+
+// hand on every record that has arrived (all of them when `wait`)
+static int rec_poll(pfm_ctx* h, bool wait) {
+  int rc = PFM_OK;
+  while (!rec_deliver_one(h, wait, &rc)) {
+  }
+  return rc;
 }
 
 // ---------------------------------------------------------------- C-ABI
@@ -1042,8 +1369,8 @@ int pfm_add_nhc(pfm_ctx* h, int group, double T, int M, double q1) {
   return PFM_OK;
 }
 
-int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end, double ms_de, int* last_step,
-            int* event) {
+static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end, double ms_de,
+                     int* last_step, int* event) {
   if (integ < PFM_NVT || integ > PFM_NVMS) return fail(h, PFM_E_INVALID, "error: unknown integrator");
   if (h->roles[0] == 0) return fail(h, PFM_E_INVALID, "error: pfm_set_roles was not called");
   if (h->nrep > 1 && integ == PFM_NVT && !h->nhc.empty())
@@ -1077,6 +1404,8 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
     int rc;
     *last_step = s;
     bool energy = (s == s1 && energy_at_end) || integ == PFM_NVMS;
+    const bool re = rec_e_at(h, s), rx = rec_x_at(h, s);
+    const bool report = energy || re || rx;  // velocities complete at the end of the step
     if (s != 0) {
       bool conv = std::fabs(h->pe - h->pe_prev) <= ms_de;
       if (h->nrep > 1 && integ == PFM_NVMS) {
@@ -1104,19 +1433,19 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
     h->rep_pe_prev = h->rep_pe;
     if ((rc = update_lists(h, s))) return rc;
     if (s % h->zm_period == 0 && (rc = zero_momentum(h))) return rc;
-    if ((rc = launch_forces(h, energy, first_writes))) return rc;
+    if ((rc = launch_forces(h, energy || re, first_writes))) return rc;
     if (s != 0) {
       if (nvt) {
         // second half step; on steps with nothing to report, the chain update
         // also takes the first half of the next step (k_nhc_chain)
         MaskQ C = chain_masks(h);
-        bool fuse = !energy && s < s1;
+        bool fuse = !report && s < s1;
         ChainArgs A{(NhcDev*)h->nhc_state, h->red_out + 16, dt / 2.0, dt / 4.0, dt / 8.0, fuse ? 1 : 0};
         PFM_LAUNCH(h, "kick2", k_kick2_ke, h->red_blocks, RED_THREADS, N, h->vel4, h->frc4, h->mask, P, C, h->red_part);
         PFM_LAUNCH(h, "nhc", k_nhc_chain, 1, RED_THREADS, C.nq, A, h->red_part, h->red_blocks);
         if (fuse) nhc_pending = true;
         else PFM_LAUNCH(h, "nhc_scale", k_scale, nb, 256, N, h->vel4, h->mask, P);
-      } else if (integ != PFM_NVMS && !energy && s < s1) {
+      } else if (integ != PFM_NVMS && !report && s < s1) {
         // nothing reads the velocities before the next step's first half
         kick_pending = true;
       } else {
@@ -1128,6 +1457,7 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
       if ((rc = energies(h, integ == PFM_NVT))) return rc;
       if ((rc = check_err(h, s))) return rc;
     }
+    if (re || rx) rec_launch(h, re, rx);
     return PFM_OK;
   };
   // Steady-state blocks of `blk` steps (the list and zero-momentum schedule
@@ -1137,9 +1467,24 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
   // of the two re-sort buffers is current, so the cache is keyed by those.
   const int blk = graph_block(h);
   const bool graphs = h->use_graphs && !h->timing && integ != PFM_NVMS && blk > 0;
+  std::vector<RecNote> notes;
   for (int s = s0; s <= s1;) {
     const bool steady = nvt ? nhc_pending : kick_pending;
-    if (graphs && s > 0 && s % blk == 0 && s + blk - 1 < s1 && steady) {
+    bool blocked = graphs && s > 0 && s % blk == 0 && s + blk - 1 < s1 && steady;
+    int ne_b = 0, nx_b = 0;
+    if (blocked && rec_on(h)) {
+      // records inside a block are captured with it; the block must end in
+      // the steady state (no record on its last step) and fit the rings
+      for (int t = s; t < s + blk; ++t) {
+        ne_b += rec_e_at(h, t) ? 1 : 0;
+        nx_b += rec_x_at(h, t) ? 1 : 0;
+      }
+      blocked = !rec_e_at(h, s + blk - 1) && !rec_x_at(h, s + blk - 1) && ne_b <= h->rec.ecap &&
+                nx_b <= h->rec.xcap;
+    }
+    if (blocked) {
+      if (ne_b + nx_b > 0 && (rc = rec_reserve(h, ne_b, nx_b))) return rc;
+      const double t_start = h->sim_time;
       std::vector<uint64_t> key = graph_key(h, integ, dt, blk, first_writes, s);
       pfm_ctx::GraphEntry* g = nullptr;
       for (auto& e : h->graphs)
@@ -1185,22 +1530,83 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
         // the host side of the block: buffers, time, energies carried over
         set_graph_buffers(h, g->after);
         *last_step = s + blk - 1;
-        h->sim_time = h->sim_time + blk * dt;
+        for (int t = 0; t < blk; ++t) h->sim_time = h->sim_time + dt;
         h->pe_prev = h->pe;
         h->rep_pe_prev = h->rep_pe;
+      }
+      if (ne_b + nx_b > 0) {
+        notes.clear();
+        double tt = t_start;
+        for (int t = s; t < s + blk; ++t) {
+          tt = tt + dt;
+          if (rec_e_at(h, t) || rec_x_at(h, t)) notes.push_back(RecNote{t, tt, rec_e_at(h, t), rec_x_at(h, t)});
+        }
+        if ((rc = rec_issue(h, integ, notes)) || (rc = rec_poll(h, false))) return rc;
       }
       s += blk;
       continue;
     }
+    const bool re = rec_e_at(h, s), rx = rec_x_at(h, s);
+    if ((re || rx) && (rc = rec_reserve(h, re ? 1 : 0, rx ? 1 : 0))) return rc;
     int rc2 = body(s);
     if (rc2 == CONVERGED) {
       *event = PFM_EV_CONVERGED;
       return PFM_OK;
     }
     if (rc2) return rc2;
+    if (re || rx) {
+      notes.assign(1, RecNote{s, h->sim_time, re, rx});
+      if ((rc = rec_issue(h, integ, notes)) || (rc = rec_poll(h, false))) return rc;
+    }
     ++s;
   }
   return check_err(h, s1);
+}
+
+int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end, double ms_de, int* last_step,
+            int* event) {
+  auto& R = h->rec;
+  int rc;
+  if (rec_on(h)) {
+    if (h->nrep > 1) return fail(h, PFM_E_INVALID, "error: records are taken for single systems only");
+    if ((rc = rec_prepare(h))) return rc;
+    HIPCHK(h, hipMemsetAsync(R.d_ctr, 0, 4 * sizeof(int), h->stream));
+    R.ne = R.nx = 0;
+  }
+  rc = run_steps(h, integ, dt, s0, s1, energy_at_end, ms_de, last_step, event);
+  if (rc == PFM_OK && !R.q.empty()) rc = rec_poll(h, true);
+  if (rc != PFM_OK && !R.q.empty()) {
+    // records of a failed run are dropped
+    if (R.copy) (void)hipStreamSynchronize(R.copy);
+    for (auto& p : R.q) R.free_ev.push_back(p.done);
+    R.q.clear();
+    R.pending_e = R.pending_x = 0;
+  }
+  return rc;
+}
+
+int pfm_set_records(pfm_ctx* h, int n_e, const int* e_periods, int n_x, const int* x_periods, int frame_slots,
+                    pfm_record_fn fn, void* user) {
+  auto& R = h->rec;
+  if (!R.q.empty()) return fail(h, PFM_E_INVALID, "error: records still pending");
+  if (n_e < 0 || n_x < 0 || (n_e > 0 && !e_periods) || (n_x > 0 && !x_periods))
+    return fail(h, PFM_E_INVALID, "error: bad record periods");
+  std::vector<int> ep, xp;
+  for (int k = 0; k < n_e; ++k)
+    if (e_periods[k] > 0) ep.push_back(e_periods[k]);
+  for (int k = 0; k < n_x; ++k)
+    if (x_periods[k] > 0) xp.push_back(x_periods[k]);
+  if (!xp.empty() && frame_slots < 1) return fail(h, PFM_E_INVALID, "error: frames need at least one slot");
+  if (fn && (!ep.empty() || !xp.empty()) && h->nrep > 1)
+    return fail(h, PFM_E_INVALID, "error: records are taken for single systems only");
+  // captured blocks hold the records of their steps
+  clear_graphs(h);
+  R.eper = ep;
+  R.xper = xp;
+  R.xslots = xp.empty() ? 0 : frame_slots;
+  R.fn = fn;
+  R.user = user;
+  return PFM_OK;
 }
 
 int pfm_replica_energies(pfm_ctx* h, double* e_inter, double* pe, double* ke, double* temp) {
@@ -1411,6 +1817,10 @@ int pfm_destroy(pfm_ctx* h) {
   if (!h) return PFM_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   clear_graphs(h);
+  rec_free(h);
+  if (h->rec.copy) (void)hipStreamDestroy(h->rec.copy);
+  if (h->rec.issued) (void)hipEventDestroy(h->rec.issued);
+  for (auto e : h->rec.free_ev) (void)hipEventDestroy(e);
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(h->pos4); F(h->vel4); F(h->frc4); F(h->pos4b); F(h->vel4b); F(h->frc4b); F(h->mask); F(h->maskb);
   F(h->orig); F(h->origb); F(h->perm); F(h->atom_rank); F(h->scan_tmp); F(h->eb); F(h->qpos); F(h->rdev); F(h->rep); F(h->repb); F(h->cell_rep); F(h->d_rep_row0); F(h->d_rep_out);

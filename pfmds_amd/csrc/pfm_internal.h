@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -427,6 +428,32 @@ struct pfm_ctx {
   };
   std::vector<GraphEntry> graphs;
   long graph_captures = 0, graph_launches = 0;
+  // output records (pfm_set_records)
+  struct Records {
+    std::vector<int> eper, xper;  // energy and frame periods
+    pfm_record_fn fn = nullptr;
+    void* user = nullptr;
+    int xslots = 0;          // frame slots asked for
+    int ecap = 0, xcap = 0;  // device/host slots of each ring
+    int ewidth = 0;          // doubles per energy slot
+    double *d_e = nullptr, *h_e = nullptr;  // [ecap][ewidth]
+    double *d_x = nullptr, *h_x = nullptr;  // [xcap][6 n]: positions then velocities
+    double* d_part = nullptr;               // out-block partial sums [11][red_blocks]
+    int* d_ctr = nullptr;  // records issued (energy, frame) and the current slots
+    long ne = 0, nx = 0;   // host mirror of the issued counts
+    hipStream_t copy = nullptr;
+    struct Pending {
+      int step, integ;
+      double time;
+      int eslot, xslot;  // -1 when absent
+      hipEvent_t done;
+    };
+    std::deque<Pending> q;
+    std::vector<hipEvent_t> free_ev;
+    hipEvent_t issued = nullptr;
+    int pending_e = 0, pending_x = 0;
+    int fn_rc = 0;
+  } rec;
   // timing
   bool timing = false;
   std::vector<KTimer> timers;

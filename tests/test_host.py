@@ -33,11 +33,11 @@ def test_oracle_mirrors_the_abi():
     lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liborc.so"))
     for s in header_symbols():
         # GPU-only extras: profiling, launch shape, replica batches (pinned
-        # against single-system GPU runs, not the checker) and the learned
-        # surrogate (pinned against tests/sur_ref.py)
+        # against single-system GPU runs, not the checker), output records
+        # (pinned against GPU runs stopped at every recorded step) and the
+        # learned surrogate (pinned against tests/sur_ref.py)
         if s in ("pfm_set_timing", "pfm_kernel_time", "pfm_set_tuning", "pfm_set_graphs", "pfm_graph_count",
-                 "pfm_create_batch",
-                 "pfm_replica_energies") \
+                 "pfm_create_batch", "pfm_replica_energies", "pfm_set_records") \
                 or s.startswith("pfm_sur_"):
             continue
         assert hasattr(lib, s.replace("pfm_", "orc_")), s
