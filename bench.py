@@ -126,6 +126,64 @@ def graphene_cu(steps, warmup):
     return out
 
 
+def output_path(steps=2000, log_every=10, traj_every=100, warmup=20):
+    """md() with its outputs on graphene on Cu(111) (94,560 atoms, nvt, mixed
+    precision): an energy record every `log_every` steps and a frame of every
+    atom every `traj_every` steps, taken (a) by stopping the run at each
+    output step (pfm_run to the step, pfm_energies, pfm_get_state: the
+    reference's synchronous pattern) and (b) with output records
+    (pfm_set_records), beside the same steps with no output.  Text
+    formatting is left out of both; the callback copies what it is handed."""
+    s = configs.graphene_on_cu()
+    dt = s.integrators[-1].dt
+    ctx = gpu_ctx(s, capi.MIXED)
+    ctx.run("nvt", dt, 0, warmup, True)
+    shape, step = tuned(ctx, "nvt", dt, capi.MIXED, warmup + 1)
+    step = -(-step // traj_every) * traj_every + 1
+    out = {"atoms": s.n, "steps": steps, "log_every": log_every, "traj_every": traj_every, "launch_shape": shape}
+    t0 = time.perf_counter()
+    ctx.run("nvt", dt, step, step + steps - 1, True)
+    out["no_output_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / steps
+    step += steps
+    t0 = time.perf_counter()
+    end = step + steps - 1
+    while step <= end:
+        t = min(end, -(-step // log_every) * log_every)
+        ctx.run("nvt", dt, step, t, True)
+        ctx.energies()
+        if t % traj_every == 0:
+            ctx.state()
+        step = t + 1
+    out["stop_at_outputs_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / steps
+    got = []
+
+    def keep(r):
+        got.append(r["step"])
+
+    ctx.set_records((log_every,), (traj_every,), 4, keep)
+    t0 = time.perf_counter()
+    ctx.run("nvt", dt, step, step + steps - 1, True)
+    out["records_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / steps
+    step += steps
+    out["records_delivered"] = len(got)
+    # the same with captured blocks: the host issues one launch per block,
+    # so the callback's host work overlaps the device's
+    ctx.set_graphs(True)
+    t0 = time.perf_counter()
+    ctx.run("nvt", dt, step, step + steps - 1, True)
+    out["records_graphs_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / steps
+    step += steps
+    ctx.set_records()
+    t0 = time.perf_counter()
+    ctx.run("nvt", dt, step, step + steps - 1, True)
+    out["no_output_graphs_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.close()
+    print(f"# output path: none {out['no_output_ms_per_step']:.4f}, stop {out['stop_at_outputs_ms_per_step']:.4f}, "
+          f"records {out['records_ms_per_step']:.4f}, records+graphs {out['records_graphs_ms_per_step']:.4f}, "
+          f"none+graphs {out['no_output_graphs_ms_per_step']:.4f} ms/step", file=sys.stderr, flush=True)
+    return out
+
+
 def surrogate_c4(train_iters=20000, frames_full=125, frames_sub=875, steps=10000, judge_every=100,
                  threshold=0.05, mixed_ms=None, fp64_ms=None, log=None):
     """Learned surrogate on graphene/Cu (94,560 atoms): frames from plain nvt
@@ -293,6 +351,8 @@ def main():
     ap.add_argument("--c4-iters", type=int, default=20000, help="surrogate training iterations")
     ap.add_argument("--c4-small", action="store_true",
                     help="surrogate with 10 + 40 frames and a 2,000-step run (profiling)")
+    ap.add_argument("--output-path", action="store_true",
+                    help="time only the run with outputs: stopping at each output step vs output records")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step path has no multi-GPU decomposition "
@@ -301,6 +361,9 @@ def main():
         sys.exit(2)
     if args.c2_only:
         print(json.dumps({"graphene_cu_nvt": graphene_cu(args.steps, args.warmup)}))
+        return
+    if args.output_path:
+        print(json.dumps({"output_path": output_path(args.steps, warmup=args.warmup)}))
         return
     if args.c3_only:
         print(json.dumps({"replica_batch": replica_batch(args.steps, args.steps, check_steps=10, n_check=2)}))

@@ -185,10 +185,12 @@ int pfm_graph_count(pfm_ctx* h, long* captured, long* launched);
  *     md_general.f90:366-380);
  *   frame at every step some frame period divides: positions and velocities
  *     of every atom, double[n][3] each in original order.
- * Records are written into device slots (frame_slots frames), copied to
- * pinned host memory on a second stream, and passed to fn in step order
- * from inside pfm_run while later steps run; the pointers in the record are
- * valid during the call only.  The trajectory is identical to stopping at
+ * The record kernels run in the step's launch sequence (inside captured
+ * blocks too) and write each record into a slot of pinned host memory
+ * (256 energy slots, frame_slots frames); pfm_run passes the records to fn
+ * in step order as their slots complete, while later steps run, and waits
+ * for the oldest when a ring is full.  The pointers in the record are valid
+ * during the call only.  The trajectory is identical to stopping at
  * every recorded step.  Every record of a pfm_run call is delivered before it
  * returns; a nonzero return from fn ends the run with PFM_E_INVALID.
  * Periods <= 0 are ignored; no periods, or fn NULL, turns records off.

@@ -306,17 +306,24 @@ __global__ void __launch_bounds__(RED_THREADS) k_rec_energy(int ne, int nb, cons
   for (int j = t; j < nchain_dbl; j += RED_THREADS) out[ne + 11 + j] = chains[j];
 }
 
-// One frame into its slot: positions then velocities, double[n][3] each, in
-// original atom order
+// One frame in original atom order into device staging: positions then
+// velocities, double[n][3] each
 __global__ void k_rec_frame(int N, const double4* __restrict__ pos, const double4* __restrict__ vel,
-                            const int* __restrict__ orig, double* __restrict__ slots, const int* __restrict__ ctr) {
+                            const int* __restrict__ orig, double* __restrict__ stage) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  double* out = slots + (size_t)ctr[3] * 6 * N;
   const size_t a = 3 * (size_t)orig[i];
   double4 p = pos[i], v = vel[i];
-  out[a] = p.x; out[a + 1] = p.y; out[a + 2] = p.z;
-  out[3 * (size_t)N + a] = v.x; out[3 * (size_t)N + a + 1] = v.y; out[3 * (size_t)N + a + 2] = v.z;
+  stage[a] = p.x; stage[a + 1] = p.y; stage[a + 2] = p.z;
+  stage[3 * (size_t)N + a] = v.x; stage[3 * (size_t)N + a + 1] = v.y; stage[3 * (size_t)N + a + 2] = v.z;
+}
+
+// the staged frame into its pinned host slot, in order, 16 bytes per store
+__global__ void k_rec_frame_out(size_t n2, const double2* __restrict__ stage, double* __restrict__ slots,
+                                const int* __restrict__ ctr) {
+  double2* out = (double2*)(slots + (size_t)ctr[3] * 2 * n2);
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n2; k += (size_t)gridDim.x * blockDim.x)
+    out[k] = stage[k];
 }
 
 // Batches: replica r owns internal atoms [row0[r], row0[r+1]) (slot 0 of the
@@ -889,6 +896,10 @@ __global__ void k_set_base(DevErr* e, int base) {
 
 static long gcd_l(long a, long b) { return b ? gcd_l(b, a % b) : a; }
 
+// captured blocks kept per handle: both re-sort buffer sets times the block
+// variants a schedule of zero-momentum and output steps produces
+static const size_t GRAPH_CACHE = 16;
+
 // steps that take output records (pfm_set_records)
 static bool rec_on(const pfm_ctx* h) { return h->rec.fn && (!h->rec.eper.empty() || !h->rec.xper.empty()); }
 static bool due(const std::vector<int>& per, int s) {
@@ -958,14 +969,14 @@ static const int REC_ECAP = 256;  // energy slots
 
 static void rec_free(pfm_ctx* h) {
   auto& R = h->rec;
-  if (R.copy) (void)hipStreamSynchronize(R.copy);
-  if (R.d_e) (void)hipFree(R.d_e);
+  // record kernels may still write the pinned slots
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (R.d_x) (void)hipFree(R.d_x);
   if (R.d_part) (void)hipFree(R.d_part);
   if (R.d_ctr) (void)hipFree(R.d_ctr);
   if (R.h_e) (void)hipHostFree(R.h_e);
   if (R.h_x) (void)hipHostFree(R.h_x);
-  R.d_e = R.d_x = R.d_part = R.h_e = R.h_x = nullptr;
+  R.d_x = R.d_part = R.h_e = R.h_x = nullptr;
   R.d_ctr = nullptr;
   R.ecap = R.xcap = R.ewidth = 0;
   for (auto& p : R.q) R.free_ev.push_back(p.done);
@@ -981,19 +992,16 @@ static int rec_prepare(pfm_ctx* h) {
   const int want_x = R.xper.empty() ? 0 : R.xslots;
   if (R.d_ctr && R.ewidth == width && R.xcap == want_x) return PFM_OK;
   rec_free(h);
-  if (!R.copy) HIPCHK(h, hipStreamCreateWithFlags(&R.copy, hipStreamNonBlocking));
-  if (!R.issued) HIPCHK(h, hipEventCreateWithFlags(&R.issued, hipEventDisableTiming));
   int rc;
   R.ewidth = width;
   R.ecap = REC_ECAP;
   R.xcap = want_x;
-  if ((rc = dalloc(h, &R.d_e, (size_t)R.ecap * width)) || (rc = dalloc(h, &R.d_part, (size_t)REC_OUT * h->red_blocks)) ||
-      (rc = dalloc(h, &R.d_ctr, 4)))
-    return rc;
-  HIPCHK(h, hipHostMalloc((void**)&R.h_e, sizeof(double) * R.ecap * width, 0));
+  if ((rc = dalloc(h, &R.d_part, (size_t)REC_OUT * h->red_blocks)) || (rc = dalloc(h, &R.d_ctr, 4))) return rc;
+  // the record kernels write the slots in pinned host memory themselves
+  HIPCHK(h, hipHostMalloc((void**)&R.h_e, sizeof(double) * R.ecap * width, hipHostMallocCoherent));
   if (R.xcap > 0) {
-    if ((rc = dalloc(h, &R.d_x, (size_t)R.xcap * 6 * h->N))) return rc;
-    HIPCHK(h, hipHostMalloc((void**)&R.h_x, sizeof(double) * (size_t)R.xcap * 6 * h->N, 0));
+    if ((rc = dalloc(h, &R.d_x, (size_t)6 * h->N))) return rc;
+    HIPCHK(h, hipHostMalloc((void**)&R.h_x, sizeof(double) * (size_t)R.xcap * 6 * h->N, hipHostMallocCoherent));
   }
   return PFM_OK;
 }
@@ -1015,9 +1023,14 @@ static void rec_launch(pfm_ctx* h, bool e, bool x) {
     PFM_LAUNCH(h, "record", k_out_part, nb, RED_THREADS, h->N, h->pos4, h->vel4, h->frc4, h->mask,
                1u << (h->roles[3] - 1), R.d_part);
     PFM_LAUNCH(h, "record", k_rec_energy, 1, RED_THREADS, ni + 1, nb, h->red_part, R.d_part, h->nhc_state,
-               (int)(h->nhc.size() * sizeof(NhcDev) / sizeof(double)), R.d_e, R.ewidth, R.d_ctr);
+               (int)(h->nhc.size() * sizeof(NhcDev) / sizeof(double)), R.h_e, R.ewidth, R.d_ctr);
   }
-  if (x) PFM_LAUNCH(h, "record", k_rec_frame, (h->N + 255) / 256, 256, h->N, h->pos4, h->vel4, h->orig, R.d_x, R.d_ctr);
+  if (x) {
+    const size_t n2 = (size_t)3 * h->N;
+    PFM_LAUNCH(h, "record", k_rec_frame, (h->N + 255) / 256, 256, h->N, h->pos4, h->vel4, h->orig, R.d_x);
+    PFM_LAUNCH(h, "record", k_rec_frame_out, (int)std::min<size_t>(1024, (n2 + 255) / 256), 256, n2,
+               (const double2*)R.d_x, R.h_x, R.d_ctr);
+  }
 }
 
 static double nhc_energy(const NhcDev& d) {
@@ -1027,8 +1040,8 @@ static double nhc_energy(const NhcDev& d) {
   return e;
 }
 
-// hand the oldest pending record to the callback; wait for its copy when
-// `wait`, else only if it has arrived.  Returns 1 when nothing was handed on.
+// hand the oldest pending record to the callback; wait for its slot when
+// `wait`, else only if it is written.  Returns 1 when nothing was handed on.
 static int rec_deliver_one(pfm_ctx* h, bool wait, int* rc) {
   auto& R = h->rec;
   *rc = PFM_OK;
@@ -1111,7 +1124,7 @@ static int rec_deliver_one(pfm_ctx* h, bool wait, int* rc) {
 }
 
 // free slots for e energy records and x frames: hand on the oldest records
-// (waiting for their copies) until they fit
+// (waiting for their slots) until they fit
 static int rec_reserve(pfm_ctx* h, int e, int x) {
   auto& R = h->rec;
   int rc;
@@ -1121,9 +1134,8 @@ static int rec_reserve(pfm_ctx* h, int e, int x) {
   return PFM_OK;
 }
 
-// queue the copies of records just launched on the handle's stream: the copy
-// stream waits for the launches, copies every slot to pinned memory and marks
-// each record's arrival with an event
+// note the records whose launches were just queued on the handle's stream:
+// an event after them marks when their pinned slots are written
 struct RecNote {
   int step;
   double time;
@@ -1132,22 +1144,15 @@ struct RecNote {
 static int rec_issue(pfm_ctx* h, int integ, const std::vector<RecNote>& notes) {
   auto& R = h->rec;
   if (notes.empty()) return PFM_OK;
-  HIPCHK(h, hipEventRecord(R.issued, h->stream));
-  HIPCHK(h, hipStreamWaitEvent(R.copy, R.issued, 0));
   for (const RecNote& n : notes) {
     pfm_ctx::Records::Pending P{n.step, integ, n.time, -1, -1, nullptr};
     if (n.e) {
       P.eslot = (int)(R.ne++ % R.ecap);
       ++R.pending_e;
-      HIPCHK(h, hipMemcpyAsync(R.h_e + (size_t)P.eslot * R.ewidth, R.d_e + (size_t)P.eslot * R.ewidth,
-                               sizeof(double) * R.ewidth, hipMemcpyDeviceToHost, R.copy));
     }
     if (n.x) {
       P.xslot = (int)(R.nx++ % R.xcap);
       ++R.pending_x;
-      const size_t w = (size_t)6 * h->N;
-      HIPCHK(h, hipMemcpyAsync(R.h_x + P.xslot * w, R.d_x + P.xslot * w, sizeof(double) * w, hipMemcpyDeviceToHost,
-                               R.copy));
     }
     if (R.free_ev.empty()) {
       hipEvent_t ev;
@@ -1156,7 +1161,7 @@ static int rec_issue(pfm_ctx* h, int integ, const std::vector<RecNote>& notes) {
     }
     P.done = R.free_ev.back();
     R.free_ev.pop_back();
-    HIPCHK(h, hipEventRecord(P.done, R.copy));
+    HIPCHK(h, hipEventRecord(P.done, h->stream));
     R.q.push_back(P);
   }
   return PFM_OK;
@@ -1511,10 +1516,16 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
         e.after = graph_buffers(h);
         HIPCHK(h, hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0));
         (void)hipGraphDestroy(graph);
-        if (h->graphs.size() >= 4) {
-          (void)hipGraphExecDestroy(h->graphs.front().exec);
-          h->graphs.erase(h->graphs.begin());
+        if (h->graphs.size() >= GRAPH_CACHE) {
+          // least recently launched block goes
+          auto old = std::min_element(h->graphs.begin(), h->graphs.end(),
+                                      [](const pfm_ctx::GraphEntry& a, const pfm_ctx::GraphEntry& b) {
+                                        return a.used < b.used;
+                                      });
+          (void)hipGraphExecDestroy(old->exec);
+          h->graphs.erase(old);
         }
+        e.used = ++h->graph_clock;
         h->graphs.push_back(e);
         ++h->graph_captures;
         ++h->graph_launches;
@@ -1523,6 +1534,7 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
         HIPCHK(h, hipGraphLaunch(h->graphs.back().exec, h->stream));
         PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, 0);
       } else {
+        g->used = ++h->graph_clock;
         PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, s);
         HIPCHK(h, hipGraphLaunch(g->exec, h->stream));
         PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, 0);
@@ -1577,7 +1589,7 @@ int pfm_run(pfm_ctx* h, int integ, double dt, int s0, int s1, int energy_at_end,
   if (rc == PFM_OK && !R.q.empty()) rc = rec_poll(h, true);
   if (rc != PFM_OK && !R.q.empty()) {
     // records of a failed run are dropped
-    if (R.copy) (void)hipStreamSynchronize(R.copy);
+    (void)hipStreamSynchronize(h->stream);
     for (auto& p : R.q) R.free_ev.push_back(p.done);
     R.q.clear();
     R.pending_e = R.pending_x = 0;
@@ -1818,8 +1830,6 @@ int pfm_destroy(pfm_ctx* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   clear_graphs(h);
   rec_free(h);
-  if (h->rec.copy) (void)hipStreamDestroy(h->rec.copy);
-  if (h->rec.issued) (void)hipEventDestroy(h->rec.issued);
   for (auto e : h->rec.free_ev) (void)hipEventDestroy(e);
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(h->pos4); F(h->vel4); F(h->frc4); F(h->pos4b); F(h->vel4b); F(h->frc4b); F(h->mask); F(h->maskb);

@@ -425,9 +425,10 @@ struct pfm_ctx {
     std::vector<uint64_t> key;    // configuration and buffer identities at the block start
     std::vector<uint64_t> after;  // buffer identities after the block
     hipGraphExec_t exec;
+    long used = 0;  // graph_clock at the last launch
   };
   std::vector<GraphEntry> graphs;
-  long graph_captures = 0, graph_launches = 0;
+  long graph_captures = 0, graph_launches = 0, graph_clock = 0;
   // output records (pfm_set_records)
   struct Records {
     std::vector<int> eper, xper;  // energy and frame periods
@@ -436,12 +437,12 @@ struct pfm_ctx {
     int xslots = 0;          // frame slots asked for
     int ecap = 0, xcap = 0;  // device/host slots of each ring
     int ewidth = 0;          // doubles per energy slot
-    double *d_e = nullptr, *h_e = nullptr;  // [ecap][ewidth]
-    double *d_x = nullptr, *h_x = nullptr;  // [xcap][6 n]: positions then velocities
+    double* h_e = nullptr;                  // pinned [ecap][ewidth]
+    double* h_x = nullptr;                  // pinned [xcap][6 n]: positions then velocities
+    double* d_x = nullptr;                  // device staging of one frame [6 n]
     double* d_part = nullptr;               // out-block partial sums [11][red_blocks]
     int* d_ctr = nullptr;  // records issued (energy, frame) and the current slots
     long ne = 0, nx = 0;   // host mirror of the issued counts
-    hipStream_t copy = nullptr;
     struct Pending {
       int step, integ;
       double time;
@@ -450,7 +451,6 @@ struct pfm_ctx {
     };
     std::deque<Pending> q;
     std::vector<hipEvent_t> free_ev;
-    hipEvent_t issued = nullptr;
     int pending_e = 0, pending_x = 0;
     int fn_rc = 0;
   } rec;

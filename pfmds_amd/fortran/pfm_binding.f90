@@ -12,6 +12,15 @@ integer(c_int), parameter :: PFM_NVT = 1, PFM_NVE = 2, PFM_NVMS = 3
 integer(c_int), parameter :: PFM_FP64 = 0, PFM_MIXED = 1
 integer(c_int), parameter :: PFM_EV_DONE = 0, PFM_EV_CONVERGED = 1
 
+!> struct pfm_record: one output record handed to the pfm_set_records callback
+type, bind(C) :: pfm_record
+  integer(c_int) :: step, integrator, has_energy, has_frame
+  real(c_double) :: sim_time, pe, ke, temp
+  real(c_double) :: force_sum(3), mass_center(3), mass_center_vel(3)
+  real(c_double) :: max_vel
+  type(c_ptr) :: e_inter, e_nhc, pos, vel
+end type pfm_record
+
 interface
   integer(c_int) function pfm_create(h, n, box, pos, vel, mass, precision) bind(C, name='pfm_create')
     import :: c_ptr, c_int, c_double
@@ -68,6 +77,14 @@ interface
     type(c_ptr), value :: h
     character(kind=c_char) :: buf(*)
     integer(c_int), value :: length
+  end function
+  integer(c_int) function pfm_set_records(h, n_e, e_periods, n_x, x_periods, frame_slots, fn, user) &
+      bind(C, name='pfm_set_records')
+    import :: c_ptr, c_int, c_funptr
+    type(c_ptr), value :: h, user
+    integer(c_int), value :: n_e, n_x, frame_slots
+    integer(c_int) :: e_periods(*), x_periods(*)
+    type(c_funptr), value :: fn
   end function
   integer(c_int) function pfm_destroy(h) bind(C, name='pfm_destroy')
     import :: c_ptr, c_int

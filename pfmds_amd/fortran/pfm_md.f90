@@ -5,8 +5,10 @@
 !> the extended-xyz input, the integrator schedule with its switching rule and
 !> nvms exit (md_simulation.f90:114-147), and the log / out / snapshot /
 !> trajectory / final-xyz outputs with their formats
-!> (md_simulation.f90:188-271, md_read_write.f90:65-107).  Everything between
-!> two output events runs on the GPU inside one pfm_run call.
+!> (md_simulation.f90:188-271, md_read_write.f90:65-107).  Each integrator's
+!> steps run on the GPU in one pfm_run call; the values the outputs need are
+!> taken on the device at their steps (pfm_set_records) and written by
+!> md_record while later steps run.
 module pfm_md
 use iso_c_binding
 use pfm_binding
@@ -20,7 +22,84 @@ type group_def
   integer, allocatable :: idx(:)
 end type group_def
 
+!> What md_record needs to write md()'s outputs
+type md_writer
+  integer :: out_id, log_id, out_period, period_traj, traj_g, all_atoms, xyz_moving, ninter, nnhc, n
+  integer :: ilog, isnap
+  character(len=128) :: prefix
+  character(len=32) :: iname, fmt_int, fmt_nhc
+  real(8) :: box(3), wall0, wall_out
+  type(group_def), allocatable :: groups(:)
+  real(8), allocatable :: mass(:), e_inter(:), e_nhc(:)
+  character(len=32), allocatable :: types(:)
+end type md_writer
+type(md_writer), save :: wr
+
 contains
+
+!> One output record (pfm_set_records): the log line, the out block, the
+!> snapshot and the trajectory frame due at rec%step, in md()'s formats
+!> (md_simulation.f90:188-229, md_read_write.f90:65-107).
+integer(c_int) function md_record(user, rec) bind(C)
+  type(c_ptr), value :: user
+  type(pfm_record), intent(in) :: rec
+  real(8), pointer :: p(:), pos(:,:), vel(:,:)
+  real(8) :: e_nhc_sum, total, conserved, fs, mcv, w1, w2
+  character(len=128) :: fname
+  integer :: t
+  call system_clock_seconds(w1)
+  t = rec%step
+  if (rec%has_energy /= 0) then
+    if (wr%ninter > 0) then
+      call c_f_pointer(rec%e_inter, p, [wr%ninter])
+      wr%e_inter(1:wr%ninter) = p
+    endif
+    e_nhc_sum = 0.d0
+    if (wr%nnhc > 0) then
+      call c_f_pointer(rec%e_nhc, p, [wr%nnhc])
+      wr%e_nhc(1:wr%nnhc) = p
+      e_nhc_sum = sum(wr%e_nhc(1:wr%nnhc))
+    endif
+    total = rec%pe + rec%ke
+    conserved = total + e_nhc_sum
+    if (mod(t, wr%ilog) == 0) then
+      write(wr%log_id,'(A6,i9,7f24.6)',advance='no') trim(wr%iname), t, rec%sim_time, conserved, e_nhc_sum, &
+          total, rec%pe, rec%ke, rec%temp
+      write(wr%log_id,wr%fmt_int,advance='no') wr%e_inter(1:wr%ninter)
+      write(wr%log_id,wr%fmt_nhc) wr%e_nhc(1:wr%nnhc)
+    endif
+    if (mod(t, wr%out_period) == 0) then
+      fs = sqrt(sum(rec%force_sum**2))
+      mcv = sqrt(sum(rec%mass_center_vel**2))
+      write(wr%out_id,'(A,f10.2)') 'exe time (s) = ', w1 - wr%wall0
+      write(wr%out_id,'(A,i12,A,A6)') 'step = ', t, ' integrator = ', trim(wr%iname)
+      write(wr%out_id,'(A,es21.9)') 'conserved energy (eV) = ', conserved
+      write(wr%out_id,'(A,es16.6)') 'forces sum (eV/A) = ', fs
+      write(wr%out_id,'(A,es16.6)') 'c.o.m. velocity (A/fs) = ', mcv
+      write(wr%out_id,'(A,3f12.6)') 'c.o.m. position (A) = ', rec%mass_center
+      write(wr%out_id,'(A,f12.6)') 'maximum velocity (A/fs) = ', rec%max_vel
+      if (trim(wr%iname) /= 'nvms' .and. fs > 1.d-14) write(wr%out_id,*) 'WARNING: forces sum is too big'
+      if (trim(wr%iname) == 'nvt' .and. wr%xyz_moving == wr%all_atoms .and. mcv > 1.d-14) &
+          write(wr%out_id,*) 'WARNING: center of mass velocity is too big'
+      write(wr%out_id,*)
+    endif
+  endif
+  if (rec%has_frame /= 0) then
+    call c_f_pointer(rec%pos, pos, [3, wr%n])
+    call c_f_pointer(rec%vel, vel, [3, wr%n])
+    if (mod(t, wr%isnap) == 0 .and. t /= 0) then
+      write(fname,'(A,A,i6.6,A)') trim(wr%prefix), 'snapshot_', t, '.xyz'
+      call write_snapshot(trim(fname), wr%groups(wr%all_atoms), wr%box, pos, vel, wr%mass, wr%types)
+    endif
+    if (mod(t, wr%period_traj) == 0) then
+      write(fname,'(A,A,i2.2,A)') trim(wr%prefix), 'traj_', wr%traj_g, '.xyz'
+      call append_trajectory(trim(fname), wr%groups(wr%traj_g), wr%box, pos, vel, wr%mass, wr%types, t)
+    endif
+  endif
+  call system_clock_seconds(w2)
+  wr%wall_out = wr%wall_out + (w2 - w1)
+  md_record = 0
+end function md_record
 
 integer function next_multiple(step, period)
   integer :: step, period
@@ -208,16 +287,16 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
   type(c_ptr) :: h
   integer(c_int) :: rc, last, ev
   integer :: fid, log_id, i, j, k, n, ntypes, ngroups, nint, nnhc, ninter, nchange, md_step, md_step_limit
-  integer :: zmp, all_moving, xyz_moving, z_moving, all_atoms, traj_g, period_traj, idx_int, t, cend, np, nl_n
+  integer :: zmp, all_moving, xyz_moving, z_moving, all_atoms, traj_g, period_traj, idx_int, cend, np, nl_n
   integer :: nhc_g, nhc_m, kind, integ_kind
   integer, allocatable :: ilen(:), isnap(:), ilog(:)
   real(8), allocatable :: idt(:)
   character(len=32), allocatable :: iname(:), names(:,:), types(:)
   real(8), allocatable :: pos(:,:), vel(:,:), frc(:,:), mass(:), e_inter(:), e_nhc(:)
-  real(8) :: box(3), ms_de, t0, nhc_t, nhc_q1, p(24), rc_(3), pe, ke, temp, sim_time, conserved, total, e_nhc_sum
-  real(8) :: fs(3), mc(3), mcv(3), tm, vmax, wall0, wall_md, wall_out, wtmp, w2
-  integer(c_int) :: gn(6), nmax(3), per(3)
-  logical :: newv, invert, energy, with_nhc_energy
+  real(8) :: box(3), ms_de, t0, nhc_t, nhc_q1, p(24), rc_(3), pe, ke, temp, sim_time, total
+  real(8) :: wall0, wall_md, wall_out, wtmp
+  integer(c_int) :: gn(6), nmax(3), per(3), eper(2), xper(2)
+  logical :: newv, invert
   character(len=128) :: str, logfilename, init_xyz, pfile, fname
   character(len=32) :: iname_cur, fmt_int, fmt_nhc, inter_name
   type(group_def), allocatable :: groups(:)
@@ -330,13 +409,34 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
   write(out_id,'(A24,i6,A)') 'RUNNING ON ', num_of_omp_treads, ' OPENMP THREADS'
   write(out_id,*)
 
+  wr%out_id = out_id
+  wr%log_id = log_id
+  wr%out_period = out_period
+  wr%period_traj = period_traj
+  wr%traj_g = traj_g
+  wr%all_atoms = all_atoms
+  wr%xyz_moving = xyz_moving
+  wr%ninter = ninter
+  wr%nnhc = nnhc
+  wr%n = n
+  wr%prefix = output_prefix
+  wr%fmt_int = fmt_int
+  wr%fmt_nhc = fmt_nhc
+  wr%box = box
+  wr%wall0 = wall0
+  wr%wall_out = 0.d0
+  wr%groups = groups
+  wr%mass = mass
+  wr%types = types
+  allocate(wr%e_inter(max(ninter,1)), wr%e_nhc(max(nnhc,1)))
+  wr%e_inter = 0.d0
+  wr%e_nhc = 0.d0
+
   call system_clock_seconds(wall_md)
-  wall_out = 0.d0
   md_step = 0
   idx_int = 0
   iname_cur = 'none'
   integ_kind = PFM_NVE
-  pe = 0.d0; ke = 0.d0; temp = 0.d0; sim_time = 0.d0; total = 0.d0
   do while (md_step <= md_step_limit)
     ! integrator switch (md_simulation.f90:121-136)
     if (idx_int == 0 .or. md_step - 1 == sum(ilen(0:idx_int))) then
@@ -359,77 +459,25 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
       end select
     endif
     cend = max(min(sum(ilen(0:idx_int)), md_step_limit), md_step)
-    t = min(cend, next_multiple(md_step, ilog(idx_int)), next_multiple(md_step, out_period), &
-            next_multiple(md_step, isnap(idx_int)), next_multiple(md_step, period_traj))
-    energy = mod(t, ilog(idx_int)) == 0 .or. mod(t, out_period) == 0
-    rc = pfm_run(h, int(integ_kind, c_int), idt(idx_int), int(md_step, c_int), int(t, c_int), &
-                 merge(1_c_int, 0_c_int, energy), ms_de, last, ev)
+    ! log and out steps take energy records, snapshot and trajectory steps frames
+    wr%iname = iname_cur
+    wr%ilog = ilog(idx_int)
+    wr%isnap = isnap(idx_int)
+    eper = (/ int(ilog(idx_int), c_int), int(out_period, c_int) /)
+    xper = (/ int(isnap(idx_int), c_int), int(period_traj, c_int) /)
+    rc = pfm_set_records(h, 2_c_int, eper, 2_c_int, xper, 4_c_int, c_funloc(md_record), c_null_ptr)
+    call pfm_check(h, rc, out_id)
+    rc = pfm_run(h, int(integ_kind, c_int), idt(idx_int), int(md_step, c_int), int(cend, c_int), 0_c_int, &
+                 ms_de, last, ev)
     call pfm_check(h, rc, out_id)
     if (ev == PFM_EV_CONVERGED) then
       write(out_id,*) 'potential energy diffrence is small enough'
       md_step = last
       exit
     endif
-    call system_clock_seconds(wtmp)
-    if (energy) then
-      rc = pfm_energies(h, e_inter, pe, ke, temp, e_nhc, sim_time)
-      with_nhc_energy = nnhc > 0
-      e_nhc_sum = 0.d0
-      if (with_nhc_energy) e_nhc_sum = sum(e_nhc(1:nnhc))
-      total = pe + ke
-      conserved = total + e_nhc_sum
-      if (mod(t, ilog(idx_int)) == 0) then
-        write(log_id,'(A6,i9,7f24.6)',advance='no') trim(iname_cur), t, sim_time, conserved, e_nhc_sum, total, &
-            pe, ke, temp
-        write(log_id,fmt_int,advance='no') e_inter(1:ninter)
-        write(log_id,fmt_nhc) e_nhc(1:nnhc)
-      endif
-    endif
-    if (mod(t, out_period) == 0 .or. (mod(t, isnap(idx_int)) == 0 .and. t /= 0) .or. mod(t, period_traj) == 0) then
-      rc = pfm_get_state(h, pos, vel, frc)
-      call pfm_check(h, rc, out_id)
-    endif
-    if (mod(t, out_period) == 0) then
-      fs = 0.d0; mc = 0.d0; mcv = 0.d0; tm = 0.d0
-      do k = 1, groups(all_atoms)%n
-        i = groups(all_atoms)%idx(k)
-        fs = fs + frc(:,i)
-        mc = mc + mass(i)*pos(:,i)
-        mcv = mcv + mass(i)*vel(:,i)
-        tm = tm + mass(i)
-      enddo
-      if (tm > 0.d0) then
-        mc = mc/tm
-        mcv = mcv/tm
-      endif
-      vmax = 0.d0
-      do i = 1, n
-        vmax = max(vmax, sum(vel(:,i)**2))
-      enddo
-      write(out_id,'(A,f10.2)') 'exe time (s) = ', wtmp - wall0
-      write(out_id,'(A,i12,A,A6)') 'step = ', t, ' integrator = ', trim(iname_cur)
-      write(out_id,'(A,es21.9)') 'conserved energy (eV) = ', conserved
-      write(out_id,'(A,es16.6)') 'forces sum (eV/A) = ', sqrt(sum(fs**2))
-      write(out_id,'(A,es16.6)') 'c.o.m. velocity (A/fs) = ', sqrt(sum(mcv**2))
-      write(out_id,'(A,3f12.6)') 'c.o.m. position (A) = ', mc
-      write(out_id,'(A,f12.6)') 'maximum velocity (A/fs) = ', sqrt(vmax)
-      if (trim(iname_cur) /= 'nvms' .and. sqrt(sum(fs**2)) > 1.d-14) write(out_id,*) 'WARNING: forces sum is too big'
-      if (trim(iname_cur) == 'nvt' .and. xyz_moving == all_atoms .and. sqrt(sum(mcv**2)) > 1.d-14) &
-          write(out_id,*) 'WARNING: center of mass velocity is too big'
-      write(out_id,*)
-    endif
-    if (mod(t, isnap(idx_int)) == 0 .and. t /= 0) then
-      write(fname,'(A,A,i6.6,A)') trim(output_prefix), 'snapshot_', t, '.xyz'
-      call write_snapshot(trim(fname), groups(all_atoms), box, pos, vel, mass, types)
-    endif
-    if (mod(t, period_traj) == 0) then
-      write(fname,'(A,A,i2.2,A)') trim(output_prefix), 'traj_', traj_g, '.xyz'
-      call append_trajectory(trim(fname), groups(traj_g), box, pos, vel, mass, types, t)
-    endif
-    call system_clock_seconds(w2)
-    wall_out = wall_out + (w2 - wtmp)
-    md_step = t + 1
+    md_step = cend + 1
   enddo
+  wall_out = wr%wall_out
   call system_clock_seconds(wtmp)
   wall_md = wtmp - wall_md
 

@@ -1,7 +1,10 @@
 """The Fortran front end (pfmds_amd/bin/run_md_simulation), reading a
 reference-format settings file and driving libpfmds through ISO_C_BINDING,
 against the Python restatement of md()'s schedule driving the CPU checker:
-same log records (f24.6 as written), same nvms exit step, same final xyz."""
+same log records (f24.6 as written), same nvms exit step, same final xyz,
+and the outputs the front end writes from device records while the run goes
+on: snapshots (f27.16), trajectory frames (f10.4) and the out block's centre
+of mass (3f12.6) and largest speed (f12.6), each to the precision written."""
 import os
 import subprocess
 
@@ -46,6 +49,7 @@ def test_frontend_matches_checker(tmp_path, case):
         s.integrators = [system.Integrator("nvms", 0.5, 40, 1000, 4), system.Integrator("nvt", 0.5, 40, 20, 4)]
         s.ms_de = 1e-12
     s.md_step_limit = sum(i.length for i in s.integrators)
+    s.period_traj = 20
     s.write_files(str(tmp_path))
     print(f"[frontend] {case} n={s.n}", flush=True)
     p = subprocess.run([EXE, "-i", "md_run_settings.txt", "-op", "10"], cwd=str(tmp_path), capture_output=True,
@@ -53,7 +57,15 @@ def test_frontend_matches_checker(tmp_path, case):
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
     got = parse_log(os.path.join(str(tmp_path), s.logfilename), len(s.interactions), len(s.nhcs))
     o = oracle_ctx(s)
-    want, last = system.run_md(s, o, out_period=10)
+    events = {}
+    g_all = s.group_indices(s.all_atoms) - 1
+    g_traj = s.group_indices(s.traj) - 1
+
+    def on_event(t, kinds):
+        p_, v_, _ = o.state()
+        events[t] = (kinds, p_, v_)
+
+    want, last = system.run_md(s, o, out_period=10, on_event=on_event)
     assert [(r[0], r[1]) for r in got] == [(r.integrator, r.step) for r in want]
     for g, w in zip(got, want):
         ref = [w.time, w.conserved, w.e_nhc, w.total, w.pe, w.ke, w.temp]
@@ -67,3 +79,52 @@ def test_frontend_matches_checker(tmp_path, case):
     g = s.group_indices(s.all_atoms) - 1
     np.testing.assert_allclose(pf, po[g], atol=1e-8)
     np.testing.assert_allclose(vf, vo[g], atol=1e-8)
+    # snapshots and trajectory frames
+    snaps = [t for t, (k, _, _) in events.items() if "snapshot" in k]
+    for t in snaps:
+        _, ps, vs, _, _ = system.read_xyz(os.path.join(str(tmp_path), f"snapshot_{t:06d}.xyz"))
+        np.testing.assert_allclose(ps, events[t][1][g_all], atol=1e-8)
+        np.testing.assert_allclose(vs, events[t][2][g_all], atol=1e-8)
+    frames = read_traj(os.path.join(str(tmp_path), f"traj_{s.traj:02d}.xyz"))
+    trajs = sorted(t for t, (k, _, _) in events.items() if "traj" in k)
+    assert [f[0] for f in frames] == trajs
+    for t, pt, vt in frames:
+        np.testing.assert_allclose(pt, events[t][1][g_traj], atol=1.0001e-4)
+        np.testing.assert_allclose(vt, events[t][2][g_traj], atol=1.0001e-4)
+    # the out block
+    outs = parse_out(p.stdout)
+    want_out = sorted(t for t, (k, _, _) in events.items() if "out" in k)
+    assert [o_[0] for o_ in outs] == want_out
+    m = s.mass[g_all]
+    for t, mc, vmax in outs:
+        _, p_, v_ = events[t]
+        np.testing.assert_allclose(mc, (m[:, None] * p_[g_all]).sum(0) / m.sum(), atol=1.0001e-6)
+        np.testing.assert_allclose(vmax, np.sqrt((v_ * v_).sum(1).max()), atol=1.0001e-6)
+
+
+def read_traj(path):
+    """Frames of a write_particle_group_append file: (step, pos, vel)."""
+    out = []
+    with open(path) as f:
+        while True:
+            head = f.readline()
+            if not head.strip():
+                break
+            n = int(head.split()[0])
+            step = int(f.readline().split()[1])
+            a = np.array([[float(x) for x in f.readline().split()[:7]] for _ in range(n)])
+            out.append((step, a[:, 0:3], a[:, 3:6]))
+    return out
+
+
+def parse_out(text):
+    """(step, centre of mass, largest speed) of every out block."""
+    out, step, mc = [], None, None
+    for ln in text.splitlines():
+        if ln.startswith("step = "):
+            step = int(ln.split("=")[1].split()[0])
+        elif ln.startswith("c.o.m. position (A) = "):
+            mc = np.array([float(x) for x in ln.split("=")[1].split()])
+        elif ln.startswith("maximum velocity (A/fs) = "):
+            out.append((step, mc, float(ln.split("=")[1])))
+    return out

@@ -131,8 +131,8 @@ def test_records_nvms_until_converged():
     print(f"[records fcc nvms] n={s.n}", flush=True)
     mk = lambda: gpu_ctx(s, capi.FP64)  # noqa: E731
     e_per, x_per = (3,), (5,)
-    ref = chunked(mk, "nvms", 1.0, 400, e_per, x_per, ms_de=1e-6)
-    got, _ = recorded(mk, "nvms", 1.0, 400, e_per, x_per, slots=2, ms_de=1e-6)
+    ref = chunked(mk, "nvms", 1.0, 400, e_per, x_per, ms_de=1e-3)
+    got, _ = recorded(mk, "nvms", 1.0, 400, e_per, x_per, slots=2, ms_de=1e-3)
     last = max(got)
     ref = {k: v for k, v in ref.items() if k <= last}
     print(f"  converged at or after step {last}", flush=True)
