@@ -230,70 +230,92 @@ __global__ void k_sub_mcv(int N, double4* __restrict__ vel, const uint32_t* __re
 
 // ------------------------------------------------------------- records
 
-// slots of the next record: ctr = {energy records issued, frames issued,
-// current energy slot, current frame slot}
-__global__ void k_rec_slot(int* ctr, int ecap, int xcap, int e, int x) {
+// Records count themselves on the device: ctr = {energy records, frames};
+// record k goes to slot k % capacity of its ring.  The last kernel of a
+// record advances the counts.
+__global__ void k_rec_bump(int* ctr, int e, int x) {
   if (threadIdx.x != 0) return;
-  if (e) { ctr[2] = ctr[0] % ecap; ctr[0] = ctr[0] + 1; }
-  if (x) { ctr[3] = ctr[1] % xcap; ctr[1] = ctr[1] + 1; }
+  if (e) ctr[0] = ctr[0] + 1;
+  if (x) ctr[1] = ctr[1] + 1;
 }
 
-// per-block partials of md()'s out block: rows 0-2 the force sum, 3-5 sum m x,
+struct RecPartArgs {
+  int ni, N;
+  const double* e_row[PFM_MAXI];  // per-row energies of every interaction
+  int e_rows[PFM_MAXI];
+  const double4 *pos, *vel, *frc;
+  const uint32_t* mask;
+  uint32_t ke_bit, all_bit;  // all_moving, all_atoms
+  double* epart;             // [ni + 1][nb]: interaction energies, kinetic energy
+  double* opart;             // [11][nb]: the out block
+};
+
+// Per-block partials of one energy record.  The energy rows and the kinetic
+// energy are summed in the order of k_sum_part and k_ke_part (the values of
+// pfm_energies); the out block of md(): rows 0-2 the force sum, 3-5 sum m x,
 // 6-8 sum m v, 9 sum m over the all_atoms group (calculate_force_sum,
-// calculate_mass_center[_velocity], md_general.f90:184-234,277-298); row 10
-// the largest v.v over every atom (find_max_velocity, md_general.f90:366-380)
-__global__ void __launch_bounds__(RED_THREADS) k_out_part(int N, const double4* __restrict__ pos,
-                                                          const double4* __restrict__ vel,
-                                                          const double4* __restrict__ frc,
-                                                          const uint32_t* __restrict__ mask, uint32_t bit,
-                                                          double* __restrict__ part) {
+// calculate_mass_center[_velocity], md_general.f90:184-234,277-298), row 10
+// the largest v.v over every atom (find_max_velocity, md_general.f90:366-380).
+__global__ void __launch_bounds__(RED_THREADS) k_rec_part(RecPartArgs A) {
   __shared__ double sh[RED_THREADS];
+  const int nb = gridDim.x, b = blockIdx.x, t = threadIdx.x;
+  for (int k = 0; k < A.ni; ++k) {
+    double a = 0.0;
+    for (int i = b * RED_THREADS + t; i < A.e_rows[k]; i += nb * RED_THREADS) a += A.e_row[k][i];
+    double s = block_sum(a, sh);
+    if (t == 0) A.epart[k * nb + b] = s;
+  }
+  double ke = 0.0;
   double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   double vmax = -1.0;
-  for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
-    double4 v = vel[i];
+  for (int i = b * RED_THREADS + t; i < A.N; i += nb * RED_THREADS) {
+    double4 v = A.vel[i];
+    uint32_t m = A.mask[i];
     double v2 = v.x * v.x + v.y * v.y + v.z * v.z;
     vmax = v2 > vmax ? v2 : vmax;
-    if (mask[i] & bit) {
-      double4 p = pos[i], f = frc[i];
+    if (m & A.ke_bit) ke += v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF;
+    if (m & A.all_bit) {
+      double4 p = A.pos[i], f = A.frc[i];
       a[0] += f.x; a[1] += f.y; a[2] += f.z;
       a[3] += v.w * p.x; a[4] += v.w * p.y; a[5] += v.w * p.z;
       a[6] += v.w * v.x; a[7] += v.w * v.y; a[8] += v.w * v.z;
       a[9] += v.w;
     }
   }
+  double s = block_sum(ke, sh);
+  if (t == 0) A.epart[A.ni * nb + b] = s;
   for (int q = 0; q < 10; ++q) {
-    double s = block_sum(a[q], sh);
-    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = s;
+    s = block_sum(a[q], sh);
+    if (t == 0) A.opart[q * nb + b] = s;
   }
-  const int t = threadIdx.x;
   sh[t] = vmax;
   __syncthreads();
   for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
     if (t < o) sh[t] = sh[t + o] > sh[t] ? sh[t + o] : sh[t];
     __syncthreads();
   }
-  if (t == 0) part[10 * gridDim.x + blockIdx.x] = sh[0];
+  if (t == 0) A.opart[10 * nb + b] = sh[0];
 }
 
-// One energy record into its slot: ne rows of energy partials reduced as
-// k_reduce_rows does (interaction energies, then the kinetic energy), the
-// out-block rows, and a copy of the chain states.
+// One energy record into its slot: the ne energy rows reduced as
+// k_reduce_rows does, the out-block rows, and a copy of the chain states;
+// then the record counts advance (the frame's too when the step has one).
 __global__ void __launch_bounds__(RED_THREADS) k_rec_energy(int ne, int nb, const double* __restrict__ epart,
                                                             const double* __restrict__ opart,
                                                             const double* __restrict__ chains, int nchain_dbl,
-                                                            double* __restrict__ slots, int width,
-                                                            const int* __restrict__ ctr) {
+                                                            double* __restrict__ slots, int width, int ecap,
+                                                            int* __restrict__ ctr, int frame) {
   __shared__ double sh[RED_THREADS];
-  double* out = slots + (size_t)ctr[2] * width;
+  const int t = threadIdx.x;
+  const int n_e = ctr[0];
+  double* out = slots + (size_t)(n_e % ecap) * width;
   for (int q = 0; q < ne + 10; ++q) {
     const double* part = q < ne ? epart + (size_t)q * nb : opart + (size_t)(q - ne) * nb;
     double a = 0.0;
-    for (int b = threadIdx.x; b < nb; b += RED_THREADS) a += part[b];
+    for (int b = t; b < nb; b += RED_THREADS) a += part[b];
     double s = block_sum(a, sh);
-    if (threadIdx.x == 0) out[q] = s;
+    if (t == 0) out[q] = s;
   }
-  const int t = threadIdx.x;
   double m = -1.0;
   for (int b = t; b < nb; b += RED_THREADS) m = opart[10 * nb + b] > m ? opart[10 * nb + b] : m;
   sh[t] = m;
@@ -304,6 +326,11 @@ __global__ void __launch_bounds__(RED_THREADS) k_rec_energy(int ne, int nb, cons
   }
   if (t == 0) out[ne + 10] = sh[0];
   for (int j = t; j < nchain_dbl; j += RED_THREADS) out[ne + 11 + j] = chains[j];
+  __syncthreads();
+  if (t == 0) {
+    ctr[0] = n_e + 1;
+    if (frame) ctr[1] = ctr[1] + 1;
+  }
 }
 
 // One frame in original atom order into device staging: positions then
@@ -319,9 +346,9 @@ __global__ void k_rec_frame(int N, const double4* __restrict__ pos, const double
 }
 
 // the staged frame into its pinned host slot, in order, 16 bytes per store
-__global__ void k_rec_frame_out(size_t n2, const double2* __restrict__ stage, double* __restrict__ slots,
+__global__ void k_rec_frame_out(size_t n2, const double2* __restrict__ stage, double* __restrict__ slots, int xcap,
                                 const int* __restrict__ ctr) {
-  double2* out = (double2*)(slots + (size_t)ctr[3] * 2 * n2);
+  double2* out = (double2*)(slots + (size_t)(ctr[1] % xcap) * 2 * n2);
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n2; k += (size_t)gridDim.x * blockDim.x)
     out[k] = stage[k];
 }
@@ -1009,28 +1036,37 @@ static int rec_prepare(pfm_ctx* h) {
 // launches that take the records of the current step (captured with it)
 static void rec_launch(pfm_ctx* h, bool e, bool x) {
   auto& R = h->rec;
-  PFM_LAUNCH(h, "record", k_rec_slot, 1, 64, R.d_ctr, R.ecap, R.xcap > 0 ? R.xcap : 1, e ? 1 : 0, x ? 1 : 0);
+  const int nb = h->red_blocks, ni = (int)h->inter.size();
   if (e) {
-    const int nb = h->red_blocks, ni = (int)h->inter.size();
+    RecPartArgs A;
+    memset(&A, 0, sizeof A);
+    A.ni = ni;
+    A.N = h->N;
     for (int k = 0; k < ni; ++k) {
-      Inter& it = h->inter[k];
-      PFM_LAUNCH(h, "record", k_sum_part, nb, RED_THREADS, it.e_rows, it.e_row, h->red_part + (size_t)k * nb);
+      A.e_row[k] = h->inter[k].e_row;
+      A.e_rows[k] = h->inter[k].e_rows;
     }
-    MaskQ Q;
-    Q.nq = 1;
-    Q.bit[0] = 1u << (h->roles[0] - 1);
-    PFM_LAUNCH(h, "record", k_ke_part, nb, RED_THREADS, h->N, h->vel4, h->mask, Q, h->red_part + (size_t)ni * nb);
-    PFM_LAUNCH(h, "record", k_out_part, nb, RED_THREADS, h->N, h->pos4, h->vel4, h->frc4, h->mask,
-               1u << (h->roles[3] - 1), R.d_part);
-    PFM_LAUNCH(h, "record", k_rec_energy, 1, RED_THREADS, ni + 1, nb, h->red_part, R.d_part, h->nhc_state,
-               (int)(h->nhc.size() * sizeof(NhcDev) / sizeof(double)), R.h_e, R.ewidth, R.d_ctr);
+    A.pos = h->pos4;
+    A.vel = h->vel4;
+    A.frc = h->frc4;
+    A.mask = h->mask;
+    A.ke_bit = 1u << (h->roles[0] - 1);
+    A.all_bit = 1u << (h->roles[3] - 1);
+    A.epart = h->red_part;
+    A.opart = R.d_part;
+    PFM_LAUNCH(h, "record", k_rec_part, nb, RED_THREADS, A);
   }
   if (x) {
     const size_t n2 = (size_t)3 * h->N;
     PFM_LAUNCH(h, "record", k_rec_frame, (h->N + 255) / 256, 256, h->N, h->pos4, h->vel4, h->orig, R.d_x);
     PFM_LAUNCH(h, "record", k_rec_frame_out, (int)std::min<size_t>(1024, (n2 + 255) / 256), 256, n2,
-               (const double2*)R.d_x, R.h_x, R.d_ctr);
+               (const double2*)R.d_x, R.h_x, R.xcap, R.d_ctr);
   }
+  if (e)
+    PFM_LAUNCH(h, "record", k_rec_energy, 1, RED_THREADS, ni + 1, nb, h->red_part, R.d_part, h->nhc_state,
+               (int)(h->nhc.size() * sizeof(NhcDev) / sizeof(double)), R.h_e, R.ewidth, R.ecap, R.d_ctr, x ? 1 : 0);
+  else
+    PFM_LAUNCH(h, "record", k_rec_bump, 1, 64, R.d_ctr, 0, 1);
 }
 
 static double nhc_energy(const NhcDev& d) {
