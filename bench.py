@@ -184,6 +184,43 @@ def output_path(steps=2000, log_every=10, traj_every=100, warmup=20):
     return out
 
 
+def moire_fit(steps=2000, cells=((30, 17, 31, 18), (32, 18, 33, 19)), jobs=8):
+    """The moire fit's first search step (pfmds_amd/moire.py): the four
+    starting points of a coordinate, eight md() runs of `steps` nvms steps on
+    two graphene-on-Cu(111) cells, one run at a time and `jobs` at once."""
+    import tempfile
+
+    from pfmds_amd import moire
+    out = {"cells": [list(c) for c in cells], "steps": steps, "evaluations": 4, "jobs": jobs}
+    lines = {}
+    for j in (1, jobs):
+        with tempfile.TemporaryDirectory() as d:
+            fs = moire.read_fitting_parameters(configs.moire_fit_case(os.path.join(d, "in"), cells=cells, steps=steps))
+            fit = moire.Fitter(fs, out_period=10**9, jobs=j, out=open(os.devnull, "w"))
+            mid = (fs.min_params + fs.max_params) / 2
+            cand = []
+            for x in (fs.min_params[0], fs.max_params[0], fs.min_params[0] + (fs.max_params[0] - fs.min_params[0]) * moire.GOLD,
+                      fs.max_params[0] - (fs.max_params[0] - fs.min_params[0]) * moire.GOLD):
+                q = mid.copy()
+                q[0] = x
+                cand.append((q, True))
+            t0 = time.perf_counter()
+            fit.evaluate(cand)
+            out[f"jobs{j}_s_per_evaluation"] = (time.perf_counter() - t0) / 4
+            lines[j] = open(os.path.join(fs.out_path, "fitfit_out.txt")).read()
+            fit.close()
+            atoms = []
+            for i in (0, 1):
+                with open(os.path.join(fs.input_path, fs.start_xyz[i])) as f:
+                    atoms.append(int(f.readline()))
+            out["atoms"] = atoms
+    out["identical_results"] = lines[1] == lines[jobs]
+    out["speedup"] = out["jobs1_s_per_evaluation"] / out[f"jobs{jobs}_s_per_evaluation"]
+    print(f"# moire fit: {out['jobs1_s_per_evaluation']:.3f} s/evaluation one at a time, "
+          f"{out[f'jobs{jobs}_s_per_evaluation']:.3f} with {jobs} at once", file=sys.stderr, flush=True)
+    return out
+
+
 def surrogate_c4(train_iters=20000, frames_full=125, frames_sub=875, steps=10000, judge_every=100,
                  threshold=0.05, mixed_ms=None, fp64_ms=None, log=None):
     """Learned surrogate on graphene/Cu (94,560 atoms): frames from plain nvt
@@ -351,6 +388,7 @@ def main():
     ap.add_argument("--c4-iters", type=int, default=20000, help="surrogate training iterations")
     ap.add_argument("--c4-small", action="store_true",
                     help="surrogate with 10 + 40 frames and a 2,000-step run (profiling)")
+    ap.add_argument("--fit", action="store_true", help="time only the moire fit's first search step")
     ap.add_argument("--output-path", action="store_true",
                     help="time only the run with outputs: stopping at each output step vs output records")
     args = ap.parse_args()
@@ -361,6 +399,9 @@ def main():
         sys.exit(2)
     if args.c2_only:
         print(json.dumps({"graphene_cu_nvt": graphene_cu(args.steps, args.warmup)}))
+        return
+    if args.fit:
+        print(json.dumps({"moire_fit": moire_fit(args.steps)}))
         return
     if args.output_path:
         print(json.dumps({"output_path": output_path(args.steps, warmup=args.warmup)}))

@@ -23,6 +23,8 @@ Parameters the workload description leaves open, fixed here:
   morsec delta = 2, R1 = 6.0 A, R2 = 7.0 A, simplified = F
   lists r_cut = R2 + skin, update_period 10 (20 for the gas)
 """
+import os
+
 import numpy as np
 
 from .system import KB, MASS_COEF, NHC, Integrator, Interaction, System
@@ -207,6 +209,47 @@ def graphene_on_cu(cx=100, cy=58, layers=6, frozen=2, gx=104, gy=60, seed=1002, 
                   integrators=ints, nhcs=[NHC(3, temperature, 3, q1)], all_moving=3, xyz_moving=3, z_moving=6,
                   all_atoms=4, traj=1, period_traj=10**9, zero_momentum_period=10**9, ms_de=1e-8,
                   md_step_limit=100000 + nvt_steps, init_xyz_filename="graphene_on_cu.xyz")
+
+
+def moire_fit_case(directory, cells=((10, 6, 10, 6), (12, 6, 12, 6)), layers=4, steps=200, period=10, be0=-0.05,
+                   grd0=(0.2, 0.2), spread=0.1, seed=1006):
+    """Input folder of a morsec moire fit (fit_gr_moire.f90:106-184) over two
+    graphene-on-Cu(111) cells (cx, cy, gx, gy each), relaxed by `steps` steps
+    of nvms: settings, start xyz and morsec parameter files per system, the
+    bounds (MORSEC_CCU times 1 -+ spread) and the fitting-parameters file,
+    whose path is returned.  The metal is named CU / CU_fixed as the analysis
+    expects; zero energy levels are 0."""
+    os.makedirs(directory, exist_ok=True)
+    rename = {"Cu": "CU", "Cu_fixed": "CU_fixed"}
+    vals = []
+    for k, (cx, cy, gx, gy) in enumerate(cells):
+        s = graphene_on_cu(cx=cx, cy=cy, gx=gx, gy=gy, layers=layers, seed=seed + k, period=period)
+        s.species = [rename.get(x, x) for x in s.species]
+        s.groups = [[rename.get(x, x) for x in g] for g in s.groups]
+        s.integrators = [Integrator("nvms", 0.5, steps, 10**9, max(1, steps // 4))]
+        s.nhcs = []
+        s.md_step_limit = steps
+        s.init_xyz_filename = f"gr_cu_{k + 1}.xyz"
+        s.logfilename = f"log_{k + 1}.txt"
+        for it in s.interactions:
+            it.param_file = f"{it.name}.txt"
+        s.write_files(directory, settings=f"settings_{k + 1}.txt")
+        os.replace(os.path.join(directory, s.init_xyz_filename), os.path.join(directory, f"start_{k + 1}.xyz"))
+        vals.append((f"settings_{k + 1}.txt", f"start_{k + 1}.xyz", s.species.count("C"), 0.0, f"final_{k + 1}.txt"))
+    d, r, a, delt, r1, r2, _ = MORSEC_CCU
+    for name, f in (("morsec_min.txt", 1 - spread), ("morsec_max.txt", 1 + spread)):
+        with open(os.path.join(directory, name), "w") as fh:
+            fh.write(f"{d * f!r} {r * f!r} {a!r} {delt * f!r}\n{r1!r} {r2!r}\nF\n")
+    path = os.path.join(directory, "fitting_parameters.txt")
+    with open(path, "w") as fh:
+        fh.write(f"{'input_path:':<16}{os.path.join(directory, '')}\n{'out_path:':<16}{os.path.join(directory, '')}\n")
+        fh.write("interaction_name morsec\noutput_prefix fit\n")
+        for k, (st, x, c, z0, fin) in enumerate(vals):
+            fh.write(f"settings_{k + 1} {st}\nstart_xyz_{k + 1} {x}\nc_num_{k + 1} {c}\nzero_energy_{k + 1} {z0!r}\n"
+                     f"final_file_{k + 1} {fin}\n")
+        fh.write(f"min_param_file morsec_min.txt\nmax_param_file morsec_max.txt\nz 0.0\nbe0 {be0!r}\n"
+                 f"grd0_1 {grd0[0]!r}\ngrd0_2 {grd0[1]!r}\n")
+    return path
 
 
 def graphene_rebo(gx=8, gy=5, acc=1.42, lz=20.0, jitter=0.0, buckle=0.0, seed=1005, temperature=300.0, dt=0.5,
