@@ -211,10 +211,27 @@ typedef struct pfm_record {
   const double* e_nhc;   /* [n_chains] chain energies (eV)                 */
   const double* pos;     /* double[n][3] or NULL                           */
   const double* vel;     /* double[n][3] or NULL                           */
+  const int* group_n;    /* [33]: active atoms of group g at the step (pfm_group_count) */
 } pfm_record;
 typedef int (*pfm_record_fn)(void* user, const pfm_record* rec);
 int pfm_set_records(pfm_ctx* h, int n_energy_periods, const int* energy_periods, int n_frame_periods,
                     const int* frame_periods, int frame_slots, pfm_record_fn fn, void* user);
+
+/* Runtime group growth: one change_group line of the settings
+ * (md_simulation.f90:63-70), applied at the top of every step as md() does
+ * (md_simulation.f90:116-119) with change_particle_group_N
+ * (md_general.f90:82-94): up to step ts1 group `to` holds as many of its
+ * atoms as group `from` does (one more at ts1), then one more every frec
+ * steps before ts2, never more than its members; the active atoms are the
+ * first n in the group's order.  Everything that reads the group sees only
+ * its active atoms: integration, kinetic energy and temperature, zero
+ * momentum, chains, and the list search (rows of inactive members stay
+ * empty until the first rebuild after they join, as in find_neighbours).
+ * Taken for role and chain groups and the groups of lj and rjl lists;
+ * single systems only.  Changes are added before the first pfm_run.
+ * pfm_group_count gives the active atoms of a group after the last step. */
+int pfm_add_group_change(pfm_ctx* h, int from, int to, int ts1, int ts2, int frec);
+int pfm_group_count(pfm_ctx* h, int gid, int* n);
 
 /* Learned surrogate (interaction kind PFM_SUR).  None of these has a
  * counterpart in the reference; they serve the training and inference of a

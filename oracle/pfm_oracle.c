@@ -14,6 +14,7 @@
  *   integrators, NHC  code_source/MOLECULAR_DYNAMICS/md_integrators.f90
  *   reductions        code_source/MOLECULAR_DYNAMICS/md_general.f90
  *   step order        code_source/MOLECULAR_DYNAMICS/md_simulation.f90:114-231
+ *   group growth      change_particle_group_N, md_general.f90:82-94
  * It keeps the reference's data layout (padded per-atom lists storing dr and
  * |dr|, brute-force O(N1*N2) search, serial converse transpose), so it is
  * also the "reference algorithm" CPU baseline.
@@ -54,7 +55,10 @@
 #define MASS_COEF (1.6605389217 / 1.6021765654 * 100.0)
 #define KT_A_DEGREE (1.3806488 / 1.6021765654 * (1.0 / 10000.0))
 
-typedef struct { int n; int* idx; } ogroup; /* 0-based atom indices */
+typedef struct { int n, size; int* idx; } ogroup; /* 0-based atom indices; n active of size */
+
+typedef struct { int from, to, ts1, ts2, frec; } ochange;
+#define MAXCHANGE 8
 
 typedef struct {
   int N, nmax, period;
@@ -89,6 +93,8 @@ struct octx {
   int nnhc;
   onhc nhc[MAXNHC];
   int g_all_moving, g_xyz, g_z, g_all, zm_period, invert_z;
+  int nchange;
+  ochange change[MAXCHANGE];
   double pe, pe_prev, ke, temp, sim_time;
   double ts[4];
   char err[512];
@@ -178,6 +184,7 @@ int orc_set_group(octx* h, int gid, int n, const int* idx) {
   ogroup* g = &h->groups[gid];
   free(g->idx);
   g->n = n;
+  g->size = n;
   g->idx = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
   for (int i = 0; i < n; ++i) {
     if (idx[i] < 1 || idx[i] > h->N) return oerr(h, PFM_E_INVALID, "error: group index out of range");
@@ -252,6 +259,34 @@ int orc_add_nhc(octx* h, int group, double T, int M, double q1) {
   c->q[0] = q1;
   for (int i = 1; i < M; ++i) c->q[i] = c->q[0] / (3.0 * c->L);
   return PFM_OK;
+}
+
+/* One runtime group change of the settings (md_simulation.f90:63-70). */
+int orc_add_group_change(octx* h, int from, int to, int ts1, int ts2, int frec) {
+  if (h->nchange >= MAXCHANGE || from < 1 || from > MAXG || to < 1 || to > MAXG || frec < 1)
+    return oerr(h, PFM_E_INVALID, "error: bad group change");
+  ochange* c = &h->change[h->nchange++];
+  c->from = from; c->to = to; c->ts1 = ts1; c->ts2 = ts2; c->frec = frec;
+  return PFM_OK;
+}
+
+int orc_group_count(octx* h, int gid, int* n) {
+  if (gid < 0 || gid > MAXG) return oerr(h, PFM_E_INVALID, "error: group number out of range");
+  *n = h->groups[gid].n;
+  return PFM_OK;
+}
+
+/* change_particle_group_N (md_general.f90:82-94): up to step ts1 the group
+ * holds as many of its atoms as `from` does, one more at ts1, then one more
+ * every frec steps before ts2, never more than it has */
+static void change_group_n(ogroup* g, int step, const ochange* c, const ogroup* from) {
+  if (step <= c->ts1) {
+    g->n = from->n;
+    if (step == c->ts1) g->n = g->n + 1;
+  } else if (step < c->ts2 && (step - c->ts1) % c->frec == 0) {
+    g->n = g->n + 1;
+  }
+  if (g->n > g->size) g->n = g->size;
 }
 
 /* --------------------------------------------------------- neighbour lists */
@@ -1206,6 +1241,8 @@ int orc_run(octx* h, int integ, double dt, int s0, int s1, int energy_at_end, do
   int rc;
   for (int s = s0; s <= s1; ++s) {
     *last_step = s;
+    for (int c = 0; c < h->nchange; ++c)
+      change_group_n(&h->groups[h->change[c].to], s, &h->change[c], &h->groups[h->change[c].from]);
     rc = check_positions(h);
     if (rc) return rc;
     if (h->invert_z) invert_z(h);

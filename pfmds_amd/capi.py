@@ -39,6 +39,8 @@ _SIGS = {
     "get_list": [C.c_void_p, C.c_int, C.c_int, _i, _i, _i, _i, C.c_int],
     "get_normals": [C.c_void_p, C.c_int, _d],
     "last_error": [C.c_void_p, C.c_char_p, C.c_int],
+    "add_group_change": [C.c_void_p] + [C.c_int] * 5,
+    "group_count": [C.c_void_p, C.c_int, _i],
 }
 _libs = {}
 
@@ -49,7 +51,7 @@ class Record(C.Structure):
                 ("sim_time", C.c_double), ("pe", C.c_double), ("ke", C.c_double), ("temp", C.c_double),
                 ("force_sum", C.c_double * 3), ("mass_center", C.c_double * 3),
                 ("mass_center_vel", C.c_double * 3), ("max_vel", C.c_double), ("e_inter", _d), ("e_nhc", _d),
-                ("pos", _d), ("vel", _d)]
+                ("pos", _d), ("vel", _d), ("group_n", _i)]
 
 
 RECORD_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, _P(Record))
@@ -195,6 +197,16 @@ class Context:
                                                 _dp(rcut), _ip(per)))
         self.n_inter += 1
 
+    def add_group_change(self, g_from, g_to, ts1, ts2, frec):
+        """One runtime group change (change_particle_group_N, md_general.f90:82-94)."""
+        self._check(self._fn("add_group_change")(self.h, int(g_from), int(g_to), int(ts1), int(ts2), int(frec)))
+
+    def group_count(self, gid):
+        """Active atoms of group gid after the last step."""
+        n = C.c_int(0)
+        self._check(self._fn("group_count")(self.h, int(gid), C.byref(n)))
+        return n.value
+
     def add_nhc(self, group, temperature, M, q1):
         self._check(self._fn("add_nhc")(self.h, group, temperature, M, q1))
         self.n_nhc += 1
@@ -291,7 +303,8 @@ class Context:
         def tramp(_user, rp):
             try:
                 r = rp.contents
-                d = dict(step=r.step, integrator=r.integrator, sim_time=r.sim_time)
+                d = dict(step=r.step, integrator=r.integrator, sim_time=r.sim_time,
+                         group_n=np.ctypeslib.as_array(r.group_n, (33,)).copy())
                 if r.has_energy:
                     d.update(pe=r.pe, ke=r.ke, temp=r.temp, force_sum=np.array(r.force_sum[:]),
                              mass_center=np.array(r.mass_center[:]), mass_center_vel=np.array(r.mass_center_vel[:]),

@@ -531,11 +531,92 @@ static int dalloc(pfm_ctx* h, T** p, size_t n) {
 
 static bool is_cos(int k) { return k == PFM_LJC || k == PFM_MORSEC; }
 
+// ---------------------------------------------------------- group growth
+
+// active atoms of a growing group: the first n of its members
+__global__ void k_group_count(int N, const int* __restrict__ orig, const int* __restrict__ rank, uint32_t bit, int n,
+                              uint32_t* __restrict__ mask) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int r = rank[orig[i]];
+  if (r < 0) return;
+  uint32_t m = mask[i];
+  mask[i] = r < n ? (m | bit) : (m & ~bit);
+}
+
+// change_particle_group_N (md_general.f90:82-94) on host counts
+static void change_counts(int* n, const int* size, const std::vector<pfm_ctx::GroupChange>& ch, int step) {
+  for (const auto& c : ch) {
+    int& g = n[c.to];
+    if (step <= c.ts1) {
+      g = n[c.from];
+      if (step == c.ts1) g = g + 1;
+    } else if (step < c.ts2 && (step - c.ts1) % c.frec == 0) {
+      g = g + 1;
+    }
+    if (g > size[c.to]) g = size[c.to];
+  }
+}
+
+// whether the changes at step s alter any count, starting from the current ones
+static bool counts_change_at(const pfm_ctx* h, int s) {
+  if (h->changes.empty()) return false;
+  int n[PFM_MAXG + 1];
+  std::copy(h->grp.n, h->grp.n + PFM_MAXG + 1, n);
+  change_counts(n, h->grp.size, h->changes, s);
+  return !std::equal(n, n + PFM_MAXG + 1, h->grp.n);
+}
+
+// the changes of step s: host counts, then the active bits of the groups that moved
+static void apply_changes(pfm_ctx* h, int s) {
+  int old[PFM_MAXG + 1];
+  std::copy(h->grp.n, h->grp.n + PFM_MAXG + 1, old);
+  change_counts(h->grp.n, h->grp.size, h->changes, s);
+  for (int g = 1; g <= PFM_MAXG; ++g)
+    if (h->grp.n[g] != old[g] && h->grp.rank[g])
+      PFM_LAUNCH(h, "group_count", k_group_count, (h->N + 255) / 256, 256, h->N, h->orig, h->grp.rank[g],
+                 1u << (g - 1), h->grp.n[g], h->mask);
+}
+
+// Groups that grow keep their members under a spare group bit for the list
+// rows.  Growth is taken for groups whose lists come from the generic search
+// (lj, rjl) and for roles and chain groups.
+static int growth_setup(pfm_ctx* h) {
+  if (h->changes.empty()) return PFM_OK;
+  if (h->nrep > 1) return fail(h, PFM_E_INVALID, "error: group changes are for single systems only");
+  bool grows[PFM_MAXG + 1] = {false};
+  for (const auto& c : h->changes) grows[c.to] = true;
+  for (auto& it : h->inter)
+    for (int k = 0; k < it.nl_n; ++k)
+      if ((grows[it.nl[k].g1] || grows[it.nl[k].g2]) && it.kind != PFM_LJ && it.kind != PFM_RJL)
+        return fail(h, PFM_E_INVALID, "error: a changing group in the lists of an interaction other than lj or rjl");
+  int spare = PFM_MAXG;
+  auto taken = [&](int g) {
+    if (!h->grp.idx[g].empty() || h->grp.used[g] || h->grp.shadow[g]) return true;
+    for (int k = 0; k < 4; ++k)
+      if (h->roles[k] == g) return true;
+    for (auto& c : h->nhc)
+      if (c.group == g) return true;
+    for (int q = 1; q <= PFM_MAXG; ++q)
+      if (h->grp.shadow[q] == g) return true;
+    return false;
+  };
+  for (int g = 1; g <= PFM_MAXG; ++g) {
+    if (!grows[g]) continue;
+    while (spare >= 1 && taken(spare)) --spare;
+    if (spare < 1) return fail(h, PFM_E_INVALID, "error: no spare group number for a changing group");
+    h->grp.shadow[g] = spare--;
+    h->grp.used[g] = true;
+  }
+  return PFM_OK;
+}
+
 // Allocate lists, cells and reductions once all interactions are known.
 static int finalize(pfm_ctx* h) {
   if (h->sorted_once) return PFM_OK;
   int rc;
   const int nrep = h->nrep;
+  if ((rc = growth_setup(h))) return rc;
   // replica of every original atom; atoms of every group per replica
   std::vector<int> rep_of(h->N);
   for (int r = 0; r < nrep; ++r)
@@ -639,7 +720,7 @@ static int finalize(pfm_ctx* h) {
   // group rows
   for (int g = 1; g <= PFM_MAXG; ++g) {
     if (!h->grp.used[g]) continue;
-    if ((rc = dalloc(h, &h->grp.rows[g], h->grp.n[g]))) return rc;
+    if ((rc = dalloc(h, &h->grp.rows[g], h->grp.size[g]))) return rc;
     if ((rc = dalloc(h, &h->grp.row_of[g], h->N))) return rc;
   }
   // list storage
@@ -691,10 +772,20 @@ static int finalize(pfm_ctx* h) {
     if ((rc = dalloc(h, &h->grp.loc[g], h->N))) return rc;
     HIPCHK(h, hipMemcpyAsync(h->grp.loc[g], loc.data(), sizeof(int) * h->N, hipMemcpyHostToDevice, h->stream));
   }
-  // group bit masks
+  // group bit masks: every member active until the first step applies the changes
   std::vector<uint32_t> mask(h->N, 0);
   for (int g = 1; g <= PFM_MAXG; ++g)
-    for (int a : h->grp.idx[g]) mask[a] |= 1u << (g - 1);
+    for (int a : h->grp.idx[g]) {
+      mask[a] |= 1u << (g - 1);
+      if (h->grp.shadow[g]) mask[a] |= 1u << (h->grp.shadow[g] - 1);
+    }
+  for (int g = 1; g <= PFM_MAXG; ++g) {
+    if (!h->grp.shadow[g]) continue;
+    std::vector<int> rank(h->N, -1);
+    for (size_t k = 0; k < h->grp.idx[g].size(); ++k) rank[h->grp.idx[g][k]] = (int)k;
+    if ((rc = dalloc(h, &h->grp.rank[g], h->N))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->grp.rank[g], rank.data(), sizeof(int) * h->N, hipMemcpyHostToDevice, h->stream));
+  }
   HIPCHK(h, hipMemcpyAsync(h->mask, mask.data(), sizeof(uint32_t) * h->N, hipMemcpyHostToDevice, h->stream));
   // chains
   if (!h->nhc.empty()) {
@@ -1104,6 +1195,7 @@ static int rec_deliver_one(pfm_ctx* h, bool wait, int* rc) {
   r.step = P.step;
   r.integrator = P.integ;
   r.sim_time = P.time;
+  r.group_n = P.group_n.data();
   const int ni = (int)h->inter.size(), nq = (int)h->nhc.size();
   std::vector<double> enhc(h->e_nhc);
   enhc.resize(nq, 0.0);
@@ -1115,7 +1207,7 @@ static int rec_deliver_one(pfm_ctx* h, bool wait, int* rc) {
     for (int k = 0; k < ni; ++k) pe += e[k];
     r.pe = pe;
     r.ke = e[ni];
-    r.temp = 2 * r.ke / PFM_KB / (3 * h->grp.n[h->roles[0]]);
+    r.temp = 2 * r.ke / PFM_KB / (3 * P.group_n[h->roles[0]]);
     const double* o = e + ni + 1;
     const double tm = o[9];
     for (int d = 0; d < 3; ++d) {
@@ -1181,7 +1273,8 @@ static int rec_issue(pfm_ctx* h, int integ, const std::vector<RecNote>& notes) {
   auto& R = h->rec;
   if (notes.empty()) return PFM_OK;
   for (const RecNote& n : notes) {
-    pfm_ctx::Records::Pending P{n.step, integ, n.time, -1, -1, nullptr};
+    pfm_ctx::Records::Pending P{n.step, integ, n.time, -1, -1, nullptr, {}};
+    P.group_n.assign(h->grp.n, h->grp.n + PFM_MAXG + 1);
     if (n.e) {
       P.eslot = (int)(R.ne++ % R.ecap);
       ++R.pending_e;
@@ -1305,6 +1398,7 @@ int pfm_set_group(pfm_ctx* h, int gid, int n, const int* idx) {
     g.push_back(a);
   }
   h->grp.n[gid] = n;
+  h->grp.size[gid] = n;
   return PFM_OK;
 }
 
@@ -1340,7 +1434,7 @@ int pfm_add_interaction(pfm_ctx* h, int kind, const double* params, int np, int 
     if (nmax[k] < 1) return fail(h, PFM_E_INVALID, "error: neighb_num_max must be >= 1");
     ListDev& L = it.nl[k];
     L.g1 = g1; L.g2 = g2; L.nmax = nmax[k]; L.rcut = rcut[k]; L.period = period[k];
-    L.nrows = h->grp.n[g1];
+    L.nrows = h->grp.size[g1];
     h->grp.used[g1] = true;
   }
   if (kind == PFM_LJ1G || kind == PFM_RJL || kind == PFM_TB || kind == PFM_REBOSC) {
@@ -1393,7 +1487,7 @@ int pfm_add_interaction(pfm_ctx* h, int kind, const double* params, int np, int 
 int pfm_add_nhc(pfm_ctx* h, int group, double T, int M, double q1) {
   if ((int)h->nhc.size() >= PFM_MAXNHC || M < 1 || M > 16 || group < 1 || group > PFM_MAXG)
     return fail(h, PFM_E_INVALID, "error: bad nhc definition");
-  int L = h->grp.n[group];
+  int L = h->grp.size[group];
   if (L < 1 || q1 < 0.0 || T < 0.0) return fail(h, PFM_E_NHC_PARAMS, "error: wrong nhc parameters");
   for (auto& c : h->nhc) {
     std::vector<int> a = h->grp.idx[c.group], b = h->grp.idx[group];
@@ -1424,7 +1518,9 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
   IntP P = int_params(h, dt);
   const int N = h->N, nb = (N + 255) / 256;
   uint32_t all_bit = 1u << (h->roles[3] - 1);
-  bool everyone = h->grp.n[h->roles[3]] == N;
+  bool all_grows = false;
+  for (const auto& c : h->changes) all_grows = all_grows || c.to == h->roles[3];
+  bool everyone = h->grp.size[h->roles[3]] == N && !all_grows;
   bool first_writes = false;
   if (everyone && !h->inter.empty()) {
     Inter& f = h->inter[0];
@@ -1444,9 +1540,13 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
   auto body = [&](int s) -> int {
     int rc;
     *last_step = s;
+    // change_particle_group_N at the top of the step (md_simulation.f90:116-119)
+    if (!h->changes.empty()) apply_changes(h, s);
     bool energy = (s == s1 && energy_at_end) || integ == PFM_NVMS;
     const bool re = rec_e_at(h, s), rx = rec_x_at(h, s);
-    const bool report = energy || re || rx;  // velocities complete at the end of the step
+    // velocities complete at the end of the step: something reads them, or the
+    // next step moves another set of atoms
+    const bool report = energy || re || rx || counts_change_at(h, s + 1);
     if (s != 0) {
       bool conv = std::fabs(h->pe - h->pe_prev) <= ms_de;
       if (h->nrep > 1 && integ == PFM_NVMS) {
@@ -1512,6 +1612,15 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
   for (int s = s0; s <= s1;) {
     const bool steady = nvt ? nhc_pending : kick_pending;
     bool blocked = graphs && s > 0 && s % blk == 0 && s + blk - 1 < s1 && steady;
+    if (blocked && !h->changes.empty()) {
+      // a block holds no change of group counts, nor does the step after it
+      int n[PFM_MAXG + 1];
+      std::copy(h->grp.n, h->grp.n + PFM_MAXG + 1, n);
+      for (int t = s; t <= s + blk && blocked; ++t) {
+        change_counts(n, h->grp.size, h->changes, t);
+        blocked = std::equal(n, n + PFM_MAXG + 1, h->grp.n);
+      }
+    }
     int ne_b = 0, nx_b = 0;
     if (blocked && rec_on(h)) {
       // records inside a block are captured with it; the block must end in
@@ -1654,6 +1763,22 @@ int pfm_set_records(pfm_ctx* h, int n_e, const int* e_periods, int n_x, const in
   R.xslots = xp.empty() ? 0 : frame_slots;
   R.fn = fn;
   R.user = user;
+  return PFM_OK;
+}
+
+int pfm_add_group_change(pfm_ctx* h, int from, int to, int ts1, int ts2, int frec) {
+  if (h->sorted_once) return fail(h, PFM_E_INVALID, "error: group changes must be added before the first pfm_run");
+  if (from < 1 || from > PFM_MAXG || to < 1 || to > PFM_MAXG)
+    return fail(h, PFM_E_INVALID, "error: group number out of range (1..32)");
+  if (frec < 1) return fail(h, PFM_E_INVALID, "error: change frequency must be >= 1");
+  if (h->nrep > 1) return fail(h, PFM_E_INVALID, "error: group changes are for single systems only");
+  h->changes.push_back(pfm_ctx::GroupChange{from, to, ts1, ts2, frec});
+  return PFM_OK;
+}
+
+int pfm_group_count(pfm_ctx* h, int gid, int* n) {
+  if (gid < 1 || gid > PFM_MAXG) return fail(h, PFM_E_INVALID, "error: group number out of range (1..32)");
+  *n = h->grp.n[gid];
   return PFM_OK;
 }
 
@@ -1874,7 +1999,7 @@ int pfm_destroy(pfm_ctx* h) {
   F(h->nhc_state);
   if (h->h_red) (void)hipHostFree(h->h_red);
   if (h->herr) (void)hipHostFree(h->herr);
-  for (int g = 0; g <= PFM_MAXG; ++g) { F(h->grp.rows[g]); F(h->grp.row_of[g]); F(h->grp.loc[g]); }
+  for (int g = 0; g <= PFM_MAXG; ++g) { F(h->grp.rows[g]); F(h->grp.row_of[g]); F(h->grp.loc[g]); F(h->grp.rank[g]); }
   for (auto& it : h->inter) {
     for (int k = 0; k < 3; ++k) { F(it.nl[k].nnum); F(it.nl[k].nbr); F(it.nl[k].rev); }
     F(it.e_row); F(it.lc); F(it.B); F(it.Bp); F(it.gnorm); F(it.gS);

@@ -350,7 +350,12 @@ struct KTimer {
 };
 
 struct Groups {
-  int n[PFM_MAXG + 1] = {0};
+  int n[PFM_MAXG + 1] = {0};     // active atoms: the first n of idx (change_particle_group_N)
+  int size[PFM_MAXG + 1] = {0};  // members
+  // a group that grows at run time keeps its members under a spare bit (the
+  // list rows) and its active atoms under its own bit
+  int shadow[PFM_MAXG + 1] = {0};
+  int* rank[PFM_MAXG + 1] = {nullptr};  // device: original atom -> position in idx or -1 (growing groups)
   std::vector<int> idx[PFM_MAXG + 1];  // host: 0-based original indices, group order
   int* rows[PFM_MAXG + 1] = {nullptr};    // device: members in internal order
   int* row_of[PFM_MAXG + 1] = {nullptr};  // device: internal atom -> row or -1
@@ -429,6 +434,11 @@ struct pfm_ctx {
   };
   std::vector<GraphEntry> graphs;
   long graph_captures = 0, graph_launches = 0, graph_clock = 0;
+  // runtime group changes (pfm_add_group_change)
+  struct GroupChange {
+    int from, to, ts1, ts2, frec;
+  };
+  std::vector<GroupChange> changes;
   // output records (pfm_set_records)
   struct Records {
     std::vector<int> eper, xper;  // energy and frame periods
@@ -448,6 +458,7 @@ struct pfm_ctx {
       double time;
       int eslot, xslot;  // -1 when absent
       hipEvent_t done;
+      std::vector<int> group_n;  // active atoms of every group at the step
     };
     std::deque<Pending> q;
     std::vector<hipEvent_t> free_ev;

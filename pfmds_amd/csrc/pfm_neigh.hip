@@ -251,9 +251,10 @@ __global__ void k_identity_rows(int N, int* __restrict__ rows, int* __restrict__
 void launch_group_rows(pfm_ctx* h) {
   int N = h->N, nb = (N + 255) / 256;
   for (int g = 1; g <= PFM_MAXG; ++g) {
-    if (!h->grp.used[g] || h->grp.n[g] == 0) continue;
-    uint32_t bit = 1u << (g - 1);
-    if (h->grp.n[g] == N) {
+    if (!h->grp.used[g] || h->grp.size[g] == 0) continue;
+    // members (a growing group's spare bit), not only the active atoms
+    uint32_t bit = 1u << ((h->grp.shadow[g] ? h->grp.shadow[g] : g) - 1);
+    if (h->grp.size[g] == N) {
       PFM_LAUNCH(h, "group_rows", k_identity_rows, nb, 256, N, h->grp.rows[g], h->grp.row_of[g]);
       continue;
     }
@@ -269,6 +270,7 @@ void launch_group_rows(pfm_ctx* h) {
 struct SearchArgs {
   int nrows;
   const int* rows;
+  uint32_t g1bit;  // active row atoms; the rows of a growing group hold inactive members with no entries
   uint32_t g2bit;
   const double4* pos;
   const uint32_t* mask;
@@ -305,6 +307,11 @@ __global__ void __launch_bounds__(256) k_build(SearchArgs a, int nblocks_real) {
   int i = a.rows[row];
   const RepDev R = a.rdev[a.rep ? a.rep[i] : 0];
   if (R.tile) return;  // searched by k_build_tile
+  if (!(a.mask[i] & a.g1bit)) {
+    // find_neighbours runs over the group's active atoms only (md_neighbours.f90:67)
+    if (t == 0) a.nnum[row] = 0;
+    return;
+  }
   double4 pi = a.pos[i];
   int c[3];
   c[0] = cell_coord(pi.x, R.inv_clen.x, R.nc[0]);
@@ -542,7 +549,7 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
     const bool inrow = qq < ncen;
     const int i = a.bin ? a.bin[cs + (inrow ? qq : 0)] : cs + (inrow ? qq : 0);
     const int row = inrow ? row_of[i] : -1;
-    const bool active = row >= 0;
+    const bool active = row >= 0 && (a.mask[i] & a.g1bit);
     const double4 pi = a.pos[i];
     const float px = (float)(pi.x - org[0]), py = (float)(pi.y - org[1]), pz = (float)(pi.z - org[2]);
     const int own = tc.off[13] + qq;  // the row atom's own tile slot
@@ -602,6 +609,7 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
         count += tot;
       }
     }
+    if (row >= 0 && !active && t == 0) a.nnum[row] = 0;  // an inactive member of a growing group
     if (active && t == 0) {
       a.nnum[row] = count < a.nmax ? count : a.nmax;
       if (count > a.nmax) raise_err(a.err, a.err_code, a.orig[i] + 1, count, a.step);
@@ -640,6 +648,7 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   SearchArgs a;
   a.nrows = L.nrows;
   a.rows = h->grp.rows[L.g1];
+  a.g1bit = 1u << (L.g1 - 1);
   a.g2bit = 1u << (L.g2 - 1);
   a.pos = h->pos4;
   a.mask = h->mask;

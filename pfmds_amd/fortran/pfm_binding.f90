@@ -19,6 +19,7 @@ type, bind(C) :: pfm_record
   real(c_double) :: force_sum(3), mass_center(3), mass_center_vel(3)
   real(c_double) :: max_vel
   type(c_ptr) :: e_inter, e_nhc, pos, vel
+  type(c_ptr) :: group_n
 end type pfm_record
 
 interface
@@ -85,6 +86,17 @@ interface
     integer(c_int), value :: n_e, n_x, frame_slots
     integer(c_int) :: e_periods(*), x_periods(*)
     type(c_funptr), value :: fn
+  end function
+  integer(c_int) function pfm_add_group_change(h, gfrom, gto, ts1, ts2, frec) bind(C, name='pfm_add_group_change')
+    import :: c_ptr, c_int
+    type(c_ptr), value :: h
+    integer(c_int), value :: gfrom, gto, ts1, ts2, frec
+  end function
+  integer(c_int) function pfm_group_count(h, gid, n) bind(C, name='pfm_group_count')
+    import :: c_ptr, c_int
+    type(c_ptr), value :: h
+    integer(c_int), value :: gid
+    integer(c_int) :: n
   end function
   integer(c_int) function pfm_destroy(h) bind(C, name='pfm_destroy')
     import :: c_ptr, c_int

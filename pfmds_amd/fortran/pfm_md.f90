@@ -44,6 +44,8 @@ integer(c_int) function md_record(user, rec) bind(C)
   type(c_ptr), value :: user
   type(pfm_record), intent(in) :: rec
   real(8), pointer :: p(:), pos(:,:), vel(:,:)
+  integer(c_int), pointer :: gn(:)
+  type(group_def) :: ga, gt
   real(8) :: e_nhc_sum, total, conserved, fs, mcv, w1, w2
   character(len=128) :: fname
   integer :: t
@@ -87,13 +89,19 @@ integer(c_int) function md_record(user, rec) bind(C)
   if (rec%has_frame /= 0) then
     call c_f_pointer(rec%pos, pos, [3, wr%n])
     call c_f_pointer(rec%vel, vel, [3, wr%n])
+    ! the groups' active atoms at the step (change_particle_group_N)
+    call c_f_pointer(rec%group_n, gn, [33])
+    ga = wr%groups(wr%all_atoms)
+    ga%n = gn(wr%all_atoms + 1)
+    gt = wr%groups(wr%traj_g)
+    gt%n = gn(wr%traj_g + 1)
     if (mod(t, wr%isnap) == 0 .and. t /= 0) then
       write(fname,'(A,A,i6.6,A)') trim(wr%prefix), 'snapshot_', t, '.xyz'
-      call write_snapshot(trim(fname), wr%groups(wr%all_atoms), wr%box, pos, vel, wr%mass, wr%types)
+      call write_snapshot(trim(fname), ga, wr%box, pos, vel, wr%mass, wr%types)
     endif
     if (mod(t, wr%period_traj) == 0) then
       write(fname,'(A,A,i2.2,A)') trim(wr%prefix), 'traj_', wr%traj_g, '.xyz'
-      call append_trajectory(trim(fname), wr%groups(wr%traj_g), wr%box, pos, vel, wr%mass, wr%types, t)
+      call append_trajectory(trim(fname), gt, wr%box, pos, vel, wr%mass, wr%types, t)
     endif
   endif
   call system_clock_seconds(w2)
@@ -285,11 +293,11 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
   integer :: out_id, all_out_id, out_period, num_of_omp_treads, rand_seed, precision
   character(len=*) :: input_path, settings_filename, output_prefix
   type(c_ptr) :: h
-  integer(c_int) :: rc, last, ev
+  integer(c_int) :: rc, last, ev, nact
   integer :: fid, log_id, i, j, k, n, ntypes, ngroups, nint, nnhc, ninter, nchange, md_step, md_step_limit
   integer :: zmp, all_moving, xyz_moving, z_moving, all_atoms, traj_g, period_traj, idx_int, cend, np, nl_n
   integer :: nhc_g, nhc_m, kind, integ_kind
-  integer, allocatable :: ilen(:), isnap(:), ilog(:)
+  integer, allocatable :: ilen(:), isnap(:), ilog(:), chg(:,:)
   real(8), allocatable :: idt(:)
   character(len=32), allocatable :: iname(:), names(:,:), types(:)
   real(8), allocatable :: pos(:,:), vel(:,:), frc(:,:), mass(:), e_inter(:), e_nhc(:)
@@ -332,10 +340,12 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
   read(fid,*) str, traj_g;         write(out_id,'(A32,i12)') str, traj_g
   read(fid,*) str, period_traj;    write(out_id,'(A32,i12)') str, period_traj
   read(fid,*) str, nchange;        write(out_id,'(A32,i12)') str, nchange
-  if (nchange /= 0) then
-    write(out_id,*) 'error: change_group_num > 0 (runtime group growth) is not on the GPU path'
-    stop 1
-  endif
+  ! runtime group changes (md_simulation.f90:63-70)
+  allocate(chg(5, max(nchange, 1)))
+  do i = 1, nchange
+    read(fid,*) str, chg(1,i), chg(2,i);          write(out_id,'(A32,2i12)') str, chg(1,i), chg(2,i)
+    read(fid,*) str, chg(3,i), chg(4,i), chg(5,i); write(out_id,'(A32,3i12)') str, chg(3,i), chg(4,i), chg(5,i)
+  enddo
   read(fid,*) str, invert;         write(out_id,'(A32,l8)') str, invert
   read(fid,*) str, nint;           write(out_id,'(A32,i12)') str, nint
   allocate(iname(0:nint), idt(0:nint), ilen(0:nint), isnap(0:nint), ilog(0:nint))
@@ -356,6 +366,11 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
     else
       rc = pfm_set_group(h, int(i, c_int), 0_c_int, groups(i)%idx)
     endif
+    call pfm_check(h, rc, out_id)
+  enddo
+  do i = 1, nchange
+    rc = pfm_add_group_change(h, int(chg(1,i), c_int), int(chg(2,i), c_int), int(chg(3,i), c_int), &
+                              int(chg(4,i), c_int), int(chg(5,i), c_int))
     call pfm_check(h, rc, out_id)
   enddo
   do i = 1, nnhc
@@ -499,6 +514,8 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
   write(log_id,fmt_int,advance='no') e_inter(1:ninter)
   if (md_step /= 0) then
     rc = pfm_get_state(h, pos, vel, frc)
+    rc = pfm_group_count(h, int(all_atoms, c_int), nact)
+    groups(all_atoms)%n = nact
     write(fname,'(A,A,A)') trim(output_prefix), 'final_', trim(init_xyz)
     call write_snapshot(trim(fname), groups(all_atoms), box, pos, vel, mass, types)
   endif

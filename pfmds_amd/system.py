@@ -89,6 +89,8 @@ class System:
     md_step_limit: int = 0
     logfilename: str = "log.txt"
     init_xyz_filename: str = "init.xyz"
+    # runtime group changes (md_simulation.f90:63-70): (from, to, ts1, ts2, frec)
+    changes: List[tuple] = dataclasses.field(default_factory=list)
 
     @property
     def n(self):
@@ -125,7 +127,11 @@ class System:
             f"all_atoms_group_num {self.all_atoms}",
             f"traj_group_num {self.traj}",
             f"period_traj {self.period_traj}",
-            "change_group_num 0",
+            f"change_group_num {len(self.changes)}",
+        ]
+        for (g_from, g_to, ts1, ts2, frec) in self.changes:
+            lines += [f"group_change {g_from} {g_to}", f"change_steps {ts1} {ts2} {frec}"]
+        lines += [
             f"invert_z_vel {'T' if self.invert_z else 'F'}",
             f"integrators_num {len(self.integrators)}",
             "name dt steps period_snapshot period_log",
@@ -215,8 +221,11 @@ def read_files(directory, settings="md_run_settings.txt"):
         groups.append([n for n in names])
     roles = [int(val()[0]) for _ in range(6)]
     nchange = int(val()[0])
-    if nchange:
-        raise ValueError("change_group_num > 0 is outside the GPU path (runtime group growth)")
+    changes = []
+    for _ in range(nchange):
+        g_from, g_to = (int(x) for x in val()[:2])
+        ts1, ts2, frec = (int(x) for x in val()[:3])
+        changes.append((g_from, g_to, ts1, ts2, frec))
     invert = _fortran_logical(val()[0])
     nint = int(val()[0])
     next(it)
@@ -249,7 +258,7 @@ def read_files(directory, settings="md_run_settings.txt"):
                   z_moving=roles[2], all_atoms=roles[3], traj=roles[4], period_traj=roles[5],
                   zero_momentum_period=zmp, invert_z=invert, new_velocities=new_vel, ms_de=ms_de,
                   initial_temperature=t0, md_step_limit=md_step_limit, logfilename=logfilename,
-                  init_xyz_filename=init_xyz)
+                  init_xyz_filename=init_xyz, changes=changes)
 
 
 # -- building a context ---------------------------------------------------------
@@ -268,6 +277,8 @@ def build_context(s: System, make):
         ctx.add_nhc(c.group, c.temperature, c.M, c.q1)
     for it in s.interactions:
         ctx.add_interaction(it.name, it.params, it.lists)
+    for c in s.changes:
+        ctx.add_group_change(*c)
     return ctx
 
 

@@ -1,0 +1,46 @@
+"""A deposition-style system for the runtime group growth tests: substrate
+atoms A and atoms D that join the moving group one at a time
+(change_particle_group_N, md_general.f90:82-94).  Group 2 (A then D, in
+index order) grows from group 1's count: one more atom at step ts1, then one
+every frec steps before ts2.  Group 2 is all_moving and xyz_moving and the
+group of an lj interaction, so atoms that have not joined neither move nor
+interact."""
+import numpy as np
+
+from pfmds_amd import configs
+from pfmds_amd.system import Integrator, Interaction, System
+
+
+def deposition(n_side=4, spacing=4.2, seed=91, ts1=5, ts2=40, frec=7, period=10, temperature=150.0,
+               kind="lj"):
+    rng = configs.SplitMix64(seed)
+    g = np.stack(np.meshgrid(*[np.arange(n_side)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pos = (g + 0.5) * spacing + 0.15 * (2 * rng.uniform(3 * len(g)).reshape(-1, 3) - 1)
+    n = len(pos)
+    L = n_side * spacing
+    species = ["A" if i < n // 2 else "D" for i in range(n)]
+    if kind == "lj":
+        mass = np.full(n, 39.95)
+        inter = Interaction("lj", [0.0104, 3.40, 6.5, 7.5], [(2, 2, 96, 8.5, period), (2, 2, 96, 8.5, period)])
+    else:
+        mass = np.full(n, 63.546)
+        inter = Interaction("rjl", list(configs.RJL_CU), [(2, 2, 96, configs.RJL_CU[6] + 0.5, period)])
+    vel = configs.maxwell(rng, mass, temperature)
+    return System(box=np.array([L, L, L]), pos=np.mod(pos, L), vel=vel, mass=mass, species=species,
+                  groups=[["A"], ["A", "D"], ["A", "D"], ["#"]], interactions=[inter],
+                  integrators=[Integrator("nve", 1.0, 60, 10**9, 10**9)], all_moving=2, xyz_moving=2, z_moving=4,
+                  all_atoms=3, traj=2, period_traj=10**9, zero_momentum_period=10**9, md_step_limit=60,
+                  changes=[(1, 2, ts1, ts2, frec)])
+
+
+def expected_counts(steps, n_from, size, ts1, ts2, frec):
+    """Group 2's active count after each step, written out from the rule:
+    n_from up to ts1 - 1, n_from + 1 at ts1, then +1 at ts1 + k*frec < ts2."""
+    out = []
+    for s in steps:
+        if s < ts1:
+            c = n_from
+        else:
+            c = n_from + 1 + len([t for t in range(ts1 + frec, min(s, ts2 - 1) + 1, frec)])
+        out.append(min(c, size))
+    return out

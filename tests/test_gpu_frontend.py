@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from pfmds_amd import configs, system
+from tests import growth_case
 from tests.harness import oracle_ctx
 
 pytestmark = pytest.mark.gpu
@@ -34,9 +35,14 @@ def parse_log(path, n_inter, n_nhc):
     return recs
 
 
-@pytest.mark.parametrize("case", ["lj_gas", "gr_cu", "rebo"])
+@pytest.mark.parametrize("case", ["lj_gas", "gr_cu", "rebo", "growth"])
 def test_frontend_matches_checker(tmp_path, case):
-    if case == "rebo":
+    if case == "growth":
+        # change_group lines in the settings; the trajectory group grows
+        s = growth_case.deposition(n_side=6, kind="rjl")
+        s.integrators = [system.Integrator("nve", 1.0, 30, 25, 5), system.Integrator("nvt", 0.5, 30, 1000, 5)]
+        s.nhcs = [system.NHC(2, 150.0, 3, 1000.0)]
+    elif case == "rebo":
         s = configs.graphene_rebo(gx=6, gy=4, jitter=0.05, buckle=0.1, seed=53)
         s.integrators = [system.Integrator("nve", 0.5, 40, 1000, 4)]
     elif case == "lj_gas":
@@ -49,7 +55,7 @@ def test_frontend_matches_checker(tmp_path, case):
         s.integrators = [system.Integrator("nvms", 0.5, 40, 1000, 4), system.Integrator("nvt", 0.5, 40, 20, 4)]
         s.ms_de = 1e-12
     s.md_step_limit = sum(i.length for i in s.integrators)
-    s.period_traj = 20
+    s.period_traj = 20 if case != "growth" else 7
     s.write_files(str(tmp_path))
     print(f"[frontend] {case} n={s.n}", flush=True)
     p = subprocess.run([EXE, "-i", "md_run_settings.txt", "-op", "10"], cwd=str(tmp_path), capture_output=True,
@@ -63,7 +69,7 @@ def test_frontend_matches_checker(tmp_path, case):
 
     def on_event(t, kinds):
         p_, v_, _ = o.state()
-        events[t] = (kinds, p_, v_)
+        events[t] = (kinds, p_, v_, o.group_count(s.traj), o.group_count(s.all_atoms))
 
     want, last = system.run_md(s, o, out_period=10, on_event=on_event)
     assert [(r[0], r[1]) for r in got] == [(r.integrator, r.step) for r in want]
@@ -80,25 +86,28 @@ def test_frontend_matches_checker(tmp_path, case):
     np.testing.assert_allclose(pf, po[g], atol=1e-8)
     np.testing.assert_allclose(vf, vo[g], atol=1e-8)
     # snapshots and trajectory frames
-    snaps = [t for t, (k, _, _) in events.items() if "snapshot" in k]
+    snaps = [t for t, ev in events.items() if "snapshot" in ev[0]]
     for t in snaps:
         _, ps, vs, _, _ = system.read_xyz(os.path.join(str(tmp_path), f"snapshot_{t:06d}.xyz"))
-        np.testing.assert_allclose(ps, events[t][1][g_all], atol=1e-8)
-        np.testing.assert_allclose(vs, events[t][2][g_all], atol=1e-8)
+        ga = g_all[:events[t][4]]
+        np.testing.assert_allclose(ps, events[t][1][ga], atol=1e-8)
+        np.testing.assert_allclose(vs, events[t][2][ga], atol=1e-8)
     frames = read_traj(os.path.join(str(tmp_path), f"traj_{s.traj:02d}.xyz"))
-    trajs = sorted(t for t, (k, _, _) in events.items() if "traj" in k)
+    trajs = sorted(t for t, ev in events.items() if "traj" in ev[0])
     assert [f[0] for f in frames] == trajs
     for t, pt, vt in frames:
-        np.testing.assert_allclose(pt, events[t][1][g_traj], atol=1.0001e-4)
-        np.testing.assert_allclose(vt, events[t][2][g_traj], atol=1.0001e-4)
+        # the trajectory group's active atoms at the step
+        gt = g_traj[:events[t][3]]
+        np.testing.assert_allclose(pt, events[t][1][gt], atol=1.0001e-4)
+        np.testing.assert_allclose(vt, events[t][2][gt], atol=1.0001e-4)
     # the out block
     outs = parse_out(p.stdout)
-    want_out = sorted(t for t, (k, _, _) in events.items() if "out" in k)
+    want_out = sorted(t for t, ev in events.items() if "out" in ev[0])
     assert [o_[0] for o_ in outs] == want_out
-    m = s.mass[g_all]
     for t, mc, vmax in outs:
-        _, p_, v_ = events[t]
-        np.testing.assert_allclose(mc, (m[:, None] * p_[g_all]).sum(0) / m.sum(), atol=1.0001e-6)
+        _, p_, v_, _, n_all = events[t]
+        m = s.mass[g_all[:n_all]]
+        np.testing.assert_allclose(mc, (m[:, None] * p_[g_all[:n_all]]).sum(0) / m.sum(), atol=1.0001e-6)
         np.testing.assert_allclose(vmax, np.sqrt((v_ * v_).sum(1).max()), atol=1.0001e-6)
 
 

@@ -1,0 +1,99 @@
+"""Runtime group growth on the GPU (pfm_add_group_change) against the CPU
+checker and against itself.
+
+Bar (FP64): positions, velocities and energies within 1e-10 relative of the
+checker after every chunk, active counts equal after every chunk, lists
+equal as sets after rebuilds, for the deposition case with lj and with rjl.
+Captured blocks are used away from the steps where a count changes and give
+bit-identical results to direct launches; records carry the active counts
+of their step; groups of interactions whose lists are not searched
+generically (tb here) are refused.
+"""
+import numpy as np
+import pytest
+
+from pfmds_amd import capi, configs
+from tests import growth_case
+from tests.harness import gpu_ctx, list_sets, oracle_ctx, rel_inf
+
+pytestmark = pytest.mark.gpu
+
+CHUNKS = [(0, 0), (1, 15), (16, 37), (38, 60)]
+
+
+@pytest.mark.parametrize("kind,side", [("lj", 4), ("rjl", 4), ("lj", 8)])
+def test_growth_matches_checker(kind, side):
+    # 4^3 atoms: lists searched per row; 8^3: three cells a side, searched per cell tile
+    s = growth_case.deposition(n_side=side, kind=kind)
+    print(f"[growth {kind}] n={s.n}", flush=True)
+    g = gpu_ctx(s, capi.FP64)
+    o = oracle_ctx(s)
+    want = growth_case.expected_counts([c[1] for c in CHUNKS], s.n // 2, s.n, 5, 40, 7)
+    for (s0, s1), n in zip(CHUNKS, want):
+        g.run("nve", 1.0, s0, s1, True)
+        o.run("nve", 1.0, s0, s1, True)
+        assert g.group_count(2) == o.group_count(2) == n
+        (pg, vg, fg), (po, vo, fo) = g.state(), o.state()
+        assert rel_inf(pg, po) < 1e-10 and rel_inf(vg, vo) < 1e-10 and rel_inf(fg, fo) < 1e-10, s1
+        eg, eo = g.energies(), o.energies()
+        assert eg["pe"] == pytest.approx(eo["pe"], rel=1e-10) and eg["temp"] == pytest.approx(eo["temp"], rel=1e-10)
+        if s1 % 10 == 0 or s1 == 15:
+            ga, _ = list_sets(g, 1, 1)
+            oa, _ = list_sets(o, 1, 1)
+            assert ga == oa, s1
+    g.close()
+    o.close()
+
+
+def test_growth_graphs_equal_direct_launches():
+    # changes every 30 steps: blocks of 10 between them are captured
+    s = growth_case.deposition(ts1=5, ts2=200, frec=30)
+    out = []
+    for graphs in (True, False):
+        c = gpu_ctx(s, capi.MIXED)
+        c.set_graphs(graphs)
+        res = []
+        for s0, s1 in ((0, 0), (1, 77), (78, 151)):
+            c.run("nve", 1.0, s0, s1, True)
+            res.append((c.state(), c.energies()["pe"], c.group_count(2)))
+        launched = c.graph_count()[1]
+        print(f"  graphs={graphs}: {launched} launches", flush=True)
+        assert (launched > 3) if graphs else launched == 0
+        out.append(res)
+        c.close()
+    for (sa, pa, na), (sb, pb, nb) in zip(*out):
+        for x, y in zip(sa, sb):
+            assert np.array_equal(x, y)
+        assert pa == pb and na == nb
+    assert out[0][-1][2] == 32 + 1 + len(range(35, 152, 30))
+
+
+def test_growth_counts_in_records():
+    s = growth_case.deposition()
+    c = gpu_ctx(s, capi.FP64)
+    got = {}
+    c.set_records((3,), (9,), 2, lambda r: got.__setitem__(r["step"], r))
+    c.run("nve", 1.0, 0, 60, False)
+    steps = sorted(got)
+    want = growth_case.expected_counts(steps, 32, 64, 5, 40, 7)
+    assert [int(got[t]["group_n"][2]) for t in steps] == want
+    assert all(int(got[t]["group_n"][1]) == 32 and int(got[t]["group_n"][3]) == 64 for t in steps)
+    # the temperature of a record divides by its step's count
+    for t in steps:
+        if "ke" in got[t]:
+            n = int(got[t]["group_n"][2])
+            assert got[t]["temp"] == pytest.approx(2 * got[t]["ke"] / capi_kb() / (3 * n), rel=1e-14)
+    c.close()
+
+
+def capi_kb():
+    return 1.3806488 / 1.6021765654 * (1.0 / 10000.0)
+
+
+def test_growth_of_a_tb_group_refused():
+    s = configs.graphene_on_cu(cx=6, cy=4, gx=6, gy=4, layers=3, seed=93)
+    s.changes = [(5, 1, 3, 50, 5)]
+    c = gpu_ctx(s, capi.FP64)
+    with pytest.raises(capi.PfmError):
+        c.run("nve", 0.5, 0, 1, True)
+    c.close()

@@ -12,6 +12,7 @@ import pytest
 
 from pfmds_amd import capi
 from tests import harness
+from tests import test_oracle_growth as growth
 from tests import test_oracle_pins as pins
 
 ORACLE_DIR = os.path.join(harness.ROOT, "oracle")
@@ -109,6 +110,12 @@ MUTANTS = {
     "nvms_signed_energy_change": (
         "fabs(h->pe - h->pe_prev) <= ms_de", "(h->pe - h->pe_prev) <= ms_de",
         pins.test_nvms_lj_dimer_minimum, ()),
+    "group_change_period_from_zero": (
+        "(step - c->ts1) % c->frec == 0", "step % c->frec == 0",
+        growth.test_growth_counts_follow_the_rule, ()),
+    "group_change_rows_over_all_members": (
+        "for (int ind = 0; ind < g1->n; ++ind) {", "for (int ind = 0; ind < g1->size; ++ind) {",
+        growth.test_growth_moves_and_lists_only_joined_atoms, ()),
 }
 
 
