@@ -170,6 +170,18 @@ int pfm_set_tuning(pfm_ctx* h, int lanes_per_row, int block);
  * launching the steps one by one.  pfm_set_graphs(h, 0) launches every step
  * directly.  No counterpart in the reference (launch mechanics). */
 int pfm_set_graphs(pfm_ctx* h, int on);
+/* nvms blocks (default on).  Under nvms md() tests abs(PE-PE_prev) <= ms_de
+ * at every step (md_simulation.f90:142-147), which needs each step's
+ * energies on the host.  pfm_run instead runs blocks of steps (the list
+ * schedule's period, up to 32, else 16; captured like other blocks) that keep
+ * every step's energies and state in device memory, reads the energies back
+ * once per block, and on the first step whose test holds restores the state
+ * of the step before it; the lists are rebuilt from a fresh sort at the next
+ * step.  Results, exit step and energies are those of testing every step.
+ * Single systems without group changes; pfm_set_speculation(h, 0) tests
+ * every step on the host.  No counterpart in the reference. */
+int pfm_set_speculation(pfm_ctx* h, int on);
+
 /* Blocks captured and blocks launched from graphs since the handle was made. */
 int pfm_graph_count(pfm_ctx* h, long* captured, long* launched);
 

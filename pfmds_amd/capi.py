@@ -99,6 +99,8 @@ def load_library(path, prefix):
         lib.pfm_sur_descriptors.restype = C.c_int
         lib.pfm_sur_loss_grad.argtypes = [C.c_void_p, C.c_int, _d, C.c_double, C.c_double, C.c_double, _d, _d]
         lib.pfm_sur_loss_grad.restype = C.c_int
+        lib.pfm_set_speculation.argtypes = [C.c_void_p, C.c_int]
+        lib.pfm_set_speculation.restype = C.c_int
         lib.pfm_set_records.argtypes = [C.c_void_p, C.c_int, _i, C.c_int, _i, C.c_int, RECORD_FN, C.c_void_p]
         lib.pfm_set_records.restype = C.c_int
     _libs[key] = lib
@@ -324,6 +326,10 @@ class Context:
         self._check(self.lib.pfm_set_records(self.h, len(energy_periods), _ip(ep), len(frame_periods), _ip(xp),
                                              int(frame_slots), fn, None))
         self._rec_fn = fn
+
+    def set_speculation(self, on):
+        """nvms blocks with one energy read-back per block (pfm_set_speculation)."""
+        self._check(self.lib.pfm_set_speculation(self.h, int(bool(on))))
 
     def graph_count(self):
         """(blocks captured, blocks launched from graphs)"""

@@ -943,6 +943,55 @@ static int energies(pfm_ctx* h, bool nvt) {
   return PFM_OK;
 }
 
+// ------------------------------------------------------------ nvms blocks
+
+// the state of one step into ring slot k, or back from it
+__global__ void k_spec_copy(int N, int back, double4* __restrict__ pos, double4* __restrict__ vel,
+                            double4* __restrict__ frc, uint32_t* __restrict__ mask, int* __restrict__ orig,
+                            double4* __restrict__ rpos, double4* __restrict__ rvel, double4* __restrict__ rfrc,
+                            uint32_t* __restrict__ rmask, int* __restrict__ rorig) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (back) {
+    pos[i] = rpos[i]; vel[i] = rvel[i]; frc[i] = rfrc[i]; mask[i] = rmask[i]; orig[i] = rorig[i];
+  } else {
+    rpos[i] = pos[i]; rvel[i] = vel[i]; rfrc[i] = frc[i]; rmask[i] = mask[i]; rorig[i] = orig[i];
+  }
+}
+
+static int spec_alloc(pfm_ctx* h, int k) {
+  if (h->spec_k >= k) return PFM_OK;
+  auto F = [](void* p) { if (p) (void)hipFree(p); };
+  F(h->spec_e); F(h->sp_pos); F(h->sp_vel); F(h->sp_frc); F(h->sp_mask); F(h->sp_orig);
+  h->spec_e = nullptr; h->sp_pos = h->sp_vel = h->sp_frc = nullptr; h->sp_mask = nullptr; h->sp_orig = nullptr;
+  h->spec_k = 0;
+  const size_t n = (size_t)k * h->N;
+  int rc;
+  if ((rc = dalloc(h, &h->spec_e, (size_t)k * (h->inter.size() + 1))) || (rc = dalloc(h, &h->sp_pos, n)) ||
+      (rc = dalloc(h, &h->sp_vel, n)) || (rc = dalloc(h, &h->sp_frc, n)) || (rc = dalloc(h, &h->sp_mask, n)) ||
+      (rc = dalloc(h, &h->sp_orig, n)))
+    return rc;
+  h->spec_k = k;
+  return PFM_OK;
+}
+
+// the energies of the step into slot k (the sums of energies()) and its state
+static void spec_record(pfm_ctx* h, int k) {
+  const int nb = h->red_blocks, ni = (int)h->inter.size(), N = h->N;
+  for (int q = 0; q < ni; ++q) {
+    Inter& it = h->inter[q];
+    PFM_LAUNCH(h, "energy", k_sum_part, nb, RED_THREADS, it.e_rows, it.e_row, h->red_part + (size_t)q * nb);
+  }
+  MaskQ Q;
+  Q.nq = 1;
+  Q.bit[0] = 1u << (h->roles[0] - 1);
+  PFM_LAUNCH(h, "energy", k_ke_part, nb, RED_THREADS, N, h->vel4, h->mask, Q, h->red_part + (size_t)ni * nb);
+  PFM_LAUNCH(h, "energy", k_reduce_rows, 1, RED_THREADS, ni + 1, nb, h->red_part, h->spec_e + (size_t)k * (ni + 1));
+  const size_t o = (size_t)k * N;
+  PFM_LAUNCH(h, "spec", k_spec_copy, (N + 255) / 256, 256, N, 0, h->pos4, h->vel4, h->frc4, h->mask, h->orig,
+             h->sp_pos + o, h->sp_vel + o, h->sp_frc + o, h->sp_mask + o, h->sp_orig + o);
+}
+
 // rebuild every list due at step s, in settings order; re-sort the atoms
 // first when all of them are due (step 0 always)
 static int update_lists(pfm_ctx* h, int s) {
@@ -955,7 +1004,8 @@ static int update_lists(pfm_ctx* h, int s) {
       if (per > 0 && s % per != 0) all_due = false;
     }
   int rc;
-  if (all_due || !h->sorted_once) {
+  if (all_due || !h->sorted_once || h->need_sort) {
+    h->need_sort = false;
     if ((rc = launch_sort_and_bin(h, s, true))) return rc;
     h->sorted_once = true;
     for (auto& it : h->inter)
@@ -1536,6 +1586,10 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
   bool nhc_pending = false;
   bool kick_pending = false;  // second half kick deferred into the next kick_drift
   const int CONVERGED = -1;
+  // nvms blocks: the steps from spec_base on record their energies and
+  // states into ring slots instead of reading energies back
+  bool spec_on = false;
+  int spec_base = 0;
   // one md() step: the body of md_simulation.f90:137-231
   auto body = [&](int s) -> int {
     int rc;
@@ -1554,7 +1608,7 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
         conv = h->rep_pe.size() == (size_t)h->nrep && h->rep_pe_prev.size() == (size_t)h->nrep;
         for (int r = 0; conv && r < h->nrep; ++r) conv = std::fabs(h->rep_pe[r] - h->rep_pe_prev[r]) <= ms_de;
       }
-      if (integ == PFM_NVMS && conv) return CONVERGED;
+      if (integ == PFM_NVMS && conv && !spec_on) return CONVERGED;
       IntP Q = P;
       Q.pre_kick = kick_pending ? 1 : 0;
       kick_pending = false;
@@ -1594,7 +1648,9 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
       }
       h->sim_time = h->sim_time + dt;
     }
-    if (energy) {
+    if (spec_on) {
+      spec_record(h, s - spec_base);
+    } else if (energy) {
       if ((rc = energies(h, integ == PFM_NVT))) return rc;
       if ((rc = check_err(h, s))) return rc;
     }
@@ -1608,7 +1664,180 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
   // of the two re-sort buffers is current, so the cache is keyed by those.
   const int blk = graph_block(h);
   const bool graphs = h->use_graphs && !h->timing && integ != PFM_NVMS && blk > 0;
+  // steps s .. s+nblk-1 from a captured graph: captured at their first
+  // occurrence (the host side of body runs then), replayed afterwards
+  auto run_block = [&](int s, int nblk) -> int {
+    int rc;
+    std::vector<uint64_t> key = graph_key(h, integ, dt, nblk, first_writes, s);
+    key.push_back((uint64_t)(spec_on ? h->spec_e : nullptr));
+    pfm_ctx::GraphEntry* g = nullptr;
+    for (auto& e : h->graphs)
+      if (e.key == key) g = &e;
+    if (!g) {
+      hipGraph_t graph;
+      h->step_base = s;
+      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+      int rc2 = PFM_OK;
+      for (int t = s; t < s + nblk && rc2 == PFM_OK; ++t) rc2 = body(t);
+      hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+      h->step_base = 0;
+      if (rc2 != PFM_OK) {
+        if (ec == hipSuccess) (void)hipGraphDestroy(graph);
+        return rc2 == CONVERGED ? fail(h, PFM_E_INVALID, "error: convergence test inside a captured block") : rc2;
+      }
+      HIPCHK(h, ec);
+      const bool same = integ == PFM_NVMS ? !kick_pending : (nvt ? nhc_pending : kick_pending);
+      if (!same) {
+        (void)hipGraphDestroy(graph);
+        return fail(h, PFM_E_INVALID, "error: captured block does not end in its starting state");
+      }
+      pfm_ctx::GraphEntry e;
+      e.key = key;
+      e.after = graph_buffers(h);
+      HIPCHK(h, hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      if (h->graphs.size() >= GRAPH_CACHE) {
+        // least recently launched block goes
+        auto old = std::min_element(h->graphs.begin(), h->graphs.end(),
+                                    [](const pfm_ctx::GraphEntry& a, const pfm_ctx::GraphEntry& b) {
+                                      return a.used < b.used;
+                                    });
+        (void)hipGraphExecDestroy(old->exec);
+        h->graphs.erase(old);
+      }
+      e.used = ++h->graph_clock;
+      h->graphs.push_back(e);
+      ++h->graph_captures;
+      ++h->graph_launches;
+      // the capture ran the block's host side; its kernels run now
+      PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, s);
+      HIPCHK(h, hipGraphLaunch(h->graphs.back().exec, h->stream));
+      PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, 0);
+    } else {
+      g->used = ++h->graph_clock;
+      PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, s);
+      HIPCHK(h, hipGraphLaunch(g->exec, h->stream));
+      PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, 0);
+      ++h->graph_launches;
+      // the host side of the block: buffers, time, energies carried over
+      set_graph_buffers(h, g->after);
+      *last_step = s + nblk - 1;
+      for (int t = 0; t < nblk; ++t) h->sim_time = h->sim_time + dt;
+      h->pe_prev = h->pe;
+      h->rep_pe_prev = h->rep_pe;
+    }
+    (void)rc;
+    return PFM_OK;
+  };
   std::vector<RecNote> notes;
+  // nvms without a read-back per step: blocks of K steps record every step's
+  // energies and state; one read-back per block finds the first step whose
+  // exit test holds (md_simulation.f90:142-147) and restores the state of the
+  // step before it
+  auto nvms_blocks = [&]() -> int {
+    int rc;
+    const int K = (blk > 0 && blk <= 32) ? blk : 16;
+    if ((rc = spec_alloc(h, K))) return rc;
+    const int ni = (int)h->inter.size(), w = ni + 1;
+    std::vector<double> ev((size_t)K * w);
+    const bool capture = h->use_graphs && blk == K;
+    for (int s = s0; s <= s1;) {
+      *last_step = s;
+      if (s != 0 && std::fabs(h->pe - h->pe_prev) <= ms_de) {
+        *event = PFM_EV_CONVERGED;
+        return PFM_OK;
+      }
+      int end = std::min(s1, (s / K + 1) * K - 1);
+      int ne_b = 0, nx_b = 0;
+      if (rec_on(h)) {
+        // the block's records fit the rings: none is handed on before its block is settled
+        for (int t = s; t <= end; ++t) {
+          int e2 = ne_b + (rec_e_at(h, t) ? 1 : 0), x2 = nx_b + (rec_x_at(h, t) ? 1 : 0);
+          if (t > s && (e2 > h->rec.ecap || x2 > h->rec.xcap)) {
+            end = t - 1;
+            break;
+          }
+          ne_b = e2;
+          nx_b = x2;
+        }
+        if (ne_b + nx_b > 0 && (rc = rec_reserve(h, ne_b, nx_b))) return rc;
+      }
+      const int n = end - s + 1;
+      const double t_start = h->sim_time, pe0 = h->pe;
+      spec_on = true;
+      spec_base = s;
+      int rc2 = PFM_OK;
+      if (capture && s > 0 && s % K == 0 && n == K) {
+        rc2 = run_block(s, K);
+      } else {
+        for (int t = s; t <= end && rc2 == PFM_OK; ++t) rc2 = body(t);
+      }
+      spec_on = false;
+      if (rc2) return rc2;
+      if (ne_b + nx_b > 0) {
+        notes.clear();
+        double tt = t_start;
+        for (int t = s; t <= end; ++t) {
+          if (t != 0) tt = tt + dt;
+          if (rec_e_at(h, t) || rec_x_at(h, t)) notes.push_back(RecNote{t, tt, rec_e_at(h, t), rec_x_at(h, t)});
+        }
+        if ((rc = rec_issue(h, integ, notes))) return rc;
+      }
+      HIPCHK(h, hipMemcpyAsync(ev.data(), h->spec_e, sizeof(double) * n * w, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipMemcpyAsync(h->herr, h->derr, sizeof(DevErr), hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      // pe of every step of the block, summed as energies() sums it; the exit
+      // test of step t reads pe(t-1) and pe(t-2)
+      std::vector<double> pe(n);
+      int tc = -1;
+      for (int j = 0; j < n; ++j) {
+        double p = 0.0;
+        for (int k = 0; k < ni; ++k) p += ev[(size_t)j * w + k];
+        pe[j] = p;
+        const double m2 = j == 0 ? pe0 : pe[j - 1];
+        if (j + 1 < n && std::fabs(p - m2) <= ms_de) {
+          tc = s + j + 1;
+          break;
+        }
+      }
+      const int jl = tc >= 0 ? tc - 1 - s : n - 1;  // the last step that stands
+      for (int k = 0; k < ni; ++k) h->inter[k].energy = ev[(size_t)jl * w + k];
+      h->pe = pe[jl];
+      h->pe_prev = jl == 0 ? pe0 : pe[jl - 1];
+      h->ke = ev[(size_t)jl * w + ni];
+      h->temp = 2 * h->ke / PFM_KB / (3 * h->grp.n[h->roles[0]]);
+      if (tc < 0) {
+        if ((rc = check_err(h, end)) || (rc = rec_poll(h, false))) return rc;
+        s = end + 1;
+        continue;
+      }
+      // steps tc..end ran ahead: their errors and records do not stand
+      const DevErr er = *h->herr;
+      if (er.code != 0 && er.step < tc) return check_err(h, end);
+      if (er.code != 0) HIPCHK(h, hipMemsetAsync(h->derr, 0, sizeof(DevErr), h->stream));
+      auto& R = h->rec;
+      while (!R.q.empty() && R.q.back().step >= tc) {
+        if (R.q.back().eslot >= 0) --R.pending_e;
+        if (R.q.back().xslot >= 0) --R.pending_x;
+        R.free_ev.push_back(R.q.back().done);
+        R.q.pop_back();
+      }
+      const size_t o = (size_t)jl * h->N;
+      PFM_LAUNCH(h, "spec", k_spec_copy, (h->N + 255) / 256, 256, h->N, 1, h->pos4, h->vel4, h->frc4, h->mask,
+                 h->orig, h->sp_pos + o, h->sp_vel + o, h->sp_frc + o, h->sp_mask + o, h->sp_orig + o);
+      // lists, cells and rows follow the restored state at the next step
+      h->need_sort = true;
+      h->sim_time = t_start;
+      for (int t = s; t < tc; ++t)
+        if (t != 0) h->sim_time = h->sim_time + dt;
+      *last_step = tc;
+      *event = PFM_EV_CONVERGED;
+      return PFM_OK;
+    }
+    return check_err(h, s1);
+  };
+  if (integ == PFM_NVMS && h->speculate && h->nrep == 1 && h->changes.empty() && !h->timing)
+    return nvms_blocks();
   for (int s = s0; s <= s1;) {
     const bool steady = nvt ? nhc_pending : kick_pending;
     bool blocked = graphs && s > 0 && s % blk == 0 && s + blk - 1 < s1 && steady;
@@ -1635,62 +1864,7 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
     if (blocked) {
       if (ne_b + nx_b > 0 && (rc = rec_reserve(h, ne_b, nx_b))) return rc;
       const double t_start = h->sim_time;
-      std::vector<uint64_t> key = graph_key(h, integ, dt, blk, first_writes, s);
-      pfm_ctx::GraphEntry* g = nullptr;
-      for (auto& e : h->graphs)
-        if (e.key == key) g = &e;
-      if (!g) {
-        hipGraph_t graph;
-        h->step_base = s;
-        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
-        int rc2 = PFM_OK;
-        for (int t = s; t < s + blk && rc2 == PFM_OK; ++t) rc2 = body(t);
-        hipError_t ec = hipStreamEndCapture(h->stream, &graph);
-        h->step_base = 0;
-        if (rc2 != PFM_OK) {
-          if (ec == hipSuccess) (void)hipGraphDestroy(graph);
-          return rc2 == CONVERGED ? fail(h, PFM_E_INVALID, "error: convergence test inside a captured block") : rc2;
-        }
-        HIPCHK(h, ec);
-        if (nvt ? !nhc_pending : !kick_pending) {
-          (void)hipGraphDestroy(graph);
-          return fail(h, PFM_E_INVALID, "error: captured block does not end in its starting state");
-        }
-        pfm_ctx::GraphEntry e;
-        e.key = key;
-        e.after = graph_buffers(h);
-        HIPCHK(h, hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(graph);
-        if (h->graphs.size() >= GRAPH_CACHE) {
-          // least recently launched block goes
-          auto old = std::min_element(h->graphs.begin(), h->graphs.end(),
-                                      [](const pfm_ctx::GraphEntry& a, const pfm_ctx::GraphEntry& b) {
-                                        return a.used < b.used;
-                                      });
-          (void)hipGraphExecDestroy(old->exec);
-          h->graphs.erase(old);
-        }
-        e.used = ++h->graph_clock;
-        h->graphs.push_back(e);
-        ++h->graph_captures;
-        ++h->graph_launches;
-        // the capture ran the block's host side; its kernels run now
-        PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, s);
-        HIPCHK(h, hipGraphLaunch(h->graphs.back().exec, h->stream));
-        PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, 0);
-      } else {
-        g->used = ++h->graph_clock;
-        PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, s);
-        HIPCHK(h, hipGraphLaunch(g->exec, h->stream));
-        PFM_LAUNCH(h, "graph_base", k_set_base, 1, 1, h->derr, 0);
-        ++h->graph_launches;
-        // the host side of the block: buffers, time, energies carried over
-        set_graph_buffers(h, g->after);
-        *last_step = s + blk - 1;
-        for (int t = 0; t < blk; ++t) h->sim_time = h->sim_time + dt;
-        h->pe_prev = h->pe;
-        h->rep_pe_prev = h->rep_pe;
-      }
+      if ((rc = run_block(s, blk))) return rc;
       if (ne_b + nx_b > 0) {
         notes.clear();
         double tt = t_start;
@@ -1773,6 +1947,11 @@ int pfm_add_group_change(pfm_ctx* h, int from, int to, int ts1, int ts2, int fre
   if (frec < 1) return fail(h, PFM_E_INVALID, "error: change frequency must be >= 1");
   if (h->nrep > 1) return fail(h, PFM_E_INVALID, "error: group changes are for single systems only");
   h->changes.push_back(pfm_ctx::GroupChange{from, to, ts1, ts2, frec});
+  return PFM_OK;
+}
+
+int pfm_set_speculation(pfm_ctx* h, int on) {
+  h->speculate = on != 0;
   return PFM_OK;
 }
 
@@ -1997,6 +2176,7 @@ int pfm_destroy(pfm_ctx* h) {
   F(h->orig); F(h->origb); F(h->perm); F(h->atom_rank); F(h->scan_tmp); F(h->eb); F(h->qpos); F(h->rdev); F(h->rep); F(h->repb); F(h->cell_rep); F(h->d_rep_row0); F(h->d_rep_out);
   F(h->red_part); F(h->red_out); F(h->derr); F(h->cell_rank); F(h->cell_count); F(h->cell_start); F(h->cell_xyz);
   F(h->nhc_state);
+  F(h->spec_e); F(h->sp_pos); F(h->sp_vel); F(h->sp_frc); F(h->sp_mask); F(h->sp_orig);
   if (h->h_red) (void)hipHostFree(h->h_red);
   if (h->herr) (void)hipHostFree(h->herr);
   for (int g = 0; g <= PFM_MAXG; ++g) { F(h->grp.rows[g]); F(h->grp.row_of[g]); F(h->grp.loc[g]); F(h->grp.rank[g]); }

@@ -421,6 +421,14 @@ struct pfm_ctx {
   double pe = 0, pe_prev = 0, ke = 0, temp = 0, sim_time = 0;
   std::vector<double> e_nhc;
   bool sorted_once = false;
+  bool need_sort = false;  // the next step re-sorts and rebuilds every list (after a restore)
+  // nvms blocks (pfm_set_speculation): per-step energies and states of a block
+  bool speculate = true;
+  int spec_k = 0;                        // ring slots
+  double* spec_e = nullptr;              // device [spec_k][n_inter + 1]: interaction energies, ke
+  double4 *sp_pos = nullptr, *sp_vel = nullptr, *sp_frc = nullptr;  // [spec_k][N]
+  uint32_t* sp_mask = nullptr;
+  int* sp_orig = nullptr;
   // tuning
   int lanes = 8, block = 256;  // lanes per list row and workgroup size of the pair passes
   // captured steady-state blocks of steps (pfm_run)
