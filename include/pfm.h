@@ -131,6 +131,10 @@ int pfm_run(pfm_ctx* h, int integrator, double dt, int step_begin, int step_end,
 int pfm_energies(pfm_ctx* h, double* e_inter, double* pe, double* ke, double* temp,
                  double* e_nhc, double* sim_time);
 
+/* Potential-energy change of the last step entered, PE - PE_prev, the value
+ * md() prints after an nvms run (md_simulation.f90:246). */
+int pfm_pe_change(pfm_ctx* h, double* de);
+
 /* Copy positions/velocities/forces (double[n][3], original atom order) to the
  * host; any pointer may be NULL.  Used for snapshots, trajectories and final
  * xyz (md_read_write.f90:65-107). */

@@ -270,6 +270,11 @@ int orc_add_group_change(octx* h, int from, int to, int ts1, int ts2, int frec) 
   return PFM_OK;
 }
 
+int orc_pe_change(octx* h, double* de) {
+  *de = h->pe - h->pe_prev;
+  return PFM_OK;
+}
+
 int orc_group_count(octx* h, int gid, int* n) {
   if (gid < 0 || gid > MAXG) return oerr(h, PFM_E_INVALID, "error: group number out of range");
   *n = h->groups[gid].n;

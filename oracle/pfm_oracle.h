@@ -39,6 +39,7 @@ int  orc_get_list(octx* h, int inter, int k, int* n_rows, int* nnum, int* nlist,
 int  orc_get_normals(octx* h, int inter, double* gr_norm);
 int  orc_add_group_change(octx* h, int from, int to, int ts1, int ts2, int frec);
 int  orc_group_count(octx* h, int gid, int* n);
+int  orc_pe_change(octx* h, double* de);
 int  orc_last_error(octx* h, char* buf, int len);
 void orc_destroy(octx* h);
 void orc_set_threads(int n);

@@ -41,6 +41,7 @@ _SIGS = {
     "last_error": [C.c_void_p, C.c_char_p, C.c_int],
     "add_group_change": [C.c_void_p] + [C.c_int] * 5,
     "group_count": [C.c_void_p, C.c_int, _i],
+    "pe_change": [C.c_void_p, _d],
 }
 _libs = {}
 
@@ -208,6 +209,12 @@ class Context:
         n = C.c_int(0)
         self._check(self._fn("group_count")(self.h, int(gid), C.byref(n)))
         return n.value
+
+    def pe_change(self):
+        """PE - PE_prev of the last step entered (printed by md() after nvms)."""
+        d = C.c_double(0.0)
+        self._check(self._fn("pe_change")(self.h, C.byref(d)))
+        return d.value
 
     def add_nhc(self, group, temperature, M, q1):
         self._check(self._fn("add_nhc")(self.h, group, temperature, M, q1))

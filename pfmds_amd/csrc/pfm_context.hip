@@ -1950,6 +1950,11 @@ int pfm_add_group_change(pfm_ctx* h, int from, int to, int ts1, int ts2, int fre
   return PFM_OK;
 }
 
+int pfm_pe_change(pfm_ctx* h, double* de) {
+  *de = h->pe - h->pe_prev;
+  return PFM_OK;
+}
+
 int pfm_set_speculation(pfm_ctx* h, int on) {
   h->speculate = on != 0;
   return PFM_OK;

@@ -98,6 +98,11 @@ interface
     integer(c_int), value :: gid
     integer(c_int) :: n
   end function
+  integer(c_int) function pfm_pe_change(h, de) bind(C, name='pfm_pe_change')
+    import :: c_ptr, c_int, c_double
+    type(c_ptr), value :: h
+    real(c_double) :: de
+  end function
   integer(c_int) function pfm_destroy(h) bind(C, name='pfm_destroy')
     import :: c_ptr, c_int
     type(c_ptr), value :: h

@@ -302,7 +302,7 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
   character(len=32), allocatable :: iname(:), names(:,:), types(:)
   real(8), allocatable :: pos(:,:), vel(:,:), frc(:,:), mass(:), e_inter(:), e_nhc(:)
   real(8) :: box(3), ms_de, t0, nhc_t, nhc_q1, p(24), rc_(3), pe, ke, temp, sim_time, total
-  real(8) :: wall0, wall_md, wall_out, wtmp
+  real(8) :: wall0, wall_md, wall_out, wtmp, dpe
   integer(c_int) :: gn(6), nmax(3), per(3), eper(2), xper(2)
   logical :: newv, invert
   character(len=128) :: str, logfilename, init_xyz, pfile, fname
@@ -496,6 +496,10 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
   call system_clock_seconds(wtmp)
   wall_md = wtmp - wall_md
 
+  if (trim(iname_cur) == 'nvms') then
+    rc = pfm_pe_change(h, dpe)
+    write(out_id,*) 'potential energy difference: ', dpe
+  endif
   write(out_id,*) 'steps number:', md_step - 1
   write(out_id,*)
   write(out_id,*) 'PERFOMANCE:'

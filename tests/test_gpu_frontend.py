@@ -79,6 +79,12 @@ def test_frontend_matches_checker(tmp_path, case):
         np.testing.assert_allclose(g[3], list(w.e_inter) + list(w.e_chains), atol=2e-9, rtol=1e-9)
     steps_line = [ln for ln in p.stdout.splitlines() if "steps number:" in ln]
     assert steps_line and int(steps_line[-1].split(":")[-1]) == last - 1
+    # md_simulation.f90:246 after an nvms run
+    de_line = [ln for ln in p.stdout.splitlines() if "potential energy difference:" in ln]
+    if s.integrators[-1].name == "nvms":
+        assert de_line and float(de_line[-1].split(":")[-1]) == pytest.approx(o.pe_change(), abs=1e-7)
+    else:
+        assert not de_line
     fin = os.path.join(str(tmp_path), "final_" + s.init_xyz_filename)
     _, pf, vf, _, _ = system.read_xyz(fin)
     po, vo, _ = o.state()
