@@ -361,6 +361,7 @@ struct Groups {
   int* row_of[PFM_MAXG + 1] = {nullptr};  // device: internal atom -> row or -1
   int* loc[PFM_MAXG + 1] = {nullptr};     // device: original atom -> group-local index or -1
   bool used[PFM_MAXG + 1] = {false};
+  bool ident[PFM_MAXG + 1] = {false};  // identity rows written (a group of every atom)
 };
 
 struct pfm_ctx {

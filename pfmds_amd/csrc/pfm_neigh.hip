@@ -133,7 +133,8 @@ __global__ void __launch_bounds__(256) k_scan_apply(int n, const int* __restrict
 // exclusive scan of n ints; tmp holds ceil(n/1024) block sums and their scan
 void scan_excl(pfm_ctx* h, int n, const int* in, int* out, int* tmp) {
   int nb = (n + SCAN_B - 1) / SCAN_B;
-  if (nb <= 1) {
+  if (n <= 16 * 1024) {
+    // one workgroup, up to 16 items a thread: one launch instead of three
     PFM_LAUNCH(h, "scan", k_scan_excl, 1, 1024, n, in, out, (int*)nullptr);
     return;
   }
@@ -255,7 +256,9 @@ void launch_group_rows(pfm_ctx* h) {
     // members (a growing group's spare bit), not only the active atoms
     uint32_t bit = 1u << ((h->grp.shadow[g] ? h->grp.shadow[g] : g) - 1);
     if (h->grp.size[g] == N) {
-      PFM_LAUNCH(h, "group_rows", k_identity_rows, nb, 256, N, h->grp.rows[g], h->grp.row_of[g]);
+      // every atom is a member: row r is internal atom r whatever the order
+      if (!h->grp.ident[g]) PFM_LAUNCH(h, "group_rows", k_identity_rows, nb, 256, N, h->grp.rows[g], h->grp.row_of[g]);
+      h->grp.ident[g] = true;
       continue;
     }
     PFM_LAUNCH(h, "group_rows", k_flag, nb, 256, N, h->mask, bit, h->scan_tmp);
