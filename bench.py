@@ -221,6 +221,27 @@ def moire_fit(steps=2000, cells=((30, 17, 31, 18), (32, 18, 33, 19)), jobs=8):
     return out
 
 
+def nvms_phase(steps=500, warmup=20):
+    """The nvms relaxation that opens the graphene/Cu workload (94,560 atoms,
+    mixed precision): the exit test on the host every step, against nvms
+    blocks (pfm_set_speculation) that read energies back once per block."""
+    s = configs.graphene_on_cu()
+    dt = s.integrators[0].dt
+    out = {"atoms": s.n, "steps": steps}
+    for name, spec in (("per_step_test", False), ("blocks", True)):
+        ctx = gpu_ctx(s, capi.MIXED)
+        ctx.set_speculation(spec)
+        ctx.run("nvms", dt, 0, warmup, True, 1e-30)
+        t0 = time.perf_counter()
+        ctx.run("nvms", dt, warmup + 1, warmup + steps, True, 1e-30)
+        out[name + "_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / steps
+        ctx.close()
+    out["speedup"] = out["per_step_test_ms_per_step"] / out["blocks_ms_per_step"]
+    print(f"# nvms: {out['per_step_test_ms_per_step']:.4f} ms/step testing every step, "
+          f"{out['blocks_ms_per_step']:.4f} in blocks", file=sys.stderr, flush=True)
+    return out
+
+
 def surrogate_c4(train_iters=20000, frames_full=125, frames_sub=875, steps=10000, judge_every=100,
                  threshold=0.05, mixed_ms=None, fp64_ms=None, log=None):
     """Learned surrogate on graphene/Cu (94,560 atoms): frames from plain nvt
@@ -389,6 +410,7 @@ def main():
     ap.add_argument("--c4-small", action="store_true",
                     help="surrogate with 10 + 40 frames and a 2,000-step run (profiling)")
     ap.add_argument("--fit", action="store_true", help="time only the moire fit's first search step")
+    ap.add_argument("--nvms", action="store_true", help="time only the graphene/Cu nvms relaxation")
     ap.add_argument("--output-path", action="store_true",
                     help="time only the run with outputs: stopping at each output step vs output records")
     args = ap.parse_args()
@@ -399,6 +421,9 @@ def main():
         sys.exit(2)
     if args.c2_only:
         print(json.dumps({"graphene_cu_nvt": graphene_cu(args.steps, args.warmup)}))
+        return
+    if args.nvms:
+        print(json.dumps({"nvms_phase": nvms_phase(min(args.steps, 2000), args.warmup)}))
         return
     if args.fit:
         print(json.dumps({"moire_fit": moire_fit(args.steps)}))
