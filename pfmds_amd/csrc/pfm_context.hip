@@ -247,7 +247,7 @@ struct RecPartArgs {
   const uint32_t* mask;
   uint32_t ke_bit, all_bit;  // all_moving, all_atoms
   double* epart;             // [ni + 1][nb]: interaction energies, kinetic energy
-  double* opart;             // [11][nb]: the out block
+  double* opart;             // [11][nb]: the out block, or null for the energies alone
 };
 
 // Per-block partials of one energy record.  The energy rows and the kinetic
@@ -284,6 +284,7 @@ __global__ void __launch_bounds__(RED_THREADS) k_rec_part(RecPartArgs A) {
   }
   double s = block_sum(ke, sh);
   if (t == 0) A.epart[A.ni * nb + b] = s;
+  if (!A.opart) return;
   for (int q = 0; q < 10; ++q) {
     s = block_sum(a[q], sh);
     if (t == 0) A.opart[q * nb + b] = s;
@@ -868,19 +869,34 @@ static void nhc_first_half(pfm_ctx* h, double dt) {
   PFM_LAUNCH(h, "nhc", k_nhc_chain, 1, RED_THREADS, Q.nq, A, h->red_part, nb);
 }
 
+// per-block partials of the interaction energies and the all_moving kinetic
+// energy into red_part, and of the out block into `opart` unless it is null
+static void energy_parts(pfm_ctx* h, double* opart) {
+  RecPartArgs A;
+  memset(&A, 0, sizeof A);
+  A.ni = (int)h->inter.size();
+  A.N = h->N;
+  for (int k = 0; k < A.ni; ++k) {
+    A.e_row[k] = h->inter[k].e_row;
+    A.e_rows[k] = h->inter[k].e_rows;
+  }
+  A.pos = h->pos4;
+  A.vel = h->vel4;
+  A.frc = h->frc4;
+  A.mask = h->mask;
+  A.ke_bit = 1u << (h->roles[0] - 1);
+  A.all_bit = 1u << (h->roles[3] - 1);
+  A.epart = h->red_part;
+  A.opart = opart;
+  PFM_LAUNCH(h, "energy", k_rec_part, h->red_blocks, RED_THREADS, A);
+}
+
 // Energies of the step: per-interaction sums of the per-row energies and the
 // all_moving group's kinetic energy; one blocking read-back.
 static int energies(pfm_ctx* h, bool nvt) {
   int nb = h->red_blocks;
   int ni = (int)h->inter.size();
-  for (int k = 0; k < ni; ++k) {
-    Inter& it = h->inter[k];
-    PFM_LAUNCH(h, "energy", k_sum_part, nb, RED_THREADS, it.e_rows, it.e_row, h->red_part + (size_t)k * nb);
-  }
-  MaskQ Q;
-  Q.nq = 1;
-  Q.bit[0] = 1u << (h->roles[0] - 1);
-  PFM_LAUNCH(h, "energy", k_ke_part, nb, RED_THREADS, h->N, h->vel4, h->mask, Q, h->red_part + (size_t)ni * nb);
+  energy_parts(h, nullptr);
   PFM_LAUNCH(h, "energy", k_reduce_rows, 1, RED_THREADS, ni + 1, nb, h->red_part, h->red_out + 32);
   HIPCHK(h, hipMemcpyAsync(h->h_red, h->red_out, sizeof(double) * (32 + PFM_MAXI + 1), hipMemcpyDeviceToHost,
                            h->stream));
@@ -978,14 +994,7 @@ static int spec_alloc(pfm_ctx* h, int k) {
 // the energies of the step into slot k (the sums of energies()) and its state
 static void spec_record(pfm_ctx* h, int k) {
   const int nb = h->red_blocks, ni = (int)h->inter.size(), N = h->N;
-  for (int q = 0; q < ni; ++q) {
-    Inter& it = h->inter[q];
-    PFM_LAUNCH(h, "energy", k_sum_part, nb, RED_THREADS, it.e_rows, it.e_row, h->red_part + (size_t)q * nb);
-  }
-  MaskQ Q;
-  Q.nq = 1;
-  Q.bit[0] = 1u << (h->roles[0] - 1);
-  PFM_LAUNCH(h, "energy", k_ke_part, nb, RED_THREADS, N, h->vel4, h->mask, Q, h->red_part + (size_t)ni * nb);
+  energy_parts(h, nullptr);
   PFM_LAUNCH(h, "energy", k_reduce_rows, 1, RED_THREADS, ni + 1, nb, h->red_part, h->spec_e + (size_t)k * (ni + 1));
   const size_t o = (size_t)k * N;
   PFM_LAUNCH(h, "spec", k_spec_copy, (N + 255) / 256, 256, N, 0, h->pos4, h->vel4, h->frc4, h->mask, h->orig,
@@ -1178,25 +1187,7 @@ static int rec_prepare(pfm_ctx* h) {
 static void rec_launch(pfm_ctx* h, bool e, bool x) {
   auto& R = h->rec;
   const int nb = h->red_blocks, ni = (int)h->inter.size();
-  if (e) {
-    RecPartArgs A;
-    memset(&A, 0, sizeof A);
-    A.ni = ni;
-    A.N = h->N;
-    for (int k = 0; k < ni; ++k) {
-      A.e_row[k] = h->inter[k].e_row;
-      A.e_rows[k] = h->inter[k].e_rows;
-    }
-    A.pos = h->pos4;
-    A.vel = h->vel4;
-    A.frc = h->frc4;
-    A.mask = h->mask;
-    A.ke_bit = 1u << (h->roles[0] - 1);
-    A.all_bit = 1u << (h->roles[3] - 1);
-    A.epart = h->red_part;
-    A.opart = R.d_part;
-    PFM_LAUNCH(h, "record", k_rec_part, nb, RED_THREADS, A);
-  }
+  if (e) energy_parts(h, R.d_part);
   if (x) {
     const size_t n2 = (size_t)3 * h->N;
     PFM_LAUNCH(h, "record", k_rec_frame, (h->N + 255) / 256, 256, h->N, h->pos4, h->vel4, h->orig, R.d_x);
