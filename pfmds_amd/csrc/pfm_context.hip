@@ -467,15 +467,23 @@ struct ChainArgs {
 __global__ void __launch_bounds__(RED_THREADS) k_nhc_chain(int nchain, ChainArgs A, const double* __restrict__ part,
                                                            int nb) {
   __shared__ double sh[RED_THREADS];
+  // the chain's serial update reads and writes its state many times over:
+  // work on a copy in LDS, loaded and stored by the whole block at once
+  __shared__ NhcDev st;
+  constexpr int W = sizeof(NhcDev) / sizeof(double);
   for (int c = 0; c < nchain; ++c) {
+    for (int j = threadIdx.x; j < W; j += RED_THREADS) ((double*)&st)[j] = ((const double*)&A.C[c])[j];
     double a = 0.0;
     for (int b = threadIdx.x; b < nb; b += RED_THREADS) a += part[c * nb + b];
-    double ke = block_sum(a, sh);
+    double ke = block_sum(a, sh);  // ends with a barrier: the copy is complete
     if (threadIdx.x == 0) {
-      double s = nhc_update(A.C[c], ke, A.ts1, A.ts2, A.ts3);
-      if (A.twice) s *= nhc_update(A.C[c], s * s * ke, A.ts1, A.ts2, A.ts3);
+      double s = nhc_update(st, ke, A.ts1, A.ts2, A.ts3);
+      if (A.twice) s *= nhc_update(st, s * s * ke, A.ts1, A.ts2, A.ts3);
       A.s_out[c] = s;
     }
+    __syncthreads();
+    for (int j = threadIdx.x; j < W; j += RED_THREADS) ((double*)&A.C[c])[j] = ((const double*)&st)[j];
+    __syncthreads();
   }
 }
 
