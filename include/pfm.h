@@ -140,9 +140,19 @@ int pfm_pe_change(pfm_ctx* h, double* de);
  * xyz (md_read_write.f90:65-107). */
 int pfm_get_state(pfm_ctx* h, double* pos, double* vel, double* force);
 
-/* Replace positions and velocities (double[n][3]); forces are recomputed at
- * the next step. */
+/* Replace positions and velocities (double[n][3]).  The next pfm_run re-sorts
+ * the atoms, rebuilds every list and, unless it starts at step 0 (which
+ * computes forces anyway), evaluates the forces at the new positions before
+ * its first half kick. */
 int pfm_set_state(pfm_ctx* h, const double* pos, const double* vel);
+
+/* Positions x[M] and velocities v[M] of Nose-Hoover chain c (1-based, in
+ * pfm_add_nhc order): the `nose_hoover_chain` state (md_general.f90:24-28)
+ * that md() carries from step to step.  pfm_set_chain takes them over (after
+ * the first pfm_run), e.g. to continue a run in another handle; the chain's
+ * masses stay those of pfm_add_nhc. */
+int pfm_get_chain(pfm_ctx* h, int c, double* x, double* v);
+int pfm_set_chain(pfm_ctx* h, int c, const double* x, const double* v);
 
 /* Download neighbour list k (1-based) of interaction inter (1-based) in the
  * reference layout of type neighbour_list (md_general.f90:30-35): n_rows,

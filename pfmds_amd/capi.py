@@ -100,6 +100,10 @@ def load_library(path, prefix):
         lib.pfm_sur_descriptors.restype = C.c_int
         lib.pfm_sur_loss_grad.argtypes = [C.c_void_p, C.c_int, _d, C.c_double, C.c_double, C.c_double, _d, _d]
         lib.pfm_sur_loss_grad.restype = C.c_int
+        lib.pfm_get_chain.argtypes = [C.c_void_p, C.c_int, _d, _d]
+        lib.pfm_get_chain.restype = C.c_int
+        lib.pfm_set_chain.argtypes = [C.c_void_p, C.c_int, _d, _d]
+        lib.pfm_set_chain.restype = C.c_int
         lib.pfm_set_speculation.argtypes = [C.c_void_p, C.c_int]
         lib.pfm_set_speculation.restype = C.c_int
         lib.pfm_set_records.argtypes = [C.c_void_p, C.c_int, _i, C.c_int, _i, C.c_int, RECORD_FN, C.c_void_p]
@@ -131,6 +135,7 @@ class Context:
         mass = np.ascontiguousarray(mass, dtype=np.float64)
         self.n_inter = 0
         self.n_nhc = 0
+        self.nhc_M = []
         self.n_rep = 1
         self._rec_fn = None
         self._rec_exc = None
@@ -219,6 +224,7 @@ class Context:
     def add_nhc(self, group, temperature, M, q1):
         self._check(self._fn("add_nhc")(self.h, group, temperature, M, q1))
         self.n_nhc += 1
+        self.nhc_M.append(int(M))
 
     # -- stepping ----------------------------------------------------------
     def run(self, integrator, dt, step_begin, step_end, energy_at_end=True, ms_de=0.0):
@@ -333,6 +339,17 @@ class Context:
         self._check(self.lib.pfm_set_records(self.h, len(energy_periods), _ip(ep), len(frame_periods), _ip(xp),
                                              int(frame_slots), fn, None))
         self._rec_fn = fn
+
+    def get_chain(self, c, M):
+        """(x[M], v[M]) of Nose-Hoover chain c (1-based)."""
+        x, v = np.zeros(max(M, 1)), np.zeros(max(M, 1))
+        self._check(self.lib.pfm_get_chain(self.h, int(c), _dp(x), _dp(v)))
+        return x[:M], v[:M]
+
+    def set_chain(self, c, x, v):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._check(self.lib.pfm_set_chain(self.h, int(c), _dp(x), _dp(v)))
 
     def set_speculation(self, on):
         """nvms blocks with one energy read-back per block (pfm_set_speculation)."""

@@ -1581,6 +1581,13 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
              (s0 == 0 && h->invert_z) ? 1 : 0, h->vel4, P.zlo, P.zhi, h->rep, h->rdev);
   if ((rc = check_err(h, s1))) return rc;
   if (h->qpos) PFM_LAUNCH(h, "quantize", k_quantize, nb, 256, N, h->pos4, h->qpos, h->qscale, h->rep, h->rdev);
+  if (h->need_forces && s0 != 0) {
+    // positions set from the host: the first half kick needs their forces
+    if ((rc = update_lists(h, s0))) return rc;
+    if (!first_writes) PFM_LAUNCH(h, "zero_forces", k_zero_masked, nb, 256, N, h->frc4, h->mask, all_bit);
+    if ((rc = launch_forces(h, false, first_writes))) return rc;
+  }
+  h->need_forces = false;
   const bool nvt = integ == PFM_NVT && !h->nhc.empty();
   bool nhc_pending = false;
   bool kick_pending = false;  // second half kick deferred into the next kick_drift
@@ -2042,6 +2049,36 @@ int pfm_set_state(pfm_ctx* h, const double* pos, const double* vel) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   for (auto& it : h->inter)
     for (int k = 0; k < it.nl_n; ++k) it.nl[k].built = false;
+  // the cells of the last sort do not hold the new positions, nor do the forces
+  if (h->sorted_once) h->need_sort = true;
+  h->need_forces = true;
+  return PFM_OK;
+}
+
+int pfm_get_chain(pfm_ctx* h, int c, double* x, double* v) {
+  if (c < 1 || c > (int)h->nhc.size()) return fail(h, PFM_E_INVALID, "error: chain number out of range");
+  if (!h->nhc_state) {
+    // before the first run: the state set up by pfm_add_nhc
+    for (int i = 0; i < h->nhc[c - 1].M; ++i) { x[i] = 0.0; v[i] = 0.0; }
+    return PFM_OK;
+  }
+  NhcDev d;
+  HIPCHK(h, hipMemcpyAsync(&d, (NhcDev*)h->nhc_state + (c - 1), sizeof d, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int i = 0; i < d.M; ++i) { x[i] = d.x[i]; v[i] = d.v[i]; }
+  return PFM_OK;
+}
+
+int pfm_set_chain(pfm_ctx* h, int c, const double* x, const double* v) {
+  if (c < 1 || c > (int)h->nhc.size()) return fail(h, PFM_E_INVALID, "error: chain number out of range");
+  if (!h->nhc_state) return fail(h, PFM_E_INVALID, "error: chain states are set after the first pfm_run");
+  NhcDev d;
+  NhcDev* dst = (NhcDev*)h->nhc_state + (c - 1);
+  HIPCHK(h, hipMemcpyAsync(&d, dst, sizeof d, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int i = 0; i < d.M; ++i) { d.x[i] = x[i]; d.v[i] = v[i]; }
+  HIPCHK(h, hipMemcpyAsync(dst, &d, sizeof d, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return PFM_OK;
 }
 

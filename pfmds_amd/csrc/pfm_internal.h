@@ -423,6 +423,7 @@ struct pfm_ctx {
   std::vector<double> e_nhc;
   bool sorted_once = false;
   bool need_sort = false;  // the next step re-sorts and rebuilds every list (after a restore)
+  bool need_forces = false;  // forces are stale: positions were set from the host
   // nvms blocks (pfm_set_speculation): per-step energies and states of a block
   bool speculate = true;
   int spec_k = 0;                        // ring slots

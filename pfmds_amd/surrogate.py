@@ -233,8 +233,8 @@ class JudgedRun:
     """Surrogate MD with the plain potential as judge.  `sur_ctx` steps the
     system; every `judge_every` steps `plain_ctx` evaluates the plain forces
     at the same positions, and the force RMSE over the moving atoms is
-    recorded; past `threshold` eV/A the run moves to `plain_ctx` (positions
-    and velocities; the Nose-Hoover chain restarts from rest there)."""
+    recorded; past `threshold` eV/A the run moves to `plain_ctx` (positions,
+    velocities and the Nose-Hoover chain states)."""
 
     def __init__(self, sur_ctx, plain_ctx, moving, judge_every=100, threshold=0.05):
         self.sur, self.plain = sur_ctx, plain_ctx
@@ -258,6 +258,8 @@ class JudgedRun:
         if rmse > self.threshold:
             self.on_plain = True
             self.switch_step = step
+            for c, M in enumerate(self.sur.nhc_M, 1):
+                self.plain.set_chain(c, *self.sur.get_chain(c, M))
         return rmse
 
     def run(self, integrator, dt, first, last):

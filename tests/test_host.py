@@ -37,7 +37,8 @@ def test_oracle_mirrors_the_abi():
         # (pinned against GPU runs stopped at every recorded step) and the
         # learned surrogate (pinned against tests/sur_ref.py)
         if s in ("pfm_set_timing", "pfm_kernel_time", "pfm_set_tuning", "pfm_set_graphs", "pfm_graph_count",
-                 "pfm_create_batch", "pfm_replica_energies", "pfm_set_records", "pfm_set_speculation") \
+                 "pfm_create_batch", "pfm_replica_energies", "pfm_set_records", "pfm_set_speculation",
+                 "pfm_get_chain", "pfm_set_chain") \
                 or s.startswith("pfm_sur_"):
             continue
         assert hasattr(lib, s.replace("pfm_", "orc_")), s
