@@ -291,7 +291,7 @@ def surrogate_c4(train_iters=20000, frames_full=125, frames_sub=875, steps=10000
     sc = sg.surrogate_context(sur_s, theta)
     pc = gpu_ctx(s, capi.MIXED)
     moving = s.group_indices(s.all_moving) - 1
-    jr = sg.JudgedRun(sc, pc, moving, judge_every=judge_every, threshold=threshold)
+    jr = sg.JudgedRun(sc, pc, moving, judge_every=judge_every, threshold=threshold, group=s.all_moving)
     jr.run("nvt", dt, 0, 20)
     sc.run("nvt", dt, 21, 21, False)
     t0 = time.perf_counter()

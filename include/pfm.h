@@ -146,6 +146,16 @@ int pfm_get_state(pfm_ctx* h, double* pos, double* vel, double* force);
  * its first half kick. */
 int pfm_set_state(pfm_ctx* h, const double* pos, const double* vel);
 
+/* Device-side transfers between two handles of the same atoms on one GPU.
+ * pfm_copy_state: dst takes src's positions and velocities, as
+ * pfm_set_state would take them from the host.  pfm_force_rmse: the RMSE
+ * (eV/A) over the 3n force components of group `group` (of handle b) between
+ * a's and b's forces.  They serve a run that checks one potential against
+ * another at the same positions (the surrogate's judge).  No counterpart in
+ * the reference. */
+int pfm_copy_state(pfm_ctx* dst, pfm_ctx* src);
+int pfm_force_rmse(pfm_ctx* a, pfm_ctx* b, int group, double* rmse);
+
 /* Positions x[M] and velocities v[M] of Nose-Hoover chain c (1-based, in
  * pfm_add_nhc order): the `nose_hoover_chain` state (md_general.f90:24-28)
  * that md() carries from step to step.  pfm_set_chain takes them over (after

@@ -100,6 +100,10 @@ def load_library(path, prefix):
         lib.pfm_sur_descriptors.restype = C.c_int
         lib.pfm_sur_loss_grad.argtypes = [C.c_void_p, C.c_int, _d, C.c_double, C.c_double, C.c_double, _d, _d]
         lib.pfm_sur_loss_grad.restype = C.c_int
+        lib.pfm_copy_state.argtypes = [C.c_void_p, C.c_void_p]
+        lib.pfm_copy_state.restype = C.c_int
+        lib.pfm_force_rmse.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _d]
+        lib.pfm_force_rmse.restype = C.c_int
         lib.pfm_get_chain.argtypes = [C.c_void_p, C.c_int, _d, _d]
         lib.pfm_get_chain.restype = C.c_int
         lib.pfm_set_chain.argtypes = [C.c_void_p, C.c_int, _d, _d]
@@ -339,6 +343,17 @@ class Context:
         self._check(self.lib.pfm_set_records(self.h, len(energy_periods), _ip(ep), len(frame_periods), _ip(xp),
                                              int(frame_slots), fn, None))
         self._rec_fn = fn
+
+    def copy_state_from(self, src):
+        """Positions and velocities of `src` (same atoms, same GPU), device to device."""
+        self._check(self.lib.pfm_copy_state(self.h, src.h))
+
+    def force_rmse(self, other, group):
+        """RMSE (eV/A) between `other`'s forces and this handle's over the 3n
+        force components of this handle's group `group`."""
+        r = C.c_double(0.0)
+        self._check(self.lib.pfm_force_rmse(other.h, self.h, int(group), C.byref(r)))
+        return r.value
 
     def get_chain(self, c, M):
         """(x[M], v[M]) of Nose-Hoover chain c (1-based)."""

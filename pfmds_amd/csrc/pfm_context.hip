@@ -986,6 +986,7 @@ __global__ void k_spec_copy(int N, int back, double4* __restrict__ pos, double4*
 static int spec_alloc(pfm_ctx* h, int k) {
   if (h->spec_k >= k) return PFM_OK;
   auto F = [](void* p) { if (p) (void)hipFree(p); };
+  F(h->xfer);
   F(h->spec_e); F(h->sp_pos); F(h->sp_vel); F(h->sp_frc); F(h->sp_mask); F(h->sp_orig);
   h->spec_e = nullptr; h->sp_pos = h->sp_vel = h->sp_frc = nullptr; h->sp_mask = nullptr; h->sp_orig = nullptr;
   h->spec_k = 0;
@@ -2055,6 +2056,94 @@ int pfm_set_state(pfm_ctx* h, const double* pos, const double* vel) {
   return PFM_OK;
 }
 
+// atom data into original order: out[orig[i]] = in[i] (two arrays at once)
+__global__ void k_to_orig(int N, const int* __restrict__ orig, const double4* __restrict__ a,
+                          const double4* __restrict__ b, double4* __restrict__ oa, double4* __restrict__ ob) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int o = orig[i];
+  oa[o] = a[i];
+  ob[o] = b[i];
+}
+
+// positions and velocities (masses kept) from original order into the handle's order
+__global__ void k_from_orig(int N, const int* __restrict__ orig, const double4* __restrict__ pa,
+                            const double4* __restrict__ va, double4* __restrict__ pos, double4* __restrict__ vel) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int o = orig[i];
+  const double4 p = pa[o], v = va[o];
+  double4 q = pos[i], w = vel[i];
+  q.x = p.x; q.y = p.y; q.z = p.z;
+  w.x = v.x; w.y = v.y; w.z = v.z;
+  pos[i] = q;
+  vel[i] = w;
+}
+
+// per-block partial sums of |fa - fb|^2 over the atoms of the group bit in
+// handle b's order, fa given in original order
+__global__ void __launch_bounds__(RED_THREADS) k_force_dev(int N, const int* __restrict__ orig,
+                                                           const double4* __restrict__ fa_orig,
+                                                           const double4* __restrict__ fb,
+                                                           const uint32_t* __restrict__ mask, uint32_t bit,
+                                                           double* __restrict__ part) {
+  __shared__ double sh[RED_THREADS];
+  double a = 0.0;
+  for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
+    if (!(mask[i] & bit)) continue;
+    const double4 x = fa_orig[orig[i]], y = fb[i];
+    const double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
+    a += dx * dx + dy * dy + dz * dz;
+  }
+  double s = block_sum(a, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// the staging of handle h, written next on another handle's stream: h's own
+// stream (its clearing of the new buffer, earlier readers) is finished first
+static int xfer_ready(pfm_ctx* h) {
+  int rc;
+  if (!h->xfer && (rc = dalloc(h, &h->xfer, 2 * (size_t)h->N))) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFM_OK;
+}
+
+int pfm_copy_state(pfm_ctx* dst, pfm_ctx* src) {
+  if (dst->N != src->N || dst->dev != src->dev)
+    return fail(dst, PFM_E_INVALID, "error: state copies need handles of one size on one device");
+  int rc;
+  if ((rc = xfer_ready(dst))) return rc;
+  const int N = src->N, nb = (N + 255) / 256;
+  // src's positions and velocities in original order, written on src's stream
+  PFM_LAUNCH(src, "xfer", k_to_orig, nb, 256, N, src->orig, src->pos4, src->vel4, dst->xfer, dst->xfer + N);
+  HIPCHK(src, hipStreamSynchronize(src->stream));
+  PFM_LAUNCH(dst, "xfer", k_from_orig, nb, 256, N, dst->orig, dst->xfer, dst->xfer + N, dst->pos4, dst->vel4);
+  for (auto& it : dst->inter)
+    for (int k = 0; k < it.nl_n; ++k) it.nl[k].built = false;
+  if (dst->sorted_once) dst->need_sort = true;
+  dst->need_forces = true;
+  return PFM_OK;
+}
+
+int pfm_force_rmse(pfm_ctx* a, pfm_ctx* b, int group, double* rmse) {
+  if (a->N != b->N || a->dev != b->dev)
+    return fail(b, PFM_E_INVALID, "error: force comparisons need handles of one size on one device");
+  if (group < 1 || group > PFM_MAXG || b->grp.n[group] == 0)
+    return fail(b, PFM_E_INVALID, "error: force comparison over an empty or unknown group");
+  int rc;
+  if ((rc = xfer_ready(b))) return rc;
+  const int N = a->N, nb = (N + 255) / 256;
+  PFM_LAUNCH(a, "xfer", k_to_orig, nb, 256, N, a->orig, a->frc4, a->frc4, b->xfer, b->xfer + N);
+  HIPCHK(a, hipStreamSynchronize(a->stream));
+  PFM_LAUNCH(b, "xfer", k_force_dev, b->red_blocks, RED_THREADS, N, b->orig, b->xfer, b->frc4, b->mask,
+             1u << (group - 1), b->red_part);
+  PFM_LAUNCH(b, "xfer", k_reduce_rows, 1, RED_THREADS, 1, b->red_blocks, b->red_part, b->red_out + 60);
+  HIPCHK(b, hipMemcpyAsync(b->h_red + 60, b->red_out + 60, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(b, hipStreamSynchronize(b->stream));
+  *rmse = std::sqrt(b->h_red[60] / (3.0 * b->grp.n[group]));
+  return PFM_OK;
+}
+
 int pfm_get_chain(pfm_ctx* h, int c, double* x, double* v) {
   if (c < 1 || c > (int)h->nhc.size()) return fail(h, PFM_E_INVALID, "error: chain number out of range");
   if (!h->nhc_state) {
@@ -2217,6 +2306,7 @@ int pfm_destroy(pfm_ctx* h) {
   F(h->orig); F(h->origb); F(h->perm); F(h->atom_rank); F(h->scan_tmp); F(h->eb); F(h->qpos); F(h->rdev); F(h->rep); F(h->repb); F(h->cell_rep); F(h->d_rep_row0); F(h->d_rep_out);
   F(h->red_part); F(h->red_out); F(h->derr); F(h->cell_rank); F(h->cell_count); F(h->cell_start); F(h->cell_xyz);
   F(h->nhc_state);
+  F(h->xfer);
   F(h->spec_e); F(h->sp_pos); F(h->sp_vel); F(h->sp_frc); F(h->sp_mask); F(h->sp_orig);
   if (h->h_red) (void)hipHostFree(h->h_red);
   if (h->herr) (void)hipHostFree(h->herr);

@@ -424,6 +424,7 @@ struct pfm_ctx {
   bool sorted_once = false;
   bool need_sort = false;  // the next step re-sorts and rebuilds every list (after a restore)
   bool need_forces = false;  // forces are stale: positions were set from the host
+  double4* xfer = nullptr;   // [2N] staging in original order (pfm_copy_state, pfm_force_rmse)
   // nvms blocks (pfm_set_speculation): per-step energies and states of a block
   bool speculate = true;
   int spec_k = 0;                        // ring slots

@@ -234,10 +234,13 @@ class JudgedRun:
     system; every `judge_every` steps `plain_ctx` evaluates the plain forces
     at the same positions, and the force RMSE over the moving atoms is
     recorded; past `threshold` eV/A the run moves to `plain_ctx` (positions,
-    velocities and the Nose-Hoover chain states)."""
+    velocities and the Nose-Hoover chain states).  With `group` (the group
+    number of the moving atoms) the state copy and the RMSE stay on the GPU
+    (pfm_copy_state, pfm_force_rmse); without it they go through the host."""
 
-    def __init__(self, sur_ctx, plain_ctx, moving, judge_every=100, threshold=0.05):
+    def __init__(self, sur_ctx, plain_ctx, moving, judge_every=100, threshold=0.05, group=None):
         self.sur, self.plain = sur_ctx, plain_ctx
+        self.group = group
         self.moving = np.asarray(moving)
         self.judge_every, self.threshold = judge_every, threshold
         self.on_plain = False
@@ -247,12 +250,17 @@ class JudgedRun:
 
     def judge(self, step):
         t0 = time.perf_counter()
-        p, v, f = self.sur.state()
-        self.plain.set_state(p, v)
-        self.plain.run("nvt", 0.5, 0, 0, False)
-        fp = self.plain.state()[2]
-        d = (f - fp)[self.moving]
-        rmse = float(np.sqrt(np.mean(d * d)))
+        if self.group is not None:
+            self.plain.copy_state_from(self.sur)
+            self.plain.run("nvt", 0.5, 0, 0, False)
+            rmse = self.plain.force_rmse(self.sur, self.group)
+        else:
+            p, v, f = self.sur.state()
+            self.plain.set_state(p, v)
+            self.plain.run("nvt", 0.5, 0, 0, False)
+            fp = self.plain.state()[2]
+            d = (f - fp)[self.moving]
+            rmse = float(np.sqrt(np.mean(d * d)))
         self.judged.append((step, rmse))
         self.t_judge += time.perf_counter() - t0
         if rmse > self.threshold:

@@ -189,17 +189,22 @@ def test_judge_switches_to_plain_potential():
     # forces stay far from the plain ones
     theta = theta_for(G, seed=7, scale=0.3)
     runs = []
-    for thr in (1e-3, 1e6):
+    for thr, group in ((1e-3, None), (1e6, None), (1e-3, s.all_moving), (1e6, s.all_moving)):
         sc = surrogate.surrogate_context(sur_s, theta)
         pc = system.build_context(s, lambda b, p, v, m: capi.Context.gpu(b, p, v, m, capi.MIXED))
         moving = s.group_indices(s.all_moving) - 1
-        jr = surrogate.JudgedRun(sc, pc, moving, judge_every=10, threshold=thr)
+        jr = surrogate.JudgedRun(sc, pc, moving, judge_every=10, threshold=thr, group=group)
         end = jr.run("nvt", 0.5, 0, 35)
         runs.append(jr)
-        print(f"[judge] threshold {thr}: judged {jr.judged} switch {jr.switch_step}", flush=True)
+        print(f"[judge] threshold {thr} group {group}: judged {jr.judged} switch {jr.switch_step}", flush=True)
         assert [j[0] for j in jr.judged] == ([10] if thr < 1 else [10, 20, 30])
         assert (end is pc) == (thr < 1)
     assert runs[0].switch_step == 10 and runs[1].switch_step is None
+    # the judge on the GPU (state copy and RMSE) matches the one through the host
+    for a, b in ((runs[0], runs[2]), (runs[1], runs[3])):
+        assert a.switch_step == b.switch_step
+        for (sa, ra), (sb, rb) in zip(a.judged, b.judged):
+            assert sa == sb and rb == pytest.approx(ra, rel=1e-12)
 
 
 def test_surrogate_runs_are_bit_identical():
