@@ -763,7 +763,8 @@ static int finalize(pfm_ctx* h) {
     }
     if (is_cos(it.kind)) {
       if ((rc = dalloc(h, &it.gnorm, h->N))) return rc;
-      if ((rc = dalloc(h, &it.gS, h->N))) return rc;
+      if ((rc = dalloc(h, &it.gC, 3 * (size_t)h->N))) return rc;
+      if ((rc = dalloc(h, &it.gnf, h->N))) return rc;
       std::vector<char> in1(h->N, 0);
       for (int x : h->grp.idx[it.nl[0].g1]) in1[x] = 1;
       it.rows_disjoint = true;
@@ -2313,7 +2314,7 @@ int pfm_destroy(pfm_ctx* h) {
   for (int g = 0; g <= PFM_MAXG; ++g) { F(h->grp.rows[g]); F(h->grp.row_of[g]); F(h->grp.loc[g]); F(h->grp.rank[g]); }
   for (auto& it : h->inter) {
     for (int k = 0; k < 3; ++k) { F(it.nl[k].nnum); F(it.nl[k].nbr); F(it.nl[k].rev); }
-    F(it.e_row); F(it.lc); F(it.B); F(it.Bp); F(it.gnorm); F(it.gS);
+    F(it.e_row); F(it.lc); F(it.B); F(it.Bp); F(it.gnorm); F(it.gC); F(it.gnf);
     sur_free(it);
   }
   for (auto e : h->evpool) (void)hipEventDestroy(e);

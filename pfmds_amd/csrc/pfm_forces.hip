@@ -507,15 +507,53 @@ __device__ __forceinline__ void cos_pair(const CosP& P, R r, R nd, R nnorm_r, R&
   S_coef = V2 * V3 * fc * ind;
 }
 
-// graphene side, pair and normal terms (*_forces_for_graphene first loop);
-// also S_i = sum V2 V3 fc/(n.dr) dr for the normal-derivative pass
+// Coefficients of the normal-derivative term (*_forces_for_graphene second
+// loop, MorseCosine.f90:81-107), per graphene atom j.  For the l-th of j's
+// three nearest neighbours i, the reference's sum over j's metal neighbours p
+// of  c V2 V3 fc /(den (n_j.dr_jp)) (dr_jp . W) n_j  is linear in dr_jp, so
+// it equals  c/den (S_j . W) n_j  with S_j = sum_p V2 V3 fc/(n_j.dr_jp) dr_jp.
+// Lanes l = t, t+T, .. of row j store that vector for i = its l-th nearest
+// neighbour; the geometry is FP64 from the FP64 positions in both precisions.
+template <int T>
+__device__ __forceinline__ void cos_nn_coef(const RowArgs& a, const CosP& P, const int* __restrict__ nn, int row,
+                                            int i, int t, double sx, double sy, double sz, const double4& g,
+                                            double4* __restrict__ gC) {
+  if (t >= 3) return;
+  const Geo<double> G = geo_of<double>(a, i);
+  const double4 pj = G.at(i);
+  double d[3][3];
+  for (int k = 0; k < 3; ++k) G.d2(pj, G.at(nn[(size_t)row * 3 + k]), d[k][0], d[k][1], d[k][2]);
+  for (int l1 = t; l1 < 3; l1 += T) {
+    int l2 = (l1 + 1) % 3, l3 = (l1 + 2) % 3;
+    double d12[3], d31[3], d23[3];
+    for (int k = 0; k < 3; ++k) {
+      d12[k] = d[l2][k] - d[l1][k];
+      d31[k] = d[l1][k] - d[l3][k];
+      d23[k] = d[l3][k] - d[l2][k];
+    }
+    double s1212 = d12[0] * d12[0] + d12[1] * d12[1] + d12[2] * d12[2];
+    double s3131 = d31[0] * d31[0] + d31[1] * d31[1] + d31[2] * d31[2];
+    double s1231 = d12[0] * d31[0] + d12[1] * d31[1] + d12[2] * d31[2];
+    double s2331 = d23[0] * d31[0] + d23[1] * d31[1] + d23[2] * d31[2];
+    double s2312 = d23[0] * d12[0] + d23[1] * d12[1] + d23[2] * d12[2];
+    double den = s1212 * s3131 - s1231 * s1231;
+    double w = sx * (d12[0] * s2331 - d31[0] * s2312) + sy * (d12[1] * s2331 - d31[1] * s2312) +
+               sz * (d12[2] * s2331 - d31[2] * s2312);
+    double c = P.nn_coef * w / den;
+    gC[(size_t)i * 3 + l1] = make_double4(c * g.x, c * g.y, c * g.z, 0);
+  }
+}
+
+// graphene side, pair and normal terms (*_forces_for_graphene first loop).
 // The row's own normal (graphenenorm.f90:38-56) is formed here from its three
 // nearest neighbours (rows of nl(3) are the rows of nl(1)) and stored for the
-// normal-derivative and metal passes.
+// metal pass (gnorm, and gnf in FP32 for the mixed path); with nearest
+// neighbours, the row's normal-derivative coefficients go to gC.
 template <int T, class R, bool E>
 __global__ void __launch_bounds__(PAIR_BLOCK) k_cos_graphene(RowArgs a, CosP P, const int* __restrict__ nn,
                                                              int simplified, double4* __restrict__ gnorm,
-                                                             double4* __restrict__ gS, int nb_real) {
+                                                             float4* __restrict__ gnf, double4* __restrict__ gC,
+                                                             int nb_real) {
   ROW_PROLOGUE(T)
   double4 g;
   if (simplified) {
@@ -547,73 +585,68 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_cos_graphene(RowArgs a, CosP P, 
     if (E) e += in ? (double)(ep * pref) : 0.0;
   });
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
-  sx = group_sum<T>(sx); sy = group_sum<T>(sy); sz = group_sum<T>(sz);
   if (E) e = group_sum<T>(e);
+  if (!simplified) {
+    sx = group_sum<T>(sx); sy = group_sum<T>(sy); sz = group_sum<T>(sz);
+    cos_nn_coef<T>(a, P, nn, row, i, t, sx, sy, sz, g, gC);
+  }
   if (t == 0) {
     write_force(a, i, fx, fy, fz);
-    gS[i] = make_double4(sx, sy, sz, 0);
     gnorm[i] = g;
+    if (sizeof(R) == 4) gnf[i] = make_float4((float)g.x, (float)g.y, (float)g.z, 0.f);
     if (E) a.e_row[row] = e;
   }
 }
 
-// normal-derivative term (*_forces_for_graphene second loop): for each of
-// the three nearest neighbours j of i, the sum over j's metal neighbours p of
-//   c * V2 V3 fc /(den (n_j.dr_jp)) (dr_jp . W) n_j
-// equals c/den (S_j . W) n_j with S_j from the graphene pass.
-__device__ __forceinline__ void cos_nn_rows(const RowArgs& a, const CosP& P, const int* __restrict__ row_of,
-                                            const int* __restrict__ rev, const double4* __restrict__ gnorm,
-                                            const double4* __restrict__ gS, int block) {
-  using R = double;  // FP64 in both precisions: three entries per graphene atom
-  ROW_PROLOGUE_AT(1, block)
-  (void)t; (void)pi; (void)G;
+// normal-derivative rows (*_forces_for_graphene second loop): row i adds, for
+// each of its three nearest neighbours j, the vector that j's graphene pass
+// stored for i's place l in j's nearest-neighbour row (cos_nn_coef)
+__device__ __forceinline__ void cos_nn_rows(const RowArgs& a, const int* __restrict__ row_of,
+                                            const int* __restrict__ rev, const double4* __restrict__ gC, int block) {
+  int lb = block;
+  int row = lb * blockDim.x + threadIdx.x;
+  if (row >= a.nrows) return;
+  const int i = a.rows[row];
+  const int n = a.nnum[row];
+  const int* nbrow = a.nbr + (size_t)row * a.stride;
+  int jj[3], ll[3];
+  for (int q = 0; q < 3; ++q) {
+    jj[q] = n == 3 ? nbrow[q] : 0;
+    ll[q] = n == 3 ? rev[(size_t)row * 3 + q] : -1;
+  }
+  int jr[3];
+  double4 c[3];
+  for (int q = 0; q < 3; ++q) {
+    jr[q] = row_of[jj[q]];
+    const bool ok = ll[q] >= 0 && ll[q] <= 2;  // reported by k_rev / k_nn_from_tb otherwise
+    c[q] = ok ? gC[(size_t)jj[q] * 3 + ll[q]] : make_double4(0, 0, 0, 0);
+    if (!ok) jr[q] = -1;
+  }
   double fx = 0, fy = 0, fz = 0;
-  for (int q = 0; q < 3 && n == 3; ++q) {
-    int j = nbrow[q];
-    int l1 = rev[(size_t)row * 3 + q];
-    int jr = row_of[j];
-    if (jr < 0 || l1 < 0 || l1 > 2) continue;  // reported by k_rev / k_nn_from_tb
-    double4 pj = a.pos[j];
-    double d[3][3];
-    for (int k = 0; k < 3; ++k) G.d2(pj, G.at(a.nbr[(size_t)jr * 3 + k]), d[k][0], d[k][1], d[k][2]);
-    int l2 = (l1 + 1) % 3, l3 = (l1 + 2) % 3;
-    double d12[3], d31[3], d23[3];
-    for (int k = 0; k < 3; ++k) {
-      d12[k] = d[l2][k] - d[l1][k];
-      d31[k] = d[l1][k] - d[l3][k];
-      d23[k] = d[l3][k] - d[l2][k];
-    }
-    double s1212 = d12[0] * d12[0] + d12[1] * d12[1] + d12[2] * d12[2];
-    double s3131 = d31[0] * d31[0] + d31[1] * d31[1] + d31[2] * d31[2];
-    double s1231 = d12[0] * d31[0] + d12[1] * d31[1] + d12[2] * d31[2];
-    double s2331 = d23[0] * d31[0] + d23[1] * d31[1] + d23[2] * d31[2];
-    double s2312 = d23[0] * d12[0] + d23[1] * d12[1] + d23[2] * d12[2];
-    double den = s1212 * s3131 - s1231 * s1231;
-    double4 S = gS[j];
-    double w = S.x * (d12[0] * s2331 - d31[0] * s2312) + S.y * (d12[1] * s2331 - d31[1] * s2312) +
-               S.z * (d12[2] * s2331 - d31[2] * s2312);
-    double c = P.nn_coef * w / den;
-    double4 g = gnorm[j];
-    fx -= c * g.x; fy -= c * g.y; fz -= c * g.z;
+  for (int q = 0; q < 3; ++q) {
+    if (jr[q] < 0) continue;
+    fx -= c[q].x; fy -= c[q].y; fz -= c[q].z;
   }
   write_force(a, i, fx, fy, fz);
 }
 
-__global__ void __launch_bounds__(256) k_cos_nn(RowArgs a, CosP P, const int* __restrict__ row_of,
-                                                const int* __restrict__ rev, const double4* __restrict__ gnorm,
-                                                const double4* __restrict__ gS, int nb_real) {
-  cos_nn_rows(a, P, row_of, rev, gnorm, gS, xcd_block(nb_real));
+__global__ void __launch_bounds__(256) k_cos_nn(RowArgs a, const int* __restrict__ row_of, const int* __restrict__ rev,
+                                                const double4* __restrict__ gC, int nb_real) {
+  int lb = xcd_block(nb_real);
+  if (lb < 0) return;
+  cos_nn_rows(a, row_of, rev, gC, lb);
 }
 
 // metal side (*_forces_for_other_atoms): rows of nl(2), neighbours are
-// graphene atoms whose normal enters V3 (morsec divides by |n|)
+// graphene atoms whose normal enters V3 (morsec divides by |n|); the mixed
+// path gathers the FP32 copy of the normals (the same values it rounds to)
 template <int T, class R>
 __device__ __forceinline__ void cos_metal_rows(const RowArgs& a, const CosP& P, const double4* __restrict__ gnorm,
-                                               int block) {
+                                               const float4* __restrict__ gnf, int block) {
   ROW_PROLOGUE_AT(T, block)
   const R R2 = P.R2, pref = P.pref;
   double fx = 0, fy = 0, fz = 0;
-  walk_row_aux<T, R>(G, nbrow, n, t, i, gnorm, [&](int, const typename Geo<R>::P& q, const double4& g, bool valid) {
+  auto f = [&](int, const typename Geo<R>::P& q, const auto& g, bool valid) {
     R dx, dy, dz;
     R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
     const R gx = g.x, gy = g.y, gz = g.z;
@@ -625,15 +658,17 @@ __device__ __forceinline__ void cos_metal_rows(const RowArgs& a, const CosP& P, 
     A = in ? A * pref : R(0);
     Bc = in ? Bc * pref : R(0);
     fx -= (double)(A * dx + Bc * gx); fy -= (double)(A * dy + Bc * gy); fz -= (double)(A * dz + Bc * gz);
-  });
+  };
+  if constexpr (sizeof(R) == 4) walk_row_aux<T, R>(G, nbrow, n, t, i, gnf, f);
+  else walk_row_aux<T, R>(G, nbrow, n, t, i, gnorm, f);
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (t == 0) write_force(a, i, fx, fy, fz);
 }
 
 template <int T, class R>
 __global__ void __launch_bounds__(PAIR_BLOCK) k_cos_metal(RowArgs a, CosP P, const double4* __restrict__ gnorm,
-                                                          int nb_real) {
-  cos_metal_rows<T, R>(a, P, gnorm, xcd_block(nb_real));
+                                                          const float4* __restrict__ gnf, int nb_real) {
+  cos_metal_rows<T, R>(a, P, gnorm, gnf, xcd_block(nb_real));
 }
 
 // the normal-derivative rows (graphene atoms) and the metal rows in one
@@ -641,12 +676,13 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_cos_metal(RowArgs a, CosP P, con
 // logical blocks [0, nb_nn) take the graphene rows, the rest the metal rows
 template <int T, class R>
 __global__ void __launch_bounds__(256) k_cos_nn_metal(RowArgs an, RowArgs am, CosP P, const int* __restrict__ row_of,
-                                                      const int* __restrict__ rev, const double4* __restrict__ gnorm,
-                                                      const double4* __restrict__ gS, int nb_nn, int nb_real) {
+                                                      const int* __restrict__ rev, const double4* __restrict__ gC,
+                                                      const double4* __restrict__ gnorm, const float4* __restrict__ gnf,
+                                                      int nb_nn, int nb_real) {
   int lb = xcd_block(nb_real);
   if (lb < 0) return;
-  if (lb < nb_nn) cos_nn_rows(an, P, row_of, rev, gnorm, gS, lb);
-  else cos_metal_rows<T, R>(am, P, gnorm, lb - nb_nn);
+  if (lb < nb_nn) cos_nn_rows(an, row_of, rev, gC, lb);
+  else cos_metal_rows<T, R>(am, P, gnorm, gnf, lb - nb_nn);
 }
 
 // ----------------------------------------------------------------- dispatch
@@ -787,8 +823,8 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         if (a.nrows > 0) {
           DISPATCH_T(T, {
             int nb = row_blocks(a.nrows, T, BS);
-            if (energy) PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, true>), pad8(nb), BS, a, P, NN.nbr, simp, it.gnorm, it.gS, nb);
-            else PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, false>), pad8(nb), BS, a, P, NN.nbr, simp, it.gnorm, it.gS, nb);
+            if (energy) PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, true>), pad8(nb), BS, a, P, NN.nbr, simp, it.gnorm, it.gnf, it.gC, nb);
+            else PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, false>), pad8(nb), BS, a, P, NN.nbr, simp, it.gnorm, it.gnf, it.gC, nb);
           });
           written = true;
         }
@@ -800,7 +836,7 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
           DISPATCH_T(T, {
             int nb = nbn + row_blocks(c.nrows, T, BS1);
             PFM_LAUNCH(h, "cos_nn_metal", (k_cos_nn_metal<T, R>), pad8(nb), BS1, b, c, P, h->grp.row_of[NN.g1],
-                       NN.rev, it.gnorm, it.gS, nbn, nb);
+                       NN.rev, it.gC, it.gnorm, it.gnf, nbn, nb);
           });
           written = true;
           break;
@@ -808,14 +844,14 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         if (nn) {
           RowArgs b = row_args(h, it, 2, written ? 1 : 0);
           int nb = row_blocks(b.nrows, 1, BS1);
-          PFM_LAUNCH(h, "cos_nn", k_cos_nn, pad8(nb), BS1, b, P, h->grp.row_of[NN.g1], NN.rev, it.gnorm, it.gS, nb);
+          PFM_LAUNCH(h, "cos_nn", k_cos_nn, pad8(nb), BS1, b, h->grp.row_of[NN.g1], NN.rev, it.gC, nb);
           written = true;
         }
         c.accumulate = written ? 1 : 0;
         if (c.nrows > 0) {
           DISPATCH_T(T, {
             int nb = row_blocks(c.nrows, T, BS);
-            PFM_LAUNCH(h, "cos_metal", (k_cos_metal<T, R>), pad8(nb), BS, c, P, it.gnorm, nb);
+            PFM_LAUNCH(h, "cos_metal", (k_cos_metal<T, R>), pad8(nb), BS, c, P, it.gnorm, it.gnf, nb);
           });
           written = true;
         }

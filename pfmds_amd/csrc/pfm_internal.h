@@ -333,7 +333,8 @@ struct Inter {
   // ljc/morsec
   double4* gnorm = nullptr;  // [N] by internal atom
   bool rows_disjoint = false;  // groups 1 and 2 share no atom (passes over both may share a launch)
-  double4* gS = nullptr;     // [N] sum of V2 V3 fc/(n.dr) dr per graphene atom
+  double4* gC = nullptr;     // [3N] per graphene atom j and nearest neighbour l: c_l n_j of the normal-derivative term
+  float4* gnf = nullptr;     // [N] FP32 copy of gnorm for the mixed metal pass
   // surrogate
   SurBuf* sur = nullptr;
   std::vector<double> sur_theta;  // parameters set before the buffers exist

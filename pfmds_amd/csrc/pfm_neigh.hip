@@ -452,25 +452,27 @@ __global__ void k_nn_from_tb(int nrows, const int* __restrict__ rows, const int*
 // Cell-tile list search: one workgroup per cell (grids at least three cells
 // wide).  The atoms of the 27 surrounding cells are staged in LDS as FP32
 // coordinates unwrapped into the frame of the centre cell's origin, with the
-// atom index in w; atoms outside group 2, and the padding after the last
-// atom, are staged far away, so they never pass the distance test.  Every
-// group-1 atom of the centre cell is a row scanned by T lanes; all rows of a
-// wavefront walk the tile in the same order, so their LDS reads are
-// broadcasts.  Each lane takes 32 of its candidates (every T-th slot) into a
-// register bitmask, two instructions per candidate after the distance; the
-// row's lanes then take their output offsets from a prefix sum over the T
-// lanes and write their hits.  The FP32 distance decides unless it lies
-// within a relative 3e-5 of r_cut**2, where the reference's FP64
+// atom index in w.  Atoms outside group 2, and the padding after the last
+// atom, are staged far away, so they never pass the distance test.  With
+// CMP (group 2 under half of the atoms: another species' list) only the
+// group-2 atoms are staged, compacted in tile order by ballots and a prefix
+// over the workgroup's four waves, so the rows walk only real candidates.  Every group-1 atom of the centre cell is a row scanned by T
+// lanes; all rows of a wavefront walk the tile in the same order, so their
+// LDS reads are broadcasts.  Each lane takes 32 of its candidates (every T-th
+// slot) into a register bitmask, two instructions per candidate after the
+// distance; the row's lanes then take their output offsets from a prefix sum
+// over the T lanes and write their hits.  The FP32 distance decides unless it
+// lies within a relative 3e-5 of r_cut**2, where the reference's FP64
 // find_distance test (min_image_dist2) on the global positions decides.
 // Tiles larger than the LDS buffer are staged in several passes.
 #define TILE_MAX 1536      // staged candidates per pass (24 KB of LDS), a multiple of 32 * T
-#define TILE_FAR 1.0e30f   // staged coordinate of atoms outside group 2 and of padding
+#define TILE_FAR 1.0e30f   // staged coordinate of the padding
 
 struct TileCells {
   int start[27], cnt[27], off[28];
 };
 
-template <int T>
+template <int T, bool CMP>
 __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __restrict__ rank_xyz,
                                                     const int* __restrict__ row_of, int nblocks_real) {
   static_assert(TILE_MAX % (32 * T) == 0, "tile passes hold whole bitmask rounds");
@@ -515,15 +517,51 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
   }
   __syncthreads();
   const int R = tc.off[27];
-  const int Rp = (R + 32 * T - 1) / (32 * T) * (32 * T);  // padded to whole rounds
   const float hi = (float)(a.rc2 * (1.0 + 3e-5)), lo = (float)(a.rc2 * (1.0 - 3e-5));
 
-  // stage tile slots [m0, m1): slot m < R belongs to the last tile cell k with
-  // off[k] <= m (off is non-decreasing, off[0] = 0); slots >= R are padding
-  auto stage = [&](int m0, int m1) {
-    for (int m = m0 + tid; m < m1; m += 256) {
-      float4 v = make_float4(TILE_FAR, TILE_FAR, TILE_FAR, __int_as_float(-1));
-      if (m < R) {
+  // stage the group-2 atoms of tile slots [s0, s1) compacted into st (slot
+  // m < R belongs to the last tile cell k with off[k] <= m; off is
+  // non-decreasing, off[0] = 0), pad them to whole rounds and return that
+  // length; every thread calls it.  A thread loads all its slots (every
+  // 256th) before the ballots, so the pass waits on one memory latency.
+  // dense: every slot was kept, so tile slots and staged slots coincide.
+  constexpr int KS = TILE_MAX / 256;
+  __shared__ int wsum[KS][4];
+  bool dense = true;
+  auto stage = [&](int s0, int s1) -> int {
+    if constexpr (!CMP) {
+      const int np = (s1 - s0 + 32 * T - 1) / (32 * T) * (32 * T);
+      for (int m = s0 + tid; m < s0 + np; m += 256) {
+        float4 v = make_float4(TILE_FAR, TILE_FAR, TILE_FAR, __int_as_float(-1));
+        if (m < s1) {
+          int k = 0;
+#pragma unroll
+          for (int sft = 16; sft > 0; sft >>= 1)
+            if (k + sft < 27 && tc.off[k + sft] <= m) k += sft;
+          int j = tc.start[k] + (m - tc.off[k]);
+          if (a.bin) j = a.bin[j];
+          int o0 = k % 3 - 1, o1 = (k / 3) % 3 - 1, o2 = k / 9 - 1;
+          double4 p = a.pos[j];
+          double x = p.x - org[0], y = p.y - org[1], z = p.z - org[2];
+          if (o0 < 0 && x > 0.0) x -= boxL.x; else if (o0 > 0 && x < 0.0) x += boxL.x;
+          if (o1 < 0 && y > 0.0) y -= boxL.y; else if (o1 > 0 && y < 0.0) y += boxL.y;
+          if (o2 < 0 && z > 0.0) z -= boxL.z; else if (o2 > 0 && z < 0.0) z += boxL.z;
+          v.w = __int_as_float(j);
+          if (a.mask[j] & a.g2bit) { v.x = (float)x; v.y = (float)y; v.z = (float)z; }
+        }
+        st[m - s0] = v;
+      }
+      return np;
+    }
+    const int lane = tid & 63, w = tid >> 6;
+    float4 v[KS];
+    bool keep[KS];
+#pragma unroll
+    for (int u = 0; u < KS; ++u) {
+      const int m = s0 + u * 256 + tid;
+      keep[u] = false;
+      v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (m < s1) {
         int k = 0;
 #pragma unroll
         for (int sft = 16; sft > 0; sft >>= 1)
@@ -532,19 +570,38 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
         if (a.bin) j = a.bin[j];
         int o0 = k % 3 - 1, o1 = (k / 3) % 3 - 1, o2 = k / 9 - 1;
         double4 p = a.pos[j];
+        keep[u] = (a.mask[j] & a.g2bit) != 0;
         // unwrap: the image of atom j next to the centre cell
         double x = p.x - org[0], y = p.y - org[1], z = p.z - org[2];
         if (o0 < 0 && x > 0.0) x -= boxL.x; else if (o0 > 0 && x < 0.0) x += boxL.x;
         if (o1 < 0 && y > 0.0) y -= boxL.y; else if (o1 > 0 && y < 0.0) y += boxL.y;
         if (o2 < 0 && z > 0.0) z -= boxL.z; else if (o2 > 0 && z < 0.0) z += boxL.z;
-        v.w = __int_as_float(j);
-        if (a.mask[j] & a.g2bit) { v.x = (float)x; v.y = (float)y; v.z = (float)z; }
+        v[u] = make_float4((float)x, (float)y, (float)z, __int_as_float(j));
       }
-      st[m - m0] = v;
     }
+    unsigned long long bal[KS];
+#pragma unroll
+    for (int u = 0; u < KS; ++u) {
+      bal[u] = __ballot(keep[u]);
+      if (lane == 0) wsum[u][w] = __popcll(bal[u]);
+    }
+    __syncthreads();
+    int base = 0;
+#pragma unroll
+    for (int u = 0; u < KS; ++u) {
+      int off = base;
+      for (int q = 0; q < w; ++q) off += wsum[u][q];
+      if (keep[u]) st[off + __popcll(bal[u] & ((1ull << lane) - 1ull))] = v[u];
+      base += wsum[u][0] + wsum[u][1] + wsum[u][2] + wsum[u][3];
+    }
+    dense = base == s1 - s0;
+    const int np = (base + 32 * T - 1) / (32 * T) * (32 * T);
+    for (int m = base + tid; m < np; m += 256) st[m] = make_float4(TILE_FAR, TILE_FAR, TILE_FAR, __int_as_float(-1));
+    return np;
   };
-  const bool single = Rp <= TILE_MAX;
-  if (single) stage(0, Rp);
+  const bool single = R <= TILE_MAX;
+  int np = 0;
+  if (single) np = stage(0, R);
 
   const int t = tid & (T - 1);
   for (int q0 = 0; q0 < ncen; q0 += 256 / T) {
@@ -555,20 +612,20 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
     const bool active = row >= 0 && (a.mask[i] & a.g1bit);
     const double4 pi = a.pos[i];
     const float px = (float)(pi.x - org[0]), py = (float)(pi.y - org[1]), pz = (float)(pi.z - org[2]);
-    const int own = tc.off[13] + qq;  // the row atom's own tile slot
     int* out = a.nbr + (size_t)(active ? row : 0) * a.stride;
     int count = 0;
-    for (int m0 = 0; m0 < Rp; m0 += TILE_MAX) {
-      const int m1 = Rp < m0 + TILE_MAX ? Rp : m0 + TILE_MAX;
+    for (int s0 = 0; s0 < R; s0 += TILE_MAX) {
       if (!single) {
         __syncthreads();
-        stage(m0, m1);
+        np = stage(s0, R < s0 + TILE_MAX ? R : s0 + TILE_MAX);
       }
       __syncthreads();
       if (!active) continue;
-      for (int c0 = m0; c0 < m1; c0 += 32 * T) {
+      // the row atom's own staged slot when the pass is dense (else found by index)
+      const int own = (!CMP || dense) ? tc.off[13] + qq - s0 : -1;
+      for (int c0 = 0; c0 < np; c0 += 32 * T) {
         // bit 31-b <-> candidate slot c0 + t + b*T
-        const float4* sv = st + (c0 - m0) + t;
+        const float4* sv = st + c0 + t;
         unsigned hit = 0u;
 #pragma unroll 8
         for (int b = 0; b < 32; ++b) {
@@ -586,11 +643,13 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
           const int b = __clz(chk);
           chk &= ~(0x80000000u >> b);
           const float4 v = sv[b * T];
+          const int j = __float_as_int(v.w);
           float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
-          if (fmaf(dx, dx, fmaf(dy, dy, dz * dz)) >= lo) {
+          if (CMP && !dense && j == i) {
+            hit &= ~(0x80000000u >> b);
+          } else if (fmaf(dx, dx, fmaf(dy, dy, dz * dz)) >= lo) {
             double ex, ey, ez;
-            if (!(min_image_dist2(pi, a.pos[__float_as_int(v.w)], hb, ex, ey, ez) < a.rc2))
-              hit &= ~(0x80000000u >> b);
+            if (!(min_image_dist2(pi, a.pos[j], hb, ex, ey, ez) < a.rc2)) hit &= ~(0x80000000u >> b);
           }
         }
         // offsets of this lane's hits inside the row
@@ -628,11 +687,16 @@ static void launch_build_T(pfm_ctx* h, SearchArgs& a, const int* row_of) {
   PFM_LAUNCH(h, "build_rows", k_build<T>, pad8(nb), 256, a, nb);
 }
 
-static void launch_build(pfm_ctx* h, SearchArgs& a, const int* row_of) {
+static void launch_build(pfm_ctx* h, SearchArgs& a, const int* row_of, bool sparse) {
   if (h->any_tile) {
     int nb = h->ncell;
-    if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", k_build_tile<4>, pad8(nb), 256, a, h->cell_xyz, row_of, nb);
-    else PFM_LAUNCH(h, "build_list", k_build_tile<8>, pad8(nb), 256, a, h->cell_xyz, row_of, nb);
+    if (sparse) {
+      if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", (k_build_tile<4, true>), pad8(nb), 256, a, h->cell_xyz, row_of, nb);
+      else PFM_LAUNCH(h, "build_list", (k_build_tile<8, true>), pad8(nb), 256, a, h->cell_xyz, row_of, nb);
+    } else {
+      if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", (k_build_tile<4, false>), pad8(nb), 256, a, h->cell_xyz, row_of, nb);
+      else PFM_LAUNCH(h, "build_list", (k_build_tile<8, false>), pad8(nb), 256, a, h->cell_xyz, row_of, nb);
+    }
   }
   if (h->any_rows) {
     if (a.nmax <= 8) launch_build_T<4>(h, a, row_of);
@@ -676,7 +740,8 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   a.err_code = PFM_E_OVERFLOW;
   a.exact = (k == 2) ? 1 : 0;
   const int* row_of = h->grp.row_of[L.g1];
-  launch_build(h, a, row_of);
+  // candidates of another species: stage only those
+  launch_build(h, a, row_of, 2L * h->grp.size[L.g2] < (long)h->N);
   int nb = (L.nrows + 255) / 256;
   if (it.kind == PFM_TB || it.kind == PFM_REBOSC || (k == 2 && !(int)it.p[it.kind == PFM_LJC ? 5 : 6])) {
     PFM_LAUNCH(h, "list_rev", k_rev, nb, 256, L.nrows, h->grp.rows[L.g1], h->grp.row_of[L.g1], L.nnum, L.nbr,
