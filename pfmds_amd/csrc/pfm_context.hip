@@ -588,8 +588,13 @@ static void apply_changes(pfm_ctx* h, int s) {
 }
 
 // Groups that grow keep their members under a spare group bit for the list
-// rows.  Growth is taken for groups whose lists come from the generic search
-// (lj, rjl) and for roles and chain groups.
+// rows.  Growth is taken for roles, chain groups and the groups of every
+// interaction whose lists come from the generic search (lj, lj1g, rjl, tb,
+// ljc/morsec): rows of members that have not joined stay empty and the
+// candidates are the active atoms, as find_neighbours's loops to group%N
+// (md_neighbours.f90:67,70).  A growing graphene group of ljc/morsec with a
+// tb list stops at its first step as the reference's does: the members that
+// have not joined have no nearest neighbours (graphenenorm.f90:30).
 static int growth_setup(pfm_ctx* h) {
   if (h->changes.empty()) return PFM_OK;
   if (h->nrep > 1) return fail(h, PFM_E_INVALID, "error: group changes are for single systems only");
@@ -597,8 +602,8 @@ static int growth_setup(pfm_ctx* h) {
   for (const auto& c : h->changes) grows[c.to] = true;
   for (auto& it : h->inter)
     for (int k = 0; k < it.nl_n; ++k)
-      if ((grows[it.nl[k].g1] || grows[it.nl[k].g2]) && it.kind != PFM_LJ && it.kind != PFM_RJL)
-        return fail(h, PFM_E_INVALID, "error: a changing group in the lists of an interaction other than lj or rjl");
+      if ((grows[it.nl[k].g1] || grows[it.nl[k].g2]) && (it.kind == PFM_REBOSC || it.kind == PFM_SUR))
+        return fail(h, PFM_E_INVALID, "error: a changing group in the lists of rebosc or of a surrogate");
   int spare = PFM_MAXG;
   auto taken = [&](int g) {
     if (!h->grp.idx[g].empty() || h->grp.used[g] || h->grp.shadow[g]) return true;

@@ -6,13 +6,18 @@ checker after every chunk, active counts equal after every chunk, lists
 equal as sets after rebuilds, for the deposition case with lj and with rjl.
 Captured blocks are used away from the steps where a count changes and give
 bit-identical results to direct launches; records carry the active counts
-of their step; groups of interactions whose lists are not searched
-generically (tb here) are refused.
+of their step.  The same bar holds for lj1g, for a tb sheet whose atoms join
+one at a time, and for metal adatoms joining the rjl and morsec metal group
+under graphene (tb + morsec with nearest-neighbour terms).  A growing
+graphene group of morsec with a tb list stops at step 0 with the reference's
+"not enough gr nearest neibs" status, on the GPU and in the checker; growth
+in the groups of rebosc is refused.
 """
 import numpy as np
 import pytest
 
 from pfmds_amd import capi, configs
+from pfmds_amd.system import Interaction
 from tests import growth_case
 from tests.harness import gpu_ctx, list_sets, oracle_ctx, rel_inf
 
@@ -21,7 +26,7 @@ pytestmark = pytest.mark.gpu
 CHUNKS = [(0, 0), (1, 15), (16, 37), (38, 60)]
 
 
-@pytest.mark.parametrize("kind,side", [("lj", 4), ("rjl", 4), ("lj", 8)])
+@pytest.mark.parametrize("kind,side", [("lj", 4), ("rjl", 4), ("lj", 8), ("lj1g", 4), ("lj1g", 8)])
 def test_growth_matches_checker(kind, side):
     # 4^3 atoms: lists searched per row; 8^3: three cells a side, searched per cell tile
     s = growth_case.deposition(n_side=side, kind=kind)
@@ -90,9 +95,45 @@ def capi_kb():
     return 1.3806488 / 1.6021765654 * (1.0 / 10000.0)
 
 
-def test_growth_of_a_tb_group_refused():
-    s = configs.graphene_on_cu(cx=6, cy=4, gx=6, gy=4, layers=3, seed=93)
-    s.changes = [(5, 1, 3, 50, 5)]
+@pytest.mark.parametrize("case", ["tb", "adatoms"])
+def test_growth_tb_and_cos_match_checker(case):
+    s = growth_case.carbon_growth_tb() if case == "tb" else growth_case.metal_adatoms()
+    g2 = 1 if case == "tb" else 2
+    print(f"[growth {case}] n={s.n}", flush=True)
+    g = gpu_ctx(s, capi.FP64)
+    o = oracle_ctx(s)
+    for s0, s1 in CHUNKS:
+        g.run("nve", 0.5, s0, s1, True)
+        o.run("nve", 0.5, s0, s1, True)
+        assert g.group_count(g2) == o.group_count(g2)
+        (pg, vg, fg), (po, vo, fo) = g.state(), o.state()
+        assert rel_inf(pg, po) < 1e-10 and rel_inf(vg, vo) < 1e-10 and rel_inf(fg, fo) < 1e-10, s1
+        assert g.energies()["pe"] == pytest.approx(o.energies()["pe"], rel=1e-10)
+        if s1 % 10 == 0:
+            for k, it in enumerate(s.interactions, start=1):
+                for li in range(1, len(it.lists) + 1):
+                    assert list_sets(g, k, li)[0] == list_sets(o, k, li)[0], (s1, k, li)
+    size = len(s.group_indices(g2))
+    assert o.group_count(g2) < size  # still growing at the end: inactive rows were exercised
+    g.close()
+    o.close()
+
+
+def test_growing_graphene_group_with_tb_stops():
+    s = growth_case.carbon_growth_with_cos()
+    codes = []
+    for ctx in (gpu_ctx(s, capi.FP64), oracle_ctx(s)):
+        with pytest.raises(capi.PfmError) as ei:
+            ctx.run("nve", 0.5, 0, 5, True)
+        assert "gr nearest neibs" in ei.value.msg
+        codes.append(ei.value.code)
+        ctx.close()
+    assert codes == [capi.E_GRAPHENE_NN, capi.E_GRAPHENE_NN]
+
+
+def test_growth_of_a_rebosc_group_refused():
+    s = growth_case.carbon_growth_tb()
+    s.interactions = [Interaction("rebosc", list(configs.REBO_C), [(1, 1, 8, 2.3, 10)])]
     c = gpu_ctx(s, capi.FP64)
     with pytest.raises(capi.PfmError):
         c.run("nve", 0.5, 0, 1, True)
