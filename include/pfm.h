@@ -202,8 +202,9 @@ int pfm_set_graphs(pfm_ctx* h, int on);
  * once per block, and on the first step whose test holds restores the state
  * of the step before it; the lists are rebuilt from a fresh sort at the next
  * step.  Results, exit step and energies are those of testing every step.
- * Single systems without group changes; pfm_set_speculation(h, 0) tests
- * every step on the host.  No counterpart in the reference. */
+ * Systems without group changes, batches included (a batch stops when every
+ * replica meets the test); pfm_set_speculation(h, 0) tests every step on
+ * the host.  No counterpart in the reference. */
 int pfm_set_speculation(pfm_ctx* h, int on);
 
 /* Blocks captured and blocks launched from graphs since the handle was made. */

@@ -907,6 +907,42 @@ static void energy_parts(pfm_ctx* h, double* opart) {
 
 // Energies of the step: per-interaction sums of the per-row energies and the
 // all_moving group's kinetic energy; one blocking read-back.
+// arguments of k_rep_energy over the handle's replicas (out left to the caller)
+static RepEnergyArgs rep_energy_args(pfm_ctx* h) {
+  RepEnergyArgs A;
+  memset(&A, 0, sizeof A);
+  const int ni = (int)h->inter.size();
+  A.ni = ni;
+  A.nrep = h->nrep;
+  for (int k = 0; k < ni; ++k) {
+    A.e_row[k] = h->inter[k].e_row;
+    A.g1[k] = h->inter[k].nl[0].g1;
+  }
+  A.row0 = h->d_rep_row0;
+  A.vel = h->vel4;
+  A.mask = h->mask;
+  A.bit = 1u << (h->roles[0] - 1);
+  return A;
+}
+
+// per-replica energies from the [nrep][ni + 1] rows of k_rep_energy
+static void set_rep_energies(pfm_ctx* h, const double* o) {
+  const int nrep = h->nrep, ni = (int)h->inter.size();
+  h->rep_e.assign(o, o + (size_t)nrep * (ni + 1));
+  h->rep_pe.assign(nrep, 0.0);
+  h->rep_ke.assign(nrep, 0.0);
+  h->rep_temp.assign(nrep, 0.0);
+  const std::vector<int>& m0 = h->rep_row0[h->roles[0]];
+  for (int r = 0; r < nrep; ++r) {
+    double p = 0.0;
+    for (int k = 0; k < ni; ++k) p += o[(size_t)r * (ni + 1) + k];
+    h->rep_pe[r] = p;
+    h->rep_ke[r] = o[(size_t)r * (ni + 1) + ni];
+    int nm = m0[r + 1] - m0[r];
+    h->rep_temp[r] = nm > 0 ? 2 * h->rep_ke[r] / PFM_KB / (3 * nm) : 0.0;
+  }
+}
+
 static int energies(pfm_ctx* h, bool nvt) {
   int nb = h->red_blocks;
   int ni = (int)h->inter.size();
@@ -932,36 +968,13 @@ static int energies(pfm_ctx* h, bool nvt) {
       int rc;
       if ((rc = dalloc(h, &h->d_rep_out, (size_t)nrep * (ni + 1)))) return rc;
     }
-    RepEnergyArgs A;
-    memset(&A, 0, sizeof A);
-    A.ni = ni;
-    A.nrep = nrep;
-    for (int k = 0; k < ni; ++k) {
-      A.e_row[k] = h->inter[k].e_row;
-      A.g1[k] = h->inter[k].nl[0].g1;
-    }
-    A.row0 = h->d_rep_row0;
-    A.vel = h->vel4;
-    A.mask = h->mask;
-    A.bit = 1u << (h->roles[0] - 1);
+    RepEnergyArgs A = rep_energy_args(h);
     A.out = h->d_rep_out;
     PFM_LAUNCH(h, "energy", k_rep_energy, nrep, RED_THREADS, A);
     std::vector<double> o((size_t)nrep * (ni + 1));
     HIPCHK(h, hipMemcpyAsync(o.data(), h->d_rep_out, sizeof(double) * o.size(), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    h->rep_e.assign(o.begin(), o.end());
-    h->rep_pe.assign(nrep, 0.0);
-    h->rep_ke.assign(nrep, 0.0);
-    h->rep_temp.assign(nrep, 0.0);
-    const std::vector<int>& m0 = h->rep_row0[h->roles[0]];
-    for (int r = 0; r < nrep; ++r) {
-      double p = 0.0;
-      for (int k = 0; k < ni; ++k) p += o[(size_t)r * (ni + 1) + k];
-      h->rep_pe[r] = p;
-      h->rep_ke[r] = o[(size_t)r * (ni + 1) + ni];
-      int nm = m0[r + 1] - m0[r];
-      h->rep_temp[r] = nm > 0 ? 2 * h->rep_ke[r] / PFM_KB / (3 * nm) : 0.0;
-    }
+    set_rep_energies(h, o.data());
   }
   for (size_t c = 0; c < cs.size(); ++c) {
     // calculate_nose_hoover_chain_energy (md_integrators.f90:247-260)
@@ -997,8 +1010,9 @@ static int spec_alloc(pfm_ctx* h, int k) {
   h->spec_e = nullptr; h->sp_pos = h->sp_vel = h->sp_frc = nullptr; h->sp_mask = nullptr; h->sp_orig = nullptr;
   h->spec_k = 0;
   const size_t n = (size_t)k * h->N;
+  const size_t nr = h->nrep > 1 ? (size_t)h->nrep : 0;  // batches: per-replica rows after the whole-system rows
   int rc;
-  if ((rc = dalloc(h, &h->spec_e, (size_t)k * (h->inter.size() + 1))) || (rc = dalloc(h, &h->sp_pos, n)) ||
+  if ((rc = dalloc(h, &h->spec_e, (size_t)k * (h->inter.size() + 1) * (1 + nr))) || (rc = dalloc(h, &h->sp_pos, n)) ||
       (rc = dalloc(h, &h->sp_vel, n)) || (rc = dalloc(h, &h->sp_frc, n)) || (rc = dalloc(h, &h->sp_mask, n)) ||
       (rc = dalloc(h, &h->sp_orig, n)))
     return rc;
@@ -1006,11 +1020,17 @@ static int spec_alloc(pfm_ctx* h, int k) {
   return PFM_OK;
 }
 
-// the energies of the step into slot k (the sums of energies()) and its state
+// the energies of the step into slot k (the sums of energies(); for a batch
+// also each replica's, after the spec_k whole-system rows) and its state
 static void spec_record(pfm_ctx* h, int k) {
   const int nb = h->red_blocks, ni = (int)h->inter.size(), N = h->N;
   energy_parts(h, nullptr);
   PFM_LAUNCH(h, "energy", k_reduce_rows, 1, RED_THREADS, ni + 1, nb, h->red_part, h->spec_e + (size_t)k * (ni + 1));
+  if (h->nrep > 1) {
+    RepEnergyArgs A = rep_energy_args(h);
+    A.out = h->spec_e + (size_t)h->spec_k * (ni + 1) + (size_t)k * h->nrep * (ni + 1);
+    PFM_LAUNCH(h, "energy", k_rep_energy, h->nrep, RED_THREADS, A);
+  }
   const size_t o = (size_t)k * N;
   PFM_LAUNCH(h, "spec", k_spec_copy, (N + 255) / 256, 256, N, 0, h->pos4, h->vel4, h->frc4, h->mask, h->orig,
              h->sp_pos + o, h->sp_vel + o, h->sp_frc + o, h->sp_mask + o, h->sp_orig + o);
@@ -1752,11 +1772,21 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
     const int K = (blk > 0 && blk <= 32) ? blk : 16;
     if ((rc = spec_alloc(h, K))) return rc;
     const int ni = (int)h->inter.size(), w = ni + 1;
-    std::vector<double> ev((size_t)K * w);
+    const int nr = h->nrep > 1 ? h->nrep : 0;
+    std::vector<double> ev((size_t)K * w * (1 + nr));
     const bool capture = h->use_graphs && blk == K;
+    // the exit test of md_simulation.f90:143 between two steps' energies; a
+    // batch stops when every replica meets it
+    auto conv = [&](double a, double b, const std::vector<double>& ra, const std::vector<double>& rb) {
+      if (!nr) return std::fabs(a - b) <= ms_de;
+      if ((int)ra.size() != nr || (int)rb.size() != nr) return false;
+      for (int r = 0; r < nr; ++r)
+        if (!(std::fabs(ra[r] - rb[r]) <= ms_de)) return false;
+      return true;
+    };
     for (int s = s0; s <= s1;) {
       *last_step = s;
-      if (s != 0 && std::fabs(h->pe - h->pe_prev) <= ms_de) {
+      if (s != 0 && conv(h->pe, h->pe_prev, h->rep_pe, h->rep_pe_prev)) {
         *event = PFM_EV_CONVERGED;
         return PFM_OK;
       }
@@ -1777,6 +1807,7 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
       }
       const int n = end - s + 1;
       const double t_start = h->sim_time, pe0 = h->pe;
+      const std::vector<double> rpe0 = h->rep_pe;
       spec_on = true;
       spec_base = s;
       int rc2 = PFM_OK;
@@ -1797,18 +1828,26 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
         if ((rc = rec_issue(h, integ, notes))) return rc;
       }
       HIPCHK(h, hipMemcpyAsync(ev.data(), h->spec_e, sizeof(double) * n * w, hipMemcpyDeviceToHost, h->stream));
+      if (nr)
+        HIPCHK(h, hipMemcpyAsync(ev.data() + (size_t)K * w, h->spec_e + (size_t)h->spec_k * w,
+                                 sizeof(double) * n * nr * w, hipMemcpyDeviceToHost, h->stream));
       HIPCHK(h, hipMemcpyAsync(h->herr, h->derr, sizeof(DevErr), hipMemcpyDeviceToHost, h->stream));
       HIPCHK(h, hipStreamSynchronize(h->stream));
       // pe of every step of the block, summed as energies() sums it; the exit
       // test of step t reads pe(t-1) and pe(t-2)
       std::vector<double> pe(n);
+      std::vector<std::vector<double>> rpe(n);
+      const double* evr = ev.data() + (size_t)K * w;  // [n][nr][w]
       int tc = -1;
       for (int j = 0; j < n; ++j) {
         double p = 0.0;
         for (int k = 0; k < ni; ++k) p += ev[(size_t)j * w + k];
         pe[j] = p;
+        rpe[j].assign(nr, 0.0);
+        for (int r = 0; r < nr; ++r)
+          for (int k = 0; k < ni; ++k) rpe[j][r] += evr[((size_t)j * nr + r) * w + k];
         const double m2 = j == 0 ? pe0 : pe[j - 1];
-        if (j + 1 < n && std::fabs(p - m2) <= ms_de) {
+        if (j + 1 < n && conv(p, m2, rpe[j], j == 0 ? rpe0 : rpe[j - 1])) {
           tc = s + j + 1;
           break;
         }
@@ -1819,6 +1858,10 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
       h->pe_prev = jl == 0 ? pe0 : pe[jl - 1];
       h->ke = ev[(size_t)jl * w + ni];
       h->temp = 2 * h->ke / PFM_KB / (3 * h->grp.n[h->roles[0]]);
+      if (nr) {
+        set_rep_energies(h, evr + (size_t)jl * nr * w);
+        h->rep_pe_prev = jl == 0 ? rpe0 : rpe[jl - 1];
+      }
       if (tc < 0) {
         if ((rc = check_err(h, end)) || (rc = rec_poll(h, false))) return rc;
         s = end + 1;
@@ -1849,8 +1892,7 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
     }
     return check_err(h, s1);
   };
-  if (integ == PFM_NVMS && h->speculate && h->nrep == 1 && h->changes.empty() && !h->timing)
-    return nvms_blocks();
+  if (integ == PFM_NVMS && h->speculate && h->changes.empty() && !h->timing) return nvms_blocks();
   for (int s = s0; s <= s1;) {
     const bool steady = nvt ? nhc_pending : kick_pending;
     bool blocked = graphs && s > 0 && s % blk == 0 && s + blk - 1 < s1 && steady;

@@ -7,14 +7,15 @@ first step or never; over split calls, in both precisions, with and without
 captured blocks, and with output records (records after the exit step are
 dropped).  After an exit the next steps start from a fresh sort, so a
 continuation agrees to rounding: over 20 steps, positions within 1e-11
-relative in FP64 and 1e-8 in mixed precision.
+relative in FP64 and 1e-8 in mixed precision.  Replica batches stop when
+every replica meets the test: the same bar with per-replica energies.
 """
 import time
 
 import numpy as np
 import pytest
 
-from pfmds_amd import capi, configs
+from pfmds_amd import capi, configs, system
 from tests.harness import gpu_ctx, rel_inf
 
 pytestmark = pytest.mark.gpu
@@ -113,3 +114,53 @@ def test_nvms_blocks_time():
         dt = (time.perf_counter() - t0) / 500
         print(f"  n={s.n} nvms blocks={spec}: {dt * 1e3:.4f} ms/step", flush=True)
         c.close()
+
+
+def batch_replicas():
+    # a dilute LJ lattice (small energy changes) and three Cu crystals near 50 K
+    return [
+        configs.lj_gas_lattice(400, 0.01, 11, 20.0),
+        configs.fcc_cu_rgl(nx=4, seed=12, dt=0.5, temperature=50.0),
+        configs.fcc_cu_rgl(nx=6, seed=13, dt=0.5, temperature=60.0),
+        configs.fcc_cu_rgl(nx=5, ny=4, nz=7, seed=15, dt=0.5, temperature=40.0),
+    ]
+
+
+def run_batch(b, prec, spec, ms_de, calls, graphs=True):
+    c = system.build_batch_context(b, prec)
+    c.set_speculation(spec)
+    c.set_graphs(graphs)
+    out = None
+    for a, e in calls:
+        last, ev = c.run("nvms", 0.5, a, e, True, ms_de)
+        out = (last, ev)
+        if ev == capi.EV_CONVERGED:
+            break
+    res = (out, c.state(), c.replica_energies(), c.energies())
+    c.close()
+    return res
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.MIXED])
+def test_nvms_blocks_batch(prec):
+    b = system.merge_replicas(batch_replicas())
+    ms_de, tc = 1e-30, None
+    # the batch's largest per-step change falls from about 7e-2 to a plateau
+    # near 5e-4 by step 150 (the first step changes by 9e-4)
+    for m in (1e-4, 5.5e-4, 7e-4, 8.5e-4, 3e-3, 1e-2):
+        out = run_batch(b, prec, False, m, [(0, 300)])[0]
+        if out[1] == capi.EV_CONVERGED and out[0] > 3:
+            ms_de, tc = m, out[0]
+            break
+    print(f"[nvms blocks batch] {b.n_rep} replicas n={b.system.n} ms_de={ms_de} exit={tc}", flush=True)
+    assert tc is not None, "no exit within 300 steps: the batch exit is not exercised"
+    for calls in ([(0, 300)], [(0, 0), (1, 37), (38, 300)]):
+        for graphs in (True, False):
+            (oa, sa, ra, ea), (ob, sb, rb, eb) = (run_batch(b, prec, sp, ms_de, calls, graphs) for sp in (False, True))
+            assert oa == ob
+            for x, y in zip(sa, sb):
+                assert np.array_equal(x, y)
+            for k in ("pe", "ke", "temp"):
+                assert np.array_equal(ra[k], rb[k]), k
+                assert ea[k] == eb[k], k
+            assert np.array_equal(ra["e_inter"], rb["e_inter"])
