@@ -752,8 +752,10 @@ static int finalize(pfm_ctx* h) {
     }
     it.e_rows = it.nl[0].nrows;
     if ((rc = dalloc(h, &it.e_row, it.e_rows))) return rc;
-    if (it.kind == PFM_LJ1G)
+    if (it.kind == PFM_LJ1G) {
+      if (h->N > 0x3fffffff) return fail(h, PFM_E_INVALID, "error: lj1g lists hold atom indices below 2^30");
       if ((rc = dalloc(h, &it.lc, h->N))) return rc;
+    }
     if (it.kind == PFM_SUR) {
       if ((rc = sur_alloc(h, it))) return rc;
       if (!it.sur_theta.empty()) {
@@ -2238,6 +2240,8 @@ int pfm_get_list(pfm_ctx* h, int inter, int k, int* n_rows, int* nnum, int* nlis
   HIPCHK(h, hipMemcpyAsync(cnt.data(), L.nnum, sizeof(int) * L.nrows, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipMemcpyAsync(nb.data(), L.nbr, sizeof(int) * nb.size(), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (it.kind == PFM_LJ1G)  // lj1g entries carry their half-list weights in bits 30-31
+    for (int& x : nb) x &= 0x3fffffff;
   std::vector<int> loc1(h->N, -1), loc2(h->N, -1);
   for (size_t q = 0; q < h->grp.idx[L.g1].size(); ++q) loc1[h->grp.idx[L.g1][q]] = (int)q;
   int g2 = (k == 3 && it.nn_from_tb >= 0) ? L.g1 : L.g2;

@@ -131,6 +131,31 @@ __device__ __forceinline__ void walk_row_aux(const Geo<R>& G, const int* __restr
   }
 }
 
+// walk_row over entries that carry a two-bit tag in bits 30-31 (lj1g's
+// weights): f(record, tag, valid)
+#define ENTRY_ATOM 0x3fffffff
+template <int T, class R, class F>
+__device__ __forceinline__ void walk_row_tag(const Geo<R>& G, const int* __restrict__ nbrow, int n, int t, int self,
+                                             F f) {
+  if constexpr (sizeof(R) == 8) {
+    for (int p = t; p < n; p += T) {
+      const unsigned e = (unsigned)nbrow[p];
+      f(G.at((int)(e & ENTRY_ATOM)), e >> 30, true);
+    }
+    return;
+  }
+  for (int p0 = 4 * t; p0 < n; p0 += 4 * T) {
+    const int4 v = *reinterpret_cast<const int4*>(nbrow + p0);
+    const unsigned ee[4] = {(unsigned)v.x, p0 + 1 < n ? (unsigned)v.y : (unsigned)self,
+                            p0 + 2 < n ? (unsigned)v.z : (unsigned)self, p0 + 3 < n ? (unsigned)v.w : (unsigned)self};
+    typename Geo<R>::P q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = G.at((int)(ee[u] & ENTRY_ATOM));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) f(q[u], ee[u] >> 30, p0 + u < n);
+  }
+}
+
 __device__ __forceinline__ void write_force(const RowArgs& a, int i, double fx, double fy, double fz) {
   double4 f = a.accumulate ? a.frc[i] : make_double4(0, 0, 0, 0);
   f.x += fx; f.y += fy; f.z += fz;
@@ -247,17 +272,16 @@ struct Lj1gP {
   double c6, c12, c6t6, c12t12, R1, R2, invw;
 };
 
+// The weights come from the entries' tags (k_lj1g_tag): bit 30 the pair is in
+// i's half list, bit 31 in j's.
 template <int T, class R, bool E>
-__global__ void __launch_bounds__(PAIR_BLOCK) k_lj1g(RowArgs a, Lj1gP P,
-                                              const int2* __restrict__ lc, int nb_real) {
+__global__ void __launch_bounds__(PAIR_BLOCK) k_lj1g(RowArgs a, Lj1gP P, int nb_real) {
   ROW_PROLOGUE(T)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, c6 = P.c6, c12 = P.c12, c6t6 = P.c6t6, c12t12 = P.c12t12;
-  const int2 mi = lc[i];
-  const int li = mi.x, ci = mi.y;
   double fx = 0, fy = 0, fz = 0, e = 0;
-  walk_row_aux<T, R>(G, nbrow, n, t, i, lc, [&](int, const typename Geo<R>::P& q, const int2& mj, bool valid) {
-    int wh = valid && mj.x < ci ? 1 : 0;
-    int w = wh + (valid && li < mj.y ? 1 : 0);
+  walk_row_tag<T, R>(G, nbrow, n, t, i, [&](const typename Geo<R>::P& q, unsigned tag, bool valid) {
+    int wh = valid && (tag & 1u) ? 1 : 0;
+    int w = wh + (valid && (tag & 2u) ? 1 : 0);
     R dx, dy, dz;
     R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
     // f_dfr_cut (cut_off_poly.f90:30-43), dfr without the 1/(R2-R1) factor
@@ -769,8 +793,8 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         int T = h->lanes;
         DISPATCH_T(T, {
           int nb = row_blocks(a.nrows, T, BS);
-          if (energy) PFM_LAUNCH(h, "lj1g", (k_lj1g<T, R, true>), pad8(nb), BS, a, P, it.lc, nb);
-          else PFM_LAUNCH(h, "lj1g", (k_lj1g<T, R, false>), pad8(nb), BS, a, P, it.lc, nb);
+          if (energy) PFM_LAUNCH(h, "lj1g", (k_lj1g<T, R, true>), pad8(nb), BS, a, P, nb);
+          else PFM_LAUNCH(h, "lj1g", (k_lj1g<T, R, false>), pad8(nb), BS, a, P, nb);
         });
         written = true;
         break;

@@ -424,6 +424,25 @@ __global__ void __launch_bounds__(256) k_lj1g_cut(int nrows, const int* __restri
   if (t == 0) lc[i].y = c;
 }
 
+// lj1g: each entry's weight in the gather, [loc(j) < cut(i)] (the pair is in
+// i's half list) and [loc(i) < cut(j)] (in j's), packed into bits 30 and 31
+// of the entry once per rebuild, so the force pass gathers positions only
+__global__ void __launch_bounds__(256) k_lj1g_tag(int nrows, const int* __restrict__ rows,
+                                                  const int* __restrict__ nnum, int* __restrict__ nbr, int stride,
+                                                  const int2* __restrict__ lc) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = gt / 8, t = threadIdx.x & 7;
+  if (r >= nrows) return;
+  const int2 mi = lc[rows[r]];
+  const int n = nnum[r];
+  for (int p = t; p < n; p += 8) {
+    const int j = nbr[(size_t)r * stride + p];
+    const int2 mj = lc[j];
+    const unsigned tag = (mj.x < mi.y ? 1u : 0u) | (mi.x < mj.y ? 2u : 0u);
+    nbr[(size_t)r * stride + p] = (int)((unsigned)j | (tag << 30));
+  }
+}
+
 // graphene nearest neighbours from the tb list: entries with |dr| < r_cut_nn,
 // exactly three per atom (find_gr_nearest_neighbors, graphenenorm.f90:8-36)
 __global__ void k_nn_from_tb(int nrows, const int* __restrict__ rows, const int* __restrict__ tnnum,
@@ -751,6 +770,8 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
     PFM_LAUNCH(h, "list_cut", k_lj1g_loc, (h->N + 255) / 256, 256, h->N, h->orig, h->grp.loc[L.g2], it.lc);
     PFM_LAUNCH(h, "list_cut", k_lj1g_cut<8>, (int)(((long)L.nrows * 8 + 255) / 256), 256, L.nrows, h->grp.rows[L.g1],
                L.nnum, L.nbr, L.stride, h->orig, it.lc);
+    PFM_LAUNCH(h, "list_cut", k_lj1g_tag, (int)(((long)L.nrows * 8 + 255) / 256), 256, L.nrows, h->grp.rows[L.g1],
+               L.nnum, L.nbr, L.stride, it.lc);
   }
   L.built = true;
   return PFM_OK;
