@@ -772,6 +772,7 @@ static int finalize(pfm_ctx* h) {
       if ((rc = dalloc(h, &it.gnorm, h->N))) return rc;
       if ((rc = dalloc(h, &it.gC, 3 * (size_t)h->N))) return rc;
       if ((rc = dalloc(h, &it.gnf, h->N))) return rc;
+      if ((rc = dalloc(h, &it.mrows, (size_t)it.nl[1].nrows + 1))) return rc;
       std::vector<char> in1(h->N, 0);
       for (int x : h->grp.idx[it.nl[0].g1]) in1[x] = 1;
       it.rows_disjoint = true;
@@ -2365,7 +2366,7 @@ int pfm_destroy(pfm_ctx* h) {
   for (int g = 0; g <= PFM_MAXG; ++g) { F(h->grp.rows[g]); F(h->grp.row_of[g]); F(h->grp.loc[g]); F(h->grp.rank[g]); }
   for (auto& it : h->inter) {
     for (int k = 0; k < 3; ++k) { F(it.nl[k].nnum); F(it.nl[k].nbr); F(it.nl[k].rev); }
-    F(it.e_row); F(it.lc); F(it.B); F(it.Bp); F(it.gnorm); F(it.gC); F(it.gnf);
+    F(it.e_row); F(it.lc); F(it.B); F(it.Bp); F(it.gnorm); F(it.gC); F(it.gnf); F(it.mrows);
     sur_free(it);
   }
   for (auto e : h->evpool) (void)hipEventDestroy(e);

@@ -663,11 +663,24 @@ __global__ void __launch_bounds__(256) k_cos_nn(RowArgs a, const int* __restrict
 
 // metal side (*_forces_for_other_atoms): rows of nl(2), neighbours are
 // graphene atoms whose normal enters V3 (morsec divides by |n|); the mixed
-// path gathers the FP32 copy of the normals (the same values it rounds to)
+// path gathers the FP32 copy of the normals (the same values it rounds to).
+// Only the rows with entries run, from the work list k_mark_rows builds with
+// the list (most metal atoms lie out of reach of the sheet); the pass always
+// adds to forces an earlier pass has written, so the other rows keep theirs.
+// Work items take hardware block order, which spreads the short list over
+// all XCDs.
 template <int T, class R>
 __device__ __forceinline__ void cos_metal_rows(const RowArgs& a, const CosP& P, const double4* __restrict__ gnorm,
-                                               const float4* __restrict__ gnf, int block) {
-  ROW_PROLOGUE_AT(T, block)
+                                               const float4* __restrict__ gnf, const int* __restrict__ mrows,
+                                               int block) {
+  const int w = (block * blockDim.x + threadIdx.x) / T, t = threadIdx.x & (T - 1);
+  if (w >= mrows[a.nrows]) return;
+  const int row = mrows[w];
+  const int i = a.rows[row];
+  const Geo<R> G = geo_of<R>(a, i);
+  const typename Geo<R>::P pi = G.at(i);
+  const int n = a.nnum[row];
+  const int* nbrow = a.nbr + (size_t)row * a.stride;
   const R R2 = P.R2, pref = P.pref;
   double fx = 0, fy = 0, fz = 0;
   auto f = [&](int, const typename Geo<R>::P& q, const auto& g, bool valid) {
@@ -691,22 +704,24 @@ __device__ __forceinline__ void cos_metal_rows(const RowArgs& a, const CosP& P, 
 
 template <int T, class R>
 __global__ void __launch_bounds__(PAIR_BLOCK) k_cos_metal(RowArgs a, CosP P, const double4* __restrict__ gnorm,
-                                                          const float4* __restrict__ gnf, int nb_real) {
-  cos_metal_rows<T, R>(a, P, gnorm, gnf, xcd_block(nb_real));
+                                                          const float4* __restrict__ gnf,
+                                                          const int* __restrict__ mrows, int nb_real) {
+  if ((int)blockIdx.x >= nb_real) return;
+  cos_metal_rows<T, R>(a, P, gnorm, gnf, mrows, blockIdx.x);
 }
 
 // the normal-derivative rows (graphene atoms) and the metal rows in one
 // launch, for groups without common atoms (every force is written once):
-// logical blocks [0, nb_nn) take the graphene rows, the rest the metal rows
+// blocks [0, nb_nn) take the graphene rows, the rest the metal work list
 template <int T, class R>
 __global__ void __launch_bounds__(256) k_cos_nn_metal(RowArgs an, RowArgs am, CosP P, const int* __restrict__ row_of,
                                                       const int* __restrict__ rev, const double4* __restrict__ gC,
                                                       const double4* __restrict__ gnorm, const float4* __restrict__ gnf,
-                                                      int nb_nn, int nb_real) {
-  int lb = xcd_block(nb_real);
-  if (lb < 0) return;
+                                                      const int* __restrict__ mrows, int nb_nn, int nb_real) {
+  const int lb = blockIdx.x;
+  if (lb >= nb_real) return;
   if (lb < nb_nn) cos_nn_rows(an, row_of, rev, gC, lb);
-  else cos_metal_rows<T, R>(am, P, gnorm, gnf, lb - nb_nn);
+  else cos_metal_rows<T, R>(am, P, gnorm, gnf, mrows, lb - nb_nn);
 }
 
 // ----------------------------------------------------------------- dispatch
@@ -853,6 +868,11 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
           written = true;
         }
         RowArgs c = row_args(h, it, 1, written ? 1 : 0);
+        if (c.nrows > 0 && !c.accumulate) {
+          // the metal pass skips rows without entries, so it never writes first
+          h->err = "error: the ljc/morsec metal pass would write the first forces of its step";
+          return PFM_E_INVALID;
+        }
         const bool nn = !simplified && NN.nrows > 0;
         if (nn && c.nrows > 0 && it.rows_disjoint && BS == BS1) {
           RowArgs b = row_args(h, it, 2, written ? 1 : 0);
@@ -860,7 +880,7 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
           DISPATCH_T(T, {
             int nb = nbn + row_blocks(c.nrows, T, BS1);
             PFM_LAUNCH(h, "cos_nn_metal", (k_cos_nn_metal<T, R>), pad8(nb), BS1, b, c, P, h->grp.row_of[NN.g1],
-                       NN.rev, it.gC, it.gnorm, it.gnf, nbn, nb);
+                       NN.rev, it.gC, it.gnorm, it.gnf, it.mrows, nbn, nb);
           });
           written = true;
           break;
@@ -875,7 +895,7 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         if (c.nrows > 0) {
           DISPATCH_T(T, {
             int nb = row_blocks(c.nrows, T, BS);
-            PFM_LAUNCH(h, "cos_metal", (k_cos_metal<T, R>), pad8(nb), BS, c, P, it.gnorm, it.gnf, nb);
+            PFM_LAUNCH(h, "cos_metal", (k_cos_metal<T, R>), pad8(nb), BS, c, P, it.gnorm, it.gnf, it.mrows, nb);
           });
           written = true;
         }

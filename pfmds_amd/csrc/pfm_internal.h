@@ -335,6 +335,7 @@ struct Inter {
   bool rows_disjoint = false;  // groups 1 and 2 share no atom (passes over both may share a launch)
   double4* gC = nullptr;     // [3N] per graphene atom j and nearest neighbour l: c_l n_j of the normal-derivative term
   float4* gnf = nullptr;     // [N] FP32 copy of gnorm for the mixed metal pass
+  int* mrows = nullptr;      // metal rows (nl(2)) with entries, in any order, and their count after them
   // surrogate
   SurBuf* sur = nullptr;
   std::vector<double> sur_theta;  // parameters set before the buffers exist

@@ -424,6 +424,21 @@ __global__ void __launch_bounds__(256) k_lj1g_cut(int nrows, const int* __restri
   if (t == 0) lc[i].y = c;
 }
 
+// ljc/morsec metal rows (nl(2)) that have entries, appended in any order
+// (each row's force is summed on its own, so the order of the work list does
+// not change any result); the count goes after the rows.  One wave-level
+// atomic per wavefront.
+__global__ void k_mark_rows(int nrows, const int* __restrict__ nnum, int* __restrict__ mrows) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool keep = r < nrows && nnum[r] > 0;
+  const unsigned long long bal = __ballot(keep);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0 && bal) base = atomicAdd(&mrows[nrows], __popcll(bal));
+  base = __shfl(base, 0);
+  if (keep) mrows[base + __popcll(bal & ((1ull << lane) - 1ull))] = r;
+}
+
 // lj1g: each entry's weight in the gather, [loc(j) < cut(i)] (the pair is in
 // i's half list) and [loc(i) < cut(j)] (in j's), packed into bits 30 and 31
 // of the entry once per rebuild, so the force pass gathers positions only
@@ -765,6 +780,10 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   if (it.kind == PFM_TB || it.kind == PFM_REBOSC || (k == 2 && !(int)it.p[it.kind == PFM_LJC ? 5 : 6])) {
     PFM_LAUNCH(h, "list_rev", k_rev, nb, 256, L.nrows, h->grp.rows[L.g1], h->grp.row_of[L.g1], L.nnum, L.nbr,
                L.stride, L.rev, h->derr, h->orig, step - h->step_base);
+  }
+  if ((it.kind == PFM_LJC || it.kind == PFM_MORSEC) && k == 1) {
+    PFM_LAUNCH(h, "list_rows", k_fill_int, 1, 64, 1, it.mrows + L.nrows, 0);
+    PFM_LAUNCH(h, "list_rows", k_mark_rows, (L.nrows + 255) / 256, 256, L.nrows, L.nnum, it.mrows);
   }
   if (it.kind == PFM_LJ1G) {
     PFM_LAUNCH(h, "list_cut", k_lj1g_loc, (h->N + 255) / 256, 256, h->N, h->orig, h->grp.loc[L.g2], it.lc);
