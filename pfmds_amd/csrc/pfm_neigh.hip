@@ -426,17 +426,23 @@ __global__ void __launch_bounds__(256) k_lj1g_cut(int nrows, const int* __restri
 
 // ljc/morsec metal rows (nl(2)) that have entries, appended in any order
 // (each row's force is summed on its own, so the order of the work list does
-// not change any result); the count goes after the rows.  One wave-level
-// atomic per wavefront.
-__global__ void k_mark_rows(int nrows, const int* __restrict__ nnum, int* __restrict__ mrows) {
+// not change any result); the count goes after the rows.  One atomic per
+// 1024-thread workgroup: wave ballots, then the wave counts' prefix in LDS.
+__global__ void __launch_bounds__(1024) k_mark_rows(int nrows, const int* __restrict__ nnum, int* __restrict__ mrows) {
+  __shared__ int wc[17];
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const bool keep = r < nrows && nnum[r] > 0;
   const unsigned long long bal = __ballot(keep);
-  const int lane = threadIdx.x & 63;
-  int base = 0;
-  if (lane == 0 && bal) base = atomicAdd(&mrows[nrows], __popcll(bal));
-  base = __shfl(base, 0);
-  if (keep) mrows[base + __popcll(bal & ((1ull << lane) - 1ull))] = r;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) wc[w] = __popcll(bal);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int q = 0; q < nw; ++q) { const int c = wc[q]; wc[q] = tot; tot += c; }
+    wc[16] = tot ? atomicAdd(&mrows[nrows], tot) : 0;
+  }
+  __syncthreads();
+  if (keep) mrows[wc[16] + wc[w] + __popcll(bal & ((1ull << lane) - 1ull))] = r;
 }
 
 // lj1g: each entry's weight in the gather, [loc(j) < cut(i)] (the pair is in
@@ -783,7 +789,7 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   }
   if ((it.kind == PFM_LJC || it.kind == PFM_MORSEC) && k == 1) {
     PFM_LAUNCH(h, "list_rows", k_fill_int, 1, 64, 1, it.mrows + L.nrows, 0);
-    PFM_LAUNCH(h, "list_rows", k_mark_rows, (L.nrows + 255) / 256, 256, L.nrows, L.nnum, it.mrows);
+    PFM_LAUNCH(h, "list_rows", k_mark_rows, (L.nrows + 1023) / 1024, 1024, L.nrows, L.nnum, it.mrows);
   }
   if (it.kind == PFM_LJ1G) {
     PFM_LAUNCH(h, "list_cut", k_lj1g_loc, (h->N + 255) / 256, 256, h->N, h->orig, h->grp.loc[L.g2], it.lc);
