@@ -537,16 +537,30 @@ __device__ __forceinline__ void cos_pair(const CosP& P, R r, R nd, R nnorm_r, R&
 // of  c V2 V3 fc /(den (n_j.dr_jp)) (dr_jp . W) n_j  is linear in dr_jp, so
 // it equals  c/den (S_j . W) n_j  with S_j = sum_p V2 V3 fc/(n_j.dr_jp) dr_jp.
 // Lanes l = t, t+T, .. of row j store that vector for i = its l-th nearest
-// neighbour; the geometry is FP64 from the FP64 positions in both precisions.
-template <int T>
+// neighbour, in FP64.  The FP64 path takes dr from the FP64 positions; the
+// mixed path from the fixed-point records it already reads (the exact wrapped
+// difference, in FP64: within two quantisation steps, 2.7e-8 A at L = 115 A).
+template <int T, class R>
 __device__ __forceinline__ void cos_nn_coef(const RowArgs& a, const CosP& P, const int* __restrict__ nn, int row,
                                             int i, int t, double sx, double sy, double sz, const double4& g,
                                             double4* __restrict__ gC) {
   if (t >= 3) return;
-  const Geo<double> G = geo_of<double>(a, i);
-  const double4 pj = G.at(i);
   double d[3][3];
-  for (int k = 0; k < 3; ++k) G.d2(pj, G.at(nn[(size_t)row * 3 + k]), d[k][0], d[k][1], d[k][2]);
+  if constexpr (sizeof(R) == 8) {
+    const Geo<double> G = geo_of<double>(a, i);
+    const double4 pj = G.at(i);
+    for (int k = 0; k < 3; ++k) G.d2(pj, G.at(nn[(size_t)row * 3 + k]), d[k][0], d[k][1], d[k][2]);
+  } else {
+    const double3 L = a.rep ? a.rdev[a.rep[i]].L : a.L;
+    const double hx = L.x / 4294967296.0, hy = L.y / 4294967296.0, hz = L.z / 4294967296.0;
+    const uint4 pj = a.qpos[i];
+    for (int k = 0; k < 3; ++k) {
+      const uint4 q = a.qpos[nn[(size_t)row * 3 + k]];
+      d[k][0] = (double)(int)(q.x - pj.x) * hx;
+      d[k][1] = (double)(int)(q.y - pj.y) * hy;
+      d[k][2] = (double)(int)(q.z - pj.z) * hz;
+    }
+  }
   for (int l1 = t; l1 < 3; l1 += T) {
     int l2 = (l1 + 1) % 3, l3 = (l1 + 2) % 3;
     double d12[3], d31[3], d23[3];
@@ -612,7 +626,7 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_cos_graphene(RowArgs a, CosP P, 
   if (E) e = group_sum<T>(e);
   if (!simplified) {
     sx = group_sum<T>(sx); sy = group_sum<T>(sy); sz = group_sum<T>(sz);
-    cos_nn_coef<T>(a, P, nn, row, i, t, sx, sy, sz, g, gC);
+    cos_nn_coef<T, R>(a, P, nn, row, i, t, sx, sy, sz, g, gC);
   }
   if (t == 0) {
     write_force(a, i, fx, fy, fz);
