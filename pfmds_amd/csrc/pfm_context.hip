@@ -590,7 +590,7 @@ static void apply_changes(pfm_ctx* h, int s) {
 // Groups that grow keep their members under a spare group bit for the list
 // rows.  Growth is taken for roles, chain groups and the groups of every
 // interaction whose lists come from the generic search (lj, lj1g, rjl, tb,
-// ljc/morsec): rows of members that have not joined stay empty and the
+// rebosc, ljc/morsec): rows of members that have not joined stay empty and the
 // candidates are the active atoms, as find_neighbours's loops to group%N
 // (md_neighbours.f90:67,70).  A growing graphene group of ljc/morsec with a
 // tb list stops at its first step as the reference's does: the members that
@@ -602,8 +602,8 @@ static int growth_setup(pfm_ctx* h) {
   for (const auto& c : h->changes) grows[c.to] = true;
   for (auto& it : h->inter)
     for (int k = 0; k < it.nl_n; ++k)
-      if ((grows[it.nl[k].g1] || grows[it.nl[k].g2]) && (it.kind == PFM_REBOSC || it.kind == PFM_SUR))
-        return fail(h, PFM_E_INVALID, "error: a changing group in the lists of rebosc or of a surrogate");
+      if ((grows[it.nl[k].g1] || grows[it.nl[k].g2]) && it.kind == PFM_SUR)
+        return fail(h, PFM_E_INVALID, "error: a changing group in the lists of a surrogate");
   int spare = PFM_MAXG;
   auto taken = [&](int g) {
     if (!h->grp.idx[g].empty() || h->grp.used[g] || h->grp.shadow[g]) return true;

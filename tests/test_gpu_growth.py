@@ -10,8 +10,9 @@ of their step.  The same bar holds for lj1g, for a tb sheet whose atoms join
 one at a time, and for metal adatoms joining the rjl and morsec metal group
 under graphene (tb + morsec with nearest-neighbour terms).  A growing
 graphene group of morsec with a tb list stops at step 0 with the reference's
-"not enough gr nearest neibs" status, on the GPU and in the checker; growth
-in the groups of rebosc is refused.
+"not enough gr nearest neibs" status, on the GPU and in the checker.  A
+rebosc sheet growing the same way meets rebosc's own bars (forces within 1e-6
+of the largest, positions and velocities within 1e-8).
 """
 import numpy as np
 import pytest
@@ -131,10 +132,23 @@ def test_growing_graphene_group_with_tb_stops():
     assert codes == [capi.E_GRAPHENE_NN, capi.E_GRAPHENE_NN]
 
 
-def test_growth_of_a_rebosc_group_refused():
+def test_growth_rebosc_matches_checker():
     s = growth_case.carbon_growth_tb()
     s.interactions = [Interaction("rebosc", list(configs.REBO_C), [(1, 1, 8, 2.3, 10)])]
-    c = gpu_ctx(s, capi.FP64)
-    with pytest.raises(capi.PfmError):
-        c.run("nve", 0.5, 0, 1, True)
-    c.close()
+    print(f"[growth rebosc] n={s.n}", flush=True)
+    g = gpu_ctx(s, capi.FP64)
+    o = oracle_ctx(s)
+    for s0, s1 in CHUNKS:
+        g.run("nve", 0.5, s0, s1, True)
+        o.run("nve", 0.5, s0, s1, True)
+        assert g.group_count(1) == o.group_count(1)
+        (pg, vg, fg), (po, vo, fo) = g.state(), o.state()
+        # rebosc's bars: central-difference forces within 1e-6 of the largest
+        # force, trajectories within 1e-8
+        assert rel_inf(pg, po) < 1e-8 and rel_inf(vg, vo) < 1e-8 and rel_inf(fg, fo) < 1e-6, s1
+        assert g.energies()["pe"] == pytest.approx(o.energies()["pe"], rel=1e-10)
+        if s1 % 10 == 0:
+            assert list_sets(g, 1, 1)[0] == list_sets(o, 1, 1)[0], s1
+    assert o.group_count(1) < len(s.group_indices(1))
+    g.close()
+    o.close()
