@@ -741,6 +741,16 @@ static int finalize(pfm_ctx* h) {
   for (auto& it : h->inter) {
     for (int k = 0; k < it.nl_n; ++k) {
       ListDev& L = it.nl[k];
+      {
+        // atoms per cell over the replicas that hold rows of the list
+        double atoms = 0.0, cells = 0.0;
+        for (int r = 0; r < nrep; ++r) {
+          if (gcnt[L.g1][r] == 0) continue;
+          atoms += h->rep_n[r];
+          cells += (double)h->hrep[r].nc[0] * h->hrep[r].nc[1] * h->hrep[r].nc[2];
+        }
+        L.sparse_cells = cells > 0.0 && atoms < 14.0 * cells;
+      }
       // rows of the pair lists start on 16 bytes and end 32 entries early
       // enough that the passes' four-entry loads stay inside the array; the
       // graphene nearest-neighbour rows (k == 2) are three entries wide

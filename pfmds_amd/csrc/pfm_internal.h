@@ -297,6 +297,7 @@ struct ListDev {
   int* nbr = nullptr;   // [nrows*stride] internal atom indices
   int* rev = nullptr;   // [nrows*stride] slot of the row atom in the neighbour's row (tb, nn lists)
   bool built = false;
+  bool sparse_cells = false;  // the replicas holding its rows average few atoms per cell: one wave per cell
 };
 
 // device buffers of a surrogate interaction (pfm_surrogate.hip)

@@ -720,6 +720,211 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
   }
 }
 
+// Wave-per-cell variant of k_build_tile for sparse grids (a workgroup per
+// cell would leave most of its lanes without rows): each of a workgroup's
+// four waves searches one cell with its own staged tile of WTILE candidates
+// per pass.  The tile order, the tests and the output are those of
+// k_build_tile; the tile's cell table lives in lanes 0..26 and is read with
+// shuffles, which every lane of the wave takes part in.
+#define WTILE 512
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int T, bool CMP>
+__global__ void __launch_bounds__(256) k_build_tile_wave(SearchArgs a, const int* __restrict__ rank_xyz,
+                                                         const int* __restrict__ row_of, int ncell,
+                                                         int nblocks_real) {
+  static_assert(WTILE % (32 * T) == 0 && WTILE % 64 == 0, "tile passes hold whole bitmask rounds");
+  constexpr int KW = WTILE / 64;
+  __shared__ float4 stw[4][WTILE];
+  const int lb = xcd_block(nblocks_real);
+  if (lb < 0) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int c = lb * 4 + wv;
+  if (c >= ncell) return;
+  const int ncen = a.cell_count[c];
+  if (ncen == 0) return;
+  const RepDev& RG = a.rdev[a.cell_rep ? a.cell_rep[c] : 0];
+  if (!RG.tile) return;  // searched by k_build
+  const int cs = a.cell_start[c];
+  {
+    bool has = false;
+    for (int m = lane; m < ncen; m += 64) has = has || row_of[a.bin ? a.bin[cs + m] : cs + m] >= 0;
+    if (!__any(has)) return;
+  }
+  float4* st = stw[wv];
+  const int pk = rank_xyz[c];
+  const int cc[3] = {pk & 1023, (pk >> 10) & 1023, (pk >> 20) & 1023};
+  const double org[3] = {cc[0] * RG.clen.x, cc[1] * RG.clen.y, cc[2] * RG.clen.z};
+  const double3 boxL = RG.L, hb = RG.hb;
+  // the 27 tile cells in lanes 0..26: first atom, count, exclusive offset
+  int kst = 0, kcn = 0;
+  if (lane < 27) {
+    const int o[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
+    int q[3];
+    for (int d = 0; d < 3; ++d) {
+      int nd = RG.nc[d];
+      q[d] = cc[d] + o[d];
+      q[d] = q[d] < 0 ? q[d] + nd : (q[d] >= nd ? q[d] - nd : q[d]);
+    }
+    const int r = a.cell_rank[RG.cell_base + (q[2] * RG.nc[1] + q[1]) * RG.nc[0] + q[0]];
+    kst = a.cell_start[r];
+    kcn = a.cell_count[r];
+  }
+  int inc = kcn;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(inc, o);
+    if (lane >= o) inc += v;
+  }
+  const int koff = inc - kcn;
+  const int R = __shfl(inc, 63);
+  const int koff13 = __shfl(koff, 13);
+  const float hi = (float)(a.rc2 * (1.0 + 3e-5)), lo = (float)(a.rc2 * (1.0 - 3e-5));
+  // tile slot m < R: the last cell k with off[k] <= m (every lane takes part)
+  auto cell_of = [&](int m) -> int {
+    int k = 0;
+#pragma unroll
+    for (int sft = 16; sft > 0; sft >>= 1) {
+      const int kk = k + sft;
+      const int o = __shfl(koff, kk < 27 ? kk : 0);
+      if (kk < 27 && o <= m) k = kk;
+    }
+    return k;
+  };
+  // stage tile slots [s0, s1) (s1 - s0 <= WTILE): every slot, atoms outside
+  // group 2 far away; with CMP only group-2 atoms, compacted in tile order.
+  // Returns the staged length padded to whole rounds.
+  bool dense = true;
+  auto stage = [&](int s0, int s1) -> int {
+    float4 v[KW];
+    bool keep[KW];
+#pragma unroll
+    for (int u = 0; u < KW; ++u) {
+      const int m = s0 + u * 64 + lane;
+      const int k = cell_of(m < s1 ? m : s0);
+      const int ks = __shfl(kst, k), ko = __shfl(koff, k);
+      v[u] = make_float4(TILE_FAR, TILE_FAR, TILE_FAR, __int_as_float(-1));
+      keep[u] = false;
+      if (m < s1) {
+        int j = ks + (m - ko);
+        if (a.bin) j = a.bin[j];
+        const int o0 = k % 3 - 1, o1 = (k / 3) % 3 - 1, o2 = k / 9 - 1;
+        const double4 p = a.pos[j];
+        keep[u] = (a.mask[j] & a.g2bit) != 0;
+        double x = p.x - org[0], y = p.y - org[1], z = p.z - org[2];
+        if (o0 < 0 && x > 0.0) x -= boxL.x; else if (o0 > 0 && x < 0.0) x += boxL.x;
+        if (o1 < 0 && y > 0.0) y -= boxL.y; else if (o1 > 0 && y < 0.0) y += boxL.y;
+        if (o2 < 0 && z > 0.0) z -= boxL.z; else if (o2 > 0 && z < 0.0) z += boxL.z;
+        v[u].w = __int_as_float(j);
+        if (!CMP || keep[u]) {
+          if (keep[u]) { v[u].x = (float)x; v[u].y = (float)y; v[u].z = (float)z; }
+        }
+      }
+    }
+    int base = 0;
+    if constexpr (!CMP) {
+#pragma unroll
+      for (int u = 0; u < KW; ++u) {
+        const int m = s0 + u * 64 + lane;
+        if (m < s1) st[m - s0] = v[u];
+      }
+      base = s1 - s0;
+    } else {
+#pragma unroll
+      for (int u = 0; u < KW; ++u) {
+        const unsigned long long bal = __ballot(keep[u]);
+        if (keep[u]) st[base + __popcll(bal & ((1ull << lane) - 1ull))] = v[u];
+        base += __popcll(bal);
+      }
+      dense = base == s1 - s0;
+    }
+    const int np = (base + 32 * T - 1) / (32 * T) * (32 * T);
+    for (int m = base + lane; m < np; m += 64) st[m] = make_float4(TILE_FAR, TILE_FAR, TILE_FAR, __int_as_float(-1));
+    return np;
+  };
+  const bool single = R <= WTILE;
+  int np = 0;
+  if (single) np = stage(0, R);
+  wave_sync();
+
+  const int t = lane & (T - 1);
+  for (int q0 = 0; q0 < ncen; q0 += 64 / T) {
+    const int qq = q0 + lane / T;
+    const bool inrow = qq < ncen;
+    const int i = a.bin ? a.bin[cs + (inrow ? qq : 0)] : cs + (inrow ? qq : 0);
+    const int row = inrow ? row_of[i] : -1;
+    const bool active = row >= 0 && (a.mask[i] & a.g1bit);
+    const double4 pi = a.pos[i];
+    const float px = (float)(pi.x - org[0]), py = (float)(pi.y - org[1]), pz = (float)(pi.z - org[2]);
+    int* out = a.nbr + (size_t)(active ? row : 0) * a.stride;
+    int count = 0;
+    for (int s0 = 0; s0 < R; s0 += WTILE) {
+      if (!single) {
+        wave_sync();
+        np = stage(s0, R < s0 + WTILE ? R : s0 + WTILE);
+        wave_sync();
+      }
+      if (!active) continue;
+      // the row atom's own staged slot when the pass is dense (else found by index)
+      const int own = (!CMP || dense) ? koff13 + qq - s0 : -1;
+      for (int c0 = 0; c0 < np; c0 += 32 * T) {
+        const float4* sv = st + c0 + t;
+        unsigned hit = 0u;
+#pragma unroll 8
+        for (int b = 0; b < 32; ++b) {
+          const float4 w = sv[b * T];
+          float dx = w.x - px, dy = w.y - py, dz = w.z - pz;
+          float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          hit = hit + hit + (d2 <= hi ? 1u : 0u);
+        }
+        const int ob = own - c0 - t;
+        if (ob >= 0 && ob < 32 * T && ob % T == 0) hit &= ~(0x80000000u >> (ob / T));
+        unsigned chk = hit;
+        while (chk) {
+          const int b = __clz(chk);
+          chk &= ~(0x80000000u >> b);
+          const float4 w = sv[b * T];
+          const int j = __float_as_int(w.w);
+          float dx = w.x - px, dy = w.y - py, dz = w.z - pz;
+          if (CMP && !dense && j == i) {
+            hit &= ~(0x80000000u >> b);
+          } else if (fmaf(dx, dx, fmaf(dy, dy, dz * dz)) >= lo) {
+            double ex, ey, ez;
+            if (!(min_image_dist2(pi, a.pos[j], hb, ex, ey, ez) < a.rc2)) hit &= ~(0x80000000u >> b);
+          }
+        }
+        const int nh = __popc(hit);
+        int pre = nh;
+#pragma unroll
+        for (int o = 1; o < T; o <<= 1) {
+          int w = __shfl_up(pre, o, T);
+          if (t >= o) pre += w;
+        }
+        const int tot = __shfl(pre, T - 1, T);
+        int slot = count + pre - nh;
+        while (hit) {
+          const int b = __clz(hit);
+          hit &= ~(0x80000000u >> b);
+          if (slot < a.nmax) out[slot] = __float_as_int(sv[b * T].w);
+          ++slot;
+        }
+        count += tot;
+      }
+    }
+    if (row >= 0 && !active && t == 0) a.nnum[row] = 0;
+    if (active && t == 0) {
+      a.nnum[row] = count < a.nmax ? count : a.nmax;
+      if (count > a.nmax) raise_err(a.err, a.err_code, a.orig[i] + 1, count, a.step);
+      else if (a.exact && count != a.nmax) raise_err(a.err, PFM_E_GRAPHENE_NN, a.orig[i] + 1, count, a.step);
+    }
+  }
+}
+
 template <int T>
 static void launch_build_T(pfm_ctx* h, SearchArgs& a, const int* row_of) {
   long threads = (long)a.nrows * T;
@@ -727,8 +932,17 @@ static void launch_build_T(pfm_ctx* h, SearchArgs& a, const int* row_of) {
   PFM_LAUNCH(h, "build_rows", k_build<T>, pad8(nb), 256, a, nb);
 }
 
-static void launch_build(pfm_ctx* h, SearchArgs& a, const int* row_of, bool sparse) {
-  if (h->any_tile) {
+static void launch_build(pfm_ctx* h, SearchArgs& a, const int* row_of, bool sparse, bool sparse_cells) {
+  if (h->any_tile && sparse_cells) {
+    const int nc = h->ncell, nb = (nc + 3) / 4;
+    if (sparse) {
+      if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", (k_build_tile_wave<4, true>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
+      else PFM_LAUNCH(h, "build_list", (k_build_tile_wave<8, true>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
+    } else {
+      if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", (k_build_tile_wave<4, false>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
+      else PFM_LAUNCH(h, "build_list", (k_build_tile_wave<8, false>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
+    }
+  } else if (h->any_tile) {
     int nb = h->ncell;
     if (sparse) {
       if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", (k_build_tile<4, true>), pad8(nb), 256, a, h->cell_xyz, row_of, nb);
@@ -781,7 +995,7 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   a.exact = (k == 2) ? 1 : 0;
   const int* row_of = h->grp.row_of[L.g1];
   // candidates of another species: stage only those
-  launch_build(h, a, row_of, 2L * h->grp.size[L.g2] < (long)h->N);
+  launch_build(h, a, row_of, 2L * h->grp.size[L.g2] < (long)h->N, L.sparse_cells);
   int nb = (L.nrows + 255) / 256;
   if (it.kind == PFM_TB || it.kind == PFM_REBOSC || (k == 2 && !(int)it.p[it.kind == PFM_LJC ? 5 : 6])) {
     PFM_LAUNCH(h, "list_rev", k_rev, nb, 256, L.nrows, h->grp.rows[L.g1], h->grp.row_of[L.g1], L.nnum, L.nbr,
