@@ -874,7 +874,9 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         int T = h->lanes;
         int simp = simplified ? 1 : 0;
         if (a.nrows > 0) {
-          DISPATCH_T(T, {
+          // graphene rows hold about 40 metal entries: at most four lanes each
+          const int Tg = T > 4 ? 4 : T;
+          DISPATCH_T(Tg, {
             int nb = row_blocks(a.nrows, T, BS);
             if (energy) PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, true>), pad8(nb), BS, a, P, NN.nbr, simp, it.gnorm, it.gnf, it.gC, nb);
             else PFM_LAUNCH(h, "cos_graphene", (k_cos_graphene<T, R, false>), pad8(nb), BS, a, P, NN.nbr, simp, it.gnorm, it.gnf, it.gC, nb);
