@@ -70,3 +70,32 @@ def test_growth_moves_and_lists_only_joined_atoms():
     nnum, _, _ = o.get_list(1, 1)
     assert (nnum[:35] > 0).all() and (nnum[35:] == 0).all()
     o.close()
+
+
+@pytest.mark.parametrize("kind", ["tb", "rebosc"])
+def test_growth_of_a_bonded_sheet(kind):
+    # the second half of a graphene sheet joins the tb (or rebosc) group one
+    # atom at a time: atoms that have not joined neither move nor bond, and
+    # the count follows the rule
+    from pfmds_amd import configs
+    from pfmds_amd.system import Interaction
+
+    s = growth_case.carbon_growth_tb(ts1=3, ts2=40, frec=4)
+    if kind == "rebosc":
+        s.interactions = [Interaction("rebosc", list(configs.REBO_C), [(1, 1, 8, 2.3, 10)])]
+    o = oracle_ctx(s)
+    p0 = s.pos.copy()
+    o.run("nve", 0.5, 0, 25, True)
+    n = o.group_count(1)
+    assert n == growth_case.expected_counts([25], len(s.pos) // 2, len(s.pos), 3, 40, 4)[0]
+    p, _, f = o.state()
+    idx = s.group_indices(1) - 1
+    moved = np.any(p != p0, axis=1)
+    assert moved[idx[:n]].all() and not moved[idx[n:]].any()
+    assert np.all(f[idx[n:]] == 0.0)
+    nnum, nl, _ = o.get_list(1, 1)
+    # rows built at step 20: members that joined after it have none yet
+    n20 = growth_case.expected_counts([20], len(s.pos) // 2, len(s.pos), 3, 40, 4)[0]
+    assert (nnum[n20:] == 0).all() and (nnum[: len(s.pos) // 2] > 0).all()
+    assert all((nl[r, : nnum[r]] <= n20).all() for r in range(n20))
+    o.close()
