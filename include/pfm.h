@@ -266,7 +266,8 @@ int pfm_set_records(pfm_ctx* h, int n_energy_periods, const int* energy_periods,
  * empty until the first rebuild after they join, as in find_neighbours).
  * Taken for role and chain groups and the groups of lj, lj1g, rjl, tb,
  * rebosc and ljc/morsec lists (not surrogate lists: PFM_E_INVALID at the
- * first run); single systems only.  Changes are added before the first
+ * first run).  In a batch the rule applies to every replica's own counts
+ * (each replica is its own md() run).  Changes are added before the first
  * pfm_run.
  * pfm_group_count gives the active atoms of a group after the last step. */
 int pfm_add_group_change(pfm_ctx* h, int from, int to, int ts1, int ts2, int frec);

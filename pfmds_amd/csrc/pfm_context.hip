@@ -542,15 +542,17 @@ static bool is_cos(int k) { return k == PFM_LJC || k == PFM_MORSEC; }
 
 // ---------------------------------------------------------- group growth
 
-// active atoms of a growing group: the first n of its members
+// active atoms of a growing group: the first n of its members (in a batch,
+// the first rn[replica] of each replica's members)
 __global__ void k_group_count(int N, const int* __restrict__ orig, const int* __restrict__ rank, uint32_t bit, int n,
-                              uint32_t* __restrict__ mask) {
+                              const int* __restrict__ rn, const int* __restrict__ rep, uint32_t* __restrict__ mask) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   int r = rank[orig[i]];
   if (r < 0) return;
+  const int lim = rn ? rn[rep[i]] : n;
   uint32_t m = mask[i];
-  mask[i] = r < n ? (m | bit) : (m & ~bit);
+  mask[i] = r < lim ? (m | bit) : (m & ~bit);
 }
 
 // change_particle_group_N (md_general.f90:82-94) on host counts
@@ -570,21 +572,58 @@ static void change_counts(int* n, const int* size, const std::vector<pfm_ctx::Gr
 // whether the changes at step s alter any count, starting from the current ones
 static bool counts_change_at(const pfm_ctx* h, int s) {
   if (h->changes.empty()) return false;
+  if (h->nrep > 1) {
+    for (int r = 0; r < h->nrep; ++r) {
+      int n[PFM_MAXG + 1], n0[PFM_MAXG + 1], size[PFM_MAXG + 1];
+      for (int g = 0; g <= PFM_MAXG; ++g) {
+        n0[g] = n[g] = h->grp.rn[g].empty() ? 0 : h->grp.rn[g][r];
+        size[g] = h->grp.rsize[g].empty() ? 0 : h->grp.rsize[g][r];
+      }
+      change_counts(n, size, h->changes, s);
+      if (!std::equal(n, n + PFM_MAXG + 1, n0)) return true;
+    }
+    return false;
+  }
   int n[PFM_MAXG + 1];
   std::copy(h->grp.n, h->grp.n + PFM_MAXG + 1, n);
   change_counts(n, h->grp.size, h->changes, s);
   return !std::equal(n, n + PFM_MAXG + 1, h->grp.n);
 }
 
-// the changes of step s: host counts, then the active bits of the groups that moved
+// the changes of step s: host counts, then the active bits of the groups that
+// moved.  A batch applies the rule to every replica's own counts (each
+// replica is its own md() run); the group's count is their sum.
 static void apply_changes(pfm_ctx* h, int s) {
   int old[PFM_MAXG + 1];
   std::copy(h->grp.n, h->grp.n + PFM_MAXG + 1, old);
-  change_counts(h->grp.n, h->grp.size, h->changes, s);
+  if (h->nrep > 1) {
+    for (int r = 0; r < h->nrep; ++r) {
+      int n[PFM_MAXG + 1], size[PFM_MAXG + 1];
+      for (int g = 0; g <= PFM_MAXG; ++g) {
+        n[g] = h->grp.rn[g].empty() ? 0 : h->grp.rn[g][r];
+        size[g] = h->grp.rsize[g].empty() ? 0 : h->grp.rsize[g][r];
+      }
+      change_counts(n, size, h->changes, s);
+      for (int g = 0; g <= PFM_MAXG; ++g)
+        if (!h->grp.rn[g].empty()) h->grp.rn[g][r] = n[g];
+    }
+    for (int g = 1; g <= PFM_MAXG; ++g) {
+      if (h->grp.rn[g].empty()) continue;
+      int tot = 0;
+      for (int r = 0; r < h->nrep; ++r) tot += h->grp.rn[g][r];
+      h->grp.n[g] = tot;
+    }
+  } else {
+    change_counts(h->grp.n, h->grp.size, h->changes, s);
+  }
   for (int g = 1; g <= PFM_MAXG; ++g)
-    if (h->grp.n[g] != old[g] && h->grp.rank[g])
+    if (h->grp.n[g] != old[g] && h->grp.rank[g]) {
+      if (h->grp.d_rn[g])
+        (void)hipMemcpyAsync(h->grp.d_rn[g], h->grp.rn[g].data(), sizeof(int) * h->nrep, hipMemcpyHostToDevice,
+                             h->stream);
       PFM_LAUNCH(h, "group_count", k_group_count, (h->N + 255) / 256, 256, h->N, h->orig, h->grp.rank[g],
-                 1u << (g - 1), h->grp.n[g], h->mask);
+                 1u << (g - 1), h->grp.n[g], h->grp.d_rn[g], h->rep, h->mask);
+    }
 }
 
 // Groups that grow keep their members under a spare group bit for the list
@@ -597,7 +636,6 @@ static void apply_changes(pfm_ctx* h, int s) {
 // have not joined have no nearest neighbours (graphenenorm.f90:30).
 static int growth_setup(pfm_ctx* h) {
   if (h->changes.empty()) return PFM_OK;
-  if (h->nrep > 1) return fail(h, PFM_E_INVALID, "error: group changes are for single systems only");
   bool grows[PFM_MAXG + 1] = {false};
   for (const auto& c : h->changes) grows[c.to] = true;
   for (auto& it : h->inter)
@@ -638,6 +676,11 @@ static int finalize(pfm_ctx* h) {
   std::vector<std::vector<int>> gcnt(PFM_MAXG + 1, std::vector<int>(nrep, 0));
   for (int g = 1; g <= PFM_MAXG; ++g)
     for (int a : h->grp.idx[g]) ++gcnt[g][rep_of[a]];
+  if (nrep > 1 && !h->changes.empty())
+    for (int g = 1; g <= PFM_MAXG; ++g) {
+      h->grp.rn[g] = gcnt[g];  // every member active until the first step applies the changes
+      h->grp.rsize[g] = gcnt[g];
+    }
   // cell size of every replica: the largest list r_cut among the lists whose
   // groups have atoms in it
   std::vector<double> rmax(nrep, 0.0);
@@ -810,7 +853,19 @@ static int finalize(pfm_ctx* h) {
   for (int g = 1; g <= PFM_MAXG; ++g) {
     if (!h->grp.shadow[g]) continue;
     std::vector<int> rank(h->N, -1);
-    for (size_t k = 0; k < h->grp.idx[g].size(); ++k) rank[h->grp.idx[g][k]] = (int)k;
+    if (nrep > 1) {
+      // position among the members of the atom's own replica
+      std::vector<int> seen(nrep, 0);
+      for (size_t k = 0; k < h->grp.idx[g].size(); ++k) {
+        const int a = h->grp.idx[g][k];
+        rank[a] = seen[rep_of[a]]++;
+      }
+      if ((rc = dalloc(h, &h->grp.d_rn[g], nrep))) return rc;
+      HIPCHK(h, hipMemcpyAsync(h->grp.d_rn[g], h->grp.rn[g].data(), sizeof(int) * nrep, hipMemcpyHostToDevice,
+                               h->stream));
+    } else {
+      for (size_t k = 0; k < h->grp.idx[g].size(); ++k) rank[h->grp.idx[g][k]] = (int)k;
+    }
     if ((rc = dalloc(h, &h->grp.rank[g], h->N))) return rc;
     HIPCHK(h, hipMemcpyAsync(h->grp.rank[g], rank.data(), sizeof(int) * h->N, hipMemcpyHostToDevice, h->stream));
   }
@@ -946,12 +1001,13 @@ static void set_rep_energies(pfm_ctx* h, const double* o) {
   h->rep_ke.assign(nrep, 0.0);
   h->rep_temp.assign(nrep, 0.0);
   const std::vector<int>& m0 = h->rep_row0[h->roles[0]];
+  const std::vector<int>& act = h->grp.rn[h->roles[0]];  // active counts of a growing batch
   for (int r = 0; r < nrep; ++r) {
     double p = 0.0;
     for (int k = 0; k < ni; ++k) p += o[(size_t)r * (ni + 1) + k];
     h->rep_pe[r] = p;
     h->rep_ke[r] = o[(size_t)r * (ni + 1) + ni];
-    int nm = m0[r + 1] - m0[r];
+    int nm = act.empty() ? m0[r + 1] - m0[r] : act[r];
     h->rep_temp[r] = nm > 0 ? 2 * h->rep_ke[r] / PFM_KB / (3 * nm) : 0.0;
   }
 }
@@ -2013,7 +2069,6 @@ int pfm_add_group_change(pfm_ctx* h, int from, int to, int ts1, int ts2, int fre
   if (from < 1 || from > PFM_MAXG || to < 1 || to > PFM_MAXG)
     return fail(h, PFM_E_INVALID, "error: group number out of range (1..32)");
   if (frec < 1) return fail(h, PFM_E_INVALID, "error: change frequency must be >= 1");
-  if (h->nrep > 1) return fail(h, PFM_E_INVALID, "error: group changes are for single systems only");
   h->changes.push_back(pfm_ctx::GroupChange{from, to, ts1, ts2, frec});
   return PFM_OK;
 }
@@ -2373,7 +2428,7 @@ int pfm_destroy(pfm_ctx* h) {
   F(h->spec_e); F(h->sp_pos); F(h->sp_vel); F(h->sp_frc); F(h->sp_mask); F(h->sp_orig);
   if (h->h_red) (void)hipHostFree(h->h_red);
   if (h->herr) (void)hipHostFree(h->herr);
-  for (int g = 0; g <= PFM_MAXG; ++g) { F(h->grp.rows[g]); F(h->grp.row_of[g]); F(h->grp.loc[g]); F(h->grp.rank[g]); }
+  for (int g = 0; g <= PFM_MAXG; ++g) { F(h->grp.rows[g]); F(h->grp.row_of[g]); F(h->grp.loc[g]); F(h->grp.rank[g]); F(h->grp.d_rn[g]); }
   for (auto& it : h->inter) {
     for (int k = 0; k < 3; ++k) { F(it.nl[k].nnum); F(it.nl[k].nbr); F(it.nl[k].rev); }
     F(it.e_row); F(it.lc); F(it.B); F(it.Bp); F(it.gnorm); F(it.gC); F(it.gnf); F(it.mrows);

@@ -358,7 +358,12 @@ struct Groups {
   // a group that grows at run time keeps its members under a spare bit (the
   // list rows) and its active atoms under its own bit
   int shadow[PFM_MAXG + 1] = {0};
-  int* rank[PFM_MAXG + 1] = {nullptr};  // device: original atom -> position in idx or -1 (growing groups)
+  int* rank[PFM_MAXG + 1] = {nullptr};  // device: original atom -> position in idx or -1 (growing groups);
+                                        // in a batch the position among its replica's members
+  // batches: active and member counts of every group per replica, and the
+  // device copy of a growing group's per-replica counts
+  std::vector<int> rn[PFM_MAXG + 1], rsize[PFM_MAXG + 1];
+  int* d_rn[PFM_MAXG + 1] = {nullptr};
   std::vector<int> idx[PFM_MAXG + 1];  // host: 0-based original indices, group order
   int* rows[PFM_MAXG + 1] = {nullptr};    // device: members in internal order
   int* row_of[PFM_MAXG + 1] = {nullptr};  // device: internal atom -> row or -1

@@ -306,7 +306,9 @@ def merge_replicas(systems):
     interactions equal in name, parameters and lists are shared.  Role groups
     that differ between replicas become the union of their species lists.
     All replicas must share the integrator schedule and periods; chains are
-    not allowed (a chain's group would span replicas)."""
+    not allowed (a chain's group would span replicas).  Group changes must be
+    the same lines in every replica (after matching the groups); the batch
+    applies them to each replica's own counts."""
     first = systems[0]
     gid = {}
     groups = []
@@ -342,13 +344,20 @@ def merge_replicas(systems):
                     names.append(n)
         return group_of(names)
 
+    changes = None
+    for s in systems:
+        ch = [(group_of(s.groups[f - 1]), group_of(s.groups[t - 1]), ts1, ts2, fr) for f, t, ts1, ts2, fr in s.changes]
+        if changes is None:
+            changes = ch
+        elif ch != changes:
+            raise ValueError("replicas of a batch share their group change lines")
     roles = {a: role(a) for a in ("all_moving", "xyz_moving", "z_moving", "all_atoms", "traj")}
     batch = System(box=np.array(first.box, dtype=float), pos=np.concatenate([s.pos for s in systems]),
                    vel=np.concatenate([s.vel for s in systems]), mass=np.concatenate([s.mass for s in systems]),
                    species=[x for s in systems for x in s.species], groups=groups, interactions=inters,
                    integrators=list(first.integrators), period_traj=first.period_traj,
                    zero_momentum_period=first.zero_momentum_period, invert_z=first.invert_z, ms_de=first.ms_de,
-                   md_step_limit=first.md_step_limit, **roles)
+                   md_step_limit=first.md_step_limit, changes=changes or [], **roles)
     return Batch(batch, np.array([s.n for s in systems], dtype=np.int32),
                  np.array([np.asarray(s.box, dtype=float) for s in systems]))
 

@@ -17,7 +17,7 @@ of the largest, positions and velocities within 1e-8).
 import numpy as np
 import pytest
 
-from pfmds_amd import capi, configs
+from pfmds_amd import capi, configs, system
 from pfmds_amd.system import Interaction
 from tests import growth_case
 from tests.harness import gpu_ctx, list_sets, oracle_ctx, rel_inf
@@ -152,3 +152,36 @@ def test_growth_rebosc_matches_checker():
     assert o.group_count(1) < len(s.group_indices(1))
     g.close()
     o.close()
+
+
+def test_growth_in_a_batch_matches_single_runs():
+    # three lj deposition systems of different sizes and seeds in one batch:
+    # each replica grows its own group by the shared rule, as its own md() run
+    reps = [growth_case.deposition(n_side=4, seed=101), growth_case.deposition(n_side=5, seed=102, spacing=4.0),
+            growth_case.deposition(n_side=3, seed=103, spacing=4.4)]
+    for s in reps:
+        # a batch matches groups by their species lists: keep the static
+        # all-atoms group apart from the growing group of the same species
+        s.groups[2] = ["A", "D", "#"]
+    b = system.merge_replicas(reps)
+    print(f"[growth batch] {b.n_rep} replicas, {b.system.n} atoms", flush=True)
+    g = system.build_batch_context(b, capi.FP64)
+    singles = [gpu_ctx(s, capi.FP64) for s in reps]
+    for s0, s1 in CHUNKS:
+        g.run("nve", 1.0, s0, s1, True)
+        for c in singles:
+            c.run("nve", 1.0, s0, s1, True)
+        pos, vel, frc = g.state()
+        er = g.replica_energies()
+        for r, c in enumerate(singles):
+            sl = b.replica_slice(r)
+            p1, v1, f1 = c.state()
+            assert rel_inf(pos[sl], p1) < 1e-10 and rel_inf(vel[sl], v1) < 1e-10 and rel_inf(frc[sl], f1) < 1e-10, (r, s1)
+            e = c.energies()
+            assert er["pe"][r] == pytest.approx(e["pe"], rel=1e-10)
+            assert er["temp"][r] == pytest.approx(e["temp"], rel=1e-10)
+        # the batch matches groups by species: its growing group is group 1
+        assert g.group_count(1) == sum(c.group_count(2) for c in singles)
+    g.close()
+    for c in singles:
+        c.close()
