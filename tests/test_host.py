@@ -165,3 +165,24 @@ def test_replica_mix_sizes():
                 assert dist.min() > 3.0
         else:
             assert min(r.box) >= 4 * 3.615 - 1e-9
+
+
+def test_batch_carries_group_changes():
+    # merge_replicas matches groups by species and maps every replica's change
+    # lines onto the batch's groups; lines that differ between replicas are refused
+    import pytest
+
+    from pfmds_amd import system
+    from tests import growth_case
+
+    reps = [growth_case.deposition(n_side=4, seed=1), growth_case.deposition(n_side=3, seed=2, spacing=4.4)]
+    for s in reps:
+        s.groups[2] = ["A", "D", "#"]
+    b = system.merge_replicas(reps)
+    g_to = b.system.groups.index(["A", "D"]) + 1
+    g_from = b.system.groups.index(["A"]) + 1
+    assert b.system.changes == [(g_from, g_to, 5, 40, 7)]
+    assert b.system.all_moving == g_to and b.system.all_atoms == b.system.groups.index(["A", "D", "#"]) + 1
+    reps[1].changes = [(1, 2, 6, 40, 7)]
+    with pytest.raises(ValueError):
+        system.merge_replicas(reps)
