@@ -261,6 +261,15 @@ int orc_add_nhc(octx* h, int group, double T, int M, double q1) {
   return PFM_OK;
 }
 
+/* Positions x[M] and velocities v[M] of chain c (1-based): the
+ * nose_hoover_chain state (md_general.f90:24-28); mirrors pfm_get_chain. */
+int orc_get_chain(octx* h, int c, double* x, double* v) {
+  if (c < 1 || c > h->nnhc) return oerr(h, PFM_E_INVALID, "error: no such chain");
+  const onhc* d = &h->nhc[c - 1];
+  for (int i = 0; i < d->M; ++i) { x[i] = d->x[i]; v[i] = d->v[i]; }
+  return PFM_OK;
+}
+
 /* One runtime group change of the settings (md_simulation.f90:63-70). */
 int orc_add_group_change(octx* h, int from, int to, int ts1, int ts2, int frec) {
   if (h->nchange >= MAXCHANGE || from < 1 || from > MAXG || to < 1 || to > MAXG || frec < 1)
@@ -1210,14 +1219,14 @@ static void nhc_energy(onhc* c) {
 }
 
 /* check_positions (md_general.f90:342-364) */
-static int check_positions(octx* h) {
+static int check_positions(octx* h, int step) {
   const double tol = 0.0000001;
   for (int i = 0; i < h->N; ++i)
     for (int k = 0; k < 3; ++k) {
       double x = h->pos[3 * i + k];
       if (!(x > (0.0 - tol) && x < (h->box[k] + tol))) {
-        snprintf(h->err, sizeof h->err, "%d particle out of cell %.10g %.10g %.10g", i + 1,
-                 h->pos[3 * i], h->pos[3 * i + 1], h->pos[3 * i + 2]);
+        snprintf(h->err, sizeof h->err, "%d particle out of cell %.10g %.10g %.10g (step %d)", i + 1,
+                 h->pos[3 * i], h->pos[3 * i + 1], h->pos[3 * i + 2], step);
         return PFM_E_OUT_OF_CELL;
       }
     }
@@ -1248,7 +1257,7 @@ int orc_run(octx* h, int integ, double dt, int s0, int s1, int energy_at_end, do
     *last_step = s;
     for (int c = 0; c < h->nchange; ++c)
       change_group_n(&h->groups[h->change[c].to], s, &h->change[c], &h->groups[h->change[c].from]);
-    rc = check_positions(h);
+    rc = check_positions(h, s);
     if (rc) return rc;
     if (h->invert_z) invert_z(h);
     if (s != 0) {

@@ -37,6 +37,7 @@ int  orc_set_state(octx* h, const double* pos, const double* vel);
 int  orc_get_list(octx* h, int inter, int k, int* n_rows, int* nnum, int* nlist,
                   int* lessnnum, int stride);
 int  orc_get_normals(octx* h, int inter, double* gr_norm);
+int  orc_get_chain(octx* h, int c, double* x, double* v);
 int  orc_add_group_change(octx* h, int from, int to, int ts1, int ts2, int frec);
 int  orc_group_count(octx* h, int gid, int* n);
 int  orc_pe_change(octx* h, double* de);

@@ -42,6 +42,7 @@ _SIGS = {
     "add_group_change": [C.c_void_p] + [C.c_int] * 5,
     "group_count": [C.c_void_p, C.c_int, _i],
     "pe_change": [C.c_void_p, _d],
+    "get_chain": [C.c_void_p, C.c_int, _d, _d],
 }
 _libs = {}
 
@@ -104,8 +105,6 @@ def load_library(path, prefix):
         lib.pfm_copy_state.restype = C.c_int
         lib.pfm_force_rmse.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _d]
         lib.pfm_force_rmse.restype = C.c_int
-        lib.pfm_get_chain.argtypes = [C.c_void_p, C.c_int, _d, _d]
-        lib.pfm_get_chain.restype = C.c_int
         lib.pfm_set_chain.argtypes = [C.c_void_p, C.c_int, _d, _d]
         lib.pfm_set_chain.restype = C.c_int
         lib.pfm_set_speculation.argtypes = [C.c_void_p, C.c_int]
@@ -358,7 +357,7 @@ class Context:
     def get_chain(self, c, M):
         """(x[M], v[M]) of Nose-Hoover chain c (1-based)."""
         x, v = np.zeros(max(M, 1)), np.zeros(max(M, 1))
-        self._check(self.lib.pfm_get_chain(self.h, int(c), _dp(x), _dp(v)))
+        self._check(self._fn("get_chain")(self.h, int(c), _dp(x), _dp(v)))
         return x[:M], v[:M]
 
     def set_chain(self, c, x, v):

@@ -79,7 +79,7 @@ __global__ void k_check(int N, const double4* __restrict__ pos, double bx, doubl
     bx = R.L.x; by = R.L.y; bz = R.L.z; zlo = R.zlo; zhi = R.zhi;
   }
   double4 p = pos[i];
-  if (!(in_cell(p.x, bx) && in_cell(p.y, by) && in_cell(p.z, bz))) raise_err(err, PFM_E_OUT_OF_CELL, orig[i] + 1, 0, step);
+  if (!(in_cell(p.x, bx) && in_cell(p.y, by) && in_cell(p.z, bz))) raise_out_of_cell(err, orig[i] + 1, step, p);
   if (invert) {
     double4 v = vel[i];
     invert_v(v, p.z, zlo, zhi);
@@ -596,6 +596,9 @@ static bool counts_change_at(const pfm_ctx* h, int s) {
 static void apply_changes(pfm_ctx* h, int s) {
   int old[PFM_MAXG + 1];
   std::copy(h->grp.n, h->grp.n + PFM_MAXG + 1, old);
+  // a batch's group moved when any replica's count moved (the per-replica
+  // counts can change while their sum stays the same)
+  bool moved[PFM_MAXG + 1] = {false};
   if (h->nrep > 1) {
     for (int r = 0; r < h->nrep; ++r) {
       int n[PFM_MAXG + 1], size[PFM_MAXG + 1];
@@ -605,7 +608,10 @@ static void apply_changes(pfm_ctx* h, int s) {
       }
       change_counts(n, size, h->changes, s);
       for (int g = 0; g <= PFM_MAXG; ++g)
-        if (!h->grp.rn[g].empty()) h->grp.rn[g][r] = n[g];
+        if (!h->grp.rn[g].empty()) {
+          moved[g] = moved[g] || h->grp.rn[g][r] != n[g];
+          h->grp.rn[g][r] = n[g];
+        }
     }
     for (int g = 1; g <= PFM_MAXG; ++g) {
       if (h->grp.rn[g].empty()) continue;
@@ -617,7 +623,7 @@ static void apply_changes(pfm_ctx* h, int s) {
     change_counts(h->grp.n, h->grp.size, h->changes, s);
   }
   for (int g = 1; g <= PFM_MAXG; ++g)
-    if (h->grp.n[g] != old[g] && h->grp.rank[g]) {
+    if ((h->grp.n[g] != old[g] || moved[g]) && h->grp.rank[g]) {
       if (h->grp.d_rn[g])
         (void)hipMemcpyAsync(h->grp.d_rn[g], h->grp.rn[g].data(), sizeof(int) * h->nrep, hipMemcpyHostToDevice,
                              h->stream);
@@ -907,7 +913,9 @@ static int check_err(pfm_ctx* h, int s1) {
   char buf[256];
   switch (e.code) {
     case PFM_E_OVERFLOW: snprintf(buf, sizeof buf, "error: too many neighbours %d %d (step %d)", e.atom, e.count, e.step); break;
-    case PFM_E_OUT_OF_CELL: snprintf(buf, sizeof buf, "%d particle out of cell (step %d)", e.atom, e.step); break;
+    case PFM_E_OUT_OF_CELL:
+      snprintf(buf, sizeof buf, "%d particle out of cell %.10g %.10g %.10g (step %d)", e.atom, e.x, e.y, e.z, e.step);
+      break;
     case PFM_E_GRAPHENE_NN:
       if (e.count < 0) snprintf(buf, sizeof buf, "l1>3 (atom %d, step %d)", e.atom, e.step);
       else if (e.count > 3) snprintf(buf, sizeof buf, "error: too many gr nearest neibs %d %d (step %d)", e.atom, e.count, e.step);
@@ -932,6 +940,17 @@ static IntP int_params(pfm_ctx* h, double dt) {
   P.zlo = 0.8 * h->box[2];
   P.zhi = 0.9 * h->box[2];
   return P;
+}
+
+// The top of the step at which an nvms run ends: md() runs check_positions
+// and invert_z_velocities before its exit test (md_simulation.f90:139-146),
+// so the state it leaves has the wall applied and an atom the last drift put
+// out of the cell still stops the run.
+static int exit_step_top(pfm_ctx* h, int s) {
+  const double zlo = 0.8 * h->box[2], zhi = 0.9 * h->box[2];
+  PFM_LAUNCH(h, "check", k_check, (h->N + 255) / 256, 256, h->N, h->pos4, h->box[0], h->box[1], h->box[2], h->derr,
+             h->orig, s, h->invert_z, h->vel4, zlo, zhi, h->rep, h->rdev);
+  return check_err(h, s);
 }
 
 static MaskQ chain_masks(pfm_ctx* h) {
@@ -1236,7 +1255,13 @@ static std::vector<uint64_t> graph_key(pfm_ctx* h, int integ, double dt, int blk
   return k;
 }
 
+// Graphs are launched asynchronously: one may still be running on the
+// handle's stream, so the stream is drained before any executable graph is
+// destroyed (the runtime and any tool hooked into it may still read the
+// graph's launch records until its last kernel has finished).
 static void clear_graphs(pfm_ctx* h) {
+  if (h->graphs.empty()) return;
+  (void)hipStreamSynchronize(h->stream);
   for (auto& e : h->graphs) (void)hipGraphExecDestroy(e.exec);
   h->graphs.clear();
 }
@@ -1804,6 +1829,8 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
                                     [](const pfm_ctx::GraphEntry& a, const pfm_ctx::GraphEntry& b) {
                                       return a.used < b.used;
                                     });
+        // the evicted block may still be running: drain the stream first
+        HIPCHK(h, hipStreamSynchronize(h->stream));
         (void)hipGraphExecDestroy(old->exec);
         h->graphs.erase(old);
       }
@@ -1857,7 +1884,7 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
       *last_step = s;
       if (s != 0 && conv(h->pe, h->pe_prev, h->rep_pe, h->rep_pe_prev)) {
         *event = PFM_EV_CONVERGED;
-        return PFM_OK;
+        return exit_step_top(h, s);
       }
       int end = std::min(s1, (s / K + 1) * K - 1);
       int ne_b = 0, nx_b = 0;
@@ -1957,7 +1984,7 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
         if (t != 0) h->sim_time = h->sim_time + dt;
       *last_step = tc;
       *event = PFM_EV_CONVERGED;
-      return PFM_OK;
+      return exit_step_top(h, tc);
     }
     return check_err(h, s1);
   };
@@ -2006,7 +2033,7 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
     int rc2 = body(s);
     if (rc2 == CONVERGED) {
       *event = PFM_EV_CONVERGED;
-      return PFM_OK;
+      return exit_step_top(h, s);
     }
     if (rc2) return rc2;
     if (re || rx) {

@@ -184,11 +184,19 @@ __device__ __forceinline__ int xcd_block(int nblocks_real) {
 // captured in; `base` holds the first step of the block being replayed (0
 // outside graphs).
 struct DevErr {
-  int code, atom, count, step, base;
+  int code, atom, count, step, base, pad;
+  double x, y, z;  // position of the atom of an out-of-cell error
 };
 __device__ __forceinline__ void raise_err(DevErr* e, int code, int atom, int count, int step) {
   if (atomicCAS(&e->code, 0, code) == 0) {
     e->atom = atom; e->count = count; e->step = step + e->base;
+  }
+}
+// check_positions (md_general.f90:342-364) reports the atom with its position
+__device__ __forceinline__ void raise_out_of_cell(DevErr* e, int atom, int step, const double4& p) {
+  if (atomicCAS(&e->code, 0, PFM_E_OUT_OF_CELL) == 0) {
+    e->atom = atom; e->count = 0; e->step = step + e->base;
+    e->x = p.x; e->y = p.y; e->z = p.z;
   }
 }
 
@@ -284,7 +292,7 @@ __device__ __forceinline__ void kick_drift_atom(int i, double4 p, uint32_t m, co
   vel[i] = v;
   if (m & P.zero_bit) frc[i] = make_double4(0, 0, 0, 0);
   if (!(in_cell(p.x, Lx) && in_cell(p.y, Ly) && in_cell(p.z, Lz)))
-    raise_err(err, PFM_E_OUT_OF_CELL, orig[i] + 1, 0, step + 1);
+    raise_out_of_cell(err, orig[i] + 1, step + 1, p);
 }
 
 // ------------------------------------------------------------------ host side

@@ -310,8 +310,7 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
   type(group_def), allocatable :: groups(:)
 
   call system_clock_seconds(wall0)
-  fid = 2017
-  open(fid, file=trim(input_path)//trim(settings_filename), status='old')
+  open(newunit=fid, file=trim(input_path)//trim(settings_filename), status='old')
   write(out_id,'(A,A)') 'settings_filename: ', trim(settings_filename)
   read(fid,*) str, md_step_limit;  write(out_id,'(A32,i12)') str, md_step_limit
   read(fid,*) str, logfilename;    write(out_id,'(A32,A,A)') str, '	', trim(logfilename)
@@ -417,10 +416,10 @@ subroutine md(out_id, all_out_id, input_path, settings_filename, output_prefix, 
   allocate(e_inter(max(ninter,1)), e_nhc(max(nnhc,1)))
   e_inter = 0.d0
   e_nhc = 0.d0
-  write(fmt_int,'("(",i0,"f20.9)")') ninter
-  write(fmt_nhc,'("(",i0,"f20.9)")') nnhc
-  log_id = 108
-  open(log_id, file=trim(output_prefix)//trim(logfilename))
+  ! energy columns of the log and summary lines: any count, f20.9 each
+  fmt_int = '(*(f20.9))'
+  fmt_nhc = '(*(f20.9))'
+  open(newunit=log_id, file=trim(output_prefix)//trim(logfilename))
   write(out_id,'(A24,i6,A)') 'RUNNING ON ', num_of_omp_treads, ' OPENMP THREADS'
   write(out_id,*)
 

@@ -103,17 +103,29 @@ def test_nvms_blocks_with_records():
     b[0].close()
 
 
-def test_nvms_blocks_time():
+def test_nvms_blocks_medium_slab():
+    """A 500-step nvms run of a medium graphene/Cu slab that never meets the
+    exit test: the blocks reach the same last step, event, state and energies
+    as testing every step, bit for bit (the timing is printed, not judged)."""
     s = configs.graphene_on_cu(cx=30, cy=17, gx=31, gy=18, layers=4, seed=84, temperature=50.0)
+    res = []
     for spec in (False, True):
         c = gpu_ctx(s, capi.MIXED)
         c.set_speculation(spec)
         c.run("nvms", 0.5, 0, 20, True, 1e-30)
         t0 = time.perf_counter()
-        c.run("nvms", 0.5, 21, 520, True, 1e-30)
+        out = c.run("nvms", 0.5, 21, 520, True, 1e-30)
         dt = (time.perf_counter() - t0) / 500
         print(f"  n={s.n} nvms blocks={spec}: {dt * 1e3:.4f} ms/step", flush=True)
+        res.append((out, c.state(), c.energies()))
         c.close()
+    (oa, sa, ea), (ob, sb, eb) = res
+    assert oa == ob == (520, capi.EV_DONE)
+    for x, y in zip(sa, sb):
+        assert np.array_equal(x, y)
+    for k in ("pe", "ke", "temp", "time"):
+        assert ea[k] == eb[k], k
+    assert np.array_equal(ea["e_inter"], eb["e_inter"])
 
 
 def batch_replicas():

@@ -1,0 +1,281 @@
+"""Pins of the CPU checker's per-atom guards and thermostat sequence, from
+what the mathematics and the reference's stated rules fix:
+
+  - check_positions (md_general.f90:342-364): the cell is accepted up to
+    1e-7 A beyond each face and refused past it; a particle whose drift
+    overshoots the box by more than one period (md()'s drift wraps once,
+    md_integrators.f90:20-24) is refused at the top of the next step, at a
+    step and position the free flight fixes;
+  - invert_z_velocities with md()'s borders 0.8 Lz and 0.9 Lz
+    (md_general.f90:382-398, md_simulation.f90:140): a reflecting wall, so
+    no particle ever crosses the middle plane of the band, |v_z| is kept,
+    and without the wall the same particles do cross;
+  - zero_momentum (md_general.f90:236-253): sum m v = 0 afterwards over the
+    all_atoms group, every member shifted by the same vector, the other
+    atoms untouched;
+  - integrate_nose_hoover_chain (md_integrators.f90:200-245): at fixed
+    kinetic energy (free particles) one step equals the Martyna-Tuckerman-
+    Klein half-step operator written as the textbook product of Liouville
+    propagators (Mol. Phys. 87, 1117 (1996), section 4: scale by
+    exp(-v_{j+1} dt/8), kick by G_j dt/4, scale again), applied twice;
+  - integrate_verlet_z_* (md_integrators.f90:33-56,79-97): x, y, v_x, v_y
+    never change, and E = PE + KE is conserved to O(dt^2) (the x,y motion
+    is frozen, so the z dynamics is Hamiltonian on its own).
+A mistake planted in the checker (tests/test_oracle_mutants.py) fails these.
+"""
+import re
+
+import numpy as np
+import pytest
+
+from pfmds_amd import capi, configs
+from pfmds_amd.system import KB, MASS_COEF, NHC, Integrator, Interaction, System
+from tests.harness import oracle_ctx
+
+
+def free_columns(n=48, box=(24.0, 24.0, 40.0), seed=7, vmax=0.02, zmoving=True):
+    """Particles on distinct (x, y) columns 3 A apart that move along z only
+    (z_moving) and never meet: lj1g with a 0.3 A cutoff sees no pair, so
+    every force is zero and each particle flies freely."""
+    rng = configs.SplitMix64(seed)
+    box = np.asarray(box, float)
+    m = int(np.ceil(np.sqrt(n)))
+    ij = np.stack(np.meshgrid(np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 2)[:n]
+    xy = ij * 3.0 + 0.5
+    z = rng.uniform(n) * box[2]
+    pos = np.column_stack([xy, z])
+    vel = (2 * rng.uniform(3 * n).reshape(n, 3) - 1) * vmax
+    mass = 10.0 + 90.0 * rng.uniform(n)
+    # no net momentum, so the zero_momentum of step 0 (0 mod any period) moves nothing
+    vel -= (mass[:, None] * vel).sum(0) / mass.sum()
+    return System(box=box, pos=pos, vel=vel, mass=mass, species=["X"] * n, groups=[["X"], ["#"]],
+                  interactions=[Interaction("lj1g", [0.01, 0.2, 0.2, 0.3], [(1, 1, 4, 0.4, 1)])],
+                  integrators=[Integrator("nve", 1.0, 10, 10**9, 10**9)],
+                  all_moving=1, xyz_moving=2 if zmoving else 1, z_moving=1 if zmoving else 2, all_atoms=1)
+
+
+def expect_error(fn, *args):
+    """The PfmError fn raises; AssertionError when it raises none (so a
+    planted mistake that removes the error fails the pin as an assertion)."""
+    try:
+        fn(*args)
+    except capi.PfmError as e:
+        return e
+    raise AssertionError("no error raised")
+
+
+def err_parts(msg):
+    """(atom, x, y, z, step) of an out-of-cell message."""
+    m = re.search(r"(\d+) particle out of cell (\S+) (\S+) (\S+) \(step (\d+)\)", msg)
+    assert m, msg
+    return int(m.group(1)), float(m.group(2)), float(m.group(3)), float(m.group(4)), int(m.group(5))
+
+
+# ---------------------------------------------------------- check_positions
+
+@pytest.mark.parametrize("axis,value,bad", [
+    (0, "L+0.9e-7", False), (0, "L+1.1e-7", True), (1, "-0.9e-7", False), (1, "-1.1e-7", True),
+    (2, "L", False), (2, "0", False)])
+def test_check_positions_tolerance(axis, value, bad):
+    s = free_columns(n=9)
+    L = s.box[axis]
+    x = {"L+0.9e-7": L + 0.9e-7, "L+1.1e-7": L + 1.1e-7, "-0.9e-7": -0.9e-7, "-1.1e-7": -1.1e-7,
+         "L": L, "0": 0.0}[value]
+    s.pos[4, axis] = x
+    o = oracle_ctx(s)
+    if not bad:
+        o.run("nve", 1.0, 0, 0, True)
+        return
+    e = expect_error(o.run, "nve", 1.0, 0, 0, True)
+    assert e.code == capi.E_OUT_OF_CELL
+    atom, *p, step = err_parts(e.msg)
+    assert (atom, step) == (5, 0)
+    assert p[axis] == pytest.approx(x, abs=1e-12)
+
+
+def test_check_positions_after_overshooting_drift():
+    """Single-wrap drift: z0 = 0.5 Lz and v_z dt = 1.7 Lz give 2.2 Lz - Lz =
+    1.2 Lz after step 1, refused at the top of step 2.  (v_z is set so that
+    it is 1.7 Lz / dt after step 0's zero_momentum.)"""
+    s = free_columns(n=9)
+    Lz, dt = s.box[2], 0.5
+    s.pos[3, 2] = 0.5 * Lz
+    s.vel[3, 2] += 1.7 * Lz / dt / (1.0 - s.mass[3] / s.mass.sum())
+    o = oracle_ctx(s)
+    o.run("nve", dt, 0, 0, True)
+    _, v0, _ = o.state()
+    z1 = 0.5 * Lz + v0[3, 2] * dt
+    assert 2.1 * Lz < z1 < 2.3 * Lz
+    o.run("nve", dt, 1, 1, True)
+    p, _, _ = o.state()
+    assert p[3, 2] == z1 - Lz  # wrapped once, so still out of the cell
+    e = expect_error(o.run, "nve", dt, 2, 5, True)
+    atom, x, y, z, step = err_parts(e.msg)
+    assert e.code == capi.E_OUT_OF_CELL and (atom, step) == (4, 2)
+    assert z == pytest.approx(z1 - Lz, rel=1e-9)
+
+
+# ------------------------------------------------------------ invert_z wall
+
+def unwrap_track(o, s, steps, dt):
+    """z of every particle after each step, unwrapped by the minimum image of
+    each step's displacement, and the velocities."""
+    p0, v0, _ = o.state()
+    zs, vs = [p0[:, 2].copy()], [v0.copy()]
+    zu = p0[:, 2].copy()
+    prev = p0[:, 2].copy()
+    for k in range(1, steps + 1):
+        o.run("nve", dt, k, k, True)
+        p, v, _ = o.state()
+        d = p[:, 2] - prev
+        d -= s.box[2] * np.round(d / s.box[2])
+        zu = zu + d
+        prev = p[:, 2].copy()
+        zs.append(zu.copy())
+        vs.append(v.copy())
+    return np.array(zs), np.array(vs)
+
+
+def test_invert_z_is_a_reflecting_wall():
+    s = free_columns(n=64, vmax=0.6, seed=9)
+    dt = 0.5
+    Lz = s.box[2]
+    # every |v_z| dt stays below the half-width of the band (0.05 Lz)
+    assert np.abs(s.vel[:, 2]).max() * dt < 0.05 * Lz
+    mid = 0.85 * Lz
+    crossings = {}
+    for wall in (True, False):
+        s.invert_z = wall
+        o = oracle_ctx(s)
+        o.run("nve", dt, 0, 0, True)
+        zs, vs = unwrap_track(o, s, 300, dt)
+        side = np.floor((zs - mid) / Lz)  # which copy of the cut circle
+        crossings[wall] = int((np.diff(side, axis=0) != 0).sum())
+        # x, y velocities and |v_z| are kept; v_z only ever changes sign
+        for k in (0, 1):
+            assert np.array_equal(vs[:, :, k], np.broadcast_to(vs[0, :, k], vs[:, :, k].shape))
+        assert np.array_equal(np.abs(vs[:, :, 2]), np.broadcast_to(np.abs(vs[0, :, 2]), vs[:, :, 2].shape))
+        flips = int((np.diff(np.sign(vs[:, :, 2]), axis=0) != 0).sum())
+        if wall:
+            assert flips > 20
+        else:
+            assert flips == 0
+    assert crossings[True] == 0
+    assert crossings[False] > 20
+
+
+# ------------------------------------------------------------ zero_momentum
+
+def test_zero_momentum():
+    s = free_columns(n=40, seed=13, zmoving=False)
+    s.species = ["X" if i % 5 else "Y" for i in range(s.n)]
+    s.groups = [["X"], ["#"], ["X", "Y"]]
+    s.all_moving, s.all_atoms = 3, 1
+    s.xyz_moving, s.z_moving = 3, 2
+    s.vel[:, :] += np.array([0.01, -0.02, 0.005])  # a net drift to remove
+    s.zero_momentum_period = 1
+    o = oracle_ctx(s)
+    o.run("nve", 1.0, 0, 0, True)
+    _, v, _ = o.state()
+    x = np.array([sp == "X" for sp in s.species])
+    m = s.mass[x][:, None]
+    p = (m * v[x]).sum(0)
+    assert np.abs(p).max() < 1e-14 * (m * np.abs(v[x])).sum()
+    shift = v[x] - s.vel[x]
+    assert np.abs(shift - shift[0]).max() < 1e-16
+    assert np.array_equal(v[~x], s.vel[~x])
+
+
+# --------------------------------------------------- Nose-Hoover chain steps
+
+def mtk_half_step(ke, xs, vs, qs, nf, kt, dt):
+    """One half step of the Martyna-Tuckerman-Klein chain written as the
+    product of its propagators; returns the particle scale factor."""
+    M = len(vs)
+
+    def force(j, ke_now):
+        if j == 0:
+            return (2.0 * ke_now - nf * kt) / qs[0]
+        return (qs[j - 1] * vs[j - 1] ** 2 - kt) / qs[j]
+
+    def update(j, ke_now):
+        if j == M - 1:
+            vs[j] += force(j, ke_now) * dt / 4
+            return
+        a = np.exp(-vs[j + 1] * dt / 8)
+        vs[j] *= a
+        vs[j] += force(j, ke_now) * dt / 4
+        vs[j] *= a
+
+    for j in range(M - 1, -1, -1):
+        update(j, ke)
+    scale = np.exp(-vs[0] * dt / 2)
+    ke = ke * scale * scale
+    for j in range(M):
+        xs[j] += vs[j] * dt / 2
+    for j in range(M):
+        update(j, ke)
+    return scale
+
+
+@pytest.mark.parametrize("M", [1, 2, 3])
+def test_nhc_step_sequence(M):
+    s = free_columns(n=36, seed=17, vmax=0.01)
+    T, q1 = 500.0, 3 * s.n * KB * 500.0 * 15.0 ** 2
+    s.nhcs = [NHC(1, T, M, q1)]
+    dt = 1.0
+    o = oracle_ctx(s)
+    o.run("nvt", dt, 0, 0, True)
+    v = s.vel.copy()
+    xs, vs = np.zeros(M), np.zeros(M)
+    qs = np.array([q1] + [q1 / (3.0 * s.n)] * (M - 1))
+    for k in range(1, 6):
+        for _ in range(2):  # the two halves of step k; free particles keep their KE between them
+            ke = 0.5 * MASS_COEF * (s.mass[:, None] * v * v).sum()
+            v = v * mtk_half_step(ke, xs, vs, qs, 3.0 * s.n, KB * T, dt)
+        o.run("nvt", dt, k, k, True)
+        x_o, v_o = o.get_chain(1, M)
+        np.testing.assert_allclose(v_o, vs, rtol=1e-12, atol=1e-18)
+        np.testing.assert_allclose(x_o, xs, rtol=1e-12, atol=1e-18)
+        _, vel, _ = o.state()
+        np.testing.assert_allclose(vel, v, rtol=1e-12, atol=1e-18)
+    assert np.abs(vs[0]) > 1e-6  # the chain moved
+
+
+# ----------------------------------------------------- z-only integrators
+
+def z_cluster(n=27, seed=19):
+    """A small argon-like LJ crystal, every atom moving along z only, with
+    nonzero v_x, v_y (which z-only integration must keep)."""
+    rng = configs.SplitMix64(seed)
+    a = 3.8
+    g = np.stack(np.meshgrid(np.arange(3), np.arange(3), np.arange(3), indexing="ij"), -1).reshape(-1, 3)
+    pos = (g + 0.5) * a + 0.15 * (2 * rng.uniform(3 * n).reshape(n, 3) - 1)
+    box = np.array([3 * a, 3 * a, 3 * a])
+    vel = (2 * rng.uniform(3 * n).reshape(n, 3) - 1) * 0.004
+    return System(box=box, pos=pos, vel=vel, mass=np.full(n, 39.95), species=["A"] * n, groups=[["A"], ["#"]],
+                  interactions=[Interaction("lj1g", [0.0104, 3.40, 4.5, 5.5], [(1, 1, 64, 5.6, 1)])],
+                  integrators=[Integrator("nve", 1.0, 10, 10**9, 10**9)],
+                  all_moving=1, xyz_moving=2, z_moving=1, all_atoms=1)
+
+
+def test_z_only_integration():
+    errs = []
+    for dt in (2.0, 1.0):
+        s = z_cluster()
+        o = oracle_ctx(s)
+        o.run("nve", dt, 0, 0, True)
+        e0 = o.energies()
+        _, v0, _ = o.state()  # after step 0's zero_momentum
+        n = int(400 / dt)
+        mx = 0.0
+        for k in range(8):
+            o.run("nve", dt, 1 + k * n // 8, (k + 1) * n // 8, True)
+            e = o.energies()
+            mx = max(mx, abs(e["pe"] + e["ke"] - e0["pe"] - e0["ke"]))
+        p, v, f = o.state()
+        assert np.array_equal(p[:, :2], s.pos[:, :2])
+        assert np.array_equal(v[:, :2], v0[:, :2])
+        assert np.abs(v[:, 2] - v0[:, 2]).max() > 1e-4  # z did move under the forces
+        errs.append(mx)
+    assert 3.0 < errs[0] / errs[1] < 5.0

@@ -13,6 +13,7 @@ import pytest
 from pfmds_amd import capi
 from tests import harness
 from tests import test_oracle_growth as growth
+from tests import test_oracle_guards as guards
 from tests import test_oracle_pins as pins
 
 ORACLE_DIR = os.path.join(harness.ROOT, "oracle")
@@ -113,6 +114,38 @@ MUTANTS = {
     "group_change_period_from_zero": (
         "(step - c->ts1) % c->frec == 0", "step % c->frec == 0",
         growth.test_growth_counts_follow_the_rule, ()),
+    "check_positions_tolerance_too_wide": (
+        "const double tol = 0.0000001;", "const double tol = 0.000001;",
+        guards.test_check_positions_tolerance, (0, "L+1.1e-7", True)),
+    "drift_wraps_until_inside": (
+        "if (x > h->box[k]) x = x - h->box[k];", "if (x > h->box[k]) x = fmod(x, h->box[k]);",
+        guards.test_check_positions_after_overshooting_drift, ()),
+    "invert_z_lower_half_direction": (
+        "z < mid && vz > 0.0", "z < mid && vz < 0.0",
+        guards.test_invert_z_is_a_reflecting_wall, ()),
+    "invert_z_upper_border": (
+        "hi = 0.9 * h->box[2]", "hi = 0.95 * h->box[2]",
+        guards.test_invert_z_is_a_reflecting_wall, ()),
+    "zero_momentum_unweighted": (
+        "mcv[k] += h->mass[i] * h->vel[3 * i + k];", "mcv[k] += h->vel[3 * i + k];",
+        guards.test_zero_momentum, ()),
+    "zero_momentum_every_atom": (
+        "for (int k = 0; k < 3; ++k) h->vel[3 * g->idx[ind] + k] -= mcv[k];",
+        "for (int k = 0; k < 3; ++k) for (int a = 0; a < h->N; ++a) if (a == g->idx[ind] || a % 5 == 0) "
+        "h->vel[3 * a + k] -= (a == g->idx[ind]) ? mcv[k] : 0.0 * mcv[k] + 1e-3;",
+        guards.test_zero_momentum, ()),
+    "nhc_first_kick_eighth_step": (
+        "v[0] = v[0] + kedif / q[0] * h->ts[2];", "v[0] = v[0] + kedif / q[0] * h->ts[3];",
+        guards.test_nhc_step_sequence, (1,), (0, 2)),
+    "nhc_second_half_old_kinetic_energy": (
+        "kedif = 2.0 * ke * (c->s * c->s) - 3.0 * c->L * kt;", "kedif = 2.0 * ke - 3.0 * c->L * kt;",
+        guards.test_nhc_step_sequence, (2,)),
+    "nhc_chain_force_unscaled": (
+        "/ q[i] * h->ts[2] * b;", "/ q[i] * h->ts[2];",
+        guards.test_nhc_step_sequence, (3,), (0, 2)),
+    "z_kick_moves_every_component": (
+        "for (int k = only_z ? 2 : 0; k < 3; ++k)\n", "for (int k = 0; k < 3; ++k)\n",
+        guards.test_z_only_integration, ()),
     "group_change_rows_over_all_members": (
         "for (int ind = 0; ind < g1->n; ++ind) {", "for (int ind = 0; ind < g1->size; ++ind) {",
         growth.test_growth_moves_and_lists_only_joined_atoms, ()),
