@@ -35,8 +35,8 @@ from tests.harness import oracle_ctx
 
 def free_columns(n=48, box=(24.0, 24.0, 40.0), seed=7, vmax=0.02, zmoving=True):
     """Particles on distinct (x, y) columns 3 A apart that move along z only
-    (z_moving) and never meet: lj1g with a 0.3 A cutoff sees no pair, so
-    every force is zero and each particle flies freely."""
+    (z_moving) and never meet: lj1g with a 1.5 A cutoff (list 2 A) sees no
+    pair, so every force is zero and each particle flies freely."""
     rng = configs.SplitMix64(seed)
     box = np.asarray(box, float)
     m = int(np.ceil(np.sqrt(n)))
@@ -49,7 +49,7 @@ def free_columns(n=48, box=(24.0, 24.0, 40.0), seed=7, vmax=0.02, zmoving=True):
     # no net momentum, so the zero_momentum of step 0 (0 mod any period) moves nothing
     vel -= (mass[:, None] * vel).sum(0) / mass.sum()
     return System(box=box, pos=pos, vel=vel, mass=mass, species=["X"] * n, groups=[["X"], ["#"]],
-                  interactions=[Interaction("lj1g", [0.01, 0.2, 0.2, 0.3], [(1, 1, 4, 0.4, 1)])],
+                  interactions=[Interaction("lj1g", [0.01, 0.8, 1.0, 1.5], [(1, 1, 8, 2.0, 1)])],
                   integrators=[Integrator("nve", 1.0, 10, 10**9, 10**9)],
                   all_moving=1, xyz_moving=2 if zmoving else 1, z_moving=1 if zmoving else 2, all_atoms=1)
 
