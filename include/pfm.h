@@ -156,6 +156,17 @@ int pfm_set_state(pfm_ctx* h, const double* pos, const double* vel);
 int pfm_copy_state(pfm_ctx* dst, pfm_ctx* src);
 int pfm_force_rmse(pfm_ctx* a, pfm_ctx* b, int group, double* rmse);
 
+/* pfm_force_rmse without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as pfm_force_rmse (bit-identical), but its staging
+ * and partial sums live in a fixed device array of the library (32 MiB per
+ * device, held until the process ends) instead of memory the library
+ * allocates, and it calls no hipMalloc/hipFree.  At most 1,048,576 atoms
+ * (PFM_E_INVALID beyond).  All handles on one device share that array:
+ * calls that overlap on one device (threads) race. */
+int pfm_force_rmse_noalloc(pfm_ctx* a, pfm_ctx* b, int group, double* rmse);
+
 /* Positions x[M] and velocities v[M] of Nose-Hoover chain c (1-based, in
  * pfm_add_nhc order): the `nose_hoover_chain` state (md_general.f90:24-28)
  * that md() carries from step to step.  pfm_set_chain takes them over (after

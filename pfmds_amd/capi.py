@@ -105,6 +105,8 @@ def load_library(path, prefix):
         lib.pfm_copy_state.restype = C.c_int
         lib.pfm_force_rmse.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _d]
         lib.pfm_force_rmse.restype = C.c_int
+        lib.pfm_force_rmse_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _d]
+        lib.pfm_force_rmse_noalloc.restype = C.c_int
         lib.pfm_set_chain.argtypes = [C.c_void_p, C.c_int, _d, _d]
         lib.pfm_set_chain.restype = C.c_int
         lib.pfm_set_speculation.argtypes = [C.c_void_p, C.c_int]
@@ -352,6 +354,13 @@ class Context:
         force components of this handle's group `group`."""
         r = C.c_double(0.0)
         self._check(self.lib.pfm_force_rmse(other.h, self.h, int(group), C.byref(r)))
+        return r.value
+
+    def force_rmse_noalloc(self, other, group):
+        """force_rmse through pfm_force_rmse_noalloc (no device allocation;
+        at most 1,048,576 atoms)."""
+        r = C.c_double()
+        self._check(self.lib.pfm_force_rmse_noalloc(other.h, self.h, int(group), C.byref(r)))
         return r.value
 
     def get_chain(self, c, M):

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 #include "pfm_internal.h"
@@ -2284,6 +2285,79 @@ int pfm_force_rmse(pfm_ctx* a, pfm_ctx* b, int group, double* rmse) {
   HIPCHK(b, hipMemcpyAsync(b->h_red + 60, b->red_out + 60, sizeof(double), hipMemcpyDeviceToHost, b->stream));
   HIPCHK(b, hipStreamSynchronize(b->stream));
   *rmse = std::sqrt(b->h_red[60] / (3.0 * b->grp.n[group]));
+  return PFM_OK;
+}
+
+// ------------------------------------------- force RMSE without allocation
+//
+// pfm_force_rmse_noalloc runs the comparison of pfm_force_rmse with its
+// staging (the forces of handle a in original order) and its partial sums in the
+// file-scope device array `scratch` instead of memory the library allocates,
+// and calls no allocator, for callers that cannot allocate (code under HIP
+// graph capture, serving loops).  Every handle on a device shares that
+// device's copy of the array, so calls that overlap on one device race.
+#define PFM_NOALLOC_MAX_ATOMS (1 << 20)  // 32 MiB of staging
+#define PFM_NOALLOC_PARTS 1024           // partial sums (red_blocks never exceeds it)
+static __device__ __align__(256) double4 scratch[PFM_NOALLOC_MAX_ATOMS + PFM_NOALLOC_PARTS / 4 + 1];
+
+#define PFM_MAX_DEVICES 64
+static std::mutex scratch_mu;
+static void* scratch_dev[PFM_MAX_DEVICES];  // address of each device's copy, set on first use
+
+// the device address of `scratch` on h's device, looked up once per device
+// with that device current
+static int scratch_on(pfm_ctx* h, double4** out) {
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (h->dev < 0 || h->dev >= PFM_MAX_DEVICES) return fail(h, PFM_E_INVALID, "error: device index out of range");
+  if (!scratch_dev[h->dev]) {
+    int cur = 0;
+    HIPCHK(h, hipGetDevice(&cur));
+    if (cur != h->dev) HIPCHK(h, hipSetDevice(h->dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != h->dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(h, PFM_E_HIP, std::string("error: HIP ") + hipGetErrorString(e) + " at hipGetSymbolAddress(scratch)");
+    scratch_dev[h->dev] = p;
+  }
+  *out = (double4*)scratch_dev[h->dev];
+  return PFM_OK;
+}
+
+// forces into original order: out[orig[i]] = f[i]
+__global__ void k_frc_to_orig(int N, const int* __restrict__ orig, const double4* __restrict__ f,
+                              double4* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) out[orig[i]] = f[i];
+}
+
+int pfm_force_rmse_noalloc(pfm_ctx* a, pfm_ctx* b, int group, double* rmse) {
+  if (a->N != b->N || a->dev != b->dev)
+    return fail(b, PFM_E_INVALID, "error: force comparisons need handles of one size on one device");
+  if (group < 1 || group > PFM_MAXG || b->grp.n[group] == 0)
+    return fail(b, PFM_E_INVALID, "error: force comparison over an empty or unknown group");
+  if (a->N > PFM_NOALLOC_MAX_ATOMS || b->red_blocks > PFM_NOALLOC_PARTS)
+    return fail(b, PFM_E_INVALID,
+                "error: pfm_force_rmse_noalloc takes at most 1048576 atoms (its fixed device scratch); use "
+                "pfm_force_rmse");
+  int rc;
+  double4* st = nullptr;
+  if ((rc = scratch_on(b, &st))) return rc;
+  double* part = (double*)(st + PFM_NOALLOC_MAX_ATOMS);
+  double* out = part + PFM_NOALLOC_PARTS;
+  const int N = a->N, nb = (N + 255) / 256;
+  // a's forces into original order on a's stream (every slot < N is written:
+  // orig is a permutation), then the comparison on b's stream
+  hipLaunchKernelGGL(k_frc_to_orig, dim3(nb), dim3(256), 0, a->stream, N, a->orig, a->frc4, st);
+  HIPCHK(a, hipGetLastError());
+  HIPCHK(a, hipStreamSynchronize(a->stream));
+  hipLaunchKernelGGL(k_force_dev, dim3(b->red_blocks), dim3(RED_THREADS), 0, b->stream, N, b->orig,
+                     (const double4*)st, b->frc4, b->mask, 1u << (group - 1), part);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(RED_THREADS), 0, b->stream, 1, b->red_blocks, part, out);
+  HIPCHK(b, hipGetLastError());
+  HIPCHK(b, hipMemcpyAsync(b->h_red + 61, out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(b, hipStreamSynchronize(b->stream));
+  *rmse = std::sqrt(b->h_red[61] / (3.0 * b->grp.n[group]));
   return PFM_OK;
 }
 
