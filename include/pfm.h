@@ -198,6 +198,17 @@ int pfm_kernel_time(pfm_ctx* h, const char* name, int* count, double* mean_ms);
  * the reference (launch-shape control for tuning and benchmarks). */
 int pfm_set_tuning(pfm_ctx* h, int lanes_per_row, int block);
 
+/* Co-scheduled passes (default on).  For graphene on a metal -- exactly
+ * rjl, tb and a ljc/morsec fed by that tb list, with the rjl rows, the
+ * graphene rows and the metal rows of ljc/morsec sharing no atom -- passes
+ * over disjoint atoms share launches: the rjl density pass with the tb
+ * bond-order pass, the rjl force pass with the graphene pass of ljc/morsec,
+ * the tb force pass (with the normal-derivative terms) with the metal rows.
+ * Results equal the separate launches' up to the order of the FP64 force
+ * sums.  pfm_set_fusion(h, 0) launches every pass on its own.  No
+ * counterpart in the reference (launch schedule). */
+int pfm_set_fusion(pfm_ctx* h, int on);
+
 /* Captured steps (default on).  pfm_run records a block of steps whose list
  * and zero-momentum schedule repeats (the least common multiple of the list
  * periods, up to 64 steps) and that holds no energy step as one HIP graph,

@@ -6,7 +6,9 @@ metal rows) walk each atom row with T lanes inside workgroups of B threads
 and the system size.  `tune` times a short run of the live handle for every
 candidate shape, keeps the fastest, then times the steps launched from
 captured graphs against launched one by one (`pfm_set_graphs`: graphs pay
-off where launches dominate, small systems) and leaves the faster set;
+off where launches dominate, small systems) and the co-scheduled passes of
+graphene on a metal against separate launches (`pfm_set_fusion`), and leaves
+the faster set;
 results are cached in a JSON file keyed by a signature of the workload.  No counterpart in the
 reference, which tunes only its OpenMP thread count
 (perfomance_settings.f90:13-25).
@@ -69,10 +71,17 @@ def tune(ctx, integrator, dt, precision, first_step=1, steps=40, repeats=2, cand
     pick = min(table, key=table.get)
     lanes, block = (int(x) for x in pick.split("x"))
     ctx.set_tuning(lanes, block)
+    # the co-scheduled passes of graphene on a metal (on by default) against
+    # separate launches, at the picked shape
+    ctx.set_fusion(False)
+    table[f"{pick}-fusion"] = best_of()
+    fused = table[pick] <= table[f"{pick}-fusion"]
+    ctx.set_fusion(fused)
     ctx.set_graphs(True)
     table[f"{pick}+graphs"] = best_of()
     graphs = table[f"{pick}+graphs"] < table[pick]
     ctx.set_graphs(graphs)
+    ctx.fused = fused
     return lanes, block, graphs, table, step
 
 
@@ -84,9 +93,9 @@ def load(path=CACHE):
         return {}
 
 
-def save(key, lanes, block, graphs, table, path=CACHE):
+def save(key, lanes, block, graphs, table, path=CACHE, fusion=True):
     data = load(path)
-    data[key] = {"lanes": lanes, "block": block, "graphs": bool(graphs), "seconds": table}
+    data[key] = {"lanes": lanes, "block": block, "graphs": bool(graphs), "fusion": bool(fusion), "seconds": table}
     os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
     tmp = path + ".tmp"
     with open(tmp, "w") as f:
@@ -101,6 +110,7 @@ def apply_cached(ctx, key, path=CACHE):
         return False
     ctx.set_tuning(int(e["lanes"]), int(e["block"]))
     ctx.set_graphs(bool(e.get("graphs", False)))
+    ctx.set_fusion(bool(e.get("fusion", True)))
     return True
 
 
@@ -113,5 +123,5 @@ def autotune(ctx, s, integrator, dt, precision, first_step=1, n_rep=1, path=CACH
         e = load(path)[key]
         return int(e["lanes"]), int(e["block"]), bool(e.get("graphs", False)), "cache", first_step
     lanes, block, graphs, table, step = tune(ctx, integrator, dt, precision, first_step, steps)
-    save(key, lanes, block, graphs, table, path)
+    save(key, lanes, block, graphs, table, path, fusion=getattr(ctx, "fused", True))
     return lanes, block, graphs, "tuned", step

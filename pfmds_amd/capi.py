@@ -93,6 +93,8 @@ def load_library(path, prefix):
         lib.pfm_set_tuning.restype = C.c_int
         lib.pfm_set_graphs.argtypes = [C.c_void_p, C.c_int]
         lib.pfm_set_graphs.restype = C.c_int
+        lib.pfm_set_fusion.argtypes = [C.c_void_p, C.c_int]
+        lib.pfm_set_fusion.restype = C.c_int
         lib.pfm_graph_count.argtypes = [C.c_void_p, _P(C.c_long), _P(C.c_long)]
         lib.pfm_graph_count.restype = C.c_int
         lib.pfm_sur_set_params.argtypes = [C.c_void_p, C.c_int, _d, C.c_int]
@@ -301,6 +303,10 @@ class Context:
 
     def set_tuning(self, lanes_per_row=0, block=0):
         self._check(self.lib.pfm_set_tuning(self.h, int(lanes_per_row), int(block)))
+
+    def set_fusion(self, on):
+        """Co-scheduled passes of graphene on a metal (pfm_set_fusion)."""
+        self._check(self.lib.pfm_set_fusion(self.h, int(bool(on))))
 
     def set_graphs(self, on):
         self._check(self.lib.pfm_set_graphs(self.h, int(bool(on))))

@@ -840,6 +840,28 @@ static int finalize(pfm_ctx* h) {
         if (in1[x]) { it.rows_disjoint = false; break; }
     }
   }
+  // graphene on a metal: exactly rjl, tb and a ljc/morsec fed by that tb
+  // list, with the rjl rows, the graphene rows and the metal rows of
+  // ljc/morsec sharing no atom (the passes of each side may then share
+  // launches, every force being summed by one row)
+  h->gm = pfm_ctx::GmPlan{};
+  if (h->inter.size() == 3) {
+    int ir = -1, itb = -1, ic = -1;
+    for (int k = 0; k < 3; ++k) {
+      const int kind = h->inter[k].kind;
+      if (kind == PFM_RJL) ir = k;
+      else if (kind == PFM_TB) itb = k;
+      else if (is_cos(kind)) ic = k;
+    }
+    if (ir >= 0 && itb >= 0 && ic >= 0 && h->inter[ic].nn_from_tb == itb && h->inter[ic].rows_disjoint) {
+      std::vector<char> inC(h->N, 0);
+      for (int x : h->grp.idx[h->inter[itb].nl[0].g1]) inC[x] = 1;
+      bool ok = true;
+      for (int x : h->grp.idx[h->inter[ir].nl[0].g1]) ok = ok && !inC[x];
+      for (int x : h->grp.idx[h->inter[ic].nl[1].g1]) ok = ok && !inC[x];
+      if (ok) h->gm = pfm_ctx::GmPlan{true, ir, itb, ic};
+    }
+  }
   // group-local index by original atom, for lj1g and rebosc
   for (auto& it : h->inter) {
     if (it.kind != PFM_LJ1G && it.kind != PFM_REBOSC) continue;
@@ -1252,7 +1274,7 @@ static std::vector<uint64_t> graph_key(pfm_ctx* h, int integ, double dt, int blk
     xm |= (uint64_t)rec_x_at(h, t) << (t - s);
   }
   k.insert(k.end(), {(uint64_t)integ, dtb, (uint64_t)blk, (uint64_t)first_writes, (uint64_t)zm,
-                     (uint64_t)h->invert_z, (uint64_t)h->lanes, (uint64_t)h->block, em, xm});
+                     (uint64_t)h->invert_z, (uint64_t)h->lanes, (uint64_t)h->block, (uint64_t)h->fuse, em, xm});
   return k;
 }
 
@@ -2488,6 +2510,11 @@ int pfm_set_tuning(pfm_ctx* h, int lanes, int block) {
       return fail(h, PFM_E_INVALID, "error: block must be 64, 128, 256, 512 or 1024");
     h->block = block;
   }
+  return PFM_OK;
+}
+
+int pfm_set_fusion(pfm_ctx* h, int on) {
+  h->fuse = on != 0;
   return PFM_OK;
 }
 

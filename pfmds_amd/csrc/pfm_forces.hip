@@ -176,8 +176,8 @@ template <class R> struct PairBlock { static constexpr int max = sizeof(R) == 4 
 // density pass: rho_i = sum_{r <= R2} exp(-2q(r/r0-1)) fc(r); stores
 // Eb_i = sqrt(rho_i) and 1/Eb_i in the w slot of atom i's position record
 template <int T, class R>
-__global__ void __launch_bounds__(PAIR_BLOCK) k_rjl_density(RowArgs a, RjlP P, double* __restrict__ eb, int nb_real) {
-  ROW_PROLOGUE(T)
+__device__ __forceinline__ void rjl_density_rows(const RowArgs& a, const RjlP& P, double* __restrict__ eb, int blk) {
+  ROW_PROLOGUE_AT(T, blk)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
   const R kq = PM<R>::kexp(-2.0 * P.q);
   double rho = 0.0;
@@ -197,13 +197,18 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_rjl_density(RowArgs a, RjlP P, d
   }
 }
 
+template <int T, class R>
+__global__ void __launch_bounds__(PAIR_BLOCK) k_rjl_density(RowArgs a, RjlP P, double* __restrict__ eb, int nb_real) {
+  rjl_density_rows<T, R>(a, P, eb, xcd_block(nb_real));
+}
+
 // embedding-force pass (RosatoGuillopeLegrand.f90:80-93):
 // F_i -= sum_j [2 A0 (p/r0 fc - df r) e_p - xi (q/r0 fc - df r/2)(1/Eb_i + 1/Eb_j) e_q]/r dr
 // with 1/Eb read from the w slot of the gathered position records
 template <int T, class R, bool E>
-__global__ void __launch_bounds__(PAIR_BLOCK) k_rjl_force(RowArgs a, RjlP P, const double* __restrict__ eb,
-                                                          int nb_real) {
-  ROW_PROLOGUE(T)
+__device__ __forceinline__ void rjl_force_rows(const RowArgs& a, const RjlP& P, const double* __restrict__ eb,
+                                               int blk) {
+  ROW_PROLOGUE_AT(T, blk)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
   const R kp = PM<R>::kexp(-P.p), kq = PM<R>::kexp(-2.0 * P.q);
   const R c2A = 2.0 * P.A0, pr0 = P.p / P.r0, qr0 = P.q / P.r0, xi = P.xi;
@@ -229,6 +234,12 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_rjl_force(RowArgs a, RjlP P, con
     write_force(a, i, fx, fy, fz);
     if (E) a.e_row[row] = P.A0 * er - P.xi * eb[i];
   }
+}
+
+template <int T, class R, bool E>
+__global__ void __launch_bounds__(PAIR_BLOCK) k_rjl_force(RowArgs a, RjlP P, const double* __restrict__ eb,
+                                                          int nb_real) {
+  rjl_force_rows<T, R, E>(a, P, eb, xcd_block(nb_real));
 }
 
 // -------------------------------------------------------------------- LJ
@@ -342,10 +353,10 @@ __device__ __forceinline__ void tb_own_bonds(const Geo<R>& G, const typename Geo
 // of the force pass: Bp = B**(1/delt+1) and
 // BT = delt/2 * Bp * fc * D/(S-1) * S * exp(-sqrt(2S) beta (r-r0)/S)
 template <class R>
-__global__ void __launch_bounds__(256) k_tb_bond(RowArgs a, TbP P, double* __restrict__ B, double* __restrict__ Bp,
-                                                 double* __restrict__ BT, int nb_real) {
+__device__ __forceinline__ void tb_bond_rows(const RowArgs& a, const TbP& P, double* __restrict__ B,
+                                             double* __restrict__ Bp, double* __restrict__ BT, int blk) {
   constexpr int T = TB_T;
-  ROW_PROLOGUE(T)
+  ROW_PROLOGUE_AT(T, blk)
   const R R2 = P.R2, a0 = P.a0, c02 = P.c02, d02 = P.d02;
   R xs[TB_U], ys[TB_U], zs[TB_U], rs[TB_U], fcs[TB_U], dfcs[TB_U];
   tb_own_bonds<R>(G, pi, nbrow, n, t, P, xs, ys, zs, rs, fcs, dfcs);
@@ -384,15 +395,36 @@ __global__ void __launch_bounds__(256) k_tb_bond(RowArgs a, TbP P, double* __res
   }
 }
 
+template <class R>
+__global__ void __launch_bounds__(256) k_tb_bond(RowArgs a, TbP P, double* __restrict__ B, double* __restrict__ Bp,
+                                                 double* __restrict__ BT, int nb_real) {
+  tb_bond_rows<R>(a, P, B, Bp, BT, xcd_block(nb_real));
+}
+
+// The normal-derivative terms of ljc/morsec for a graphene row (the rows of
+// the tb list are the rows of the nearest-neighbour list nl(3)): row i adds,
+// for each of its three nearest neighbours j, the vector that j's graphene
+// pass stored for i's place in j's nearest-neighbour row (cos_nn_coef).
+struct NdArgs {
+  const int* nnum;    // nl(3) row lengths
+  const int* nbr;     // nl(3) entries
+  const int* rev;     // place of the row atom in the neighbour's nl(3) row
+  const int* row_of;  // internal atom -> row of the graphene group
+  const double4* gC;
+  int stride;
+};
+
 // force pass (TersoffBrenner.f90:193-242): for each bond (i,j): dB_ij/dr_i
-// over i's row, dB_ji/dr_i and i as third body of j's bonds over j's row
-template <class R, bool E>
-__global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* __restrict__ row_of,
-                                                  const int* __restrict__ rev, const double* __restrict__ B,
-                                                  const double* __restrict__ Bp, const double* __restrict__ BT,
-                                                  int nb_real) {
+// over i's row, dB_ji/dr_i and i as third body of j's bonds over j's row;
+// with ND also the row's normal-derivative terms (lane q < 3 takes the q-th
+// nearest neighbour)
+template <class R, bool E, bool ND>
+__device__ __forceinline__ void tb_force_rows(const RowArgs& a, const TbP& P, const int* __restrict__ row_of,
+                                              const int* __restrict__ rev, const double* __restrict__ B,
+                                              const double* __restrict__ Bp, const double* __restrict__ BT,
+                                              const NdArgs& nd, int blk) {
   constexpr int T = TB_T;
-  ROW_PROLOGUE(T)
+  ROW_PROLOGUE_AT(T, blk)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, a0 = P.a0, c02 = P.c02, d02 = P.d02;
   const R D_S1 = R(P.D / (P.S - 1.0)), S = P.S, r0 = P.r0;
   const R s2S = R(sqrt(2.0 * P.S) * P.beta), s2oS = R(sqrt(2.0 / P.S) * P.beta), hdel = R(-P.delt * 0.5);
@@ -489,12 +521,28 @@ __global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* _
       if (E) e += 0.5 * (double)(pre * (ea - Bm * R(0.5) * se));
     }
   }
+  if (ND && t < 3 && nd.nnum[row] == 3) {
+    const int j = nd.nbr[(size_t)row * nd.stride + t];
+    const int l = nd.rev[(size_t)row * 3 + t];
+    if (l >= 0 && l <= 2 && nd.row_of[j] >= 0) {  // reported by k_rev / k_nn_from_tb otherwise
+      const double4 c = nd.gC[(size_t)j * 3 + l];
+      fx -= c.x; fy -= c.y; fz -= c.z;
+    }
+  }
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (E) e = group_sum<T>(e);
   if (t == 0) {
     write_force(a, i, fx, fy, fz);
     if (E) a.e_row[row] = e;
   }
+}
+
+template <class R, bool E>
+__global__ void __launch_bounds__(256) k_tb_force(RowArgs a, TbP P, const int* __restrict__ row_of,
+                                                  const int* __restrict__ rev, const double* __restrict__ B,
+                                                  const double* __restrict__ Bp, const double* __restrict__ BT,
+                                                  int nb_real) {
+  tb_force_rows<R, E, false>(a, P, row_of, rev, B, Bp, BT, NdArgs{}, xcd_block(nb_real));
 }
 
 // ------------------------------------------------------------ ljc / morsec
@@ -588,11 +636,10 @@ __device__ __forceinline__ void cos_nn_coef(const RowArgs& a, const CosP& P, con
 // metal pass (gnorm, and gnf in FP32 for the mixed path); with nearest
 // neighbours, the row's normal-derivative coefficients go to gC.
 template <int T, class R, bool E>
-__global__ void __launch_bounds__(PAIR_BLOCK) k_cos_graphene(RowArgs a, CosP P, const int* __restrict__ nn,
-                                                             int simplified, double4* __restrict__ gnorm,
-                                                             float4* __restrict__ gnf, double4* __restrict__ gC,
-                                                             int nb_real) {
-  ROW_PROLOGUE(T)
+__device__ __forceinline__ void cos_graphene_rows(const RowArgs& a, const CosP& P, const int* __restrict__ nn,
+                                                  int simplified, double4* __restrict__ gnorm,
+                                                  float4* __restrict__ gnf, double4* __restrict__ gC, int blk) {
+  ROW_PROLOGUE_AT(T, blk)
   double4 g;
   if (simplified) {
     g = make_double4(0, 0, 1, 0);
@@ -634,6 +681,14 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_cos_graphene(RowArgs a, CosP P, 
     if (sizeof(R) == 4) gnf[i] = make_float4((float)g.x, (float)g.y, (float)g.z, 0.f);
     if (E) a.e_row[row] = e;
   }
+}
+
+template <int T, class R, bool E>
+__global__ void __launch_bounds__(PAIR_BLOCK) k_cos_graphene(RowArgs a, CosP P, const int* __restrict__ nn,
+                                                             int simplified, double4* __restrict__ gnorm,
+                                                             float4* __restrict__ gnf, double4* __restrict__ gC,
+                                                             int nb_real) {
+  cos_graphene_rows<T, R, E>(a, P, nn, simplified, gnorm, gnf, gC, xcd_block(nb_real));
 }
 
 // normal-derivative rows (*_forces_for_graphene second loop): row i adds, for
@@ -738,6 +793,65 @@ __global__ void __launch_bounds__(256) k_cos_nn_metal(RowArgs an, RowArgs am, Co
   else cos_metal_rows<T, R>(am, P, gnorm, gnf, mrows, lb - nb_nn);
 }
 
+// ------------------------------------------------- graphene on a metal
+//
+// With rjl on the metal, tb on the graphene and ljc/morsec between them (no
+// common atom), the passes of the two sides share launches: the graphene
+// passes are short rows (three bonds, or about forty metal entries) whose
+// blocks wait on chains of dependent loads, and on their own they fill a
+// fraction of the GPU.  Three launches replace six:
+//   first:  tb bond orders (graphene rows) | rjl densities (metal rows)
+//   second: ljc/morsec graphene pass       | rjl forces
+//   third:  tb forces + normal-derivative  | ljc/morsec metal rows
+// The graphene range comes first in each grid, so its long blocks start
+// early; each range is XCD-remapped on its own (the metal work list keeps
+// hardware order).  Every atom's force is summed by one row per launch.
+
+template <int T, class R>
+__global__ void __launch_bounds__(256) k_gm_first(RowArgs ac, TbP Pt, double* __restrict__ B, double* __restrict__ Bp,
+                                                  double* __restrict__ BT, int npc, int nbc, RowArgs am, RjlP Pr,
+                                                  double* __restrict__ eb, int npm, int nbm) {
+  const int b = blockIdx.x;
+  if (b < npc) {
+    const int lb = xcd_local(b, npc, nbc);
+    if (lb >= 0) tb_bond_rows<R>(ac, Pt, B, Bp, BT, lb);
+  } else {
+    const int lb = xcd_local(b - npc, npm, nbm);
+    if (lb >= 0) rjl_density_rows<T, R>(am, Pr, eb, lb);
+  }
+}
+
+template <int T, class R, bool E>
+__global__ void __launch_bounds__(256) k_gm_second(RowArgs ag, CosP Pc, const int* __restrict__ nn, int simplified,
+                                                   double4* __restrict__ gnorm, float4* __restrict__ gnf,
+                                                   double4* __restrict__ gC, int npc, int nbc, RowArgs am, RjlP Pr,
+                                                   const double* __restrict__ eb, int npm, int nbm) {
+  const int b = blockIdx.x;
+  if (b < npc) {
+    const int lb = xcd_local(b, npc, nbc);
+    if (lb >= 0) cos_graphene_rows<(T < 4 ? T : 4), R, E>(ag, Pc, nn, simplified, gnorm, gnf, gC, lb);
+  } else {
+    const int lb = xcd_local(b - npc, npm, nbm);
+    if (lb >= 0) rjl_force_rows<T, R, E>(am, Pr, eb, lb);
+  }
+}
+
+template <int T, class R, bool E, bool ND>
+__global__ void __launch_bounds__(256) k_gm_third(RowArgs at, TbP Pt, const int* __restrict__ row_of,
+                                                  const int* __restrict__ rev, const double* __restrict__ B,
+                                                  const double* __restrict__ Bp, const double* __restrict__ BT,
+                                                  NdArgs nd, int npc, int nbc, RowArgs amet, CosP Pc,
+                                                  const double4* __restrict__ gnorm, const float4* __restrict__ gnf,
+                                                  const int* __restrict__ mrows, int nbm) {
+  const int b = blockIdx.x;
+  if (b < npc) {
+    const int lb = xcd_local(b, npc, nbc);
+    if (lb >= 0) tb_force_rows<R, E, ND>(at, Pt, row_of, rev, B, Bp, BT, nd, lb);
+  } else if (b - npc < nbm) {
+    cos_metal_rows<T, R>(amet, Pc, gnorm, gnf, mrows, b - npc);
+  }
+}
+
 // ----------------------------------------------------------------- dispatch
 
 static RowArgs row_args(pfm_ctx* h, Inter& it, int k, int accumulate) {
@@ -775,9 +889,92 @@ static int row_blocks(int nrows, int T, int block) { return (int)(((long)nrows *
     default: { constexpr int T = 8; __VA_ARGS__; } break;  \
   }
 
+// the lane counts of the co-scheduled launches
+#define DISPATCH_T3(T_, ...)                                 \
+  switch (T_) {                                            \
+    case 4: { constexpr int T = 4; __VA_ARGS__; } break;   \
+    case 16: { constexpr int T = 16; __VA_ARGS__; } break; \
+    default: { constexpr int T = 8; __VA_ARGS__; } break;  \
+  }
+
+static TbP tb_params(const Inter& it) {
+  return TbP{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6] * it.p[6], it.p[7] * it.p[7],
+             it.p[8], it.p[9], 1.0 / (it.p[9] - it.p[8])};
+}
+static RjlP rjl_params(const Inter& it) {
+  return RjlP{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6], 1.0 / (it.p[6] - it.p[5])};
+}
+static CosP cos_params(const Inter& it) {
+  if (it.kind == PFM_MORSEC)
+    return CosP{it.p[0], 0.0, it.p[2], it.p[1], it.p[3], it.p[4], it.p[5], 1.0 / (it.p[5] - it.p[4]),
+                2.0 * it.p[0] * it.p[3], 1};
+  return CosP{4.0 * it.p[0], it.p[1], 0.0, 0.0, it.p[2], it.p[3], it.p[4], 1.0 / (it.p[4] - it.p[3]),
+              4.0 * it.p[0] * it.p[2], 0};
+}
+
+// The three co-scheduled launches of graphene on a metal (h->gm); every
+// pass adds to forces the step's kick cleared.  Returns 1 when the lane
+// count has no fused variant (the caller launches the passes one by one).
+template <class R>
+static int gm_forces(pfm_ctx* h, bool energy) {
+  const int T = h->lanes;
+  if (T != 4 && T != 8 && T != 16) return 1;
+  Inter& ir = h->inter[h->gm.rjl];
+  Inter& it = h->inter[h->gm.tb];
+  Inter& ic = h->inter[h->gm.cos];
+  if (ir.nl[0].nrows == 0 || it.nl[0].nrows == 0 || ic.nl[0].nrows == 0) return 1;
+  const int BS = 256;
+  RowArgs am = row_args(h, ir, 0, 1), at = row_args(h, it, 0, 1), ag = row_args(h, ic, 0, 1);
+  RowArgs amet = row_args(h, ic, 1, 1);
+  const RjlP Pr = rjl_params(ir);
+  const TbP Pt = tb_params(it);
+  const CosP Pc = cos_params(ic);
+  const bool morse = ic.kind == PFM_MORSEC;
+  const int simp = (int)ic.p[morse ? 6 : 5] ? 1 : 0;
+  ListDev& NN = ic.nl[2];
+  const bool nd = !simp && NN.nrows > 0;
+  double* BT = it.Bp + (size_t)it.nl[0].nrows * it.nl[0].stride;
+  const int* row_of_tb = h->grp.row_of[it.nl[0].g1];
+  NdArgs ndA{NN.nnum, NN.nbr, NN.rev, h->grp.row_of[NN.g1], ic.gC, NN.stride};
+  // blocks of each range
+  const int nbt = row_blocks(at.nrows, TB_T, BS), npt = pad8(nbt);
+  const int nbm = row_blocks(am.nrows, T, BS), npm = pad8(nbm);
+  const int Tg = T < 4 ? T : 4;
+  const int nbg = row_blocks(ag.nrows, Tg, BS), npg = pad8(nbg);
+  const int nbmet = row_blocks(amet.nrows, T, BS);
+  DISPATCH_T3(T, {
+    PFM_LAUNCH(h, "gm_first", (k_gm_first<T, R>), npt + npm, BS, at, Pt, it.B, it.Bp, BT, npt, nbt, am, Pr, h->eb,
+               npm, nbm);
+    if (energy)
+      PFM_LAUNCH(h, "gm_second", (k_gm_second<T, R, true>), npg + npm, BS, ag, Pc, NN.nbr, simp, ic.gnorm, ic.gnf,
+                 ic.gC, npg, nbg, am, Pr, h->eb, npm, nbm);
+    else
+      PFM_LAUNCH(h, "gm_second", (k_gm_second<T, R, false>), npg + npm, BS, ag, Pc, NN.nbr, simp, ic.gnorm, ic.gnf,
+                 ic.gC, npg, nbg, am, Pr, h->eb, npm, nbm);
+    const int g3 = npt + nbmet;
+    if (energy && nd)
+      PFM_LAUNCH(h, "gm_third", (k_gm_third<T, R, true, true>), g3, BS, at, Pt, row_of_tb, it.nl[0].rev, it.B, it.Bp,
+                 BT, ndA, npt, nbt, amet, Pc, ic.gnorm, ic.gnf, ic.mrows, nbmet);
+    else if (energy)
+      PFM_LAUNCH(h, "gm_third", (k_gm_third<T, R, true, false>), g3, BS, at, Pt, row_of_tb, it.nl[0].rev, it.B, it.Bp,
+                 BT, ndA, npt, nbt, amet, Pc, ic.gnorm, ic.gnf, ic.mrows, nbmet);
+    else if (nd)
+      PFM_LAUNCH(h, "gm_third", (k_gm_third<T, R, false, true>), g3, BS, at, Pt, row_of_tb, it.nl[0].rev, it.B, it.Bp,
+                 BT, ndA, npt, nbt, amet, Pc, ic.gnorm, ic.gnf, ic.mrows, nbmet);
+    else
+      PFM_LAUNCH(h, "gm_third", (k_gm_third<T, R, false, false>), g3, BS, at, Pt, row_of_tb, it.nl[0].rev, it.B,
+                 it.Bp, BT, ndA, npt, nbt, amet, Pc, ic.gnorm, ic.gnf, ic.mrows, nbmet);
+  });
+  return PFM_OK;
+}
+
 template <class R>
 static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
   bool written = !first_writes;  // has any pass written this step's forces yet
+  if (h->fuse && h->gm.ok && written) {
+    const int rc = gm_forces<R>(h, energy);
+    if (rc != 1) return rc;
+  }
   const int BS = std::min(h->block, PairBlock<R>::max);  // row passes (T lanes per row)
   const int BS1 = 256;           // one-lane-per-row passes (tb, cos_nn)
   for (size_t ii = 0; ii < h->inter.size(); ++ii) {

@@ -179,6 +179,14 @@ __device__ __forceinline__ int xcd_block(int nblocks_real) {
   return lb < nblocks_real ? lb : -1;
 }
 
+// xcd_block for one sub-range of a launch that holds several passes: the
+// sub-range starts on a multiple of 8 blocks and spans np (a multiple of 8)
+// blocks, lb is the block's index inside it; returns -1 for padding blocks.
+__device__ __forceinline__ int xcd_local(int lb, int np, int nblocks_real) {
+  int m = (lb & 7) * (np >> 3) + (lb >> 3);
+  return m < nblocks_real ? m : -1;
+}
+
 // device error record: code, 1-based atom, count, step.  Kernels replayed
 // from a captured graph pass their step relative to the block they were
 // captured in; `base` holds the first step of the block being replayed (0
@@ -450,6 +458,13 @@ struct pfm_ctx {
   int* sp_orig = nullptr;
   // tuning
   int lanes = 8, block = 256;  // lanes per list row and workgroup size of the pair passes
+  // graphene on a metal (rjl on the metal, tb on the graphene, ljc/morsec
+  // between them, no common atom): passes over disjoint atoms share launches
+  struct GmPlan {
+    bool ok = false;
+    int rjl = -1, tb = -1, cos = -1;
+  } gm;
+  bool fuse = true;  // pfm_set_fusion
   // captured steady-state blocks of steps (pfm_run)
   bool use_graphs = true;
   int step_base = 0;  // first step of the block being captured (kernel step arguments are relative to it)

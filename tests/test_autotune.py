@@ -1,6 +1,8 @@
 """The launch-shape autotuner's host logic with a stand-in handle (no GPU):
 the fastest candidate is set and cached, FP64 skips workgroups above 256
-threads, and a cached shape is applied without timing."""
+threads, graphs and co-scheduled passes are kept only when faster, and a
+cached shape is applied without timing."""
+import re
 import time
 
 from pfmds_amd import autotune, capi, configs
@@ -9,9 +11,13 @@ from pfmds_amd import autotune, capi, configs
 class FakeCtx:
     """Steps cost a time that depends on the launch shape."""
 
-    def __init__(self, cost, graph_gain=1.0):
+    def __init__(self, cost, graph_gain=1.0, fusion_gain=1.0):
         self.cost, self.shape, self.set_calls, self.runs = cost, None, [], []
         self.graphs, self.graph_gain = False, graph_gain
+        self.fusion, self.fusion_gain = True, fusion_gain
+
+    def set_fusion(self, on):
+        self.fusion = on
 
     def set_tuning(self, lanes, block):
         self.shape = (lanes, block)
@@ -23,6 +29,7 @@ class FakeCtx:
     def run(self, integ, dt, s0, s1, energy):
         self.runs.append((s0, s1))
         t = self.cost[self.shape] * (s1 - s0 + 1)
+        t = t / self.fusion_gain if self.fusion else t
         time.sleep(t / self.graph_gain if self.graphs else t)
 
 
@@ -51,11 +58,21 @@ def test_graphs_chosen_when_faster():
     assert autotune.tune(slow, "nve", 1.0, capi.MIXED, steps=3, repeats=1)[2] is False and not slow.graphs
 
 
+def test_fusion_kept_only_when_faster():
+    cost = {c: 1e-3 for c in autotune.CANDIDATES}
+    fast = FakeCtx(cost, fusion_gain=3.0)
+    autotune.tune(fast, "nve", 1.0, capi.MIXED, steps=3, repeats=1)
+    assert fast.fusion and fast.fused
+    slow = FakeCtx(cost, fusion_gain=0.3)
+    autotune.tune(slow, "nve", 1.0, capi.MIXED, steps=3, repeats=1)
+    assert not slow.fusion and not slow.fused
+
+
 def test_fp64_skips_large_workgroups(tmp_path):
     cost = {c: 1e-4 for c in autotune.CANDIDATES}
     ctx = FakeCtx(cost)
     _, _, _, table, _ = autotune.tune(ctx, "nve", 1.0, capi.FP64, steps=2, repeats=1)
-    assert all(int(k.split("+")[0].split("x")[1]) <= 256 for k in table)
+    assert all(int(re.match(r"\d+x(\d+)", k).group(1)) <= 256 for k in table)
     assert "8x256" in table
 
 

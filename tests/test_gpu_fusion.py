@@ -1,0 +1,83 @@
+"""Co-scheduled passes of graphene on a metal (pfm_set_fusion) against the
+same passes launched one by one, and against the CPU checker.
+
+The fused launches run the same row arithmetic; only the order of the FP64
+force sums of a graphene atom differs (its graphene-pass, tb and
+normal-derivative contributions are added in another order).  Bar: FP64
+forces within 1e-13 relative (inf-norm) of the separate launches and
+energies within 1e-13; mixed precision within 1e-7 relative; the fused FP64
+path against the checker as the parity cases (forces 1e-10 at step 0,
+positions 1e-10 after 25 nvt steps); morsec and ljc, `simplified` F and T,
+lane counts 4, 8 and 16, captured blocks and direct launches.
+"""
+import numpy as np
+import pytest
+
+from pfmds_amd import capi, configs
+from pfmds_amd.system import Interaction
+from tests.harness import gpu_ctx, oracle_ctx, rel_inf
+
+pytestmark = pytest.mark.gpu
+
+
+def slab(kind="morsec", simplified=False, seed=71):
+    s = configs.graphene_on_cu(cx=14, cy=8, gx=14, gy=8, layers=4, seed=seed, jitter=0.1)
+    if kind == "ljc" or simplified:
+        mor = s.interactions[2]
+        if kind == "ljc":
+            params = [0.005, 3.2, 2.0, 6.0, 7.0, 1.0 if simplified else 0.0]
+        else:
+            params = list(configs.MORSEC_CCU[:6]) + [1.0 if simplified else 0.0]
+        s.interactions[2] = Interaction(kind, params, mor.lists)
+    return s
+
+
+def run(s, precision, fuse, lanes, graphs=True, steps=20):
+    c = gpu_ctx(s, precision)
+    c.set_fusion(fuse)
+    c.set_tuning(lanes, 256)
+    c.set_graphs(graphs)
+    c.run("nvt", 0.5, 0, 0, True)
+    f0 = c.state()[2]
+    e0 = c.energies()["e_inter"]
+    c.run("nvt", 0.5, 1, steps, True)
+    return f0, e0, c.state(), c.energies()["e_inter"]
+
+
+@pytest.mark.parametrize("kind,simplified", [("morsec", False), ("morsec", True), ("ljc", False)])
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_fused_equals_separate_fp64(kind, simplified, lanes):
+    s = slab(kind, simplified)
+    print(f"[fusion fp64] {kind} simplified={simplified} lanes={lanes} n={s.n}", flush=True)
+    a = run(s, capi.FP64, True, lanes)
+    b = run(s, capi.FP64, False, lanes)
+    assert rel_inf(a[0], b[0]) < 1e-13
+    np.testing.assert_allclose(a[1], b[1], rtol=1e-13)
+    assert rel_inf(a[2][0], b[2][0]) < 1e-12
+    assert rel_inf(a[2][2], b[2][2]) < 1e-11
+    np.testing.assert_allclose(a[3], b[3], rtol=1e-12)
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_fused_equals_separate_mixed(graphs):
+    s = slab()
+    a = run(s, capi.MIXED, True, 4, graphs)
+    b = run(s, capi.MIXED, False, 4, graphs)
+    assert rel_inf(a[0], b[0]) < 1e-7
+    np.testing.assert_allclose(a[1], b[1], rtol=1e-7)
+    assert rel_inf(a[2][0], b[2][0]) < 1e-9
+
+
+def test_fused_against_checker():
+    s = slab(seed=72)
+    g, o = gpu_ctx(s), oracle_ctx(s)
+    g.set_fusion(True)
+    for c in (g, o):
+        c.run("nvt", 0.5, 0, 0, True)
+    assert rel_inf(g.state()[2], o.state()[2]) < 1e-10
+    np.testing.assert_allclose(g.energies()["e_inter"], o.energies()["e_inter"], rtol=1e-10)
+    for c in (g, o):
+        c.run("nvt", 0.5, 1, 25, True)
+    pg, vg, fg = g.state()
+    po, vo, fo = o.state()
+    assert rel_inf(pg, po) < 1e-10 and rel_inf(vg, vo) < 1e-8 and rel_inf(fg, fo) < 1e-8
