@@ -20,6 +20,7 @@ import argparse
 import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -52,10 +53,12 @@ def timed_run(ctx, s, steps, warmup, precision):
     ctx.run("nve", dt, 0, warmup, True)
     shape, first = tuned(ctx, "nve", dt, precision, warmup + 1)
     e0 = ctx.energies()
+    # pfm_run returns after the stream has finished (its last status check
+    # synchronises); no energy evaluation inside the timed steps
     t0 = time.perf_counter()
-    ctx.run("nve", dt, first, first + steps - 1, True)
+    ctx.run("nve", dt, first, first + steps - 1, False)
     t1 = time.perf_counter()
-    return t1 - t0, e0, ctx.energies(), shape
+    return t1 - t0, e0, shape
 
 
 def mean_list_len(ctx):
@@ -94,10 +97,30 @@ def measured_traffic(kernel):
     return None
 
 
-def graphene_cu(steps, warmup):
+def nve_drift(ctx, n_atoms, dt, first, ps, segments=20):
+    """Slope (eV/atom/ns) of (E_tot(t) - E_tot(0))/N over `ps` picoseconds
+    of nve from step `first` on (the handle has run step first - 1),
+    sampled at `segments` equal segments; the steps run and the fitted span."""
+    nseg = max(1, int(round(ps * 1000.0 / dt / segments)))
+    ts, es = [], []
+    step = first
+    for k in range(segments + 1):
+        last = step + (nseg if k else 1) - 1
+        ctx.run("nve", dt, step, last, True)
+        e = ctx.energies()
+        ts.append(e["time"])
+        es.append(e["pe"] + e["ke"])
+        step = last + 1
+    t = np.array(ts) * 1e-6  # fs -> ns
+    slope = float(np.polyfit(t, (np.array(es) - es[0]) / n_atoms, 1)[0])
+    return slope, step, (ts[-1] - ts[0]) * 1e-3
+
+
+def graphene_cu(steps, warmup, drift_ps=20.0):
     """Graphene on Cu(111) (94,560 atoms: tb + morsec + rjl, nvt at 300 K,
-    dt = 0.5 fs) in FP64 and mixed precision, and the mixed force RMSE at
-    identical positions."""
+    dt = 0.5 fs) in FP64 and mixed precision, the mixed force RMSE at
+    identical positions, and the NVE drift of both paths from there (energy
+    per atom of all 94,560 atoms; the frozen Cu layers do no work)."""
     s = configs.graphene_on_cu()
     dt = s.integrators[-1].dt
     out, ctxs = {}, {}
@@ -106,7 +129,7 @@ def graphene_cu(steps, warmup):
         ctx.run("nvt", dt, 0, warmup, True)
         shape, first = tuned(ctx, "nvt", dt, prec, warmup + 1)
         t0 = time.perf_counter()
-        ctx.run("nvt", dt, first, first + steps - 1, True)
+        ctx.run("nvt", dt, first, first + steps - 1, False)
         t = time.perf_counter() - t0
         out[name + "_launch_shape"] = shape
         out[name + "_ms_per_step"] = t * 1e3 / steps
@@ -121,6 +144,16 @@ def graphene_cu(steps, warmup):
     out["force_rmse_eV_per_A"] = float(np.sqrt(np.mean((f32 - f64) ** 2)))
     out["atoms"] = s.n
     out["steps"] = steps
+    if drift_ps > 0:
+        d = {}
+        for name, ctx in ctxs.items():
+            d[name], _, span = nve_drift(ctx, s.n, dt, 1, drift_ps)
+        d["fit_ps"] = span
+        d["unit"] = "eV/atom/ns"
+        d["start"] = "the mixed path's state after its timed nvt steps, both paths"
+        out["nve_drift"] = d
+        print(f"# graphene/Cu NVE drift over {span:.1f} ps: mixed {d['mixed']:.3e}, fp64 {d['fp64']:.3e} eV/atom/ns",
+              file=sys.stderr, flush=True)
     print(f"# graphene/Cu: mixed {out['mixed_ms_per_step']:.4f} ms/step, fp64 {out['fp64_ms_per_step']:.4f} ms/step",
           file=sys.stderr, flush=True)
     return out
@@ -345,7 +378,7 @@ def replica_batch(steps, fp64_steps, check_steps=100, n_check=8):
         g.run("nve", 0.5, 0, 20, True)
         shape, first = tuned(g, "nve", 0.5, prec, 21)
         t0 = time.perf_counter()
-        g.run("nve", 0.5, first, first + n - 1, True)
+        g.run("nve", 0.5, first, first + n - 1, False)
         t = time.perf_counter() - t0
         out[name + "_launch_shape"] = shape
         out[name + "_ms_per_step"] = t * 1e3 / n
@@ -375,21 +408,126 @@ def replica_batch(steps, fp64_steps, check_steps=100, n_check=8):
     return out
 
 
-def cpu_baseline(s, threads):
+def host_cpu():
+    """The host's CPU model (lscpu), its physical cores, the CPUs this process
+    may run on, the cgroup's CPU quota and the CPU share OMP_NUM_THREADS
+    grants this job (a GPU box grants one GPU's job 16 CPUs); `threads` = the
+    physical cores among the usable CPUs, capped by the quota and the share."""
+    info = {"model": None, "physical_cores": None, "usable_cpus": len(os.sched_getaffinity(0)),
+            "quota_cpus": None}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+        rows = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True,
+                              timeout=20).stdout.splitlines()
+        rows = [r.split(",") for r in rows if r and not r.startswith("#")]
+        info["physical_cores"] = len({(r[1], r[2]) for r in rows})
+        usable = os.sched_getaffinity(0)
+        usable_cores = len({(r[1], r[2]) for r in rows if int(r[0]) in usable})
+    except (OSError, ValueError, subprocess.SubprocessError):
+        usable_cores = info["usable_cpus"]
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            info["quota_cpus"] = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    threads = usable_cores
+    if info["quota_cpus"]:
+        threads = min(threads, max(1, int(info["quota_cpus"])))
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        info["share_cpus"] = int(share)
+        threads = min(threads, int(share))
+    info["threads"] = max(1, threads)
+    return info
+
+
+def _oracle(threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from tests.harness import oracle_ctx, oracle_lib
 
     lib = oracle_lib()
     lib.orc_set_threads.argtypes = [ctypes.c_int]
     lib.orc_set_threads(threads)
+    return oracle_ctx
+
+
+def _cpu_leg_system(which, threads):
+    """One CPU leg in this (child) process: the oracle, which restates the
+    reference algorithm (brute-force O(N^2) list search, stored distances),
+    on one list-rebuild period (steps 0-9) of C1 (nve) or C2 (nvt)."""
+    oracle_ctx = _oracle(threads)
+    s = configs.fcc_cu_rgl() if which == "c1" else configs.graphene_on_cu()
+    integ, dt = ("nve", 1.0) if which == "c1" else ("nvt", 0.5)
     o = oracle_ctx(s)
     period = s.interactions[0].lists[0][4]
     t0 = time.perf_counter()
-    o.run("nve", s.integrators[0].dt, 0, period - 1, True)
+    o.run(integ, dt, 0, period - 1, True)
     t = time.perf_counter() - t0
-    return {"value": s.n * period / t, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-            "sample": f"steps 0-{period - 1} (one list-rebuild period) of the same {s.n}-atom system, "
-                      f"{t:.1f} s wall"}
+    return {"value": s.n * period / t, "atoms": s.n, "steps": period, "seconds": t}
+
+
+def _farm_one(k):
+    s = configs.replica_mix(only=k)[k]
+    oracle_ctx = _oracle(1)
+    o = oracle_ctx(s)
+    o.run("nve", 0.5, 0, 19, True)
+    return s.n * 20
+
+
+def _cpu_leg_farm(threads, per_worker=2):
+    """The reference's farm of independent runs (run_md_simulation_mpi.f90:68-88)
+    modelled on this host: one replica per core (OMP_NUM_THREADS=1), a
+    size-stratified sample of the 512 replicas (per_worker per core), 20 steps
+    each (one LJ rebuild period, two Cu ones)."""
+    import multiprocessing as mp
+
+    reps = configs.replica_mix()
+    order = np.argsort([r.n for r in reps], kind="stable")
+    pick = [int(order[i]) for i in np.linspace(0, len(reps) - 1, per_worker * threads).astype(int)]
+    env_threads = os.environ.get("OMP_NUM_THREADS")
+    os.environ["OMP_NUM_THREADS"] = "1"
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(threads) as pool:
+        work = sum(pool.map(_farm_one, pick, chunksize=1))
+    t = time.perf_counter() - t0
+    if env_threads is not None:
+        os.environ["OMP_NUM_THREADS"] = env_threads
+    return {"value": work / t, "replicas": pick, "atoms": int(sum(reps[k].n for k in pick)), "steps": 20,
+            "seconds": t}
+
+
+def cpu_baselines():
+    """The CPU legs, each in a child process with OMP_NUM_THREADS = the usable
+    physical cores, OMP_PLACES=cores, OMP_PROC_BIND=close."""
+    info = host_cpu()
+    n = info["threads"]
+    env = dict(os.environ, OMP_NUM_THREADS=str(n), OMP_PLACES="cores", OMP_PROC_BIND="close")
+    out = {}
+    for leg in ("c1", "c2", "farm"):
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", leg, "--cpu-threads", str(n)],
+                           env=env, capture_output=True, text=True, timeout=900)
+        if p.returncode != 0:
+            raise RuntimeError(f"cpu leg {leg} failed: {p.stderr[-2000:]}")
+        out[leg] = json.loads(p.stdout.strip().splitlines()[-1])
+        print(f"# cpu {leg}: {out[leg]['value']:.4e} atom-steps/s ({out[leg]['seconds']:.1f} s, {n} threads)",
+              file=sys.stderr, flush=True)
+    c1 = out["c1"]
+    return {"value": c1["value"], "unit": "atom-steps/s", "cores": n, "kind": "port",
+            "sample": f"steps 0-{c1['steps'] - 1} (one list-rebuild period) of the same {c1['atoms']}-atom system, "
+                      f"{c1['seconds']:.1f} s wall, OpenMP on {n} threads (OMP_PLACES=cores)",
+            "cpu": info,
+            "graphene_cu_nvt": {"value": out["c2"]["value"], "unit": "atom-steps/s",
+                                "sample": f"nvt steps 0-{out['c2']['steps'] - 1} of the {out['c2']['atoms']}-atom "
+                                          f"graphene/Cu system, {out['c2']['seconds']:.1f} s wall"},
+            "replica_farm": {"value": out["farm"]["value"], "unit": "atom-steps/s",
+                             "sample": f"{len(out['farm']['replicas'])} of the 512 replicas (size-stratified, "
+                                       f"{out['farm']['atoms']} atoms), 20 nve steps each, one replica per "
+                                       f"core (OMP_NUM_THREADS=1), {out['farm']['seconds']:.1f} s wall"}}
 
 
 def main():
@@ -413,14 +551,22 @@ def main():
     ap.add_argument("--nvms", action="store_true", help="time only the graphene/Cu nvms relaxation")
     ap.add_argument("--output-path", action="store_true",
                     help="time only the run with outputs: stopping at each output step vs output records")
+    ap.add_argument("--cpu-leg", choices=("c1", "c2", "farm"), help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-threads", type=int, default=1, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_leg:
+        # one CPU leg, run by cpu_baselines() in a child process
+        leg = _cpu_leg_farm(args.cpu_threads) if args.cpu_leg == "farm" else _cpu_leg_system(args.cpu_leg,
+                                                                                              args.cpu_threads)
+        print(json.dumps(leg))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step path has no multi-GPU decomposition "
               "(replicas or spatial domains would shard it; see the design notes). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     if args.c2_only:
-        print(json.dumps({"graphene_cu_nvt": graphene_cu(args.steps, args.warmup)}))
+        print(json.dumps({"graphene_cu_nvt": graphene_cu(args.steps, args.warmup, drift_ps=0)}))
         return
     if args.nvms:
         print(json.dumps({"nvms_phase": nvms_phase(min(args.steps, 2000), args.warmup)}))
@@ -435,7 +581,7 @@ def main():
         print(json.dumps({"replica_batch": replica_batch(args.steps, args.steps, check_steps=10, n_check=2)}))
         return
     if args.c4_only:
-        c2 = graphene_cu(200, 20)
+        c2 = graphene_cu(200, 20, drift_ps=0)
         kw = dict(frames_full=10, frames_sub=40, steps=2000) if args.c4_small else {}
         print(json.dumps({"surrogate": surrogate_c4(args.c4_iters, mixed_ms=c2["mixed_ms_per_step"],
                                                     fp64_ms=c2["fp64_ms_per_step"], **kw), "graphene_cu_nvt": c2}))
@@ -445,8 +591,8 @@ def main():
     ctxs = {}
     for name, prec in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
         ctx = gpu_ctx(s, prec)
-        t, e0, e1, shape = timed_run(ctx, s, args.steps, args.warmup, prec)
-        res[name] = dict(seconds=t, value=s.n * args.steps / t, e0=e0, e1=e1, shape=shape)
+        t, e0, shape = timed_run(ctx, s, args.steps, args.warmup, prec)
+        res[name] = dict(seconds=t, value=s.n * args.steps / t, e0=e0, shape=shape)
         ctxs[name] = ctx
         print(f"# {name}: {t * 1e3 / args.steps:.4f} ms/step, {s.n * args.steps / t:.4e} atom-steps/s", file=sys.stderr,
               flush=True)
@@ -515,8 +661,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(s, threads)
+        cpu = cpu_baselines()
 
     mixed, fp64 = res["mixed"], res["fp64"]
     line = {

@@ -183,6 +183,15 @@ int pfm_set_chain(pfm_ctx* h, int c, const double* x, const double* v);
 int pfm_get_list(pfm_ctx* h, int inter, int k, int* n_rows, int* nnum, int* nlist,
                  int* lessnnum, int stride);
 
+/* Per-row energies of interaction inter (1-based) at the last energy
+ * evaluation, one per row of its first list in the handle's row order;
+ * *n_rows their number (e_row may be NULL to ask for it).  Their sum is the
+ * interaction's energy (a compensated sum, as every energy and kinetic
+ * energy reduction of the library).  The reference keeps no per-row
+ * energies (its *_energy routines sum directly, e.g.
+ * RosatoGuillopeLegrand.f90:23-49); for inspection and tests. */
+int pfm_row_energies(pfm_ctx* h, int inter, int* n_rows, double* e_row);
+
 /* Graphene normals gr_norm(3,N_g1) of a ljc/morsec interaction
  * (graphenenorm.f90:38-56), double[N_g1][3] in group order. */
 int pfm_get_normals(pfm_ctx* h, int inter, double* gr_norm);

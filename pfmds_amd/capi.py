@@ -95,6 +95,8 @@ def load_library(path, prefix):
         lib.pfm_set_graphs.restype = C.c_int
         lib.pfm_set_fusion.argtypes = [C.c_void_p, C.c_int]
         lib.pfm_set_fusion.restype = C.c_int
+        lib.pfm_row_energies.argtypes = [C.c_void_p, C.c_int, _i, _d]
+        lib.pfm_row_energies.restype = C.c_int
         lib.pfm_graph_count.argtypes = [C.c_void_p, _P(C.c_long), _P(C.c_long)]
         lib.pfm_graph_count.restype = C.c_int
         lib.pfm_sur_set_params.argtypes = [C.c_void_p, C.c_int, _d, C.c_int]
@@ -303,6 +305,15 @@ class Context:
 
     def set_tuning(self, lanes_per_row=0, block=0):
         self._check(self.lib.pfm_set_tuning(self.h, int(lanes_per_row), int(block)))
+
+    def row_energies(self, inter):
+        """Per-row energies of interaction `inter` (1-based) at the last energy
+        evaluation (pfm_row_energies)."""
+        n = C.c_int()
+        self._check(self.lib.pfm_row_energies(self.h, int(inter), C.byref(n), None))
+        out = np.zeros(max(n.value, 1))
+        self._check(self.lib.pfm_row_energies(self.h, int(inter), C.byref(n), _dp(out)))
+        return out[:n.value]
 
     def set_fusion(self, on):
         """Co-scheduled passes of graphene on a metal (pfm_set_fusion)."""

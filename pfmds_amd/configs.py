@@ -307,16 +307,20 @@ def lj_gas_lattice(n, density, seed, temperature, dt=0.5, steps=2000, r1=6.5, r2
                   all_atoms=3, traj=3, md_step_limit=steps, init_xyz_filename="lj_gas.xyz")
 
 
-def replica_mix(n_rep=512, seed=1003, n_min=256, n_max=32768, steps=2000, dt=0.5):
+def replica_mix(n_rep=512, seed=1003, n_min=256, n_max=32768, steps=2000, dt=0.5, only=None):
     """Independent replicas of mixed size and kind: N_k = round(exp U(ln n_min,
     ln n_max)); even k a binary LJ gas at T ~ U(50, 300) K and density
     ~ U(0.01, 0.03) A^-3, odd k fcc Cu (RGL) of nx*ny*nz*4 ~ N_k atoms with at
     least four cells per side (list nmax 128) at T ~ U(300, 1200) K; replica k is seeded
-    seed*10**4 + k.  All step with nve at dt.  Returns the list of Systems."""
+    seed*10**4 + k.  All step with nve at dt.  Returns the list of Systems
+    (with `only` = k, replica k alone and None for the others)."""
     rng = SplitMix64(seed)
     u = rng.uniform(4 * n_rep).reshape(n_rep, 4)
     out = []
     for k in range(n_rep):
+        if only is not None and k != only:
+            out.append(None)
+            continue
         nk = int(round(np.exp(np.log(n_min) + u[k, 0] * (np.log(n_max) - np.log(n_min)))))
         sk = seed * 10**4 + k
         if k % 2 == 0:

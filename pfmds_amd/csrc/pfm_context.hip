@@ -96,17 +96,39 @@ struct MaskQ {
   uint32_t bit[RED_MAXQ];
 };
 
-__device__ __forceinline__ double block_sum(double v, double* sh) {
+// compensated sum of one (s, c) pair per thread over the block, in a fixed
+// tree order; sh holds 2 * RED_THREADS doubles
+__device__ __forceinline__ CSum block_csum(CSum v, double* sh) {
   int t = threadIdx.x;
-  sh[t] = v;
+  sh[t] = v.s;
+  sh[RED_THREADS + t] = v.c;
   __syncthreads();
   for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
-    if (t < o) sh[t] += sh[t + o];
+    if (t < o) {
+      CSum a{sh[t], sh[RED_THREADS + t]};
+      a.add(CSum{sh[t + o], sh[RED_THREADS + t + o]});
+      sh[t] = a.s;
+      sh[RED_THREADS + t] = a.c;
+    }
     __syncthreads();
   }
-  double r = sh[0];
+  CSum r{sh[0], sh[RED_THREADS]};
   __syncthreads();
   return r;
+}
+
+// Per-block partials of compensated sums: row q of a grid of nb blocks keeps
+// the s parts at part[2q nb + b] and the c parts at part[(2q+1) nb + b].
+__device__ __forceinline__ void put_part(double* part, int q, int nb, int b, const CSum& v) {
+  part[(size_t)(2 * q) * nb + b] = v.s;
+  part[(size_t)(2 * q + 1) * nb + b] = v.c;
+}
+// the compensated sum of row q's nb partials, in a fixed order
+__device__ __forceinline__ CSum reduce_part(const double* part, int q, int nb, double* sh) {
+  CSum a{0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += RED_THREADS)
+    a.add(CSum{part[(size_t)(2 * q) * nb + b], part[(size_t)(2 * q + 1) * nb + b]});
+  return block_csum(a, sh);
 }
 
 // second half: kick xyz/z, then for nvms the projections
@@ -162,19 +184,19 @@ __global__ void k_zero_masked(int N, double4* __restrict__ frc, const uint32_t* 
 __global__ void __launch_bounds__(RED_THREADS) k_ke_part(int N, const double4* __restrict__ vel,
                                                          const uint32_t* __restrict__ mask, MaskQ Q,
                                                          double* __restrict__ part) {
-  __shared__ double sh[RED_THREADS];
-  double acc[RED_MAXQ];
-  for (int q = 0; q < Q.nq; ++q) acc[q] = 0.0;
+  __shared__ double sh[2 * RED_THREADS];
+  CSum acc[RED_MAXQ];
+  for (int q = 0; q < Q.nq; ++q) acc[q] = CSum{0.0, 0.0};
   for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
     double4 v = vel[i];
     uint32_t m = mask[i];
     double ke = v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF;
     for (int q = 0; q < Q.nq; ++q)
-      if (m & Q.bit[q]) acc[q] += ke;
+      if (m & Q.bit[q]) acc[q].add(ke);
   }
   for (int q = 0; q < Q.nq; ++q) {
-    double s = block_sum(acc[q], sh);
-    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = s;
+    CSum s = block_csum(acc[q], sh);
+    if (threadIdx.x == 0) put_part(part, q, gridDim.x, blockIdx.x, s);
   }
 }
 
@@ -182,40 +204,28 @@ __global__ void __launch_bounds__(RED_THREADS) k_ke_part(int N, const double4* _
 __global__ void __launch_bounds__(RED_THREADS) k_mom_part(int N, const double4* __restrict__ vel,
                                                           const uint32_t* __restrict__ mask, uint32_t bit,
                                                           double* __restrict__ part) {
-  __shared__ double sh[RED_THREADS];
-  double a[4] = {0, 0, 0, 0};
+  __shared__ double sh[2 * RED_THREADS];
+  CSum a[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
   for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
     if (mask[i] & bit) {
       double4 v = vel[i];
-      a[0] += v.w * v.x; a[1] += v.w * v.y; a[2] += v.w * v.z; a[3] += v.w;
+      a[0].add(v.w * v.x); a[1].add(v.w * v.y); a[2].add(v.w * v.z); a[3].add(v.w);
     }
   }
   for (int q = 0; q < 4; ++q) {
-    double s = block_sum(a[q], sh);
-    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = s;
+    CSum s = block_csum(a[q], sh);
+    if (threadIdx.x == 0) put_part(part, q, gridDim.x, blockIdx.x, s);
   }
 }
 
 // reduce nq rows of nb partials each (fixed order) into out[q]
 __global__ void __launch_bounds__(RED_THREADS) k_reduce_rows(int nq, int nb, const double* __restrict__ part,
                                                              double* __restrict__ out) {
-  __shared__ double sh[RED_THREADS];
+  __shared__ double sh[2 * RED_THREADS];
   for (int q = 0; q < nq; ++q) {
-    double a = 0.0;
-    for (int b = threadIdx.x; b < nb; b += RED_THREADS) a += part[q * nb + b];
-    double s = block_sum(a, sh);
-    if (threadIdx.x == 0) out[q] = s;
+    CSum s = reduce_part(part, q, nb, sh);
+    if (threadIdx.x == 0) out[q] = s.value();
   }
-}
-
-// sum of a double array (per-row energies) into partials
-__global__ void __launch_bounds__(RED_THREADS) k_sum_part(int n, const double* __restrict__ x,
-                                                          double* __restrict__ part) {
-  __shared__ double sh[RED_THREADS];
-  double a = 0.0;
-  for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) a += x[i];
-  double s = block_sum(a, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // zero_momentum: v -= sum(m v)/sum(m) over the group (md_general.f90:236-253)
@@ -247,8 +257,9 @@ struct RecPartArgs {
   const double4 *pos, *vel, *frc;
   const uint32_t* mask;
   uint32_t ke_bit, all_bit;  // all_moving, all_atoms
-  double* epart;             // [ni + 1][nb]: interaction energies, kinetic energy
-  double* opart;             // [11][nb]: the out block, or null for the energies alone
+  double* epart;             // [ni + 1] compensated partial rows: interaction energies, kinetic energy
+  double* opart;             // [10] compensated partial rows of the out block and [nb] largest
+                             // v.v (at 20 nb), or null for the energies alone
 };
 
 // Per-block partials of one energy record.  The energy rows and the kinetic
@@ -258,37 +269,38 @@ struct RecPartArgs {
 // calculate_mass_center[_velocity], md_general.f90:184-234,277-298), row 10
 // the largest v.v over every atom (find_max_velocity, md_general.f90:366-380).
 __global__ void __launch_bounds__(RED_THREADS) k_rec_part(RecPartArgs A) {
-  __shared__ double sh[RED_THREADS];
+  __shared__ double sh[2 * RED_THREADS];
   const int nb = gridDim.x, b = blockIdx.x, t = threadIdx.x;
   for (int k = 0; k < A.ni; ++k) {
-    double a = 0.0;
-    for (int i = b * RED_THREADS + t; i < A.e_rows[k]; i += nb * RED_THREADS) a += A.e_row[k][i];
-    double s = block_sum(a, sh);
-    if (t == 0) A.epart[k * nb + b] = s;
+    CSum a{0.0, 0.0};
+    for (int i = b * RED_THREADS + t; i < A.e_rows[k]; i += nb * RED_THREADS) a.add(A.e_row[k][i]);
+    CSum s = block_csum(a, sh);
+    if (t == 0) put_part(A.epart, k, nb, b, s);
   }
-  double ke = 0.0;
-  double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  CSum ke{0.0, 0.0};
+  CSum a[10];
+  for (int q = 0; q < 10; ++q) a[q] = CSum{0.0, 0.0};
   double vmax = -1.0;
   for (int i = b * RED_THREADS + t; i < A.N; i += nb * RED_THREADS) {
     double4 v = A.vel[i];
     uint32_t m = A.mask[i];
     double v2 = v.x * v.x + v.y * v.y + v.z * v.z;
     vmax = v2 > vmax ? v2 : vmax;
-    if (m & A.ke_bit) ke += v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF;
-    if (m & A.all_bit) {
+    if (m & A.ke_bit) ke.add(v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF);
+    if (A.opart && (m & A.all_bit)) {
       double4 p = A.pos[i], f = A.frc[i];
-      a[0] += f.x; a[1] += f.y; a[2] += f.z;
-      a[3] += v.w * p.x; a[4] += v.w * p.y; a[5] += v.w * p.z;
-      a[6] += v.w * v.x; a[7] += v.w * v.y; a[8] += v.w * v.z;
-      a[9] += v.w;
+      a[0].add(f.x); a[1].add(f.y); a[2].add(f.z);
+      a[3].add(v.w * p.x); a[4].add(v.w * p.y); a[5].add(v.w * p.z);
+      a[6].add(v.w * v.x); a[7].add(v.w * v.y); a[8].add(v.w * v.z);
+      a[9].add(v.w);
     }
   }
-  double s = block_sum(ke, sh);
-  if (t == 0) A.epart[A.ni * nb + b] = s;
+  CSum s = block_csum(ke, sh);
+  if (t == 0) put_part(A.epart, A.ni, nb, b, s);
   if (!A.opart) return;
   for (int q = 0; q < 10; ++q) {
-    s = block_sum(a[q], sh);
-    if (t == 0) A.opart[q * nb + b] = s;
+    s = block_csum(a[q], sh);
+    if (t == 0) put_part(A.opart, q, nb, b, s);
   }
   sh[t] = vmax;
   __syncthreads();
@@ -296,7 +308,7 @@ __global__ void __launch_bounds__(RED_THREADS) k_rec_part(RecPartArgs A) {
     if (t < o) sh[t] = sh[t + o] > sh[t] ? sh[t + o] : sh[t];
     __syncthreads();
   }
-  if (t == 0) A.opart[10 * nb + b] = sh[0];
+  if (t == 0) A.opart[20 * nb + b] = sh[0];
 }
 
 // One energy record into its slot: the ne energy rows reduced as
@@ -307,19 +319,16 @@ __global__ void __launch_bounds__(RED_THREADS) k_rec_energy(int ne, int nb, cons
                                                             const double* __restrict__ chains, int nchain_dbl,
                                                             double* __restrict__ slots, int width, int ecap,
                                                             int* __restrict__ ctr, int frame) {
-  __shared__ double sh[RED_THREADS];
+  __shared__ double sh[2 * RED_THREADS];
   const int t = threadIdx.x;
   const int n_e = ctr[0];
   double* out = slots + (size_t)(n_e % ecap) * width;
   for (int q = 0; q < ne + 10; ++q) {
-    const double* part = q < ne ? epart + (size_t)q * nb : opart + (size_t)(q - ne) * nb;
-    double a = 0.0;
-    for (int b = t; b < nb; b += RED_THREADS) a += part[b];
-    double s = block_sum(a, sh);
-    if (t == 0) out[q] = s;
+    CSum s = q < ne ? reduce_part(epart, q, nb, sh) : reduce_part(opart, q - ne, nb, sh);
+    if (t == 0) out[q] = s.value();
   }
   double m = -1.0;
-  for (int b = t; b < nb; b += RED_THREADS) m = opart[10 * nb + b] > m ? opart[10 * nb + b] : m;
+  for (int b = t; b < nb; b += RED_THREADS) m = opart[20 * nb + b] > m ? opart[20 * nb + b] : m;
   sh[t] = m;
   __syncthreads();
   for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
@@ -363,17 +372,18 @@ __global__ void k_rec_frame_out(size_t n2, const double2* __restrict__ stage, do
 __global__ void __launch_bounds__(RED_THREADS) k_rep_zero_mom(int nrep, const int* __restrict__ row0,
                                                               double4* __restrict__ vel,
                                                               const uint32_t* __restrict__ mask, uint32_t bit) {
-  __shared__ double sh[RED_THREADS];
+  __shared__ double sh[2 * RED_THREADS];
   const int r = blockIdx.x;
   const int a0 = row0[r], a1 = row0[r + 1];
-  double a[4] = {0, 0, 0, 0};
+  CSum c[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
   for (int i = a0 + threadIdx.x; i < a1; i += RED_THREADS) {
     if (mask[i] & bit) {
       double4 v = vel[i];
-      a[0] += v.w * v.x; a[1] += v.w * v.y; a[2] += v.w * v.z; a[3] += v.w;
+      c[0].add(v.w * v.x); c[1].add(v.w * v.y); c[2].add(v.w * v.z); c[3].add(v.w);
     }
   }
-  for (int q = 0; q < 4; ++q) a[q] = block_sum(a[q], sh);
+  double a[4];
+  for (int q = 0; q < 4; ++q) a[q] = block_csum(c[q], sh).value();
   if (a[3] == 0.0) return;
   for (int i = a0 + threadIdx.x; i < a1; i += RED_THREADS) {
     if (mask[i] & bit) {
@@ -396,24 +406,24 @@ struct RepEnergyArgs {
 };
 
 __global__ void __launch_bounds__(RED_THREADS) k_rep_energy(RepEnergyArgs A) {
-  __shared__ double sh[RED_THREADS];
+  __shared__ double sh[2 * RED_THREADS];
   const int r = blockIdx.x, w = A.nrep + 1;
   for (int k = 0; k < A.ni; ++k) {
     const int r0 = A.row0[A.g1[k] * w + r], r1 = A.row0[A.g1[k] * w + r + 1];
-    double e = 0.0;
-    for (int q = r0 + threadIdx.x; q < r1; q += RED_THREADS) e += A.e_row[k][q];
-    e = block_sum(e, sh);
-    if (threadIdx.x == 0) A.out[r * (A.ni + 1) + k] = e;
+    CSum e{0.0, 0.0};
+    for (int q = r0 + threadIdx.x; q < r1; q += RED_THREADS) e.add(A.e_row[k][q]);
+    e = block_csum(e, sh);
+    if (threadIdx.x == 0) A.out[r * (A.ni + 1) + k] = e.value();
   }
-  double ke = 0.0;
+  CSum ke{0.0, 0.0};
   for (int i = A.row0[r] + threadIdx.x; i < A.row0[r + 1]; i += RED_THREADS) {
     if (A.mask[i] & A.bit) {
       double4 v = A.vel[i];
-      ke += v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF;
+      ke.add(v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF);
     }
   }
-  ke = block_sum(ke, sh);
-  if (threadIdx.x == 0) A.out[r * (A.ni + 1) + A.ni] = ke;
+  ke = block_csum(ke, sh);
+  if (threadIdx.x == 0) A.out[r * (A.ni + 1) + A.ni] = ke.value();
 }
 
 // integrate_nose_hoover_chain (md_integrators.f90:200-245) on one chain with
@@ -467,16 +477,14 @@ struct ChainArgs {
 
 __global__ void __launch_bounds__(RED_THREADS) k_nhc_chain(int nchain, ChainArgs A, const double* __restrict__ part,
                                                            int nb) {
-  __shared__ double sh[RED_THREADS];
+  __shared__ double sh[2 * RED_THREADS];
   // the chain's serial update reads and writes its state many times over:
   // work on a copy in LDS, loaded and stored by the whole block at once
   __shared__ NhcDev st;
   constexpr int W = sizeof(NhcDev) / sizeof(double);
   for (int c = 0; c < nchain; ++c) {
     for (int j = threadIdx.x; j < W; j += RED_THREADS) ((double*)&st)[j] = ((const double*)&A.C[c])[j];
-    double a = 0.0;
-    for (int b = threadIdx.x; b < nb; b += RED_THREADS) a += part[c * nb + b];
-    double ke = block_sum(a, sh);  // ends with a barrier: the copy is complete
+    const double ke = reduce_part(part, c, nb, sh).value();  // ends with a barrier: the copy is complete
     if (threadIdx.x == 0) {
       double s = nhc_update(st, ke, A.ts1, A.ts2, A.ts3);
       if (A.twice) s *= nhc_update(st, s * s * ke, A.ts1, A.ts2, A.ts3);
@@ -496,9 +504,9 @@ __global__ void __launch_bounds__(RED_THREADS) k_kick2_ke(int N, double4* __rest
                                                           const double4* __restrict__ frc,
                                                           const uint32_t* __restrict__ mask, IntP P, MaskQ Q,
                                                           double* __restrict__ part) {
-  __shared__ double sh[RED_THREADS];
-  double acc[RED_MAXQ];
-  for (int q = 0; q < Q.nq; ++q) acc[q] = 0.0;
+  __shared__ double sh[2 * RED_THREADS];
+  CSum acc[RED_MAXQ];
+  for (int q = 0; q < Q.nq; ++q) acc[q] = CSum{0.0, 0.0};
   for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
     uint32_t m = mask[i];
     bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
@@ -512,11 +520,11 @@ __global__ void __launch_bounds__(RED_THREADS) k_kick2_ke(int N, double4* __rest
     }
     double ke = v.w * (v.x * v.x + v.y * v.y + v.z * v.z) / 2 * PFM_MASS_COEF;
     for (int q = 0; q < Q.nq; ++q)
-      if (m & Q.bit[q]) acc[q] += ke;
+      if (m & Q.bit[q]) acc[q].add(ke);
   }
   for (int q = 0; q < Q.nq; ++q) {
-    double s = block_sum(acc[q], sh);
-    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = s;
+    CSum s = block_csum(acc[q], sh);
+    if (threadIdx.x == 0) put_part(part, q, gridDim.x, blockIdx.x, s);
   }
 }
 
@@ -1325,7 +1333,8 @@ static int rec_prepare(pfm_ctx* h) {
   R.ewidth = width;
   R.ecap = REC_ECAP;
   R.xcap = want_x;
-  if ((rc = dalloc(h, &R.d_part, (size_t)REC_OUT * h->red_blocks)) || (rc = dalloc(h, &R.d_ctr, 4))) return rc;
+  if ((rc = dalloc(h, &R.d_part, (size_t)(2 * (REC_OUT - 1) + 1) * h->red_blocks)) || (rc = dalloc(h, &R.d_ctr, 4)))
+    return rc;
   // the record kernels write the slots in pinned host memory themselves
   HIPCHK(h, hipHostMalloc((void**)&R.h_e, sizeof(double) * R.ecap * width, hipHostMallocCoherent));
   if (R.xcap > 0) {
@@ -1550,7 +1559,8 @@ int pfm_create_batch(pfm_ctx** out, int nrep, const int* n_atoms, const double* 
   for (int d = 0; d < 3; ++d) (&h->qscale.x)[d] = 4294967296.0 / boxes[d];
   if (precision == PFM_MIXED && (rc = dalloc(h, &h->qpos, n))) return rc;
   h->red_blocks = std::max(1, std::min(1024, (n + RED_THREADS - 1) / RED_THREADS));
-  if ((rc = dalloc(h, &h->red_part, (size_t)(PFM_MAXI + 1 + RED_MAXQ) * h->red_blocks))) return rc;
+  // compensated partial rows: an (s, c) pair per block
+  if ((rc = dalloc(h, &h->red_part, (size_t)2 * (PFM_MAXI + 1 + RED_MAXQ) * h->red_blocks))) return rc;
   if ((rc = dalloc(h, &h->red_out, 64))) return rc;
   if ((rc = dalloc(h, &h->derr, 1))) return rc;
   HIPCHK(h, hipHostMalloc((void**)&h->h_red, sizeof(double) * 64, 0));
@@ -2253,16 +2263,16 @@ __global__ void __launch_bounds__(RED_THREADS) k_force_dev(int N, const int* __r
                                                            const double4* __restrict__ fb,
                                                            const uint32_t* __restrict__ mask, uint32_t bit,
                                                            double* __restrict__ part) {
-  __shared__ double sh[RED_THREADS];
-  double a = 0.0;
+  __shared__ double sh[2 * RED_THREADS];
+  CSum a{0.0, 0.0};
   for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < N; i += gridDim.x * RED_THREADS) {
     if (!(mask[i] & bit)) continue;
     const double4 x = fa_orig[orig[i]], y = fb[i];
     const double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
-    a += dx * dx + dy * dy + dz * dz;
+    a.add(dx * dx + dy * dy + dz * dz);
   }
-  double s = block_sum(a, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  CSum s = block_csum(a, sh);
+  if (threadIdx.x == 0) put_part(part, 0, gridDim.x, blockIdx.x, s);
 }
 
 // the staging of handle h, written next on another handle's stream: h's own
@@ -2319,8 +2329,8 @@ int pfm_force_rmse(pfm_ctx* a, pfm_ctx* b, int group, double* rmse) {
 // graph capture, serving loops).  Every handle on a device shares that
 // device's copy of the array, so calls that overlap on one device race.
 #define PFM_NOALLOC_MAX_ATOMS (1 << 20)  // 32 MiB of staging
-#define PFM_NOALLOC_PARTS 1024           // partial sums (red_blocks never exceeds it)
-static __device__ __align__(256) double4 scratch[PFM_NOALLOC_MAX_ATOMS + PFM_NOALLOC_PARTS / 4 + 1];
+#define PFM_NOALLOC_PARTS 1024           // blocks of partial sums (red_blocks never exceeds it)
+static __device__ __align__(256) double4 scratch[PFM_NOALLOC_MAX_ATOMS + 2 * PFM_NOALLOC_PARTS / 4 + 1];
 
 #define PFM_MAX_DEVICES 64
 static std::mutex scratch_mu;
@@ -2366,7 +2376,7 @@ int pfm_force_rmse_noalloc(pfm_ctx* a, pfm_ctx* b, int group, double* rmse) {
   double4* st = nullptr;
   if ((rc = scratch_on(b, &st))) return rc;
   double* part = (double*)(st + PFM_NOALLOC_MAX_ATOMS);
-  double* out = part + PFM_NOALLOC_PARTS;
+  double* out = part + 2 * PFM_NOALLOC_PARTS;  // after the (s, c) partial rows
   const int N = a->N, nb = (N + 255) / 256;
   // a's forces into original order on a's stream (every slot < N is written:
   // orig is a permutation), then the comparison on b's stream
@@ -2450,6 +2460,18 @@ int pfm_get_list(pfm_ctx* h, int inter, int k, int* n_rows, int* nnum, int* nlis
       if (less < 0 && h->grp.idx[g2][e[p]] > a) less = p;
     }
     if (lessnnum) lessnnum[row] = less < 0 ? n : less;
+  }
+  return PFM_OK;
+}
+
+int pfm_row_energies(pfm_ctx* h, int inter, int* n_rows, double* e_row) {
+  if (inter < 1 || inter > (int)h->inter.size()) return fail(h, PFM_E_INVALID, "error: no such interaction");
+  const Inter& it = h->inter[inter - 1];
+  *n_rows = it.e_rows;
+  if (e_row && it.e_rows > 0) {
+    if (!it.e_row) return fail(h, PFM_E_INVALID, "error: no energy evaluated yet");
+    HIPCHK(h, hipMemcpyAsync(e_row, it.e_row, sizeof(double) * it.e_rows, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
   }
   return PFM_OK;
 }

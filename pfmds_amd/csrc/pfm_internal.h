@@ -161,6 +161,25 @@ __device__ __forceinline__ void cutoff_bl(R r, R R1, R inv_w, R kdf, R& f, R& df
   dfr = kdf * sn;
 }
 
+// Neumaier's compensated sum (Neumaier 1974, "Rundungsfehleranalyse einiger
+// Verfahren zur Summation endlicher Summen"): s holds the running sum, c the
+// rounding errors of its additions, and the sum is s + c.  The energy,
+// kinetic-energy, momentum and force-deviation reductions carry (s, c) pairs
+// from each thread through the block trees and the per-block partials.
+struct CSum {
+  double s, c;
+  __device__ __forceinline__ void add(double x) {
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ void add(const CSum& o) {
+    add(o.s);
+    c += o.c;
+  }
+  __device__ __forceinline__ double value() const { return s + c; }
+};
+
 // sum over the aligned group of T lanes that share one atom row
 template <int T, class V>
 __device__ __forceinline__ V group_sum(V v) {
