@@ -37,6 +37,9 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
 
+GRAPHS = True  # --no-graphs: every step launched directly (profiling)
+
+
 def gpu_ctx(s, precision):
     return system.build_context(s, lambda b, p, v, m: capi.Context.gpu(b, p, v, m, precision))
 
@@ -44,8 +47,25 @@ def gpu_ctx(s, precision):
 def tuned(ctx, integrator, dt, precision, first):
     """Launch shape and launch mode from the autotuner, tried on the live
     handle from step `first` on; returns ("LxB[+graphs]", next step)."""
-    lanes, block, graphs, _, nxt = autotune.tune(ctx, integrator, dt, precision, first, steps=100, repeats=3)
+    lanes, block, graphs, _, nxt = autotune.tune(ctx, integrator, dt, precision, first, steps=100, repeats=3,
+                                                 graphs=GRAPHS)
     return f"{lanes}x{block}" + ("+graphs" if graphs else ""), nxt
+
+
+def kernels_only(which, steps=50, warmup=10):
+    """A short mixed-precision run of one workload (c1: fcc Cu RGL 131,072
+    atoms, nve; c2: graphene on Cu(111) 94,560 atoms, nvt) with a fixed launch
+    shape (8 lanes x 256 threads, co-scheduled passes), no autotuning and no
+    FP64 leg: the program of the counter passes."""
+    s = configs.fcc_cu_rgl() if which == "c1" else configs.graphene_on_cu()
+    integ, dt = ("nve", 1.0) if which == "c1" else ("nvt", 0.5)
+    ctx = gpu_ctx(s, capi.MIXED)
+    ctx.set_tuning(8, 256)
+    ctx.run(integ, dt, 0, warmup, True)
+    t0 = time.perf_counter()
+    ctx.run(integ, dt, warmup + 1, warmup + steps, False)
+    t = time.perf_counter() - t0
+    return {"workload": which, "atoms": s.n, "steps": steps, "ms_per_step": t * 1e3 / steps}
 
 
 def timed_run(ctx, s, steps, warmup, precision):
@@ -551,6 +571,9 @@ def main():
     ap.add_argument("--nvms", action="store_true", help="time only the graphene/Cu nvms relaxation")
     ap.add_argument("--output-path", action="store_true",
                     help="time only the run with outputs: stopping at each output step vs output records")
+    ap.add_argument("--kernels", choices=("c1", "c2"),
+                    help="only a short fixed-shape mixed run of one workload (counter passes)")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every step directly (profiling)")
     ap.add_argument("--cpu-leg", choices=("c1", "c2", "farm"), help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=1, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -565,6 +588,13 @@ def main():
               "(replicas or spatial domains would shard it; see the design notes). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if args.no_graphs:
+        global GRAPHS
+        GRAPHS = False
+        capi.GRAPHS_DEFAULT = False
+    if args.kernels:
+        print(json.dumps({"kernels": kernels_only(args.kernels)}))
+        return
     if args.c2_only:
         print(json.dumps({"graphene_cu_nvt": graphene_cu(args.steps, args.warmup, drift_ps=0)}))
         return

@@ -45,11 +45,11 @@ def time_shape(ctx, integrator, dt, first, steps):
     return time.perf_counter() - t0
 
 
-def tune(ctx, integrator, dt, precision, first_step=1, steps=40, repeats=2, candidates=CANDIDATES):
+def tune(ctx, integrator, dt, precision, first_step=1, steps=40, repeats=2, candidates=CANDIDATES, graphs=True):
     """Try every candidate shape on the handle, from step `first_step` on
     (the handle must have run step first_step - 1), set the fastest, then the
-    faster of graph and direct launches, and return (lanes, block, graphs,
-    {setting: seconds}, next step)."""
+    faster of graph and direct launches (direct only with graphs=False), and
+    return (lanes, block, graphs, {setting: seconds}, next step)."""
     table = {}
     step = first_step
 
@@ -77,9 +77,10 @@ def tune(ctx, integrator, dt, precision, first_step=1, steps=40, repeats=2, cand
     table[f"{pick}-fusion"] = best_of()
     fused = table[pick] <= table[f"{pick}-fusion"]
     ctx.set_fusion(fused)
-    ctx.set_graphs(True)
-    table[f"{pick}+graphs"] = best_of()
-    graphs = table[f"{pick}+graphs"] < table[pick]
+    if graphs:
+        ctx.set_graphs(True)
+        table[f"{pick}+graphs"] = best_of()
+        graphs = table[f"{pick}+graphs"] < table[pick]
     ctx.set_graphs(graphs)
     ctx.fused = fused
     return lanes, block, graphs, table, step

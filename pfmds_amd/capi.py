@@ -22,6 +22,9 @@ EV_DONE, EV_CONVERGED = 0, 1
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GPU_LIB = os.path.join(HERE, "lib", "libpfmds.so")
+# captured steps on new GPU handles (pfm_set_graphs); a profiling run may
+# switch them off for every handle it makes
+GRAPHS_DEFAULT = True
 
 _P = C.POINTER
 _d = _P(C.c_double)
@@ -165,11 +168,17 @@ class Context:
 
     @classmethod
     def gpu(cls, box, pos, vel, mass, precision=FP64, path=GPU_LIB):
-        return cls(load_library(path, "pfm"), "pfm", box, pos, vel, mass, precision)
+        c = cls(load_library(path, "pfm"), "pfm", box, pos, vel, mass, precision)
+        if not GRAPHS_DEFAULT:
+            c.set_graphs(False)
+        return c
 
     @classmethod
     def gpu_batch(cls, n_atoms, boxes, pos, vel, mass, precision=FP64, path=GPU_LIB):
-        return cls(load_library(path, "pfm"), "pfm", boxes, pos, vel, mass, precision, n_atoms=n_atoms)
+        c = cls(load_library(path, "pfm"), "pfm", boxes, pos, vel, mass, precision, n_atoms=n_atoms)
+        if not GRAPHS_DEFAULT:
+            c.set_graphs(False)
+        return c
 
     def _fn(self, name):
         return getattr(self.lib, f"{self.pfx}_{name}")

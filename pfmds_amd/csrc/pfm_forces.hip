@@ -216,8 +216,8 @@ __device__ __forceinline__ void rjl_force_rows(const RowArgs& a, const RjlP& P, 
   double fx = 0, fy = 0, fz = 0, er = 0;
   walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
     R dx, dy, dz;
-    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
-    R ir = PM<R>::rcp(r);
+    R r, ir;
+    dist_and_inverse<R>(G.d2(pi, q, dx, dy, dz), r, ir);
     R fc, dfr;
     cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
     R x = r * ir0 - R(1);
@@ -255,8 +255,8 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_lj(RowArgs a, LjP P, int nb_real
   double fx = 0, fy = 0, fz = 0, e = 0;
   walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
     R dx, dy, dz;
-    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
-    R ir = PM<R>::rcp(r);
+    R r, ir;
+    dist_and_inverse<R>(G.d2(pi, q, dx, dy, dz), r, ir);
     R fc, dfr;
     cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
     R s = sig * ir, s2 = s * s, V = s2 * s2 * s2;
@@ -553,9 +553,9 @@ struct CosP {
 };
 
 template <class R>
-__device__ __forceinline__ void cos_pair(const CosP& P, R r, R nd, R nnorm_r, R& A, R& Bc, R& S_coef, R& epair) {
+__device__ __forceinline__ void cos_pair(const CosP& P, R r, R ir, R nd, R nnorm_r, R& A, R& Bc, R& S_coef,
+                                         R& epair) {
   const R R1 = P.R1, invw = P.invw, delt = P.delt, kdf = -0.5 * PFM_PI * P.invw;
-  R ir = PM<R>::rcp(r);
   R fc, dfr;
   cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
   R dfc = dfr * ir;
@@ -657,10 +657,11 @@ __device__ __forceinline__ void cos_graphene_rows(const RowArgs& a, const CosP& 
   double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, e = 0;
   walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
     R dx, dy, dz;
-    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
+    R r, ir;
+    dist_and_inverse<R>(G.d2(pi, q, dx, dy, dz), r, ir);
     R nd = nx * dx + ny * dy + nz * dz;
     R A, Bc, Sc, ep;
-    cos_pair<R>(P, r, nd, PM<R>::rcp(r), A, Bc, Sc, ep);
+    cos_pair<R>(P, r, ir, nd, ir, A, Bc, Sc, ep);
     bool in = valid && r < R2;
     A = in ? A * pref : R(0);
     Bc = in ? Bc * pref : R(0);
@@ -754,12 +755,13 @@ __device__ __forceinline__ void cos_metal_rows(const RowArgs& a, const CosP& P, 
   double fx = 0, fy = 0, fz = 0;
   auto f = [&](int, const typename Geo<R>::P& q, const auto& g, bool valid) {
     R dx, dy, dz;
-    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
+    R r, ir;
+    dist_and_inverse<R>(G.d2(pi, q, dx, dy, dz), r, ir);
     const R gx = g.x, gy = g.y, gz = g.z;
     R nd = gx * dx + gy * dy + gz * dz;
     R nn = P.morse ? PM<R>::sqrt_(gx * gx + gy * gy + gz * gz) * r : r;
     R A, Bc, Sc, ep;
-    cos_pair<R>(P, r, nd, PM<R>::rcp(nn), A, Bc, Sc, ep);
+    cos_pair<R>(P, r, ir, nd, P.morse ? PM<R>::rcp(nn) : ir, A, Bc, Sc, ep);
     bool in = valid && r < R2;
     A = in ? A * pref : R(0);
     Bc = in ? Bc * pref : R(0);

@@ -7,6 +7,7 @@
 // group-1 atom (rows in internal order); distances are never stored: every
 // pass recomputes dr from pos4.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -159,6 +160,29 @@ __device__ __forceinline__ void cutoff_bl(R r, R R1, R inv_w, R kdf, R& f, R& df
   PM<R>::sincospi_(x, sn, cs);
   f = R(0.5) + R(0.5) * cs;
   dfr = kdf * sn;
+}
+// Mixed path: the argument in revolutions t = (r - R1) / (2 (R2 - R1)) is one
+// fused multiply-add (its constants are loop invariants), clamped to [0, 1/2]
+// by one v_med3_f32, then v_sin_f32 / v_cos_f32 of 2 pi t = pi x.
+template <>
+__device__ __forceinline__ void cutoff_bl<float>(float r, float R1, float inv_w, float kdf, float& f, float& dfr) {
+  const float a = 0.5f * inv_w, b = -0.5f * R1 * inv_w;
+  const float t = __builtin_amdgcn_fmed3f(__builtin_fmaf(r, a, b), 0.0f, 0.5f);
+  f = 0.5f + 0.5f * __builtin_amdgcn_cosf(t);
+  dfr = kdf * __builtin_amdgcn_sinf(t);
+}
+
+// r = |dr| and 1/r from dr.dr: FP64 with the correctly rounded square root
+// and division; the mixed path from one v_rsq_f32 (r = d2 / sqrt(d2))
+template <class R>
+__device__ __forceinline__ void dist_and_inverse(R d2, R& r, R& ir) {
+  r = PM<R>::sqrt_(d2);
+  ir = PM<R>::rcp(r);
+}
+template <>
+__device__ __forceinline__ void dist_and_inverse<float>(float d2, float& r, float& ir) {
+  ir = __builtin_amdgcn_rsqf(d2);
+  r = d2 * ir;
 }
 
 // Neumaier's compensated sum (Neumaier 1974, "Rundungsfehleranalyse einiger
@@ -547,12 +571,19 @@ struct pfm_ctx {
 void tim_begin(pfm_ctx* h, const char* name, hipEvent_t* a, hipEvent_t* b);
 void tim_end(pfm_ctx* h, hipEvent_t a, hipEvent_t b, const char* name);
 
-#define PFM_LAUNCH(h, name, kern, grid, blk, ...)                                   \
-  do {                                                                              \
-    hipEvent_t _a = nullptr, _b = nullptr;                                          \
-    if ((h)->timing) tim_begin((h), name, &_a, &_b);                                \
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(blk), 0, (h)->stream, __VA_ARGS__);   \
-    if ((h)->timing) tim_end((h), _a, _b, name);                                    \
+// Timed launches (pfm_set_timing) go through hipExtLaunchKernelGGL, whose
+// start and stop events take the timestamps of the kernel's own dispatch, so
+// a mean duration excludes the gaps between launches.
+#define PFM_LAUNCH(h, name, kern, grid, blk, ...)                                                       \
+  do {                                                                                                  \
+    if ((h)->timing) {                                                                                  \
+      hipEvent_t _a = nullptr, _b = nullptr;                                                            \
+      tim_begin((h), name, &_a, &_b);                                                                   \
+      hipExtLaunchKernelGGL(kern, dim3(grid), dim3(blk), 0, (h)->stream, _a, _b, 0, __VA_ARGS__);       \
+      tim_end((h), _a, _b, name);                                                                       \
+    } else {                                                                                            \
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(blk), 0, (h)->stream, __VA_ARGS__);                     \
+    }                                                                                                   \
   } while (0)
 
 static inline int pad8(int nb) { return ((nb + 7) / 8) * 8; }
