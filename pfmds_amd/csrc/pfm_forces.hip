@@ -248,9 +248,10 @@ struct LjP {
   double eps4, sig, R1, R2, invw;
 };
 
+// (E with a.e_row null: the converse rows nl(2), which carry no energy)
 template <int T, class R, bool E>
-__global__ void __launch_bounds__(PAIR_BLOCK) k_lj(RowArgs a, LjP P, int nb_real) {
-  ROW_PROLOGUE(T)
+__device__ __forceinline__ void lj_rows(const RowArgs& a, const LjP& P, int blk) {
+  ROW_PROLOGUE_AT(T, blk)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, sig = P.sig, e4 = P.eps4, kdf = -0.5 * PFM_PI * P.invw;
   double fx = 0, fy = 0, fz = 0, e = 0;
   walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
@@ -270,8 +271,13 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_lj(RowArgs a, LjP P, int nb_real
   if (E) e = group_sum<T>(e);
   if (t == 0) {
     write_force(a, i, fx, fy, fz);
-    if (E) a.e_row[row] = e;
+    if (E && a.e_row) a.e_row[row] = e;
   }
+}
+
+template <int T, class R, bool E>
+__global__ void __launch_bounds__(PAIR_BLOCK) k_lj(RowArgs a, LjP P, int nb_real) {
+  lj_rows<T, R, E>(a, P, xcd_block(nb_real));
 }
 
 // lj1g: the reference sums entries p <= lessnnum(i) of each row once and
@@ -286,8 +292,8 @@ struct Lj1gP {
 // The weights come from the entries' tags (k_lj1g_tag): bit 30 the pair is in
 // i's half list, bit 31 in j's.
 template <int T, class R, bool E>
-__global__ void __launch_bounds__(PAIR_BLOCK) k_lj1g(RowArgs a, Lj1gP P, int nb_real) {
-  ROW_PROLOGUE(T)
+__device__ __forceinline__ void lj1g_rows(const RowArgs& a, const Lj1gP& P, int blk) {
+  ROW_PROLOGUE_AT(T, blk)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, c6 = P.c6, c12 = P.c12, c6t6 = P.c6t6, c12t12 = P.c12t12;
   double fx = 0, fy = 0, fz = 0, e = 0;
   walk_row_tag<T, R>(G, nbrow, n, t, i, [&](const typename Geo<R>::P& q, unsigned tag, bool valid) {
@@ -316,6 +322,11 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_lj1g(RowArgs a, Lj1gP P, int nb_
     write_force(a, i, fx, fy, fz);
     if (E) a.e_row[row] = e;
   }
+}
+
+template <int T, class R, bool E>
+__global__ void __launch_bounds__(PAIR_BLOCK) k_lj1g(RowArgs a, Lj1gP P, int nb_real) {
+  lj1g_rows<T, R, E>(a, P, xcd_block(nb_real));
 }
 
 // -------------------------------------------------------------------- TB
@@ -795,6 +806,48 @@ __global__ void __launch_bounds__(256) k_cos_nn_metal(RowArgs an, RowArgs am, Co
   else cos_metal_rows<T, R>(am, P, gnorm, gnf, mrows, lb - nb_nn);
 }
 
+// ------------------------------------------------ passes sharing a launch
+//
+// lj, lj1g and rjl passes whose rows write disjoint atoms (the species of a
+// gas, the replicas of a batch that hold other materials) run as sub-ranges
+// of one launch: a binary gas's lj1g A and B rows and a batch's rjl
+// densities together, then the lj rows of both directions and the rjl
+// forces.  Block b belongs to the last sub-range starting at or before it;
+// each sub-range is XCD-remapped on its own.
+
+#define MULTI_MAXP 6
+enum MultiKind { MK_LJ1G = 0, MK_LJ = 1, MK_RJL_DENSITY = 2, MK_RJL_FORCE = 3 };
+struct MultiPass {
+  int kind, start, np, nb;  // first block of the sub-range, its padded and real block counts
+  RowArgs a;
+  union {
+    RjlP rjl;
+    LjP lj;
+    Lj1gP lj1g;
+  } p;
+  double* eb;
+};
+struct MultiArgs {
+  int n;
+  MultiPass s[MULTI_MAXP];
+};
+
+template <int T, class R, bool E>
+__global__ void __launch_bounds__(256) k_multi(MultiArgs A) {
+  const int b = blockIdx.x;
+  int k = 0;
+  while (k + 1 < A.n && b >= A.s[k + 1].start) ++k;
+  const MultiPass& S = A.s[k];
+  const int lb = xcd_local(b - S.start, S.np, S.nb);
+  if (lb < 0) return;
+  switch (S.kind) {
+    case MK_LJ1G: lj1g_rows<T, R, E>(S.a, S.p.lj1g, lb); break;
+    case MK_LJ: lj_rows<T, R, E>(S.a, S.p.lj, lb); break;
+    case MK_RJL_DENSITY: rjl_density_rows<T, R>(S.a, S.p.rjl, S.eb, lb); break;
+    default: rjl_force_rows<T, R, E>(S.a, S.p.rjl, S.eb, lb); break;
+  }
+}
+
 // ------------------------------------------------- graphene on a metal
 //
 // With rjl on the metal, tb on the graphene and ljc/morsec between them (no
@@ -914,6 +967,115 @@ static CosP cos_params(const Inter& it) {
               4.0 * it.p[0] * it.p[2], 0};
 }
 
+static LjP lj_params(const Inter& it) {
+  return LjP{4.0 * it.p[0], it.p[1], it.p[2], it.p[3], 1.0 / (it.p[3] - it.p[2])};
+}
+static Lj1gP lj1g_params(const Inter& it) {
+  double eps = it.p[0], sig = it.p[1], s6 = sig * sig * sig * sig * sig * sig;
+  return Lj1gP{4.0 * eps * s6, 4.0 * eps * s6 * s6, 6.0 * 4.0 * eps * s6, 12.0 * 4.0 * eps * s6 * s6,
+               it.p[2], it.p[3], 1.0 / (it.p[3] - it.p[2])};
+}
+
+// Phases of lj / lj1g / rjl passes: each pass goes into the earliest phase
+// after its prerequisite (an rjl force after its density) holding no pass
+// that writes any of its atoms (forces, or the density's per-atom value).
+// Worth it when fewer launches result; other interactions keep separate
+// launches (plan not taken).
+static void plan_multi(pfm_ctx* h) {
+  auto& M = h->mp;
+  M.planned = true;
+  M.ok = false;
+  M.phases.clear();
+  struct P {
+    pfm_ctx::MultiPlan::Pass pass;
+    int group, after;
+  };
+  std::vector<P> passes;
+  for (size_t ii = 0; ii < h->inter.size(); ++ii) {
+    const Inter& it = h->inter[ii];
+    const int I = (int)ii;
+    if (it.kind == PFM_LJ1G) {
+      passes.push_back({{I, MK_LJ1G, 0}, it.nl[0].g1, -1});
+    } else if (it.kind == PFM_LJ) {
+      for (int k = 0; k < 2; ++k) passes.push_back({{I, MK_LJ, k}, it.nl[k].g1, -1});
+    } else if (it.kind == PFM_RJL) {
+      passes.push_back({{I, MK_RJL_DENSITY, 0}, it.nl[0].g1, -1});
+      passes.push_back({{I, MK_RJL_FORCE, 0}, it.nl[0].g1, (int)passes.size() - 1});
+    } else {
+      return;
+    }
+  }
+  if (passes.size() > MULTI_MAXP) return;
+  // groups sharing an atom
+  auto overlap = [&](int g1, int g2) {
+    if (g1 == g2) return true;
+    std::vector<char> in(h->N, 0);
+    for (int x : h->grp.idx[g1]) in[x] = 1;
+    for (int x : h->grp.idx[g2])
+      if (in[x]) return true;
+    return false;
+  };
+  std::vector<int> phase_of(passes.size(), 0);
+  int nphase = 0;
+  for (size_t a = 0; a < passes.size(); ++a) {
+    int ph = passes[a].after >= 0 ? phase_of[passes[a].after] + 1 : 0;
+    for (;; ++ph) {
+      bool clash = false;
+      for (size_t b = 0; b < a && !clash; ++b) {
+        if (phase_of[b] != ph) continue;
+        // a density and a force pass of one atom set write different things,
+        // but the force pass of another interaction would read the
+        // density's per-atom value while it is written: keep them apart too
+        clash = overlap(passes[a].group, passes[b].group);
+      }
+      if (!clash) break;
+    }
+    phase_of[a] = ph;
+    nphase = std::max(nphase, ph + 1);
+  }
+  if (nphase >= (int)passes.size()) return;
+  M.phases.assign(nphase, {});
+  for (size_t a = 0; a < passes.size(); ++a) M.phases[phase_of[a]].push_back(passes[a].pass);
+  M.ok = true;
+}
+
+// the phases of h->mp, one launch each; every pass adds to forces the kick
+// cleared.  Returns 1 when the lane count has no variant here.
+template <class R>
+static int multi_forces(pfm_ctx* h, bool energy) {
+  const int T = h->lanes;
+  if (T != 4 && T != 8 && T != 16) return 1;
+  const int BS = 256;
+  for (const auto& ph : h->mp.phases) {
+    MultiArgs A;
+    memset(&A, 0, sizeof A);
+    int start = 0;
+    for (const auto& ps : ph) {
+      Inter& it = h->inter[ps.inter];
+      MultiPass& S = A.s[A.n];
+      S.kind = ps.kind;
+      S.a = row_args(h, it, ps.list, 1);
+      if (S.a.nrows == 0) continue;
+      if (ps.kind == MK_LJ && ps.list == 1) S.a.e_row = nullptr;  // nl(2) rows carry no energy
+      if (ps.kind == MK_LJ1G) S.p.lj1g = lj1g_params(it);
+      else if (ps.kind == MK_LJ) S.p.lj = lj_params(it);
+      else S.p.rjl = rjl_params(it);
+      S.eb = h->eb;
+      S.nb = row_blocks(S.a.nrows, T, BS);
+      S.np = pad8(S.nb);
+      S.start = start;
+      start += S.np;
+      ++A.n;
+    }
+    if (A.n == 0) continue;
+    DISPATCH_T3(T, {
+      if (energy) PFM_LAUNCH(h, "multi", (k_multi<T, R, true>), start, BS, A);
+      else PFM_LAUNCH(h, "multi", (k_multi<T, R, false>), start, BS, A);
+    });
+  }
+  return PFM_OK;
+}
+
 // The three co-scheduled launches of graphene on a metal (h->gm); every
 // pass adds to forces the step's kick cleared.  Returns 1 when the lane
 // count has no fused variant (the caller launches the passes one by one).
@@ -975,6 +1137,11 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
   bool written = !first_writes;  // has any pass written this step's forces yet
   if (h->fuse && h->gm.ok && written) {
     const int rc = gm_forces<R>(h, energy);
+    if (rc != 1) return rc;
+  }
+  if (!h->mp.planned) plan_multi(h);
+  if (h->fuse && h->mp.ok && written) {
+    const int rc = multi_forces<R>(h, energy);
     if (rc != 1) return rc;
   }
   const int BS = std::min(h->block, PairBlock<R>::max);  // row passes (T lanes per row)

@@ -508,6 +508,15 @@ struct pfm_ctx {
     int rjl = -1, tb = -1, cos = -1;
   } gm;
   bool fuse = true;  // pfm_set_fusion
+  // lj / lj1g / rjl passes grouped into launches (phases) whose passes write
+  // disjoint atoms; planned at the first force evaluation
+  struct MultiPlan {
+    bool planned = false, ok = false;
+    struct Pass {
+      int inter, kind, list;  // kind: 0 lj1g, 1 lj rows of list `list`, 2 rjl density, 3 rjl force
+    };
+    std::vector<std::vector<Pass>> phases;
+  } mp;
   // captured steady-state blocks of steps (pfm_run)
   bool use_graphs = true;
   int step_base = 0;  // first step of the block being captured (kernel step arguments are relative to it)

@@ -1,5 +1,6 @@
-"""Co-scheduled passes of graphene on a metal (pfm_set_fusion) against the
-same passes launched one by one, and against the CPU checker.
+"""Co-scheduled passes (pfm_set_fusion) against the same passes launched one
+by one, and against the CPU checker: graphene on a metal, and lj / lj1g /
+rjl passes over disjoint atoms (a binary gas, a batch of replicas).
 
 The fused launches run the same row arithmetic; only the order of the FP64
 force sums of a graphene atom differs (its graphene-pass, tb and
@@ -81,3 +82,62 @@ def test_fused_against_checker():
     pg, vg, fg = g.state()
     po, vo, fo = o.state()
     assert rel_inf(pg, po) < 1e-10 and rel_inf(vg, vo) < 1e-8 and rel_inf(fg, fo) < 1e-8
+
+
+# ------------------------------------------ lj / lj1g / rjl passes sharing launches
+
+def run_plain(ctx_fn, integ, fuse, lanes, steps=20, dt=0.5):
+    c = ctx_fn()
+    c.set_fusion(fuse)
+    c.set_tuning(lanes, 256)
+    c.run(integ, dt, 0, 0, True)
+    f0, e0 = c.state()[2], c.energies()["e_inter"]
+    c.run(integ, dt, 1, steps, True)
+    return f0, e0, c.state(), c.energies()["e_inter"]
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_gas_passes_sharing_launches_fp64(lanes):
+    """Binary gas: lj1g A | lj1g B, then lj A->B | lj B->A."""
+    s = configs.binary_lj_gas(n=600, box=32.0, seed=73)
+    a = run_plain(lambda: gpu_ctx(s, capi.FP64), "nvt", True, lanes)
+    b = run_plain(lambda: gpu_ctx(s, capi.FP64), "nvt", False, lanes)
+    assert rel_inf(a[0], b[0]) < 1e-13
+    np.testing.assert_allclose(a[1], b[1], rtol=1e-13)
+    assert rel_inf(a[2][0], b[2][0]) < 1e-12
+    np.testing.assert_allclose(a[3], b[3], rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision,tol", [(capi.FP64, 1e-13), (capi.MIXED, 1e-7)])
+def test_batch_passes_sharing_launches(precision, tol):
+    """A batch of LJ gases and Cu crystals: lj1g A | lj1g B | rjl densities,
+    then lj A->B | lj B->A | rjl forces; per-replica energies too."""
+    from pfmds_amd import system
+    from tests.test_gpu_batch import replicas
+    b = system.merge_replicas(replicas())
+    out = []
+    for fuse in (True, False):
+        c = system.build_batch_context(b, precision)
+        c.set_fusion(fuse)
+        c.set_tuning(8, 256)
+        c.run("nve", 0.5, 0, 0, True)
+        f0 = c.state()[2]
+        c.run("nve", 0.5, 1, 20, True)
+        out.append((f0, c.state(), c.replica_energies()))
+    (fa, sa, ra), (fb, sb, rb) = out
+    assert rel_inf(fa, fb) < tol
+    assert rel_inf(sa[0], sb[0]) < tol
+    np.testing.assert_allclose(ra["pe"], rb["pe"], rtol=max(tol, 1e-12))
+
+
+def test_gas_sharing_launches_against_checker():
+    s = configs.binary_lj_gas(n=600, box=32.0, seed=74)
+    g, o = gpu_ctx(s), oracle_ctx(s)
+    g.set_fusion(True)
+    for c in (g, o):
+        c.run("nvt", 0.5, 0, 0, True)
+        c.run("nvt", 0.5, 1, 25, True)
+    pg, vg, fg = g.state()
+    po, vo, fo = o.state()
+    assert rel_inf(pg, po) < 1e-10 and rel_inf(vg, vo) < 1e-8 and rel_inf(fg, fo) < 1e-8
+    np.testing.assert_allclose(g.energies()["e_inter"], o.energies()["e_inter"], rtol=1e-9)
