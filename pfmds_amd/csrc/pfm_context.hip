@@ -137,9 +137,9 @@ __global__ void k_kick2(int N, double4* __restrict__ vel, const double4* __restr
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   uint32_t m = mask[i];
+  double4 v = vel[i], f = frc[i];  // loaded beside the mask, not after it
   bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
   if (!(bx || bz)) return;
-  double4 v = vel[i], f = frc[i];
   double k = 1.0 / v.w / PFM_MASS_COEF;
   if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
   if (bz) v.z = v.z + f.z * k * P.ts1;
@@ -510,8 +510,8 @@ __global__ void __launch_bounds__(RED_THREADS) k_kick2_ke(int N, double4* __rest
     uint32_t m = mask[i];
     bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
     double4 v = vel[i];
+    const double4 f = frc[i];  // not waiting on the mask
     if (bx || bz) {
-      double4 f = frc[i];
       double k = 1.0 / v.w / PFM_MASS_COEF;
       if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }
       if (bz) v.z = v.z + f.z * k * P.ts1;

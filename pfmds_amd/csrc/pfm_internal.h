@@ -312,9 +312,11 @@ __device__ __forceinline__ void kick_drift_atom(int i, double4 p, uint32_t m, co
     const RepDev& R = rdev[rep[i]];
     Lx = R.L.x; Ly = R.L.y; Lz = R.L.z; zlo = R.zlo; zhi = R.zhi; qs = R.qs;
   }
+  // the force is read whether or not the atom moves, so its load does not
+  // wait on the mask's
   double4 v = vel[i];
+  const double4 f = frc[i];
   bool bx = (m & P.bxyz) != 0, bz = (m & P.bz) != 0;
-  double4 f = (bx || bz) ? frc[i] : make_double4(0, 0, 0, 0);
   if (P.pre_kick && (bx || bz)) {
     double k = 1.0 / v.w / PFM_MASS_COEF;
     if (bx) { v.x = v.x + f.x * k * P.ts1; v.y = v.y + f.y * k * P.ts1; v.z = v.z + f.z * k * P.ts1; }

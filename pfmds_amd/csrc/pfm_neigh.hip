@@ -63,8 +63,8 @@ __global__ void k_cell_keys(int N, const double4* __restrict__ pos, const int* _
 
 // exclusive scan of n ints by one workgroup of 1024 threads (small n: the
 // cell counts, and the block sums of scan_excl below)
-__global__ void __launch_bounds__(1024) k_scan_excl(int n, const int* __restrict__ in, int* __restrict__ out,
-                                                    int* __restrict__ total) {
+__device__ __forceinline__ void scan_excl_wg(int n, const int* __restrict__ in, int* __restrict__ out,
+                                             int* __restrict__ total) {
   __shared__ int s[1024];
   int t = threadIdx.x;
   int chunk = (n + 1023) / 1024;
@@ -86,6 +86,11 @@ __global__ void __launch_bounds__(1024) k_scan_excl(int n, const int* __restrict
     run += v;
   }
   if (t == 1023 && total) *total = s[1023];
+}
+
+__global__ void __launch_bounds__(1024) k_scan_excl(int n, const int* __restrict__ in, int* __restrict__ out,
+                                                    int* __restrict__ total) {
+  scan_excl_wg(n, in, out, total);
 }
 
 #define SCAN_B 1024  // items per workgroup of the multi-block scan (256 threads x 4)
@@ -226,20 +231,64 @@ int launch_sort_and_bin(pfm_ctx* h, int step, bool permute) {
 }
 
 // --------------------------------------------------------------- group rows
+//
+// The rows of up to GR_MAX groups in three launches whatever their number:
+// per 256-atom block and group the member count (wave ballots), one
+// workgroup per group scanning its block counts, then every member's row
+// from its block's offset, its wave's offset and its rank in the wave.  Rows
+// come out in internal order, as a stable compaction gives them.
 
-__global__ void k_flag(int N, const uint32_t* __restrict__ mask, uint32_t bit, int* __restrict__ flag) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < N) flag[i] = (mask[i] & bit) ? 1 : 0;
+#define GR_MAX 8
+struct GroupRowsArgs {
+  int ng, nb;
+  uint32_t bit[GR_MAX];
+  int* rows[GR_MAX];
+  int* row_of[GR_MAX];
+};
+
+__global__ void __launch_bounds__(256) k_grp_count(int N, const uint32_t* __restrict__ mask, GroupRowsArgs G,
+                                                   int* __restrict__ cnt) {
+  __shared__ int wc[GR_MAX][4];
+  const int i = blockIdx.x * 256 + threadIdx.x, w = threadIdx.x >> 6;
+  const uint32_t m = i < N ? mask[i] : 0u;
+  for (int q = 0; q < G.ng; ++q) {
+    const unsigned long long b = __ballot((m & G.bit[q]) != 0);
+    if ((threadIdx.x & 63) == 0) wc[q][w] = __popcll(b);
+  }
+  __syncthreads();
+  if (threadIdx.x < G.ng) {
+    const int q = threadIdx.x;
+    cnt[q * G.nb + blockIdx.x] = wc[q][0] + wc[q][1] + wc[q][2] + wc[q][3];
+  }
 }
-__global__ void k_compact(int N, const uint32_t* __restrict__ mask, uint32_t bit, const int* __restrict__ pre,
-                          int* __restrict__ rows, int* __restrict__ row_of) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+
+__global__ void __launch_bounds__(1024) k_grp_scan(int nb, int* __restrict__ cnt, int* __restrict__ off) {
+  const int q = blockIdx.x;
+  scan_excl_wg(nb, cnt + (size_t)q * nb, off + (size_t)q * nb, nullptr);
+}
+
+__global__ void __launch_bounds__(256) k_grp_write(int N, const uint32_t* __restrict__ mask, GroupRowsArgs G,
+                                                   const int* __restrict__ off) {
+  __shared__ int wc[GR_MAX][4];
+  const int i = blockIdx.x * 256 + threadIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t m = i < N ? mask[i] : 0u;
+  unsigned long long bal[GR_MAX];
+  for (int q = 0; q < G.ng; ++q) {
+    bal[q] = __ballot((m & G.bit[q]) != 0);
+    if (lane == 0) wc[q][w] = __popcll(bal[q]);
+  }
+  __syncthreads();
   if (i >= N) return;
-  if (mask[i] & bit) {
-    rows[pre[i]] = i;
-    row_of[i] = pre[i];
-  } else {
-    row_of[i] = -1;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int q = 0; q < G.ng; ++q) {
+    if (m & G.bit[q]) {
+      int r = off[q * G.nb + blockIdx.x] + __popcll(bal[q] & below);
+      for (int v = 0; v < w; ++v) r += wc[q][v];
+      G.rows[q][r] = i;
+      G.row_of[q][i] = r;
+    } else {
+      G.row_of[q][i] = -1;
+    }
   }
 }
 
@@ -251,6 +300,17 @@ __global__ void k_identity_rows(int N, int* __restrict__ rows, int* __restrict__
 // rows of every group that some list or role uses: members in internal order
 void launch_group_rows(pfm_ctx* h) {
   int N = h->N, nb = (N + 255) / 256;
+  GroupRowsArgs G{};
+  G.nb = nb;
+  auto flush = [&]() {
+    if (!G.ng) return;
+    int* cnt = h->scan_tmp;
+    int* off = h->scan_tmp + (size_t)GR_MAX * nb;
+    PFM_LAUNCH(h, "group_rows", k_grp_count, nb, 256, N, h->mask, G, cnt);
+    PFM_LAUNCH(h, "group_rows", k_grp_scan, G.ng, 1024, nb, cnt, off);
+    PFM_LAUNCH(h, "group_rows", k_grp_write, nb, 256, N, h->mask, G, off);
+    G.ng = 0;
+  };
   for (int g = 1; g <= PFM_MAXG; ++g) {
     if (!h->grp.used[g] || h->grp.size[g] == 0) continue;
     // members (a growing group's spare bit), not only the active atoms
@@ -261,11 +321,12 @@ void launch_group_rows(pfm_ctx* h) {
       h->grp.ident[g] = true;
       continue;
     }
-    PFM_LAUNCH(h, "group_rows", k_flag, nb, 256, N, h->mask, bit, h->scan_tmp);
-    scan_excl(h, N, h->scan_tmp, h->scan_tmp + N, h->scan_tmp + 2 * (size_t)N);
-    PFM_LAUNCH(h, "group_rows", k_compact, nb, 256, N, h->mask, bit, h->scan_tmp + N, h->grp.rows[g],
-               h->grp.row_of[g]);
+    G.bit[G.ng] = bit;
+    G.rows[G.ng] = h->grp.rows[g];
+    G.row_of[G.ng] = h->grp.row_of[g];
+    if (++G.ng == GR_MAX) flush();
   }
+  flush();
 }
 
 // ------------------------------------------------------------ list building
