@@ -473,6 +473,7 @@ struct pfm_ctx {
   int nc[3] = {1, 1, 1}, ncell = 1;
   double cell_len[3];
   int *cell_rank = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr, *atom_rank = nullptr;
+  uint32_t* cell_grp = nullptr;  // inside the cell_count allocation
   int* cell_xyz = nullptr;  // by Morton rank: x | y<<10 | z<<20
   int* scan_tmp = nullptr;
   // RGL per-atom densities

@@ -159,25 +159,33 @@ __global__ void k_scatter(int N, const int* __restrict__ atom_rank, const int* _
 
 // order the atoms of each cell by original index, so the layout (and every
 // sum over it) does not depend on the order of the scatter's atomics: one
-// 64-lane workgroup per cell, each lane ranks its entries against the cell
+// 64-lane workgroup per cell, each lane ranks its entries against the cell.
+// Also the union of the cell's group bits (cell_grp), which lets a list
+// search skip tiles that hold no candidate.
 __global__ void __launch_bounds__(64) k_cell_sort(int ncell, const int* __restrict__ cell_start,
                                                   const int* __restrict__ cell_count, const int* __restrict__ perm_in,
-                                                  int* __restrict__ perm_out, const int* __restrict__ orig) {
+                                                  int* __restrict__ perm_out, const int* __restrict__ orig,
+                                                  const uint32_t* __restrict__ mask, uint32_t* __restrict__ cell_grp) {
   __shared__ int key[512];
   int c = blockIdx.x;
   if (c >= ncell) return;
   int s = cell_start[c], n = cell_count[c];
   bool lds = n <= 512;
+  uint32_t g = 0u;
   if (lds)
     for (int k = threadIdx.x; k < n; k += 64) key[k] = orig[perm_in[s + k]];
   __syncthreads();
   for (int k = threadIdx.x; k < n; k += 64) {
     int v = perm_in[s + k];
+    g |= mask[v];
     int kk = lds ? key[k] : orig[v];
     int rank = 0;
     for (int m = 0; m < n; ++m) rank += ((lds ? key[m] : orig[perm_in[s + m]]) < kk) ? 1 : 0;
     perm_out[s + rank] = v;
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) g |= (uint32_t)__shfl_xor((int)g, o);
+  if (threadIdx.x == 0) cell_grp[c] = g;
 }
 
 __global__ void k_permute(int N, const int* __restrict__ perm, const double4* __restrict__ p0,
@@ -210,7 +218,7 @@ int launch_sort_and_bin(pfm_ctx* h, int step, bool permute) {
   scan_excl(h, h->ncell, h->cell_count, h->cell_start, h->scan_tmp);
   PFM_LAUNCH(h, "scatter", k_scatter, nb, 256, N, h->atom_rank, h->cell_start, h->cell_fill, h->scan_tmp + N);
   PFM_LAUNCH(h, "cell_sort", k_cell_sort, h->ncell, 64, h->ncell, h->cell_start, h->cell_count, h->scan_tmp + N,
-             h->perm, h->orig);
+             h->perm, h->orig, h->mask, h->cell_grp);
   if (!permute) {
     // lists that are not due keep their internal indices: the search reads
     // the atoms of every cell through the cell-ordered index
@@ -347,6 +355,7 @@ struct SearchArgs {
   const int* cell_rank;
   const int* cell_start;
   const int* cell_count;
+  const uint32_t* cell_grp;  // union of the group bits of each cell's atoms
   int nmax, stride;
   int* nnum;
   int* nbr;
@@ -566,6 +575,15 @@ __global__ void k_nn_from_tb(int nrows, const int* __restrict__ rows, const int*
 // lies within a relative 3e-5 of r_cut**2, where the reference's FP64
 // find_distance test (min_image_dist2) on the global positions decides.
 // Tiles larger than the LDS buffer are staged in several passes.
+// the row of atom i (if it has one) when its tile holds no candidate: what
+// the search writes for a row with no entries
+__device__ __forceinline__ void empty_row(const SearchArgs& a, const int* __restrict__ row_of, int i) {
+  const int row = row_of[i];
+  if (row < 0) return;
+  a.nnum[row] = 0;
+  if ((a.mask[i] & a.g1bit) && a.exact && a.nmax != 0) raise_err(a.err, PFM_E_GRAPHENE_NN, a.orig[i] + 1, 0, a.step);
+}
+
 #define TILE_MAX 1536      // staged candidates per pass (24 KB of LDS), a multiple of 32 * T
 #define TILE_FAR 1.0e30f   // staged coordinate of the padding
 
@@ -598,6 +616,7 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
   const int cc[3] = {pk & 1023, (pk >> 10) & 1023, (pk >> 20) & 1023};
   const double org[3] = {cc[0] * RG.clen.x, cc[1] * RG.clen.y, cc[2] * RG.clen.z};
   const double3 boxL = RG.L, hb = RG.hb;
+  bool cand = false;
   if (tid < 27) {
     int o[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
     int q[3];
@@ -609,8 +628,13 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
     int r = a.cell_rank[RG.cell_base + (q[2] * RG.nc[1] + q[1]) * RG.nc[0] + q[0]];
     tc.start[tid] = a.cell_start[r];
     tc.cnt[tid] = a.cell_count[r];
+    cand = (a.cell_grp[r] & a.g2bit) != 0;
   }
-  __syncthreads();
+  if (!__syncthreads_or(cand)) {
+    // no group-2 atom in the tile: every row of the cell is empty
+    for (int m = tid; m < ncen; m += 256) empty_row(a, row_of, a.bin ? a.bin[cs + m] : cs + m);
+    return;
+  }
   if (tid == 0) {
     int off = 0;
     for (int k = 0; k < 27; ++k) { tc.off[k] = off; off += tc.cnt[k]; }
@@ -824,6 +848,7 @@ __global__ void __launch_bounds__(256) k_build_tile_wave(SearchArgs a, const int
   const double3 boxL = RG.L, hb = RG.hb;
   // the 27 tile cells in lanes 0..26: first atom, count, exclusive offset
   int kst = 0, kcn = 0;
+  bool cand = false;
   if (lane < 27) {
     const int o[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
     int q[3];
@@ -835,6 +860,11 @@ __global__ void __launch_bounds__(256) k_build_tile_wave(SearchArgs a, const int
     const int r = a.cell_rank[RG.cell_base + (q[2] * RG.nc[1] + q[1]) * RG.nc[0] + q[0]];
     kst = a.cell_start[r];
     kcn = a.cell_count[r];
+    cand = (a.cell_grp[r] & a.g2bit) != 0;
+  }
+  if (!__any(cand)) {
+    for (int m = lane; m < ncen; m += 64) empty_row(a, row_of, a.bin ? a.bin[cs + m] : cs + m);
+    return;
   }
   int inc = kcn;
 #pragma unroll
@@ -1046,6 +1076,7 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   a.cell_rank = h->cell_rank;
   a.cell_start = h->cell_start;
   a.cell_count = h->cell_count;
+  a.cell_grp = h->cell_grp;
   a.nmax = L.nmax;
   a.stride = L.stride;
   a.nnum = L.nnum;
