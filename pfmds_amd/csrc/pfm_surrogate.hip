@@ -184,21 +184,36 @@ __device__ __forceinline__ float sur_dr_rec(const Geo<R>& Gm, const typename Geo
   return x * x + y * y + z * z;
 }
 
-// The row walks gather the next entry's record (index, position, group
-// bits) before the arithmetic on the current one.
-#define SUR_WALK_BEGIN                                                        \
-  int jn = eg < n ? nbrow[eg] : i;                                            \
-  typename Geo<R>::P pn = Gm.at(jn);                                          \
-  uint32_t mn = mask[jn];                                                     \
-  for (int p = eg; p < n; p += 4) {                                           \
-    const int j = jn;                                                         \
-    const typename Geo<R>::P pj = pn;                                         \
-    const uint32_t mj = mn;                                                   \
-    if (p + 4 < n) {                                                          \
-      jn = nbrow[p + 4];                                                      \
-      pn = Gm.at(jn);                                                         \
-      mn = mask[jn];                                                          \
-    }
+// The row walks take the row in blocks of four consecutive entries: the lanes
+// of entry group eg take blocks eg, eg + 4, ...; a block costs one 16-byte
+// index load (rows start on 16 bytes and are padded past their end), then
+// the four records, group bits and AUXF values are gathered before the
+// arithmetic on the first, so a block waits on two memory latencies rather
+// than every entry on one.  Entries past the row's end are skipped.
+#define SUR_NOAUX(j) 0
+#define SUR_WALK_BEGIN(AUXT, AUXF)                                                                    \
+  for (int p0 = 4 * eg; p0 < n; p0 += 16) {                                                           \
+    const int4 v4_ = *reinterpret_cast<const int4*>(nbrow + p0);                                      \
+    const int jj_[4] = {v4_.x, p0 + 1 < n ? v4_.y : i, p0 + 2 < n ? v4_.z : i, p0 + 3 < n ? v4_.w : i}; \
+    typename Geo<R>::P pp_[4];                                                                        \
+    uint32_t mm_[4];                                                                                  \
+    AUXT ax_[4];                                                                                      \
+    _Pragma("unroll") for (int u_ = 0; u_ < 4; ++u_) {                                                \
+      pp_[u_] = Gm.at(jj_[u_]);                                                                       \
+      mm_[u_] = mask[jj_[u_]];                                                                        \
+      ax_[u_] = AUXF(jj_[u_]);                                                                        \
+    }                                                                                                 \
+    _Pragma("unroll") for (int u_ = 0; u_ < 4; ++u_) {                                                \
+      const int p = p0 + u_;                                                                          \
+      if (p >= n) continue;                                                                           \
+      const int j = jj_[u_];                                                                          \
+      const typename Geo<R>::P pj = pp_[u_];                                                          \
+      const uint32_t mj = mm_[u_];                                                                    \
+      const AUXT aj = ax_[u_];                                                                        \
+      (void)j; (void)aj;
+#define SUR_WALK_END \
+  }                  \
+  }
 
 __device__ __forceinline__ void sur_cut(const SurK& K, float r, float& fc, float& fcp) {
   float s, c;
@@ -232,8 +247,7 @@ __global__ void __launch_bounds__(256) k_sur_desc(RowArgsS a, SurK K, const uint
   for (int t = 0; t < NMON; ++t) { mc[t] = 0.0f; mx[t] = 0.0f; }
 #pragma unroll
   for (int k = 0; k < 6; ++k) { gc[k] = 0.0f; gx[k] = 0.0f; }
-  SUR_WALK_BEGIN
-    (void)j;
+  SUR_WALK_BEGIN(int, SUR_NOAUX)
     float dx, dy, dz;
     float r2 = sur_dr_rec<R>(Gm, pi, pj, dx, dy, dz);
     if (!(r2 < K.rc2)) continue;
@@ -258,7 +272,7 @@ __global__ void __launch_bounds__(256) k_sur_desc(RowArgsS a, SurK K, const uint
       gc[k] += cj ? v : 0.0f;
       gx[k] += cj ? 0.0f : v;
     }
-  }
+  SUR_WALK_END
   sum_groups<NMON>(mc);
   sum_groups<NMON>(mx);
   sum_groups<1>(dd);
@@ -396,8 +410,7 @@ __global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint
   float4* drow = K.d + (size_t)row * K.stride0;
   double sf[3] = {0.0, 0.0, 0.0};
   // the two eta lanes of an entry group walk the same entries
-  SUR_WALK_BEGIN
-    (void)j;
+  SUR_WALK_BEGIN(int, SUR_NOAUX)
     float dx, dy, dz;
     float r2 = sur_dr_rec<R>(Gm, pi, pj, dx, dy, dz);
     float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
@@ -442,7 +455,7 @@ __global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint
     c2 += __shfl_xor(c2, 1, 8);
     if (ke == 0) drow[p] = make_float4(c0, c1, c2, 0.0f);
     sf[0] += c0; sf[1] += c1; sf[2] += c2;
-  }
+  SUR_WALK_END
   sum_groups<3>(sf);
   if (q == 0) K.fself[row] = make_double4(sf[0], sf[1], sf[2], 0.0);
 }
@@ -567,7 +580,8 @@ __global__ void __launch_bounds__(256) k_sur_tangent(RowArgsS a, SurK K, const u
 #pragma unroll
   for (int k = 0; k < 6; ++k) { dgc[k] = 0.0f; dgx[k] = 0.0f; }
   const float4 ri = K.res[i];
-  SUR_WALK_BEGIN
+#define SUR_RES(j) K.res[j]
+  SUR_WALK_BEGIN(float4, SUR_RES)
     float dx, dy, dz;
     float r2 = sur_dr_rec<R>(Gm, pi, pj, dx, dy, dz);
     if (!(r2 < K.rc2)) continue;
@@ -576,7 +590,7 @@ __global__ void __launch_bounds__(256) k_sur_tangent(RowArgsS a, SurK K, const u
     float fc, fcp;
     sur_cut(K, r, fc, fcp);
     float u[3] = {dx * ir, dy * ir, dz * ir};
-    float4 rj = K.res[j];
+    const float4 rj = aj;
     float de[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
     float ud = u[0] * de[0] + u[1] * de[1] + u[2] * de[2];
     float ex = __expf(-eta * r2);
@@ -604,7 +618,8 @@ __global__ void __launch_bounds__(256) k_sur_tangent(RowArgsS a, SurK K, const u
       dgc[k] += cj ? pk : 0.0f;
       dgx[k] += cj ? 0.0f : pk;
     }
-  }
+  SUR_WALK_END
+#undef SUR_RES
   sum_groups<NMON>(dmc);
   sum_groups<NMON>(dmx);
   sum_groups<1>(ddd);
