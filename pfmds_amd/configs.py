@@ -279,7 +279,8 @@ def graphene_rebo(gx=8, gy=5, acc=1.42, lz=20.0, jitter=0.0, buckle=0.0, seed=10
 def lj_gas_lattice(n, density, seed, temperature, dt=0.5, steps=2000, r1=6.5, r2=7.5, skin=1.0, nmax=160,
                    period=20, jitter=0.1):
     """Binary LJ gas (the interactions of binary_lj_gas) of n atoms at
-    `density` (A^-3), on a jittered simple cubic lattice: spacing
+    `density` (A^-3), on a jittered simple cubic lattice whose sites the two
+    species take in random order: spacing
     (1/density)**(1/3) - 2*jitter stays above binary_lj_gas's 3.0 A minimum
     for density <= 0.03, and the start costs O(n)."""
     rng = SplitMix64(seed)
@@ -290,6 +291,9 @@ def lj_gas_lattice(n, density, seed, temperature, dt=0.5, steps=2000, r1=6.5, r2
     pick = np.argsort(rng.uniform(len(g)), kind="stable")[:n]
     pos = (g[np.sort(pick)] + 0.5) * a + jitter * (2.0 * rng.uniform(3 * n).reshape(n, 3) - 1.0)
     pos = np.mod(pos, L)
+    # the lattice sites in a seeded random order: the first n/2 atoms (A)
+    # and the rest (B) are mixed through the box, as in binary_lj_gas
+    pos = pos[np.argsort(rng.uniform(n), kind="stable")]
     species = ["A" if i < n // 2 else "B" for i in range(n)]
     mass = np.where(np.array(species) == "A", 39.95, 83.80)
     vel = maxwell(rng, mass, temperature)
