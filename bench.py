@@ -46,10 +46,14 @@ def gpu_ctx(s, precision):
 
 def tuned(ctx, integrator, dt, precision, first):
     """Launch shape and launch mode from the autotuner, tried on the live
-    handle from step `first` on; returns ("LxB[+graphs]", next step)."""
+    handle from step `first` on; returns ("LxB[-fusion][:iKxL...][+graphs]",
+    next step): lanes x workgroup size, separate launches instead of the
+    co-scheduled ones, interaction K's own lane count L, graph launches."""
     lanes, block, graphs, _, nxt = autotune.tune(ctx, integrator, dt, precision, first, steps=100, repeats=3,
                                                  graphs=GRAPHS)
-    return f"{lanes}x{block}" + ("+graphs" if graphs else ""), nxt
+    shape = f"{lanes}x{block}" + ("" if getattr(ctx, "fused", True) else "-fusion")
+    shape += "".join(f":i{k}x{v}" for k, v in sorted(getattr(ctx, "pass_lanes", {}).items()))
+    return shape + ("+graphs" if graphs else ""), nxt
 
 
 def kernels_only(which, steps=50, warmup=10):

@@ -213,10 +213,20 @@ int pfm_set_tuning(pfm_ctx* h, int lanes_per_row, int block);
  * over disjoint atoms share launches: the rjl density pass with the tb
  * bond-order pass, the rjl force pass with the graphene pass of ljc/morsec,
  * the tb force pass (with the normal-derivative terms) with the metal rows.
+ * When every interaction is lj, lj1g or rjl, passes whose rows share no
+ * atom share launches in phases (a binary gas's lj1g A and B, then its lj
+ * rows of both directions; the rjl passes of a batch beside them).
  * Results equal the separate launches' up to the order of the FP64 force
  * sums.  pfm_set_fusion(h, 0) launches every pass on its own.  No
  * counterpart in the reference (launch schedule). */
 int pfm_set_fusion(pfm_ctx* h, int on);
+
+/* Lanes per row (1,2,4,8,16,32) of the row passes of interaction `inter`
+ * (1-based, settings order) when they are launched on their own; 0 returns
+ * it to the count of pfm_set_tuning.  Co-scheduled launches take the
+ * pfm_set_tuning count for all of their passes.  No counterpart in the
+ * reference (launch-shape control for the autotuner). */
+int pfm_set_pass_lanes(pfm_ctx* h, int inter, int lanes);
 
 /* Captured steps (default on).  pfm_run records a block of steps whose list
  * and zero-momentum schedule repeats (the least common multiple of the list

@@ -7,8 +7,9 @@ and the system size.  `tune` times a short run of the live handle for every
 candidate shape, keeps the fastest, then times the steps launched from
 captured graphs against launched one by one (`pfm_set_graphs`: graphs pay
 off where launches dominate, small systems) and the co-scheduled passes of
-graphene on a metal against separate launches (`pfm_set_fusion`), and leaves
-the faster set;
+graphene on a metal against separate launches (`pfm_set_fusion`); with
+separate launches, the lanes per row of each interaction's own passes
+(`pfm_set_pass_lanes`, one interaction at a time); and leaves the faster set;
 results are cached in a JSON file keyed by a signature of the workload.  No counterpart in the
 reference, which tunes only its OpenMP thread count
 (perfomance_settings.f90:13-25).
@@ -19,6 +20,10 @@ import time
 
 # (lanes per row, workgroup size); FP64 passes are limited to 256 threads
 CANDIDATES = [(4, 256), (8, 128), (8, 256), (8, 512), (16, 256), (16, 512), (32, 256)]
+# lanes per row tried for each interaction's own passes, and the gain a
+# per-interaction count must show over the shared one to be kept
+PASS_LANES = (4, 8, 16)
+PASS_MARGIN = 0.02
 CACHE = os.environ.get("PFMDS_TUNING_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "pfmds_amd", "tuning.json"))
 
@@ -48,8 +53,11 @@ def time_shape(ctx, integrator, dt, first, steps):
 def tune(ctx, integrator, dt, precision, first_step=1, steps=40, repeats=2, candidates=CANDIDATES, graphs=True):
     """Try every candidate shape on the handle, from step `first_step` on
     (the handle must have run step first_step - 1), set the fastest, then the
-    faster of graph and direct launches (direct only with graphs=False), and
-    return (lanes, block, graphs, {setting: seconds}, next step)."""
+    faster of co-scheduled and separate launches, with separate launches the
+    lanes of each interaction's passes, then the faster of graph and direct
+    launches (direct only with graphs=False), and return (lanes, block,
+    graphs, {setting: seconds}, next step); the choices of fusion and pass
+    lanes are left on the handle as ctx.fused and ctx.pass_lanes."""
     table = {}
     step = first_step
 
@@ -77,12 +85,31 @@ def tune(ctx, integrator, dt, precision, first_step=1, steps=40, repeats=2, cand
     table[f"{pick}-fusion"] = best_of()
     fused = table[pick] <= table[f"{pick}-fusion"]
     ctx.set_fusion(fused)
+    pass_lanes = {}
+    n_inter = getattr(ctx, "n_inter", 0)
+    if not fused and n_inter > 1:
+        best = table[f"{pick}-fusion"]
+        for k in range(1, n_inter + 1):
+            keep, keep_t = 0, best
+            for pl in PASS_LANES:
+                if pl == lanes:
+                    continue
+                ctx.set_pass_lanes(k, pl)
+                t = best_of()
+                table[f"{pick}-fusion:i{k}x{pl}"] = t
+                if t < keep_t * (1.0 - PASS_MARGIN):
+                    keep, keep_t = pl, t
+            ctx.set_pass_lanes(k, keep)
+            if keep:
+                pass_lanes[k] = keep
+                best = keep_t
     if graphs:
         ctx.set_graphs(True)
         table[f"{pick}+graphs"] = best_of()
         graphs = table[f"{pick}+graphs"] < table[pick]
     ctx.set_graphs(graphs)
     ctx.fused = fused
+    ctx.pass_lanes = pass_lanes
     return lanes, block, graphs, table, step
 
 
@@ -94,9 +121,10 @@ def load(path=CACHE):
         return {}
 
 
-def save(key, lanes, block, graphs, table, path=CACHE, fusion=True):
+def save(key, lanes, block, graphs, table, path=CACHE, fusion=True, pass_lanes=None):
     data = load(path)
-    data[key] = {"lanes": lanes, "block": block, "graphs": bool(graphs), "fusion": bool(fusion), "seconds": table}
+    data[key] = {"lanes": lanes, "block": block, "graphs": bool(graphs), "fusion": bool(fusion),
+                 "pass_lanes": {str(k): int(v) for k, v in (pass_lanes or {}).items()}, "seconds": table}
     os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
     tmp = path + ".tmp"
     with open(tmp, "w") as f:
@@ -112,6 +140,8 @@ def apply_cached(ctx, key, path=CACHE):
     ctx.set_tuning(int(e["lanes"]), int(e["block"]))
     ctx.set_graphs(bool(e.get("graphs", False)))
     ctx.set_fusion(bool(e.get("fusion", True)))
+    for k, v in e.get("pass_lanes", {}).items():
+        ctx.set_pass_lanes(int(k), int(v))
     return True
 
 
@@ -124,5 +154,6 @@ def autotune(ctx, s, integrator, dt, precision, first_step=1, n_rep=1, path=CACH
         e = load(path)[key]
         return int(e["lanes"]), int(e["block"]), bool(e.get("graphs", False)), "cache", first_step
     lanes, block, graphs, table, step = tune(ctx, integrator, dt, precision, first_step, steps)
-    save(key, lanes, block, graphs, table, path, fusion=getattr(ctx, "fused", True))
+    save(key, lanes, block, graphs, table, path, fusion=getattr(ctx, "fused", True),
+         pass_lanes=getattr(ctx, "pass_lanes", None))
     return lanes, block, graphs, "tuned", step

@@ -98,6 +98,8 @@ def load_library(path, prefix):
         lib.pfm_set_graphs.restype = C.c_int
         lib.pfm_set_fusion.argtypes = [C.c_void_p, C.c_int]
         lib.pfm_set_fusion.restype = C.c_int
+        lib.pfm_set_pass_lanes.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.pfm_set_pass_lanes.restype = C.c_int
         lib.pfm_row_energies.argtypes = [C.c_void_p, C.c_int, _i, _d]
         lib.pfm_row_energies.restype = C.c_int
         lib.pfm_graph_count.argtypes = [C.c_void_p, _P(C.c_long), _P(C.c_long)]
@@ -325,8 +327,13 @@ class Context:
         return out[:n.value]
 
     def set_fusion(self, on):
-        """Co-scheduled passes of graphene on a metal (pfm_set_fusion)."""
+        """Co-scheduled passes over disjoint atoms (pfm_set_fusion)."""
         self._check(self.lib.pfm_set_fusion(self.h, int(bool(on))))
+
+    def set_pass_lanes(self, inter, lanes):
+        """Lanes per row of interaction `inter`'s separately launched passes
+        (1-based; 0: the pfm_set_tuning count) (pfm_set_pass_lanes)."""
+        self._check(self.lib.pfm_set_pass_lanes(self.h, int(inter), int(lanes)))
 
     def set_graphs(self, on):
         self._check(self.lib.pfm_set_graphs(self.h, int(bool(on))))

@@ -1284,6 +1284,7 @@ static std::vector<uint64_t> graph_key(pfm_ctx* h, int integ, double dt, int blk
   }
   k.insert(k.end(), {(uint64_t)integ, dtb, (uint64_t)blk, (uint64_t)first_writes, (uint64_t)zm,
                      (uint64_t)h->invert_z, (uint64_t)h->lanes, (uint64_t)h->block, (uint64_t)h->fuse, em, xm});
+  for (const auto& it : h->inter) k.push_back((uint64_t)it.lanes);
   return k;
 }
 
@@ -2538,6 +2539,14 @@ int pfm_set_tuning(pfm_ctx* h, int lanes, int block) {
 
 int pfm_set_fusion(pfm_ctx* h, int on) {
   h->fuse = on != 0;
+  return PFM_OK;
+}
+
+int pfm_set_pass_lanes(pfm_ctx* h, int inter, int lanes) {
+  if (inter < 1 || inter > (int)h->inter.size()) return fail(h, PFM_E_INVALID, "error: no such interaction");
+  if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32)
+    return fail(h, PFM_E_INVALID, "error: lanes_per_row must be 0,1,2,4,8,16 or 32");
+  h->inter[inter - 1].lanes = lanes;
   return PFM_OK;
 }
 

@@ -934,6 +934,10 @@ static RowArgs row_args(pfm_ctx* h, Inter& it, int k, int accumulate) {
 
 static int row_blocks(int nrows, int T, int block) { return (int)(((long)nrows * T + block - 1) / block); }
 
+// lanes per row of an interaction's separately launched row passes (the
+// co-scheduled launches take the handle's count for all their passes)
+static int pass_lanes(const pfm_ctx* h, const Inter& it) { return it.lanes ? it.lanes : h->lanes; }
+
 #define DISPATCH_T(T_, ...)                       \
   switch (T_) {                                   \
     case 1: { constexpr int T = 1; __VA_ARGS__; } break;   \
@@ -1153,7 +1157,7 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         RowArgs a = row_args(h, it, 0, written ? 1 : 0);
         if (a.nrows == 0) break;
         RjlP P{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6], 1.0 / (it.p[6] - it.p[5])};
-        int T = h->lanes;
+        int T = pass_lanes(h, it);
         DISPATCH_T(T, {
           int nb = row_blocks(a.nrows, T, BS);
           PFM_LAUNCH(h, "rjl_density", (k_rjl_density<T, R>), pad8(nb), BS, a, P, h->eb, nb);
@@ -1169,7 +1173,7 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
           RowArgs a = row_args(h, it, k, written ? 1 : 0);
           if (a.nrows == 0) continue;
           bool E = energy && k == 0;
-          int T = h->lanes;
+          int T = pass_lanes(h, it);
           DISPATCH_T(T, {
             int nb = row_blocks(a.nrows, T, BS);
             if (E) PFM_LAUNCH(h, "lj", (k_lj<T, R, true>), pad8(nb), BS, a, P, nb);
@@ -1185,7 +1189,7 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         double eps = it.p[0], sig = it.p[1], s6 = sig * sig * sig * sig * sig * sig;
         Lj1gP P{4.0 * eps * s6, 4.0 * eps * s6 * s6, 6.0 * 4.0 * eps * s6, 12.0 * 4.0 * eps * s6 * s6,
                 it.p[2], it.p[3], 1.0 / (it.p[3] - it.p[2])};
-        int T = h->lanes;
+        int T = pass_lanes(h, it);
         DISPATCH_T(T, {
           int nb = row_blocks(a.nrows, T, BS);
           if (energy) PFM_LAUNCH(h, "lj1g", (k_lj1g<T, R, true>), pad8(nb), BS, a, P, nb);
@@ -1237,7 +1241,7 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
         }
         ListDev& NN = it.nl[2];
         RowArgs a = row_args(h, it, 0, written ? 1 : 0);
-        int T = h->lanes;
+        int T = pass_lanes(h, it);
         int simp = simplified ? 1 : 0;
         if (a.nrows > 0) {
           // graphene rows hold about 40 metal entries: at most four lanes each

@@ -384,6 +384,7 @@ struct Inter {
   double p[24] = {0};
   ListDev nl[3];
   int nn_from_tb = -1;  // ljc/morsec: index of the tb interaction whose list feeds nl(3)
+  int lanes = 0;        // lanes per row of its separately launched row passes (0: the handle's)
   double energy = 0.0;
   double* e_row = nullptr;  // per-row energies of the last energy step
   int e_rows = 0;

@@ -751,14 +751,18 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
       for (int c0 = 0; c0 < np; c0 += 32 * T) {
         // bit 31-b <-> candidate slot c0 + t + b*T
         const float4* sv = st + c0 + t;
-        unsigned hit = 0u;
+        // two instructions per candidate after the differences: the margin
+        // hi - d2 as a chain of fused multiply-adds, whose sign bit (set for
+        // d2 > hi, and for the far padding's -inf) shifts into the miss mask
+        unsigned miss = 0u;
 #pragma unroll 8
         for (int b = 0; b < 32; ++b) {
           const float4 v = sv[b * T];
           float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
-          float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          hit = hit + hit + (d2 <= hi ? 1u : 0u);
+          const float m = fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, hi)));
+          miss = __builtin_amdgcn_alignbit(miss, __float_as_uint(m), 31);
         }
+        unsigned hit = ~miss;
         // the row atom is not its own neighbour
         const int ob = own - c0 - t;
         if (ob >= 0 && ob < 32 * T && ob % T == 0) hit &= ~(0x80000000u >> (ob / T));
@@ -965,14 +969,15 @@ __global__ void __launch_bounds__(256) k_build_tile_wave(SearchArgs a, const int
       const int own = (!CMP || dense) ? koff13 + qq - s0 : -1;
       for (int c0 = 0; c0 < np; c0 += 32 * T) {
         const float4* sv = st + c0 + t;
-        unsigned hit = 0u;
+        unsigned miss = 0u;  // as in k_build_tile
 #pragma unroll 8
         for (int b = 0; b < 32; ++b) {
           const float4 w = sv[b * T];
           float dx = w.x - px, dy = w.y - py, dz = w.z - pz;
-          float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          hit = hit + hit + (d2 <= hi ? 1u : 0u);
+          const float m = fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, hi)));
+          miss = __builtin_amdgcn_alignbit(miss, __float_as_uint(m), 31);
         }
+        unsigned hit = ~miss;
         const int ob = own - c0 - t;
         if (ob >= 0 && ob < 32 * T && ob % T == 0) hit &= ~(0x80000000u >> (ob / T));
         unsigned chk = hit;

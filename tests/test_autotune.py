@@ -82,3 +82,43 @@ def test_signature_separates_workloads():
     assert autotune.signature(a, capi.MIXED) != autotune.signature(b, capi.MIXED)
     assert autotune.signature(a, capi.MIXED) != autotune.signature(a, capi.FP64)
     assert autotune.signature(a, capi.MIXED) == autotune.signature(configs.fcc_cu_rgl(nx=3), capi.MIXED)
+
+
+class PassCtx(FakeCtx):
+    """Two interactions; interaction 2's own passes run faster at 16 lanes."""
+
+    n_inter = 2
+
+    def __init__(self, cost, **kw):
+        super().__init__(cost, **kw)
+        self.pass_set = {}
+
+    def set_pass_lanes(self, inter, lanes):
+        self.pass_set[inter] = lanes
+
+    def run(self, integ, dt, s0, s1, energy):
+        f = 0.5 if (not self.fusion and self.pass_set.get(2) == 16) else 1.0
+        self.runs.append((s0, s1))
+        t = self.cost[self.shape] * (s1 - s0 + 1) * f
+        t = t / self.fusion_gain if self.fusion else t
+        time.sleep(t / self.graph_gain if self.graphs else t)
+
+
+def test_pass_lanes_with_separate_launches(tmp_path):
+    cost = {c: 1e-3 for c in autotune.CANDIDATES}
+    cost[(8, 256)] = 5e-4
+    ctx = PassCtx(cost, fusion_gain=0.5)
+    s = configs.binary_lj_gas(n=200, box=20.0, seed=5)
+    path = str(tmp_path / "t.json")
+    autotune.autotune(ctx, s, "nve", 1.0, capi.MIXED, path=path, steps=4)
+    assert not ctx.fused and ctx.pass_lanes == {2: 16} and ctx.pass_set == {1: 0, 2: 16}
+    ctx2 = PassCtx(cost)
+    assert autotune.apply_cached(ctx2, autotune.signature(s, capi.MIXED), path)
+    assert ctx2.pass_set == {2: 16} and ctx2.fusion is False and ctx2.runs == []
+
+
+def test_pass_lanes_not_tried_with_shared_launches():
+    cost = {c: 1e-3 for c in autotune.CANDIDATES}
+    ctx = PassCtx(cost, fusion_gain=3.0)
+    autotune.tune(ctx, "nve", 1.0, capi.MIXED, steps=3, repeats=1)
+    assert ctx.fused and ctx.pass_lanes == {} and ctx.pass_set == {}

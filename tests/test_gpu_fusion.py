@@ -141,3 +141,37 @@ def test_gas_sharing_launches_against_checker():
     po, vo, fo = o.state()
     assert rel_inf(pg, po) < 1e-10 and rel_inf(vg, vo) < 1e-8 and rel_inf(fg, fo) < 1e-8
     np.testing.assert_allclose(g.energies()["e_inter"], o.energies()["e_inter"], rtol=1e-9)
+
+
+# ------------------------------------------ lanes per row of one interaction
+
+@pytest.mark.parametrize("precision,tol", [(capi.FP64, 1e-13), (capi.MIXED, 1e-7)])
+def test_pass_lanes_equal_shared_lanes(precision, tol):
+    """Separate launches with interaction 1 (lj1g A) at 4 lanes and 3 (lj)
+    at 16 against every pass at 8; captured blocks, so the block cache must
+    tell the settings apart."""
+    s = configs.binary_lj_gas(n=600, box=32.0, seed=75)
+    out = []
+    for per in ({}, {1: 4, 3: 16}):
+        c = gpu_ctx(s, precision)
+        c.set_fusion(False)
+        c.set_tuning(8, 256)
+        c.set_graphs(True)
+        c.run("nvt", 0.5, 0, 0, True)
+        c.run("nvt", 0.5, 1, 20, True)   # blocks captured at 8 lanes
+        for k, v in per.items():
+            c.set_pass_lanes(k, v)
+        c.run("nvt", 0.5, 21, 60, True)
+        out.append((c.state(), c.energies()["e_inter"]))
+    (sa, ea), (sb, eb) = out
+    assert rel_inf(sa[2], sb[2]) < tol and rel_inf(sa[0], sb[0]) < tol
+    np.testing.assert_allclose(ea, eb, rtol=max(tol, 1e-12))
+
+
+def test_pass_lanes_rejects_bad_values():
+    s = configs.binary_lj_gas(n=200, box=20.0, seed=76)
+    c = gpu_ctx(s)
+    for inter, lanes in ((0, 8), (4, 8), (1, 3), (1, 64)):
+        with pytest.raises(capi.PfmError):
+            c.set_pass_lanes(inter, lanes)
+    c.set_pass_lanes(1, 0)

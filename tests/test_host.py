@@ -37,6 +37,7 @@ def test_oracle_mirrors_the_abi():
         # (pinned against GPU runs stopped at every recorded step) and the
         # learned surrogate (pinned against tests/sur_ref.py)
         if s in ("pfm_set_timing", "pfm_kernel_time", "pfm_set_tuning", "pfm_set_graphs", "pfm_graph_count", "pfm_set_fusion",
+                 "pfm_set_pass_lanes",
                  "pfm_row_energies",
                  "pfm_create_batch", "pfm_replica_energies", "pfm_set_records", "pfm_set_speculation",
                  "pfm_set_chain", "pfm_copy_state", "pfm_force_rmse", "pfm_force_rmse_noalloc") \
