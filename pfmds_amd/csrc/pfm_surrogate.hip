@@ -307,16 +307,45 @@ __global__ void __launch_bounds__(256) k_sur_desc(RowArgsS a, SurK K, const uint
 
 // Perceptron on MFMA: 16 atoms per wavefront.  Writes E_i (e_row) and
 // dE_i/dG_i.
+// The weights and the float parameters are staged in LDS once per
+// workgroup (four wavefronts, 64 atoms): the A operands are then LDS reads
+// instead of four L2 streams of the same 28 KB.  Rows are padded by four
+// BF16 (two banks), so the 16 rows of a 16x16 tile fall in distinct banks.
+#define SUR_LP1 (NFP + 4)  // W1 row pitch in LDS
+#define SUR_LPH (NH + 4)   // W1T, W2, W2T row pitch in LDS
+#define SUR_LW1 0
+#define SUR_LW1T (NH * SUR_LP1)
+#define SUR_LW2 (SUR_LW1T + NFP * SUR_LPH)
+#define SUR_LW2T (SUR_LW2 + NH * SUR_LPH)
+#define SUR_LNW (SUR_LW2T + NH * SUR_LPH)
+
+__device__ __forceinline__ void stage_rows(uint16_t* __restrict__ dst, int pitch, const uint16_t* __restrict__ src,
+                                           int rows, int ld) {
+  const int q = ld / 4;  // 8-byte chunks per row
+  for (int k = threadIdx.x; k < rows * q; k += blockDim.x) {
+    const int r = k / q, c = k - r * q;
+    *reinterpret_cast<v4s*>(dst + r * pitch + 4 * c) = *reinterpret_cast<const v4s*>(src + r * ld + 4 * c);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
+  __shared__ __align__(16) uint16_t sW[SUR_LNW];
+  __shared__ float sfp[SUR_NFP_];
+  stage_rows(sW + SUR_LW1, SUR_LP1, K.W + SUR_OW1, NH, NFP);
+  stage_rows(sW + SUR_LW1T, SUR_LPH, K.W + SUR_OW1T, NFP, NH);
+  stage_rows(sW + SUR_LW2, SUR_LPH, K.W + SUR_OW2, NH, NH);
+  stage_rows(sW + SUR_LW2T, SUR_LPH, K.W + SUR_OW2T, NH, NH);
+  for (int k = threadIdx.x; k < SUR_NFP_; k += blockDim.x) sfp[k] = K.fp[k];
+  __syncthreads();
   const int lane = threadIdx.x & 63, kq = lane >> 4;
   const int grp = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int row = grp * 16 + (lane & 15);
   const bool ok = row < nrows;
-  const float* fp = K.fp;
-  const uint16_t* W1 = K.W + SUR_OW1;
-  const uint16_t* W1T = K.W + SUR_OW1T;
-  const uint16_t* W2 = K.W + SUR_OW2;
-  const uint16_t* W2T = K.W + SUR_OW2T;
+  const float* fp = sfp;
+  const uint16_t* W1 = sW + SUR_LW1;
+  const uint16_t* W1T = sW + SUR_LW1T;
+  const uint16_t* W2 = sW + SUR_LW2;
+  const uint16_t* W2T = sW + SUR_LW2T;
   v4s X[3];
 #pragma unroll
   for (int kk = 0; kk < 3; ++kk) {
@@ -334,7 +363,7 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
   for (int t = 0; t < 4; ++t) {
     v4f z = {0, 0, 0, 0};
 #pragma unroll
-    for (int kk = 0; kk < 3; ++kk) z = mfma16(lda(W1, NFP, 16 * t, 16 * kk, lane), X[kk], z);
+    for (int kk = 0; kk < 3; ++kk) z = mfma16(lda(W1, SUR_LP1, 16 * t, 16 * kk, lane), X[kk], z);
 #pragma unroll
     for (int v = 0; v < 4; ++v) a1[t][v] = tanh_f(z[v] + fp[SUR_B1 + 16 * t + 4 * kq + v]);
     A1[t] = to_bf16(a1[t]);
@@ -345,7 +374,7 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
   for (int t2 = 0; t2 < 4; ++t2) {
     v4f z = {0, 0, 0, 0};
 #pragma unroll
-    for (int t = 0; t < 4; ++t) z = mfma16(lda(W2, NH, 16 * t2, 16 * t, lane), A1[t], z);
+    for (int t = 0; t < 4; ++t) z = mfma16(lda(W2, SUR_LPH, 16 * t2, 16 * t, lane), A1[t], z);
     v4f g;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -364,7 +393,7 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
   for (int t = 0; t < 4; ++t) {
     v4f z = {0, 0, 0, 0};
 #pragma unroll
-    for (int t2 = 0; t2 < 4; ++t2) z = mfma16(lda(W2T, NH, 16 * t, 16 * t2, lane), G2[t2], z);
+    for (int t2 = 0; t2 < 4; ++t2) z = mfma16(lda(W2T, SUR_LPH, 16 * t, 16 * t2, lane), G2[t2], z);
     v4f g;
 #pragma unroll
     for (int v = 0; v < 4; ++v) g[v] = z[v] * (1.0f - a1[t][v] * a1[t][v]);
@@ -374,7 +403,7 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
   for (int tt = 0; tt < 3; ++tt) {
     v4f z = {0, 0, 0, 0};
 #pragma unroll
-    for (int t = 0; t < 4; ++t) z = mfma16(lda(W1T, NH, 16 * tt, 16 * t, lane), G1[t], z);
+    for (int t = 0; t < 4; ++t) z = mfma16(lda(W1T, SUR_LPH, 16 * tt, 16 * t, lane), G1[t], z);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       int f = 16 * tt + 4 * kq + v;
