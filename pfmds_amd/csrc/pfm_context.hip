@@ -96,22 +96,21 @@ struct MaskQ {
 };
 
 // compensated sum of one (s, c) pair per thread over the block, in a fixed
-// tree order; sh holds 2 * RED_THREADS doubles
+// order: a butterfly over each wavefront's 64 lanes (cross-lane shuffles, no
+// barrier), then the four wavefront sums in LDS in wavefront order.  Every
+// thread returns the block's pair.  sh holds at least 8 doubles (callers
+// pass 2 * RED_THREADS).
 __device__ __forceinline__ CSum block_csum(CSum v, double* sh) {
-  int t = threadIdx.x;
-  sh[t] = v.s;
-  sh[RED_THREADS + t] = v.c;
-  __syncthreads();
-  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
-    if (t < o) {
-      CSum a{sh[t], sh[RED_THREADS + t]};
-      a.add(CSum{sh[t + o], sh[RED_THREADS + t + o]});
-      sh[t] = a.s;
-      sh[RED_THREADS + t] = a.c;
-    }
-    __syncthreads();
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v.add(CSum{__shfl_xor(v.s, o), __shfl_xor(v.c, o)});
+  const int w = threadIdx.x >> 6, nw = RED_THREADS >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[w] = v.s;
+    sh[nw + w] = v.c;
   }
-  CSum r{sh[0], sh[RED_THREADS]};
+  __syncthreads();
+  CSum r{sh[0], sh[nw]};
+  for (int k = 1; k < nw; ++k) r.add(CSum{sh[k], sh[nw + k]});
   __syncthreads();
   return r;
 }

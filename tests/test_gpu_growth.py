@@ -185,3 +185,31 @@ def test_growth_in_a_batch_matches_single_runs():
     g.close()
     for c in singles:
         c.close()
+
+
+def test_growth_in_a_batch_matches_checker():
+    """Each replica of a growing batch against the checker running that
+    replica alone: positions within 1e-10, velocities and forces within 1e-8
+    relative, per-replica energies within 1e-9, the same group counts."""
+    reps = [growth_case.deposition(n_side=4, seed=111), growth_case.deposition(n_side=3, seed=112, spacing=4.4)]
+    for s in reps:
+        s.groups[2] = ["A", "D", "#"]
+    b = system.merge_replicas(reps)
+    print(f"[growth batch vs checker] {b.n_rep} replicas, {b.system.n} atoms", flush=True)
+    g = system.build_batch_context(b, capi.FP64)
+    checkers = [oracle_ctx(s) for s in reps]
+    for s0, s1 in CHUNKS:
+        g.run("nve", 1.0, s0, s1, True)
+        for o in checkers:
+            o.run("nve", 1.0, s0, s1, True)
+        pos, vel, frc = g.state()
+        er = g.replica_energies()
+        for r, o in enumerate(checkers):
+            sl = b.replica_slice(r)
+            p1, v1, f1 = o.state()
+            assert rel_inf(pos[sl], p1) < 1e-10 and rel_inf(vel[sl], v1) < 1e-8 and rel_inf(frc[sl], f1) < 1e-8, (r, s1)
+            assert er["pe"][r] == pytest.approx(o.energies()["pe"], rel=1e-9)
+        assert g.group_count(1) == sum(o.group_count(2) for o in checkers)
+    g.close()
+    for o in checkers:
+        o.close()
