@@ -343,6 +343,7 @@ struct SearchArgs {
   int nrows;
   const int* rows;
   uint32_t g1bit;  // active row atoms; the rows of a growing group hold inactive members with no entries
+  uint32_t g1mem;  // members of group 1 (a growing group's spare bit): the atoms that have rows
   uint32_t g2bit;
   const double4* pos;
   const uint32_t* mask;
@@ -600,7 +601,7 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
   int c = xcd_block(nblocks_real);
   if (c < 0) return;
   const int ncen = a.cell_count[c];
-  if (ncen == 0) return;
+  if (ncen == 0 || !(a.cell_grp[c] & a.g1mem)) return;  // no member of group 1 in the cell
   const RepDev& RG = a.rdev[a.cell_rep ? a.cell_rep[c] : 0];
   if (!RG.tile) return;  // searched by k_build
   const int cs = a.cell_start[c];
@@ -836,7 +837,7 @@ __global__ void __launch_bounds__(256) k_build_tile_wave(SearchArgs a, const int
   const int c = lb * 4 + wv;
   if (c >= ncell) return;
   const int ncen = a.cell_count[c];
-  if (ncen == 0) return;
+  if (ncen == 0 || !(a.cell_grp[c] & a.g1mem)) return;  // no member of group 1 in the cell
   const RepDev& RG = a.rdev[a.cell_rep ? a.cell_rep[c] : 0];
   if (!RG.tile) return;  // searched by k_build
   const int cs = a.cell_start[c];
@@ -1066,6 +1067,7 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   a.nrows = L.nrows;
   a.rows = h->grp.rows[L.g1];
   a.g1bit = 1u << (L.g1 - 1);
+  a.g1mem = 1u << ((h->grp.shadow[L.g1] ? h->grp.shadow[L.g1] : L.g1) - 1);
   a.g2bit = 1u << (L.g2 - 1);
   a.pos = h->pos4;
   a.mask = h->mask;
