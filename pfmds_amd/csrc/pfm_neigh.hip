@@ -606,13 +606,8 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
   if (!RG.tile) return;  // searched by k_build
   const int cs = a.cell_start[c];
   const int tid = threadIdx.x;
-  // cells without a row of this list (another species, another replica's
-  // interactions) stage nothing
-  {
-    int has = 0;
-    for (int m = tid; m < ncen; m += 256) has |= row_of[a.bin ? a.bin[cs + m] : cs + m] >= 0;
-    if (!__syncthreads_or(has)) return;
-  }
+  // (cells without a row of this list, another species or another
+  // replica's interactions, returned above: every group-1 member has a row)
   const int pk = rank_xyz[c];
   const int cc[3] = {pk & 1023, (pk >> 10) & 1023, (pk >> 20) & 1023};
   const double org[3] = {cc[0] * RG.clen.x, cc[1] * RG.clen.y, cc[2] * RG.clen.z};
@@ -841,11 +836,6 @@ __global__ void __launch_bounds__(256) k_build_tile_wave(SearchArgs a, const int
   const RepDev& RG = a.rdev[a.cell_rep ? a.cell_rep[c] : 0];
   if (!RG.tile) return;  // searched by k_build
   const int cs = a.cell_start[c];
-  {
-    bool has = false;
-    for (int m = lane; m < ncen; m += 64) has = has || row_of[a.bin ? a.bin[cs + m] : cs + m] >= 0;
-    if (!__any(has)) return;
-  }
   float4* st = stw[wv];
   const int pk = rank_xyz[c];
   const int cc[3] = {pk & 1023, (pk >> 10) & 1023, (pk >> 20) & 1023};
