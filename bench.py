@@ -58,12 +58,18 @@ def tuned(ctx, integrator, dt, precision, first):
 
 def kernels_only(which, steps=50, warmup=10):
     """A short mixed-precision run of one workload (c1: fcc Cu RGL 131,072
-    atoms, nve; c2: graphene on Cu(111) 94,560 atoms, nvt) with a fixed launch
-    shape (8 lanes x 256 threads, co-scheduled passes), no autotuning and no
-    FP64 leg: the program of the counter passes."""
-    s = configs.fcc_cu_rgl() if which == "c1" else configs.graphene_on_cu()
-    integ, dt = ("nve", 1.0) if which == "c1" else ("nvt", 0.5)
-    ctx = gpu_ctx(s, capi.MIXED)
+    atoms, nve; c2: graphene on Cu(111) 94,560 atoms, nvt; c3: the 512
+    replicas, 3,374,724 atoms, nve at 0.5 fs) with a fixed launch shape (8
+    lanes x 256 threads, co-scheduled passes), no autotuning and no FP64 leg:
+    the program of the counter passes."""
+    if which == "c3":
+        b = system.merge_replicas(configs.replica_mix())
+        s, integ, dt = b.system, "nve", 0.5
+        ctx = system.build_batch_context(b, capi.MIXED)
+    else:
+        s = configs.fcc_cu_rgl() if which == "c1" else configs.graphene_on_cu()
+        integ, dt = ("nve", 1.0) if which == "c1" else ("nvt", 0.5)
+        ctx = gpu_ctx(s, capi.MIXED)
     ctx.set_tuning(8, 256)
     ctx.run(integ, dt, 0, warmup, True)
     t0 = time.perf_counter()
@@ -575,7 +581,7 @@ def main():
     ap.add_argument("--nvms", action="store_true", help="time only the graphene/Cu nvms relaxation")
     ap.add_argument("--output-path", action="store_true",
                     help="time only the run with outputs: stopping at each output step vs output records")
-    ap.add_argument("--kernels", choices=("c1", "c2"),
+    ap.add_argument("--kernels", choices=("c1", "c2", "c3"),
                     help="only a short fixed-shape mixed run of one workload (counter passes)")
     ap.add_argument("--no-graphs", action="store_true", help="launch every step directly (profiling)")
     ap.add_argument("--cpu-leg", choices=("c1", "c2", "farm"), help=argparse.SUPPRESS)
