@@ -300,15 +300,26 @@ __device__ __forceinline__ void lj1g_rows(const RowArgs& a, const Lj1gP& P, int 
     int wh = valid && (tag & 1u) ? 1 : 0;
     int w = wh + (valid && (tag & 2u) ? 1 : 0);
     R dx, dy, dz;
-    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
+    R r, invr2;
+    R tr;
+    if constexpr (sizeof(R) == 4) {
+      // r and 1/r^2 from one v_rsq_f32; the clamp to [R1, R2] in one v_med3
+      R ir;
+      dist_and_inverse<R>(G.d2(pi, q, dx, dy, dz), r, ir);
+      invr2 = ir * ir;
+      tr = __builtin_amdgcn_fmed3f(r, R1, R2);
+    } else {
+      r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
+      invr2 = PM<R>::rcp(r * r);
+      tr = r < R1 ? R1 : (r > R2 ? R2 : r);
+    }
     // f_dfr_cut (cut_off_poly.f90:30-43), dfr without the 1/(R2-R1) factor
-    R tr = r < R1 ? R1 : (r > R2 ? R2 : r);
     R x = (tr - R1) * invw, x2 = x * x;
     R f = R(1) + x2 * x * (R(-10) + R(15) * x - R(6) * x2);
     R dfr = tr * x2 * (R(-30) + R(60) * x - R(30) * x2);
-    R invr2 = PM<R>::rcp(r * r), U = invr2 * invr2 * invr2;
+    R U = invr2 * invr2 * invr2;
     R Fs = U * invr2 * ((c12t12 * U - c6t6) * f - (c12 * U - c6) * dfr);
-    R cw = w ? Fs * R(w) : R(0);
+    R cw = w ? Fs * R(w) : R(0);  // past the row's end r = 0 and Fs is not finite
     fx -= (double)(cw * dx); fy -= (double)(cw * dy); fz -= (double)(cw * dz);
     if (E) {
       // cut_off_poly f_cut (cut_off_poly.f90:6-17)
