@@ -215,9 +215,10 @@ int pfm_set_tuning(pfm_ctx* h, int lanes_per_row, int block);
  * the tb force pass (with the normal-derivative terms) with the metal rows.
  * When every interaction is lj, lj1g or rjl, passes whose rows share no
  * atom share launches in phases (a binary gas's lj1g A and B, then its lj
- * rows of both directions; the rjl passes of a batch beside them).
- * Results equal the separate launches' up to the order of the FP64 force
- * sums.  pfm_set_fusion(h, 0) launches every pass on its own.  No
+ * rows of both directions; the rjl passes of a batch beside them).  The
+ * surrogate's descriptor, network and backward passes run as one kernel
+ * (the same bits as three).  Results equal the separate launches' up to
+ * the order of the FP64 force sums.  pfm_set_fusion(h, 0) launches every pass on its own.  No
  * counterpart in the reference (launch schedule). */
 int pfm_set_fusion(pfm_ctx* h, int on);
 

@@ -611,7 +611,7 @@ void scan_excl(pfm_ctx* h, int n, const int* in, int* out, int* tmp);
 int sur_alloc(pfm_ctx* h, Inter& it);
 void sur_free(Inter& it);
 void launch_sur_converse(pfm_ctx* h, Inter& it, int step);
-int launch_sur_forces(pfm_ctx* h, Inter& it, int accumulate);
+int launch_sur_forces(pfm_ctx* h, Inter& it, int accumulate, bool keep = false);
 int launch_rebo(pfm_ctx* h, Inter& it, bool energy, int accumulate);
 // interactions with a converse list nl(2) built with nl(1)'s cutoff and period
 static inline bool has_converse(int kind) {

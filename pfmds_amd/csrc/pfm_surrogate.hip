@@ -239,10 +239,18 @@ __device__ __forceinline__ void g5_adj(const float* g, float* dS) {
   dS[4] = a;
 }
 
+// The row passes as functions of one row's lanes, shared by the separate
+// kernels (training reads their stored moments and features) and the fused
+// inference kernel (k_sur_fused), so both give the same bits.
+
+// moments and radial sums of the row over lane q's entries, summed over the
+// four entry groups (every lane of the row holds the totals of its eta)
 template <class R>
-__global__ void __launch_bounds__(256) k_sur_desc(RowArgsS a, SurK K, const uint32_t* __restrict__ mask, int nb_real) {
-  SUR_ROW_PROLOGUE
-  float mc[NMON], mx[NMON], dd[1] = {0.0f}, gc[6], gx[6];
+__device__ __forceinline__ void sur_moments(const SurK& K, const Geo<R>& Gm, const typename Geo<R>::P& pi,
+                                            const int* __restrict__ nbrow, int n, int i, int eg, float eta,
+                                            const float* e2k, const uint32_t* __restrict__ mask, float* mc,
+                                            float* mx, float& dd0, float* gc, float* gx) {
+  float dd[1] = {0.0f};
 #pragma unroll
   for (int t = 0; t < NMON; ++t) { mc[t] = 0.0f; mx[t] = 0.0f; }
 #pragma unroll
@@ -278,16 +286,14 @@ __global__ void __launch_bounds__(256) k_sur_desc(RowArgsS a, SurK K, const uint
   sum_groups<1>(dd);
   sum_groups<6>(gc);
   sum_groups<6>(gx);
-  if (eg != 0) return;
-  float* Gr = K.G + (size_t)row * NF;
+  dd0 = dd[0];
+}
+
+// the 20 descriptors of angular eta ke from the row's sums, into Gr[0..39]
+__device__ __forceinline__ void sur_features(int ke, const float* mc, const float* mx, float dd, const float* gc,
+                                             const float* gx, float* Gr) {
 #pragma unroll
   for (int k = 0; k < 6; ++k) { Gr[6 * ke + k] = gc[k]; Gr[12 + 6 * ke + k] = gx[k]; }
-  float* Mc = K.M + ((size_t)row * 4 + ke) * 36;
-  float* Mx = K.M + ((size_t)row * 4 + 2 + ke) * 36;
-#pragma unroll
-  for (int t = 0; t < NMON; ++t) { Mc[t] = mc[t]; Mx[t] = mx[t]; }
-  Mc[NMON] = dd[0];
-  Mx[NMON] = 0.0f;
   float Scc[5] = {0, 0, 0, 0, 0}, Scx[5] = {0, 0, 0, 0, 0};
 #pragma unroll
   for (int t = 0; t < NMON; ++t) {
@@ -295,7 +301,7 @@ __global__ void __launch_bounds__(256) k_sur_desc(RowArgsS a, SurK K, const uint
     Scx[kDeg[t]] += kMul[t] * mc[t] * mx[t];
   }
 #pragma unroll
-  for (int m = 0; m < 5; ++m) Scc[m] -= dd[0];
+  for (int m = 0; m < 5; ++m) Scc[m] -= dd;
   float o[4];
   g5_of(Scc, o);
 #pragma unroll
@@ -305,12 +311,26 @@ __global__ void __launch_bounds__(256) k_sur_desc(RowArgsS a, SurK K, const uint
   for (int m = 0; m < 4; ++m) Gr[32 + ke * 4 + m] = o[m];
 }
 
+template <class R>
+__global__ void __launch_bounds__(256) k_sur_desc(RowArgsS a, SurK K, const uint32_t* __restrict__ mask, int nb_real) {
+  SUR_ROW_PROLOGUE
+  float mc[NMON], mx[NMON], dd, gc[6], gx[6];
+  sur_moments<R>(K, Gm, pi, nbrow, n, i, eg, eta, e2k, mask, mc, mx, dd, gc, gx);
+  if (eg != 0) return;
+  float* Mc = K.M + ((size_t)row * 4 + ke) * 36;
+  float* Mx = K.M + ((size_t)row * 4 + 2 + ke) * 36;
+#pragma unroll
+  for (int t = 0; t < NMON; ++t) { Mc[t] = mc[t]; Mx[t] = mx[t]; }
+  Mc[NMON] = dd;
+  Mx[NMON] = 0.0f;
+  sur_features(ke, mc, mx, dd, gc, gx, K.G + (size_t)row * NF);
+}
+
 // Perceptron on MFMA: 16 atoms per wavefront.  Writes E_i (e_row) and
-// dE_i/dG_i.
-// The weights and the float parameters are staged in LDS once per
-// workgroup (four wavefronts, 64 atoms): the A operands are then LDS reads
-// instead of four L2 streams of the same 28 KB.  Rows are padded by four
-// BF16 (two banks), so the 16 rows of a 16x16 tile fall in distinct banks.
+// dE_i/dG_i.  The weights and the float parameters are staged in LDS once
+// per workgroup: the A operands are then LDS reads instead of L2 streams of
+// the same 28 KB for every wavefront.  Rows are padded by four BF16 (two
+// banks), so the 16 rows of a 16x16 tile fall in distinct banks.
 #define SUR_LP1 (NFP + 4)  // W1 row pitch in LDS
 #define SUR_LPH (NH + 4)   // W1T, W2, W2T row pitch in LDS
 #define SUR_LW1 0
@@ -328,20 +348,22 @@ __device__ __forceinline__ void stage_rows(uint16_t* __restrict__ dst, int pitch
   }
 }
 
-__global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
-  __shared__ __align__(16) uint16_t sW[SUR_LNW];
-  __shared__ float sfp[SUR_NFP_];
+// the network's weights and float parameters into LDS (every thread of the
+// workgroup takes part; a barrier follows)
+__device__ __forceinline__ void stage_network(const SurK& K, uint16_t* sW, float* sfp) {
   stage_rows(sW + SUR_LW1, SUR_LP1, K.W + SUR_OW1, NH, NFP);
   stage_rows(sW + SUR_LW1T, SUR_LPH, K.W + SUR_OW1T, NFP, NH);
   stage_rows(sW + SUR_LW2, SUR_LPH, K.W + SUR_OW2, NH, NH);
   stage_rows(sW + SUR_LW2T, SUR_LPH, K.W + SUR_OW2T, NH, NH);
   for (int k = threadIdx.x; k < SUR_NFP_; k += blockDim.x) sfp[k] = K.fp[k];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, kq = lane >> 4;
-  const int grp = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int row = grp * 16 + (lane & 15);
-  const bool ok = row < nrows;
-  const float* fp = sfp;
+}
+
+// one wavefront: the network on 16 rows.  Lane l takes row l % 16 of the
+// group; Grow is that row's 40 descriptors, e_out its energy slot and gGrow
+// its dE/dG row (ok false: a padding row, nothing read or written).
+__device__ __forceinline__ void sur_mlp16(int lane, bool ok, const float* Grow, const float* fp, const uint16_t* sW,
+                                          double* e_out, float* gGrow) {
+  const int kq = lane >> 4;
   const uint16_t* W1 = sW + SUR_LW1;
   const uint16_t* W1T = sW + SUR_LW1T;
   const uint16_t* W2 = sW + SUR_LW2;
@@ -353,7 +375,7 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       int f = 16 * kk + 4 * kq + v;
-      x[v] = (ok && f < NF) ? (K.G[(size_t)row * NF + f] - fp[SUR_MU + f]) * fp[SUR_ISG + f] : 0.0f;
+      x[v] = (ok && f < NF) ? (Grow[f] - fp[SUR_MU + f]) * fp[SUR_ISG + f] : 0.0f;
     }
     X[kk] = to_bf16(x);
   }
@@ -387,7 +409,7 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
   }
   e += __shfl_xor(e, 16);
   e += __shfl_xor(e, 32);
-  if (ok && kq == 0) K.e_row[row] = (double)(e + fp[SUR_B3]);
+  if (ok && kq == 0) *e_out = (double)(e + fp[SUR_B3]);
   v4s G1[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -407,19 +429,33 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       int f = 16 * tt + 4 * kq + v;
-      if (ok && f < NF) K.gG[(size_t)row * NF + f] = z[v] * fp[SUR_ISG + f];
+      if (ok && f < NF) gGrow[f] = z[v] * fp[SUR_ISG + f];
     }
   }
 }
 
-// dE_i/d(dr_ij) of every entry of the row, and the row atom's own share
-// sum_j dE_i/d(dr_ij) (FP64)
+__global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
+  __shared__ __align__(16) uint16_t sW[SUR_LNW];
+  __shared__ float sfp[SUR_NFP_];
+  stage_network(K, sW, sfp);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int grp = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int row = grp * 16 + (lane & 15);
+  const bool ok = row < nrows;
+  const size_t rr = ok ? (size_t)row : 0;
+  sur_mlp16(lane, ok, K.G + rr * NF, sfp, sW, K.e_row + rr, K.gG + rr * NF);
+}
+
+// dE_i/d(dr_ij) of every entry of the row (into drow) from dE/dG of the row
+// (g) and its moments (Mc, Mx: this lane's eta); returns the lane's share of
+// sum_j dE_i/d(dr_ij)
 template <class R>
-__global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint32_t* __restrict__ mask, int nb_real) {
-  SUR_ROW_PROLOGUE
-  const float* g = K.gG + (size_t)row * NF;
-  const float* Mc = K.M + ((size_t)row * 4 + ke) * 36;
-  const float* Mx = K.M + ((size_t)row * 4 + 2 + ke) * 36;
+__device__ __forceinline__ void sur_back_walk(const SurK& K, const Geo<R>& Gm, const typename Geo<R>::P& pi,
+                                              const int* __restrict__ nbrow, int n, int i, int ke, int eg, float eta,
+                                              const float* e2k, const uint32_t* __restrict__ mask,
+                                              const float* g, const float* Mc, const float* Mx, float4* drow,
+                                              double* sf) {
   float gcc[4], gcx[4], dScc[5], dScx[5];
 #pragma unroll
   for (int m = 0; m < 4; ++m) { gcc[m] = g[24 + ke * 4 + m]; gcx[m] = g[32 + ke * 4 + m]; }
@@ -436,8 +472,7 @@ __global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint
   float cc[6], cx[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) { cc[k] = g[6 * ke + k]; cx[k] = g[12 + 6 * ke + k]; }
-  float4* drow = K.d + (size_t)row * K.stride0;
-  double sf[3] = {0.0, 0.0, 0.0};
+  sf[0] = 0.0; sf[1] = 0.0; sf[2] = 0.0;
   // the two eta lanes of an entry group walk the same entries
   SUR_WALK_BEGIN(int, SUR_NOAUX)
     float dx, dy, dz;
@@ -486,6 +521,63 @@ __global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint
     sf[0] += c0; sf[1] += c1; sf[2] += c2;
   SUR_WALK_END
   sum_groups<3>(sf);
+}
+
+// dE_i/d(dr_ij) of every entry of the row, and the row atom's own share
+// sum_j dE_i/d(dr_ij) (FP64)
+template <class R>
+__global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint32_t* __restrict__ mask, int nb_real) {
+  SUR_ROW_PROLOGUE
+  double sf[3];
+  sur_back_walk<R>(K, Gm, pi, nbrow, n, i, ke, eg, eta, e2k, mask, K.gG + (size_t)row * NF,
+                   K.M + ((size_t)row * 4 + ke) * 36, K.M + ((size_t)row * 4 + 2 + ke) * 36,
+                   K.d + (size_t)row * K.stride0, sf);
+  if (q == 0) K.fself[row] = make_double4(sf[0], sf[1], sf[2], 0.0);
+}
+
+// The three row passes of a step in one kernel (inference): a workgroup's
+// 32 rows take their moments, their descriptors go to LDS, two wavefronts
+// run the network on them (weights staged in LDS), and the rows walk again
+// for dE/d(dr) with their moments still in registers.  The moments,
+// descriptors and dE/dG never reach memory; the results are those of
+// k_sur_desc, k_sur_mlp and k_sur_back.
+template <class R>
+__global__ void __launch_bounds__(256) k_sur_fused(RowArgsS a, SurK K, const uint32_t* __restrict__ mask,
+                                                   int nb_real) {
+  __shared__ __align__(16) uint16_t sW[SUR_LNW];
+  __shared__ float sfp[SUR_NFP_];
+  __shared__ float sG[32 * NF], sgG[32 * NF];
+  const int lb = xcd_block(nb_real);
+  if (lb < 0) return;
+  stage_network(K, sW, sfp);
+  const int row = (lb * blockDim.x + threadIdx.x) >> 3, q = threadIdx.x & 7, lrow = threadIdx.x >> 3;
+  const bool live = row < a.nrows;
+  const int i = live ? a.rows[row] : a.rows[0];
+  const Geo<R> Gm = sgeo<R>(a, i);
+  const typename Geo<R>::P pi = Gm.at(i);
+  const int n = live ? a.nnum[row] : 0;
+  const int* nbrow = a.nbr + (size_t)(live ? row : 0) * a.stride;
+  const int ke = q & 1, eg = q >> 1;
+  const float eta = K.etaa[ke];
+  float e2k[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) e2k[k] = K.eta2[6 * ke + k];
+  float mc[NMON], mx[NMON], dd, gc[6], gx[6];
+  sur_moments<R>(K, Gm, pi, nbrow, n, i, eg, eta, e2k, mask, mc, mx, dd, gc, gx);
+  if (eg == 0) sur_features(ke, mc, mx, dd, gc, gx, sG + lrow * NF);
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (w < 2) {
+    const int lr = 16 * w + (lane & 15);
+    const int rr = lb * 32 + lr;
+    const bool ok = rr < a.nrows;
+    sur_mlp16(lane, ok, sG + lr * NF, sfp, sW, K.e_row + (ok ? rr : 0), sgG + lr * NF);
+  }
+  __syncthreads();
+  if (!live) return;
+  double sf[3];
+  sur_back_walk<R>(K, Gm, pi, nbrow, n, i, ke, eg, eta, e2k, mask, sgG + lrow * NF, mc, mx,
+                   K.d + (size_t)row * K.stride0, sf);
   if (q == 0) K.fself[row] = make_double4(sf[0], sf[1], sf[2], 0.0);
 }
 
@@ -1021,16 +1113,22 @@ void launch_sur_converse(pfm_ctx* h, Inter& it, int step) {
   PFM_LAUNCH(h, "list_conv", k_sur_clip, (L1.nrows + 255) / 256, 256, L1.nrows, L1.nnum, L1.nmax);
 }
 
+// keep (the training passes read the moments, descriptors and dE/dG) or
+// pfm_set_fusion(h, 0): three kernels; else the fused kernel
 template <class R>
-static void sur_forces_T(pfm_ctx* h, Inter& it, int accumulate, double4* frc) {
+static void sur_forces_T(pfm_ctx* h, Inter& it, int accumulate, double4* frc, bool keep) {
   SurK K = sur_k(h, it);
   RowArgsS a = sur_rows(h, it.nl[0]);
   if (a.nrows > 0) {
     int nb = (int)(((long)a.nrows * 8 + 255) / 256);
-    PFM_LAUNCH(h, "sur_desc", k_sur_desc<R>, pad8(nb), 256, a, K, h->mask, nb);
-    int ng = (a.nrows + 15) / 16;
-    PFM_LAUNCH(h, "sur_mlp", k_sur_mlp, (ng + 3) / 4, 256, a.nrows, K);
-    PFM_LAUNCH(h, "sur_back", k_sur_back<R>, pad8(nb), 256, a, K, h->mask, nb);
+    if (keep || !h->fuse) {
+      PFM_LAUNCH(h, "sur_desc", k_sur_desc<R>, pad8(nb), 256, a, K, h->mask, nb);
+      int ng = (a.nrows + 15) / 16;
+      PFM_LAUNCH(h, "sur_mlp", k_sur_mlp, (ng + 3) / 4, 256, a.nrows, K);
+      PFM_LAUNCH(h, "sur_back", k_sur_back<R>, pad8(nb), 256, a, K, h->mask, nb);
+    } else {
+      PFM_LAUNCH(h, "sur_fused", k_sur_fused<R>, pad8(nb), 256, a, K, h->mask, nb);
+    }
   }
   RowArgsS c = sur_rows(h, it.nl[1]);
   if (c.nrows > 0) {
@@ -1041,10 +1139,10 @@ static void sur_forces_T(pfm_ctx* h, Inter& it, int accumulate, double4* frc) {
   }
 }
 
-int launch_sur_forces(pfm_ctx* h, Inter& it, int accumulate) {
+int launch_sur_forces(pfm_ctx* h, Inter& it, int accumulate, bool keep) {
   if (!it.sur || !it.sur->have_params) return PFM_E_INVALID;
-  if (h->precision == PFM_MIXED) sur_forces_T<float>(h, it, accumulate, h->frc4);
-  else sur_forces_T<double>(h, it, accumulate, h->frc4);
+  if (h->precision == PFM_MIXED) sur_forces_T<float>(h, it, accumulate, h->frc4, keep);
+  else sur_forces_T<double>(h, it, accumulate, h->frc4, keep);
   return PFM_OK;
 }
 
@@ -1145,7 +1243,7 @@ int pfm_sur_loss_grad(pfm_ctx* h, int inter, const double* f_ref, double e_ref, 
   const int N = h->N, nrows = it.nl[0].nrows;
   HIPCHK(h, hipMemcpyAsync(B.fref, f_ref, sizeof(double) * 3 * N, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemsetAsync(h->frc4, 0, sizeof(double4) * N, h->stream));
-  if ((rc = launch_sur_forces(h, it, 1))) return sur_fail(h, rc, "error: surrogate parameters not set");
+  if ((rc = launch_sur_forces(h, it, 1, true))) return sur_fail(h, rc, "error: surrogate parameters not set");
   const int nrp = (N + 255) / 256;
   const uint32_t bit2 = 1u << (it.nl[0].g2 - 1);
   PFM_LAUNCH(h, "sur_resid", k_sur_resid, nrp, 256, N, h->frc4, B.fref, h->orig, h->mask, bit2, B.res, B.rpart);

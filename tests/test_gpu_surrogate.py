@@ -226,3 +226,27 @@ def test_surrogate_runs_are_bit_identical():
     for other in out[1:]:
         for a, b in zip(out[0], other):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64])
+def test_fused_inference_equals_separate_kernels(precision):
+    """The one-kernel inference pass (k_sur_fused) against the descriptor,
+    network and backward kernels (pfm_set_fusion(h, 0)): the same bits for
+    forces, energies and 30 nvt steps."""
+    s = slab(seed=58)
+    G, _ = ref_descriptors(s, brute=False)
+    _, sur_s, _ = surrogate.split_system(s)
+    theta = theta_for(G, seed=11, scale=0.3)
+    out = []
+    for fuse in (True, False):
+        c = surrogate.surrogate_context(sur_s, theta, precision)
+        c.set_fusion(fuse)
+        c.run("nvt", 0.5, 0, 0, True)
+        f0, e0 = c.state()[2], c.energies()["e_inter"]
+        c.run("nvt", 0.5, 1, 30, True)
+        out.append((f0, e0, c.state(), c.energies()["e_inter"]))
+        c.close()
+    (fa, ea, sa, ea2), (fb, eb, sb, eb2) = out
+    assert np.array_equal(fa, fb) and np.array_equal(ea, eb) and np.array_equal(ea2, eb2)
+    for x, y in zip(sa, sb):
+        assert np.array_equal(x, y)
