@@ -595,10 +595,14 @@ __global__ void __launch_bounds__(256) k_sur_gather(RowArgsS c, const int* __res
   int row = gt / T, t = threadIdx.x & (T - 1);
   if (row >= c.nrows) return;
   int a = c.rows[row];
+  int n = c.nnum[row];
+  const int r0 = row_of1[a];
+  // a metal atom out of the carbons' reach adds nothing to a force an
+  // earlier pass wrote
+  if (n == 0 && r0 < 0 && accumulate) return;
   // the converse entries in units of 2^-40 eV/A: integer sums do not depend
   // on the order of the row
   long long qx = 0, qy = 0, qz = 0;
-  int n = c.nnum[row];
   const int* rv = rev + (size_t)row * c.stride;
   for (int p = t; p < n; p += T) {
     float4 v = d[rv[p]];
@@ -613,7 +617,6 @@ __global__ void __launch_bounds__(256) k_sur_gather(RowArgsS c, const int* __res
          fz = (double)qz * (1.0 / 1099511627776.0);
   if (t == 0) {
     double4 f = accumulate ? frc[a] : make_double4(0, 0, 0, 0);
-    int r0 = row_of1[a];
     if (r0 >= 0) {
       double4 s = fself[r0];
       fx += s.x; fy += s.y; fz += s.z;
@@ -1132,7 +1135,7 @@ static void sur_forces_T(pfm_ctx* h, Inter& it, int accumulate, double4* frc, bo
   }
   RowArgsS c = sur_rows(h, it.nl[1]);
   if (c.nrows > 0) {
-    constexpr int T = 8;
+    constexpr int T = 16;  // carbon rows hold about 90 converse entries
     int nb = (int)(((long)c.nrows * T + 255) / 256);
     PFM_LAUNCH(h, "sur_gather", k_sur_gather<T>, pad8(nb), 256, c, h->grp.row_of[it.nl[0].g1], it.sur->fself,
                it.nl[1].rev, it.sur->d, frc, accumulate, nb);
