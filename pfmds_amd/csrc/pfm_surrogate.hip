@@ -48,6 +48,11 @@
 namespace {
 
 constexpr int NF = 40, NFP = 48, NH = 64, NMON = 35;
+// entry groups per row of the row passes (two lanes each, one per angular eta)
+#ifndef SUR_EG
+#define SUR_EG 4
+#endif
+#define SUR_L (2 * SUR_EG)
 // monomials u_x^a u_y^b u_z^c of degree 0..4 in degree order; each is its
 // parent times one coordinate
 constexpr int kPar[NMON] = {0, 0, 0, 0, 1, 2, 3, 2, 3, 3, 4, 5, 6, 7, 8, 9, 7, 8,
@@ -143,16 +148,16 @@ template <> __device__ __forceinline__ Geo<float> sgeo<float>(const RowArgsS& a,
   return Geo<float>{a.qpos, a.hq.x, a.hq.y, a.hq.z};
 }
 
-// Eight lanes per group-1 row: lane q takes the angular eta q & 1 and the
-// entries p = q/2, q/2 + 4, ... of the row; it accumulates both species
-// through arithmetic masks (no divergence between carbon and metal
-// neighbours) and six of the twelve radial etas.  The four entry groups are
-// summed by butterfly shuffles at the end.
+// SUR_L = 2 SUR_EG lanes per group-1 row: lane q takes the angular eta q & 1
+// and entry group q/2 of the row (the walk below); it accumulates both
+// species through arithmetic masks (no divergence between carbon and metal
+// neighbours) and six of the twelve radial etas.  The SUR_EG entry groups
+// are summed by butterfly shuffles at the end.
 #define SUR_ROW_PROLOGUE                                  \
   int lb = xcd_block(nb_real);                            \
   if (lb < 0) return;                                     \
   int gt = lb * blockDim.x + threadIdx.x;                 \
-  int row = gt >> 3, q = threadIdx.x & 7;                 \
+  int row = gt / SUR_L, q = threadIdx.x & (SUR_L - 1);   \
   if (row >= a.nrows) return;                             \
   int i = a.rows[row];                                    \
   const Geo<R> Gm = sgeo<R>(a, i);                        \
@@ -164,13 +169,13 @@ template <> __device__ __forceinline__ Geo<float> sgeo<float>(const RowArgsS& a,
   float e2k[6];                                           \
   _Pragma("unroll") for (int k = 0; k < 6; ++k) e2k[k] = K.eta2[6 * ke + k];
 
-// sum over the four entry groups of a row (lanes q, q^2, q^4, q^6)
+// sum over the SUR_EG entry groups of a row (lanes q ^ 2k)
 template <int N, class V>
 __device__ __forceinline__ void sum_groups(V* v) {
 #pragma unroll
   for (int c = 0; c < N; ++c) {
-    v[c] += __shfl_xor(v[c], 2, 8);
-    v[c] += __shfl_xor(v[c], 4, 8);
+#pragma unroll
+    for (int o = 2; o < SUR_L; o <<= 1) v[c] += __shfl_xor(v[c], o, SUR_L);
   }
 }
 
@@ -185,14 +190,14 @@ __device__ __forceinline__ float sur_dr_rec(const Geo<R>& Gm, const typename Geo
 }
 
 // The row walks take the row in blocks of four consecutive entries: the lanes
-// of entry group eg take blocks eg, eg + 4, ...; a block costs one 16-byte
+// of entry group eg take blocks eg, eg + SUR_EG, ...; a block costs one 16-byte
 // index load (rows start on 16 bytes and are padded past their end), then
 // the four records, group bits and AUXF values are gathered before the
 // arithmetic on the first, so a block waits on two memory latencies rather
 // than every entry on one.  Entries past the row's end are skipped.
 #define SUR_NOAUX(j) 0
 #define SUR_WALK_BEGIN(AUXT, AUXF)                                                                    \
-  for (int p0 = 4 * eg; p0 < n; p0 += 16) {                                                           \
+  for (int p0 = 4 * eg; p0 < n; p0 += 4 * SUR_EG) {                                                   \
     const int4 v4_ = *reinterpret_cast<const int4*>(nbrow + p0);                                      \
     const int jj_[4] = {v4_.x, p0 + 1 < n ? v4_.y : i, p0 + 2 < n ? v4_.z : i, p0 + 3 < n ? v4_.w : i}; \
     typename Geo<R>::P pp_[4];                                                                        \
@@ -514,9 +519,9 @@ __device__ __forceinline__ void sur_back_walk(const SurK& K, const Geo<R>& Gm, c
       c1 = cu * u[1] + wr * gu[1];
       c2 = cu * u[2] + wr * gu[2];
     }
-    c0 += __shfl_xor(c0, 1, 8);
-    c1 += __shfl_xor(c1, 1, 8);
-    c2 += __shfl_xor(c2, 1, 8);
+    c0 += __shfl_xor(c0, 1, SUR_L);
+    c1 += __shfl_xor(c1, 1, SUR_L);
+    c2 += __shfl_xor(c2, 1, SUR_L);
     if (ke == 0) drow[p] = make_float4(c0, c1, c2, 0.0f);
     sf[0] += c0; sf[1] += c1; sf[2] += c2;
   SUR_WALK_END
@@ -536,8 +541,9 @@ __global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint
 }
 
 // The three row passes of a step in one kernel (inference): a workgroup's
-// 32 rows take their moments, their descriptors go to LDS, two wavefronts
-// run the network on them (weights staged in LDS), and the rows walk again
+// 256 / SUR_L rows take their moments, their descriptors go to LDS, one
+// wavefront per 16 rows runs the network on them (weights staged in LDS),
+// and the rows walk again
 // for dE/d(dr) with their moments still in registers.  The moments,
 // descriptors and dE/dG never reach memory; the results are those of
 // k_sur_desc, k_sur_mlp and k_sur_back.
@@ -546,11 +552,12 @@ __global__ void __launch_bounds__(256) k_sur_fused(RowArgsS a, SurK K, const uin
                                                    int nb_real) {
   __shared__ __align__(16) uint16_t sW[SUR_LNW];
   __shared__ float sfp[SUR_NFP_];
-  __shared__ float sG[32 * NF], sgG[32 * NF];
+  constexpr int RB = 256 / SUR_L;  // rows of the workgroup
+  __shared__ float sG[RB * NF], sgG[RB * NF];
   const int lb = xcd_block(nb_real);
   if (lb < 0) return;
   stage_network(K, sW, sfp);
-  const int row = (lb * blockDim.x + threadIdx.x) >> 3, q = threadIdx.x & 7, lrow = threadIdx.x >> 3;
+  const int row = (lb * blockDim.x + threadIdx.x) / SUR_L, q = threadIdx.x & (SUR_L - 1), lrow = threadIdx.x / SUR_L;
   const bool live = row < a.nrows;
   const int i = live ? a.rows[row] : a.rows[0];
   const Geo<R> Gm = sgeo<R>(a, i);
@@ -567,9 +574,9 @@ __global__ void __launch_bounds__(256) k_sur_fused(RowArgsS a, SurK K, const uin
   if (eg == 0) sur_features(ke, mc, mx, dd, gc, gx, sG + lrow * NF);
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (w < 2) {
+  if (w < RB / 16) {
     const int lr = 16 * w + (lane & 15);
-    const int rr = lb * 32 + lr;
+    const int rr = lb * RB + lr;
     const bool ok = rr < a.nrows;
     sur_mlp16(lane, ok, sG + lr * NF, sfp, sW, K.e_row + (ok ? rr : 0), sgG + lr * NF);
   }
@@ -1123,7 +1130,7 @@ static void sur_forces_T(pfm_ctx* h, Inter& it, int accumulate, double4* frc, bo
   SurK K = sur_k(h, it);
   RowArgsS a = sur_rows(h, it.nl[0]);
   if (a.nrows > 0) {
-    int nb = (int)(((long)a.nrows * 8 + 255) / 256);
+    int nb = (int)(((long)a.nrows * SUR_L + 255) / 256);
     if (keep || !h->fuse) {
       PFM_LAUNCH(h, "sur_desc", k_sur_desc<R>, pad8(nb), 256, a, K, h->mask, nb);
       int ng = (a.nrows + 15) / 16;
@@ -1214,7 +1221,7 @@ int pfm_sur_descriptors(pfm_ctx* h, int inter, double* G) {
   SurK K = sur_k(h, it);
   RowArgsS a = sur_rows(h, L);
   if (a.nrows == 0) return PFM_OK;
-  int nb = (int)(((long)a.nrows * 8 + 255) / 256);
+  int nb = (int)(((long)a.nrows * SUR_L + 255) / 256);
   if (h->precision == PFM_MIXED) PFM_LAUNCH(h, "sur_desc", k_sur_desc<float>, pad8(nb), 256, a, K, h->mask, nb);
   else PFM_LAUNCH(h, "sur_desc", k_sur_desc<double>, pad8(nb), 256, a, K, h->mask, nb);
   std::vector<float> g((size_t)a.nrows * NF);
@@ -1253,7 +1260,7 @@ int pfm_sur_loss_grad(pfm_ctx* h, int inter, const double* f_ref, double e_ref, 
   SurK K = sur_k(h, it);
   RowArgsS a = sur_rows(h, it.nl[0]);
   if (nrows > 0) {
-    int nb = (int)(((long)nrows * 8 + 255) / 256);
+    int nb = (int)(((long)nrows * SUR_L + 255) / 256);
     if (h->precision == PFM_MIXED) PFM_LAUNCH(h, "sur_tangent", k_sur_tangent<float>, pad8(nb), 256, a, K, h->mask, nb);
     else PFM_LAUNCH(h, "sur_tangent", k_sur_tangent<double>, pad8(nb), 256, a, K, h->mask, nb);
   }
