@@ -441,7 +441,8 @@ def replica_batch(steps, fp64_steps, check_steps=100, n_check=8):
 def host_cpu():
     """The host's CPU model (lscpu), its physical cores, the CPUs this process
     may run on, the cgroup's CPU quota and the CPU share OMP_NUM_THREADS
-    grants this job (a GPU box grants one GPU's job 16 CPUs); `threads` = the
+    grants this job (a node that gives a job one of its GPUs gives it 16
+    CPUs); `threads` = the
     physical cores among the usable CPUs, capped by the quota and the share."""
     info = {"model": None, "physical_cores": None, "usable_cpus": len(os.sched_getaffinity(0)),
             "quota_cpus": None}

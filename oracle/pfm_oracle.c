@@ -1,7 +1,7 @@
 /*
  * pfm_oracle.c -- CPU FP64 restatement of the PFMDS per-step MD path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker of the HIP path, and the CPU
+ * TEST INFRASTRUCTURE ONLY (the oracle of the HIP path, and the CPU
  * baseline timed by bench.py).  Nothing in the product loads this file.
  *
  * It follows the reference's own algorithm and loop order, file by file:

@@ -2,7 +2,7 @@
  * pfm_oracle.h -- CPU FP64 restatement of the PFMDS molecular-dynamics step
  * (reference: code_source/MOLECULAR_DYNAMICS and code_source/INTERACTION_POTENTIALS).
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle for the HIP path:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  The product (libpfmds.so and the Fortran front end) never links
  * or calls it.

@@ -3,7 +3,7 @@ the source tree):
 
   pfmds_amd/lib/libpfmds.so          HIP library, gfx950 only (hipcc)
   pfmds_amd/bin/run_md_simulation    Fortran front end (amdflang), links libpfmds.so
-  oracle/liborc.so                   CPU checker used by the tests (gcc)
+  oracle/liborc.so                   CPU oracle used by the tests (gcc)
 """
 import os
 import subprocess

@@ -2,7 +2,7 @@
 
 `Context` wraps one handle of libpfmds.so (the gfx950 HIP library).  The same
 class can wrap any library that exports the same entry points under another
-prefix, which is how the tests drive the CPU checker with identical calls.
+prefix, which is how the tests drive the CPU oracle with identical calls.
 
 The product path has no CPU fallback: `Context.gpu()` raises if the HIP
 library is missing or no device is visible.

@@ -10,7 +10,7 @@ formats (settings grammar, extended xyz, one parameter file per interaction).
 integrator switching, the nvms exit, and the steps on which energies, log
 lines, snapshots and trajectory frames happen.  It hands each run of steps
 between two such events to a `capi.Context` (the HIP library, or in tests the
-CPU checker), exactly as the Fortran front end does.
+CPU oracle), exactly as the Fortran front end does.
 """
 import dataclasses
 import os

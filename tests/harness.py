@@ -1,4 +1,4 @@
-"""Shared test helpers: contexts on the HIP library and on the CPU checker
+"""Shared test helpers: contexts on the HIP library and on the CPU oracle
 (oracle/liborc.so), built from the same System description."""
 import os
 

@@ -1,7 +1,7 @@
 """Energies written from the potentials' mathematical definitions, vectorised
 over explicit pair/triplet index arrays in torch float64, used to pin the CPU
-checker: its forces must equal -grad E computed by autograd, its energies
-must equal these sums.  Nothing here uses the checker's loop structure
+oracle: its forces must equal -grad E computed by autograd, its energies
+must equal these sums.  Nothing here uses the oracle's loop structure
 (neighbour lists, stored distances, the reference's force formulas).
 
 Definitions (reference files give the same functions):

@@ -1,6 +1,6 @@
 """Replica batches (pfm_create_batch) against the same systems run one at a
 time on the GPU, whose single-system path the parity tests pin to the CPU
-checker.
+oracle.
 
 Bar: per replica, positions, velocities and forces after 30 nve steps within
 1e-10 (FP64) and 1e-6 (mixed) relative of the single run (the replicas differ

@@ -1,10 +1,10 @@
 """Every benchmark configuration on the GPU, at its full size.
 
   C0  the binary LJ gas, 4,000 atoms in the 60 A cube, against the CPU
-      checker: lists and FP64 forces at step 0, a 50-step nvt trajectory, and
+      oracle: lists and FP64 forces at step 0, a 50-step nvt trajectory, and
       md()'s schedule (nvt, then nve, then nvms to its exit) driven through
       the same loop for both;
-  C3  replicas: each replica of a batch against the checker running that
+  C3  replicas: each replica of a batch against the oracle running that
       replica alone; the 512-replica, 3,374,724-atom batch through
       size-independent properties (every replica's force sum vanishes, two
       runs are bit-identical, the replicas' energies add up to the batch's);
@@ -16,7 +16,7 @@
 Bars: as the parity cases (lists as sets, forces 1e-10 at step 0, positions
 1e-10 and velocities/forces 1e-8 after 50 steps); schedule: the same exit
 step, log energies within 1e-9 relative, final positions within 1e-8
-relative; batch replicas within 1e-10 of the checker after 30 steps in FP64.
+relative; batch replicas within 1e-10 of the oracle after 30 steps in FP64.
 """
 import numpy as np
 import pytest
@@ -75,7 +75,7 @@ def test_c0_schedule_nvt_nve_nvms():
     s.integrators = [Integrator("nvt", 0.5, 30, 10**9, 10), Integrator("nve", 0.5, 30, 10**9, 10),
                      Integrator("nvms", 0.5, 400, 10**9, 10)]
     s.md_step_limit = 460
-    s.ms_de = 0.15  # the checker exits at step 386
+    s.ms_de = 0.15  # the oracle exits at step 386
     out = []
     for c in (gpu_ctx(s), oracle_ctx(s)):
         recs, last = system.run_md(s, c, out_period=10)
@@ -93,7 +93,7 @@ def test_c0_schedule_nvt_nve_nvms():
 
 # ------------------------------------------------------------------------ C3
 
-def test_batch_replicas_match_checker():
+def test_batch_replicas_match_oracle():
     reps = replicas()
     b = system.merge_replicas(reps)
     g = system.build_batch_context(b, capi.FP64)
@@ -105,7 +105,7 @@ def test_batch_replicas_match_checker():
         o.run("nve", 0.5, 0, STEPS, True)
         p1, v1, f1 = o.state()
         sl = b.replica_slice(r)
-        print(f"[batch vs checker] replica {r}: {s.n} atoms", flush=True)
+        print(f"[batch vs oracle] replica {r}: {s.n} atoms", flush=True)
         assert rel_inf(pos[sl], p1) < 1e-10
         assert rel_inf(vel[sl], v1) < 1e-8
         assert rel_inf(frc[sl], f1) < 1e-8

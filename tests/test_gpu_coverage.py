@@ -1,4 +1,4 @@
-"""HIP path against the CPU checker on the paths the basic parity cases leave
+"""HIP path against the CPU oracle on the paths the basic parity cases leave
 out: ljc (both `simplified` modes), morsec with `simplified` = T, lj1g with a
 switching width R2-R1 = 2 A (where f_dfr_cut's missing 1/(R2-R1) changes the
 force) and in a two-species group whose order is not ascending (lessnnum
@@ -10,7 +10,7 @@ Bar (as the basic parity cases): lists identical as sorted sets and lessnnum
 identical; FP64 forces and per-interaction energies within 1e-10 relative at
 step 0; 25-step trajectories with positions within 1e-10 and velocities and
 forces within 1e-8; 300 free-flight steps against the wall within 1e-12;
-status codes, atom indices and steps identical to the checker's.
+status codes, atom indices and steps identical to the oracle's.
 """
 import numpy as np
 import pytest
@@ -161,7 +161,7 @@ def test_nvms_exit_step_runs_the_wall():
     res = []
     for c in (gpu_ctx(s), oracle_ctx(s)):
         c.run("nvms", 1.0, 0, 0, True, 1e-2)
-        last, ev = c.run("nvms", 1.0, 1, 400, True, 1e-2)  # the checker exits at step 297
+        last, ev = c.run("nvms", 1.0, 1, 400, True, 1e-2)  # the oracle exits at step 297
         res.append((last, ev, c.state()))
     (lg, eg, sg), (lo, eo, so) = res
     assert eg == eo == capi.EV_CONVERGED and lg == lo

@@ -1,6 +1,6 @@
 """The Fortran front end (pfmds_amd/bin/run_md_simulation), reading a
 reference-format settings file and driving libpfmds through ISO_C_BINDING,
-against the Python restatement of md()'s schedule driving the CPU checker:
+against the Python restatement of md()'s schedule driving the CPU oracle:
 same log records (f24.6 as written), same nvms exit step, same final xyz,
 and the outputs the front end writes from device records while the run goes
 on: snapshots (f27.16), trajectory frames (f10.4) and the out block's centre
@@ -36,7 +36,7 @@ def parse_log(path, n_inter, n_nhc):
 
 
 @pytest.mark.parametrize("case", ["lj_gas", "gr_cu", "rebo", "growth"])
-def test_frontend_matches_checker(tmp_path, case):
+def test_frontend_matches_oracle(tmp_path, case):
     if case == "growth":
         # change_group lines in the settings; the trajectory group grows
         s = growth_case.deposition(n_side=6, kind="rjl")

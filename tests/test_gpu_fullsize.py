@@ -2,7 +2,7 @@
 on the system size: fcc Cu RGL 32^3 cells (131,072 atoms) and graphene on
 Cu(111) (94,560 atoms).
 
-Bar: the perfect crystal has the CPU checker's energy per atom (taken on a
+Bar: the perfect crystal has the CPU oracle's energy per atom (taken on a
 4^3-cell crystal, same lattice) within 1e-10 relative and zero force; the sum
 of all forces vanishes (Newton's third law of every pair, the full-list
 gathers form each pair force twice with dr and -dr): |sum F| within 1e-12 of
@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 def test_perfect_fcc_energy_per_atom_matches_small_crystal():
     big = configs.fcc_cu_rgl(nx=32, sigma=0.0, temperature=0.0)
     small = configs.fcc_cu_rgl(nx=4, sigma=0.0, temperature=0.0)
-    print(f"[fcc] {big.n} atoms vs checker on {small.n}", flush=True)
+    print(f"[fcc] {big.n} atoms vs oracle on {small.n}", flush=True)
     g = gpu_ctx(big)
     g.run("nve", 1.0, 0, 0, True)
     o = oracle_ctx(small)

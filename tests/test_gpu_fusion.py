@@ -1,5 +1,5 @@
 """Co-scheduled passes (pfm_set_fusion) against the same passes launched one
-by one, and against the CPU checker: graphene on a metal, and lj / lj1g /
+by one, and against the CPU oracle: graphene on a metal, and lj / lj1g /
 rjl passes over disjoint atoms (a binary gas, a batch of replicas).
 
 The fused launches run the same row arithmetic; only the order of the FP64
@@ -7,7 +7,7 @@ force sums of a graphene atom differs (its graphene-pass, tb and
 normal-derivative contributions are added in another order).  Bar: FP64
 forces within 1e-13 relative (inf-norm) of the separate launches and
 energies within 1e-13; mixed precision within 1e-7 relative; the fused FP64
-path against the checker as the parity cases (forces 1e-10 at step 0,
+path against the oracle as the parity cases (forces 1e-10 at step 0,
 positions 1e-10 after 25 nvt steps); morsec and ljc, `simplified` F and T,
 lane counts 4, 8 and 16, captured blocks and direct launches.
 """
@@ -69,7 +69,7 @@ def test_fused_equals_separate_mixed(graphs):
     assert rel_inf(a[2][0], b[2][0]) < 1e-9
 
 
-def test_fused_against_checker():
+def test_fused_against_oracle():
     s = slab(seed=72)
     g, o = gpu_ctx(s), oracle_ctx(s)
     g.set_fusion(True)
@@ -130,7 +130,7 @@ def test_batch_passes_sharing_launches(precision, tol):
     np.testing.assert_allclose(ra["pe"], rb["pe"], rtol=max(tol, 1e-12))
 
 
-def test_gas_sharing_launches_against_checker():
+def test_gas_sharing_launches_against_oracle():
     s = configs.binary_lj_gas(n=600, box=32.0, seed=74)
     g, o = gpu_ctx(s), oracle_ctx(s)
     g.set_fusion(True)

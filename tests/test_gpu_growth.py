@@ -1,8 +1,8 @@
 """Runtime group growth on the GPU (pfm_add_group_change) against the CPU
-checker and against itself.
+oracle and against itself.
 
 Bar (FP64): positions, velocities and energies within 1e-10 relative of the
-checker after every chunk, active counts equal after every chunk, lists
+oracle after every chunk, active counts equal after every chunk, lists
 equal as sets after rebuilds, for the deposition case with lj and with rjl.
 Captured blocks are used away from the steps where a count changes and give
 bit-identical results to direct launches; records carry the active counts
@@ -10,7 +10,7 @@ of their step.  The same bar holds for lj1g, for a tb sheet whose atoms join
 one at a time, and for metal adatoms joining the rjl and morsec metal group
 under graphene (tb + morsec with nearest-neighbour terms).  A growing
 graphene group of morsec with a tb list stops at step 0 with the reference's
-"not enough gr nearest neibs" status, on the GPU and in the checker.  A
+"not enough gr nearest neibs" status, on the GPU and in the oracle.  A
 rebosc sheet growing the same way meets rebosc's own bars (forces within 1e-6
 of the largest, positions and velocities within 1e-8).
 """
@@ -28,7 +28,7 @@ CHUNKS = [(0, 0), (1, 15), (16, 37), (38, 60)]
 
 
 @pytest.mark.parametrize("kind,side", [("lj", 4), ("rjl", 4), ("lj", 8), ("lj1g", 4), ("lj1g", 8)])
-def test_growth_matches_checker(kind, side):
+def test_growth_matches_oracle(kind, side):
     # 4^3 atoms: lists searched per row; 8^3: three cells a side, searched per cell tile
     s = growth_case.deposition(n_side=side, kind=kind)
     print(f"[growth {kind}] n={s.n}", flush=True)
@@ -97,7 +97,7 @@ def capi_kb():
 
 
 @pytest.mark.parametrize("case", ["tb", "adatoms"])
-def test_growth_tb_and_cos_match_checker(case):
+def test_growth_tb_and_cos_match_oracle(case):
     s = growth_case.carbon_growth_tb() if case == "tb" else growth_case.metal_adatoms()
     g2 = 1 if case == "tb" else 2
     print(f"[growth {case}] n={s.n}", flush=True)
@@ -132,7 +132,7 @@ def test_growing_graphene_group_with_tb_stops():
     assert codes == [capi.E_GRAPHENE_NN, capi.E_GRAPHENE_NN]
 
 
-def test_growth_rebosc_matches_checker():
+def test_growth_rebosc_matches_oracle():
     s = growth_case.carbon_growth_tb()
     s.interactions = [Interaction("rebosc", list(configs.REBO_C), [(1, 1, 8, 2.3, 10)])]
     print(f"[growth rebosc] n={s.n}", flush=True)
@@ -187,29 +187,29 @@ def test_growth_in_a_batch_matches_single_runs():
         c.close()
 
 
-def test_growth_in_a_batch_matches_checker():
-    """Each replica of a growing batch against the checker running that
+def test_growth_in_a_batch_matches_oracle():
+    """Each replica of a growing batch against the oracle running that
     replica alone: positions within 1e-10, velocities and forces within 1e-8
     relative, per-replica energies within 1e-9, the same group counts."""
     reps = [growth_case.deposition(n_side=4, seed=111), growth_case.deposition(n_side=3, seed=112, spacing=4.4)]
     for s in reps:
         s.groups[2] = ["A", "D", "#"]
     b = system.merge_replicas(reps)
-    print(f"[growth batch vs checker] {b.n_rep} replicas, {b.system.n} atoms", flush=True)
+    print(f"[growth batch vs oracle] {b.n_rep} replicas, {b.system.n} atoms", flush=True)
     g = system.build_batch_context(b, capi.FP64)
-    checkers = [oracle_ctx(s) for s in reps]
+    oracles = [oracle_ctx(s) for s in reps]
     for s0, s1 in CHUNKS:
         g.run("nve", 1.0, s0, s1, True)
-        for o in checkers:
+        for o in oracles:
             o.run("nve", 1.0, s0, s1, True)
         pos, vel, frc = g.state()
         er = g.replica_energies()
-        for r, o in enumerate(checkers):
+        for r, o in enumerate(oracles):
             sl = b.replica_slice(r)
             p1, v1, f1 = o.state()
             assert rel_inf(pos[sl], p1) < 1e-10 and rel_inf(vel[sl], v1) < 1e-8 and rel_inf(frc[sl], f1) < 1e-8, (r, s1)
             assert er["pe"][r] == pytest.approx(o.energies()["pe"], rel=1e-9)
-        assert g.group_count(1) == sum(o.group_count(2) for o in checkers)
+        assert g.group_count(1) == sum(o.group_count(2) for o in oracles)
     g.close()
-    for o in checkers:
+    for o in oracles:
         o.close()

@@ -1,4 +1,4 @@
-"""Neighbour-list search on the GPU against the CPU checker, at the corners of
+"""Neighbour-list search on the GPU against the CPU oracle, at the corners of
 the cell-tile search: distances tied with r_cut to the last bit, cells with
 more rows than a wavefront and tiles larger than one LDS pass, and rows that
 overflow nmax.

@@ -5,7 +5,7 @@ relaxations each, from the start xyz) and one evaluation that continues from
 the previous evaluation's final xyz, run with one md() at a time and with
 eight at once: fit_out.txt and every final file byte-identical.  The first
 evaluation's binding energy, height difference and corrugation of both
-systems against the CPU checker running md()'s schedule on the same inputs,
+systems against the CPU oracle running md()'s schedule on the same inputs,
 to 1e-6 (fit_out.txt prints six decimals).
 """
 import io
@@ -31,7 +31,7 @@ def fit_once(root, jobs):
     return fit, fs
 
 
-def test_fit_sweep_concurrent_equals_sequential_and_checker(tmp_path):
+def test_fit_sweep_concurrent_equals_sequential_and_oracle(tmp_path):
     print("[moire] jobs=8", flush=True)
     f8, fs8 = fit_once(str(tmp_path / "p"), 8)
     print("[moire] jobs=1", flush=True)
@@ -50,7 +50,7 @@ def test_fit_sweep_concurrent_equals_sequential_and_checker(tmp_path):
                 fa, fb = fa[32:], fb[32:]
             assert fa == fb, (sim, name)
 
-    # evaluation 1 against the checker: md()'s schedule on the same inputs
+    # evaluation 1 against the oracle: md()'s schedule on the same inputs
     p = f8.points[0]
     for i in (0, 1):
         chk = str(tmp_path / f"chk{i}")
@@ -74,7 +74,7 @@ def test_fit_sweep_concurrent_equals_sequential_and_checker(tmp_path):
         zm = [pos[j, 2] for nm in moire.METAL_NAMES for j in np.nonzero(sp == nm)[0]]
         a1 = moire.position_analysis(zc, fs8.z, 1000.0)
         a2 = moire.position_analysis(zm, fs8.z, 1000.0)
-        print(f"  system {i + 1}: be {p.be[i]:.9f} checker {be:.9f}, grd {p.grd[i]:.9f} checker {a1[2] - a1[1]:.9f}",
+        print(f"  system {i + 1}: be {p.be[i]:.9f} oracle {be:.9f}, grd {p.grd[i]:.9f} oracle {a1[2] - a1[1]:.9f}",
               flush=True)
         assert p.be[i] == pytest.approx(be, abs=1e-6)
         assert p.bd[i] == pytest.approx(a1[0] - a2[0], abs=1e-6)

@@ -1,4 +1,4 @@
-"""HIP path against the CPU checker on the same seeded inputs (C-ABI calls).
+"""HIP path against the CPU oracle on the same seeded inputs (C-ABI calls).
 
 Bar: neighbour lists identical as sorted sets (bit-exact membership test);
 FP64 forces and per-interaction energies within 1e-10 relative (inf-norm of

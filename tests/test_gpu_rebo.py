@@ -1,4 +1,4 @@
-"""rebosc on the GPU against the CPU checker (the reference's energy and its
+"""rebosc on the GPU against the CPU oracle (the reference's energy and its
 central differences over truncated lists).
 
 Bars: lists identical as sorted sets; energies within 1e-10 relative; forces

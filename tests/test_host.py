@@ -33,7 +33,7 @@ def test_oracle_mirrors_the_abi():
     lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liborc.so"))
     for s in header_symbols():
         # GPU-only extras: profiling, launch shape, replica batches (pinned
-        # against single-system GPU runs, not the checker), output records
+        # against single-system GPU runs, not the oracle), output records
         # (pinned against GPU runs stopped at every recorded step) and the
         # learned surrogate (pinned against tests/sur_ref.py)
         if s in ("pfm_set_timing", "pfm_kernel_time", "pfm_set_tuning", "pfm_set_graphs", "pfm_graph_count", "pfm_set_fusion",

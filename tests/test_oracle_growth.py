@@ -1,4 +1,4 @@
-"""Runtime group growth in the CPU checker (change_particle_group_N,
+"""Runtime group growth in the CPU oracle (change_particle_group_N,
 md_general.f90:82-94, applied at the top of every step,
 md_simulation.f90:116-119), pinned by what the rule and md()'s loops fix:
 

@@ -1,4 +1,4 @@
-"""Pins of the CPU checker's per-atom guards and thermostat sequence, from
+"""Pins of the CPU oracle's per-atom guards and thermostat sequence, from
 what the mathematics and the reference's stated rules fix:
 
   - check_positions (md_general.f90:342-364): the cell is accepted up to
@@ -21,7 +21,7 @@ what the mathematics and the reference's stated rules fix:
   - integrate_verlet_z_* (md_integrators.f90:33-56,79-97): x, y, v_x, v_y
     never change, and E = PE + KE is conserved to O(dt^2) (the x,y motion
     is frozen, so the z dynamics is Hamiltonian on its own).
-A mistake planted in the checker (tests/test_oracle_mutants.py) fails these.
+A mistake planted in the oracle (tests/test_oracle_mutants.py) fails these.
 """
 import re
 

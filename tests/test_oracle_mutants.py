@@ -1,10 +1,10 @@
-"""The pins of the CPU checker have teeth: each test plants one plausible
+"""The pins of the CPU oracle have teeth: each test plants one plausible
 mistake in a copy of oracle/pfm_oracle.c (a dropped term, a flipped sign, a
-wrong index or factor), builds it with the checker's own flags, points the
+wrong index or factor), builds it with the oracle's own flags, points the
 helpers at the copy and asserts that the pin guarding that code now fails.
 
 A mutant "fails" a pin when the pin raises AssertionError, or when the
-mutated checker itself stops with an error (PfmError) on the pin's input."""
+mutated oracle itself stops with an error (PfmError) on the pin's input."""
 import os
 import subprocess
 
@@ -163,7 +163,7 @@ def build_mutant(tmp_path, old, new, site=(0, 1)):
     path = tmp_path / "pfm_oracle_mutant.c"
     path.write_text(src[:at] + new + src[at + len(old):])
     so = tmp_path / "liborc_mutant.so"
-    # -I oracle/: "pfm_oracle.h" and "../include/pfm.h" resolve as for the checker
+    # -I oracle/: "pfm_oracle.h" and "../include/pfm.h" resolve as for the oracle
     subprocess.run(["gcc", *CFLAGS, "-I", ORACLE_DIR, "-shared", "-o", str(so), str(path), "-lm"], check=True,
                    capture_output=True, timeout=120)
     return str(so)

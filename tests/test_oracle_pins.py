@@ -1,4 +1,4 @@
-"""Pins of the CPU checker (oracle/liborc.so) before it is trusted.
+"""Pins of the CPU oracle (oracle/liborc.so) before it is trusted.
 
 The reference ships no golden vectors, fixtures or tests, and it cannot be
 built here (it needs the GNU rand/srand extension that amdflang lacks), so
@@ -13,7 +13,7 @@ every pin comes from what the mathematics fixes:
   - integrators: NVE time reversibility and energy conservation, the NHC
     conserved quantity, nvms convergence to the analytic LJ dimer minimum;
   - unit constants.
-A mistake planted in the checker (tests/test_oracle_mutants.py) fails these.
+A mistake planted in the oracle (tests/test_oracle_mutants.py) fails these.
 """
 import numpy as np
 import pytest
@@ -356,7 +356,7 @@ def test_nvms_velocity_rule():
     after the velocity-Verlet kicks, an xyz atom keeps only the projection
     (v.F/F.F) F of its velocity when v.F > 0 and is stopped otherwise; a
     z-only atom keeps its velocity when v.F > 0 and is stopped otherwise. The
-    forces at the new positions come from a fresh checker context."""
+    forces at the new positions come from a fresh oracle context."""
     eps, sig, dt = 0.0104, 3.40, 2.0
     pos = np.array([[5.0, 5.0, 5.0], [8.7, 5.4, 5.3], [5.3, 8.9, 4.6], [8.5, 8.6, 8.9]])
     box = np.array([30.0, 30.0, 30.0])
