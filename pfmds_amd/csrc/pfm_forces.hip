@@ -84,8 +84,16 @@ template <> __device__ __forceinline__ void store_w<float>(const RowArgs& a, int
 // contribution without branching).
 // The FP64 passes are bound by their arithmetic; they keep one entry per
 // iteration (every T-th entry), which costs fewer registers.
-template <int T, class R, class F>
-__device__ __forceinline__ void walk_row(const Geo<R>& G, const int* __restrict__ nbrow, int n, int t, int self, F f) {
+// The FP32 walk calls flush() after every block of four entries: passes that
+// sum a block's contributions in FP32 registers (RowAcc<float>) move them to
+// the row's FP64 sums there, one conversion and FP64 add per block and
+// component instead of one per entry.
+struct NoFlush {
+  __device__ __forceinline__ void operator()() const {}
+};
+template <int T, class R, class F, class B = NoFlush>
+__device__ __forceinline__ void walk_row(const Geo<R>& G, const int* __restrict__ nbrow, int n, int t, int self, F f,
+                                         B flush = B{}) {
   if constexpr (sizeof(R) == 8) {
     for (int p = t; p < n; p += T) {
       const int j = nbrow[p];
@@ -101,8 +109,37 @@ __device__ __forceinline__ void walk_row(const Geo<R>& G, const int* __restrict_
     for (int u = 0; u < 4; ++u) q[u] = G.at(jj[u]);
 #pragma unroll
     for (int u = 0; u < 4; ++u) f(jj[u], q[u], p0 + u < n);
+    flush();
   }
 }
+
+// Per-lane sums of a row pass: force(c, dr) subtracts c dr from the row's
+// force, energy(v) adds v to its energy.  FP64: every term goes straight into
+// the FP64 sums.  FP32 terms: a block of four entries is summed in FP32 and
+// flush() adds the block to the FP64 sums.  The two sides of a pair then
+// round in different blocks, so a system's forces sum to zero within about
+// 1e-8 of sum |F_i| instead of within FP64 rounding.
+template <class R> struct RowAcc;
+template <> struct RowAcc<double> {
+  double fx = 0, fy = 0, fz = 0, e = 0;
+  __device__ __forceinline__ void force(double c, double dx, double dy, double dz) {
+    fx -= c * dx; fy -= c * dy; fz -= c * dz;
+  }
+  __device__ __forceinline__ void energy(double v) { e += v; }
+  __device__ __forceinline__ void flush() {}
+};
+template <> struct RowAcc<float> {
+  double fx = 0, fy = 0, fz = 0, e = 0;
+  float bx = 0, by = 0, bz = 0, be = 0;
+  __device__ __forceinline__ void force(float c, float dx, float dy, float dz) {
+    bx = __builtin_fmaf(c, dx, bx); by = __builtin_fmaf(c, dy, by); bz = __builtin_fmaf(c, dz, bz);
+  }
+  __device__ __forceinline__ void energy(float v) { be += v; }
+  __device__ __forceinline__ void flush() {
+    fx -= (double)bx; fy -= (double)by; fz -= (double)bz; e += (double)be;
+    bx = by = bz = be = 0.0f;
+  }
+};
 
 // walk_row with one per-atom side value of each neighbour (aux[j]) gathered
 // together with its position record: f(j, record, aux value, valid)
@@ -134,9 +171,9 @@ __device__ __forceinline__ void walk_row_aux(const Geo<R>& G, const int* __restr
 // walk_row over entries that carry a two-bit tag in bits 30-31 (lj1g's
 // weights): f(record, tag, valid)
 #define ENTRY_ATOM 0x3fffffff
-template <int T, class R, class F>
+template <int T, class R, class F, class B = NoFlush>
 __device__ __forceinline__ void walk_row_tag(const Geo<R>& G, const int* __restrict__ nbrow, int n, int t, int self,
-                                             F f) {
+                                             F f, B flush = B{}) {
   if constexpr (sizeof(R) == 8) {
     for (int p = t; p < n; p += T) {
       const unsigned e = (unsigned)nbrow[p];
@@ -153,6 +190,7 @@ __device__ __forceinline__ void walk_row_tag(const Geo<R>& G, const int* __restr
     for (int u = 0; u < 4; ++u) q[u] = G.at((int)(ee[u] & ENTRY_ATOM));
 #pragma unroll
     for (int u = 0; u < 4; ++u) f(q[u], ee[u] >> 30, p0 + u < n);
+    flush();
   }
 }
 
@@ -178,17 +216,32 @@ template <class R> struct PairBlock { static constexpr int max = sizeof(R) == 4 
 template <int T, class R>
 __device__ __forceinline__ void rjl_density_rows(const RowArgs& a, const RjlP& P, double* __restrict__ eb, int blk) {
   ROW_PROLOGUE_AT(T, blk)
-  const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
-  const R kq = PM<R>::kexp(-2.0 * P.q);
   double rho = 0.0;
-  walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
-    R dx, dy, dz;
-    R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
-    R fc, dfr;
-    cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
-    R e = PM<R>::expk(kq * (r * ir0 - R(1))) * fc;
-    rho += (double)(valid && r <= R2 ? e : R(0));
-  });
+  if constexpr (sizeof(R) == 4) {
+    // exp(-2q(r/r0 - 1)) = 2^(r kr + k0); cutoff argument in revolutions as in cutoff_bl<float>
+    const float R2 = P.R2, ca = 0.5 * P.invw, cb = -0.5 * P.R1 * P.invw;
+    const float kr = PM<float>::kexp(-2.0 * P.q / P.r0), k0 = PM<float>::kexp(2.0 * P.q);
+    RowAcc<float> acc;
+    walk_row<T, float>(G, nbrow, n, t, i, [&](int, const uint4& q, bool valid) {
+      float dx, dy, dz;
+      const float r = __builtin_amdgcn_sqrtf(G.d2(pi, q, dx, dy, dz));
+      const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fmed3f(__builtin_fmaf(r, ca, cb), 0.0f, 0.5f));
+      const float e = __builtin_amdgcn_exp2f(__builtin_fmaf(r, kr, k0)) * __builtin_fmaf(0.5f, c, 0.5f);
+      acc.energy(valid && r <= R2 ? e : 0.0f);
+    }, [&] { acc.flush(); });
+    rho = acc.e;
+  } else {
+    const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
+    const R kq = PM<R>::kexp(-2.0 * P.q);
+    walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
+      R dx, dy, dz;
+      R r = PM<R>::sqrt_(G.d2(pi, q, dx, dy, dz));
+      R fc, dfr;
+      cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
+      R e = PM<R>::expk(kq * (r * ir0 - R(1))) * fc;
+      rho += (double)(valid && r <= R2 ? e : R(0));
+    });
+  }
   rho = group_sum<T>(rho);
   if (t == 0) {
     double s = sqrt(rho);
@@ -209,25 +262,59 @@ template <int T, class R, bool E>
 __device__ __forceinline__ void rjl_force_rows(const RowArgs& a, const RjlP& P, const double* __restrict__ eb,
                                                int blk) {
   ROW_PROLOGUE_AT(T, blk)
-  const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
-  const R kp = PM<R>::kexp(-P.p), kq = PM<R>::kexp(-2.0 * P.q);
-  const R c2A = 2.0 * P.A0, pr0 = P.p / P.r0, qr0 = P.q / P.r0, xi = P.xi;
-  const R ie = Geo<R>::w(pi);
   double fx = 0, fy = 0, fz = 0, er = 0;
-  walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
-    R dx, dy, dz;
-    R r, ir;
-    dist_and_inverse<R>(G.d2(pi, q, dx, dy, dz), r, ir);
-    R fc, dfr;
-    cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
-    R x = r * ir0 - R(1);
-    R ep = PM<R>::expk(kp * x), eq = PM<R>::expk(kq * x);
-    R c = (c2A * (pr0 * fc - dfr) * ep - xi * (qr0 * fc - R(0.5) * dfr) * (ie + Geo<R>::w(q)) * eq) * ir;
-    bool in = valid && r <= R2;
-    c = in ? c : R(0);
-    fx -= (double)(c * dx); fy -= (double)(c * dy); fz -= (double)(c * dz);
-    if (E) er += in ? (double)(ep * fc) : 0.0;
-  });
+  if constexpr (sizeof(R) == 4) {
+    // The two brackets with fc = (1 + cos)/2 and df r = kdf sin folded into
+    // their constants: 2 A0 (p/r0 fc - df r) = Ac (1 + cos) + As sin and
+    // xi (q/r0 fc - df r/2) = Bc (1 + cos) + Bs sin; the exponentials as
+    // 2^(r k + k0).
+    const double kdf = -0.5 * PFM_PI * P.invw;
+    const float R2 = P.R2, ca = 0.5 * P.invw, cb = -0.5 * P.R1 * P.invw;
+    const float Ac = P.A0 * P.p / P.r0, As = -2.0 * P.A0 * kdf;
+    const float Bc = 0.5 * P.xi * P.q / P.r0, Bs = -0.5 * P.xi * kdf;
+    const float kpr = PM<float>::kexp(-P.p / P.r0), kp0 = PM<float>::kexp(P.p);
+    const float kqr = PM<float>::kexp(-2.0 * P.q / P.r0), kq0 = PM<float>::kexp(2.0 * P.q);
+    const float ie = Geo<float>::w(pi);
+    RowAcc<float> acc;
+    walk_row<T, float>(G, nbrow, n, t, i, [&](int, const uint4& q, bool valid) {
+      float dx, dy, dz, r, ir;
+      dist_and_inverse<float>(G.d2(pi, q, dx, dy, dz), r, ir);
+      const float x = __builtin_amdgcn_fmed3f(__builtin_fmaf(r, ca, cb), 0.0f, 0.5f);
+      const float sn = __builtin_amdgcn_sinf(x), cs = __builtin_amdgcn_cosf(x);
+      const float ep = __builtin_amdgcn_exp2f(__builtin_fmaf(r, kpr, kp0));
+      const float eq = __builtin_amdgcn_exp2f(__builtin_fmaf(r, kqr, kq0));
+      const float A = __builtin_fmaf(As, sn, __builtin_fmaf(Ac, cs, Ac));
+      const float B = __builtin_fmaf(Bs, sn, __builtin_fmaf(Bc, cs, Bc));
+      const bool in = valid && r <= R2;
+      float c = __builtin_fmaf(A, ep, -(B * (ie + Geo<float>::w(q)) * eq)) * ir;
+      c = in ? c : 0.0f;
+      acc.force(c, dx, dy, dz);
+      if (E) {
+        const float e = ep * __builtin_fmaf(0.5f, cs, 0.5f);
+        acc.energy(in ? e : 0.0f);
+      }
+    }, [&] { acc.flush(); });
+    fx = acc.fx; fy = acc.fy; fz = acc.fz; er = acc.e;
+  } else {
+    const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
+    const R kp = PM<R>::kexp(-P.p), kq = PM<R>::kexp(-2.0 * P.q);
+    const R c2A = 2.0 * P.A0, pr0 = P.p / P.r0, qr0 = P.q / P.r0, xi = P.xi;
+    const R ie = Geo<R>::w(pi);
+    walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
+      R dx, dy, dz;
+      R r, ir;
+      dist_and_inverse<R>(G.d2(pi, q, dx, dy, dz), r, ir);
+      R fc, dfr;
+      cutoff_bl<R>(r, R1, invw, kdf, fc, dfr);
+      R x = r * ir0 - R(1);
+      R ep = PM<R>::expk(kp * x), eq = PM<R>::expk(kq * x);
+      R c = (c2A * (pr0 * fc - dfr) * ep - xi * (qr0 * fc - R(0.5) * dfr) * (ie + Geo<R>::w(q)) * eq) * ir;
+      bool in = valid && r <= R2;
+      c = in ? c : R(0);
+      fx -= (double)(c * dx); fy -= (double)(c * dy); fz -= (double)(c * dz);
+      if (E) er += in ? (double)(ep * fc) : 0.0;
+    });
+  }
   fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
   if (E) er = group_sum<T>(er);
   if (t == 0) {
@@ -253,7 +340,7 @@ template <int T, class R, bool E>
 __device__ __forceinline__ void lj_rows(const RowArgs& a, const LjP& P, int blk) {
   ROW_PROLOGUE_AT(T, blk)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, sig = P.sig, e4 = P.eps4, kdf = -0.5 * PFM_PI * P.invw;
-  double fx = 0, fy = 0, fz = 0, e = 0;
+  RowAcc<R> acc;
   walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
     R dx, dy, dz;
     R r, ir;
@@ -264,11 +351,14 @@ __device__ __forceinline__ void lj_rows(const RowArgs& a, const LjP& P, int blk)
     R c = e4 * (V * (R(12) * V - R(6)) * fc * ir - V * (V - R(1)) * dfr) * ir;
     bool in = valid && r < R2;
     c = in ? c : R(0);
-    fx -= (double)(c * dx); fy -= (double)(c * dy); fz -= (double)(c * dz);
-    if (E) e += in ? (double)(e4 * V * (V - R(1)) * fc) : 0.0;
-  });
-  fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
-  if (E) e = group_sum<T>(e);
+    acc.force(c, dx, dy, dz);
+    if (E) {
+      const R v = e4 * V * (V - R(1)) * fc;
+      acc.energy(in ? v : R(0));
+    }
+  }, [&] { acc.flush(); });
+  const double fx = group_sum<T>(acc.fx), fy = group_sum<T>(acc.fy), fz = group_sum<T>(acc.fz);
+  const double e = E ? group_sum<T>(acc.e) : 0.0;
   if (t == 0) {
     write_force(a, i, fx, fy, fz);
     if (E && a.e_row) a.e_row[row] = e;
@@ -295,7 +385,7 @@ template <int T, class R, bool E>
 __device__ __forceinline__ void lj1g_rows(const RowArgs& a, const Lj1gP& P, int blk) {
   ROW_PROLOGUE_AT(T, blk)
   const R R1 = P.R1, R2 = P.R2, invw = P.invw, c6 = P.c6, c12 = P.c12, c6t6 = P.c6t6, c12t12 = P.c12t12;
-  double fx = 0, fy = 0, fz = 0, e = 0;
+  RowAcc<R> acc;
   walk_row_tag<T, R>(G, nbrow, n, t, i, [&](const typename Geo<R>::P& q, unsigned tag, bool valid) {
     int wh = valid && (tag & 1u) ? 1 : 0;
     int w = wh + (valid && (tag & 2u) ? 1 : 0);
@@ -320,15 +410,16 @@ __device__ __forceinline__ void lj1g_rows(const RowArgs& a, const Lj1gP& P, int 
     R U = invr2 * invr2 * invr2;
     R Fs = U * invr2 * ((c12t12 * U - c6t6) * f - (c12 * U - c6) * dfr);
     R cw = w ? Fs * R(w) : R(0);  // past the row's end r = 0 and Fs is not finite
-    fx -= (double)(cw * dx); fy -= (double)(cw * dy); fz -= (double)(cw * dz);
+    acc.force(cw, dx, dy, dz);
     if (E) {
       // cut_off_poly f_cut (cut_off_poly.f90:6-17)
       R fe = (r > R1 && r < R2) ? f : (r <= R1 ? R(1) : R(0));
-      e += wh ? (double)(U * (c12 * U - c6) * fe) : 0.0;
+      const R v = U * (c12 * U - c6) * fe;
+      acc.energy(wh ? v : R(0));
     }
-  });
-  fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
-  if (E) e = group_sum<T>(e);
+  }, [&] { acc.flush(); });
+  const double fx = group_sum<T>(acc.fx), fy = group_sum<T>(acc.fy), fz = group_sum<T>(acc.fz);
+  const double e = E ? group_sum<T>(acc.e) : 0.0;
   if (t == 0) {
     write_force(a, i, fx, fy, fz);
     if (E) a.e_row[row] = e;

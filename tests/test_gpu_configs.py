@@ -6,7 +6,8 @@
       the same loop for both;
   C3  replicas: each replica of a batch against the oracle running that
       replica alone; the 512-replica, 3,374,724-atom batch through
-      size-independent properties (every replica's force sum vanishes, two
+      size-independent properties (every replica's force sum vanishes to
+      2e-8 of its sum of |F| in mixed precision, two
       runs are bit-identical, the replicas' energies add up to the batch's);
   C4  the learned surrogate on the 94,560-atom graphene/Cu system: the
       surrogate's own forces sum to zero, the whole system's force sum
@@ -139,7 +140,9 @@ def test_c3_full_batch_properties():
         assert np.array_equal(ra[k], rb[k])
     fs = per_replica_force_sums(b, sa[2])
     print(f"[c3 full] {b.n_rep} replicas, largest relative force sum {fs.max():.2e}", flush=True)
-    assert fs.max() < 1e-9
+    # mixed precision: the RGL rows sum each block of four FP32 pair terms in
+    # FP32, so the two sides of a pair do not cancel exactly (the LJ rows do)
+    assert fs.max() < 2e-8
     np.testing.assert_allclose(ra["pe"].sum(), ea["pe"], rtol=1e-12)
     np.testing.assert_allclose(ra["ke"].sum(), ea["ke"], rtol=1e-12)
 
