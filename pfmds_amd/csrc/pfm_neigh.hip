@@ -638,7 +638,7 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
   }
   __syncthreads();
   const int R = tc.off[27];
-  const float hi = (float)(a.rc2 * (1.0 + 3e-5)), lo = (float)(a.rc2 * (1.0 - 3e-5));
+  const float hi = (float)(a.rc2 * (1.0 + 3e-5)), lo = (float)(a.rc2 * (1.0 - 3e-5)), win = hi - lo;
 
   // stage the group-2 atoms of tile slots [s0, s1) compacted into st (slot
   // m < R belongs to the last tile cell k with off[k] <= m; off is
@@ -747,23 +747,27 @@ __global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __r
       for (int c0 = 0; c0 < np; c0 += 32 * T) {
         // bit 31-b <-> candidate slot c0 + t + b*T
         const float4* sv = st + c0 + t;
-        // two instructions per candidate after the differences: the margin
-        // hi - d2 as a chain of fused multiply-adds, whose sign bit (set for
-        // d2 > hi, and for the far padding's -inf) shifts into the miss mask
-        unsigned miss = 0u;
+        // two masks with two instructions each per candidate after the
+        // differences: the margin m = hi - d2 as a chain of fused
+        // multiply-adds, whose sign bit (set for d2 > hi, and for the far
+        // padding's -inf) shifts into the miss mask, and the sign of
+        // (hi - lo) - m, set for d2 < lo (clearly inside r_cut)
+        unsigned miss = 0u, inside = 0u;
 #pragma unroll 8
         for (int b = 0; b < 32; ++b) {
           const float4 v = sv[b * T];
           float dx = v.x - px, dy = v.y - py, dz = v.z - pz;
           const float m = fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, hi)));
           miss = __builtin_amdgcn_alignbit(miss, __float_as_uint(m), 31);
+          inside = __builtin_amdgcn_alignbit(inside, __float_as_uint(win - m), 31);
         }
         unsigned hit = ~miss;
         // the row atom is not its own neighbour
         const int ob = own - c0 - t;
         if (ob >= 0 && ob < 32 * T && ob % T == 0) hit &= ~(0x80000000u >> (ob / T));
-        // entries inside the FP32 window: the FP64 test decides
-        unsigned chk = hit;
+        // hits inside the FP32 window: the FP64 test decides (with compacted
+        // staging that is not dense every hit is also checked for the row atom)
+        unsigned chk = CMP && !dense ? hit : hit & ~inside;
         while (chk) {
           const int b = __clz(chk);
           chk &= ~(0x80000000u >> b);
@@ -870,7 +874,7 @@ __global__ void __launch_bounds__(256) k_build_tile_wave(SearchArgs a, const int
   const int koff = inc - kcn;
   const int R = __shfl(inc, 63);
   const int koff13 = __shfl(koff, 13);
-  const float hi = (float)(a.rc2 * (1.0 + 3e-5)), lo = (float)(a.rc2 * (1.0 - 3e-5));
+  const float hi = (float)(a.rc2 * (1.0 + 3e-5)), lo = (float)(a.rc2 * (1.0 - 3e-5)), win = hi - lo;
   // tile slot m < R: the last cell k with off[k] <= m (every lane takes part)
   auto cell_of = [&](int m) -> int {
     int k = 0;
@@ -960,18 +964,19 @@ __global__ void __launch_bounds__(256) k_build_tile_wave(SearchArgs a, const int
       const int own = (!CMP || dense) ? koff13 + qq - s0 : -1;
       for (int c0 = 0; c0 < np; c0 += 32 * T) {
         const float4* sv = st + c0 + t;
-        unsigned miss = 0u;  // as in k_build_tile
+        unsigned miss = 0u, inside = 0u;  // as in k_build_tile
 #pragma unroll 8
         for (int b = 0; b < 32; ++b) {
           const float4 w = sv[b * T];
           float dx = w.x - px, dy = w.y - py, dz = w.z - pz;
           const float m = fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, hi)));
           miss = __builtin_amdgcn_alignbit(miss, __float_as_uint(m), 31);
+          inside = __builtin_amdgcn_alignbit(inside, __float_as_uint(win - m), 31);
         }
         unsigned hit = ~miss;
         const int ob = own - c0 - t;
         if (ob >= 0 && ob < 32 * T && ob % T == 0) hit &= ~(0x80000000u >> (ob / T));
-        unsigned chk = hit;
+        unsigned chk = CMP && !dense ? hit : hit & ~inside;
         while (chk) {
           const int b = __clz(chk);
           chk &= ~(0x80000000u >> b);
