@@ -113,18 +113,21 @@ __device__ __forceinline__ void walk_row(const Geo<R>& G, const int* __restrict_
   }
 }
 
-// Per-lane sums of a row pass: force(c, dr) subtracts c dr from the row's
-// force, energy(v) adds v to its energy.  FP64: every term goes straight into
-// the FP64 sums.  FP32 terms: a block of four entries is summed in FP32 and
-// flush() adds the block to the FP64 sums.  The two sides of a pair then
-// round in different blocks, so a system's forces sum to zero within about
-// 1e-8 of sum |F_i| instead of within FP64 rounding.
+// Per-lane sums of a row pass, a vector (fx, fy, fz) and a scalar e:
+// force(c, dr) subtracts c dr, add(v) / sub(v) add / subtract the vector v,
+// energy(x) adds x to e.  FP64: every term goes straight into the FP64 sums.
+// FP32 terms: a block of four entries is summed in FP32 and flush() adds the
+// block to the FP64 sums.  The two sides of a pair then round in different
+// blocks, so a system's forces sum to zero within about 1e-8 of sum |F_i|
+// instead of within FP64 rounding.
 template <class R> struct RowAcc;
 template <> struct RowAcc<double> {
   double fx = 0, fy = 0, fz = 0, e = 0;
   __device__ __forceinline__ void force(double c, double dx, double dy, double dz) {
     fx -= c * dx; fy -= c * dy; fz -= c * dz;
   }
+  __device__ __forceinline__ void add(double x, double y, double z) { fx += x; fy += y; fz += z; }
+  __device__ __forceinline__ void sub(double x, double y, double z) { fx -= x; fy -= y; fz -= z; }
   __device__ __forceinline__ void energy(double v) { e += v; }
   __device__ __forceinline__ void flush() {}
 };
@@ -132,20 +135,22 @@ template <> struct RowAcc<float> {
   double fx = 0, fy = 0, fz = 0, e = 0;
   float bx = 0, by = 0, bz = 0, be = 0;
   __device__ __forceinline__ void force(float c, float dx, float dy, float dz) {
-    bx = __builtin_fmaf(c, dx, bx); by = __builtin_fmaf(c, dy, by); bz = __builtin_fmaf(c, dz, bz);
+    bx = __builtin_fmaf(-c, dx, bx); by = __builtin_fmaf(-c, dy, by); bz = __builtin_fmaf(-c, dz, bz);
   }
+  __device__ __forceinline__ void add(float x, float y, float z) { bx += x; by += y; bz += z; }
+  __device__ __forceinline__ void sub(float x, float y, float z) { bx -= x; by -= y; bz -= z; }
   __device__ __forceinline__ void energy(float v) { be += v; }
   __device__ __forceinline__ void flush() {
-    fx -= (double)bx; fy -= (double)by; fz -= (double)bz; e += (double)be;
+    fx += (double)bx; fy += (double)by; fz += (double)bz; e += (double)be;
     bx = by = bz = be = 0.0f;
   }
 };
 
 // walk_row with one per-atom side value of each neighbour (aux[j]) gathered
 // together with its position record: f(j, record, aux value, valid)
-template <int T, class R, class A, class F>
+template <int T, class R, class A, class F, class B = NoFlush>
 __device__ __forceinline__ void walk_row_aux(const Geo<R>& G, const int* __restrict__ nbrow, int n, int t, int self,
-                                             const A* __restrict__ aux, F f) {
+                                             const A* __restrict__ aux, F f, B flush = B{}) {
   if constexpr (sizeof(R) == 8) {
     for (int p = t; p < n; p += T) {
       const int j = nbrow[p];
@@ -165,6 +170,7 @@ __device__ __forceinline__ void walk_row_aux(const Geo<R>& G, const int* __restr
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) f(jj[u], q[u], x[u], p0 + u < n);
+    flush();
   }
 }
 
@@ -767,7 +773,7 @@ __device__ __forceinline__ void cos_graphene_rows(const RowArgs& a, const CosP& 
     g = make_double4(gx * inv, gy * inv, gz * inv, 0);
   }
   const R nx = g.x, ny = g.y, nz = g.z, R2 = P.R2, pref = P.pref;
-  double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, e = 0;
+  RowAcc<R> acc, sacc;  // force and energy; S
   walk_row<T, R>(G, nbrow, n, t, i, [&](int, const typename Geo<R>::P& q, bool valid) {
     R dx, dy, dz;
     R r, ir;
@@ -779,14 +785,17 @@ __device__ __forceinline__ void cos_graphene_rows(const RowArgs& a, const CosP& 
     A = in ? A * pref : R(0);
     Bc = in ? Bc * pref : R(0);
     Sc = in ? Sc : R(0);
-    fx -= (double)(A * dx + Bc * nx); fy -= (double)(A * dy + Bc * ny); fz -= (double)(A * dz + Bc * nz);
-    sx += (double)(Sc * dx); sy += (double)(Sc * dy); sz += (double)(Sc * dz);
-    if (E) e += in ? (double)(ep * pref) : 0.0;
-  });
-  fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
-  if (E) e = group_sum<T>(e);
+    acc.sub(A * dx + Bc * nx, A * dy + Bc * ny, A * dz + Bc * nz);
+    sacc.add(Sc * dx, Sc * dy, Sc * dz);
+    if (E) {
+      const R v = ep * pref;
+      acc.energy(in ? v : R(0));
+    }
+  }, [&] { acc.flush(); sacc.flush(); });
+  const double fx = group_sum<T>(acc.fx), fy = group_sum<T>(acc.fy), fz = group_sum<T>(acc.fz);
+  const double e = E ? group_sum<T>(acc.e) : 0.0;
   if (!simplified) {
-    sx = group_sum<T>(sx); sy = group_sum<T>(sy); sz = group_sum<T>(sz);
+    const double sx = group_sum<T>(sacc.fx), sy = group_sum<T>(sacc.fy), sz = group_sum<T>(sacc.fz);
     cos_nn_coef<T, R>(a, P, nn, row, i, t, sx, sy, sz, g, gC);
   }
   if (t == 0) {
@@ -865,7 +874,7 @@ __device__ __forceinline__ void cos_metal_rows(const RowArgs& a, const CosP& P, 
   const int n = a.nnum[row];
   const int* nbrow = a.nbr + (size_t)row * a.stride;
   const R R2 = P.R2, pref = P.pref;
-  double fx = 0, fy = 0, fz = 0;
+  RowAcc<R> acc;
   auto f = [&](int, const typename Geo<R>::P& q, const auto& g, bool valid) {
     R dx, dy, dz;
     R r, ir;
@@ -878,11 +887,12 @@ __device__ __forceinline__ void cos_metal_rows(const RowArgs& a, const CosP& P, 
     bool in = valid && r < R2;
     A = in ? A * pref : R(0);
     Bc = in ? Bc * pref : R(0);
-    fx -= (double)(A * dx + Bc * gx); fy -= (double)(A * dy + Bc * gy); fz -= (double)(A * dz + Bc * gz);
+    acc.sub(A * dx + Bc * gx, A * dy + Bc * gy, A * dz + Bc * gz);
   };
-  if constexpr (sizeof(R) == 4) walk_row_aux<T, R>(G, nbrow, n, t, i, gnf, f);
-  else walk_row_aux<T, R>(G, nbrow, n, t, i, gnorm, f);
-  fx = group_sum<T>(fx); fy = group_sum<T>(fy); fz = group_sum<T>(fz);
+  auto flush = [&] { acc.flush(); };
+  if constexpr (sizeof(R) == 4) walk_row_aux<T, R>(G, nbrow, n, t, i, gnf, f, flush);
+  else walk_row_aux<T, R>(G, nbrow, n, t, i, gnorm, f, flush);
+  const double fx = group_sum<T>(acc.fx), fy = group_sum<T>(acc.fy), fz = group_sum<T>(acc.fz);
   if (t == 0) write_force(a, i, fx, fy, fz);
 }
 
