@@ -1730,6 +1730,8 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
     Inter& f = h->inter[0];
     first_writes = (f.kind == PFM_RJL || f.kind == PFM_LJ1G || f.kind == PFM_LJ || f.kind == PFM_TB) &&
                    f.nl[0].nrows == N;
+    // or the shared lj / lj1g / rjl launches write every atom first
+    first_writes = first_writes || multi_first_writes(h);
   }
   // check_positions (and invert_z) of the first step of the call
   PFM_LAUNCH(h, "check", k_check, nb, 256, N, h->pos4, h->box[0], h->box[1], h->box[2], h->derr, h->orig, s0,

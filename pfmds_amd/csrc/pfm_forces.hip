@@ -1152,13 +1152,50 @@ static void plan_multi(pfm_ctx* h) {
   if (nphase >= (int)passes.size()) return;
   M.phases.assign(nphase, {});
   for (size_t a = 0; a < passes.size(); ++a) M.phases[phase_of[a]].push_back(passes[a].pass);
+  // first writers in launch order (phases in turn; the passes of a phase
+  // hold disjoint atoms): a force pass is first when none of its group's
+  // atoms is written earlier; a force pass with some atoms written earlier
+  // and some not leaves the plan without first writes
+  std::vector<char> written(h->N, 0);
+  M.first_ok = true;
+  for (auto& ph : M.phases) {
+    for (auto& ps : ph) {
+      ps.first = false;
+      if (ps.kind == MK_RJL_DENSITY) continue;
+      const Inter& it = h->inter[ps.inter];
+      const auto& members = h->grp.idx[it.nl[ps.list].g1];
+      size_t before = 0;
+      for (int x : members) before += written[x] ? 1 : 0;
+      if (before == 0) ps.first = true;
+      else if (before != members.size()) M.first_ok = false;
+    }
+    for (auto& ps : ph)
+      if (ps.first)
+        for (int x : h->grp.idx[h->inter[ps.inter].nl[ps.list].g1]) written[x] = 1;
+  }
   M.ok = true;
 }
 
-// the phases of h->mp, one launch each; every pass adds to forces the kick
-// cleared.  Returns 1 when the lane count has no variant here.
+// The shared launches write every atom's force first when the plan has first
+// writers only (plan_multi), no group changes, and the first writers' rows
+// cover every atom now: their groups are disjoint, so the row counts add up
+// to N exactly then.
+bool multi_first_writes(pfm_ctx* h) {
+  if (!h->mp.planned) plan_multi(h);
+  const int T = h->lanes;
+  if (!h->fuse || !h->mp.ok || !h->mp.first_ok || !h->changes.empty() || (T != 4 && T != 8 && T != 16)) return false;
+  long rows = 0;
+  for (const auto& ph : h->mp.phases)
+    for (const auto& ps : ph)
+      if (ps.first) rows += h->inter[ps.inter].nl[ps.list].nrows;
+  return rows == h->N;
+}
+
+// the phases of h->mp, one launch each.  Every pass adds to forces the kick
+// cleared, or with `first` (multi_first_writes) the first writer of each atom
+// writes its force.  Returns 1 when the lane count has no variant here.
 template <class R>
-static int multi_forces(pfm_ctx* h, bool energy) {
+static int multi_forces(pfm_ctx* h, bool energy, bool first) {
   const int T = h->lanes;
   if (T != 4 && T != 8 && T != 16) return 1;
   const int BS = 256;
@@ -1166,11 +1203,11 @@ static int multi_forces(pfm_ctx* h, bool energy) {
     MultiArgs A;
     memset(&A, 0, sizeof A);
     int start = 0;
-    for (const auto& ps : ph) {
+      for (const auto& ps : ph) {
       Inter& it = h->inter[ps.inter];
       MultiPass& S = A.s[A.n];
       S.kind = ps.kind;
-      S.a = row_args(h, it, ps.list, 1);
+      S.a = row_args(h, it, ps.list, first && ps.first ? 0 : 1);
       if (S.a.nrows == 0) continue;
       if (ps.kind == MK_LJ && ps.list == 1) S.a.e_row = nullptr;  // nl(2) rows carry no energy
       if (ps.kind == MK_LJ1G) S.p.lj1g = lj1g_params(it);
@@ -1256,8 +1293,8 @@ static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
     if (rc != 1) return rc;
   }
   if (!h->mp.planned) plan_multi(h);
-  if (h->fuse && h->mp.ok && written) {
-    const int rc = multi_forces<R>(h, energy);
+  if (h->fuse && h->mp.ok && (written || multi_first_writes(h))) {
+    const int rc = multi_forces<R>(h, energy, !written);
     if (rc != 1) return rc;
   }
   const int BS = std::min(h->block, PairBlock<R>::max);  // row passes (T lanes per row)

@@ -516,8 +516,12 @@ struct pfm_ctx {
   // disjoint atoms; planned at the first force evaluation
   struct MultiPlan {
     bool planned = false, ok = false;
+    // first_ok: every force pass is its atoms' first writer in launch order
+    // or writes only atoms an earlier pass wrote (no mixed group)
+    bool first_ok = false;
     struct Pass {
       int inter, kind, list;  // kind: 0 lj1g, 1 lj rows of list `list`, 2 rjl density, 3 rjl force
+      bool first;             // a force pass whose atoms no earlier pass writes
     };
     std::vector<std::vector<Pass>> phases;
   } mp;
@@ -606,6 +610,9 @@ int launch_sort_and_bin(pfm_ctx* h, int step, bool permute);
 int launch_build_list(pfm_ctx* h, int ii, int k, int step);
 int launch_nn_from_tb(pfm_ctx* h, int ii, int step);
 int launch_forces(pfm_ctx* h, bool energy, bool first_writes);
+// the shared lj / lj1g / rjl launches can write every atom's force first
+// (no zeroing before them): see plan_multi
+bool multi_first_writes(pfm_ctx* h);
 void launch_group_rows(pfm_ctx* h);
 void scan_excl(pfm_ctx* h, int n, const int* in, int* out, int* tmp);
 int sur_alloc(pfm_ctx* h, Inter& it);

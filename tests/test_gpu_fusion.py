@@ -143,6 +143,28 @@ def test_gas_sharing_launches_against_oracle():
     np.testing.assert_allclose(g.energies()["e_inter"], o.energies()["e_inter"], rtol=1e-9)
 
 
+def test_sharing_launches_with_partly_written_group_against_oracle():
+    """lj1g over A, then lj1g over A+B: the second pass finds some of its
+    atoms written by the first and others not, so the shared launches cannot
+    write each atom's force first; the step zeroes forces and every pass adds
+    (the gas above takes the other way: lj1g A | lj1g B write, the lj rows
+    add)."""
+    s = configs.binary_lj_gas(n=600, box=32.0, seed=77)
+    a, b = s.interactions[0], s.interactions[1]
+    s.interactions = [a, Interaction("lj1g", b.params, [(3, 3) + tuple(b.lists[0][2:])])]
+    g, o = gpu_ctx(s), oracle_ctx(s)
+    g.set_fusion(True)
+    for c in (g, o):
+        c.run("nvt", 0.5, 0, 0, True)
+    assert rel_inf(g.state()[2], o.state()[2]) < 1e-10
+    for c in (g, o):
+        c.run("nvt", 0.5, 1, 25, True)
+    pg, vg, fg = g.state()
+    po, vo, fo = o.state()
+    assert rel_inf(pg, po) < 1e-10 and rel_inf(vg, vo) < 1e-8 and rel_inf(fg, fo) < 1e-8
+    np.testing.assert_allclose(g.energies()["e_inter"], o.energies()["e_inter"], rtol=1e-9)
+
+
 # ------------------------------------------ lanes per row of one interaction
 
 @pytest.mark.parametrize("precision,tol", [(capi.FP64, 1e-13), (capi.MIXED, 1e-7)])
