@@ -867,7 +867,12 @@ static int finalize(pfm_ctx* h) {
       bool ok = true;
       for (int x : h->grp.idx[h->inter[ir].nl[0].g1]) ok = ok && !inC[x];
       for (int x : h->grp.idx[h->inter[ic].nl[1].g1]) ok = ok && !inC[x];
-      if (ok) h->gm = pfm_ctx::GmPlan{true, ir, itb, ic};
+      if (ok) {
+        h->gm = pfm_ctx::GmPlan{true, ir, itb, ic, true};
+        std::vector<char> inG(h->N, 0);
+        for (int x : h->grp.idx[h->inter[ic].nl[0].g1]) inG[x] = 1;
+        for (int x : h->grp.idx[h->inter[ir].nl[0].g1]) h->gm.first_ok = h->gm.first_ok && !inG[x];
+      }
     }
   }
   // group-local index by original atom, for lj1g and rebosc
@@ -1730,8 +1735,8 @@ static int run_steps(pfm_ctx* h, int integ, double dt, int s0, int s1, int energ
     Inter& f = h->inter[0];
     first_writes = (f.kind == PFM_RJL || f.kind == PFM_LJ1G || f.kind == PFM_LJ || f.kind == PFM_TB) &&
                    f.nl[0].nrows == N;
-    // or the shared lj / lj1g / rjl launches write every atom first
-    first_writes = first_writes || multi_first_writes(h);
+    // or the shared launches write every atom first
+    first_writes = first_writes || multi_first_writes(h) || gm_first_writes(h);
   }
   // check_positions (and invert_z) of the first step of the call
   PFM_LAUNCH(h, "check", k_check, nb, 256, N, h->pos4, h->box[0], h->box[1], h->box[2], h->derr, h->orig, s0,

@@ -1230,10 +1230,11 @@ static int multi_forces(pfm_ctx* h, bool energy, bool first) {
 }
 
 // The three co-scheduled launches of graphene on a metal (h->gm); every
-// pass adds to forces the step's kick cleared.  Returns 1 when the lane
+// pass adds to forces the step's kick cleared, or with `first`
+// (gm_first_writes) the second launch writes them.  Returns 1 when the lane
 // count has no fused variant (the caller launches the passes one by one).
 template <class R>
-static int gm_forces(pfm_ctx* h, bool energy) {
+static int gm_forces(pfm_ctx* h, bool energy, bool first) {
   const int T = h->lanes;
   if (T != 4 && T != 8 && T != 16) return 1;
   Inter& ir = h->inter[h->gm.rjl];
@@ -1241,7 +1242,10 @@ static int gm_forces(pfm_ctx* h, bool energy) {
   Inter& ic = h->inter[h->gm.cos];
   if (ir.nl[0].nrows == 0 || it.nl[0].nrows == 0 || ic.nl[0].nrows == 0) return 1;
   const int BS = 256;
-  RowArgs am = row_args(h, ir, 0, 1), at = row_args(h, it, 0, 1), ag = row_args(h, ic, 0, 1);
+  // with `first` the second launch's passes (graphene rows, rjl forces) write
+  // their rows' forces and the third launch adds to them
+  const int acc2 = first ? 0 : 1;
+  RowArgs am = row_args(h, ir, 0, acc2), at = row_args(h, it, 0, 1), ag = row_args(h, ic, 0, acc2);
   RowArgs amet = row_args(h, ic, 1, 1);
   const RjlP Pr = rjl_params(ir);
   const TbP Pt = tb_params(it);
@@ -1285,11 +1289,26 @@ static int gm_forces(pfm_ctx* h, bool energy) {
   return PFM_OK;
 }
 
+// The rjl rows and the graphene rows of the second launch write every atom's
+// force first when they share no atom (h->gm.first_ok), no group changes are
+// pending and their rows add up to N; the first launch writes no force and
+// the third only adds.  The conditions under which gm_forces returns 1 are
+// excluded here.
+bool gm_first_writes(pfm_ctx* h) {
+  const int T = h->lanes;
+  if (!h->fuse || !h->gm.ok || !h->gm.first_ok || !h->changes.empty() || (T != 4 && T != 8 && T != 16)) return false;
+  const Inter& ir = h->inter[h->gm.rjl];
+  const Inter& it = h->inter[h->gm.tb];
+  const Inter& ic = h->inter[h->gm.cos];
+  if (ir.nl[0].nrows == 0 || it.nl[0].nrows == 0 || ic.nl[0].nrows == 0) return false;
+  return (long)ir.nl[0].nrows + ic.nl[0].nrows == h->N;
+}
+
 template <class R>
 static int forces_T(pfm_ctx* h, bool energy, bool first_writes) {
   bool written = !first_writes;  // has any pass written this step's forces yet
-  if (h->fuse && h->gm.ok && written) {
-    const int rc = gm_forces<R>(h, energy);
+  if (h->fuse && h->gm.ok && (written || gm_first_writes(h))) {
+    const int rc = gm_forces<R>(h, energy, !written);
     if (rc != 1) return rc;
   }
   if (!h->mp.planned) plan_multi(h);

@@ -510,6 +510,7 @@ struct pfm_ctx {
   struct GmPlan {
     bool ok = false;
     int rjl = -1, tb = -1, cos = -1;
+    bool first_ok = false;  // the rjl rows and the ljc/morsec graphene rows share no atom
   } gm;
   bool fuse = true;  // pfm_set_fusion
   // lj / lj1g / rjl passes grouped into launches (phases) whose passes write
@@ -613,6 +614,9 @@ int launch_forces(pfm_ctx* h, bool energy, bool first_writes);
 // the shared lj / lj1g / rjl launches can write every atom's force first
 // (no zeroing before them): see plan_multi
 bool multi_first_writes(pfm_ctx* h);
+// the co-scheduled launches of graphene on a metal can write every atom's
+// force first: see gm_forces
+bool gm_first_writes(pfm_ctx* h);
 void launch_group_rows(pfm_ctx* h);
 void scan_excl(pfm_ctx* h, int n, const int* in, int* out, int* tmp);
 int sur_alloc(pfm_ctx* h, Inter& it);
