@@ -593,7 +593,7 @@ struct TileCells {
 };
 
 template <int T, bool CMP>
-__global__ void __launch_bounds__(256) k_build_tile(SearchArgs a, const int* __restrict__ rank_xyz,
+__global__ void __launch_bounds__(256, CMP ? 5 : 6) k_build_tile(SearchArgs a, const int* __restrict__ rank_xyz,
                                                     const int* __restrict__ row_of, int nblocks_real) {
   static_assert(TILE_MAX % (32 * T) == 0, "tile passes hold whole bitmask rounds");
   __shared__ float4 st[TILE_MAX];
@@ -824,7 +824,7 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 template <int T, bool CMP>
-__global__ void __launch_bounds__(256) k_build_tile_wave(SearchArgs a, const int* __restrict__ rank_xyz,
+__global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const int* __restrict__ rank_xyz,
                                                          const int* __restrict__ row_of, int ncell,
                                                          int nblocks_real) {
   static_assert(WTILE % (32 * T) == 0 && WTILE % 64 == 0, "tile passes hold whole bitmask rounds");
@@ -890,49 +890,53 @@ __global__ void __launch_bounds__(256) k_build_tile_wave(SearchArgs a, const int
   // group 2 far away; with CMP only group-2 atoms, compacted in tile order.
   // Returns the staged length padded to whole rounds.
   bool dense = true;
+  // slots in chunks of KC per lane (KC * 64 per wave): a tile of up to that
+  // many candidates waits on one memory latency, and the registers hold one
+  // chunk
+  constexpr int KC = KW < 4 ? KW : 4;
   auto stage = [&](int s0, int s1) -> int {
-    float4 v[KW];
-    bool keep[KW];
+    int base = 0;
+    for (int cb = 0; cb < KW && s0 + cb * 64 < s1; cb += KC) {
+      float4 v[KC];
+      bool keep[KC];
 #pragma unroll
-    for (int u = 0; u < KW; ++u) {
-      const int m = s0 + u * 64 + lane;
-      const int k = cell_of(m < s1 ? m : s0);
-      const int ks = __shfl(kst, k), ko = __shfl(koff, k);
-      v[u] = make_float4(TILE_FAR, TILE_FAR, TILE_FAR, __int_as_float(-1));
-      keep[u] = false;
-      if (m < s1) {
-        int j = ks + (m - ko);
-        if (a.bin) j = a.bin[j];
-        const int o0 = k % 3 - 1, o1 = (k / 3) % 3 - 1, o2 = k / 9 - 1;
-        const double4 p = a.pos[j];
-        keep[u] = (a.mask[j] & a.g2bit) != 0;
-        double x = p.x - org[0], y = p.y - org[1], z = p.z - org[2];
-        if (o0 < 0 && x > 0.0) x -= boxL.x; else if (o0 > 0 && x < 0.0) x += boxL.x;
-        if (o1 < 0 && y > 0.0) y -= boxL.y; else if (o1 > 0 && y < 0.0) y += boxL.y;
-        if (o2 < 0 && z > 0.0) z -= boxL.z; else if (o2 > 0 && z < 0.0) z += boxL.z;
-        v[u].w = __int_as_float(j);
-        if (!CMP || keep[u]) {
+      for (int u = 0; u < KC; ++u) {
+        const int m = s0 + (cb + u) * 64 + lane;
+        const int k = cell_of(m < s1 ? m : s0);
+        const int ks = __shfl(kst, k), ko = __shfl(koff, k);
+        v[u] = make_float4(TILE_FAR, TILE_FAR, TILE_FAR, __int_as_float(-1));
+        keep[u] = false;
+        if (m < s1) {
+          int j = ks + (m - ko);
+          if (a.bin) j = a.bin[j];
+          const int o0 = k % 3 - 1, o1 = (k / 3) % 3 - 1, o2 = k / 9 - 1;
+          const double4 p = a.pos[j];
+          keep[u] = (a.mask[j] & a.g2bit) != 0;
+          double x = p.x - org[0], y = p.y - org[1], z = p.z - org[2];
+          if (o0 < 0 && x > 0.0) x -= boxL.x; else if (o0 > 0 && x < 0.0) x += boxL.x;
+          if (o1 < 0 && y > 0.0) y -= boxL.y; else if (o1 > 0 && y < 0.0) y += boxL.y;
+          if (o2 < 0 && z > 0.0) z -= boxL.z; else if (o2 > 0 && z < 0.0) z += boxL.z;
+          v[u].w = __int_as_float(j);
           if (keep[u]) { v[u].x = (float)x; v[u].y = (float)y; v[u].z = (float)z; }
         }
       }
-    }
-    int base = 0;
-    if constexpr (!CMP) {
+      if constexpr (!CMP) {
 #pragma unroll
-      for (int u = 0; u < KW; ++u) {
-        const int m = s0 + u * 64 + lane;
-        if (m < s1) st[m - s0] = v[u];
-      }
-      base = s1 - s0;
-    } else {
+        for (int u = 0; u < KC; ++u) {
+          const int m = s0 + (cb + u) * 64 + lane;
+          if (m < s1) st[m - s0] = v[u];
+        }
+      } else {
 #pragma unroll
-      for (int u = 0; u < KW; ++u) {
-        const unsigned long long bal = __ballot(keep[u]);
-        if (keep[u]) st[base + __popcll(bal & ((1ull << lane) - 1ull))] = v[u];
-        base += __popcll(bal);
+        for (int u = 0; u < KC; ++u) {
+          const unsigned long long bal = __ballot(keep[u]);
+          if (keep[u]) st[base + __popcll(bal & ((1ull << lane) - 1ull))] = v[u];
+          base += __popcll(bal);
+        }
       }
-      dense = base == s1 - s0;
     }
+    if constexpr (!CMP) base = s1 - s0;
+    else dense = base == s1 - s0;
     const int np = (base + 32 * T - 1) / (32 * T) * (32 * T);
     for (int m = base + lane; m < np; m += 64) st[m] = make_float4(TILE_FAR, TILE_FAR, TILE_FAR, __int_as_float(-1));
     return np;
@@ -965,7 +969,7 @@ __global__ void __launch_bounds__(256) k_build_tile_wave(SearchArgs a, const int
       for (int c0 = 0; c0 < np; c0 += 32 * T) {
         const float4* sv = st + c0 + t;
         unsigned miss = 0u, inside = 0u;  // as in k_build_tile
-#pragma unroll 8
+#pragma unroll 4
         for (int b = 0; b < 32; ++b) {
           const float4 w = sv[b * T];
           float dx = w.x - px, dy = w.y - py, dz = w.z - pz;
