@@ -785,9 +785,10 @@ static int finalize(pfm_ctx* h) {
   if ((rc = dalloc(h, &h->d_rep_row0, row0.size()))) return rc;
   HIPCHK(h, hipMemcpyAsync(h->d_rep_row0, row0.data(), sizeof(int) * row0.size(), hipMemcpyHostToDevice, h->stream));
   // counts, fill counters, then each cell's union of group bits
-  if ((rc = dalloc(h, &h->cell_count, 3 * (size_t)h->ncell))) return rc;
+  if ((rc = dalloc(h, &h->cell_count, 4 * (size_t)h->ncell))) return rc;
   h->cell_fill = h->cell_count + h->ncell;
   h->cell_grp = (uint32_t*)(h->cell_count + 2 * (size_t)h->ncell);
+  h->cell_tmp = h->cell_count + 3 * (size_t)h->ncell;
   if ((rc = dalloc(h, &h->cell_start, h->ncell))) return rc;
   // group rows
   for (int g = 1; g <= PFM_MAXG; ++g) {

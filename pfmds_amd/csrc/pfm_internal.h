@@ -475,6 +475,10 @@ struct pfm_ctx {
   double cell_len[3];
   int *cell_rank = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr, *atom_rank = nullptr;
   uint32_t* cell_grp = nullptr;  // inside the cell_count allocation
+  // the binning's own counts (inside the cell_count allocation): counted by
+  // k_cell_keys, copied to cell_count and cleared with cell_fill by
+  // k_cell_sort, so they start every binning at zero
+  int* cell_tmp = nullptr;
   int* cell_xyz = nullptr;  // by Morton rank: x | y<<10 | z<<20
   int* scan_tmp = nullptr;
   // RGL per-atom densities

@@ -157,19 +157,37 @@ __global__ void k_scatter(int N, const int* __restrict__ atom_rank, const int* _
   if (valid) perm[cell_start[r] + slot] = i;
 }
 
+// the per-atom arrays in their current order (0) and in cell order (1)
+struct PermArgs {
+  const double4 *p0, *v0, *f0;
+  const uint32_t* m0;
+  const int *o0, *r0;
+  double4 *p1, *v1, *f1;
+  uint32_t* m1;
+  int *o1, *r1;
+  uint4* q1;  // fixed-point copy of the new positions (mixed path), or null
+  const RepDev* rdev;
+};
+
 // order the atoms of each cell by original index, so the layout (and every
 // sum over it) does not depend on the order of the scatter's atomics: one
 // 64-lane workgroup per cell, each lane ranks its entries against the cell.
-// Also the union of the cell's group bits (cell_grp), which lets a list
-// search skip tiles that hold no candidate.
+// PERM: every atom's data moves to its new place (the re-sort); otherwise
+// the order goes to perm_out (binning without moving the atoms).  Also the
+// cell's count for the searches, the union of its group bits (cell_grp,
+// which lets a list search skip tiles that hold no candidate), and the
+// binning's counters cleared for the next binning.
+template <bool PERM>
 __global__ void __launch_bounds__(64) k_cell_sort(int ncell, const int* __restrict__ cell_start,
-                                                  const int* __restrict__ cell_count, const int* __restrict__ perm_in,
+                                                  int* __restrict__ cell_tmp, int* __restrict__ cell_fill,
+                                                  int* __restrict__ cell_count, const int* __restrict__ perm_in,
                                                   int* __restrict__ perm_out, const int* __restrict__ orig,
-                                                  const uint32_t* __restrict__ mask, uint32_t* __restrict__ cell_grp) {
+                                                  const uint32_t* __restrict__ mask, uint32_t* __restrict__ cell_grp,
+                                                  PermArgs P) {
   __shared__ int key[512];
   int c = blockIdx.x;
   if (c >= ncell) return;
-  int s = cell_start[c], n = cell_count[c];
+  int s = cell_start[c], n = cell_tmp[c];
   bool lds = n <= 512;
   uint32_t g = 0u;
   if (lds)
@@ -181,52 +199,53 @@ __global__ void __launch_bounds__(64) k_cell_sort(int ncell, const int* __restri
     int kk = lds ? key[k] : orig[v];
     int rank = 0;
     for (int m = 0; m < n; ++m) rank += ((lds ? key[m] : orig[perm_in[s + m]]) < kk) ? 1 : 0;
-    perm_out[s + rank] = v;
+    if constexpr (PERM) {
+      const int d = s + rank;
+      const double4 p = P.p0[v];
+      P.p1[d] = p;
+      int r = 0;
+      if (P.r0) { r = P.r0[v]; P.r1[d] = r; }
+      if (P.q1) P.q1[d] = quantize(p, P.rdev[r].qs, 0u);
+      P.v1[d] = P.v0[v];
+      P.f1[d] = P.f0[v];
+      P.m1[d] = P.m0[v];
+      P.o1[d] = P.o0[v];
+    } else {
+      perm_out[s + rank] = v;
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) g |= (uint32_t)__shfl_xor((int)g, o);
-  if (threadIdx.x == 0) cell_grp[c] = g;
+  if (threadIdx.x == 0) {
+    cell_grp[c] = g;
+    cell_count[c] = n;
+    cell_tmp[c] = 0;
+    cell_fill[c] = 0;
+  }
 }
 
-__global__ void k_permute(int N, const int* __restrict__ perm, const double4* __restrict__ p0,
-                          const double4* __restrict__ v0, const double4* __restrict__ f0,
-                          const uint32_t* __restrict__ m0, const int* __restrict__ o0, double4* __restrict__ p1,
-                          double4* __restrict__ v1, double4* __restrict__ f1, uint32_t* __restrict__ m1,
-                          int* __restrict__ o1, uint4* __restrict__ q1, const int* __restrict__ r0,
-                          int* __restrict__ r1, const RepDev* __restrict__ rdev) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= N) return;
-  int i = perm[k];
-  double4 p = p0[i];
-  p1[k] = p;
-  int r = 0;
-  if (r0) { r = r0[i]; r1[k] = r; }
-  if (q1) q1[k] = quantize(p, rdev[r].qs, 0u);
-  v1[k] = v0[i];
-  f1[k] = f0[i];
-  m1[k] = m0[i];
-  o1[k] = o0[i];
-}
-
-// Bin atoms into cells and re-order every per-atom array in Morton cell order.
+// Bin atoms into cells and re-order every per-atom array in Morton cell
+// order.  The binning counters (cell_tmp, cell_fill) are zero on entry: the
+// allocation clears them and k_cell_sort clears them again for the next call.
 int launch_sort_and_bin(pfm_ctx* h, int step, bool permute) {
   (void)step;
   int N = h->N, nb = (N + 255) / 256;
-  PFM_LAUNCH(h, "cell_keys", k_fill_int, (2 * h->ncell + 255) / 256, 256, 2 * h->ncell, h->cell_count, 0);
   PFM_LAUNCH(h, "cell_keys", k_cell_keys, nb, 256, N, h->pos4, h->rep, h->rdev, h->cell_rank, h->atom_rank,
-             h->cell_count);
-  scan_excl(h, h->ncell, h->cell_count, h->cell_start, h->scan_tmp);
+             h->cell_tmp);
+  scan_excl(h, h->ncell, h->cell_tmp, h->cell_start, h->scan_tmp);
   PFM_LAUNCH(h, "scatter", k_scatter, nb, 256, N, h->atom_rank, h->cell_start, h->cell_fill, h->scan_tmp + N);
-  PFM_LAUNCH(h, "cell_sort", k_cell_sort, h->ncell, 64, h->ncell, h->cell_start, h->cell_count, h->scan_tmp + N,
-             h->perm, h->orig, h->mask, h->cell_grp);
+  PermArgs P{h->pos4, h->vel4, h->frc4, h->mask, h->orig, h->rep, h->pos4b, h->vel4b, h->frc4b, h->maskb,
+             h->origb, h->repb, h->qpos, h->rdev};
   if (!permute) {
     // lists that are not due keep their internal indices: the search reads
     // the atoms of every cell through the cell-ordered index
+    PFM_LAUNCH(h, "cell_sort", k_cell_sort<false>, h->ncell, 64, h->ncell, h->cell_start, h->cell_tmp,
+               h->cell_fill, h->cell_count, h->scan_tmp + N, h->perm, h->orig, h->mask, h->cell_grp, P);
     h->bin = h->perm;
     return PFM_OK;
   }
-  PFM_LAUNCH(h, "permute", k_permute, nb, 256, N, h->perm, h->pos4, h->vel4, h->frc4, h->mask, h->orig, h->pos4b,
-             h->vel4b, h->frc4b, h->maskb, h->origb, h->qpos, h->rep, h->repb, h->rdev);
+  PFM_LAUNCH(h, "cell_sort", k_cell_sort<true>, h->ncell, 64, h->ncell, h->cell_start, h->cell_tmp, h->cell_fill,
+             h->cell_count, h->scan_tmp + N, h->perm, h->orig, h->mask, h->cell_grp, P);
   std::swap(h->pos4, h->pos4b);
   std::swap(h->vel4, h->vel4b);
   std::swap(h->frc4, h->frc4b);
