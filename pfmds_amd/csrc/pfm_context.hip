@@ -728,6 +728,7 @@ static int finalize(pfm_ctx* h) {
     R.hb = make_double3(0.5 * bx[0], 0.5 * bx[1], 0.5 * bx[2]);
     R.qs = make_double3(4294967296.0 / bx[0], 4294967296.0 / bx[1], 4294967296.0 / bx[2]);
     R.hq = make_float3((float)(bx[0] / 4294967296.0), (float)(bx[1] / 4294967296.0), (float)(bx[2] / 4294967296.0));
+    R.hiso = (R.hq.x == R.hq.y && R.hq.y == R.hq.z) ? R.hq.x : 0.0f;
     double cl[3];
     for (int d = 0; d < 3; ++d) {
       R.nc[d] = nc[d];

@@ -17,6 +17,7 @@
 //           (graphenenorm.f90:38-56, LennardJonesCosine.f90, MorseCosine.f90)
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "pfm_internal.h"
 
@@ -28,6 +29,7 @@ struct RowArgs {
   const double4* pos;  // FP64 positions
   const uint4* qpos;   // fixed-point positions (mixed path)
   float3 hq;           // L / 2^32 per axis
+  float hiso;          // L / 2^32 when the box is a cube, else 0 (one replica; else RepDev::hiso)
   double3 hb, L, iL;
   const int* rep;      // replica of every internal atom (null: one replica, geometry above)
   const RepDev* rdev;
@@ -208,6 +210,27 @@ __device__ __forceinline__ void write_force(const RowArgs& a, int i, double fx, 
 
 // ------------------------------------------------------------------- RGL
 
+// Mixed-path dr of the RGL passes and its square.  Any box: the fixed-point
+// differences scaled per axis (Geo<float>::d2).  ISO (the row's replica is a
+// cube, h = L / 2^32 on every axis): the unscaled differences and
+// h^2 (ux^2 + uy^2 + uz^2), two multiplies fewer; dr = h (ux, uy, uz), so
+// a pass that sums c dr sums c u and multiplies the row's sum by h.  The
+// choice is the replica's own, so a replica takes the same arithmetic in a
+// batch as alone.
+__device__ __forceinline__ float hiso_of(const RowArgs& a, int i) { return a.rep ? a.rdev[a.rep[i]].hiso : a.hiso; }
+template <bool ISO>
+__device__ __forceinline__ float rjl_d2(const Geo<float>& G, const uint4& a, const uint4& b, float h2, float& dx,
+                                        float& dy, float& dz) {
+  if constexpr (ISO) {
+    dx = (float)(int)(b.x - a.x);
+    dy = (float)(int)(b.y - a.y);
+    dz = (float)(int)(b.z - a.z);
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) * h2;
+  } else {
+    return G.d2(a, b, dx, dy, dz);
+  }
+}
+
 struct RjlP {
   double A0, xi, p, q, r0, R1, R2, invw;
 };
@@ -227,15 +250,21 @@ __device__ __forceinline__ void rjl_density_rows(const RowArgs& a, const RjlP& P
     // exp(-2q(r/r0 - 1)) = 2^(r kr + k0); cutoff argument in revolutions as in cutoff_bl<float>
     const float R2 = P.R2, ca = 0.5 * P.invw, cb = -0.5 * P.R1 * P.invw;
     const float kr = PM<float>::kexp(-2.0 * P.q / P.r0), k0 = PM<float>::kexp(2.0 * P.q);
-    RowAcc<float> acc;
-    walk_row<T, float>(G, nbrow, n, t, i, [&](int, const uint4& q, bool valid) {
-      float dx, dy, dz;
-      const float r = __builtin_amdgcn_sqrtf(G.d2(pi, q, dx, dy, dz));
-      const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fmed3f(__builtin_fmaf(r, ca, cb), 0.0f, 0.5f));
-      const float e = __builtin_amdgcn_exp2f(__builtin_fmaf(r, kr, k0)) * __builtin_fmaf(0.5f, c, 0.5f);
-      acc.energy(valid && r <= R2 ? e : 0.0f);
-    }, [&] { acc.flush(); });
-    rho = acc.e;
+    const float hiso = hiso_of(a, i);
+    auto body = [&](auto iso) {
+      constexpr bool ISO = decltype(iso)::value;
+      const float h2 = hiso * hiso;
+      RowAcc<float> acc;
+      walk_row<T, float>(G, nbrow, n, t, i, [&](int, const uint4& q, bool valid) {
+        float dx, dy, dz;
+        const float r = __builtin_amdgcn_sqrtf(rjl_d2<ISO>(G, pi, q, h2, dx, dy, dz));
+        const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fmed3f(__builtin_fmaf(r, ca, cb), 0.0f, 0.5f));
+        const float e = __builtin_amdgcn_exp2f(__builtin_fmaf(r, kr, k0)) * __builtin_fmaf(0.5f, c, 0.5f);
+        acc.energy(valid && r <= R2 ? e : 0.0f);
+      }, [&] { acc.flush(); });
+      return acc.e;
+    };
+    rho = hiso > 0.0f ? body(std::true_type{}) : body(std::false_type{});
   } else {
     const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
     const R kq = PM<R>::kexp(-2.0 * P.q);
@@ -281,26 +310,35 @@ __device__ __forceinline__ void rjl_force_rows(const RowArgs& a, const RjlP& P, 
     const float kpr = PM<float>::kexp(-P.p / P.r0), kp0 = PM<float>::kexp(P.p);
     const float kqr = PM<float>::kexp(-2.0 * P.q / P.r0), kq0 = PM<float>::kexp(2.0 * P.q);
     const float ie = Geo<float>::w(pi);
-    RowAcc<float> acc;
-    walk_row<T, float>(G, nbrow, n, t, i, [&](int, const uint4& q, bool valid) {
-      float dx, dy, dz, r, ir;
-      dist_and_inverse<float>(G.d2(pi, q, dx, dy, dz), r, ir);
-      const float x = __builtin_amdgcn_fmed3f(__builtin_fmaf(r, ca, cb), 0.0f, 0.5f);
-      const float sn = __builtin_amdgcn_sinf(x), cs = __builtin_amdgcn_cosf(x);
-      const float ep = __builtin_amdgcn_exp2f(__builtin_fmaf(r, kpr, kp0));
-      const float eq = __builtin_amdgcn_exp2f(__builtin_fmaf(r, kqr, kq0));
-      const float A = __builtin_fmaf(As, sn, __builtin_fmaf(Ac, cs, Ac));
-      const float B = __builtin_fmaf(Bs, sn, __builtin_fmaf(Bc, cs, Bc));
-      const bool in = valid && r <= R2;
-      float c = __builtin_fmaf(A, ep, -(B * (ie + Geo<float>::w(q)) * eq)) * ir;
-      c = in ? c : 0.0f;
-      acc.force(c, dx, dy, dz);
-      if (E) {
-        const float e = ep * __builtin_fmaf(0.5f, cs, 0.5f);
-        acc.energy(in ? e : 0.0f);
-      }
-    }, [&] { acc.flush(); });
-    fx = acc.fx; fy = acc.fy; fz = acc.fz; er = acc.e;
+    const float hiso = hiso_of(a, i);
+    auto body = [&](auto iso) {
+      constexpr bool ISO = decltype(iso)::value;
+      const float h2 = hiso * hiso;
+      RowAcc<float> acc;
+      walk_row<T, float>(G, nbrow, n, t, i, [&](int, const uint4& q, bool valid) {
+        float dx, dy, dz, r, ir;
+        dist_and_inverse<float>(rjl_d2<ISO>(G, pi, q, h2, dx, dy, dz), r, ir);
+        const float x = __builtin_amdgcn_fmed3f(__builtin_fmaf(r, ca, cb), 0.0f, 0.5f);
+        const float sn = __builtin_amdgcn_sinf(x), cs = __builtin_amdgcn_cosf(x);
+        const float ep = __builtin_amdgcn_exp2f(__builtin_fmaf(r, kpr, kp0));
+        const float eq = __builtin_amdgcn_exp2f(__builtin_fmaf(r, kqr, kq0));
+        const float A = __builtin_fmaf(As, sn, __builtin_fmaf(Ac, cs, Ac));
+        const float B = __builtin_fmaf(Bs, sn, __builtin_fmaf(Bc, cs, Bc));
+        const bool in = valid && r <= R2;
+        float c = __builtin_fmaf(A, ep, -(B * (ie + Geo<float>::w(q)) * eq)) * ir;
+        c = in ? c : 0.0f;
+        acc.force(c, dx, dy, dz);
+        if (E) {
+          const float e = ep * __builtin_fmaf(0.5f, cs, 0.5f);
+          acc.energy(in ? e : 0.0f);
+        }
+      }, [&] { acc.flush(); });
+      // with ISO the sums are of c u: dr = h u
+      const double hs = ISO ? (double)hiso : 1.0;
+      fx = acc.fx * hs; fy = acc.fy * hs; fz = acc.fz * hs; er = acc.e;
+    };
+    if (hiso > 0.0f) body(std::true_type{});
+    else body(std::false_type{});
   } else {
     const R R1 = P.R1, R2 = P.R2, invw = P.invw, ir0 = 1.0 / P.r0, kdf = -0.5 * PFM_PI * P.invw;
     const R kp = PM<R>::kexp(-P.p), kq = PM<R>::kexp(-2.0 * P.q);
@@ -1038,6 +1076,7 @@ static RowArgs row_args(pfm_ctx* h, Inter& it, int k, int accumulate) {
   a.iL = make_double3(1.0 / h->box[0], 1.0 / h->box[1], 1.0 / h->box[2]);
   a.rep = h->rep;
   a.rdev = h->rdev;
+  a.hiso = (a.hq.x == a.hq.y && a.hq.y == a.hq.z) ? a.hq.x : 0.0f;
   a.frc = h->frc4;
   a.accumulate = accumulate;
   a.e_row = it.e_row;

@@ -258,6 +258,7 @@ struct RepDev {
   double3 L, iL, hb;    // box, 1/box, box/2
   double3 qs;           // 2^32 / L
   float3 hq;            // L / 2^32
+  float hiso;           // hq.x when the box is a cube (hq equal on every axis), else 0
   double3 clen, inv_clen;
   int nc[3];
   int cell_base;        // global rank (and linear index) of the grid's first cell
