@@ -850,6 +850,45 @@ static int finalize(pfm_ctx* h) {
         if (in1[x]) { it.rows_disjoint = false; break; }
     }
   }
+  // pair searches (ListDev::pair_*): lj / lj1g lists in build order, each
+  // paired with the first later unpaired one of the same rows, cutoff and
+  // period when both are searched one wave per cell with over eight entries
+  {
+    struct Cand { int ii, k; };
+    std::vector<Cand> order;
+    for (size_t ii = 0; ii < h->inter.size(); ++ii) {
+      Inter& it = h->inter[ii];
+      if (it.kind != PFM_LJ && it.kind != PFM_LJ1G) continue;
+      order.push_back({(int)ii, 0});
+      if (has_converse(it.kind)) order.push_back({(int)ii, 1});
+    }
+    auto rc_of = [&](const Cand& c) { Inter& it = h->inter[c.ii]; return c.k == 1 ? it.nl[0].rcut : it.nl[c.k].rcut; };
+    auto per_of = [&](const Cand& c) { Inter& it = h->inter[c.ii]; return c.k == 1 ? it.nl[0].period : it.nl[c.k].period; };
+    for (auto& c : order) {
+      ListDev& L = h->inter[c.ii].nl[c.k];
+      L.pair_ii = L.pair_k = -1;
+      L.pair_lead = L.pair_sparse = false;
+      L.done_step = -1;
+    }
+    for (size_t x = 0; x < order.size(); ++x) {
+      ListDev& A = h->inter[order[x].ii].nl[order[x].k];
+      if (A.pair_ii >= 0 || !A.sparse_cells || A.nmax <= 8) continue;
+      for (size_t y = x + 1; y < order.size(); ++y) {
+        ListDev& B = h->inter[order[y].ii].nl[order[y].k];
+        if (B.pair_ii >= 0 || !B.sparse_cells || B.nmax <= 8 || B.g1 != A.g1 || rc_of(order[y]) != rc_of(order[x]) ||
+            per_of(order[y]) != per_of(order[x]))
+          continue;
+        A.pair_ii = order[y].ii; A.pair_k = order[y].k; A.pair_lead = true;
+        B.pair_ii = order[x].ii; B.pair_k = order[x].k;
+        std::vector<char> in(h->N, 0);
+        long nu = 0;
+        for (int v : h->grp.idx[A.g2]) if (!in[v]) { in[v] = 1; ++nu; }
+        for (int v : h->grp.idx[B.g2]) if (!in[v]) { in[v] = 1; ++nu; }
+        A.pair_sparse = 2 * nu < (long)h->N;
+        break;
+      }
+    }
+  }
   // graphene on a metal: exactly rjl, tb and a ljc/morsec fed by that tb
   // list, with the rjl rows, the graphene rows and the metal rows of
   // ljc/morsec sharing no atom (the passes of each side may then share
@@ -1164,6 +1203,18 @@ static void spec_record(pfm_ctx* h, int k) {
 
 // rebuild every list due at step s, in settings order; re-sort the atoms
 // first when all of them are due (step 0 always)
+// list k of interaction ii, with its pair when it leads one; a list its
+// lead built at this step is skipped
+static int build_or_pair(pfm_ctx* h, int ii, int k, int s) {
+  ListDev& L = h->inter[ii].nl[k];
+  if (L.pair_ii >= 0 && !L.pair_lead && L.done_step == s) return PFM_OK;
+  if (L.pair_lead) {
+    h->inter[L.pair_ii].nl[L.pair_k].done_step = s;
+    return launch_build_pair(h, ii, k, L.pair_ii, L.pair_k, s);
+  }
+  return launch_build_list(h, ii, k, s);
+}
+
 static int update_lists(pfm_ctx* h, int s) {
   bool all_due = true;
   for (auto& it : h->inter)
@@ -1194,10 +1245,10 @@ static int update_lists(pfm_ctx* h, int s) {
     Inter& it = h->inter[ii];
     bool due = s % it.nl[0].period == 0 || !it.nl[0].built;
     if (due) {
-      if ((rc = launch_build_list(h, (int)ii, 0, s))) return rc;
+      if ((rc = build_or_pair(h, (int)ii, 0, s))) return rc;
       if (it.kind == PFM_SUR) launch_sur_converse(h, it, s);
       else if (has_converse(it.kind))
-        if ((rc = launch_build_list(h, (int)ii, 1, s))) return rc;
+        if ((rc = build_or_pair(h, (int)ii, 1, s))) return rc;
     }
     if (is_cos(it.kind)) {
       if (it.nn_from_tb >= 0) {

@@ -360,6 +360,13 @@ struct ListDev {
   int* rev = nullptr;   // [nrows*stride] slot of the row atom in the neighbour's row (tb, nn lists)
   bool built = false;
   bool sparse_cells = false;  // the replicas holding its rows average few atoms per cell: one wave per cell
+  // Pair search: an lj / lj1g list and another with the same rows, cutoff and
+  // rebuild steps, both searched one wave per cell, are built in one pass
+  // (launch_build_pair) by the earlier of the two in build order (the lead)
+  int pair_ii = -1, pair_k = -1;
+  bool pair_lead = false;
+  bool pair_sparse = false;  // the union of the two candidate groups is under half of the atoms
+  int done_step = -1;        // the step whose update built this list with its lead
 };
 
 // device buffers of a surrogate interaction (pfm_surrogate.hip)
@@ -614,6 +621,7 @@ static inline int pad8(int nb) { return ((nb + 7) / 8) * 8; }
 // kernels' host launchers
 int launch_sort_and_bin(pfm_ctx* h, int step, bool permute);
 int launch_build_list(pfm_ctx* h, int ii, int k, int step);
+int launch_build_pair(pfm_ctx* h, int ii, int k, int jj, int kk, int step);
 int launch_nn_from_tb(pfm_ctx* h, int ii, int step);
 int launch_forces(pfm_ctx* h, bool energy, bool first_writes);
 // the shared lj / lj1g / rjl launches can write every atom's force first

@@ -382,6 +382,13 @@ struct SearchArgs {
   DevErr* err;
   int step, err_code;
   int exact;  // require exactly nmax entries (graphene nearest neighbours)
+  // a second list with the same rows, cutoff and rebuild steps searched in
+  // the same pass (k_build_tile_wave<T, CMP, true>): g2bit is then the union
+  // of the two candidate groups, g2bit1 / g2bit2 the groups of the two lists
+  uint32_t g2bit1, g2bit2;
+  int nmax2, stride2;
+  int* nnum2;
+  int* nbr2;
 };
 
 // T lanes per row atom scan the 27 (or fewer, for boxes under three cells
@@ -601,6 +608,7 @@ __device__ __forceinline__ void empty_row(const SearchArgs& a, const int* __rest
   const int row = row_of[i];
   if (row < 0) return;
   a.nnum[row] = 0;
+  if (a.nnum2) a.nnum2[row] = 0;
   if ((a.mask[i] & a.g1bit) && a.exact && a.nmax != 0) raise_err(a.err, PFM_E_GRAPHENE_NN, a.orig[i] + 1, 0, a.step);
 }
 
@@ -842,7 +850,8 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int T, bool CMP>
+#define ENTRY_IDX 0x3fffffff  // staged index without the pair search's group bits
+template <int T, bool CMP, bool PAIR>
 __global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const int* __restrict__ rank_xyz,
                                                          const int* __restrict__ row_of, int ncell,
                                                          int nblocks_real) {
@@ -935,7 +944,14 @@ __global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const 
           if (o0 < 0 && x > 0.0) x -= boxL.x; else if (o0 > 0 && x < 0.0) x += boxL.x;
           if (o1 < 0 && y > 0.0) y -= boxL.y; else if (o1 > 0 && y < 0.0) y += boxL.y;
           if (o2 < 0 && z > 0.0) z -= boxL.z; else if (o2 > 0 && z < 0.0) z += boxL.z;
-          v[u].w = __int_as_float(j);
+          if constexpr (PAIR) {
+            // bit 30: in list 1's candidate group, bit 31: in list 2's
+            const uint32_t mj = a.mask[j];
+            v[u].w = __uint_as_float((uint32_t)j | ((mj & a.g2bit1) ? 0x40000000u : 0u) |
+                                     ((mj & a.g2bit2) ? 0x80000000u : 0u));
+          } else {
+            v[u].w = __int_as_float(j);
+          }
           if (keep[u]) { v[u].x = (float)x; v[u].y = (float)y; v[u].z = (float)z; }
         }
       }
@@ -975,7 +991,28 @@ __global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const 
     const double4 pi = a.pos[i];
     const float px = (float)(pi.x - org[0]), py = (float)(pi.y - org[1]), pz = (float)(pi.z - org[2]);
     int* out = a.nbr + (size_t)(active ? row : 0) * a.stride;
-    int count = 0;
+    int* out2 = PAIR ? a.nbr2 + (size_t)(active ? row : 0) * a.stride2 : nullptr;
+    int count = 0, count2 = 0;
+    // the row's hits of one round into one list: offsets from a prefix sum
+    // over the row's lanes, entries without the group bits
+    auto put = [&](unsigned hit, const float4* sv, int* o, int nmax, int& cnt) {
+      const int nh = __popc(hit);
+      int pre = nh;
+#pragma unroll
+      for (int d = 1; d < T; d <<= 1) {
+        int w = __shfl_up(pre, d, T);
+        if (t >= d) pre += w;
+      }
+      const int tot = __shfl(pre, T - 1, T);
+      int slot = cnt + pre - nh;
+      while (hit) {
+        const int b = __clz(hit);
+        hit &= ~(0x80000000u >> b);
+        if (slot < nmax) o[slot] = __float_as_int(sv[b * T].w) & ENTRY_IDX;
+        ++slot;
+      }
+      cnt += tot;
+    };
     for (int s0 = 0; s0 < R; s0 += WTILE) {
       if (!single) {
         wave_sync();
@@ -988,6 +1025,7 @@ __global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const 
       for (int c0 = 0; c0 < np; c0 += 32 * T) {
         const float4* sv = st + c0 + t;
         unsigned miss = 0u, inside = 0u;  // as in k_build_tile
+        unsigned m1 = 0u, m2 = 0u;        // PAIR: candidates in list 1's / list 2's group
 #pragma unroll 4
         for (int b = 0; b < 32; ++b) {
           const float4 w = sv[b * T];
@@ -995,6 +1033,11 @@ __global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const 
           const float m = fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, hi)));
           miss = __builtin_amdgcn_alignbit(miss, __float_as_uint(m), 31);
           inside = __builtin_amdgcn_alignbit(inside, __float_as_uint(win - m), 31);
+          if constexpr (PAIR) {
+            const uint32_t wb = __float_as_uint(w.w);
+            m1 = __builtin_amdgcn_alignbit(m1, wb << 1, 31);
+            m2 = __builtin_amdgcn_alignbit(m2, wb, 31);
+          }
         }
         unsigned hit = ~miss;
         const int ob = own - c0 - t;
@@ -1004,7 +1047,7 @@ __global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const 
           const int b = __clz(chk);
           chk &= ~(0x80000000u >> b);
           const float4 w = sv[b * T];
-          const int j = __float_as_int(w.w);
+          const int j = __float_as_int(w.w) & ENTRY_IDX;
           float dx = w.x - px, dy = w.y - py, dz = w.z - pz;
           if (CMP && !dense && j == i) {
             hit &= ~(0x80000000u >> b);
@@ -1013,29 +1056,26 @@ __global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const 
             if (!(min_image_dist2(pi, a.pos[j], hb, ex, ey, ez) < a.rc2)) hit &= ~(0x80000000u >> b);
           }
         }
-        const int nh = __popc(hit);
-        int pre = nh;
-#pragma unroll
-        for (int o = 1; o < T; o <<= 1) {
-          int w = __shfl_up(pre, o, T);
-          if (t >= o) pre += w;
+        if constexpr (PAIR) {
+          put(hit & m1, sv, out, a.nmax, count);
+          put(hit & m2, sv, out2, a.nmax2, count2);
+        } else {
+          put(hit, sv, out, a.nmax, count);
         }
-        const int tot = __shfl(pre, T - 1, T);
-        int slot = count + pre - nh;
-        while (hit) {
-          const int b = __clz(hit);
-          hit &= ~(0x80000000u >> b);
-          if (slot < a.nmax) out[slot] = __float_as_int(sv[b * T].w);
-          ++slot;
-        }
-        count += tot;
       }
     }
-    if (row >= 0 && !active && t == 0) a.nnum[row] = 0;
+    if (row >= 0 && !active && t == 0) {
+      a.nnum[row] = 0;
+      if (PAIR) a.nnum2[row] = 0;
+    }
     if (active && t == 0) {
       a.nnum[row] = count < a.nmax ? count : a.nmax;
       if (count > a.nmax) raise_err(a.err, a.err_code, a.orig[i] + 1, count, a.step);
       else if (a.exact && count != a.nmax) raise_err(a.err, PFM_E_GRAPHENE_NN, a.orig[i] + 1, count, a.step);
+      if (PAIR) {
+        a.nnum2[row] = count2 < a.nmax2 ? count2 : a.nmax2;
+        if (count2 > a.nmax2) raise_err(a.err, a.err_code, a.orig[i] + 1, count2, a.step);
+      }
     }
   }
 }
@@ -1051,11 +1091,11 @@ static void launch_build(pfm_ctx* h, SearchArgs& a, const int* row_of, bool spar
   if (h->any_tile && sparse_cells) {
     const int nc = h->ncell, nb = (nc + 3) / 4;
     if (sparse) {
-      if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", (k_build_tile_wave<4, true>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
-      else PFM_LAUNCH(h, "build_list", (k_build_tile_wave<8, true>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
+      if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", (k_build_tile_wave<4, true, false>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
+      else PFM_LAUNCH(h, "build_list", (k_build_tile_wave<8, true, false>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
     } else {
-      if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", (k_build_tile_wave<4, false>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
-      else PFM_LAUNCH(h, "build_list", (k_build_tile_wave<8, false>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
+      if (a.nmax <= 8) PFM_LAUNCH(h, "build_list", (k_build_tile_wave<4, false, false>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
+      else PFM_LAUNCH(h, "build_list", (k_build_tile_wave<8, false, false>), pad8(nb), 256, a, h->cell_xyz, row_of, nc, nb);
     }
   } else if (h->any_tile) {
     int nb = h->ncell;
@@ -1077,11 +1117,11 @@ static void launch_build(pfm_ctx* h, SearchArgs& a, const int* row_of, bool spar
 // converse of list 0: same r_cut and schedule (md_interactions.f90:152-155),
 // found by searching from the group-2 side, which gives the same set because
 // find_distance's dr2 is symmetric in its two atoms.
-int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
+// the search arguments of list k of interaction ii
+static SearchArgs search_args(pfm_ctx* h, int ii, int k, int step) {
   Inter& it = h->inter[ii];
   ListDev& L = it.nl[k];
-  if (L.nrows == 0) { L.built = true; return PFM_OK; }
-  SearchArgs a;
+  SearchArgs a{};
   a.nrows = L.nrows;
   a.rows = h->grp.rows[L.g1];
   a.g1bit = 1u << (L.g1 - 1);
@@ -1110,9 +1150,14 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
   a.step = step - h->step_base;
   a.err_code = PFM_E_OVERFLOW;
   a.exact = (k == 2) ? 1 : 0;
-  const int* row_of = h->grp.row_of[L.g1];
-  // candidates of another species: stage only those
-  launch_build(h, a, row_of, 2L * h->grp.size[L.g2] < (long)h->N, L.sparse_cells);
+  return a;
+}
+
+// what follows a list's search: tb reverse slots, the ljc/morsec metal work
+// list, lj1g half-list tags
+static void list_post(pfm_ctx* h, int ii, int k, int step) {
+  Inter& it = h->inter[ii];
+  ListDev& L = it.nl[k];
   int nb = (L.nrows + 255) / 256;
   if (it.kind == PFM_TB || it.kind == PFM_REBOSC || (k == 2 && !(int)it.p[it.kind == PFM_LJC ? 5 : 6])) {
     PFM_LAUNCH(h, "list_rev", k_rev, nb, 256, L.nrows, h->grp.rows[L.g1], h->grp.row_of[L.g1], L.nnum, L.nbr,
@@ -1130,6 +1175,48 @@ int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
                L.nnum, L.nbr, L.stride, it.lc);
   }
   L.built = true;
+}
+
+int launch_build_list(pfm_ctx* h, int ii, int k, int step) {
+  ListDev& L = h->inter[ii].nl[k];
+  if (L.nrows == 0) { L.built = true; return PFM_OK; }
+  SearchArgs a = search_args(h, ii, k, step);
+  // candidates of another species: stage only those
+  launch_build(h, a, h->grp.row_of[L.g1], 2L * h->grp.size[L.g2] < (long)h->N, L.sparse_cells);
+  list_post(h, ii, k, step);
+  return PFM_OK;
+}
+
+// Two lists of one pair (ListDev::pair): one wavefront-per-cell pass stages
+// the union of their candidate groups once and writes both (replicas that
+// take the row search run it per list).
+int launch_build_pair(pfm_ctx* h, int ii, int k, int jj, int kk, int step) {
+  ListDev& L1 = h->inter[ii].nl[k];
+  ListDev& L2 = h->inter[jj].nl[kk];
+  SearchArgs a = search_args(h, ii, k, step);
+  SearchArgs b = search_args(h, jj, kk, step);
+  const int* row_of = h->grp.row_of[L1.g1];
+  if (h->any_tile && L1.nrows > 0) {
+    SearchArgs p = a;
+    p.g2bit1 = a.g2bit;
+    p.g2bit2 = b.g2bit;
+    p.g2bit = a.g2bit | b.g2bit;
+    p.nmax2 = b.nmax;
+    p.stride2 = b.stride;
+    p.nnum2 = b.nnum;
+    p.nbr2 = b.nbr;
+    const int nc = h->ncell, nb = (nc + 3) / 4;
+    if (L1.pair_sparse)
+      PFM_LAUNCH(h, "build_list", (k_build_tile_wave<8, true, true>), pad8(nb), 256, p, h->cell_xyz, row_of, nc, nb);
+    else
+      PFM_LAUNCH(h, "build_list", (k_build_tile_wave<8, false, true>), pad8(nb), 256, p, h->cell_xyz, row_of, nc, nb);
+  }
+  if (h->any_rows) {
+    if (L1.nrows > 0) launch_build_T<8>(h, a, row_of);
+    if (L2.nrows > 0) launch_build_T<8>(h, b, row_of);
+  }
+  list_post(h, ii, k, step);
+  list_post(h, jj, kk, step);
   return PFM_OK;
 }
 

@@ -115,6 +115,37 @@ def test_batch_replicas_match_oracle():
         np.testing.assert_allclose(er["ke"][r], e["ke"], rtol=1e-9)
 
 
+def test_pair_search_compacted_staging_against_oracle():
+    """Two sparse gases (under 14 atoms a cell: one wavefront per cell) beside
+    a larger Cu crystal: lj1g A and lj A->B (and lj1g B with lj B->A) are
+    searched in one pass, and the union of their candidate groups is under
+    half of the atoms, so the pass stages them compacted.  Forces at step 0
+    and the state after 30 steps against the oracle per replica (FP64)."""
+    reps = [configs.lj_gas_lattice(600, 0.01, 31, 150.0, steps=STEPS),
+            configs.lj_gas_lattice(500, 0.012, 32, 100.0, steps=STEPS),
+            configs.fcc_cu_rgl(nx=7, seed=33, steps=STEPS, dt=0.5)]
+    b = system.merge_replicas(reps)
+    assert 2 * (reps[0].n + reps[1].n) < b.system.n
+    g = system.build_batch_context(b, capi.FP64)
+    g.run("nve", 0.5, 0, 0, True)
+    f0 = g.state()[2]
+    for r, s in enumerate(reps):
+        o = oracle_ctx(s)
+        o.run("nve", 0.5, 0, 0, True)
+        print(f"[pair search] replica {r}: {s.n} atoms, step 0", flush=True)
+        assert rel_inf(f0[b.replica_slice(r)], o.state()[2]) < 1e-10
+    g.run("nve", 0.5, 1, STEPS, True)
+    pos, vel, frc = g.state()
+    for r, s in enumerate(reps):
+        o = oracle_ctx(s)
+        o.run("nve", 0.5, 0, STEPS, True)
+        p1, v1, f1 = o.state()
+        sl = b.replica_slice(r)
+        assert rel_inf(pos[sl], p1) < 1e-10
+        assert rel_inf(vel[sl], v1) < 1e-8
+        assert rel_inf(frc[sl], f1) < 1e-8
+
+
 def per_replica_force_sums(b, frc):
     out = []
     for r in range(b.n_rep):
