@@ -869,6 +869,8 @@ static int finalize(pfm_ctx* h) {
       L.pair_ii = L.pair_k = -1;
       L.pair_lead = L.pair_sparse = false;
       L.done_step = -1;
+      L.quad_b1i = L.quad_b1k = L.quad_b2i = L.quad_b2k = -1;
+      L.quad_done = -1;
     }
     for (size_t x = 0; x < order.size(); ++x) {
       ListDev& A = h->inter[order[x].ii].nl[order[x].k];
@@ -885,6 +887,38 @@ static int finalize(pfm_ctx* h) {
         for (int v : h->grp.idx[A.g2]) if (!in[v]) { in[v] = 1; ++nu; }
         for (int v : h->grp.idx[B.g2]) if (!in[v]) { in[v] = 1; ++nu; }
         A.pair_sparse = 2 * nu < (long)h->N;
+        break;
+      }
+    }
+    // quads: two pairs (leads P and Q, P first) whose row groups share no
+    // atom and whose lists take the same two candidate groups, with the same
+    // cutoff, period and staging
+    std::vector<Cand> leads;
+    for (auto& c : order)
+      if (h->inter[c.ii].nl[c.k].pair_lead) leads.push_back(c);
+    for (size_t x = 0; x < leads.size(); ++x) {
+      ListDev& P = h->inter[leads[x].ii].nl[leads[x].k];
+      if (!P.pair_lead || P.quad_b1i >= 0) continue;
+      ListDev& Pp = h->inter[P.pair_ii].nl[P.pair_k];
+      for (size_t y = x + 1; y < leads.size(); ++y) {
+        ListDev& Q = h->inter[leads[y].ii].nl[leads[y].k];
+        if (!Q.pair_lead || Q.quad_b1i >= 0) continue;
+        ListDev& Qp = h->inter[Q.pair_ii].nl[Q.pair_k];
+        if (rc_of(leads[y]) != rc_of(leads[x]) || per_of(leads[y]) != per_of(leads[x]) ||
+            Q.pair_sparse != P.pair_sparse || P.g2 == Pp.g2)
+          continue;
+        // Q's lists: the one on P's candidate group and the one on P's partner's
+        Cand q1{leads[y].ii, leads[y].k}, q2{Q.pair_ii, Q.pair_k};
+        if (Q.g2 == Pp.g2 && Qp.g2 == P.g2) std::swap(q1, q2);
+        else if (!(Q.g2 == P.g2 && Qp.g2 == Pp.g2)) continue;
+        std::vector<char> in(h->N, 0);
+        bool disjoint = true;
+        for (int v : h->grp.idx[P.g1]) in[v] = 1;
+        for (int v : h->grp.idx[Q.g1]) disjoint = disjoint && !in[v];
+        if (!disjoint) continue;
+        P.quad_b1i = q1.ii; P.quad_b1k = q1.k;
+        P.quad_b2i = q2.ii; P.quad_b2k = q2.k;
+        Q.pair_lead = false;  // built in P's pass
         break;
       }
     }
@@ -1207,9 +1241,14 @@ static void spec_record(pfm_ctx* h, int k) {
 // lead built at this step is skipped
 static int build_or_pair(pfm_ctx* h, int ii, int k, int s) {
   ListDev& L = h->inter[ii].nl[k];
-  if (L.pair_ii >= 0 && !L.pair_lead && L.done_step == s) return PFM_OK;
+  if (L.pair_ii >= 0 && !L.pair_lead && (L.done_step == s || L.quad_done == s)) return PFM_OK;
   if (L.pair_lead) {
     h->inter[L.pair_ii].nl[L.pair_k].done_step = s;
+    if (L.quad_b1i >= 0) {
+      h->inter[L.quad_b1i].nl[L.quad_b1k].quad_done = s;
+      h->inter[L.quad_b2i].nl[L.quad_b2k].quad_done = s;
+      return launch_build_quad(h, ii, k, L.pair_ii, L.pair_k, L.quad_b1i, L.quad_b1k, L.quad_b2i, L.quad_b2k, s);
+    }
     return launch_build_pair(h, ii, k, L.pair_ii, L.pair_k, s);
   }
   return launch_build_list(h, ii, k, s);

@@ -367,6 +367,11 @@ struct ListDev {
   bool pair_lead = false;
   bool pair_sparse = false;  // the union of the two candidate groups is under half of the atoms
   int done_step = -1;        // the step whose update built this list with its lead
+  // Quad search (a pair's lead only): a second pair whose rows are another
+  // group and whose lists take the same two candidate groups, built in the
+  // same pass; b1 takes this lead's candidate group, b2 its partner's
+  int quad_b1i = -1, quad_b1k = -1, quad_b2i = -1, quad_b2k = -1;
+  int quad_done = -1;  // the step whose update built this list in a quad lead's pass
 };
 
 // device buffers of a surrogate interaction (pfm_surrogate.hip)
@@ -622,6 +627,7 @@ static inline int pad8(int nb) { return ((nb + 7) / 8) * 8; }
 int launch_sort_and_bin(pfm_ctx* h, int step, bool permute);
 int launch_build_list(pfm_ctx* h, int ii, int k, int step);
 int launch_build_pair(pfm_ctx* h, int ii, int k, int jj, int kk, int step);
+int launch_build_quad(pfm_ctx* h, int ii, int k, int jj, int kk, int b1i, int b1k, int b2i, int b2k, int step);
 int launch_nn_from_tb(pfm_ctx* h, int ii, int step);
 int launch_forces(pfm_ctx* h, bool energy, bool first_writes);
 // the shared lj / lj1g / rjl launches can write every atom's force first

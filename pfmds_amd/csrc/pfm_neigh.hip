@@ -389,6 +389,14 @@ struct SearchArgs {
   int nmax2, stride2;
   int* nnum2;
   int* nbr2;
+  // a second pair in the same pass (k_build_tile_wave<T, CMP, true, true>):
+  // rows of another group (side b, disjoint from group 1), whose two lists
+  // take the candidates of g2bit1 (b1) and of g2bit2 (b2); g1mem is then
+  // the union of the two row groups' member bits
+  uint32_t g1bit_b, g1mem_b;
+  const int* row_of_b;
+  int nmax_b1, stride_b1, nmax_b2, stride_b2;
+  int *nnum_b1, *nbr_b1, *nnum_b2, *nbr_b2;
 };
 
 // T lanes per row atom scan the 27 (or fewer, for boxes under three cells
@@ -851,7 +859,7 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 #define ENTRY_IDX 0x3fffffff  // staged index without the pair search's group bits
-template <int T, bool CMP, bool PAIR>
+template <int T, bool CMP, bool PAIR, bool QUAD = false>
 __global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const int* __restrict__ rank_xyz,
                                                          const int* __restrict__ row_of, int ncell,
                                                          int nblocks_real) {
@@ -890,7 +898,15 @@ __global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const 
     cand = (a.cell_grp[r] & a.g2bit) != 0;
   }
   if (!__any(cand)) {
-    for (int m = lane; m < ncen; m += 64) empty_row(a, row_of, a.bin ? a.bin[cs + m] : cs + m);
+    for (int m = lane; m < ncen; m += 64) {
+      const int i = a.bin ? a.bin[cs + m] : cs + m;
+      if (QUAD && (a.mask[i] & a.g1mem_b)) {
+        const int row = a.row_of_b[i];
+        if (row >= 0) { a.nnum_b1[row] = 0; a.nnum_b2[row] = 0; }
+      } else {
+        empty_row(a, row_of, i);
+      }
+    }
     return;
   }
   int inc = kcn;
@@ -986,12 +1002,20 @@ __global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const 
     const int qq = q0 + lane / T;
     const bool inrow = qq < ncen;
     const int i = a.bin ? a.bin[cs + (inrow ? qq : 0)] : cs + (inrow ? qq : 0);
-    const int row = inrow ? row_of[i] : -1;
-    const bool active = row >= 0 && (a.mask[i] & a.g1bit);
+    const uint32_t mi = a.mask[i];
+    // QUAD: rows of side b take its row index and its two lists
+    const bool sb = QUAD && (mi & a.g1mem_b) != 0;
+    const int row = inrow ? (sb ? a.row_of_b[i] : row_of[i]) : -1;
+    const bool active = row >= 0 && (mi & (sb ? a.g1bit_b : a.g1bit));
     const double4 pi = a.pos[i];
     const float px = (float)(pi.x - org[0]), py = (float)(pi.y - org[1]), pz = (float)(pi.z - org[2]);
-    int* out = a.nbr + (size_t)(active ? row : 0) * a.stride;
-    int* out2 = PAIR ? a.nbr2 + (size_t)(active ? row : 0) * a.stride2 : nullptr;
+    int* const nbr1 = sb ? a.nbr_b1 : a.nbr;
+    int* const nbr2 = sb ? a.nbr_b2 : a.nbr2;
+    int* const nnum1 = sb ? a.nnum_b1 : a.nnum;
+    int* const nnum2 = sb ? a.nnum_b2 : a.nnum2;
+    const int nmax1 = sb ? a.nmax_b1 : a.nmax, nmax2 = sb ? a.nmax_b2 : a.nmax2;
+    int* out = nbr1 + (size_t)(active ? row : 0) * (sb ? a.stride_b1 : a.stride);
+    int* out2 = PAIR ? nbr2 + (size_t)(active ? row : 0) * (sb ? a.stride_b2 : a.stride2) : nullptr;
     int count = 0, count2 = 0;
     // the row's hits of one round into one list: offsets from a prefix sum
     // over the row's lanes, entries without the group bits
@@ -1057,24 +1081,24 @@ __global__ void __launch_bounds__(256, 5) k_build_tile_wave(SearchArgs a, const 
           }
         }
         if constexpr (PAIR) {
-          put(hit & m1, sv, out, a.nmax, count);
-          put(hit & m2, sv, out2, a.nmax2, count2);
+          put(hit & m1, sv, out, nmax1, count);
+          put(hit & m2, sv, out2, nmax2, count2);
         } else {
           put(hit, sv, out, a.nmax, count);
         }
       }
     }
     if (row >= 0 && !active && t == 0) {
-      a.nnum[row] = 0;
-      if (PAIR) a.nnum2[row] = 0;
+      nnum1[row] = 0;
+      if (PAIR) nnum2[row] = 0;
     }
     if (active && t == 0) {
-      a.nnum[row] = count < a.nmax ? count : a.nmax;
-      if (count > a.nmax) raise_err(a.err, a.err_code, a.orig[i] + 1, count, a.step);
-      else if (a.exact && count != a.nmax) raise_err(a.err, PFM_E_GRAPHENE_NN, a.orig[i] + 1, count, a.step);
+      nnum1[row] = count < nmax1 ? count : nmax1;
+      if (count > nmax1) raise_err(a.err, a.err_code, a.orig[i] + 1, count, a.step);
+      else if (a.exact && count != nmax1) raise_err(a.err, PFM_E_GRAPHENE_NN, a.orig[i] + 1, count, a.step);
       if (PAIR) {
-        a.nnum2[row] = count2 < a.nmax2 ? count2 : a.nmax2;
-        if (count2 > a.nmax2) raise_err(a.err, a.err_code, a.orig[i] + 1, count2, a.step);
+        nnum2[row] = count2 < nmax2 ? count2 : nmax2;
+        if (count2 > nmax2) raise_err(a.err, a.err_code, a.orig[i] + 1, count2, a.step);
       }
     }
   }
@@ -1217,6 +1241,47 @@ int launch_build_pair(pfm_ctx* h, int ii, int k, int jj, int kk, int step) {
   }
   list_post(h, ii, k, step);
   list_post(h, jj, kk, step);
+  return PFM_OK;
+}
+
+// Two pairs in one pass (ListDev::quad_*): the lead pair (ii, k) + (jj, kk)
+// and a second pair with rows of another group whose lists take the same two
+// candidate groups, (b1 for the lead's group, b2 for its partner's).
+int launch_build_quad(pfm_ctx* h, int ii, int k, int jj, int kk, int b1i, int b1k, int b2i, int b2k, int step) {
+  ListDev& L1 = h->inter[ii].nl[k];
+  ListDev& B1 = h->inter[b1i].nl[b1k];
+  SearchArgs a = search_args(h, ii, k, step);
+  SearchArgs b = search_args(h, jj, kk, step);
+  SearchArgs c = search_args(h, b1i, b1k, step);
+  SearchArgs d = search_args(h, b2i, b2k, step);
+  const int* row_of = h->grp.row_of[L1.g1];
+  const int* row_of_b = h->grp.row_of[B1.g1];
+  if (h->any_tile && (L1.nrows > 0 || B1.nrows > 0)) {
+    SearchArgs p = a;
+    p.g2bit1 = a.g2bit;
+    p.g2bit2 = b.g2bit;
+    p.g2bit = a.g2bit | b.g2bit;
+    p.nmax2 = b.nmax; p.stride2 = b.stride; p.nnum2 = b.nnum; p.nbr2 = b.nbr;
+    p.g1bit_b = c.g1bit;
+    p.g1mem_b = c.g1mem;
+    p.g1mem = a.g1mem | c.g1mem;
+    p.row_of_b = row_of_b;
+    p.nmax_b1 = c.nmax; p.stride_b1 = c.stride; p.nnum_b1 = c.nnum; p.nbr_b1 = c.nbr;
+    p.nmax_b2 = d.nmax; p.stride_b2 = d.stride; p.nnum_b2 = d.nnum; p.nbr_b2 = d.nbr;
+    const int nc = h->ncell, nb = (nc + 3) / 4;
+    if (L1.pair_sparse)
+      PFM_LAUNCH(h, "build_list", (k_build_tile_wave<8, true, true, true>), pad8(nb), 256, p, h->cell_xyz, row_of, nc, nb);
+    else
+      PFM_LAUNCH(h, "build_list", (k_build_tile_wave<8, false, true, true>), pad8(nb), 256, p, h->cell_xyz, row_of, nc, nb);
+  }
+  if (h->any_rows) {
+    if (L1.nrows > 0) { launch_build_T<8>(h, a, row_of); launch_build_T<8>(h, b, row_of); }
+    if (B1.nrows > 0) { launch_build_T<8>(h, c, row_of_b); launch_build_T<8>(h, d, row_of_b); }
+  }
+  list_post(h, ii, k, step);
+  list_post(h, jj, kk, step);
+  list_post(h, b1i, b1k, step);
+  list_post(h, b2i, b2k, step);
   return PFM_OK;
 }
 

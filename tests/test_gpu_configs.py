@@ -117,9 +117,10 @@ def test_batch_replicas_match_oracle():
 
 def test_pair_search_compacted_staging_against_oracle():
     """Two sparse gases (under 14 atoms a cell: one wavefront per cell) beside
-    a larger Cu crystal: lj1g A and lj A->B (and lj1g B with lj B->A) are
-    searched in one pass, and the union of their candidate groups is under
-    half of the atoms, so the pass stages them compacted.  Forces at step 0
+    a larger Cu crystal: lj1g A and lj A->B, and lj1g B with lj B->A, are
+    searched in one pass (rows of A and of B, four lists), and the union of
+    their candidate groups is under half of the atoms, so the pass stages
+    them compacted.  Forces at step 0
     and the state after 30 steps against the oracle per replica (FP64)."""
     reps = [configs.lj_gas_lattice(600, 0.01, 31, 150.0, steps=STEPS),
             configs.lj_gas_lattice(500, 0.012, 32, 100.0, steps=STEPS),
