@@ -452,27 +452,51 @@ __global__ void __launch_bounds__(256) k_sur_mlp(int nrows, SurK K) {
   sur_mlp16(lane, ok, K.G + rr * NF, sfp, sW, K.e_row + rr, K.gG + rr * NF);
 }
 
-// dE_i/d(dr_ij) of every entry of the row (into drow) from dE/dG of the row
-// (g) and its moments (Mc, Mx: this lane's eta); returns the lane's share of
-// sum_j dE_i/d(dr_ij)
-template <class R>
-__device__ __forceinline__ void sur_back_walk(const SurK& K, const Geo<R>& Gm, const typename Geo<R>::P& pi,
-                                              const int* __restrict__ nbrow, int n, int i, int ke, int eg, float eta,
-                                              const float* e2k, const uint32_t* __restrict__ mask,
-                                              const float* g, const float* Mc, const float* Mx, float4* drow,
-                                              double* sf) {
+// the monomial adjoints of the row's angular sums for this lane's eta: ac
+// for carbon neighbours, ax for the others, from dE/dG of the row (g) and
+// its moments (Mc, Mx)
+__device__ __forceinline__ void sur_back_coef(const float* g, int ke, const float* Mc, const float* Mx, float* ac,
+                                              float* ax) {
   float gcc[4], gcx[4], dScc[5], dScx[5];
 #pragma unroll
   for (int m = 0; m < 4; ++m) { gcc[m] = g[24 + ke * 4 + m]; gcx[m] = g[32 + ke * 4 + m]; }
   g5_adj(gcc, dScc);
   g5_adj(gcx, dScx);
-  float ac[NMON], ax[NMON];
 #pragma unroll
   for (int t = 0; t < NMON; ++t) {
     float mcv = Mc[t], mxv = Mx[t];
     ac[t] = kMul[t] * (2.0f * mcv * dScc[kDeg[t]] + mxv * dScx[kDeg[t]]);
     ax[t] = kMul[t] * mcv * dScx[kDeg[t]];
   }
+}
+
+// the adjoints held in registers (k_sur_back) ...
+struct SurCoefReg {
+  const float* ac;
+  const float* ax;
+  __device__ __forceinline__ float operator()(bool cj, int t) const { return cj ? ac[t] : ax[t]; }
+};
+// ... or in LDS (k_sur_fused): ax at s[0..NMON), ac at s[NMON..2 NMON); the
+// address depends on the entry's species, so the reads stay in the entry loop
+// instead of occupying 70 registers for the whole walk
+struct SurCoefLds {
+  const float* s;
+  __device__ __forceinline__ float operator()(bool cj, int t) const { return s[(cj ? NMON : 0) + t]; }
+};
+
+// dE_i/d(dr_ij) of every entry of the row (into drow) from dE/dG of the row
+// (g) and the monomial adjoints of this lane's eta (coef); returns the lane's
+// share of sum_j dE_i/d(dr_ij)
+template <class R, class C>
+__device__ __forceinline__ void sur_back_walk(const SurK& K, const Geo<R>& Gm, const typename Geo<R>::P& pi,
+                                              const int* __restrict__ nbrow, int n, int i, int ke, int eg, float eta,
+                                              const float* e2k, const uint32_t* __restrict__ mask,
+                                              const float* g, const C& coef, float4* drow, double* sf) {
+  float gcc[4], gcx[4], dScc[5], dScx[5];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) { gcc[m] = g[24 + ke * 4 + m]; gcx[m] = g[32 + ke * 4 + m]; }
+  g5_adj(gcc, dScc);
+  g5_adj(gcx, dScx);
   const float dD = -(dScc[0] + dScc[1] + dScc[2] + dScc[3] + dScc[4]);
   float cc[6], cx[6];
 #pragma unroll
@@ -498,7 +522,7 @@ __device__ __forceinline__ void sur_back_walk(const SurK& K, const Geo<R>& Gm, c
       float P = 0.0f;
 #pragma unroll
       for (int t = 0; t < NMON; ++t) {
-        adj[t] = cj ? ac[t] : ax[t];
+        adj[t] = coef(cj, t);
         P += adj[t] * mono[t];
       }
       float gu[3] = {0.0f, 0.0f, 0.0f};
@@ -533,9 +557,11 @@ __device__ __forceinline__ void sur_back_walk(const SurK& K, const Geo<R>& Gm, c
 template <class R>
 __global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint32_t* __restrict__ mask, int nb_real) {
   SUR_ROW_PROLOGUE
+  const float* g = K.gG + (size_t)row * NF;
+  float ac[NMON], ax[NMON];
+  sur_back_coef(g, ke, K.M + ((size_t)row * 4 + ke) * 36, K.M + ((size_t)row * 4 + 2 + ke) * 36, ac, ax);
   double sf[3];
-  sur_back_walk<R>(K, Gm, pi, nbrow, n, i, ke, eg, eta, e2k, mask, K.gG + (size_t)row * NF,
-                   K.M + ((size_t)row * 4 + ke) * 36, K.M + ((size_t)row * 4 + 2 + ke) * 36,
+  sur_back_walk<R>(K, Gm, pi, nbrow, n, i, ke, eg, eta, e2k, mask, g, SurCoefReg{ac, ax},
                    K.d + (size_t)row * K.stride0, sf);
   if (q == 0) K.fself[row] = make_double4(sf[0], sf[1], sf[2], 0.0);
 }
@@ -544,15 +570,25 @@ __global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint
 // 256 / SUR_L rows take their moments, their descriptors go to LDS, one
 // wavefront per 16 rows runs the network on them (weights staged in LDS),
 // and the rows walk again
-// for dE/d(dr) with their moments still in registers.  The moments,
-// descriptors and dE/dG never reach memory; the results are those of
-// k_sur_desc, k_sur_mlp and k_sur_back.
+// for dE/d(dr), the monomial adjoints of their moments in LDS where the
+// weights were.  The moments, descriptors and dE/dG never reach memory; the
+// results are those of k_sur_desc, k_sur_mlp and k_sur_back.
+// Three workgroups fit a CU by LDS; the bound of three waves per SIMD caps
+// the registers at 168 (a few spill around the moments walk).
 template <class R>
-__global__ void __launch_bounds__(256) k_sur_fused(RowArgsS a, SurK K, const uint32_t* __restrict__ mask,
+__global__ void __launch_bounds__(256, 3) k_sur_fused(RowArgsS a, SurK K, const uint32_t* __restrict__ mask,
                                                    int nb_real) {
-  __shared__ __align__(16) uint16_t sW[SUR_LNW];
-  __shared__ float sfp[SUR_NFP_];
   constexpr int RB = 256 / SUR_L;  // rows of the workgroup
+  // adjoints of one row and eta: ax then ac; the odd pitch puts the 16
+  // row/eta blocks of a wavefront in distinct banks
+  constexpr int CP = 2 * NMON + 3;
+  static_assert(RB * 2 * CP * 4 <= SUR_LNW * 2, "adjoints exceed the weight tile");
+  __shared__ __align__(16) union {
+    uint16_t w[SUR_LNW];
+    float c[RB * 2 * CP];
+  } sWC;
+  uint16_t* sW = sWC.w;
+  __shared__ float sfp[SUR_NFP_];
   __shared__ float sG[RB * NF], sgG[RB * NF];
   const int lb = xcd_block(nb_real);
   if (lb < 0) return;
@@ -581,9 +617,20 @@ __global__ void __launch_bounds__(256) k_sur_fused(RowArgsS a, SurK K, const uin
     sur_mlp16(lane, ok, sG + lr * NF, sfp, sW, K.e_row + (ok ? rr : 0), sgG + lr * NF);
   }
   __syncthreads();
+  // the weights are no longer read: their tile takes the adjoints
+  float* sC = sWC.c + (lrow * 2 + ke) * CP;
+  if (live) {
+    float ac[NMON], ax[NMON];
+    sur_back_coef(sgG + lrow * NF, ke, mc, mx, ac, ax);
+    if (eg == 0) {
+#pragma unroll
+      for (int t = 0; t < NMON; ++t) { sC[t] = ax[t]; sC[NMON + t] = ac[t]; }
+    }
+  }
+  __syncthreads();
   if (!live) return;
   double sf[3];
-  sur_back_walk<R>(K, Gm, pi, nbrow, n, i, ke, eg, eta, e2k, mask, sgG + lrow * NF, mc, mx,
+  sur_back_walk<R>(K, Gm, pi, nbrow, n, i, ke, eg, eta, e2k, mask, sgG + lrow * NF, SurCoefLds{sC},
                    K.d + (size_t)row * K.stride0, sf);
   if (q == 0) K.fself[row] = make_double4(sf[0], sf[1], sf[2], 0.0);
 }
