@@ -1030,11 +1030,14 @@ __global__ void __launch_bounds__(256) k_sur_wgrad(TrainArr A, int S, float* __r
 }
 
 // gradient in the parameter layout mu[40] isg[40] W1[64][40] b1[64]
-// W2[64][64] b2[64] w3[64] b3 (mu and isg are fixed: zero)
-__global__ void k_sur_grad_reduce(int S, int ngrp, int nrows, const float* __restrict__ wpart,
-                                  const float* __restrict__ bpart, const double* __restrict__ scal,
-                                  double* __restrict__ grad) {
-  int x = blockIdx.x * blockDim.x + threadIdx.x;
+// W2[64][64] b2[64] w3[64] b3 (mu and isg are fixed: zero); one wavefront
+// per parameter: lane l sums the partials l, l + 64, ... and a butterfly
+// over the wavefront adds the lanes, a fixed order, so runs repeat bit for
+// bit (one lane per parameter chained up to a few thousand dependent loads)
+__global__ void __launch_bounds__(256) k_sur_grad_reduce(int S, int ngrp, int nrows, const float* __restrict__ wpart,
+                                                         const float* __restrict__ bpart,
+                                                         const double* __restrict__ scal, double* __restrict__ grad) {
+  const int x = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (x >= PFM_SUR_NTHETA) return;
   double s = 0.0;
   int tile = -1, hh = 0, cc = 0, boff = -1;
@@ -1052,16 +1055,16 @@ __global__ void k_sur_grad_reduce(int S, int ngrp, int nrows, const float* __res
   } else if (x < 80 + 64 * NF + 64 + 64 * 64 + 128) {
     int y = x - (80 + 64 * NF + 64 + 64 * 64);
     boff = 64 + y;  // b2 then w3
-  } else {
-    s = scal[3] * nrows;  // b3
   }
   if (tile >= 0) {
     int ln = (cc & 15) + 16 * ((hh & 15) >> 2), v = hh & 3;
-    for (int k = 0; k < S; ++k) s += wpart[((size_t)k * 28 + tile) * 256 + ln * 4 + v];
+    for (int k = lane; k < S; k += 64) s += wpart[((size_t)k * 28 + tile) * 256 + ln * 4 + v];
   } else if (boff >= 0) {
-    for (int k = 0; k < ngrp; ++k) s += bpart[(size_t)k * 192 + boff];
+    for (int k = lane; k < ngrp; k += 64) s += bpart[(size_t)k * 192 + boff];
   }
-  grad[x] = s;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) grad[x] = x == PFM_SUR_NTHETA - 1 ? scal[3] * nrows : s;  // b3 last
 }
 
 // ---------------------------------------------------------------- host side
@@ -1327,7 +1330,7 @@ int pfm_sur_loss_grad(pfm_ctx* h, int inter, const double* f_ref, double e_ref, 
   const int ng = B.Np / 16;
   PFM_LAUNCH(h, "sur_train_fwd", k_sur_train_fwd, (ng + 3) / 4, 256, nrows, K, A, B.scal, B.bpart);
   PFM_LAUNCH(h, "sur_wgrad", k_sur_wgrad, (28 * B.S + 3) / 4, 256, A, B.S, B.wpart);
-  PFM_LAUNCH(h, "sur_grad", k_sur_grad_reduce, (PFM_SUR_NTHETA + 255) / 256, 256, B.S, ng, nrows, B.wpart, B.bpart,
+  PFM_LAUNCH(h, "sur_grad", k_sur_grad_reduce, (PFM_SUR_NTHETA + 3) / 4, 256, B.S, ng, nrows, B.wpart, B.bpart,
              B.scal, B.grad);
   double sc[5];
   HIPCHK(h, hipMemcpyAsync(sc, B.scal, sizeof sc, hipMemcpyDeviceToHost, h->stream));
