@@ -12,7 +12,8 @@ and the backward signals), forces within 3e-2 of the float64 network (the
 BF16 rounding itself); the sum of the surrogate forces zero to 1e-10 of
 sum |F| (every d_ij enters twice with opposite signs); the training loss and
 gradient of pfm_sur_loss_grad within 3e-2 of torch's double backward
-(cosine above 0.999); training on frames of the plain potential cuts the
+(cosine above 0.999), zero for the fixed scaling, and the same bits on a
+repeated call; training on frames of the plain potential cuts the
 held-out force error below half of both the untrained network's and the
 error of leaving the graphene terms out; the judge moves a run onto the
 plain potential when the force error passes the threshold.
@@ -157,6 +158,11 @@ def test_loss_gradient_against_double_backward():
         rb = np.linalg.norm(grad[a:b] - gref[a:b]) / np.linalg.norm(gref[a:b])
         print(f"  {name}: rel {rb:.2e}", flush=True)
         assert rb < 5e-2, name
+    # mu and isg are fixed: zero gradient
+    assert not grad[:80].any()
+    # every partial sum is reduced in a fixed order: a second call gives the same bits
+    loss2, grad2 = ctx.sur_loss_grad(1, f_ref, e_ref, w_e, w_f)
+    assert np.array_equal(loss, loss2) and np.array_equal(grad, grad2)
 
 
 def test_training_gains_over_untrained_and_over_no_term():
