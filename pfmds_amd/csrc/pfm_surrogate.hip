@@ -196,8 +196,9 @@ __device__ __forceinline__ float sur_dr_rec(const Geo<R>& Gm, const typename Geo
 // arithmetic on the first, so a block waits on two memory latencies rather
 // than every entry on one.  Entries past the row's end are skipped.
 #define SUR_NOAUX(j) 0
-#define SUR_WALK_BEGIN(AUXT, AUXF)                                                                    \
-  for (int p0 = 4 * eg; p0 < n; p0 += 4 * SUR_EG) {                                                   \
+#define SUR_WALK_BEGIN(AUXT, AUXF) SUR_WALK_BEGIN_G(AUXT, AUXF, eg, SUR_EG)
+#define SUR_WALK_BEGIN_G(AUXT, AUXF, GRP, NGRP)                                                       \
+  for (int p0 = 4 * (GRP); p0 < n; p0 += 4 * (NGRP)) {                                                \
     const int4 v4_ = *reinterpret_cast<const int4*>(nbrow + p0);                                      \
     const int jj_[4] = {v4_.x, p0 + 1 < n ? v4_.y : i, p0 + 2 < n ? v4_.z : i, p0 + 3 < n ? v4_.w : i}; \
     typename Geo<R>::P pp_[4];                                                                        \
@@ -470,40 +471,38 @@ __device__ __forceinline__ void sur_back_coef(const float* g, int ke, const floa
   }
 }
 
-// the adjoints held in registers (k_sur_back) ...
-struct SurCoefReg {
-  const float* ac;
-  const float* ax;
-  __device__ __forceinline__ float operator()(bool cj, int t) const { return cj ? ac[t] : ax[t]; }
-};
-// ... or in LDS (k_sur_fused): ax at s[0..NMON), ac at s[NMON..2 NMON); the
-// address depends on the entry's species, so the reads stay in the entry loop
-// instead of occupying 70 registers for the whole walk
+// the adjoints of one row and eta in LDS: ax at s[0..NMON), ac at
+// s[NMON..2 NMON); the address depends on the entry's species, so the reads
+// stay in the entry loop instead of occupying 70 registers for the whole walk
 struct SurCoefLds {
   const float* s;
   __device__ __forceinline__ float operator()(bool cj, int t) const { return s[(cj ? NMON : 0) + t]; }
 };
 
 // dE_i/d(dr_ij) of every entry of the row (into drow) from dE/dG of the row
-// (g) and the monomial adjoints of this lane's eta (coef); returns the lane's
-// share of sum_j dE_i/d(dr_ij)
+// (g, in LDS) and the monomial adjoints of the two angular etas (coef0,
+// coef1).  Lane q of the row takes the entry blocks q, q + SUR_L, ... and
+// both etas of each entry, so the separation, cutoff and monomials are formed
+// once per entry; the etas run one after the other (a loop, so only one set
+// of adjoints is live) and d_ij is the eta-0 share plus the eta-1 share.
+// Every lane returns the row's sum_j dE_i/d(dr_ij) (FP64).
 template <class R, class C>
 __device__ __forceinline__ void sur_back_walk(const SurK& K, const Geo<R>& Gm, const typename Geo<R>::P& pi,
-                                              const int* __restrict__ nbrow, int n, int i, int ke, int eg, float eta,
-                                              const float* e2k, const uint32_t* __restrict__ mask,
-                                              const float* g, const C& coef, float4* drow, double* sf) {
-  float gcc[4], gcx[4], dScc[5], dScx[5];
+                                              const int* __restrict__ nbrow, int n, int i, int q,
+                                              const uint32_t* __restrict__ mask, const float* g, const C& coef0,
+                                              const C& coef1, float4* drow, double* sf) {
+  float dD[2];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) { gcc[m] = g[24 + ke * 4 + m]; gcx[m] = g[32 + ke * 4 + m]; }
-  g5_adj(gcc, dScc);
-  g5_adj(gcx, dScx);
-  const float dD = -(dScc[0] + dScc[1] + dScc[2] + dScc[3] + dScc[4]);
-  float cc[6], cx[6];
+  for (int ke = 0; ke < 2; ++ke) {
+    float gcc[4], gcx[4], dScc[5], dScx[5];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) { cc[k] = g[6 * ke + k]; cx[k] = g[12 + 6 * ke + k]; }
+    for (int m = 0; m < 4; ++m) { gcc[m] = g[24 + ke * 4 + m]; gcx[m] = g[32 + ke * 4 + m]; }
+    g5_adj(gcc, dScc);
+    g5_adj(gcx, dScx);
+    dD[ke] = -(dScc[0] + dScc[1] + dScc[2] + dScc[3] + dScc[4]);
+  }
   sf[0] = 0.0; sf[1] = 0.0; sf[2] = 0.0;
-  // the two eta lanes of an entry group walk the same entries
-  SUR_WALK_BEGIN(int, SUR_NOAUX)
+  SUR_WALK_BEGIN_G(int, SUR_NOAUX, q, SUR_L)
     float dx, dy, dz;
     float r2 = sur_dr_rec<R>(Gm, pi, pj, dx, dy, dz);
     float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
@@ -513,56 +512,86 @@ __device__ __forceinline__ void sur_back_walk(const SurK& K, const Geo<R>& Gm, c
       float fc, fcp;
       sur_cut(K, r, fc, fcp);
       float u[3] = {dx * ir, dy * ir, dz * ir};
-      float ex = __expf(-eta * r2);
-      float w = ex * fc, wp = ex * (fcp - 2.0f * eta * r * fc);
-      float mono[NMON], adj[NMON];
+      float mono[NMON];
       mono[0] = 1.0f;
 #pragma unroll
       for (int t = 1; t < NMON; ++t) mono[t] = mono[kPar[t]] * u[kCrd[t]];
-      float P = 0.0f;
+      // radial dE/dG of the entry's species: the first 12 or the next 12
+      const float* gr = g + (cj ? 0 : 12);
+#pragma unroll 1
+      for (int ke = 0; ke < 2; ++ke) {
+        const C coef = ke ? coef1 : coef0;
+        const float eta = K.etaa[ke];
+        float ex = __expf(-eta * r2);
+        float w = ex * fc, wp = ex * (fcp - 2.0f * eta * r * fc);
+        float adj[NMON];
+        float P = 0.0f;
 #pragma unroll
-      for (int t = 0; t < NMON; ++t) {
-        adj[t] = coef(cj, t);
-        P += adj[t] * mono[t];
-      }
-      float gu[3] = {0.0f, 0.0f, 0.0f};
+        for (int t = 0; t < NMON; ++t) {
+          adj[t] = coef(cj, t);
+          P += adj[t] * mono[t];
+        }
+        float gu[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int t = NMON - 1; t >= 1; --t) {
-        adj[kPar[t]] += adj[t] * u[kCrd[t]];
-        gu[kCrd[t]] += adj[t] * mono[kPar[t]];
-      }
-      float Q = u[0] * gu[0] + u[1] * gu[1] + u[2] * gu[2];
-      float wr = w * ir;
-      float cu = wp * P - wr * Q + (cj ? 2.0f * dD * w * wp : 0.0f);
+        for (int t = NMON - 1; t >= 1; --t) {
+          adj[kPar[t]] += adj[t] * u[kCrd[t]];
+          gu[kCrd[t]] += adj[t] * mono[kPar[t]];
+        }
+        float Q = u[0] * gu[0] + u[1] * gu[1] + u[2] * gu[2];
+        float wr = w * ir;
+        float cu = wp * P - wr * Q + (cj ? 2.0f * (ke ? dD[1] : dD[0]) * w * wp : 0.0f);
 #pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        float x = __expf(-e2k[k] * r2);
-        cu += (cj ? cc[k] : cx[k]) * x * (fcp - 2.0f * e2k[k] * r * fc);
+        for (int k = 0; k < 6; ++k) {
+          const float e2 = K.eta2[6 * ke + k];
+          float x = __expf(-e2 * r2);
+          cu += gr[6 * ke + k] * x * (fcp - 2.0f * e2 * r * fc);
+        }
+        c0 += cu * u[0] + wr * gu[0];
+        c1 += cu * u[1] + wr * gu[1];
+        c2 += cu * u[2] + wr * gu[2];
       }
-      c0 = cu * u[0] + wr * gu[0];
-      c1 = cu * u[1] + wr * gu[1];
-      c2 = cu * u[2] + wr * gu[2];
     }
-    c0 += __shfl_xor(c0, 1, SUR_L);
-    c1 += __shfl_xor(c1, 1, SUR_L);
-    c2 += __shfl_xor(c2, 1, SUR_L);
-    if (ke == 0) drow[p] = make_float4(c0, c1, c2, 0.0f);
+    drow[p] = make_float4(c0, c1, c2, 0.0f);
     sf[0] += c0; sf[1] += c1; sf[2] += c2;
   SUR_WALK_END
-  sum_groups<3>(sf);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int o = 1; o < SUR_L; o <<= 1) sf[c] += __shfl_xor(sf[c], o, SUR_L);
+  }
 }
 
 // dE_i/d(dr_ij) of every entry of the row, and the row atom's own share
-// sum_j dE_i/d(dr_ij) (FP64)
+// sum_j dE_i/d(dr_ij) (FP64); the row's dE/dG and (lanes 0 and 1) the
+// adjoints of each eta go to LDS first
 template <class R>
 __global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint32_t* __restrict__ mask, int nb_real) {
-  SUR_ROW_PROLOGUE
-  const float* g = K.gG + (size_t)row * NF;
-  float ac[NMON], ax[NMON];
-  sur_back_coef(g, ke, K.M + ((size_t)row * 4 + ke) * 36, K.M + ((size_t)row * 4 + 2 + ke) * 36, ac, ax);
+  constexpr int RB = 256 / SUR_L, CP = 2 * NMON + 3;
+  __shared__ float sc[RB * 2 * CP], sg[RB * NF];
+  const int lb = xcd_block(nb_real);
+  if (lb < 0) return;
+  const int row = (lb * blockDim.x + threadIdx.x) / SUR_L, q = threadIdx.x & (SUR_L - 1);
+  const int lrow = threadIdx.x / SUR_L;
+  const bool live = row < a.nrows;
+  if (live)
+    for (int k = q; k < NF; k += SUR_L) sg[lrow * NF + k] = K.gG[(size_t)row * NF + k];
+  if (live && q < 2) {
+    float ac[NMON], ax[NMON];
+    sur_back_coef(K.gG + (size_t)row * NF, q, K.M + ((size_t)row * 4 + q) * 36, K.M + ((size_t)row * 4 + 2 + q) * 36,
+                  ac, ax);
+    float* sC = sc + (lrow * 2 + q) * CP;
+#pragma unroll
+    for (int t = 0; t < NMON; ++t) { sC[t] = ax[t]; sC[NMON + t] = ac[t]; }
+  }
+  __syncthreads();
+  if (!live) return;
+  const int i = a.rows[row];
+  const Geo<R> Gm = sgeo<R>(a, i);
+  const typename Geo<R>::P pi = Gm.at(i);
   double sf[3];
-  sur_back_walk<R>(K, Gm, pi, nbrow, n, i, ke, eg, eta, e2k, mask, g, SurCoefReg{ac, ax},
-                   K.d + (size_t)row * K.stride0, sf);
+  sur_back_walk<R>(K, Gm, pi, a.nbr + (size_t)row * a.stride, a.nnum[row], i, q, mask, sg + lrow * NF,
+                   SurCoefLds{sc + lrow * 2 * CP}, SurCoefLds{sc + (lrow * 2 + 1) * CP}, K.d + (size_t)row * K.stride0,
+                   sf);
   if (q == 0) K.fself[row] = make_double4(sf[0], sf[1], sf[2], 0.0);
 }
 
@@ -571,7 +600,7 @@ __global__ void __launch_bounds__(256) k_sur_back(RowArgsS a, SurK K, const uint
 // wavefront per 16 rows runs the network on them (weights staged in LDS),
 // and the rows walk again
 // for dE/d(dr), the monomial adjoints of their moments in LDS where the
-// weights were.  The moments, descriptors and dE/dG never reach memory; the
+// weights were, each lane taking both etas of its entries.  The moments, descriptors and dE/dG never reach memory; the
 // results are those of k_sur_desc, k_sur_mlp and k_sur_back.
 // Three workgroups fit a CU by LDS; the bound of three waves per SIMD caps
 // the registers at 168 (a few spill around the moments walk).
@@ -630,8 +659,8 @@ __global__ void __launch_bounds__(256, 3) k_sur_fused(RowArgsS a, SurK K, const 
   __syncthreads();
   if (!live) return;
   double sf[3];
-  sur_back_walk<R>(K, Gm, pi, nbrow, n, i, ke, eg, eta, e2k, mask, sgG + lrow * NF, SurCoefLds{sC},
-                   K.d + (size_t)row * K.stride0, sf);
+  sur_back_walk<R>(K, Gm, pi, nbrow, n, i, q, mask, sgG + lrow * NF, SurCoefLds{sWC.c + lrow * 2 * CP},
+                   SurCoefLds{sWC.c + (lrow * 2 + 1) * CP}, K.d + (size_t)row * K.stride0, sf);
   if (q == 0) K.fself[row] = make_double4(sf[0], sf[1], sf[2], 0.0);
 }
 
